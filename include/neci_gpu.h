@@ -1,0 +1,210 @@
+/*
+ * neci_gpu.h -- C ABI of the MI355X FCIQMC iteration engine.
+ *
+ * The engine replaces the body of NECI's PerformFCIMCycPar
+ * (src/FciMCPar.F90:1226-1873) for one MPI rank's share of the walkers: the
+ * walker list, the walker hash and the spawn buffers stay resident in GPU
+ * memory, and one ngpu_iterate() call performs spawn -> merge -> death ->
+ * annihilation -> hash statistics.  The five procedure pointers of NECI's
+ * plugin surface (src/proc_ptrs.F90:251-258, bound by init_fcimc_fn_pointers,
+ * src/fcimc_initialisation.fpp:2029-2283) are selected once by
+ * ngpu_set_excitgen()/ngpu_set_integrals()/ngpu_set_lattice(); host-callable
+ * batched forms of each pointer are exported for parity testing.
+ *
+ * Conventions (those of the reference boundary):
+ *   - caller-owned host arrays, plain pointers and sizes;
+ *   - orbitals are 1-based spin orbitals, odd = beta (macros.h:21);
+ *   - determinants are NIfD+1 = nbasis/64+1 int64 words (BitReps.F90:174);
+ *   - walker rows use the CurrentDets layout: det words | lenof_sign FP64
+ *     signs stored as bit patterns | flag word (BitReps.F90:164-214), with
+ *     NECI's flag bit numbers (bit_rep_data.F90:87-118);
+ *   - every function returns an ngpu_status; errors mirror the reference's
+ *     err/stop_all sites (fcimc_helper.F90:483-510, load_balancer.fpp:540-544,
+ *     fcimc_pointed_fns.F90:649-673).  No exceptions cross the ABI;
+ *   - one handle per host rank; a handle is not thread-safe.
+ */
+#ifndef NECI_GPU_H
+#define NECI_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NGPU_MAX_REPLICAS 16
+
+typedef struct ngpu_engine ngpu_t;
+
+typedef enum {
+    NGPU_OK = 0,
+    NGPU_ERR_SPAWN_FULL = 1,   /* checkValidSpawnedList, fcimc_helper.F90:483-510 (MEMORYFACSPAWN) */
+    NGPU_ERR_WALKERS_FULL = 2, /* AddNewHashDet, load_balancer.fpp:540-544 (MEMORYFACPART) */
+    NGPU_ERR_DEATH_PROB = 3,   /* attempt_die_normal, fcimc_pointed_fns.F90:649-673 (tau too big) */
+    NGPU_ERR_ARG = 4,
+    NGPU_ERR_HIP = 5,
+    NGPU_ERR_HASH_FULL = 6,    /* every bucket of the walker hash is taken (MEMORYFACHASH) */
+    NGPU_ERR_STATE = 7         /* call out of order (e.g. iterate before set_reference) */
+} ngpu_status;
+
+enum { NGPU_MODEL_HUBBARD = 0, NGPU_MODEL_MOLECULAR = 1 };
+/* singles: "on-the-fly-heat-bath" (excit_gens_int_weighted.F90:442) */
+enum { NGPU_SINGLES_HEATBATH = 1 };
+/* PCHB doubles particle selection (gasci_pchb_doubles_select_particles.fpp:38-44);
+   holes are always FAST-FAST spin-orbital resolved */
+enum { NGPU_PART_UNIF_UNIF = 0, NGPU_PART_UNIF_FAST = 1 };
+/* flag bit layout of walker rows at the boundary */
+enum { NGPU_FLAGS_MNECI = 0, NGPU_FLAGS_NECI = 1 };
+
+/* System and run parameters (SystemData / CalcData globals the engine needs). */
+typedef struct {
+    int32_t nel;                 /* electrons */
+    int32_t nbasis;              /* spin orbitals */
+    int32_t n_replicas;          /* lenof_sign (1 = neci, 2 = dneci, R = mneci) */
+    int32_t model;               /* NGPU_MODEL_* */
+    int32_t t_trunc_initiator;   /* TRUNCINITIATOR */
+    int32_t t_all_real_coeff;    /* ALLREALCOEFF */
+    int32_t t_real_spawn_cutoff; /* REALSPAWNCUTOFF on/off */
+    int32_t flag_layout;         /* NGPU_FLAGS_* */
+    double initiator_walk_no;    /* ADDTOINITIATOR (default 3) */
+    double real_spawn_cutoff;    /* REALSPAWNCUTOFF value (default 0.95) */
+    double occupied_thresh;      /* SETOCCUPIEDTHRESH (default 1) */
+    double av_mc_excits;         /* AvMCExcits (default 1) */
+    int64_t max_walkers;         /* MaxWalkersPart = MEMORYFACPART * InitWalkers */
+    int64_t max_spawned;         /* MaxSpawned = MEMORYFACSPAWN * InitWalkers * inum_runs */
+    int64_t n_hash_buckets;      /* nWalkerHashes = nint(MEMORYFACHASH * InitWalkers) */
+    uint32_t seed;               /* SEED (keys the counter-based RNG) */
+    int32_t pad;
+} ngpu_sys_t;
+
+/* Per-iteration inputs (Iter, tau, DiagSft relative to Hii). */
+typedef struct {
+    double tau;
+    double diag_sft[NGPU_MAX_REPLICAS];
+    uint32_t iter;
+    int32_t pad;
+} ngpu_iter_in_t;
+
+/* Per-iteration outputs: fcimc_iter_data (FciMCData.F90:445) plus the
+   accumulators of SumEContrib, CalcParentFlag and CalcHashTableStats. */
+typedef struct {
+    double nborn[NGPU_MAX_REPLICAS], ndied[NGPU_MAX_REPLICAS], nannihil[NGPU_MAX_REPLICAS];
+    double naborted[NGPU_MAX_REPLICAS], nremoved[NGPU_MAX_REPLICAS];
+    double tot_parts[NGPU_MAX_REPLICAS], norm_sq[NGPU_MAX_REPLICAS];
+    double enum_cyc[NGPU_MAX_REPLICAS], hf_cyc[NGPU_MAX_REPLICAS], noat_doubs[NGPU_MAX_REPLICAS];
+    double spawn_from_sing[NGPU_MAX_REPLICAS], inst_no_at_hf[NGPU_MAX_REPLICAS];
+    double no_init_dets[NGPU_MAX_REPLICAS], no_init_walk[NGPU_MAX_REPLICAS];
+    double no_noninit_dets[NGPU_MAX_REPLICAS], no_noninit_walk[NGPU_MAX_REPLICAS];
+    double n_valid_excits, n_invalid_excits, max_cyc_spawn;
+    double bloom_count[3], bloom_size[3];
+    int64_t tot_walkers;         /* occupied determinants after the iteration */
+    int64_t n_attempts, n_spawned, n_unique, n_new;
+    int32_t err, pad;
+} ngpu_iter_out_t;
+
+/* ---- lifetime and set-up (replaces the allocations of InitFCIMCCalcPar,
+        fcimc_initialisation.fpp:1543-1760) ---- */
+int ngpu_create(ngpu_t **h, const ngpu_sys_t *sys);
+int ngpu_destroy(ngpu_t *h);
+
+/* Integrals as READFCIINT leaves them (readint.F90:585-950): TMAT2D(nbasis,nbasis)
+   column major; UMAT packed by UMatInd_base (UMatCache.F90:257) over spatial
+   orbitals, UMAT(1) at umat[0]; ECORE.  Selects get_spawn_helement =
+   get_helement_det_only (fcimc_initialisation.fpp:2210). */
+int ngpu_set_integrals(ngpu_t *h, const double *tmat2d, const double *umat, int64_t umat_len,
+                       double ecore);
+
+/* Real-space Hubbard lattice (init_real_space_hubbard, real_space_hubbard.F90:140):
+   nbr[(site-1)*maxnbr + k] = 1-based neighbour site, 0 terminates the row.
+   Selects gen_excit_rs_hubbard and get_helement_lattice. */
+int ngpu_set_lattice(ngpu_t *h, const int32_t *nbr, int32_t nsites, int32_t maxnbr, double bhub,
+                     double uhub);
+
+/* Molecular excitation generator: FCI-PCHB doubles (FAST-FAST, spin-orbital
+   resolved, gasci_pchb_doubles_spinorb_fastweighted.fpp) with the given particle
+   selection, singles with probability psingles (gen_exc_sd,
+   excitation_generators.F90:112).  Builds the alias tables (aliasSampling.F90:209)
+   on the host and uploads them.  Requires ngpu_set_integrals first. */
+int ngpu_set_excitgen(ngpu_t *h, int32_t singles, int32_t particles, double psingles,
+                      double pparallel);
+
+/* RandomHash2(1:nbasis) (fcimc_initialisation.fpp:890-915) for FindWalkerHash. */
+int ngpu_set_hash(ngpu_t *h, const int32_t *random_hash2);
+
+/* Reference determinant ilutRef (W words) and Hii. */
+int ngpu_set_reference(ngpu_t *h, const int64_t *ilut_ref, double hii);
+
+/* Walkers in CurrentDets layout (rows of nbasis/64+1+n_replicas+1 int64).
+   The engine computes det_diagH / det_offdiagH itself (AddNewHashDet,
+   load_balancer.fpp:514-629). */
+int ngpu_upload_walkers(ngpu_t *h, const int64_t *rows, int64_t n);
+/* Occupied walkers (and core dets) back in CurrentDets layout, with
+   det_diagH (= H_ii - Hii) and det_offdiagH; *n = number of occupied rows. */
+int ngpu_download_walkers(ngpu_t *h, int64_t *rows, double *diag_h, double *offdiag_h,
+                          int64_t cap, int64_t *n);
+
+/* One FCIQMC iteration (PerformFCIMCycPar, FciMCPar.F90:1177-1920).  Runs
+   asynchronously on the engine's stream when out == NULL; otherwise waits and
+   fills out.  A failure inside the iteration is reported by the next call that
+   waits. */
+int ngpu_iterate(ngpu_t *h, const ngpu_iter_in_t *in, ngpu_iter_out_t *out);
+/* Wait for the last iteration and return its statistics. */
+int ngpu_get_stats(ngpu_t *h, ngpu_iter_out_t *out);
+
+/* Stage timing with events on the engine stream (set_timer/halt_timer analogue,
+   lib/timing.F90:122-188).  Stages: 0 prepass(initiator, proj-E, death),
+   1 attempt scan, 2 spawn, 3 sort, 4 merge, 5 annihilate, 6 insert, 7 stats. */
+int ngpu_set_profiling(ngpu_t *h, int32_t on);
+int ngpu_get_stage_times(ngpu_t *h, double *ms_total, int32_t nstages, int64_t *n_iters);
+
+/* ---- parity entry points: batched host-callable forms of the plugin surface ---- */
+
+/* FindWalkerHash (hash.F90:23-37) of the determinants in dets (W words each). */
+int ngpu_find_walker_hash_batch(ngpu_t *h, const int64_t *dets, int64_t n, int32_t *buckets);
+
+/* get_spawn_helement (proc_ptrs.F90:82-97) for pairs (ilutI, ilutJ): H_IJ. */
+int ngpu_get_spawn_helement_batch(ngpu_t *h, const int64_t *ilut_i, const int64_t *ilut_j,
+                                  int64_t n, double *hel);
+
+/* get_diagonal_matel / get_off_diagonal_matel (matel_getter.F90:27-107):
+   diag = H_ii (total), offdiag = H(det, ref). */
+int ngpu_diag_helement_batch(ngpu_t *h, const int64_t *dets, int64_t n, double *diag,
+                             double *offdiag);
+
+/* generate_excitation (proc_ptrs.F90:13-32) from each det with six explicit
+   uniforms per call (draws[6*i..6*i+5]); ic = 0 marks a null excitation.
+   ex[4*i] = (src1, src2 | tgt1, tgt2) for doubles, (src, tgt, 0, 0) for singles. */
+int ngpu_generate_excitation_batch(ngpu_t *h, const int64_t *ilut, const double *draws,
+                                   int64_t n, int64_t *ilut_j, int32_t *ic, int32_t *ex,
+                                   int32_t *parity, double *pgen);
+
+/* calc_pgen of the bound generator for given excitations. */
+int ngpu_calc_pgen_batch(ngpu_t *h, const int64_t *ilut, const int32_t *ic, const int32_t *ex,
+                         int64_t n, double *pgen);
+
+/* encode_child = FindExcitBitDet (DetBitOps.F90:584-623); pure host function. */
+int ngpu_encode_child(const int64_t *ilut_i, int32_t nwords, int32_t ic, const int32_t *ex,
+                      int64_t *ilut_j);
+
+/* attempt_die_normal (fcimc_pointed_fns.F90:573-705) for n (Kii, sign) pairs of one
+   replica with explicit uniforms; ndie = deaths (negative = births). */
+int ngpu_attempt_die_batch(ngpu_t *h, const double *kii, const double *sign, const double *udraw,
+                           int64_t n, double tau, double diag_sft, double *ndie);
+
+/* attempt_create_normal (fcimc_pointed_fns.F90:178-491): child amplitude for
+   n (hel, pgen, parent sign) triples with explicit rounding uniforms. */
+int ngpu_attempt_create_batch(ngpu_t *h, const double *hel, const double *pgen,
+                              const double *parent_sign, const double *udraw, int64_t n,
+                              double tau, double *child);
+
+/* CompressSpawnedList + AnnihilateSpawnedParts + CalcHashTableStats
+   (Annihilation.F90:249-1352, load_balancer.fpp:646-805) on an explicit spawned
+   list against the resident walkers.  sinfo: bits 0-7 replica, bit 8 parent
+   initiator.  merged_* receive the compressed list (any order). */
+int ngpu_annihilate(ngpu_t *h, uint32_t iter, const int64_t *sdet, const double *sval,
+                    const uint32_t *sinfo, int64_t n, ngpu_iter_out_t *out, int64_t *merged_det,
+                    double *merged_sgn, uint32_t *merged_flags, int64_t *n_merged);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

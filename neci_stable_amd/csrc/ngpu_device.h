@@ -1,0 +1,665 @@
+// ngpu_device.h -- device functions of the FCIQMC iteration engine (gfx950).
+//
+// Determinants are kept as W 64-bit words (orbital k -> word (k-1)/64, bit
+// (k-1)%64, macros.h:16,74-77; odd k = beta, macros.h:21).  Nothing here
+// materialises the occupied-orbital list nI: electron selection, Slater-Condon
+// sums, fermionic parity and FindWalkerHash all iterate set bits in ascending
+// order (the order of nI), so each lane works from its 8W-byte key in
+// registers and no per-lane array spills to scratch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ngpu {
+
+constexpr int kMaxRep = 16;
+constexpr double kEps = 1e-13;               // constants.F90:28 EPS
+constexpr uint32_t kFlagCore = 1u << 16;     // deterministic (core) determinant
+constexpr uint32_t kFlagRemoved = 1u << 17;  // empty slot of the walker table
+constexpr uint32_t kInfoInit = 1u << 8;      // spawn record: parent was initiator
+constexpr uint32_t kInfoCoreParent = 1u << 9;
+constexpr int kBucketEntries = 4;            // walker hash: entries per bucket
+constexpr int32_t kSlotEmpty = -1;
+constexpr int32_t kSlotTomb = -2;
+
+// ------------------------------------------------------------------ RNG
+// Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter-based.
+struct U4 { uint32_t x, y, z, w; };
+
+__host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+    U4 n;
+    n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+    n.y = (uint32_t)p1;
+    n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+    n.w = (uint32_t)p0;
+    c = n;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+__host__ __device__ inline uint64_t splitmix_fin(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+template <int W>
+__host__ __device__ inline uint64_t det_hash64(const uint64_t* d) {
+  if (W == 1) return d[0];
+  return d[0] ^ splitmix_fin(d[1]);
+}
+
+__host__ __device__ inline double u01(uint32_t lo, uint32_t hi) {
+  uint64_t x = ((uint64_t)hi << 32) | (uint64_t)lo;
+  return (double)(x >> 11) * 0x1.0p-53;
+}
+
+// Two uniforms for (det, iteration, replica, k, purpose).  Purposes:
+// 0 per-det (num-to-spawn, death), 1-3 spawn attempt k, 4 new-det rounding,
+// 5 statistics rounding.
+template <int W>
+__host__ __device__ inline void draw2(uint32_t seed, const uint64_t* det, uint32_t iter, int rep,
+                                      uint32_t k, uint32_t purpose, double& ua, double& ub) {
+  uint64_t dh = det_hash64<W>(det);
+  U4 c{(uint32_t)dh, (uint32_t)(dh >> 32), iter, (k << 3) | purpose};
+  U4 o = philox4x32_10(c, seed, (uint32_t)rep);
+  ua = u01(o.x, o.y);
+  ub = u01(o.z, o.w);
+}
+
+// ------------------------------------------------------------ bit helpers
+__host__ __device__ inline int popc64(uint64_t x) { return __builtin_popcountll(x); }
+__host__ __device__ inline int ctz64(uint64_t x) { return __builtin_ctzll(x); }
+
+template <int W>
+__host__ __device__ inline bool test_orb(const uint64_t* d, int orb) {
+  return (d[(orb - 1) >> 6] >> ((orb - 1) & 63)) & 1ull;
+}
+template <int W>
+__host__ __device__ inline void flip_orb(uint64_t* d, int orb) {
+  d[(orb - 1) >> 6] ^= 1ull << ((orb - 1) & 63);
+}
+template <int W>
+__host__ __device__ inline bool det_eq(const uint64_t* a, const uint64_t* b) {
+  bool e = true;
+#pragma unroll
+  for (int w = 0; w < W; ++w) e = e && (a[w] == b[w]);
+  return e;
+}
+template <int W>
+__host__ __device__ inline int popc_det(const uint64_t* d) {
+  int c = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) c += popc64(d[w]);
+  return c;
+}
+// FindBitExcitLevel(a, b) = popcount(a & ~b)   (DetBitOps.F90:140-180)
+template <int W>
+__host__ __device__ inline int excit_level(const uint64_t* a, const uint64_t* b) {
+  int c = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) c += popc64(a[w] & ~b[w]);
+  return c;
+}
+
+// rank-k (0-based) set bit of x, x has more than k bits set
+__host__ __device__ inline int select64(uint64_t x, int k) {
+  int pos = 0;
+  int c = popc64(x & 0xFFFFFFFFull);
+  if (k >= c) { k -= c; x >>= 32; pos += 32; }
+  c = popc64(x & 0xFFFFull);
+  if (k >= c) { k -= c; x >>= 16; pos += 16; }
+  c = popc64(x & 0xFFull);
+  if (k >= c) { k -= c; x >>= 8; pos += 8; }
+  c = popc64(x & 0xFull);
+  if (k >= c) { k -= c; x >>= 4; pos += 4; }
+  c = popc64(x & 0x3ull);
+  if (k >= c) { k -= c; x >>= 2; pos += 2; }
+  c = (int)(x & 1ull);
+  if (k >= c) { pos += 1; }
+  return pos;
+}
+
+// orbital (1-based) of the e-th (1-based) set bit of the masked determinant
+template <int W>
+__host__ __device__ inline int select_orb(const uint64_t* d, uint64_t mask, int e) {
+  int k = e - 1;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    uint64_t x = d[w] & mask;
+    int c = popc64(x);
+    if (k < c) return 64 * w + select64(x, k) + 1;
+    k -= c;
+  }
+  return 0;
+}
+
+// number of occupied orbitals strictly between orbitals a and b
+template <int W>
+__host__ __device__ inline int occ_between(const uint64_t* d, int a, int b) {
+  int lo = a < b ? a : b, hi = a < b ? b : a;  // count bits lo+1 .. hi-1 (1-based)
+  int c = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    int base = 64 * w;  // orbitals base+1 .. base+64
+    int l = lo - base;  // bits with orbital > lo  -> bit index >= l
+    int h = hi - base;  // orbital < hi -> bit index <= h-2
+    uint64_t m = ~0ull;
+    if (l >= 64) m = 0; else if (l > 0) m &= ~0ull << l;
+    if (h <= 1) m = 0; else if (h - 1 < 64) m &= (~0ull >> (64 - (h - 1)));
+    c += popc64(d[w] & m);
+  }
+  return c;
+}
+
+// make_single parity (excit_parity.F90:15-76): odd number of occupied
+// orbitals strictly between src and tgt
+template <int W>
+__host__ __device__ inline bool single_parity(const uint64_t* d, int src, int tgt) {
+  return occ_between<W>(d, src, tgt) & 1;
+}
+// make_double parity (excit_parity.F90:78-172) for sorted s1<s2, t1<t2:
+// move s1->t1 in D, then s2->t2 in the intermediate determinant
+template <int W>
+__host__ __device__ inline bool double_parity(const uint64_t* d, int s1, int s2, int t1, int t2) {
+  uint64_t d1[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) d1[w] = d[w];
+  int p1 = occ_between<W>(d, s1, t1);
+  flip_orb<W>(d1, s1);
+  flip_orb<W>(d1, t1);
+  int p2 = occ_between<W>(d1, s2, t2);
+  return (p1 + p2) & 1;
+}
+
+// ----------------------------------------------------------- hashing
+// FindWalkerHash (hash.F90:23-37): wrapping int64 accumulation over the
+// occupied orbitals in ascending order, bucket = |mod(h, L)| + 1 (1-based).
+template <int W>
+__host__ __device__ inline int64_t find_walker_hash(const uint64_t* d, const int32_t* rh, int64_t L) {
+  uint64_t h = 0;
+  int i = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    uint64_t x = d[w];
+    while (x) {
+      int orb = 64 * w + ctz64(x) + 1;
+      x &= x - 1;
+      ++i;
+      int32_t term = rh[orb - 1] * i;
+      h = 1099511628211ull * h + (uint64_t)(int64_t)term;
+    }
+  }
+  int64_t m = (int64_t)h % L;
+  if (m < 0) m = -m;
+  return m + 1;
+}
+
+// --------------------------------------------------------- integrals
+// UMatInd_base (UMatCache.F90:257-303), real hermitian integrals, 1-based
+__host__ __device__ inline int64_t umat_ind(int i, int j, int k, int l) {
+  int64_t A = i > k ? ((int64_t)i * (i - 1)) / 2 + k : ((int64_t)k * (k - 1)) / 2 + i;
+  int64_t B = j > l ? ((int64_t)j * (j - 1)) / 2 + l : ((int64_t)l * (l - 1)) / 2 + j;
+  return A > B ? (A * (A - 1)) / 2 + B : (B * (B - 1)) / 2 + A;
+}
+
+__host__ __device__ inline bool is_beta(int orb) { return orb & 1; }
+__host__ __device__ inline int spat(int orb) { return (orb + 1) >> 1; }
+
+struct AliasEnt {  // one alias-table entry (aliasSampling.F90 bias/alias pair)
+  double bias;
+  int32_t alias;  // 0-based
+  int32_t pad;
+};
+
+// Read-only system description passed by value to kernels.
+struct DevSys {
+  int nel, nbasis, S, model;
+  int t_init, t_real, t_rsc, singles;
+  int particles, nsites, maxnbr, ijmax;
+  double init_walk_no, rsc, occ_thresh, avmc;
+  double hii, ecore, bhub, uhub;
+  double psingles, pdoubles, pparallel;
+  uint32_t seed;
+  int nalpha_ref, nbeta_ref;
+  uint64_t ref[2];
+  int64_t L;                   // walker hash buckets
+  const int32_t* rh;           // RandomHash2
+  const int32_t* nbr;          // lattice
+  const double* tmat;          // TMAT2D column major
+  const double* umat;          // packed UMAT, [UMatInd-1]
+  const AliasEnt* ab_tab;      // [ijmax][ijmax]
+  const double* ab_prob;
+  const uint8_t* ab_ok;
+  const AliasEnt* j_tab;       // [nbasis][nbasis]
+  const double* j_prob;
+  const uint8_t* j_ok;
+  const int32_t* ab_tgt;       // tgtOrbs: ab_tgt[2*(ab-1)] = B <= A = ab_tgt[2*(ab-1)+1]
+};
+
+__device__ __host__ inline double U(const DevSys& s, int i, int j, int k, int l) {
+  return s.umat[umat_ind(i, j, k, l) - 1];
+}
+__device__ __host__ inline double T(const DevSys& s, int i, int j) {
+  return s.tmat[(size_t)(i - 1) + (size_t)(j - 1) * (size_t)s.nbasis];
+}
+
+// sltcnd_0_base (sltcnd.fpp:585-622): ordering of the FP sums follows the
+// reference loops exactly (inner partial sum per i, then exchange, then one-body)
+template <int W>
+__device__ __host__ inline double sltcnd_0(const DevSys& s, const uint64_t* d) {
+  double hel_sing = 0.0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    uint64_t x = d[w];
+    while (x) {
+      int o = 64 * w + ctz64(x) + 1;
+      x &= x - 1;
+      hel_sing += T(s, o, o);
+    }
+  }
+  double hel_doub = 0.0, hel_tmp = 0.0;
+  // i runs over electrons; the inner loops run over electrons above i
+#pragma unroll
+  for (int wi = 0; wi < W; ++wi) {
+    uint64_t xi = d[wi];
+    while (xi) {
+      int oi = 64 * wi + ctz64(xi) + 1;
+      xi &= xi - 1;
+      int idi = spat(oi);
+      double t = 0.0;
+      bool any = false;
+#pragma unroll
+      for (int wj = wi; wj < W; ++wj) {
+        uint64_t xj = (wj == wi) ? xi : d[wj];
+        while (xj) {
+          int oj = 64 * wj + ctz64(xj) + 1;
+          xj &= xj - 1;
+          int idj = spat(oj);
+          t += U(s, idi, idj, idi, idj);
+          any = true;
+        }
+      }
+      if (any) hel_doub += t;
+    }
+  }
+#pragma unroll
+  for (int wi = 0; wi < W; ++wi) {
+    uint64_t xi = d[wi];
+    while (xi) {
+      int oi = 64 * wi + ctz64(xi) + 1;
+      xi &= xi - 1;
+#pragma unroll
+      for (int wj = wi; wj < W; ++wj) {
+        uint64_t xj = (wj == wi) ? xi : d[wj];
+        while (xj) {
+          int oj = 64 * wj + ctz64(xj) + 1;
+          xj &= xj - 1;
+          if (is_beta(oi) == is_beta(oj)) hel_tmp -= U(s, spat(oi), spat(oj), spat(oj), spat(oi));
+        }
+      }
+    }
+  }
+  return hel_doub + hel_tmp + hel_sing;
+}
+
+// sltcnd_1_kernel (sltcnd.fpp:643-680), no parity
+template <int W>
+__device__ __host__ inline double sltcnd_1(const DevSys& s, const uint64_t* d, int src, int tgt) {
+  int a = spat(src), b = spat(tgt);
+  double hel = 0.0;
+  if (is_beta(src) == is_beta(tgt)) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      uint64_t x = d[w];
+      while (x) {
+        int o = 64 * w + ctz64(x) + 1;
+        x &= x - 1;
+        if (o != src) {
+          int id = spat(o);
+          hel += U(s, a, id, b, id);
+        }
+      }
+    }
+    uint64_t same = is_beta(src) ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      uint64_t x = d[w] & same;
+      while (x) {
+        int o = 64 * w + ctz64(x) + 1;
+        x &= x - 1;
+        if (o != src) {
+          int id = spat(o);
+          hel -= U(s, a, id, id, b);
+        }
+      }
+    }
+  }
+  hel += T(s, src, tgt);
+  return hel;
+}
+
+// sltcnd_2_kernel (sltcnd.fpp:690-711), ex = [s1 s2; t1 t2]
+__device__ __host__ inline double sltcnd_2(const DevSys& s, int s1, int s2, int t1, int t2) {
+  double hel;
+  if (is_beta(s1) == is_beta(t1) && is_beta(s2) == is_beta(t2))
+    hel = U(s, spat(s1), spat(s2), spat(t1), spat(t2));
+  else
+    hel = 0.0;
+  if (is_beta(s1) == is_beta(t2) && is_beta(s2) == is_beta(t1))
+    hel -= U(s, spat(s1), spat(s2), spat(t2), spat(t1));
+  return hel;
+}
+
+__device__ __host__ inline bool hub_neighbours(const DevSys& s, int sa, int sb) {
+  for (int k = 0; k < s.maxnbr; ++k) {
+    int n = s.nbr[(sa - 1) * s.maxnbr + k];
+    if (n == 0) break;
+    if (n == sb) return true;
+  }
+  return false;
+}
+
+// get_helement_det_only / get_helement_lattice for an excitation
+// ic = 1: (src, tgt); ic = 2: (s1, s2, t1, t2) sorted
+template <int W>
+__device__ __host__ inline double helement_ex(const DevSys& s, const uint64_t* d, int ic, int e0,
+                                              int e1, int e2, int e3, bool par) {
+  double hel;
+  if (s.model == 0) {
+    if (ic != 1) return 0.0;
+    hel = (is_beta(e0) == is_beta(e1) && hub_neighbours(s, spat(e0), spat(e1))) ? s.bhub : 0.0;
+  } else if (ic == 1) {
+    hel = sltcnd_1<W>(s, d, e0, e1);
+  } else {
+    hel = sltcnd_2(s, e0, e1, e2, e3);
+  }
+  return par ? -hel : hel;
+}
+
+// get_diagonal_matel (total H_ii)
+template <int W>
+__device__ __host__ inline double diag_helement(const DevSys& s, const uint64_t* d) {
+  if (s.model == 0) {
+    // real_space_hubbard.F90:2303-2321: U times doubly occupied sites
+    int nd = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) nd += popc64(d[w] & (d[w] >> 1) & 0x5555555555555555ull);
+    return s.uhub * (double)nd;
+  }
+  return sltcnd_0<W>(s, d) + s.ecore;
+}
+
+// general H(I, J) through the excitation between two bitstrings
+template <int W>
+__device__ __host__ inline double helement_pair(const DevSys& s, const uint64_t* dI, const uint64_t* dJ) {
+  int ic = excit_level<W>(dI, dJ);
+  if (ic == 0) return diag_helement<W>(s, dI);
+  if (ic > 2) return 0.0;
+  uint64_t src[W], tgt[W];
+#pragma unroll
+  for (int w = 0; w < W; ++w) { src[w] = dI[w] & ~dJ[w]; tgt[w] = dJ[w] & ~dI[w]; }
+  int s1 = select_orb<W>(src, ~0ull, 1), t1 = select_orb<W>(tgt, ~0ull, 1);
+  if (ic == 1) return helement_ex<W>(s, dI, 1, s1, t1, 0, 0, single_parity<W>(dI, s1, t1));
+  int s2 = select_orb<W>(src, ~0ull, 2), t2 = select_orb<W>(tgt, ~0ull, 2);
+  return helement_ex<W>(s, dI, 2, s1, s2, t1, t2, double_parity<W>(dI, s1, s2, t1, t2));
+}
+
+// get_off_diagonal_matel (matel_getter.F90:56-107), tNoBrillouin: H(det, ref)
+template <int W>
+__device__ __host__ inline double offdiag_helement(const DevSys& s, const uint64_t* d) {
+  int ex = excit_level<W>(s.ref, d);
+  if (ex != 1 && ex != 2) return 0.0;
+  return helement_pair<W>(s, d, s.ref);
+}
+
+// ------------------------------------------------ excitation generation
+struct Excit {
+  int ic;          // 0 = null
+  int e0, e1, e2, e3;
+  bool parity;
+  double pgen;
+};
+
+// sample_AliasTable_t (aliasSampling.F90:310-332), 0-based
+__device__ __host__ inline int alias_sample(const AliasEnt* tab, int n, double r) {
+  double x = (double)n * r;
+  int pos = (int)x + 1;
+  double b = x + 1.0 - (double)pos;
+  if (b < 0.0) b = 0.0;
+  AliasEnt e = tab[pos - 1];
+  return (b < e.bias) ? pos - 1 : e.alias;
+}
+
+__device__ __host__ inline int fuse_index(int x, int y) {
+  return x < y ? x + y * (y - 1) / 2 : y + x * (x - 1) / 2;
+}
+
+// gen_excit_rs_hubbard (real_space_hubbard.F90:1938-2033)
+template <int W>
+__device__ __host__ inline Excit gen_hubbard(const DevSys& s, const uint64_t* d, const double* dr) {
+  Excit ex{};
+  int nel = s.nel;
+  int elec = 1 + (int)(dr[0] * (double)nel);
+  double p_elec = 1.0 / (double)nel;
+  int src = select_orb<W>(d, ~0ull, elec);
+  int site = spat(src);
+  double cum_sum = 0.0;
+  double a = fabs(s.bhub);
+  for (int k = 0; k < s.maxnbr; ++k) {
+    int ns = s.nbr[(site - 1) * s.maxnbr + k];
+    if (ns == 0) break;
+    int orb = is_beta(src) ? 2 * ns - 1 : 2 * ns;
+    if (!test_orb<W>(d, orb)) cum_sum += a;
+  }
+  if (cum_sum < kEps) return ex;
+  double r = dr[1] * cum_sum;
+  double cum = 0.0, prev = 0.0;
+  int orb = 0, k = 0;
+  for (; k < s.maxnbr; ++k) {
+    int ns = s.nbr[(site - 1) * s.maxnbr + k];
+    if (ns == 0) break;
+    int o = is_beta(src) ? 2 * ns - 1 : 2 * ns;
+    prev = cum;
+    if (!test_orb<W>(d, o)) cum += a;
+    orb = o;
+    bool last = (k + 1 >= s.maxnbr) || s.nbr[(site - 1) * s.maxnbr + k + 1] == 0;
+    if (!(cum < r) || last) break;
+  }
+  if (cum - prev == 0.0) return ex;
+  double p_orb = (cum - prev) / cum_sum;
+  ex.ic = 1;
+  ex.e0 = src;
+  ex.e1 = orb;
+  ex.parity = single_parity<W>(d, src, orb);
+  ex.pgen = p_elec * p_orb;
+  return ex;
+}
+
+// gen_single_4ind_ex + select_orb_sing (excit_gens_int_weighted.F90:442-620).
+// The cumulative list is recomputed in a second pass in the same order
+// instead of being stored, which reproduces the same FP values.
+template <int W>
+__device__ __host__ inline Excit gen_single_heatbath(const DevSys& s, const uint64_t* d, double dsel,
+                                                     double dorb) {
+  Excit ex{};
+  int nel = s.nel;
+  int elec = 1 + (int)floor(dsel * (double)nel);
+  int src = select_orb<W>(d, ~0ull, elec);
+  int nsp = s.nbasis / 2;
+  int off = is_beta(src) ? 1 : 0;
+  double cum_sum = 0.0;
+  for (int i = 1; i <= nsp; ++i) {
+    int orb = 2 * i - off;
+    double cpt = 0.0;
+    if (!test_orb<W>(d, orb)) cpt = fabs(sltcnd_1<W>(s, d, src, orb));
+    cum_sum += cpt;
+  }
+  if (fabs(cum_sum) <= kEps) return ex;
+  double r = dorb * cum_sum;
+  double cum = 0.0, cpt = 0.0;
+  int orb = 0;
+  for (int i = 1; i <= nsp; ++i) {
+    orb = 2 * i - off;
+    cpt = 0.0;
+    if (!test_orb<W>(d, orb)) cpt = fabs(sltcnd_1<W>(s, d, src, orb));
+    cum += cpt;
+    if (!(cum < r) || i == nsp) break;
+  }
+  if (cpt == 0.0) return ex;
+  double pg = cpt / cum_sum;
+  pg = pg / (double)nel;
+  ex.ic = 1;
+  ex.e0 = src;
+  ex.e1 = orb;
+  ex.parity = single_parity<W>(d, src, orb);
+  ex.pgen = pg;
+  return ex;
+}
+
+// pick_biased_elecs (excit_gens_int_weighted.F90:722-848) on bitstrings: the
+// pair index is decoded with integer arithmetic equivalent to the reference's
+// ceiling((1+sqrt(9+8*idx))/2)
+template <int W>
+__device__ __host__ inline void pick_biased_elecs(const DevSys& s, const uint64_t* d, double r, int& o1,
+                                                  int& o2, double& pgen) {
+  const uint64_t kA = 0xAAAAAAAAAAAAAAAAull, kB = 0x5555555555555555ull;
+  int nalpha = 0, nbeta = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) { nalpha += popc64(d[w] & kA); nbeta += popc64(d[w] & kB); }
+  int aa = nalpha * (nalpha - 1) / 2, bb = nbeta * (nbeta - 1) / 2;
+  int par = aa + bb, ab = nalpha * nbeta;
+  double pb = s.pparallel;
+  if (r < pb) {
+    r = (r / pb) * (double)par;
+    int idx = (int)floor(r);
+    uint64_t mask = kA;
+    if (idx >= aa) { idx -= aa; mask = kB; }
+    int n1 = 2;
+    while (n1 * (n1 - 1) / 2 <= idx) ++n1;
+    int n2 = idx + 1 - ((n1 - 1) * (n1 - 2)) / 2;
+    o1 = select_orb<W>(d, mask, n2);
+    o2 = select_orb<W>(d, mask, n1);
+    pgen = pb / (double)par;
+  } else {
+    pgen = (1.0 - pb) / (double)ab;
+    r = ((r - pb) / (1.0 - pb)) * (double)ab;
+    int idx = (int)floor(r);
+    int na = 1 + idx % nalpha;
+    int nb = 1 + (int)floor((double)idx / (double)nalpha);
+    int oa = select_orb<W>(d, kA, na), ob = select_orb<W>(d, kB, nb);
+    o1 = oa < ob ? oa : ob;
+    o2 = oa < ob ? ob : oa;
+  }
+}
+
+// GAS_doubles_PCHB_spinorb_gen_exc (gasci_pchb_doubles_spinorb_fastweighted.fpp:131-244)
+template <int W>
+__device__ __host__ inline Excit gen_double_pchb(const DevSys& s, const uint64_t* d, const double* dr) {
+  Excit ex{};
+  int nel = s.nel, n = s.nbasis;
+  int s1, s2;
+  double p;
+  if (s.particles == 1) {
+    // draw_PC_FastWeightedParticles_t (gasci_pchb_doubles_select_particles.fpp:515-560)
+    int e1 = (int)(dr[1] * (double)nel) + 1;
+    s1 = select_orb<W>(d, ~0ull, e1);
+    if (!s.j_ok[s1 - 1]) return ex;
+    const AliasEnt* row = s.j_tab + (size_t)(s1 - 1) * n;
+    int j = alias_sample(row, n, dr[2]);
+    double p_J_I = s.j_prob[(size_t)(s1 - 1) * n + j];
+    s2 = j + 1;
+    if (!test_orb<W>(d, s2)) return ex;
+    double p_I_J = s.j_ok[s2 - 1] ? s.j_prob[(size_t)(s2 - 1) * n + (s1 - 1)] : 0.0;
+    p = (p_J_I + p_I_J) / (double)nel;
+    if (s1 > s2) { int t = s1; s1 = s2; s2 = t; }
+  } else {
+    pick_biased_elecs<W>(s, d, dr[1], s1, s2, p);
+  }
+  int ij = fuse_index(s1, s2);
+  if (!s.ab_ok[ij - 1]) return ex;
+  const AliasEnt* row = s.ab_tab + (size_t)(ij - 1) * s.ijmax;
+  int ab = alias_sample(row, s.ijmax, dr[3]) + 1;
+  double pgh = s.ab_prob[(size_t)(ij - 1) * s.ijmax + ab - 1];
+  int t1 = s.ab_tgt[2 * (ab - 1)], t2 = s.ab_tgt[2 * (ab - 1) + 1];
+  if (t1 == t2 || test_orb<W>(d, t1) || test_orb<W>(d, t2)) return ex;
+  if (fabs(pgh) <= kEps) return ex;
+  ex.ic = 2;
+  ex.e0 = s1; ex.e1 = s2; ex.e2 = t1; ex.e3 = t2;
+  ex.parity = double_parity<W>(d, s1, s2, t1, t2);
+  ex.pgen = p * pgh;
+  return ex;
+}
+
+// generate_excitation as bound by init_fcimc_fn_pointers
+template <int W>
+__device__ __host__ inline Excit gen_excit(const DevSys& s, const uint64_t* d, const double* dr) {
+  if (s.model == 0) return gen_hubbard<W>(s, d, dr);
+  Excit ex;
+  if (dr[0] < s.psingles) {  // gen_exc_sd (excitation_generators.F90:112-138)
+    ex = gen_single_heatbath<W>(s, d, dr[1], dr[2]);
+    ex.pgen = ex.pgen * s.psingles;
+  } else {
+    ex = gen_double_pchb<W>(s, d, dr);
+    ex.pgen = ex.pgen * s.pdoubles;
+  }
+  return ex;
+}
+
+// child determinant: encode_child = FindExcitBitDet (DetBitOps.F90:584-623)
+template <int W>
+__device__ __host__ inline void apply_excit(const uint64_t* d, const Excit& ex, uint64_t* out) {
+#pragma unroll
+  for (int w = 0; w < W; ++w) out[w] = d[w];
+  if (ex.ic == 1) {
+    flip_orb<W>(out, ex.e0);
+    flip_orb<W>(out, ex.e1);
+  } else {
+    flip_orb<W>(out, ex.e0);
+    flip_orb<W>(out, ex.e1);
+    flip_orb<W>(out, ex.e2);
+    flip_orb<W>(out, ex.e3);
+  }
+}
+
+// stochastic_round_r (lib/util_mod.fpp:207-224)
+__device__ __host__ inline double stochastic_round(double x, double r) {
+  double i = (double)(int64_t)x;
+  double res = x - i;
+  if (fabs(res) >= 1.0e-12)
+    if (fabs(res) > r) i = i + (x >= 0.0 ? 1.0 : -1.0);
+  return i;
+}
+
+// attempt_create_normal core (fcimc_pointed_fns.F90:178-491)
+__device__ __host__ inline double attempt_create(const DevSys& s, double tau, double hel, double pgen,
+                                                 double wsign, double r, double& raw) {
+  double prob = pgen * s.avmc;
+  double nspawn = -tau * hel * wsign / prob;
+  raw = nspawn;
+  if (s.t_real) {
+    if (s.t_rsc && fabs(nspawn) < s.rsc) nspawn = s.rsc * stochastic_round(nspawn / s.rsc, r);
+  } else {
+    nspawn = stochastic_round(nspawn, r);
+  }
+  return nspawn;
+}
+
+// attempt_die_normal core for one replica (fcimc_pointed_fns.F90:573-705)
+__device__ __host__ inline double attempt_die(bool real_coeff, double fac, double absc, double r) {
+  if (real_coeff) return fac * absc;
+  double rat = fac * absc;
+  double nd = (double)(int64_t)rat;
+  rat = rat - nd;
+  if (fabs(rat) > r) nd = nd + (rat >= 0.0 ? 1.0 : -1.0);
+  return nd;
+}
+
+}  // namespace ngpu

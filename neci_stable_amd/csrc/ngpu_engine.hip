@@ -1,0 +1,1028 @@
+// ngpu_engine.hip -- engine state, set-up and the C ABI declared in
+// include/neci_gpu.h.  One engine = one rank's walkers on one GPU.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "ngpu_kernels.h"
+
+using namespace ngpu;
+
+namespace {
+
+constexpr int kStages = 8;
+
+#define HIP_OK(x)                                                        \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ != hipSuccess) {                                              \
+      fprintf(stderr, "[neci_gpu] %s:%d %s\n", __FILE__, __LINE__,       \
+              hipGetErrorString(e_));                                    \
+      return NGPU_ERR_HIP;                                               \
+    }                                                                    \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t n) {
+  if (n == 0) n = 1;
+  return hipMalloc((void**)p, n * sizeof(T));
+}
+
+// NECI flag bit numbers (bit_rep_data.F90:87-118)
+const int kMneciInit[16] = {13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 57, 58};
+const int kMneciDeterm[16] = {41, 42, 43, 44, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 61, 62};
+const int kNeciInit[2] = {13, 14};
+const int kNeciDeterm[2] = {19, 20};
+
+}  // namespace
+
+struct ngpu_engine {
+  ngpu_sys_t sys{};
+  int W = 1, S = 1;
+  int nb_chunk = 1024;  // blocks of the chunked kernels (prepass, scans, radix)
+  int nb_grid = 2048;   // blocks of the grid-stride kernels
+  hipStream_t stream = nullptr;
+  DevSys ds{};
+  Walk wk{};
+  Counters* cnt = nullptr;
+  ngpu_iter_out_t* dout = nullptr;
+  ngpu_iter_out_t* hout = nullptr;  // pinned
+  Counters* hcnt = nullptr;         // pinned
+  // attempts
+  int32_t* att_cnt = nullptr;
+  int64_t* att_off = nullptr;
+  uint8_t* pinfo = nullptr;
+  int64_t* blk_part = nullptr;
+  // spawns, sort, merge, new
+  int64_t scap = 0;
+  uint64_t* sp_key = nullptr;
+  double* sp_val = nullptr;
+  uint32_t* sp_info = nullptr;
+  uint64_t* sk[2] = {nullptr, nullptr};
+  uint32_t* si[2] = {nullptr, nullptr};
+  uint32_t* hist = nullptr;
+  uint64_t* u_key = nullptr;
+  double* u_sgn = nullptr;
+  uint32_t* u_flag = nullptr;
+  uint64_t* n_key = nullptr;
+  double* n_sgn = nullptr;
+  uint32_t* n_flag = nullptr;
+  // system tables
+  int32_t* d_rh = nullptr;
+  int32_t* d_nbr = nullptr;
+  double* d_tmat = nullptr;
+  double* d_umat = nullptr;
+  AliasEnt* d_ab_tab = nullptr;
+  double* d_ab_prob = nullptr;
+  uint8_t* d_ab_ok = nullptr;
+  AliasEnt* d_j_tab = nullptr;
+  double* d_j_prob = nullptr;
+  uint8_t* d_j_ok = nullptr;
+  int32_t* d_ab_tgt = nullptr;
+  bool have_hash = false, have_ref = false, have_model = false, have_excitgen = false;
+  bool pending = false;
+  // profiling
+  bool prof = false;
+  hipEvent_t ev[kStages + 1];
+  bool ev_ok = false;
+  double stage_ms[kStages] = {0};
+  int64_t prof_iters = 0;
+  bool prof_pending = false;
+};
+
+namespace {
+
+int ready(ngpu_t* h) {
+  if (!h) return NGPU_ERR_ARG;
+  if (!h->have_hash || !h->have_ref || !h->have_model) return NGPU_ERR_STATE;
+  if (h->sys.model == NGPU_MODEL_MOLECULAR && !h->have_excitgen) return NGPU_ERR_STATE;
+  return NGPU_OK;
+}
+
+int err_from_mask(unsigned int m) {
+  for (int e = 1; e < 16; ++e)
+    if (m & (1u << e)) return e;
+  return NGPU_OK;
+}
+
+__global__ void k_begin_iter(ngpu_iter_out_t* out, Counters* cnt) {
+  double* g = (double*)out;
+  for (int i = threadIdx.x; i < S_NDOUBLE; i += blockDim.x) g[i] = 0.0;
+  if (threadIdx.x == 0) {
+    cnt->n_attempts = 0;
+    cnt->n_spawned = 0;
+    cnt->n_unique = 0;
+    cnt->n_new = 0;
+    cnt->n_occ = 0;
+    cnt->err = 0;
+    out->tot_walkers = 0;
+    out->n_attempts = out->n_spawned = out->n_unique = out->n_new = 0;
+    out->err = 0;
+  }
+}
+
+__global__ void k_end_iter(ngpu_iter_out_t* out, Counters* cnt, int64_t scap) {
+  out->tot_walkers = (int64_t)cnt->n_occ;
+  out->n_attempts = (int64_t)cnt->n_attempts;
+  int64_t ns = (int64_t)cnt->n_spawned;
+  out->n_spawned = ns < scap ? ns : scap;
+  out->n_unique = (int64_t)cnt->n_unique;
+  out->n_new = (int64_t)cnt->n_new;
+  out->err = (int32_t)cnt->err;
+}
+
+__global__ void k_set_count(unsigned long long* p, unsigned long long v) { *p = v; }
+
+void rec(ngpu_t* h, int stage) {
+  if (h->prof && h->ev_ok) (void)hipEventRecord(h->ev[stage], h->stream);
+}
+
+// radix sort of the spawned keys; returns the buffer index holding the result
+template <int W>
+int radix_sort(ngpu_t* h, const unsigned long long* n_ptr) {
+  int nbits_hi = W == 1 ? h->sys.nbasis : 64;      // bits used in word 0
+  int nbits_lo = W == 2 ? h->sys.nbasis - 64 : 0;  // bits used in word 1
+  struct Pass { int word, shift; };
+  std::vector<Pass> passes;
+  if (W == 2)
+    for (int sh = 0; sh < nbits_lo; sh += 8) passes.push_back({1, sh});
+  for (int sh = 0; sh < nbits_hi; sh += 8) passes.push_back({0, sh});
+  int cur = 0;
+  const uint64_t* kin = h->sp_key;
+  const uint32_t* iin = nullptr;
+  for (size_t p = 0; p < passes.size(); ++p) {
+    int out = (int)(p & 1);
+    hipLaunchKernelGGL(k_radix_hist<W>, dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, kin, n_ptr, h->scap,
+                       passes[p].word, passes[p].shift, h->hist);
+    hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(kBlock), 0, h->stream, h->hist, h->nb_chunk);
+    hipLaunchKernelGGL(k_radix_scatter<W>, dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, kin, iin,
+                       h->sk[out], h->si[out], n_ptr, h->scap, passes[p].word, passes[p].shift, h->hist);
+    kin = h->sk[out];
+    iin = h->si[out];
+    cur = out;
+  }
+  return cur;
+}
+
+// merge -> annihilate -> insert -> stats, shared by iterate and the explicit-list entry
+template <int W>
+void annihilation_chain(ngpu_t* h, uint32_t iter) {
+  rec(h, 3);
+  int cur = radix_sort<W>(h, &h->cnt->n_spawned);
+  rec(h, 4);
+  hipLaunchKernelGGL(k_merge<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->cnt, h->dout,
+                     h->sk[cur], h->si[cur], h->sp_val, h->sp_info, h->scap, h->u_key, h->u_sgn, h->u_flag);
+  rec(h, 5);
+  hipLaunchKernelGGL(k_annihilate<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
+                     h->dout, h->u_key, h->u_sgn, h->u_flag, h->n_key, h->n_sgn, h->n_flag, iter);
+  rec(h, 6);
+  hipLaunchKernelGGL(k_insert<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
+                     h->n_key, h->n_sgn, h->n_flag, &h->cnt->n_new);
+  hipLaunchKernelGGL(k_post_insert, dim3(1), dim3(1), 0, h->stream, h->cnt);
+  rec(h, 7);
+  hipLaunchKernelGGL(k_stats<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->dout,
+                     iter);
+  hipLaunchKernelGGL(k_end_iter, dim3(1), dim3(1), 0, h->stream, h->dout, h->cnt, h->scap);
+  rec(h, 8);
+}
+
+template <int W>
+void iterate_chain(ngpu_t* h, const ngpu_iter_in_t* in) {
+  Shift sft;
+  for (int r = 0; r < kMaxRep; ++r) sft.v[r] = r < h->S ? in->diag_sft[r] : 0.0;
+  rec(h, 0);
+  hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->cnt);
+  hipLaunchKernelGGL(k_prepass<W>, dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
+                     h->dout, h->att_cnt, h->pinfo, h->blk_part, sft, in->tau, in->iter);
+  rec(h, 1);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, h->stream, h->blk_part, h->nb_chunk, h->cnt);
+  hipLaunchKernelGGL(k_scan_apply, dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, h->att_cnt, h->att_off,
+                     h->blk_part, h->cnt, h->S);
+  rec(h, 2);
+  hipLaunchKernelGGL(k_spawn<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->dout,
+                     h->att_off, h->pinfo, h->sp_key, h->sp_val, h->sp_info, h->scap, in->tau, in->iter);
+  annihilation_chain<W>(h, in->iter);
+}
+
+int finish(ngpu_t* h, ngpu_iter_out_t* out) {
+  HIP_OK(hipStreamSynchronize(h->stream));
+  h->pending = false;
+  if (h->prof && h->ev_ok && h->prof_pending) {
+    for (int s = 0; s < kStages; ++s) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, h->ev[s], h->ev[s + 1]) == hipSuccess) h->stage_ms[s] += ms;
+    }
+    h->prof_iters++;
+    h->prof_pending = false;
+  }
+  if (out) *out = *h->hout;
+  return err_from_mask((unsigned)h->hout->err);
+}
+
+// ---- host restatement of the PCHB alias tables ----
+// init_AliasTable_t (aliasSampling.F90:209-297), init_probs (:412-418)
+void build_alias(const std::vector<double>& w, AliasEnt* tab, double* prob) {
+  int n = (int)w.size();
+  double sum = 0.0;
+  for (int i = 0; i < n; ++i) sum += w[i];
+  std::vector<double> bias(n);
+  std::vector<int> over, under;
+  over.reserve(n);
+  under.reserve(n);
+  std::vector<int> alias(n, 0);
+  for (int i = 0; i < n; ++i) bias[i] = w[i] / sum * (double)n;
+  for (int i = 0; i < n; ++i) (bias[i] > 1.0 ? over : under).push_back(i);
+  std::reverse(over.begin(), over.end());
+  while (!over.empty() && !under.empty()) {
+    int i = over.back(), j = under.back();
+    alias[j] = i;
+    bias[i] = bias[i] + bias[j] - 1.0;
+    over.pop_back();
+    under.pop_back();
+    (bias[i] > 1.0 ? over : under).push_back(i);
+  }
+  for (int i : over) { bias[i] = 1.0; alias[i] = i; }
+  for (int i : under) { bias[i] = 1.0; alias[i] = i; }
+  for (int i = 0; i < n; ++i) {
+    tab[i].bias = bias[i];
+    tab[i].alias = alias[i];
+    tab[i].pad = 0;
+    prob[i] = w[i] / sum;
+  }
+}
+
+template <int W>
+__global__ void k_batch_hash(DevSys s, const uint64_t* d, int64_t n, int32_t* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (int32_t)find_walker_hash<W>(d + i * W, s.rh, s.L);
+}
+
+template <int W>
+__global__ void k_batch_hel(DevSys s, const uint64_t* a, const uint64_t* b, int64_t n, double* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = helement_pair<W>(s, a + i * W, b + i * W);
+}
+
+template <int W>
+__global__ void k_batch_diag(DevSys s, const uint64_t* d, int64_t n, double* diag, double* off) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    diag[i] = diag_helement<W>(s, d + i * W);
+    off[i] = offdiag_helement<W>(s, d + i * W);
+  }
+}
+
+template <int W>
+__global__ void k_batch_gen(DevSys s, const uint64_t* d, const double* draws, int64_t n, uint64_t* dj, int32_t* ic,
+                            int32_t* ex, int32_t* par, double* pgen) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double dr[6];
+    for (int q = 0; q < 6; ++q) dr[q] = draws[i * 6 + q];
+    Excit e = gen_excit<W>(s, d + i * W, dr);
+    ic[i] = e.ic;
+    ex[4 * i] = e.e0; ex[4 * i + 1] = e.e1; ex[4 * i + 2] = e.e2; ex[4 * i + 3] = e.e3;
+    par[i] = e.parity ? 1 : 0;
+    pgen[i] = e.ic ? e.pgen : 0.0;
+    if (e.ic) apply_excit<W>(d + i * W, e, dj + i * W);
+    else
+      for (int w = 0; w < W; ++w) dj[i * W + w] = 0;
+  }
+}
+
+// calc_pgen of the bound generator (calc_pgen_rs_hubbard, real_space_hubbard.F90:2035;
+// pgen_single_4ind, excit_gens_int_weighted.F90:491; GAS_doubles_PCHB_spinorb_get_pgen,
+// gasci_pchb_doubles_spinorb_fastweighted.fpp:246; get_pgen_sd, excitation_generators.F90:141)
+template <int W>
+__device__ double calc_pgen(const DevSys& s, const uint64_t* d, int ic, const int32_t* ex) {
+  if (s.model == 0) {
+    if (ic != 1) return 0.0;
+    int src = ex[0], tgt = ex[1], site = spat(src);
+    double cum_sum = 0.0, cpt = 0.0;
+    for (int k = 0; k < s.maxnbr; ++k) {
+      int ns = s.nbr[(site - 1) * s.maxnbr + k];
+      if (ns == 0) break;
+      int orb = is_beta(src) ? 2 * ns - 1 : 2 * ns;
+      double elem = test_orb<W>(d, orb) ? 0.0 : fabs(s.bhub);
+      cum_sum += elem;
+      if (orb == tgt) cpt = elem;
+    }
+    if (cum_sum < kEps) return 0.0;
+    return (1.0 / (double)s.nel) * (cpt / cum_sum);
+  }
+  if (ic == 1) {
+    int src = ex[0], tgt = ex[1];
+    double pgen = 1.0 / (double)s.nel;
+    int nsp = s.nbasis / 2, off = is_beta(tgt) ? 1 : 0;
+    double cum_sum = 0.0, cpt_tgt = 0.0;
+    for (int i = 1; i <= nsp; ++i) {
+      int orb = 2 * i - off;
+      if (!test_orb<W>(d, orb)) {
+        double cpt = fabs(sltcnd_1<W>(s, d, src, orb));
+        cum_sum += cpt;
+        if (orb == tgt) cpt_tgt = cpt;
+      }
+    }
+    if (fabs(cum_sum) <= kEps) pgen = 0.0; else pgen = pgen * cpt_tgt / cum_sum;
+    return s.psingles * pgen;
+  }
+  if (ic == 2) {
+    int I = ex[0], J = ex[1], n = s.nbasis;
+    double p;
+    if (s.particles == 1) {
+      double a = s.j_ok[I - 1] ? s.j_prob[(size_t)(I - 1) * n + (J - 1)] : 0.0;
+      double b = s.j_ok[J - 1] ? s.j_prob[(size_t)(J - 1) * n + (I - 1)] : 0.0;
+      p = (a + b) / (double)s.nel;
+    } else {
+      const uint64_t kA = 0xAAAAAAAAAAAAAAAAull, kB = 0x5555555555555555ull;
+      int na = 0, nb = 0;
+      for (int w = 0; w < W; ++w) { na += popc64(d[w] & kA); nb += popc64(d[w] & kB); }
+      int par = na * (na - 1) / 2 + nb * (nb - 1) / 2;
+      if (is_beta(I) == is_beta(J)) p = s.pparallel / (double)par;
+      else p = (1.0 - s.pparallel) / (double)(na * nb);
+    }
+    int ij = fuse_index(I, J), ab = fuse_index(ex[2], ex[3]);
+    double ph = s.ab_ok[ij - 1] ? s.ab_prob[(size_t)(ij - 1) * s.ijmax + ab - 1] : 0.0;
+    p = p * ph;
+    return s.pdoubles * p;
+  }
+  return 0.0;
+}
+
+template <int W>
+__global__ void k_batch_pgen(DevSys s, const uint64_t* d, const int32_t* ic, const int32_t* ex, int64_t n, double* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = calc_pgen<W>(s, d + i * W, ic[i], ex + 4 * i);
+}
+
+__global__ void k_batch_die(bool real_coeff, const double* kii, const double* sg, const double* u, int64_t n,
+                            double tau, double sft, double* nd) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    nd[i] = attempt_die(real_coeff, tau * (kii[i] - sft), fabs(sg[i]), u[i]);
+}
+
+__global__ void k_batch_create(DevSys s, const double* hel, const double* pgen, const double* psg, const double* u,
+                               int64_t n, double tau, double* child) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double raw;
+    child[i] = attempt_create(s, tau, hel[i], pgen[i], psg[i] >= 0.0 ? 1.0 : -1.0, u[i], raw);
+  }
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  hipError_t alloc(size_t n) { return dalloc(&p, n); }
+};
+
+int grid_for(int64_t n) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > 4096) g = 4096;
+  return (int)g;
+}
+
+int upload_rows_to_new_list(ngpu_t* h, const int64_t* rows, int64_t n) {
+  const int W = h->W, S = h->S, RL = W + S + 1;
+  std::vector<uint64_t> key((size_t)n * W);
+  std::vector<double> sg((size_t)n * S);
+  std::vector<uint32_t> fl((size_t)n);
+  const int* init_bits = h->sys.flag_layout == NGPU_FLAGS_NECI ? kNeciInit : kMneciInit;
+  const int* det_bits = h->sys.flag_layout == NGPU_FLAGS_NECI ? kNeciDeterm : kMneciDeterm;
+  int maxr = h->sys.flag_layout == NGPU_FLAGS_NECI ? 2 : 16;
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t* row = rows + i * RL;
+    for (int w = 0; w < W; ++w) key[i * W + w] = (uint64_t)row[w];
+    memcpy(&sg[i * S], row + W, sizeof(double) * S);
+    uint64_t nf = (uint64_t)row[W + S];
+    uint32_t f = 0;
+    for (int r = 0; r < S && r < maxr; ++r) {
+      if ((nf >> init_bits[r]) & 1ull) f |= 1u << r;
+      if ((nf >> det_bits[r]) & 1ull) f |= kFlagCore;
+    }
+    fl[i] = f;
+  }
+  HIP_OK(hipMemcpyAsync(h->n_key, key.data(), key.size() * 8, hipMemcpyHostToDevice, h->stream));
+  HIP_OK(hipMemcpyAsync(h->n_sgn, sg.data(), sg.size() * 8, hipMemcpyHostToDevice, h->stream));
+  HIP_OK(hipMemcpyAsync(h->n_flag, fl.data(), fl.size() * 4, hipMemcpyHostToDevice, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return NGPU_OK;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
+  if (!out || !sys) return NGPU_ERR_ARG;
+  *out = nullptr;
+  if (sys->nel < 1 || sys->nel > 64 || sys->nbasis < 2 || sys->nbasis > 127 || sys->nel > sys->nbasis)
+    return NGPU_ERR_ARG;
+  if (sys->n_replicas < 1 || sys->n_replicas > NGPU_MAX_REPLICAS) return NGPU_ERR_ARG;
+  if (sys->max_walkers < 1 || sys->max_walkers > (int64_t)INT32_MAX || sys->max_spawned < 1 ||
+      sys->max_spawned > (int64_t)UINT32_MAX || sys->n_hash_buckets < 1)
+    return NGPU_ERR_ARG;
+  if (sys->model != NGPU_MODEL_HUBBARD && sys->model != NGPU_MODEL_MOLECULAR) return NGPU_ERR_ARG;
+  ngpu_t* h = new ngpu_engine();
+  h->sys = *sys;
+  h->W = sys->nbasis / 64 + 1;
+  h->S = sys->n_replicas;
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+    delete h;
+    return NGPU_ERR_HIP;
+  }
+  h->nb_chunk = std::max(256, 4 * prop.multiProcessorCount);
+  h->nb_grid = std::max(512, 8 * prop.multiProcessorCount);
+  int rc = NGPU_OK;
+  auto fail = [&](hipError_t e) {
+    if (e != hipSuccess && rc == NGPU_OK) {
+      fprintf(stderr, "[neci_gpu] create: %s\n", hipGetErrorString(e));
+      rc = NGPU_ERR_HIP;
+    }
+  };
+  fail(hipStreamCreate(&h->stream));
+  const int64_t cap = sys->max_walkers, scap = sys->max_spawned;
+  const int W = h->W, S = h->S;
+  h->scap = scap;
+  Walk& wk = h->wk;
+  wk.cap = cap;
+  wk.L = sys->n_hash_buckets;
+  fail(dalloc(&wk.keys, (size_t)cap * W));
+  fail(dalloc(&wk.sgn, (size_t)cap * S));
+  fail(dalloc(&wk.flags, (size_t)cap));
+  fail(dalloc(&wk.diagH, (size_t)cap));
+  fail(dalloc(&wk.offdiagH, (size_t)cap));
+  fail(dalloc(&wk.hpos, (size_t)cap));
+  fail(dalloc(&wk.freeq, (size_t)cap));
+  fail(dalloc(&wk.htab, (size_t)wk.L * 8 * W));
+  fail(dalloc(&h->att_cnt, (size_t)cap * S));
+  fail(dalloc(&h->att_off, (size_t)cap * S + 1));
+  fail(dalloc(&h->pinfo, (size_t)cap * S));
+  fail(dalloc(&h->blk_part, (size_t)h->nb_chunk));
+  fail(dalloc(&h->sp_key, (size_t)scap * W));
+  fail(dalloc(&h->sp_val, (size_t)scap));
+  fail(dalloc(&h->sp_info, (size_t)scap));
+  for (int q = 0; q < 2; ++q) {
+    fail(dalloc(&h->sk[q], (size_t)scap * W));
+    fail(dalloc(&h->si[q], (size_t)scap));
+  }
+  fail(dalloc(&h->hist, (size_t)h->nb_chunk * kRadixBins));
+  fail(dalloc(&h->u_key, (size_t)scap * W));
+  fail(dalloc(&h->u_sgn, (size_t)scap * S));
+  fail(dalloc(&h->u_flag, (size_t)scap));
+  int64_t ncap = std::max<int64_t>(scap, cap);
+  fail(dalloc(&h->n_key, (size_t)ncap * W));
+  fail(dalloc(&h->n_sgn, (size_t)ncap * S));
+  fail(dalloc(&h->n_flag, (size_t)ncap));
+  fail(dalloc(&h->cnt, 1));
+  fail(dalloc(&h->dout, 1));
+  fail(hipHostMalloc((void**)&h->hout, sizeof(ngpu_iter_out_t), hipHostMallocDefault));
+  fail(hipHostMalloc((void**)&h->hcnt, sizeof(Counters), hipHostMallocDefault));
+  if (rc == NGPU_OK) {
+    fail(hipMemsetAsync(wk.htab, 0xFF, (size_t)wk.L * 8 * W * sizeof(uint64_t), h->stream));
+    fail(hipMemsetAsync(h->cnt, 0, sizeof(Counters), h->stream));
+    fail(hipMemsetAsync(h->dout, 0, sizeof(ngpu_iter_out_t), h->stream));
+    fail(hipMemsetAsync(wk.flags, 0, (size_t)cap * sizeof(uint32_t), h->stream));
+    fail(hipStreamSynchronize(h->stream));
+    memset(h->hout, 0, sizeof(ngpu_iter_out_t));
+  }
+  h->ds.nel = sys->nel;
+  h->ds.nbasis = sys->nbasis;
+  h->ds.S = S;
+  h->ds.model = sys->model;
+  h->ds.t_init = sys->t_trunc_initiator;
+  h->ds.t_real = sys->t_all_real_coeff;
+  h->ds.t_rsc = sys->t_real_spawn_cutoff;
+  h->ds.init_walk_no = sys->initiator_walk_no;
+  h->ds.rsc = sys->real_spawn_cutoff;
+  h->ds.occ_thresh = sys->occupied_thresh;
+  h->ds.avmc = sys->av_mc_excits;
+  h->ds.seed = sys->seed;
+  h->ds.L = sys->n_hash_buckets;
+  h->ds.pdoubles = 1.0;
+  if (rc == NGPU_OK) {
+    h->ev_ok = true;
+    for (int s = 0; s <= kStages; ++s)
+      if (hipEventCreate(&h->ev[s]) != hipSuccess) h->ev_ok = false;
+  }
+  if (rc != NGPU_OK) {
+    ngpu_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return NGPU_OK;
+}
+
+int ngpu_destroy(ngpu_t* h) {
+  if (!h) return NGPU_ERR_ARG;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  void* ptrs[] = {h->wk.keys, h->wk.sgn, h->wk.flags, h->wk.diagH, h->wk.offdiagH, h->wk.hpos, h->wk.freeq,
+                  h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->sp_key, h->sp_val, h->sp_info,
+                  h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->u_key, h->u_sgn, h->u_flag, h->n_key,
+                  h->n_sgn, h->n_flag, h->cnt, h->dout, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_ab_tab,
+                  h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (h->hout) (void)hipHostFree(h->hout);
+  if (h->hcnt) (void)hipHostFree(h->hcnt);
+  if (h->ev_ok)
+    for (int s = 0; s <= kStages; ++s) (void)hipEventDestroy(h->ev[s]);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return NGPU_OK;
+}
+
+int ngpu_set_integrals(ngpu_t* h, const double* tmat2d, const double* umat, int64_t umat_len, double ecore) {
+  if (!h || !tmat2d || !umat || umat_len < 1 || h->sys.model != NGPU_MODEL_MOLECULAR) return NGPU_ERR_ARG;
+  int64_t nsp = h->sys.nbasis / 2;
+  int64_t npair = nsp * (nsp + 1) / 2;
+  if (umat_len < npair * (npair + 1) / 2) return NGPU_ERR_ARG;
+  size_t nt = (size_t)h->sys.nbasis * h->sys.nbasis;
+  if (h->d_tmat) (void)hipFree(h->d_tmat);
+  if (h->d_umat) (void)hipFree(h->d_umat);
+  h->d_tmat = nullptr;
+  h->d_umat = nullptr;
+  HIP_OK(dalloc(&h->d_tmat, nt));
+  HIP_OK(dalloc(&h->d_umat, (size_t)umat_len));
+  HIP_OK(hipMemcpy(h->d_tmat, tmat2d, nt * 8, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(h->d_umat, umat, (size_t)umat_len * 8, hipMemcpyHostToDevice));
+  h->ds.tmat = h->d_tmat;
+  h->ds.umat = h->d_umat;
+  h->ds.ecore = ecore;
+  h->have_model = true;
+  return NGPU_OK;
+}
+
+int ngpu_set_lattice(ngpu_t* h, const int32_t* nbr, int32_t nsites, int32_t maxnbr, double bhub, double uhub) {
+  if (!h || !nbr || h->sys.model != NGPU_MODEL_HUBBARD) return NGPU_ERR_ARG;
+  if (nsites * 2 != h->sys.nbasis || maxnbr < 1 || maxnbr > 16) return NGPU_ERR_ARG;
+  for (int64_t i = 0; i < (int64_t)nsites * maxnbr; ++i)
+    if (nbr[i] < 0 || nbr[i] > nsites) return NGPU_ERR_ARG;
+  if (h->d_nbr) (void)hipFree(h->d_nbr);
+  h->d_nbr = nullptr;
+  HIP_OK(dalloc(&h->d_nbr, (size_t)nsites * maxnbr));
+  HIP_OK(hipMemcpy(h->d_nbr, nbr, (size_t)nsites * maxnbr * 4, hipMemcpyHostToDevice));
+  h->ds.nbr = h->d_nbr;
+  h->ds.nsites = nsites;
+  h->ds.maxnbr = maxnbr;
+  h->ds.bhub = bhub;
+  h->ds.uhub = uhub;
+  h->have_model = true;
+  return NGPU_OK;
+}
+
+// GAS_doubles_PCHB_spinorb_compute_samplers (gasci_pchb_doubles_spinorb_fastweighted.fpp:280-347)
+// for the FCI case and the J sampler of init_PC_WeightedParticles_t
+// (gasci_pchb_doubles_select_particles.fpp:254-301)
+int ngpu_set_excitgen(ngpu_t* h, int32_t singles, int32_t particles, double psingles, double pparallel) {
+  if (!h || h->sys.model != NGPU_MODEL_MOLECULAR || !h->have_model) return NGPU_ERR_ARG;
+  if (singles != NGPU_SINGLES_HEATBATH) return NGPU_ERR_ARG;
+  if (particles != NGPU_PART_UNIF_UNIF && particles != NGPU_PART_UNIF_FAST) return NGPU_ERR_ARG;
+  if (!(psingles >= 0.0 && psingles <= 1.0) || !(pparallel >= 0.0 && pparallel <= 1.0)) return NGPU_ERR_ARG;
+  const int n = h->sys.nbasis;
+  const int ijmax = fuse_index(n, n);
+  std::vector<double> tmat((size_t)n * n);
+  int64_t nsp = n / 2, npair = nsp * (nsp + 1) / 2, ulen = npair * (npair + 1) / 2;
+  std::vector<double> umat((size_t)ulen);
+  HIP_OK(hipMemcpy(tmat.data(), h->d_tmat, tmat.size() * 8, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(umat.data(), h->d_umat, umat.size() * 8, hipMemcpyDeviceToHost));
+  DevSys hs = h->ds;
+  hs.tmat = tmat.data();
+  hs.umat = umat.data();
+  std::vector<AliasEnt> ab_tab((size_t)ijmax * ijmax);
+  std::vector<double> ab_prob((size_t)ijmax * ijmax, 0.0);
+  std::vector<uint8_t> ab_ok(ijmax, 0);
+  std::vector<double> ijw((size_t)n * n, 0.0);
+  std::vector<double> w(ijmax);
+  for (int I = 1; I <= n; ++I)
+    for (int J = I + 1; J <= n; ++J) {
+      std::fill(w.begin(), w.end(), 0.0);
+      for (int A = 1; A <= n; ++A) {
+        if (A == I || A == J) continue;
+        for (int B = 1; B <= n; ++B) {
+          if (B == I || B == J || B == A) continue;
+          int t1 = std::min(A, B), t2 = std::max(A, B);
+          int ms_s = (is_beta(I) ? -1 : 1) + (is_beta(J) ? -1 : 1);
+          int ms_t = (is_beta(t1) ? -1 : 1) + (is_beta(t2) ? -1 : 1);
+          if (ms_s == ms_t) w[fuse_index(A, B) - 1] = fabs(sltcnd_2(hs, I, J, t1, t2));
+        }
+      }
+      double sw = 0.0;
+      for (double x : w) sw += x;
+      int IJ = fuse_index(I, J);
+      if (!(fabs(sw) <= kEps)) {
+        build_alias(w, &ab_tab[(size_t)(IJ - 1) * ijmax], &ab_prob[(size_t)(IJ - 1) * ijmax]);
+        ab_ok[IJ - 1] = 1;
+      }
+      ijw[(size_t)(I - 1) + (size_t)(J - 1) * n] = sw;
+      ijw[(size_t)(J - 1) + (size_t)(I - 1) * n] = sw;
+    }
+  std::vector<AliasEnt> j_tab((size_t)n * n);
+  std::vector<double> j_prob((size_t)n * n, 0.0);
+  std::vector<uint8_t> j_ok(n, 0);
+  for (int I = 1; I <= n; ++I) {
+    std::vector<double> row(ijw.begin() + (size_t)(I - 1) * n, ijw.begin() + (size_t)I * n);
+    double sw = 0.0;
+    for (double x : row) sw += x;
+    if (!(fabs(sw) <= kEps)) {
+      build_alias(row, &j_tab[(size_t)(I - 1) * n], &j_prob[(size_t)(I - 1) * n]);
+      j_ok[I - 1] = 1;
+    }
+  }
+  std::vector<int32_t> tgt(2 * (size_t)ijmax);
+  for (int A = 1; A <= n; ++A)
+    for (int B = 1; B <= A; ++B) {
+      int AB = fuse_index(A, B);
+      tgt[2 * (AB - 1)] = B;
+      tgt[2 * (AB - 1) + 1] = A;
+    }
+  void* olds[] = {h->d_ab_tab, h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt};
+  for (void* p : olds)
+    if (p) (void)hipFree(p);
+  HIP_OK(dalloc(&h->d_ab_tab, ab_tab.size()));
+  HIP_OK(dalloc(&h->d_ab_prob, ab_prob.size()));
+  HIP_OK(dalloc(&h->d_ab_ok, ab_ok.size()));
+  HIP_OK(dalloc(&h->d_j_tab, j_tab.size()));
+  HIP_OK(dalloc(&h->d_j_prob, j_prob.size()));
+  HIP_OK(dalloc(&h->d_j_ok, j_ok.size()));
+  HIP_OK(dalloc(&h->d_ab_tgt, tgt.size()));
+  HIP_OK(hipMemcpy(h->d_ab_tab, ab_tab.data(), ab_tab.size() * sizeof(AliasEnt), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(h->d_ab_prob, ab_prob.data(), ab_prob.size() * 8, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(h->d_ab_ok, ab_ok.data(), ab_ok.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(h->d_j_tab, j_tab.data(), j_tab.size() * sizeof(AliasEnt), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(h->d_j_prob, j_prob.data(), j_prob.size() * 8, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(h->d_j_ok, j_ok.data(), j_ok.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(h->d_ab_tgt, tgt.data(), tgt.size() * 4, hipMemcpyHostToDevice));
+  h->ds.ab_tab = h->d_ab_tab;
+  h->ds.ab_prob = h->d_ab_prob;
+  h->ds.ab_ok = h->d_ab_ok;
+  h->ds.j_tab = h->d_j_tab;
+  h->ds.j_prob = h->d_j_prob;
+  h->ds.j_ok = h->d_j_ok;
+  h->ds.ab_tgt = h->d_ab_tgt;
+  h->ds.ijmax = ijmax;
+  h->ds.singles = singles;
+  h->ds.particles = particles;
+  h->ds.psingles = psingles;
+  h->ds.pdoubles = 1.0 - psingles;
+  h->ds.pparallel = pparallel;
+  h->have_excitgen = true;
+  return NGPU_OK;
+}
+
+int ngpu_set_hash(ngpu_t* h, const int32_t* random_hash2) {
+  if (!h || !random_hash2) return NGPU_ERR_ARG;
+  for (int i = 0; i < h->sys.nbasis; ++i)
+    if (random_hash2[i] < 1 || random_hash2[i] > 1000 * h->sys.nbasis) return NGPU_ERR_ARG;
+  if (!h->d_rh) HIP_OK(dalloc(&h->d_rh, (size_t)h->sys.nbasis));
+  HIP_OK(hipMemcpy(h->d_rh, random_hash2, (size_t)h->sys.nbasis * 4, hipMemcpyHostToDevice));
+  h->ds.rh = h->d_rh;
+  h->have_hash = true;
+  return NGPU_OK;
+}
+
+int ngpu_set_reference(ngpu_t* h, const int64_t* ilut_ref, double hii) {
+  if (!h || !ilut_ref) return NGPU_ERR_ARG;
+  int nel = 0;
+  for (int w = 0; w < h->W; ++w) nel += __builtin_popcountll((uint64_t)ilut_ref[w]);
+  if (nel != h->sys.nel) return NGPU_ERR_ARG;
+  h->ds.ref[0] = (uint64_t)ilut_ref[0];
+  h->ds.ref[1] = h->W > 1 ? (uint64_t)ilut_ref[1] : 0;
+  h->ds.hii = hii;
+  h->have_ref = true;
+  return NGPU_OK;
+}
+
+int ngpu_upload_walkers(ngpu_t* h, const int64_t* rows, int64_t n) {
+  int rc = ready(h);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && !rows)) return NGPU_ERR_ARG;
+  if (n == 0) return NGPU_OK;
+  int64_t ncap = std::max<int64_t>(h->scap, h->wk.cap);
+  if (n > ncap) return NGPU_ERR_WALKERS_FULL;
+  for (int64_t i = 0; i < n; ++i) {
+    int nel = 0;
+    for (int w = 0; w < h->W; ++w) nel += __builtin_popcountll((uint64_t)rows[i * (h->W + h->S + 1) + w]);
+    if (nel != h->sys.nel) return NGPU_ERR_ARG;
+  }
+  rc = upload_rows_to_new_list(h, rows, n);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_new, (unsigned long long)n);
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, (unsigned long long*)&h->cnt->err, 0ull);
+  if (h->W == 1)
+    hipLaunchKernelGGL(k_insert<1>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->n_key,
+                       h->n_sgn, h->n_flag, &h->cnt->n_new);
+  else
+    hipLaunchKernelGGL(k_insert<2>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->n_key,
+                       h->n_sgn, h->n_flag, &h->cnt->n_new);
+  hipLaunchKernelGGL(k_post_insert, dim3(1), dim3(1), 0, h->stream, h->cnt);
+  HIP_OK(hipMemcpyAsync(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  return err_from_mask(h->hcnt->err);
+}
+
+int ngpu_download_walkers(ngpu_t* h, int64_t* rows, double* diag_h, double* offdiag_h, int64_t cap, int64_t* n) {
+  if (!h || !n) return NGPU_ERR_ARG;
+  HIP_OK(hipStreamSynchronize(h->stream));
+  HIP_OK(hipMemcpy(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost));
+  int64_t ns = (int64_t)h->hcnt->nslots;
+  const int W = h->W, S = h->S, RL = W + S + 1;
+  std::vector<uint64_t> key((size_t)ns * W);
+  std::vector<double> sg((size_t)ns * S), dh((size_t)ns), oh((size_t)ns);
+  std::vector<uint32_t> fl((size_t)ns);
+  if (ns > 0) {
+    HIP_OK(hipMemcpy(key.data(), h->wk.keys, key.size() * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(sg.data(), h->wk.sgn, sg.size() * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(fl.data(), h->wk.flags, fl.size() * 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(dh.data(), h->wk.diagH, dh.size() * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(oh.data(), h->wk.offdiagH, oh.size() * 8, hipMemcpyDeviceToHost));
+  }
+  const int* init_bits = h->sys.flag_layout == NGPU_FLAGS_NECI ? kNeciInit : kMneciInit;
+  const int* det_bits = h->sys.flag_layout == NGPU_FLAGS_NECI ? kNeciDeterm : kMneciDeterm;
+  int maxr = h->sys.flag_layout == NGPU_FLAGS_NECI ? 2 : 16;
+  int64_t m = 0;
+  for (int64_t i = 0; i < ns; ++i) {
+    if (fl[i] & kFlagRemoved) continue;
+    bool occ = false;
+    for (int r = 0; r < S; ++r) occ = occ || !(fabs(sg[i * S + r]) < 1e-12);
+    if (!occ && !(fl[i] & kFlagCore)) continue;
+    if (m < cap && rows) {
+      int64_t* row = rows + m * RL;
+      for (int w = 0; w < W; ++w) row[w] = (int64_t)key[i * W + w];
+      memcpy(row + W, &sg[i * S], sizeof(double) * S);
+      uint64_t nf = 0;
+      for (int r = 0; r < S && r < maxr; ++r) {
+        if ((fl[i] >> r) & 1u) nf |= 1ull << init_bits[r];
+        if (fl[i] & kFlagCore) nf |= 1ull << det_bits[r];
+      }
+      row[W + S] = (int64_t)nf;
+      if (diag_h) diag_h[m] = dh[i];
+      if (offdiag_h) offdiag_h[m] = oh[i];
+    }
+    ++m;
+  }
+  *n = m;
+  if (!rows) return NGPU_OK;  // size query
+  return m > cap ? NGPU_ERR_ARG : NGPU_OK;
+}
+
+int ngpu_iterate(ngpu_t* h, const ngpu_iter_in_t* in, ngpu_iter_out_t* out) {
+  int rc = ready(h);
+  if (rc) return rc;
+  if (!in || !(in->tau > 0.0)) return NGPU_ERR_ARG;
+  if (h->pending) {
+    rc = finish(h, nullptr);
+    if (rc) return rc;
+  }
+  if (h->W == 1) iterate_chain<1>(h, in);
+  else iterate_chain<2>(h, in);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(h->hout, h->dout, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));
+  h->pending = true;
+  h->prof_pending = h->prof;
+  if (out) return finish(h, out);
+  return NGPU_OK;
+}
+
+int ngpu_get_stats(ngpu_t* h, ngpu_iter_out_t* out) {
+  if (!h) return NGPU_ERR_ARG;
+  return finish(h, out);
+}
+
+int ngpu_set_profiling(ngpu_t* h, int32_t on) {
+  if (!h) return NGPU_ERR_ARG;
+  h->prof = on != 0;
+  for (int s = 0; s < kStages; ++s) h->stage_ms[s] = 0.0;
+  h->prof_iters = 0;
+  return NGPU_OK;
+}
+
+int ngpu_get_stage_times(ngpu_t* h, double* ms_total, int32_t nstages, int64_t* n_iters) {
+  if (!h || !ms_total) return NGPU_ERR_ARG;
+  for (int s = 0; s < nstages && s < kStages; ++s) ms_total[s] = h->stage_ms[s];
+  if (n_iters) *n_iters = h->prof_iters;
+  return NGPU_OK;
+}
+
+// ------------------------------------------------------------ parity entries
+int ngpu_find_walker_hash_batch(ngpu_t* h, const int64_t* dets, int64_t n, int32_t* buckets) {
+  if (!h || !h->have_hash || n < 0 || (n && (!dets || !buckets))) return NGPU_ERR_ARG;
+  if (!n) return NGPU_OK;
+  DevBuf<uint64_t> d;
+  DevBuf<int32_t> o;
+  HIP_OK(d.alloc((size_t)n * h->W));
+  HIP_OK(o.alloc((size_t)n));
+  HIP_OK(hipMemcpy(d.p, dets, (size_t)n * h->W * 8, hipMemcpyHostToDevice));
+  if (h->W == 1) hipLaunchKernelGGL(k_batch_hash<1>, dim3(grid_for(n)), dim3(kBlock), 0, 0, h->ds, d.p, n, o.p);
+  else hipLaunchKernelGGL(k_batch_hash<2>, dim3(grid_for(n)), dim3(kBlock), 0, 0, h->ds, d.p, n, o.p);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(buckets, o.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+  return NGPU_OK;
+}
+
+int ngpu_get_spawn_helement_batch(ngpu_t* h, const int64_t* a, const int64_t* b, int64_t n, double* hel) {
+  if (!h || !h->have_model || n < 0 || (n && (!a || !b || !hel))) return NGPU_ERR_ARG;
+  if (!n) return NGPU_OK;
+  DevBuf<uint64_t> da, db;
+  DevBuf<double> o;
+  HIP_OK(da.alloc((size_t)n * h->W));
+  HIP_OK(db.alloc((size_t)n * h->W));
+  HIP_OK(o.alloc((size_t)n));
+  HIP_OK(hipMemcpy(da.p, a, (size_t)n * h->W * 8, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(db.p, b, (size_t)n * h->W * 8, hipMemcpyHostToDevice));
+  if (h->W == 1) hipLaunchKernelGGL(k_batch_hel<1>, dim3(grid_for(n)), dim3(kBlock), 0, 0, h->ds, da.p, db.p, n, o.p);
+  else hipLaunchKernelGGL(k_batch_hel<2>, dim3(grid_for(n)), dim3(kBlock), 0, 0, h->ds, da.p, db.p, n, o.p);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(hel, o.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+  return NGPU_OK;
+}
+
+int ngpu_diag_helement_batch(ngpu_t* h, const int64_t* dets, int64_t n, double* diag, double* offdiag) {
+  if (!h || !h->have_model || !h->have_ref || n < 0 || (n && (!dets || !diag || !offdiag))) return NGPU_ERR_ARG;
+  if (!n) return NGPU_OK;
+  DevBuf<uint64_t> d;
+  DevBuf<double> o1, o2;
+  HIP_OK(d.alloc((size_t)n * h->W));
+  HIP_OK(o1.alloc((size_t)n));
+  HIP_OK(o2.alloc((size_t)n));
+  HIP_OK(hipMemcpy(d.p, dets, (size_t)n * h->W * 8, hipMemcpyHostToDevice));
+  if (h->W == 1) hipLaunchKernelGGL(k_batch_diag<1>, dim3(grid_for(n)), dim3(kBlock), 0, 0, h->ds, d.p, n, o1.p, o2.p);
+  else hipLaunchKernelGGL(k_batch_diag<2>, dim3(grid_for(n)), dim3(kBlock), 0, 0, h->ds, d.p, n, o1.p, o2.p);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(diag, o1.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(offdiag, o2.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+  return NGPU_OK;
+}
+
+int ngpu_generate_excitation_batch(ngpu_t* h, const int64_t* ilut, const double* draws, int64_t n, int64_t* ilut_j,
+                                   int32_t* ic, int32_t* ex, int32_t* parity, double* pgen) {
+  if (!h || !h->have_model || n < 0) return NGPU_ERR_ARG;
+  if (h->sys.model == NGPU_MODEL_MOLECULAR && !h->have_excitgen) return NGPU_ERR_STATE;
+  if (!n) return NGPU_OK;
+  if (!ilut || !draws || !ilut_j || !ic || !ex || !parity || !pgen) return NGPU_ERR_ARG;
+  for (int64_t i = 0; i < n * 6; ++i)
+    if (!(draws[i] >= 0.0 && draws[i] < 1.0)) return NGPU_ERR_ARG;
+  for (int64_t i = 0; i < n; ++i) {
+    int nel = 0;
+    for (int w = 0; w < h->W; ++w) nel += __builtin_popcountll((uint64_t)ilut[i * h->W + w]);
+    if (nel != h->sys.nel) return NGPU_ERR_ARG;
+  }
+  const int W = h->W;
+  DevBuf<uint64_t> d, dj;
+  DevBuf<double> dr, pg;
+  DevBuf<int32_t> dic, dex, dpar;
+  HIP_OK(d.alloc((size_t)n * W));
+  HIP_OK(dj.alloc((size_t)n * W));
+  HIP_OK(dr.alloc((size_t)n * 6));
+  HIP_OK(pg.alloc((size_t)n));
+  HIP_OK(dic.alloc((size_t)n));
+  HIP_OK(dex.alloc((size_t)n * 4));
+  HIP_OK(dpar.alloc((size_t)n));
+  HIP_OK(hipMemcpy(d.p, ilut, (size_t)n * W * 8, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dr.p, draws, (size_t)n * 6 * 8, hipMemcpyHostToDevice));
+  if (W == 1)
+    hipLaunchKernelGGL(k_batch_gen<1>, dim3(grid_for(n)), dim3(kBlock), 0, 0, h->ds, d.p, dr.p, n, dj.p, dic.p, dex.p,
+                       dpar.p, pg.p);
+  else
+    hipLaunchKernelGGL(k_batch_gen<2>, dim3(grid_for(n)), dim3(kBlock), 0, 0, h->ds, d.p, dr.p, n, dj.p, dic.p, dex.p,
+                       dpar.p, pg.p);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(ilut_j, dj.p, (size_t)n * W * 8, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(ic, dic.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(ex, dex.p, (size_t)n * 16, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(parity, dpar.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(pgen, pg.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+  return NGPU_OK;
+}
+
+int ngpu_calc_pgen_batch(ngpu_t* h, const int64_t* ilut, const int32_t* ic, const int32_t* ex, int64_t n,
+                         double* pgen) {
+  if (!h || !h->have_model || n < 0) return NGPU_ERR_ARG;
+  if (h->sys.model == NGPU_MODEL_MOLECULAR && !h->have_excitgen) return NGPU_ERR_STATE;
+  if (!n) return NGPU_OK;
+  if (!ilut || !ic || !ex || !pgen) return NGPU_ERR_ARG;
+  const int W = h->W;
+  for (int64_t i = 0; i < n; ++i) {
+    if (ic[i] != 1 && ic[i] != 2) return NGPU_ERR_ARG;
+    for (int q = 0; q < 2 * ic[i]; ++q)
+      if (ex[4 * i + q] < 1 || ex[4 * i + q] > h->sys.nbasis) return NGPU_ERR_ARG;
+  }
+  DevBuf<uint64_t> d;
+  DevBuf<int32_t> dic, dex;
+  DevBuf<double> o;
+  HIP_OK(d.alloc((size_t)n * W));
+  HIP_OK(dic.alloc((size_t)n));
+  HIP_OK(dex.alloc((size_t)n * 4));
+  HIP_OK(o.alloc((size_t)n));
+  HIP_OK(hipMemcpy(d.p, ilut, (size_t)n * W * 8, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dic.p, ic, (size_t)n * 4, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dex.p, ex, (size_t)n * 16, hipMemcpyHostToDevice));
+  if (W == 1)
+    hipLaunchKernelGGL(k_batch_pgen<1>, dim3(grid_for(n)), dim3(kBlock), 0, 0, h->ds, d.p, dic.p, dex.p, n, o.p);
+  else
+    hipLaunchKernelGGL(k_batch_pgen<2>, dim3(grid_for(n)), dim3(kBlock), 0, 0, h->ds, d.p, dic.p, dex.p, n, o.p);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(pgen, o.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+  return NGPU_OK;
+}
+
+int ngpu_encode_child(const int64_t* ilut_i, int32_t nwords, int32_t ic, const int32_t* ex, int64_t* ilut_j) {
+  if (!ilut_i || !ex || !ilut_j || nwords < 1 || nwords > 2 || (ic != 1 && ic != 2)) return NGPU_ERR_ARG;
+  for (int w = 0; w < nwords; ++w) ilut_j[w] = ilut_i[w];
+  for (int q = 0; q < 2 * ic; ++q) {
+    int orb = ex[q];
+    if (orb < 1 || orb > 64 * nwords) return NGPU_ERR_ARG;
+    uint64_t bit = 1ull << ((orb - 1) & 63);
+    if (q < ic) ilut_j[(orb - 1) >> 6] = (int64_t)((uint64_t)ilut_j[(orb - 1) >> 6] & ~bit);
+    else ilut_j[(orb - 1) >> 6] = (int64_t)((uint64_t)ilut_j[(orb - 1) >> 6] | bit);
+  }
+  return NGPU_OK;
+}
+
+int ngpu_attempt_die_batch(ngpu_t* h, const double* kii, const double* sign, const double* udraw, int64_t n,
+                           double tau, double diag_sft, double* ndie) {
+  if (!h || n < 0 || (n && (!kii || !sign || !udraw || !ndie))) return NGPU_ERR_ARG;
+  if (!n) return NGPU_OK;
+  DevBuf<double> a, b, c, o;
+  HIP_OK(a.alloc((size_t)n));
+  HIP_OK(b.alloc((size_t)n));
+  HIP_OK(c.alloc((size_t)n));
+  HIP_OK(o.alloc((size_t)n));
+  HIP_OK(hipMemcpy(a.p, kii, (size_t)n * 8, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(b.p, sign, (size_t)n * 8, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c.p, udraw, (size_t)n * 8, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_batch_die, dim3(grid_for(n)), dim3(kBlock), 0, 0, h->sys.t_all_real_coeff != 0, a.p, b.p, c.p, n,
+                     tau, diag_sft, o.p);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(ndie, o.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+  return NGPU_OK;
+}
+
+int ngpu_attempt_create_batch(ngpu_t* h, const double* hel, const double* pgen, const double* parent_sign,
+                              const double* udraw, int64_t n, double tau, double* child) {
+  if (!h || n < 0 || (n && (!hel || !pgen || !parent_sign || !udraw || !child))) return NGPU_ERR_ARG;
+  if (!n) return NGPU_OK;
+  DevBuf<double> a, b, c, d, o;
+  HIP_OK(a.alloc((size_t)n));
+  HIP_OK(b.alloc((size_t)n));
+  HIP_OK(c.alloc((size_t)n));
+  HIP_OK(d.alloc((size_t)n));
+  HIP_OK(o.alloc((size_t)n));
+  HIP_OK(hipMemcpy(a.p, hel, (size_t)n * 8, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(b.p, pgen, (size_t)n * 8, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(c.p, parent_sign, (size_t)n * 8, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d.p, udraw, (size_t)n * 8, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_batch_create, dim3(grid_for(n)), dim3(kBlock), 0, 0, h->ds, a.p, b.p, c.p, d.p, n, tau, o.p);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(child, o.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+  return NGPU_OK;
+}
+
+int ngpu_annihilate(ngpu_t* h, uint32_t iter, const int64_t* sdet, const double* sval, const uint32_t* sinfo,
+                    int64_t n, ngpu_iter_out_t* out, int64_t* merged_det, double* merged_sgn, uint32_t* merged_flags,
+                    int64_t* n_merged) {
+  int rc = ready(h);
+  if (rc) return rc;
+  if (n < 0 || (n && (!sdet || !sval || !sinfo))) return NGPU_ERR_ARG;
+  if (n > h->scap) return NGPU_ERR_SPAWN_FULL;
+  for (int64_t i = 0; i < n; ++i)
+    if ((int)(sinfo[i] & 0xFFu) >= h->S) return NGPU_ERR_ARG;
+  if (h->pending) {
+    rc = finish(h, nullptr);
+    if (rc) return rc;
+  }
+  const int W = h->W, S = h->S;
+  if (n) {
+    HIP_OK(hipMemcpyAsync(h->sp_key, sdet, (size_t)n * W * 8, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(h->sp_val, sval, (size_t)n * 8, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(h->sp_info, sinfo, (size_t)n * 4, hipMemcpyHostToDevice, h->stream));
+  }
+  hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->cnt);
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_spawned, (unsigned long long)n);
+  bool p = h->prof;
+  h->prof = false;
+  if (W == 1) annihilation_chain<1>(h, iter);
+  else annihilation_chain<2>(h, iter);
+  h->prof = p;
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(h->hout, h->dout, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipMemcpyAsync(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
+  h->pending = true;
+  h->prof_pending = false;
+  rc = finish(h, out);
+  int64_t nu = (int64_t)h->hcnt->n_unique;
+  if (n_merged) *n_merged = nu;
+  if (nu > 0 && merged_det && merged_sgn && merged_flags) {
+    HIP_OK(hipMemcpy(merged_det, h->u_key, (size_t)nu * W * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(merged_sgn, h->u_sgn, (size_t)nu * S * 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(merged_flags, h->u_flag, (size_t)nu * 4, hipMemcpyDeviceToHost));
+  }
+  return rc;
+}
+
+}  // extern "C"
