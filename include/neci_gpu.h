@@ -149,6 +149,11 @@ int ngpu_download_walkers(ngpu_t *h, int64_t *rows, double *diag_h, double *offd
 int ngpu_iterate(ngpu_t *h, const ngpu_iter_in_t *in, ngpu_iter_out_t *out);
 /* Wait for the last iteration and return its statistics. */
 int ngpu_get_stats(ngpu_t *h, ngpu_iter_out_t *out);
+/* Update cycle (calculate_new_shift_wrapper, fcimc_iter_utilities.F90:1403):
+   wait, return the statistics summed over the iterations since the previous
+   call (maxima for max_cyc_spawn/bloom_size, last value for tot_walkers and
+   inst_no_at_hf) plus the last iteration's, and start a new cycle. */
+int ngpu_get_cycle_stats(ngpu_t *h, ngpu_iter_out_t *sum, ngpu_iter_out_t *last, int32_t *n_iters);
 
 /* Stage timing with events on the engine stream (set_timer/halt_timer analogue,
    lib/timing.F90:122-188).  Stages: 0 prepass(initiator, proj-E, death),

@@ -100,6 +100,7 @@ _SIGS = [
     ("ngpu_download_walkers", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int64, P(ctypes.c_int64)]),
     ("ngpu_iterate", ctypes.c_int, [_vp, P(IterInT), P(IterOutT)]),
     ("ngpu_get_stats", ctypes.c_int, [_vp, P(IterOutT)]),
+    ("ngpu_get_cycle_stats", ctypes.c_int, [_vp, P(IterOutT), P(IterOutT), P(ctypes.c_int32)]),
     ("ngpu_set_profiling", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("ngpu_get_stage_times", ctypes.c_int, [_vp, _vp, ctypes.c_int32, P(ctypes.c_int64)]),
     ("ngpu_find_walker_hash_batch", ctypes.c_int, [_vp, _vp, ctypes.c_int64, _vp]),

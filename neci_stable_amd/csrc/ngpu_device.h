@@ -537,6 +537,9 @@ __device__ __host__ inline void pick_biased_elecs(const DevSys& s, const uint64_
   int aa = nalpha * (nalpha - 1) / 2, bb = nbeta * (nbeta - 1) / 2;
   int par = aa + bb, ab = nalpha * nbeta;
   double pb = s.pparallel;
+  o1 = o2 = 0;
+  pgen = 0.0;
+  if (r < pb ? par == 0 : ab == 0) return;  // no pair of the drawn spin type
   if (r < pb) {
     r = (r / pb) * (double)par;
     int idx = (int)floor(r);
@@ -582,6 +585,7 @@ __device__ __host__ inline Excit gen_double_pchb(const DevSys& s, const uint64_t
     if (s1 > s2) { int t = s1; s1 = s2; s2 = t; }
   } else {
     pick_biased_elecs<W>(s, d, dr[1], s1, s2, p);
+    if (s1 == 0 || s2 == 0) return ex;
   }
   int ij = fuse_index(s1, s2);
   if (!s.ab_ok[ij - 1]) return ex;

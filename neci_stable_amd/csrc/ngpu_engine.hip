@@ -14,6 +14,7 @@ using namespace ngpu;
 namespace {
 
 constexpr int kStages = 8;
+constexpr int kEvRing = 32;  // iterations whose stage events may be in flight
 
 #define HIP_OK(x)                                                        \
   do {                                                                   \
@@ -86,11 +87,14 @@ struct ngpu_engine {
   bool pending = false;
   // profiling
   bool prof = false;
-  hipEvent_t ev[kStages + 1];
+  hipEvent_t ev[kEvRing][kStages + 1];
+  bool ev_pending[kEvRing] = {false};
+  int ev_cur = 0;
+  int64_t ev_next = 0;
   bool ev_ok = false;
   double stage_ms[kStages] = {0};
   int64_t prof_iters = 0;
-  bool prof_pending = false;
+  ngpu_iter_out_t* dcyc = nullptr;  // statistics summed over the update cycle
 };
 
 namespace {
@@ -124,20 +128,54 @@ __global__ void k_begin_iter(ngpu_iter_out_t* out, Counters* cnt) {
   }
 }
 
-__global__ void k_end_iter(ngpu_iter_out_t* out, Counters* cnt, int64_t scap) {
-  out->tot_walkers = (int64_t)cnt->n_occ;
-  out->n_attempts = (int64_t)cnt->n_attempts;
-  int64_t ns = (int64_t)cnt->n_spawned;
-  out->n_spawned = ns < scap ? ns : scap;
-  out->n_unique = (int64_t)cnt->n_unique;
-  out->n_new = (int64_t)cnt->n_new;
-  out->err = (int32_t)cnt->err;
+// final counters of the iteration, then accumulation into the cycle sums
+// (ENumCyc, HFCyc, NoBorn, ... are cycle accumulators in the reference too)
+__global__ void k_end_iter(ngpu_iter_out_t* out, ngpu_iter_out_t* cyc, Counters* cnt, int64_t scap) {
+  __shared__ int done;
+  if (threadIdx.x == 0) {
+    out->tot_walkers = (int64_t)cnt->n_occ;
+    out->n_attempts = (int64_t)cnt->n_attempts;
+    int64_t ns = (int64_t)cnt->n_spawned;
+    out->n_spawned = ns < scap ? ns : scap;
+    out->n_unique = (int64_t)cnt->n_unique;
+    out->n_new = (int64_t)cnt->n_new;
+    out->err = (int32_t)cnt->err;
+    done = 1;
+  }
+  __syncthreads();
+  const double* g = (const double*)out;
+  double* c = (double*)cyc;
+  for (int i = threadIdx.x; i < S_NDOUBLE; i += blockDim.x) {
+    bool is_max = (i == S_MAXSPAWN) || (i >= S_BLOOMS && i < S_BLOOMS + 3);
+    if (is_max) c[i] = c[i] > g[i] ? c[i] : g[i];
+    else if (i >= soff(F_INSTHF, 0) && i < soff(F_INSTHF, kMaxRep)) c[i] = g[i];
+    else c[i] += g[i];
+  }
+  if (threadIdx.x == 0 && done) {
+    cyc->tot_walkers = out->tot_walkers;
+    cyc->n_attempts += out->n_attempts;
+    cyc->n_spawned += out->n_spawned;
+    cyc->n_unique += out->n_unique;
+    cyc->n_new += out->n_new;
+    cyc->err |= out->err;
+    cyc->pad += 1;  // iterations in the cycle
+  }
+}
+
+__global__ void k_reset_cycle(ngpu_iter_out_t* cyc) {
+  double* c = (double*)cyc;
+  for (int i = threadIdx.x; i < S_NDOUBLE; i += blockDim.x) c[i] = 0.0;
+  if (threadIdx.x == 0) {
+    cyc->tot_walkers = cyc->n_attempts = cyc->n_spawned = cyc->n_unique = cyc->n_new = 0;
+    cyc->err = 0;
+    cyc->pad = 0;
+  }
 }
 
 __global__ void k_set_count(unsigned long long* p, unsigned long long v) { *p = v; }
 
 void rec(ngpu_t* h, int stage) {
-  if (h->prof && h->ev_ok) (void)hipEventRecord(h->ev[stage], h->stream);
+  if (h->prof && h->ev_ok) (void)hipEventRecord(h->ev[h->ev_cur][stage], h->stream);
 }
 
 // radix sort of the spawned keys; returns the buffer index holding the result
@@ -185,7 +223,7 @@ void annihilation_chain(ngpu_t* h, uint32_t iter) {
   rec(h, 7);
   hipLaunchKernelGGL(k_stats<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->dout,
                      iter);
-  hipLaunchKernelGGL(k_end_iter, dim3(1), dim3(1), 0, h->stream, h->dout, h->cnt, h->scap);
+  hipLaunchKernelGGL(k_end_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dcyc, h->cnt, h->scap);
   rec(h, 8);
 }
 
@@ -207,17 +245,22 @@ void iterate_chain(ngpu_t* h, const ngpu_iter_in_t* in) {
   annihilation_chain<W>(h, in->iter);
 }
 
+void harvest(ngpu_t* h, int slot) {
+  if (!h->ev_pending[slot]) return;
+  (void)hipEventSynchronize(h->ev[slot][kStages]);
+  for (int s = 0; s < kStages; ++s) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->ev[slot][s], h->ev[slot][s + 1]) == hipSuccess) h->stage_ms[s] += ms;
+  }
+  h->prof_iters++;
+  h->ev_pending[slot] = false;
+}
+
 int finish(ngpu_t* h, ngpu_iter_out_t* out) {
   HIP_OK(hipStreamSynchronize(h->stream));
   h->pending = false;
-  if (h->prof && h->ev_ok && h->prof_pending) {
-    for (int s = 0; s < kStages; ++s) {
-      float ms = 0.f;
-      if (hipEventElapsedTime(&ms, h->ev[s], h->ev[s + 1]) == hipSuccess) h->stage_ms[s] += ms;
-    }
-    h->prof_iters++;
-    h->prof_pending = false;
-  }
+  if (h->ev_ok)
+    for (int q = 0; q < kEvRing; ++q) harvest(h, q);
   if (out) *out = *h->hout;
   return err_from_mask((unsigned)h->hout->err);
 }
@@ -481,12 +524,14 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
   fail(dalloc(&h->n_flag, (size_t)ncap));
   fail(dalloc(&h->cnt, 1));
   fail(dalloc(&h->dout, 1));
+  fail(dalloc(&h->dcyc, 1));
   fail(hipHostMalloc((void**)&h->hout, sizeof(ngpu_iter_out_t), hipHostMallocDefault));
   fail(hipHostMalloc((void**)&h->hcnt, sizeof(Counters), hipHostMallocDefault));
   if (rc == NGPU_OK) {
     fail(hipMemsetAsync(wk.htab, 0xFF, (size_t)wk.L * 8 * W * sizeof(uint64_t), h->stream));
     fail(hipMemsetAsync(h->cnt, 0, sizeof(Counters), h->stream));
     fail(hipMemsetAsync(h->dout, 0, sizeof(ngpu_iter_out_t), h->stream));
+    fail(hipMemsetAsync(h->dcyc, 0, sizeof(ngpu_iter_out_t), h->stream));
     fail(hipMemsetAsync(wk.flags, 0, (size_t)cap * sizeof(uint32_t), h->stream));
     fail(hipStreamSynchronize(h->stream));
     memset(h->hout, 0, sizeof(ngpu_iter_out_t));
@@ -507,8 +552,9 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
   h->ds.pdoubles = 1.0;
   if (rc == NGPU_OK) {
     h->ev_ok = true;
-    for (int s = 0; s <= kStages; ++s)
-      if (hipEventCreate(&h->ev[s]) != hipSuccess) h->ev_ok = false;
+    for (int q = 0; q < kEvRing; ++q)
+      for (int s = 0; s <= kStages; ++s)
+        if (hipEventCreate(&h->ev[q][s]) != hipSuccess) h->ev_ok = false;
   }
   if (rc != NGPU_OK) {
     ngpu_destroy(h);
@@ -524,14 +570,15 @@ int ngpu_destroy(ngpu_t* h) {
   void* ptrs[] = {h->wk.keys, h->wk.sgn, h->wk.flags, h->wk.diagH, h->wk.offdiagH, h->wk.hpos, h->wk.freeq,
                   h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->sp_key, h->sp_val, h->sp_info,
                   h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->u_key, h->u_sgn, h->u_flag, h->n_key,
-                  h->n_sgn, h->n_flag, h->cnt, h->dout, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_ab_tab,
+                  h->n_sgn, h->n_flag, h->cnt, h->dout, h->dcyc, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_ab_tab,
                   h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->hout) (void)hipHostFree(h->hout);
   if (h->hcnt) (void)hipHostFree(h->hcnt);
   if (h->ev_ok)
-    for (int s = 0; s <= kStages; ++s) (void)hipEventDestroy(h->ev[s]);
+    for (int q = 0; q < kEvRing; ++q)
+      for (int s = 0; s <= kStages; ++s) (void)hipEventDestroy(h->ev[q][s]);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return NGPU_OK;
@@ -779,23 +826,45 @@ int ngpu_iterate(ngpu_t* h, const ngpu_iter_in_t* in, ngpu_iter_out_t* out) {
     rc = finish(h, nullptr);
     if (rc) return rc;
   }
+  if (h->prof && h->ev_ok) {
+    h->ev_cur = (int)(h->ev_next % kEvRing);
+    harvest(h, h->ev_cur);
+  }
   if (h->W == 1) iterate_chain<1>(h, in);
   else iterate_chain<2>(h, in);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpyAsync(h->hout, h->dout, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));
+  if (h->prof && h->ev_ok) {
+    h->ev_pending[h->ev_cur] = true;
+    h->ev_next++;
+  }
+  if (out) HIP_OK(hipMemcpyAsync(h->hout, h->dout, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));
   h->pending = true;
-  h->prof_pending = h->prof;
   if (out) return finish(h, out);
   return NGPU_OK;
 }
 
 int ngpu_get_stats(ngpu_t* h, ngpu_iter_out_t* out) {
   if (!h) return NGPU_ERR_ARG;
+  HIP_OK(hipMemcpyAsync(h->hout, h->dout, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));
   return finish(h, out);
+}
+
+int ngpu_get_cycle_stats(ngpu_t* h, ngpu_iter_out_t* sum, ngpu_iter_out_t* last, int32_t* n_iters) {
+  if (!h) return NGPU_ERR_ARG;
+  HIP_OK(hipMemcpyAsync(h->hout, h->dout, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));
+  ngpu_iter_out_t tmp;
+  HIP_OK(hipMemcpyAsync(&tmp, h->dcyc, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));
+  hipLaunchKernelGGL(k_reset_cycle, dim3(1), dim3(kBlock), 0, h->stream, h->dcyc);
+  int rc = finish(h, last);
+  if (sum) *sum = tmp;
+  if (n_iters) *n_iters = tmp.pad;
+  if (rc == NGPU_OK) rc = err_from_mask((unsigned)tmp.err);
+  return rc;
 }
 
 int ngpu_set_profiling(ngpu_t* h, int32_t on) {
   if (!h) return NGPU_ERR_ARG;
+  if (h->pending) (void)finish(h, nullptr);
   h->prof = on != 0;
   for (int s = 0; s < kStages; ++s) h->stage_ms[s] = 0.0;
   h->prof_iters = 0;
@@ -1013,7 +1082,6 @@ int ngpu_annihilate(ngpu_t* h, uint32_t iter, const int64_t* sdet, const double*
   HIP_OK(hipMemcpyAsync(h->hout, h->dout, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipMemcpyAsync(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
   h->pending = true;
-  h->prof_pending = false;
   rc = finish(h, out);
   int64_t nu = (int64_t)h->hcnt->n_unique;
   if (n_merged) *n_merged = nu;

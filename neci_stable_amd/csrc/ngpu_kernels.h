@@ -15,6 +15,7 @@
 #pragma once
 #include "ngpu_device.h"
 #include "../../include/neci_gpu.h"
+#include <cstddef>
 
 namespace ngpu {
 
@@ -34,6 +35,8 @@ constexpr int S_BLOOMC = S_NVALID + 3;
 constexpr int S_BLOOMS = S_NVALID + 6;
 constexpr int S_NDOUBLE = S_NVALID + 9;
 __host__ __device__ constexpr int soff(int f, int r) { return f * kMaxRep + r; }
+static_assert(sizeof(double) * S_NDOUBLE == offsetof(ngpu_iter_out_t, tot_walkers),
+              "statistics block of ngpu_iter_out_t must be S_NDOUBLE doubles");
 
 struct Counters {
   unsigned long long nslots;

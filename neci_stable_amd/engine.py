@@ -132,6 +132,14 @@ class Engine:
         check(self.lib.ngpu_get_stats(self.h, ctypes.byref(self._out)), "ngpu_get_stats")
         return self._out.to_dict(self.S)
 
+    def cycle_stats(self):
+        """(sums over the cycle, last iteration, number of iterations); starts a new cycle."""
+        sm = IterOutT()
+        n = ctypes.c_int32()
+        check(self.lib.ngpu_get_cycle_stats(self.h, ctypes.byref(sm), ctypes.byref(self._out), ctypes.byref(n)),
+              "ngpu_get_cycle_stats")
+        return sm.to_dict(self.S), self._out.to_dict(self.S), n.value
+
     def set_profiling(self, on=True):
         check(self.lib.ngpu_set_profiling(self.h, int(on)), "ngpu_set_profiling")
 

@@ -17,36 +17,51 @@ import time
 
 import numpy as np
 
-SUM_FIELDS = ["nborn", "ndied", "nannihil", "naborted", "nremoved", "enum_cyc", "hf_cyc",
-              "noat_doubs", "spawn_from_sing", "no_init_dets", "no_init_walk", "no_noninit_dets",
-              "no_noninit_walk"]
+class OracleCycleAdapter:
+    """Gives a synchronous backend (iterate(it, shift) -> stats) the cycle API
+    of the engine: asynchronous iterate + cycle_stats() sums."""
 
+    def __init__(self, backend, nrep):
+        self.b = backend
+        self.S = nrep
+        self.hii = backend.hii
+        self._reset()
 
-class CycleAccumulator:
-    def __init__(self, nrep):
-        self.nrep = nrep
-        self.reset()
-
-    def reset(self):
+    def _reset(self):
+        self.sum = None
         self.n = 0
-        self.sums = {f: np.zeros(self.nrep) for f in SUM_FIELDS}
-        self.sum_tot_parts = np.zeros(self.nrep)
-        self.attempts = 0
         self.last = None
 
-    def add(self, st):
-        self.n += 1
-        for f in SUM_FIELDS:
-            self.sums[f] += st[f]
-        self.sum_tot_parts += st["tot_parts"]
-        self.attempts += st["n_attempts"]
+    def iterate(self, it, tau, shift, wait=False):
+        st = self.b.iterate(it, shift)
+        if st.get("rc", 0) or st.get("err", 0):
+            raise RuntimeError(f"iteration {it} failed with status {st.get('rc')}/{st.get('err')}")
         self.last = st
+        self.n += 1
+        if self.sum is None:
+            self.sum = {k: (np.array(v, dtype=float).copy() if isinstance(v, np.ndarray) else v)
+                        for k, v in st.items()}
+        else:
+            for k, v in st.items():
+                if k in ("max_cyc_spawn",):
+                    self.sum[k] = max(self.sum[k], v)
+                elif k in ("bloom_size",):
+                    self.sum[k] = np.maximum(self.sum[k], v)
+                elif k in ("tot_walkers", "inst_no_at_hf", "rc", "err"):
+                    self.sum[k] = v
+                else:
+                    self.sum[k] = self.sum[k] + v
+        return st if wait else None
+
+    def cycle_stats(self):
+        out = (self.sum, self.last, self.n)
+        self._reset()
+        return out
 
 
 class FciMCPar:
-    """Runs NMCyc iterations of a backend exposing iterate(it, tau, shift) and
-    cycle statistics.  backend: neci_stable_amd.Engine (GPU) or any object with
-    the same iterate/stats API."""
+    """Runs iterations of a backend with the engine's cycle API: iterate(it,
+    tau, shift, wait=False) and cycle_stats() -> (sums, last, n_iters)."""
 
     def __init__(self, backend, *, tau, target_walkers, steps_shift=10, shift_damp=0.1,
                  diag_shift=0.0, jump_shift=True, hii=None, log=None):
@@ -62,61 +77,66 @@ class FciMCPar:
         self.jump_shift = jump_shift
         self.hii = backend.hii if hii is None else hii
         self.old_av_walkers = None
-        self.acc = CycleAccumulator(self.S)
         self.iter = 0
-        self.rows = []       # FCIMCStats-like rows per update cycle
-        self.shift_hist = []  # (iter, shift, proj_e) for blocking analysis
+        self.rows = []        # FCIMCStats-like rows per update cycle
+        self.shift_hist = []  # (iter, shift, proj_e) once every replica varies its shift
         self.sum_enum = np.zeros(self.S)
         self.sum_noathf = np.zeros(self.S)
         self.walker_iters = 0.0
+        self.last = None
         self.log = log
 
-    def _update_shift(self):
-        a = self.acc
-        av = a.sum_tot_parts / float(a.n)
+    def _update_shift(self, sm, last, n):
+        av = sm["tot_parts"] / float(n)
         growth = None
         if self.old_av_walkers is not None and np.all(self.old_av_walkers != 0):
             growth = av / self.old_av_walkers
-        hf = a.sums["hf_cyc"]
-        en = a.sums["enum_cyc"]
-        proje = np.where(np.abs(hf) > 1e-13, en / np.where(hf == 0, 1, hf), np.nan)
+        hf = sm["hf_cyc"]
+        en = sm["enum_cyc"]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            proje = np.where(np.abs(hf) > 1e-13, en / np.where(hf == 0, 1, hf), np.nan)
         defer = np.zeros(self.S, dtype=bool)
-        tot_parts = a.last["tot_parts"]
+        tot_parts = last["tot_parts"]
         for r in range(self.S):
-            if self.single_part[r]:
-                if tot_parts[r] > self.target:
-                    self.single_part[r] = False
-                    self.vary_shift_iter[r] = self.iter
-                    if self.jump_shift and np.isfinite(proje[r]):
-                        self.shift[r] = proje[r]
-                        defer[r] = True
-            if not self.single_part[r] and not defer[r] and growth is not None:
-                if growth[r] > 0:
-                    self.shift[r] -= math.log(growth[r]) * self.shift_damp / (self.tau * self.steps_shift)
+            if self.single_part[r] and tot_parts[r] > self.target:
+                self.single_part[r] = False
+                self.vary_shift_iter[r] = self.iter
+                if self.jump_shift and np.isfinite(proje[r]):
+                    self.shift[r] = proje[r]
+                    defer[r] = True
+            if not self.single_part[r] and not defer[r] and growth is not None and growth[r] > 0:
+                self.shift[r] -= math.log(growth[r]) * self.shift_damp / (self.tau * self.steps_shift)
         self.old_av_walkers = av
         if not np.all(self.single_part):
             self.sum_enum += en
             self.sum_noathf += hf
         row = dict(iter=self.iter, shift=self.shift.copy(), tot_parts=tot_parts.copy(),
-                   tot_walkers=a.last["tot_walkers"], proj_e=proje, hf=hf / a.n,
-                   nborn=a.sums["nborn"], ndied=a.sums["ndied"], nannihil=a.sums["nannihil"],
-                   naborted=a.sums["naborted"], single_part=self.single_part.copy())
+                   tot_walkers=last["tot_walkers"], proj_e=proje, hf=hf / n,
+                   nborn=sm["nborn"], ndied=sm["ndied"], nannihil=sm["nannihil"],
+                   naborted=sm["naborted"], attempts=sm["n_attempts"], spawned=sm["n_spawned"],
+                   unique=sm["n_unique"], new=sm["n_new"],
+                   single_part=self.single_part.copy())
         self.rows.append(row)
         if not np.any(self.single_part):
             self.shift_hist.append((self.iter, self.shift.copy(), proje.copy()))
         if self.log:
             self.log(row)
-        a.reset()
 
     def run(self, n_iter):
+        """n_iter iterations, update cycle every StepsSft; returns wall seconds."""
         t0 = time.perf_counter()
-        for _ in range(n_iter):
-            self.iter += 1
-            st = self.b.iterate(self.iter, self.tau, self.shift)
-            self.acc.add(st)
-            self.walker_iters += float(np.sum(st["tot_parts"]))
+        done = 0
+        while done < n_iter:
+            k = min(self.steps_shift - self.iter % self.steps_shift, n_iter - done)
+            for _ in range(k):
+                self.iter += 1
+                self.b.iterate(self.iter, self.tau, self.shift, wait=False)
+            done += k
+            sm, last, n = self.b.cycle_stats()
+            self.walker_iters += float(np.sum(sm["tot_parts"]))
+            self.last = last
             if self.iter % self.steps_shift == 0:
-                self._update_shift()
+                self._update_shift(sm, last, n)
         return time.perf_counter() - t0
 
     def projected_energy(self):

@@ -603,8 +603,8 @@ static int gen_single_heatbath(const osys_t *s, const uint64_t *ilut, const int3
 }
 
 /* pick_biased_elecs (excit_gens_int_weighted.F90:722-848) */
-static void pick_biased_elecs(const osys_t *s, const int32_t *nI, double r, int *elecs,
-                              double *pgen)
+static int pick_biased_elecs(const osys_t *s, const int32_t *nI, double r, int *elecs,
+                             double *pgen)
 {
     int nel = s->nel;
     int nalpha = 0, nbeta = 0;
@@ -613,6 +613,7 @@ static void pick_biased_elecs(const osys_t *s, const int32_t *nI, double r, int 
     int par = aa + bb, ab = nalpha * nbeta;
     double pb = s->pparallel;
     int al_req, be_req, al_num[2] = {0, 0}, be_num[2] = {0, 0};
+    if (r < pb ? par == 0 : ab == 0) return 0; /* no pair of the drawn spin type */
     if (r < pb) {
         r = (r / pb) * (double)par;
         int idx = (int)floor(r);
@@ -646,6 +647,7 @@ static void pick_biased_elecs(const osys_t *s, const int32_t *nI, double r, int 
         }
         if (al_req == 0 && be_req == 0) break;
     }
+    return 1;
 }
 
 /* GAS_doubles_PCHB_spinorb_gen_exc (gasci_pchb_doubles_spinorb_fastweighted.fpp:131-244)
@@ -675,7 +677,7 @@ static int gen_double_pchb(const osys_t *s, const uint64_t *ilut, const int32_t 
             t = elecs[0]; elecs[0] = elecs[1]; elecs[1] = t;
         }
     } else {
-        pick_biased_elecs(s, nI, d[1], elecs, &p);
+        if (!pick_biased_elecs(s, nI, d[1], elecs, &p)) return 0;
         srcs[0] = nI[elecs[0] - 1];
         srcs[1] = nI[elecs[1] - 1];
     }
@@ -918,6 +920,16 @@ int32_t o_add_walker(ostate_t *st, const uint64_t *det, const double *sgn, uint3
     if (hash_lookup(st, det, b) >= 0) return 4;
     return add_new_hash_det(st, det, sgn, flags, o_diag_helement(&st->s, nI),
                             o_offdiag_helement(&st->s, det), b);
+}
+
+int32_t o_add_walkers(ostate_t *st, const uint64_t *dets, const double *sgns, const uint32_t *flags,
+                      int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        int rc = o_add_walker(st, dets + i * st->W, sgns + i * st->S, flags[i]);
+        if (rc) return rc;
+    }
+    return 0;
 }
 
 int64_t o_get_walkers(ostate_t *st, uint64_t *dets, double *sgns, uint32_t *flags, double *diagh,
@@ -1355,6 +1367,7 @@ int32_t o_annihilate_list(ostate_t *st, uint32_t iter, const uint64_t *sdet, con
         ((double *)(sp + W))[r] = sval[i];
         if (sinfo[i] & SINFO_INIT) sp[W + S] |= 1ull << r;
     }
+    out->n_spawned = n;
     int64_t nu = compress_spawned(st, n, out);
     out->n_unique = nu;
     for (int64_t i = 0; i < nu; ++i) {

@@ -118,6 +118,8 @@ double o_calc_pgen(const osys_t *s, const uint64_t *ilut, int32_t ic, const int3
 ostate_t *o_create(const osys_t *sys);
 void o_destroy(ostate_t *st);
 int32_t o_add_walker(ostate_t *st, const uint64_t *det, const double *sgn, uint32_t flags);
+int32_t o_add_walkers(ostate_t *st, const uint64_t *dets, const double *sgns, const uint32_t *flags,
+                      int64_t n);
 int64_t o_get_walkers(ostate_t *st, uint64_t *dets, double *sgns, uint32_t *flags,
                       double *diagh, double *offdiagh, int64_t cap);
 int32_t o_iterate(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_t *out);

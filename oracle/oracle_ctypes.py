@@ -87,6 +87,7 @@ def lib():
             "o_destroy": (None, [_vp]),
             "o_add_walker": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint32]),
             "o_get_walkers": (ctypes.c_int64, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int64]),
+            "o_add_walkers": (ctypes.c_int32, [_vp, _vp, _vp, _vp, ctypes.c_int64]),
             "o_iterate": (ctypes.c_int32, [_vp, ctypes.c_uint32, _vp, _vp]),
             "o_annihilate_list": (ctypes.c_int32, [_vp, ctypes.c_uint32, _vp, _vp, _vp, ctypes.c_int64, _vp,
                                                    _vp, _vp, _vp, _vp]),
@@ -184,6 +185,14 @@ class Oracle:
             rc = self.L.o_add_walker(self.st, p(d), p(sg), int(flags[i]))
             if rc:
                 raise RuntimeError(f"o_add_walker {rc}")
+
+    def add_walkers_bulk(self, dets, signs, flags):
+        dets = np.ascontiguousarray(dets, dtype=np.int64).reshape(-1, self.W).view(np.uint64)
+        signs = np.ascontiguousarray(signs, dtype=np.float64).reshape(-1, self.S)
+        flags = np.ascontiguousarray(flags, dtype=np.uint32)
+        rc = self.L.o_add_walkers(self.st, p(dets), p(signs), p(flags), dets.shape[0])
+        if rc:
+            raise RuntimeError(f"o_add_walkers {rc}")
 
     def walkers(self, cap=10_000_000):
         n = self.L.o_get_walkers(self.st, None, None, None, None, None, 0)

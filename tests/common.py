@@ -21,9 +21,9 @@ def hubbard_spec(nsites=10, nel=10, u=4.0, **kw):
     return Spec(HubbardChain(nsites, nel, u, bhub=-1.0), **kw)
 
 
-def mol_spec(nspat=8, nel=6, rank=8, seed=7, **kw):
+def mol_spec(nspat=8, nel=6, rank=8, mseed=7, **kw):
     kw.setdefault("init_walkers", 2000)
-    return Spec(molecule(nspat, nel, rank, seed), **kw)
+    return Spec(molecule(nspat, nel, rank, mseed), **kw)
 
 
 def internal_to_neci_flags(flags, S):
