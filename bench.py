@@ -3,9 +3,9 @@
 
 Workload (config 1 of the measurement plan): synthetic C1 molecular
 Hamiltonian, 10 electrons in 20 spatial orbitals (seed 7, Cholesky rank 60),
-initiator FCIQMC (threshold 3), PCHB doubles (UNIF-FAST particles, FAST-FAST
-spin-orbital-resolved holes) + on-the-fly heat-bath singles, 1e6 target
-walkers, tau = 0.005, FP64 signs.
+initiator FCIQMC (threshold 3), PCHB doubles (UNIF-UNIF particles, FAST-FAST
+spin-orbital-resolved holes) + on-the-fly heat-bath singles with pSingles 0.1,
+1e6 target walkers, tau = 0.005, SHIFTDAMP 0.5, MEMORYFACHASH 2, FP64 signs.
 
 A step is one FCIQMC iteration (spawn, merge, death, annihilation, hash
 statistics) over the resident walker list; every 10 steps the host runs the
@@ -27,7 +27,7 @@ STAGES = ["prepass", "scan", "spawn", "sort", "merge", "annihilate", "insert", "
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(stage, c, W=1, S=1, nel=10, nsp=20, psingles=0.019, nbasis=40):
+def algorithmic_bytes(stage, c, W=1, S=1, nel=10, nsp=20, psingles=0.1, nbasis=40):
     """Minimum compulsory bytes of one launch of a stage for per-iteration counts c.
 
     N_d occupied dets, N_a attempts, N_s spawns, N_u unique spawned dets,
@@ -41,7 +41,7 @@ def algorithmic_bytes(stage, c, W=1, S=1, nel=10, nsp=20, psingles=0.019, nbasis
     if stage == "spawn":
         nvirt = nsp - nel // 2
         single = nvirt * (8 + 2 * nel * 8) + (8 + 2 * nel * 8)   # heat-bath list + H_ij gathers
-        double = 64 + 16                                         # alias entries/probs + 2 UMAT gathers
+        double = 16 + 8 + 8 + 16                                 # AB alias entry, prob, tgtOrbs, 2 UMAT gathers
         per_attempt = K + 8 + 1 + psingles * single + (1 - psingles) * double
         return Na * per_attempt + Ns * (K + 8 + 4)
     if stage == "sort":      # one read+write pass of (key, index)
@@ -61,7 +61,8 @@ def build_workload(args):
     from neci_stable_amd import Spec, synthetic_molecule
     mol = synthetic_molecule(20, 10, 60, 7)
     spec = Spec(mol, init_walkers=args.walkers, seed=1234, trunc_initiator=True, init_walk_no=3.0,
-                particles=1, memory_fac_spawn=8.0, memory_fac_part=10.0, memory_fac_hash=0.7)
+                particles=0, psingles=0.1, pparallel=0.5, memory_fac_spawn=8.0,
+                memory_fac_part=10.0, memory_fac_hash=2.0)
     return spec
 
 
@@ -120,7 +121,7 @@ def main():
     tau = 0.005
     eng = Engine(spec)
     eng.start_single_part(100.0)
-    drv = FciMCPar(eng, tau=tau, target_walkers=args.walkers, steps_shift=10, shift_damp=0.1)
+    drv = FciMCPar(eng, tau=tau, target_walkers=args.walkers, steps_shift=10, shift_damp=0.5)
     tg = time.perf_counter()
     while drv.single_part.any() and drv.iter < args.max_growth_iters:
         drv.run(10)
@@ -160,7 +161,7 @@ def main():
         "steps": K, "warmup": args.warmup, "ms_per_step": dt * 1e3 / K, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "synthetic molecular 10e/20o (seed 7, rank 60), initiator 3, "
-                               "PCHB unif-fast:fast-fast + heat-bath singles, tau 0.005, plain FP64",
+                               "PCHB unif-unif:fast-fast + heat-bath singles (psingles 0.1), tau 0.005, plain FP64",
                    "target_walkers": args.walkers,
                    "mean_tot_parts": walker_iters / K, "mean_determinants": counts["Nd"],
                    "attempts_per_step": counts["Na"], "spawns_per_step": counts["Ns"],

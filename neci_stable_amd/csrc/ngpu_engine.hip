@@ -178,6 +178,16 @@ void rec(ngpu_t* h, int stage) {
   if (h->prof && h->ev_ok) (void)hipEventRecord(h->ev[h->ev_cur][stage], h->stream);
 }
 
+// tombstone reclamation and the load guard, launched unconditionally (the
+// kernels exit at once unless the device-side check asks for a rebuild)
+template <int W>
+void hash_upkeep(ngpu_t* h) {
+  hipLaunchKernelGGL(k_hash_check, dim3(1), dim3(1), 0, h->stream, h->cnt, h->wk.L * (int64_t)kBucketEntries);
+  hipLaunchKernelGGL(k_hash_clear<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->wk, h->cnt);
+  hipLaunchKernelGGL(k_hash_refill<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt);
+  hipLaunchKernelGGL(k_hash_done, dim3(1), dim3(1), 0, h->stream, h->cnt);
+}
+
 // radix sort of the spawned keys; returns the buffer index holding the result
 template <int W>
 int radix_sort(ngpu_t* h, const unsigned long long* n_ptr) {
@@ -223,6 +233,7 @@ void annihilation_chain(ngpu_t* h, uint32_t iter) {
   rec(h, 7);
   hipLaunchKernelGGL(k_stats<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->dout,
                      iter);
+  hash_upkeep<W>(h);
   hipLaunchKernelGGL(k_end_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dcyc, h->cnt, h->scap);
   rec(h, 8);
 }
@@ -768,6 +779,8 @@ int ngpu_upload_walkers(ngpu_t* h, const int64_t* rows, int64_t n) {
     hipLaunchKernelGGL(k_insert<2>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->n_key,
                        h->n_sgn, h->n_flag, &h->cnt->n_new);
   hipLaunchKernelGGL(k_post_insert, dim3(1), dim3(1), 0, h->stream, h->cnt);
+  if (h->W == 1) hash_upkeep<1>(h);
+  else hash_upkeep<2>(h);
   HIP_OK(hipMemcpyAsync(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
   return err_from_mask(h->hcnt->err);

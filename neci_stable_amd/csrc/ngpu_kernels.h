@@ -44,8 +44,12 @@ struct Counters {
   unsigned long long n_attempts, n_spawned, n_unique, n_new;
   unsigned long long n_live, n_tomb, n_occ;
   unsigned int err;
-  unsigned int pad;
+  unsigned int rebuild;  // set when tombstones must be reclaimed
 };
+
+// probe limit of the walker hash: the load guard (k_hash_check) keeps probe
+// sequences short, the cap turns a saturated table into an error, not a stall
+constexpr int64_t kMaxProbe = 4096;
 
 struct Walk {
   uint64_t* keys;     // cap * W
@@ -130,14 +134,15 @@ __device__ inline void remove_det(const Walk& wk, Counters* cnt, int64_t slot) {
 
 // hash lookup: returns slot or -1
 template <int W>
-__device__ inline int64_t hash_lookup(const Walk& wk, const uint64_t* key, int64_t home) {
+__device__ inline int64_t hash_lookup(const Walk& wk, const uint64_t* key, int64_t home, Counters* cnt) {
   int64_t b = home;
-  for (int64_t probe = 0; probe < wk.L; ++probe) {
+  int64_t maxp = wk.L < kMaxProbe ? wk.L : kMaxProbe;
+  for (int64_t probe = 0; probe < maxp; ++probe) {
     const uint64_t* bk = wk.htab + (size_t)b * 8 * W;
     const int32_t* sl = (const int32_t*)(bk + 4 * W);
 #pragma unroll
     for (int e = 0; e < kBucketEntries; ++e) {
-      int32_t s = __builtin_nontemporal_load(&sl[e]);
+      int32_t s = sl[e];
       if (s == kSlotEmpty) return -1;
       if (s >= 0) {
         bool eq = true;
@@ -148,6 +153,7 @@ __device__ inline int64_t hash_lookup(const Walk& wk, const uint64_t* key, int64
     }
     b = (b + 1 == wk.L) ? 0 : b + 1;
   }
+  if (maxp < wk.L) atomicOr(&cnt->err, 1u << NGPU_ERR_HASH_FULL);
   return -1;
 }
 
@@ -155,12 +161,13 @@ __device__ inline int64_t hash_lookup(const Walk& wk, const uint64_t* key, int64
 template <int W>
 __device__ inline int64_t hash_insert(const Walk& wk, const uint64_t* key, int64_t home, int32_t slot) {
   int64_t b = home;
-  for (int64_t probe = 0; probe < wk.L; ++probe) {
+  int64_t maxp = wk.L < kMaxProbe ? wk.L : kMaxProbe;
+  for (int64_t probe = 0; probe < maxp; ++probe) {
     uint64_t* bk = wk.htab + (size_t)b * 8 * W;
     int32_t* sl = (int32_t*)(bk + 4 * W);
 #pragma unroll
     for (int e = 0; e < kBucketEntries; ++e) {
-      int32_t cur = __builtin_nontemporal_load(&sl[e]);
+      int32_t cur = sl[e];
       while (cur == kSlotEmpty || cur == kSlotTomb) {
         int32_t prev = atomicCAS(&sl[e], cur, slot);
         if (prev == cur) {
@@ -642,7 +649,7 @@ __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counte
       for (int r = 0; r < S; ++r) ss[r] = u_sgn[i * S + r];
       sflag = u_flag[i];
       int64_t home = find_walker_hash<W>(key, s.rh, s.L) - 1;
-      int64_t pos = hash_lookup<W>(wk, key, home);
+      int64_t pos = hash_lookup<W>(wk, key, home, cnt);
       if (pos >= 0) {
         double cs[kMaxRep];
         double tot = 0.0;
@@ -820,6 +827,53 @@ __global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* c
   }
   if (my_occ) atomicAdd(&cnt->n_occ, my_occ);
   bs.flush(out);
+}
+
+// ---------------------------------------------------------- hash upkeep
+// Tombstones left by removals are reclaimed by rebuilding the index in place
+// once live + tombstone entries pass 60% of the entries and tombstones alone
+// pass 10%; more than 80% live
+// entries is a capacity error (MEMORYFACHASH too small), reported instead of
+// letting probe sequences grow without bound.
+__global__ void k_hash_check(Counters* cnt, int64_t entries) {
+  unsigned long long used = cnt->n_live + cnt->n_tomb;
+  cnt->rebuild = (used * 10ull > (unsigned long long)entries * 6ull &&
+                  cnt->n_tomb * 10ull > (unsigned long long)entries) ? 1u : 0u;
+  if (cnt->n_live * 10ull > (unsigned long long)entries * 8ull) cnt->err |= 1u << NGPU_ERR_HASH_FULL;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_hash_clear(Walk wk, const Counters* cnt) {
+  if (!cnt->rebuild) return;
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < wk.L; b += (int64_t)gridDim.x * blockDim.x) {
+    int32_t* sl = (int32_t*)(wk.htab + (size_t)b * 8 * W + 4 * W);
+#pragma unroll
+    for (int e = 0; e < kBucketEntries; ++e) sl[e] = kSlotEmpty;
+  }
+}
+
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_hash_refill(DevSys s, Walk wk, Counters* cnt) {
+  if (!cnt->rebuild) return;
+  int64_t nslots = (int64_t)cnt->nslots;
+  for (int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < nslots;
+       slot += (int64_t)gridDim.x * blockDim.x) {
+    if (wk.flags[slot] & kFlagRemoved) continue;
+    uint64_t key[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
+    int64_t home = find_walker_hash<W>(key, s.rh, s.L) - 1;
+    int64_t hp = hash_insert<W>(wk, key, home, (int32_t)slot);
+    if (hp < 0) atomicOr(&cnt->err, 1u << NGPU_ERR_HASH_FULL);
+    else wk.hpos[slot] = hp;
+  }
+}
+
+__global__ void k_hash_done(Counters* cnt) {
+  if (cnt->rebuild) {
+    cnt->n_tomb = 0;
+    cnt->rebuild = 0;
+  }
 }
 
 }  // namespace ngpu

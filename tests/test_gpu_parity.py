@@ -241,3 +241,32 @@ def test_iterations_match_oracle(name, nrep, real):
     else:
         n = assert_same_walkers(eng, orc)
         assert n > 50
+
+
+def test_hash_rebuild_keeps_parity():
+    """A walker hash small enough that tombstones trigger in-place rebuilds:
+    the walker set must still follow the oracle exactly."""
+    tau = 0.01
+    spec = SPECS["hubbard10"](init_walkers=4000, seed=5, n_hash_buckets=160)
+    eng = engine(spec)
+    orc = oracle(spec, tau)
+    eng.start_single_part(40.0)
+    orc.add_walkers(orc.ref[None, :], np.full((1, 1), 40.0), [1])
+    for it in range(1, 81):
+        a = eng.iterate(it, tau, 0.6)
+        b = orc.iterate(it, 0.6)
+        assert b["rc"] == 0
+        assert_stats(a, b, 1)
+    assert assert_same_walkers(eng, orc) > 100
+
+
+def test_hash_full_is_an_error():
+    """More live determinants than 80% of the hash entries: NGPU_ERR_HASH_FULL."""
+    from neci_stable_amd._lib import NgpuError
+    spec = SPECS["hubbard10"](init_walkers=4000, n_hash_buckets=8)
+    eng = engine(spec)
+    rng = np.random.default_rng(3)
+    dets = np.unique(random_dets(spec, 60, rng), axis=0)
+    with pytest.raises(NgpuError) as e:
+        eng.upload(eng.rows(dets, np.ones((len(dets), 1))))
+    assert e.value.code == 6
