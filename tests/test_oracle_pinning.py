@@ -1,0 +1,282 @@
+"""Pins the CPU oracle (CPU only, no GPU): each oracle function is checked
+against something that fixes it independently of its own code: published
+known-answer vectors, the reference's own unit-test values, brute-force
+second quantisation, exhaustive enumeration, or exact diagonalisation.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import ed_reference as ed
+import oracle_ctypes as oc
+from common import hubbard_spec, mol_spec, molecule
+from neci_stable_amd import HubbardChain, Spec, decode_det, encode_det
+from neci_stable_amd.systems import find_walker_hash, umat_ind, umat_len
+
+
+# --------------------------------------------------------------- Philox
+
+
+@pytest.mark.parametrize("ctr,key,expect", [
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+])
+def test_philox_known_answers(ctr, key, expect):
+    """Random123 philox4x32_10 known-answer vectors (kat_vectors)."""
+    assert oc.philox(ctr, key) == expect
+
+
+# -------------------------------------------------------------- hashing
+
+
+def test_find_walker_hash_against_integer_restatement():
+    """hash.F90:23-37 restated with Python big integers and explicit int64 wrap."""
+    rng = np.random.default_rng(1)
+    for nbasis, nel in [(20, 10), (40, 10), (80, 26), (126, 30)]:
+        rh = rng.permutation(np.arange(1, 1000 * nbasis + 1))[:nbasis].astype(np.int32)
+        for L in [7, 1000, 700000, 2 ** 27]:
+            for _ in range(200):
+                orbs = sorted(rng.choice(np.arange(1, nbasis + 1), nel, replace=False).tolist())
+                assert oc.find_walker_hash(orbs, rh, L) == find_walker_hash(orbs, rh, L)
+    # a hand-checked value: h = 1099511628211*(1*3) + 2*5 = 3298534884643 ; mod 1000 = 643
+    assert find_walker_hash([1, 2], np.array([3, 5], dtype=np.int32), 1000) == 644
+    assert oc.find_walker_hash([1, 2], np.array([3, 5], dtype=np.int32), 1000) == 644
+
+
+def test_find_walker_hash_negative_wrap():
+    """Products that wrap int64 negative take |mod(h, L)| + 1 with a truncating mod."""
+    rh = np.array([40000, 39999, 39998, 39997, 39996, 39995, 39994, 39993], dtype=np.int32)
+    orbs = [1, 2, 3, 4, 5, 6, 7, 8]
+    h = 0
+    for i, o in enumerate(orbs, 1):
+        h = (1099511628211 * h + int(rh[o - 1]) * i) % (1 << 64)
+    hs = h - (1 << 64) if h >= 1 << 63 else h
+    assert hs < 0
+    L = 999983
+    m = abs(hs) % L  # Fortran mod truncates toward zero: |mod(h, L)| = |h| mod L
+    assert oc.find_walker_hash(orbs, rh, L) == m + 1
+
+
+# ------------------------------------------------------------ integrals
+
+
+def test_umat_ind_is_a_bijection_on_8fold_classes():
+    n = 7
+    seen = {}
+    for i, j, k, l in itertools.product(range(1, n + 1), repeat=4):
+        idx = umat_ind(i, j, k, l)
+        assert idx == oc.lib().o_umat_ind(i, j, k, l)
+        # <ij|kl> = <kj|il> = <il|kj> = <jk... : the 8 permutations of physical notation
+        for p in [(k, j, i, l), (i, l, k, j), (k, l, i, j), (j, i, l, k), (l, i, j, k), (j, k, l, i),
+                  (l, k, j, i)]:
+            assert umat_ind(*p) == idx
+        cls = tuple(sorted([tuple(sorted((i, k))), tuple(sorted((j, l)))]))
+        seen.setdefault(idx, set()).add(cls)
+    assert all(len(v) == 1 for v in seen.values())
+    assert sorted(seen) == list(range(1, umat_len(n) + 1))
+
+
+# --------------------------------------------------------- excitations
+
+
+def _perm_parity(seq):
+    seq = list(seq)
+    par = 0
+    for i in range(len(seq)):
+        for j in range(i + 1, len(seq)):
+            if seq[i] > seq[j]:
+                par ^= 1
+    return par
+
+
+@pytest.mark.parametrize("nb,nel", [(8, 3), (10, 5), (12, 4)])
+def test_make_single_and_double_exhaustive(nb, nel):
+    """nJ must be the sorted result and tParity the sign of the permutation that
+    sorts nI with sources replaced in place by the targets (the fermionic sign)."""
+    for nI in itertools.combinations(range(1, nb + 1), nel):
+        virt = [o for o in range(1, nb + 1) if o not in nI]
+        for e in range(1, nel + 1):
+            for t in virt:
+                nJ, ex, par = oc.make_single(nI, e, t)
+                rep = list(nI)
+                rep[e - 1] = t
+                assert list(nJ) == sorted(rep)
+                assert par == _perm_parity(rep)
+        for e1, e2 in itertools.combinations(range(1, nel + 1), 2):
+            for t1, t2 in itertools.combinations(virt, 2):
+                nJ, ex, par = oc.make_double(nI, e1, e2, t1, t2)
+                rep = list(nI)
+                rep[e1 - 1] = t1
+                rep[e2 - 1] = t2
+                assert list(nJ) == sorted(rep)
+                assert par == _perm_parity(rep)
+
+
+# ------------------------------------------------------ matrix elements
+
+
+def test_slater_condon_against_second_quantisation():
+    """Every H_IJ of a 5-orbital (10 spin orbital) molecule from the oracle's
+    Slater-Condon rules with packed UMAT equals the brute-force second-quantised
+    matrix built from the dense (pq|rs) tensor."""
+    mol = molecule(5, 4, rank=6, seed=11)
+    spec = Spec(mol, init_walkers=100)
+    orc = oc.Oracle(spec, 0.01)
+    dets = ed.all_dets(10, 4, nalpha=2)
+    H = ed.molecular_hamiltonian_dense(mol.h1, mol.eri, mol.ecore, dets)
+    orbs = [decode_det(np.array([d], dtype=np.int64)) for d in dets]
+    for a in range(len(dets)):
+        for b in range(len(dets)):
+            v = orc.helement(orbs[a], orbs[b])
+            assert abs(v - H[a, b]) <= 1e-11 * max(1.0, abs(H[a, b])), (orbs[a], orbs[b], v, H[a, b])
+    # hermiticity and the diagonal through its own routine
+    for a in range(len(dets)):
+        assert abs(orc.diag(orbs[a]) - H[a, a]) <= 1e-11 * max(1.0, abs(H[a, a]))
+
+
+def _hub_oracle(nsites, nel, u, bhub, periodic):
+    sysm = HubbardChain(nsites, nel, u, bhub=bhub, periodic=periodic)
+    spec = Spec(sysm, init_walkers=100, ref_orbs=list(range(1, nel + 1)))
+    return oc.Oracle(spec, 0.01)
+
+
+def test_hubbard_helements_reference_unit_test_values():
+    """get_helement_test (unit_tests/real_space_hubbard/test_real_space_hubbard.F90:2465-2595)."""
+    o = _hub_oracle(2, 2, 0.0, 1.0, periodic=False)
+    assert o.helement([1, 2], [1, 2]) == 0.0
+    o = _hub_oracle(2, 2, 1.0, 1.0, periodic=False)
+    assert o.helement([1, 2], [1, 2]) == 1.0
+    assert o.helement([1, 2], [1, 4]) == 1.0
+    assert o.helement([1, 2], [2, 3]) == -1.0
+    assert o.helement([1, 2], [2, 4]) == 0.0
+    o = _hub_oracle(3, 2, 0.0, 1.0, periodic=True)
+    assert o.helement([1, 3], [1, 3]) == 0.0
+    assert o.helement([1, 3], [1, 5]) == 1.0
+    assert o.helement([1, 3], [3, 5]) == -1.0
+    o = _hub_oracle(3, 4, 1.0, 1.0, periodic=True)
+    assert o.helement([1, 2, 3, 4], [1, 2, 3, 4]) == 2.0
+    assert o.helement([1, 2, 3, 4], [1, 2, 3, 6]) == 1.0
+    assert o.helement([1, 2, 3, 4], [1, 3, 4, 6]) == 1.0
+    assert o.helement([1, 2, 3, 4], [2, 3, 4, 5]) == -1.0
+
+
+def test_hubbard_helements_against_second_quantisation():
+    sysm = HubbardChain(6, 6, 3.0, bhub=-1.0)
+    spec = Spec(sysm, init_walkers=100)
+    o = oc.Oracle(spec, 0.01)
+    dets = ed.hubbard_dets(6, 3, 3)
+    H = ed.hubbard_hamiltonian(6, sysm.nbr, 3.0, -1.0, dets).toarray()
+    orbs = [decode_det(np.array([d], dtype=np.int64)) for d in dets]
+    for a in range(0, len(dets), 3):
+        for b in range(len(dets)):
+            assert o.helement(orbs[a], orbs[b]) == H[b, a]
+
+
+# ---------------------------------------------------------- alias tables
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_alias_table_reproduces_weights(seed):
+    """test_aliasTables (unit_tests/sampler/test_aliasTables.F90:135,170): the
+    distribution encoded by (bias, alias) equals w / sum(w); probs == w / sum(w)."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 300))
+    w = rng.random(n) * (rng.random(n) < 0.7)
+    if w.sum() == 0:
+        w[0] = 1.0
+    bias, alias, prob = oc.alias_build(w)
+    implied = bias / n
+    for j in range(n):
+        implied[alias[j]] += (1.0 - bias[j]) / n
+    np.testing.assert_allclose(implied, w / w.sum(), atol=1e-13)
+    ssum = 0.0
+    for x in w:  # Fortran sum(): sequential
+        ssum += float(x)
+    np.testing.assert_array_equal(prob, w / ssum)
+
+
+# ---------------------------------------------------- generator statistics
+
+
+@pytest.mark.parametrize("particles", [0, 1])
+def test_generator_pgen_consistency_and_frequencies(particles):
+    """run_excit_gen_tester (src/unit_test_helpers.F90:438-638): the returned pgen
+    equals calc_pgen; over N draws every connected det appears and its frequency
+    matches pgen (15% relative criterion of pchb_excitgen_test_helper.F90:105-111)."""
+    mol = molecule(5, 4, rank=6, seed=11)
+    spec = Spec(mol, init_walkers=100, particles=particles, psingles=0.3)
+    o = oc.Oracle(spec, 0.01)
+    ref = o.ref
+    rng = np.random.default_rng(4)
+    n = 300000
+    counts = {}
+    pgens = {}
+    draws = rng.random((n, 6))
+    for i in range(n):
+        r = o.gen_excit(ref, draws[i])
+        if r is None:
+            continue
+        dj, ic, ex, tp, pg = r
+        key = tuple(dj)
+        counts[key] = counts.get(key, 0) + 1
+        if key in pgens:
+            assert pgens[key] == pg
+        else:
+            pgens[key] = pg
+            assert abs(o.calc_pgen(ref, ic, ex) - pg) <= 1e-14
+    # connected dets with nonzero H must all be generated
+    occ = spec.ref_orbs
+    for d in ed.all_dets(10, 4, nalpha=2):
+        orbs = decode_det(np.array([d], dtype=np.int64))
+        ic = len(set(occ) - set(orbs))
+        if ic in (1, 2) and abs(o.helement(occ, orbs)) > 1e-10:
+            assert tuple(encode_det(orbs, 10)) in counts, orbs
+    for key, c in counts.items():
+        if pgens[key] * n > 400:
+            assert abs(c / n - pgens[key]) / pgens[key] < 0.15
+    total = sum(pgens.values())
+    assert total <= 1.0 + 1e-12
+
+
+# ----------------------------------------------------- end-to-end energy
+
+
+def test_oracle_fciqmc_energy_vs_exact_diagonalisation():
+    """Plain FCIQMC on a 6-site Hubbard ring: shift within 3 sigma of ED."""
+    from neci_stable_amd.fcimc import FciMCPar, OracleCycleAdapter, reblock
+    sysm = HubbardChain(6, 6, 4.0, bhub=-1.0)
+    spec = Spec(sysm, init_walkers=2000, seed=99, trunc_initiator=False)
+    o = oc.Oracle(spec, 0.02)
+    o.add_walkers(o.ref[None, :], np.array([[10.0]]), [1])
+    drv = FciMCPar(OracleCycleAdapter(o, 1), tau=0.02, target_walkers=2000)
+    drv.run(6000)
+    sh = np.array([h[1][0] for h in drv.shift_hist])
+    mean, err = reblock(sh[len(sh) // 4:])
+    e_ed = ed.ground_state(ed.hubbard_hamiltonian(6, sysm.nbr, 4.0, -1.0, ed.hubbard_dets(6, 3, 3)))
+    assert abs(mean + o.hii - e_ed) < 3 * err + 1e-3, (mean, err, e_ed)
+
+
+# ------------------------------------------------ C ABI of the engine
+
+
+def test_engine_library_exports_every_declared_symbol():
+    import ctypes
+    import re
+    import os
+    from neci_stable_amd import _lib
+    hdr = open(os.path.join(os.path.dirname(_lib.__file__), "..", "include", "neci_gpu.h")).read()
+    names = set(re.findall(r"^int\s+(ngpu_\w+)\s*\(", hdr, re.M))
+    assert len(names) >= 20
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+    assert names == set(_lib.EXPORTED)
+
+
+def test_engine_refuses_to_run_without_gpu_or_library(monkeypatch, tmp_path):
+    from neci_stable_amd import _lib
+    with pytest.raises(RuntimeError):
+        _lib.load_library(str(tmp_path / "missing.so"))
