@@ -1,0 +1,142 @@
+! neci_gpu_mod -- ISO_C_BINDING interface to the GPU iteration engine
+! (include/neci_gpu.h).  This is the module a NECI host build uses in place of
+! the body of PerformFCIMCycPar (src/FciMCPar.F90:1226-1873): integrals and
+! keywords stay on the host, one ngpu_iterate call per iteration, the update
+! cycle (update_shift) stays in Fortran.
+module neci_gpu_mod
+    use iso_c_binding
+    implicit none
+    private
+
+    integer(c_int), parameter, public :: NGPU_MAX_REPLICAS = 16
+    integer(c_int), parameter, public :: NGPU_MODEL_HUBBARD = 0, NGPU_MODEL_MOLECULAR = 1
+    integer(c_int), parameter, public :: NGPU_SINGLES_HEATBATH = 1
+    integer(c_int), parameter, public :: NGPU_PART_UNIF_UNIF = 0, NGPU_PART_UNIF_FAST = 1
+
+    type, bind(c), public :: ngpu_sys_t
+        integer(c_int32_t) :: nel, nbasis, n_replicas, model
+        integer(c_int32_t) :: t_trunc_initiator, t_all_real_coeff, t_real_spawn_cutoff, flag_layout
+        real(c_double) :: initiator_walk_no, real_spawn_cutoff, occupied_thresh, av_mc_excits
+        integer(c_int64_t) :: max_walkers, max_spawned, n_hash_buckets
+        integer(c_int32_t) :: seed, pad
+    end type
+
+    type, bind(c), public :: ngpu_iter_in_t
+        real(c_double) :: tau
+        real(c_double) :: diag_sft(NGPU_MAX_REPLICAS)
+        integer(c_int32_t) :: iter, pad
+    end type
+
+    type, bind(c), public :: ngpu_iter_out_t
+        real(c_double) :: nborn(NGPU_MAX_REPLICAS), ndied(NGPU_MAX_REPLICAS), nannihil(NGPU_MAX_REPLICAS)
+        real(c_double) :: naborted(NGPU_MAX_REPLICAS), nremoved(NGPU_MAX_REPLICAS)
+        real(c_double) :: tot_parts(NGPU_MAX_REPLICAS), norm_sq(NGPU_MAX_REPLICAS)
+        real(c_double) :: enum_cyc(NGPU_MAX_REPLICAS), hf_cyc(NGPU_MAX_REPLICAS), noat_doubs(NGPU_MAX_REPLICAS)
+        real(c_double) :: spawn_from_sing(NGPU_MAX_REPLICAS), inst_no_at_hf(NGPU_MAX_REPLICAS)
+        real(c_double) :: no_init_dets(NGPU_MAX_REPLICAS), no_init_walk(NGPU_MAX_REPLICAS)
+        real(c_double) :: no_noninit_dets(NGPU_MAX_REPLICAS), no_noninit_walk(NGPU_MAX_REPLICAS)
+        real(c_double) :: n_valid_excits, n_invalid_excits, max_cyc_spawn
+        real(c_double) :: bloom_count(3), bloom_size(3)
+        integer(c_int64_t) :: tot_walkers, n_attempts, n_spawned, n_unique, n_new
+        integer(c_int32_t) :: err, pad
+    end type
+
+    public :: ngpu_create, ngpu_destroy, ngpu_set_integrals, ngpu_set_lattice, ngpu_set_excitgen
+    public :: ngpu_set_hash, ngpu_set_reference, ngpu_upload_walkers, ngpu_download_walkers
+    public :: ngpu_iterate, ngpu_get_stats, ngpu_get_cycle_stats, ngpu_find_walker_hash_batch
+    public :: ngpu_diag_helement_batch
+
+    interface
+        integer(c_int) function ngpu_create(h, sys) bind(c, name="ngpu_create")
+            import :: c_int, c_ptr, ngpu_sys_t
+            type(c_ptr), intent(out) :: h
+            type(ngpu_sys_t), intent(in) :: sys
+        end function
+        integer(c_int) function ngpu_destroy(h) bind(c, name="ngpu_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+        end function
+        integer(c_int) function ngpu_set_integrals(h, tmat2d, umat, umat_len, ecore) &
+            bind(c, name="ngpu_set_integrals")
+            import :: c_int, c_ptr, c_double, c_int64_t
+            type(c_ptr), value :: h
+            real(c_double), intent(in) :: tmat2d(*), umat(*)
+            integer(c_int64_t), value :: umat_len
+            real(c_double), value :: ecore
+        end function
+        integer(c_int) function ngpu_set_lattice(h, nbr, nsites, maxnbr, bhub, uhub) &
+            bind(c, name="ngpu_set_lattice")
+            import :: c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), intent(in) :: nbr(*)
+            integer(c_int32_t), value :: nsites, maxnbr
+            real(c_double), value :: bhub, uhub
+        end function
+        integer(c_int) function ngpu_set_excitgen(h, singles, particles, psingles, pparallel) &
+            bind(c, name="ngpu_set_excitgen")
+            import :: c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: singles, particles
+            real(c_double), value :: psingles, pparallel
+        end function
+        integer(c_int) function ngpu_set_hash(h, random_hash2) bind(c, name="ngpu_set_hash")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), intent(in) :: random_hash2(*)
+        end function
+        integer(c_int) function ngpu_set_reference(h, ilut_ref, hii) bind(c, name="ngpu_set_reference")
+            import :: c_int, c_ptr, c_int64_t, c_double
+            type(c_ptr), value :: h
+            integer(c_int64_t), intent(in) :: ilut_ref(*)
+            real(c_double), value :: hii
+        end function
+        integer(c_int) function ngpu_upload_walkers(h, rows, n) bind(c, name="ngpu_upload_walkers")
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: h
+            integer(c_int64_t), intent(in) :: rows(*)
+            integer(c_int64_t), value :: n
+        end function
+        integer(c_int) function ngpu_download_walkers(h, rows, diag_h, offdiag_h, cap, n) &
+            bind(c, name="ngpu_download_walkers")
+            import :: c_int, c_ptr, c_int64_t, c_double
+            type(c_ptr), value :: h
+            integer(c_int64_t), intent(out) :: rows(*)
+            real(c_double), intent(out) :: diag_h(*), offdiag_h(*)
+            integer(c_int64_t), value :: cap
+            integer(c_int64_t), intent(out) :: n
+        end function
+        integer(c_int) function ngpu_iterate(h, inp, out) bind(c, name="ngpu_iterate")
+            import :: c_int, c_ptr, ngpu_iter_in_t
+            type(c_ptr), value :: h
+            type(ngpu_iter_in_t), intent(in) :: inp
+            type(c_ptr), value :: out
+        end function
+        integer(c_int) function ngpu_get_stats(h, out) bind(c, name="ngpu_get_stats")
+            import :: c_int, c_ptr, ngpu_iter_out_t
+            type(c_ptr), value :: h
+            type(ngpu_iter_out_t), intent(out) :: out
+        end function
+        integer(c_int) function ngpu_get_cycle_stats(h, sum, last, n_iters) bind(c, name="ngpu_get_cycle_stats")
+            import :: c_int, c_ptr, ngpu_iter_out_t, c_int32_t
+            type(c_ptr), value :: h
+            type(ngpu_iter_out_t), intent(out) :: sum, last
+            integer(c_int32_t), intent(out) :: n_iters
+        end function
+        integer(c_int) function ngpu_find_walker_hash_batch(h, dets, n, buckets) &
+            bind(c, name="ngpu_find_walker_hash_batch")
+            import :: c_int, c_ptr, c_int64_t, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int64_t), intent(in) :: dets(*)
+            integer(c_int64_t), value :: n
+            integer(c_int32_t), intent(out) :: buckets(*)
+        end function
+        integer(c_int) function ngpu_diag_helement_batch(h, dets, n, diag, offdiag) &
+            bind(c, name="ngpu_diag_helement_batch")
+            import :: c_int, c_ptr, c_int64_t, c_double
+            type(c_ptr), value :: h
+            integer(c_int64_t), intent(in) :: dets(*)
+            integer(c_int64_t), value :: n
+            real(c_double), intent(out) :: diag(*), offdiag(*)
+        end function
+    end interface
+end module neci_gpu_mod
