@@ -65,6 +65,8 @@ struct ngpu_engine {
   uint64_t* sk[2] = {nullptr, nullptr};
   uint32_t* si[2] = {nullptr, nullptr};
   uint32_t* hist = nullptr;
+  uint32_t* digit_tot = nullptr;
+  int64_t* sq = nullptr;  // queued single-excitation attempts
   uint64_t* u_key = nullptr;
   double* u_sgn = nullptr;
   uint32_t* u_flag = nullptr;
@@ -121,6 +123,7 @@ __global__ void k_begin_iter(ngpu_iter_out_t* out, Counters* cnt) {
     cnt->n_unique = 0;
     cnt->n_new = 0;
     cnt->n_occ = 0;
+    cnt->n_singles = 0;
     cnt->err = 0;
     out->tot_walkers = 0;
     out->n_attempts = out->n_spawned = out->n_unique = out->n_new = 0;
@@ -205,9 +208,11 @@ int radix_sort(ngpu_t* h, const unsigned long long* n_ptr) {
     int out = (int)(p & 1);
     hipLaunchKernelGGL(k_radix_hist<W>, dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, kin, n_ptr, h->scap,
                        passes[p].word, passes[p].shift, h->hist);
-    hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(kBlock), 0, h->stream, h->hist, h->nb_chunk);
+    hipLaunchKernelGGL(k_radix_scan, dim3(kRadixBins), dim3(kBlock), 0, h->stream, h->hist, h->nb_chunk,
+                       h->digit_tot);
     hipLaunchKernelGGL(k_radix_scatter<W>, dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, kin, iin,
-                       h->sk[out], h->si[out], n_ptr, h->scap, passes[p].word, passes[p].shift, h->hist);
+                       h->sk[out], h->si[out], n_ptr, h->scap, passes[p].word, passes[p].shift, h->hist,
+                       h->digit_tot);
     kin = h->sk[out];
     iin = h->si[out];
     cur = out;
@@ -216,44 +221,48 @@ int radix_sort(ngpu_t* h, const unsigned long long* n_ptr) {
 }
 
 // merge -> annihilate -> insert -> stats, shared by iterate and the explicit-list entry
-template <int W>
+template <int W, int R>
 void annihilation_chain(ngpu_t* h, uint32_t iter) {
   rec(h, 3);
   int cur = radix_sort<W>(h, &h->cnt->n_spawned);
   rec(h, 4);
-  hipLaunchKernelGGL(k_merge<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->cnt, h->dout,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_merge<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->cnt, h->dout,
                      h->sk[cur], h->si[cur], h->sp_val, h->sp_info, h->scap, h->u_key, h->u_sgn, h->u_flag);
   rec(h, 5);
-  hipLaunchKernelGGL(k_annihilate<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_annihilate<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
                      h->dout, h->u_key, h->u_sgn, h->u_flag, h->n_key, h->n_sgn, h->n_flag, iter);
   rec(h, 6);
-  hipLaunchKernelGGL(k_insert<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_insert<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
                      h->n_key, h->n_sgn, h->n_flag, &h->cnt->n_new);
-  hipLaunchKernelGGL(k_post_insert, dim3(1), dim3(1), 0, h->stream, h->cnt);
+  hipLaunchKernelGGL(k_post_insert, dim3(1), dim3(1), 0, h->stream, h->cnt, h->wk.cap);
   rec(h, 7);
-  hipLaunchKernelGGL(k_stats<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->dout,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_stats<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->dout,
                      iter);
   hash_upkeep<W>(h);
   hipLaunchKernelGGL(k_end_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dcyc, h->cnt, h->scap);
   rec(h, 8);
 }
 
-template <int W>
+template <int W, int R>
 void iterate_chain(ngpu_t* h, const ngpu_iter_in_t* in) {
   Shift sft;
   for (int r = 0; r < kMaxRep; ++r) sft.v[r] = r < h->S ? in->diag_sft[r] : 0.0;
   rec(h, 0);
   hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->cnt);
-  hipLaunchKernelGGL(k_prepass<W>, dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_prepass<W, R>), dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
                      h->dout, h->att_cnt, h->pinfo, h->blk_part, sft, in->tau, in->iter);
   rec(h, 1);
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, h->stream, h->blk_part, h->nb_chunk, h->cnt);
   hipLaunchKernelGGL(k_scan_apply, dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, h->att_cnt, h->att_off,
                      h->blk_part, h->cnt, h->S);
   rec(h, 2);
-  hipLaunchKernelGGL(k_spawn<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->dout,
-                     h->att_off, h->pinfo, h->sp_key, h->sp_val, h->sp_info, h->scap, in->tau, in->iter);
-  annihilation_chain<W>(h, in->iter);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spawn<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->dout,
+                     h->att_off, h->pinfo, h->sp_key, h->sp_val, h->sp_info, h->scap, h->sq, in->tau, in->iter);
+  if (h->sys.model == NGPU_MODEL_MOLECULAR)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spawn_singles<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
+                       h->dout, h->att_off, h->pinfo, h->sp_key, h->sp_val, h->sp_info, h->scap, h->sq, in->tau,
+                       in->iter);
+  annihilation_chain<W, R>(h, in->iter);
 }
 
 void harvest(ngpu_t* h, int slot) {
@@ -526,6 +535,8 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
     fail(dalloc(&h->si[q], (size_t)scap));
   }
   fail(dalloc(&h->hist, (size_t)h->nb_chunk * kRadixBins));
+  fail(dalloc(&h->digit_tot, (size_t)kRadixBins));
+  fail(dalloc(&h->sq, (size_t)scap));
   fail(dalloc(&h->u_key, (size_t)scap * W));
   fail(dalloc(&h->u_sgn, (size_t)scap * S));
   fail(dalloc(&h->u_flag, (size_t)scap));
@@ -580,7 +591,7 @@ int ngpu_destroy(ngpu_t* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void* ptrs[] = {h->wk.keys, h->wk.sgn, h->wk.flags, h->wk.diagH, h->wk.offdiagH, h->wk.hpos, h->wk.freeq,
                   h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->sp_key, h->sp_val, h->sp_info,
-                  h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->u_key, h->u_sgn, h->u_flag, h->n_key,
+                  h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->digit_tot, h->sq, h->u_key, h->u_sgn, h->u_flag, h->n_key,
                   h->n_sgn, h->n_flag, h->cnt, h->dout, h->dcyc, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_ab_tab,
                   h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt};
   for (void* p : ptrs)
@@ -772,13 +783,13 @@ int ngpu_upload_walkers(ngpu_t* h, const int64_t* rows, int64_t n) {
   if (rc) return rc;
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_new, (unsigned long long)n);
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, (unsigned long long*)&h->cnt->err, 0ull);
-  if (h->W == 1)
-    hipLaunchKernelGGL(k_insert<1>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->n_key,
+  {
+    auto kern = h->W == 1 ? (h->S == 1 ? k_insert<1, 1> : k_insert<1, kMaxRep>)
+                          : (h->S == 1 ? k_insert<2, 1> : k_insert<2, kMaxRep>);
+    hipLaunchKernelGGL(kern, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->n_key,
                        h->n_sgn, h->n_flag, &h->cnt->n_new);
-  else
-    hipLaunchKernelGGL(k_insert<2>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->n_key,
-                       h->n_sgn, h->n_flag, &h->cnt->n_new);
-  hipLaunchKernelGGL(k_post_insert, dim3(1), dim3(1), 0, h->stream, h->cnt);
+  }
+  hipLaunchKernelGGL(k_post_insert, dim3(1), dim3(1), 0, h->stream, h->cnt, h->wk.cap);
   if (h->W == 1) hash_upkeep<1>(h);
   else hash_upkeep<2>(h);
   HIP_OK(hipMemcpyAsync(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
@@ -843,8 +854,8 @@ int ngpu_iterate(ngpu_t* h, const ngpu_iter_in_t* in, ngpu_iter_out_t* out) {
     h->ev_cur = (int)(h->ev_next % kEvRing);
     harvest(h, h->ev_cur);
   }
-  if (h->W == 1) iterate_chain<1>(h, in);
-  else iterate_chain<2>(h, in);
+  if (h->W == 1) h->S == 1 ? iterate_chain<1, 1>(h, in) : iterate_chain<1, kMaxRep>(h, in);
+  else h->S == 1 ? iterate_chain<2, 1>(h, in) : iterate_chain<2, kMaxRep>(h, in);
   HIP_OK(hipGetLastError());
   if (h->prof && h->ev_ok) {
     h->ev_pending[h->ev_cur] = true;
@@ -1088,8 +1099,8 @@ int ngpu_annihilate(ngpu_t* h, uint32_t iter, const int64_t* sdet, const double*
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_spawned, (unsigned long long)n);
   bool p = h->prof;
   h->prof = false;
-  if (W == 1) annihilation_chain<1>(h, iter);
-  else annihilation_chain<2>(h, iter);
+  if (W == 1) h->S == 1 ? annihilation_chain<1, 1>(h, iter) : annihilation_chain<1, kMaxRep>(h, iter);
+  else h->S == 1 ? annihilation_chain<2, 1>(h, iter) : annihilation_chain<2, kMaxRep>(h, iter);
   h->prof = p;
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(h->hout, h->dout, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));

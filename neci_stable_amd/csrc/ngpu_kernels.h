@@ -45,6 +45,7 @@ struct Counters {
   unsigned long long n_live, n_tomb, n_occ;
   unsigned int err;
   unsigned int rebuild;  // set when tombstones must be reclaimed
+  unsigned long long n_singles;  // queued single-excitation attempts
 };
 
 // probe limit of the walker hash: the load guard (k_hash_check) keeps probe
@@ -87,7 +88,24 @@ __device__ inline long long wave_append(bool pred, unsigned long long* counter, 
   return (long long)(base + (unsigned long long)(popc64(m & lanemask_lt()) * n));
 }
 
-// block-local statistics accumulators in LDS, flushed once per block
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    double w = __shfl_xor(v, o);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+__device__ inline unsigned long long wave_count(bool p) { return (unsigned long long)popc64(__ballot(p)); }
+
+// Block-local statistics in LDS.  Every add/max is a wave-wide call (all 64
+// lanes, inactive lanes passing 0): the wave reduces with shuffles and one lane
+// updates LDS, so no LDS address is contended by more than one lane per wave.
 struct BlockStats {
   double* acc;  // LDS, S_NDOUBLE entries
   __device__ void zero() {
@@ -95,10 +113,13 @@ struct BlockStats {
     __syncthreads();
   }
   __device__ void add(int i, double v) {
-    if (v != 0.0) atomicAdd(&acc[i], v);
+    double t = wave_sum(v);
+    if (lane_id() == 0 && t != 0.0) atomicAdd(&acc[i], t);
   }
   __device__ void max(int i, double v) {  // v >= 0
-    if (v > 0.0) atomicMax((unsigned long long*)&acc[i], (unsigned long long)__double_as_longlong(v));
+    double t = wave_max(v);
+    if (lane_id() == 0 && t > 0.0)
+      atomicMax((unsigned long long*)&acc[i], (unsigned long long)__double_as_longlong(t));
   }
   __device__ void flush(ngpu_iter_out_t* out) {
     __syncthreads();
@@ -117,19 +138,23 @@ struct BlockStats {
   }
 };
 
-// remove a det from the walker hash and the list (RemoveHashDet, load_balancer.fpp:631-644)
+// RemoveHashDet (load_balancer.fpp:631-644) for the lanes with rm set: the
+// hash entry becomes a tombstone, the slot joins the free queue.  Wave-wide call.
 template <int W>
-__device__ inline void remove_det(const Walk& wk, Counters* cnt, int64_t slot) {
-  int64_t hp = wk.hpos[slot];
-  int64_t b = hp >> 2;
-  int e = (int)(hp & 3);
-  int32_t* sl = (int32_t*)(wk.htab + (size_t)b * 8 * W + 4 * W);
-  sl[e] = kSlotTomb;
-  wk.flags[slot] |= kFlagRemoved;
-  unsigned long long q = atomicAdd(&cnt->free_tail, 1ull);
-  wk.freeq[q % (unsigned long long)wk.cap] = slot;
-  atomicAdd(&cnt->n_tomb, 1ull);
-  atomicSub(&cnt->n_live, 1ull);
+__device__ inline void remove_dets(const Walk& wk, Counters* cnt, bool rm, int64_t slot) {
+  long long q = wave_append(rm, &cnt->free_tail);
+  if (rm) {
+    int64_t hp = wk.hpos[slot];
+    int32_t* sl = (int32_t*)(wk.htab + (size_t)(hp >> 2) * 8 * W + 4 * W);
+    sl[hp & 3] = kSlotTomb;
+    wk.flags[slot] |= kFlagRemoved;
+    wk.freeq[(unsigned long long)q % (unsigned long long)wk.cap] = slot;
+  }
+  unsigned long long c = wave_count(rm);
+  if (c && lane_id() == 0) {
+    atomicAdd(&cnt->n_tomb, c);
+    atomicSub(&cnt->n_live, c);
+  }
 }
 
 // hash lookup: returns slot or -1
@@ -212,12 +237,22 @@ __device__ inline int64_t block_exclusive_scan(int64_t v, int64_t* lds, int64_t&
   return pre + x - v;
 }
 
+// trips of a grid-stride loop that every lane of the grid runs (keeps wave-wide
+// calls convergent); the element of this lane in trip t is t*stride + gid
+__device__ inline int64_t grid_trips(int64_t n) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  return (n + stride - 1) / stride;
+}
+__device__ inline int64_t grid_elem(int64_t t) {
+  return t * (int64_t)gridDim.x * blockDim.x + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+}
+
 // ------------------------------------------------------------- prepass
 // CalcParentFlag_det + SumEContrib + decide_num_to_spawn + walker_death for
 // every occupied slot (FciMCPar.F90:1294-1758, fcimc_helper.F90:518-802,
 // 1036-1321, 2160-2407).  Blocks own contiguous slot ranges so that the block
 // sums of attempt counts feed the scan directly.
-template <int W>
+template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters* cnt,
                                                     ngpu_iter_out_t* out, int32_t* att_cnt,
                                                     uint8_t* pinfo, int64_t* blk_part, Shift sft,
@@ -226,103 +261,105 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
   __shared__ int64_t red[kBlock / kWave];
   BlockStats bs{acc_lds};
   bs.zero();
-  const int S = s.S;
+  const int S = R == 1 ? 1 : s.S;
   int64_t nslots = (int64_t)cnt->nslots;
   int64_t begin, end;
   chunk_of(nslots, begin, end);
   int64_t my_att = 0;
   bool death_err = false;
-  for (int64_t slot = begin + threadIdx.x; slot < end; slot += blockDim.x) {
-    uint32_t f = wk.flags[slot];
-    if (f & kFlagRemoved) {
-      for (int r = 0; r < S; ++r) att_cnt[slot * S + r] = 0;
-      continue;
-    }
+  for (int64_t base = begin; base < end; base += blockDim.x) {
+    int64_t slot = base + threadIdx.x;
+    bool valid = slot < end;
+    uint32_t f = valid ? wk.flags[slot] : kFlagRemoved;
+    bool live = !(f & kFlagRemoved);
     uint64_t key[W];
-#pragma unroll
-    for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
-    double sg[kMaxRep];
+    double sg[R];
     bool occ = false;
+#pragma unroll
+    for (int w = 0; w < W; ++w) key[w] = live ? wk.keys[slot * W + w] : 0ull;
     for (int r = 0; r < S; ++r) {
-      sg[r] = wk.sgn[slot * S + r];
+      sg[r] = live ? wk.sgn[slot * S + r] : 0.0;
       occ = occ || !(fabs(sg[r]) < 1.e-12);
     }
     bool core = (f & kFlagCore) != 0;
-    if (!occ) {
-      for (int r = 0; r < S; ++r) att_cnt[slot * S + r] = 0;
-      continue;
-    }
-    int exlevel = excit_level<W>(s.ref, key);
-    double hoff = wk.offdiagH[slot];
-    if (s.t_init) {
-      bool isref = det_eq<W>(key, s.ref);
-      for (int r = 0; r < S; ++r) {
+    int exlevel = occ ? excit_level<W>(s.ref, key) : -1;
+    double hoff = occ ? wk.offdiagH[slot] : 0.0;
+    double kii = occ ? wk.diagH[slot] : 0.0;
+    bool isref = occ && det_eq<W>(key, s.ref);
+    bool any_left = false;
+    double cp[R];
+    for (int r = 0; r < S; ++r) {
+      // initiator flag (CalcParentFlag_det / TestInitiator_explicit)
+      bool now = false;
+      if (occ && s.t_init) {
         bool was = (f >> r) & 1u;
         bool pop = fabs(sg[r]) > s.init_walk_no;
-        bool now = was ? (pop || isref || core) : pop;
-        if (now) {
-          f |= 1u << r;
-          bs.add(soff(F_NOINITDETS, r), 1.0);
-          bs.add(soff(F_NOINITWALK, r), fabs(sg[r]));
-        } else {
-          f &= ~(1u << r);
-          bs.add(soff(F_NONINITDETS, r), 1.0);
-          bs.add(soff(F_NONINITWALK, r), fabs(sg[r]));
+        now = was ? (pop || isref || core) : pop;
+        f = now ? (f | (1u << r)) : (f & ~(1u << r));
+      }
+      if (s.t_init) {
+        bool ini = occ && now, non = occ && !now;
+        bs.add(soff(F_NOINITDETS, r), ini ? 1.0 : 0.0);
+        bs.add(soff(F_NOINITWALK, r), ini ? fabs(sg[r]) : 0.0);
+        bs.add(soff(F_NONINITDETS, r), non ? 1.0 : 0.0);
+        bs.add(soff(F_NONINITWALK, r), non ? fabs(sg[r]) : 0.0);
+      }
+      // SumEContrib
+      bs.add(soff(F_HFCYC, r), exlevel == 0 ? sg[r] : 0.0);
+      bs.add(soff(F_NOATDOUBS, r), exlevel == 2 ? fabs(sg[r]) : 0.0);
+      bs.add(soff(F_ENUM, r), occ ? hoff * sg[r] : 0.0);
+      // decide_num_to_spawn and attempt_die share one Philox block
+      double died = 0.0, born = 0.0;
+      cp[r] = sg[r];
+      if (occ) {
+        double ua, ub;
+        draw2<W>(s.seed, key, iter, r, 0, 0, ua, ub);
+        double pa = sg[r] * s.avmc;
+        int64_t nsp = (int64_t)pa;
+        if (nsp < 0) nsp = -nsp;
+        if (fabs(fabs(pa) - (double)nsp) > 1.e-12) {
+          double extra = fabs(pa) - (double)nsp;
+          if (extra > ua) nsp++;
         }
+        att_cnt[slot * S + r] = (int32_t)nsp;
+        my_att += nsp;
+        pinfo[slot * S + r] = (uint8_t)((sg[r] < 0.0 ? 1 : 0) | (((f >> r) & 1u) ? 2 : 0));
+        double fac = tau * (kii - sft.v[r]);
+        if (fac > 2.0) death_err = true;
+        double idie = core ? 0.0 : attempt_die(s.t_real != 0, fac, fabs(sg[r]), ub);
+        double a = fabs(sg[r]);
+        died = idie < a ? idie : a;
+        double bb = idie - a;
+        born = bb > 0.0 ? bb : 0.0;
+        cp[r] = sg[r] - idie * (sg[r] >= 0.0 ? 1.0 : -1.0);
+        any_left = any_left || (fabs(cp[r]) > 1.0e-12);
+      } else if (valid) {
+        att_cnt[slot * S + r] = 0;
       }
+      bs.add(soff(F_NDIED, r), died);
+      bs.add(soff(F_NBORN, r), born);
     }
+    bool abort_anti = occ && s.t_init && any_left;
+    if (abort_anti) any_left = false;
     for (int r = 0; r < S; ++r) {
-      if (exlevel == 0) bs.add(soff(F_HFCYC, r), sg[r]);
-      if (exlevel == 2) bs.add(soff(F_NOATDOUBS, r), fabs(sg[r]));
-      bs.add(soff(F_ENUM, r), hoff * sg[r]);
-    }
-    double kii = wk.diagH[slot];
-    bool any_left = false;
-    double cp[kMaxRep];
-    for (int r = 0; r < S; ++r) {
-      double ua, ub;
-      draw2<W>(s.seed, key, iter, r, 0, 0, ua, ub);
-      double pa = sg[r] * s.avmc;
-      int64_t nsp = (int64_t)pa;
-      if (nsp < 0) nsp = -nsp;
-      if (fabs(fabs(pa) - (double)nsp) > 1.e-12) {
-        double extra = fabs(pa) - (double)nsp;
-        if (extra > ua) nsp++;
-      }
-      att_cnt[slot * S + r] = (int32_t)nsp;
-      my_att += nsp;
-      pinfo[slot * S + r] = (uint8_t)((sg[r] < 0.0 ? 1 : 0) | (((f >> r) & 1u) ? 2 : 0));
-      // death
-      double fac = tau * (kii - sft.v[r]);
-      if (fac > 2.0) death_err = true;
-      double idie = core ? 0.0 : attempt_die(s.t_real != 0, fac, fabs(sg[r]), ub);
-      double a = fabs(sg[r]);
-      bs.add(soff(F_NDIED, r), idie < a ? idie : a);
-      double bb = idie - a;
-      if (bb > 0.0) bs.add(soff(F_NBORN, r), bb);
-      cp[r] = sg[r] - idie * (sg[r] >= 0.0 ? 1.0 : -1.0);
-      any_left = any_left || (fabs(cp[r]) > 1.0e-12);
-    }
-    if (s.t_init && any_left) {
-      for (int r = 0; r < S; ++r)
+      double ab = 0.0;
+      if (abort_anti) {
         if ((cp[r] > 0.0) != (sg[r] > 0.0)) {
-          bs.add(soff(F_NABORTED, r), fabs(cp[r]));
+          ab = fabs(cp[r]);
           cp[r] = 0.0;
         }
-      any_left = false;
-      for (int r = 0; r < S; ++r) any_left = any_left || (fabs(cp[r]) > 1.0e-12);
+        any_left = any_left || (fabs(cp[r]) > 1.0e-12);
+      }
+      bs.add(soff(F_NABORTED, r), ab);
     }
-    if (any_left || core) {
-      for (int r = 0; r < S; ++r) wk.sgn[slot * S + r] = cp[r];
+    bool rm = occ && !(any_left || core);
+    if (occ) {
+      for (int r = 0; r < S; ++r) wk.sgn[slot * S + r] = rm ? 0.0 : cp[r];
       wk.flags[slot] = f;
-    } else {
-      for (int r = 0; r < S; ++r) wk.sgn[slot * S + r] = 0.0;
-      wk.flags[slot] = f;
-      remove_det<W>(wk, cnt, slot);
     }
+    remove_dets<W>(wk, cnt, rm, slot);
   }
   if (death_err) atomicOr(&cnt->err, 1u << NGPU_ERR_DEATH_PROB);
-  // block sum of attempts
   int64_t tot;
   (void)block_exclusive_scan(my_att, red, tot);
   if (threadIdx.x == 0) blk_part[blockIdx.x] = tot;
@@ -369,81 +406,155 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(const int32_t* att_cnt, i
 }
 
 // ------------------------------------------------------------- spawning
-template <int W>
-__global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* cnt, ngpu_iter_out_t* out,
-                                                  const int64_t* att_off, const uint8_t* pinfo,
-                                                  uint64_t* sp_key, double* sp_val, uint32_t* sp_info,
-                                                  int64_t scap, double tau, uint32_t iter) {
-  __shared__ double acc_lds[S_NDOUBLE];
-  BlockStats bs{acc_lds};
-  bs.zero();
-  const int S = s.S;
-  const int64_t total = (int64_t)cnt->n_attempts;
-  const int64_t npairs = (int64_t)cnt->nslots * S;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  bool overflow = false;
-  // every lane runs the same number of loop trips so wave_append stays convergent
-  int64_t trips = (total + stride - 1) / stride;
-  for (int64_t t = 0; t < trips; ++t) {
-    int64_t a = t * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool emit = false;
-    uint64_t child[W];
-    double nspawn = 0.0;
-    uint32_t info = 0;
-    if (a < total) {
-      // parent pair: last p with att_off[p] <= a
-      int64_t lo = 0, hi = npairs - 1;
-      while (lo < hi) {
-        int64_t mid = (lo + hi + 1) >> 1;
-        if (att_off[mid] <= a) lo = mid; else hi = mid - 1;
-      }
-      int64_t p = lo;
-      int64_t slot = p / S;
-      int r = (int)(p - slot * S);
-      uint32_t k = (uint32_t)(a - att_off[p]);
-      uint64_t key[W];
+// parent (slot, replica) of attempt a: last pair p with att_off[p] <= a
+__device__ inline int64_t parent_pair(const int64_t* att_off, int64_t npairs, int64_t a) {
+  int64_t lo = 0, hi = npairs - 1;
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if (att_off[mid] <= a) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// one spawning attempt (attempt_create_normal + create_particle); mode 0: any
+// excitation, 1: doubles only (singles are queued), 2: singles only.  Wave-wide.
+template <int W, int R>
+__device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* cnt, BlockStats& bs,
+                                     const int64_t* att_off, const uint8_t* pinfo, int64_t npairs,
+                                     bool active, int64_t a, int mode, uint64_t* sp_key, double* sp_val,
+                                     uint32_t* sp_info, int64_t scap, int64_t* sq, double tau,
+                                     uint32_t iter, bool& overflow) {
+  const int S = R == 1 ? 1 : s.S;
+  bool emit = false, queue = false;
+  uint64_t child[W];
+  double nspawn = 0.0, an = 0.0, raw = 0.0;
+  uint32_t info = 0;
+  int r = 0, ic = 0;
+  double valid = 0.0, invalid = 0.0;
+  if (active) {
+    int64_t p = parent_pair(att_off, npairs, a);
+    int64_t slot = p / S;
+    r = (int)(p - slot * S);
+    uint32_t k = (uint32_t)(a - att_off[p]);
+    uint64_t key[W];
 #pragma unroll
-      for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
-      uint8_t pi = pinfo[p];
-      double dr[6];
-      draw2<W>(s.seed, key, iter, r, k, 1, dr[0], dr[1]);
+    for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
+    double dr[6];
+    draw2<W>(s.seed, key, iter, r, k, 1, dr[0], dr[1]);
+    bool single = s.model == 1 && dr[0] < s.psingles;
+    if (mode == 1 && single) {
+      queue = true;
+    } else {
       draw2<W>(s.seed, key, iter, r, k, 2, dr[2], dr[3]);
       draw2<W>(s.seed, key, iter, r, k, 3, dr[4], dr[5]);
-      Excit ex = gen_excit<W>(s, key, dr);
-      if (ex.ic == 0) {
-        bs.add(S_NINVALID, 1.0);
+      Excit ex;
+      if (s.model == 0) {
+        ex = gen_hubbard<W>(s, key, dr);
+      } else if (single) {
+        ex = gen_single_heatbath<W>(s, key, dr[1], dr[2]);
+        ex.pgen = ex.pgen * s.psingles;
       } else {
-        bs.add(S_NVALID, 1.0);
+        ex = gen_double_pchb<W>(s, key, dr);
+        ex.pgen = ex.pgen * s.pdoubles;
+      }
+      if (ex.ic == 0) {
+        invalid = 1.0;
+      } else {
+        valid = 1.0;
+        ic = ex.ic;
         double hel = helement_ex<W>(s, key, ex.ic, ex.e0, ex.e1, ex.e2, ex.e3, ex.parity);
+        uint8_t pi = pinfo[p];
         double wsign = (pi & 1) ? -1.0 : 1.0;
-        double raw;
         nspawn = attempt_create(s, tau, hel, ex.pgen, wsign, dr[5], raw);
-        bs.max(S_MAXSPAWN, fabs(raw));
+        raw = fabs(raw);
         if (fabs(nspawn) > kEps) {
           emit = true;
-          double an = fabs(nspawn);
-          bs.add(soff(F_NBORN, r), an);
-          if (ex.ic == 1) bs.add(soff(F_SPAWNSING, r), an);
-          if (an > s.init_walk_no) {
-            bs.add(S_BLOOMC + ex.ic, 1.0);
-            bs.max(S_BLOOMS + ex.ic, an);
-          }
+          an = fabs(nspawn);
           apply_excit<W>(key, ex, child);
           info = (uint32_t)r | ((s.t_init && (pi & 2)) ? kInfoInit : 0u);
         }
       }
     }
-    long long idx = wave_append(emit, &cnt->n_spawned);
-    if (emit) {
-      if (idx < scap) {
+  }
+  long long qi = wave_append(queue, &cnt->n_singles);
+  if (queue) {
+    if (qi < scap) sq[qi] = a;
+    else overflow = true;
+  }
+  bs.add(S_NINVALID, invalid);
+  bs.add(S_NVALID, valid);
+  bs.max(S_MAXSPAWN, raw);
+  for (int rr = 0; rr < S; ++rr) {
+    bs.add(soff(F_NBORN, rr), rr == r ? an : 0.0);
+    if (mode != 1) bs.add(soff(F_SPAWNSING, rr), (rr == r && ic == 1) ? an : 0.0);
+  }
+  bool bloom = emit && an > s.init_walk_no;
+  if (mode != 2) {
+    bs.add(S_BLOOMC + 2, (bloom && ic == 2) ? 1.0 : 0.0);
+    bs.max(S_BLOOMS + 2, (bloom && ic == 2) ? an : 0.0);
+  }
+  if (mode != 1) {
+    bs.add(S_BLOOMC + 1, (bloom && ic == 1) ? 1.0 : 0.0);
+    bs.max(S_BLOOMS + 1, (bloom && ic == 1) ? an : 0.0);
+  }
+  long long idx = wave_append(emit, &cnt->n_spawned);
+  if (emit) {
+    if (idx < scap) {
 #pragma unroll
-        for (int w = 0; w < W; ++w) sp_key[idx * W + w] = child[w];
-        sp_val[idx] = nspawn;
-        sp_info[idx] = info;
-      } else {
-        overflow = true;
-      }
+      for (int w = 0; w < W; ++w) sp_key[idx * W + w] = child[w];
+      sp_val[idx] = nspawn;
+      sp_info[idx] = info;
+    } else {
+      overflow = true;
     }
+  }
+}
+
+// all attempts: Hubbard spawns directly; molecular doubles spawn here while
+// singles (the heat-bath list is ~20x longer per attempt) go to a queue so that
+// waves do not idle on a few long lanes
+template <int W, int R>
+__global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* cnt, ngpu_iter_out_t* out,
+                                                  const int64_t* att_off, const uint8_t* pinfo,
+                                                  uint64_t* sp_key, double* sp_val, uint32_t* sp_info,
+                                                  int64_t scap, int64_t* sq, double tau, uint32_t iter) {
+  __shared__ double acc_lds[S_NDOUBLE];
+  BlockStats bs{acc_lds};
+  bs.zero();
+  const int64_t total = (int64_t)cnt->n_attempts;
+  const int64_t npairs = (int64_t)cnt->nslots * s.S;
+  bool overflow = false;
+  int mode = s.model == 1 ? 1 : 0;
+  int64_t trips = grid_trips(total);
+  for (int64_t t = 0; t < trips; ++t) {
+    int64_t a = grid_elem(t);
+    spawn_attempt<W, R>(s, wk, cnt, bs, att_off, pinfo, npairs, a < total, a, mode, sp_key, sp_val, sp_info,
+                     scap, sq, tau, iter, overflow);
+  }
+  if (overflow) atomicOr(&cnt->err, 1u << NGPU_ERR_SPAWN_FULL);
+  bs.flush(out);
+}
+
+template <int W, int R>
+__global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Counters* cnt, ngpu_iter_out_t* out,
+                                                          const int64_t* att_off, const uint8_t* pinfo,
+                                                          uint64_t* sp_key, double* sp_val, uint32_t* sp_info,
+                                                          int64_t scap, const int64_t* sq, double tau,
+                                                          uint32_t iter) {
+  __shared__ double acc_lds[S_NDOUBLE];
+  BlockStats bs{acc_lds};
+  bs.zero();
+  int64_t nq = (int64_t)cnt->n_singles;
+  if (nq > scap) nq = scap;
+  const int64_t npairs = (int64_t)cnt->nslots * s.S;
+  bool overflow = false;
+  int64_t trips = grid_trips(nq);
+  for (int64_t t = 0; t < trips; ++t) {
+    int64_t q = grid_elem(t);
+    bool act = q < nq;
+    int64_t a = act ? sq[q] : 0;
+    spawn_attempt<W, R>(s, wk, cnt, bs, att_off, pinfo, npairs, act, a, 2, sp_key, sp_val, sp_info, scap,
+                     nullptr, tau, iter, overflow);
   }
   if (overflow) atomicOr(&cnt->err, 1u << NGPU_ERR_SPAWN_FULL);
   bs.flush(out);
@@ -451,7 +562,7 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
 
 // ---------------------------------------------------------- radix sort
 // Stable LSD radix sort, 8-bit digits, reduce-then-scan: per-block digit
-// histograms, one digit-major exclusive scan, stable block-local scatter.
+// histograms, a per-digit scan over blocks, stable block-local scatter.
 constexpr int kRadixBins = 256;
 
 template <int W>
@@ -477,36 +588,42 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const uint64_t* keys, con
   for (int d = threadIdx.x; d < kRadixBins; d += blockDim.x) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
 }
 
-// digit-major exclusive scan of nb*256 counters (single block)
-__global__ void __launch_bounds__(kBlock) k_radix_scan(uint32_t* hist, int nb) {
+// one block per digit: exclusive scan of that digit's per-block counts
+__global__ void __launch_bounds__(kBlock) k_radix_scan(uint32_t* hist, int nb, uint32_t* digit_tot) {
   __shared__ int64_t red[kBlock / kWave];
-  int64_t total_n = (int64_t)nb * kRadixBins;
-  int64_t per = (total_n + kBlock - 1) / kBlock;
-  int64_t b0 = threadIdx.x * per;
+  uint32_t* row = hist + (size_t)blockIdx.x * nb;
+  int per = (nb + kBlock - 1) / kBlock;
+  int b0 = threadIdx.x * per;
   int64_t loc = 0;
-  for (int64_t i = 0; i < per; ++i)
-    if (b0 + i < total_n) loc += hist[b0 + i];
+  for (int i = 0; i < per; ++i)
+    if (b0 + i < nb) loc += row[b0 + i];
   int64_t tot;
   int64_t pre = block_exclusive_scan(loc, red, tot);
-  for (int64_t i = 0; i < per; ++i)
-    if (b0 + i < total_n) {
-      uint32_t v = hist[b0 + i];
-      hist[b0 + i] = (uint32_t)pre;
+  for (int i = 0; i < per; ++i)
+    if (b0 + i < nb) {
+      uint32_t v = row[b0 + i];
+      row[b0 + i] = (uint32_t)pre;
       pre += v;
     }
+  if (threadIdx.x == 0) digit_tot[blockIdx.x] = (uint32_t)tot;
 }
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* kin, const uint32_t* iin,
                                                           uint64_t* kout, uint32_t* iout,
                                                           const unsigned long long* n_ptr, int64_t ncap,
-                                                          int word, int shift, const uint32_t* hist) {
+                                                          int word, int shift, const uint32_t* hist,
+                                                          const uint32_t* digit_tot) {
   __shared__ uint32_t run[kRadixBins];
   __shared__ uint32_t wcnt[kBlock / kWave][kRadixBins];
+  __shared__ int64_t red[kBlock / kWave];
   int wv = threadIdx.x / kWave;
-  for (int d = threadIdx.x; d < kRadixBins; d += blockDim.x) {
-    run[d] = hist[(size_t)d * gridDim.x + blockIdx.x];
-    for (int q = 0; q < kBlock / kWave; ++q) wcnt[q][d] = 0;
+  {
+    // digit bases: exclusive scan of the digit totals (blockDim == kRadixBins)
+    int64_t tot;
+    int64_t base = block_exclusive_scan((int64_t)digit_tot[threadIdx.x], red, tot);
+    run[threadIdx.x] = (uint32_t)base + hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x];
+    for (int q = 0; q < kBlock / kWave; ++q) wcnt[q][threadIdx.x] = 0;
   }
   __syncthreads();
   int64_t n = (int64_t)*n_ptr;
@@ -543,9 +660,9 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* kin, c
     }
     __syncthreads();
     for (int dd = threadIdx.x; dd < kRadixBins; dd += blockDim.x) {
-      uint32_t s = 0;
-      for (int q = 0; q < kBlock / kWave; ++q) { s += wcnt[q][dd]; wcnt[q][dd] = 0; }
-      run[dd] += s;
+      uint32_t sum = 0;
+      for (int q = 0; q < kBlock / kWave; ++q) { sum += wcnt[q][dd]; wcnt[q][dd] = 0; }
+      run[dd] += sum;
     }
     __syncthreads();
   }
@@ -555,7 +672,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* kin, c
 // CompressSpawnedList (Annihilation.F90:249-515) + FindResidualParticle
 // (:551-634): the head of each run of equal keys sums the run in sorted
 // (= emission, the sort is stable) order.
-template <int W>
+template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, ngpu_iter_out_t* out,
                                                   const uint64_t* skey, const uint32_t* sidx,
                                                   const double* sp_val, const uint32_t* sp_info,
@@ -564,15 +681,16 @@ __global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, ngpu_
   __shared__ double acc_lds[S_NDOUBLE];
   BlockStats bs{acc_lds};
   bs.zero();
-  const int S = s.S;
+  const int S = R == 1 ? 1 : s.S;
   int64_t n = (int64_t)cnt->n_spawned;
   if (n > scap) n = scap;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  int64_t trips = (n + stride - 1) / stride;
+  int64_t trips = grid_trips(n);
+  double ann[R];
+  for (int r = 0; r < S; ++r) ann[r] = 0.0;
   for (int64_t t = 0; t < trips; ++t) {
-    int64_t i = t * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = grid_elem(t);
     bool keep = false;
-    double sum[kMaxRep];
+    double sum[R];
     uint32_t flag = 0;
     uint64_t key[W];
     if (i < n) {
@@ -601,7 +719,7 @@ __global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, ngpu_
             }
             if (cv * nv < 0.0) {
               double x = fabs(cv), y = fabs(nv);
-              bs.add(soff(F_NANNIHIL, r), 2.0 * (x < y ? x : y));
+              ann[r] += 2.0 * (x < y ? x : y);
             }
             sum[r] = cv + nv;
           }
@@ -619,13 +737,14 @@ __global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, ngpu_
       u_flag[u] = flag;
     }
   }
+  for (int r = 0; r < S; ++r) bs.add(soff(F_NANNIHIL, r), ann[r]);
   bs.flush(out);
 }
 
 // ---------------------------------------------------------- annihilation
 // AnnihilateSpawnedParts (Annihilation.F90:965-1352): found dets are updated
 // in place (each unique key is owned by one lane); absent dets go to the new list.
-template <int W>
+template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counters* cnt, ngpu_iter_out_t* out,
                                                        const uint64_t* u_key, const double* u_sgn,
                                                        const uint32_t* u_flag, uint64_t* n_key,
@@ -633,15 +752,17 @@ __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counte
   __shared__ double acc_lds[S_NDOUBLE];
   BlockStats bs{acc_lds};
   bs.zero();
-  const int S = s.S;
+  const int S = R == 1 ? 1 : s.S;
   int64_t nu = (int64_t)cnt->n_unique;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  int64_t trips = (nu + stride - 1) / stride;
+  int64_t trips = grid_trips(nu);
+  double st_ab[R], st_an[R], st_rm[R], st_bn[R];
+  for (int r = 0; r < S; ++r) st_ab[r] = st_an[r] = st_rm[r] = st_bn[r] = 0.0;
   for (int64_t t = 0; t < trips; ++t) {
-    int64_t i = t * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool isnew = false;
+    int64_t i = grid_elem(t);
+    bool isnew = false, rm = false;
+    int64_t pos = -1;
     uint64_t key[W];
-    double ss[kMaxRep];
+    double ss[R];
     uint32_t sflag = 0;
     if (i < nu) {
 #pragma unroll
@@ -649,9 +770,9 @@ __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counte
       for (int r = 0; r < S; ++r) ss[r] = u_sgn[i * S + r];
       sflag = u_flag[i];
       int64_t home = find_walker_hash<W>(key, s.rh, s.L) - 1;
-      int64_t pos = hash_lookup<W>(wk, key, home, cnt);
+      pos = hash_lookup<W>(wk, key, home, cnt);
       if (pos >= 0) {
-        double cs[kMaxRep];
+        double cs[R];
         double tot = 0.0;
         for (int r = 0; r < S; ++r) { cs[r] = wk.sgn[pos * S + r]; tot += fabs(cs[r]); }
         bool determ = (wk.flags[pos] & kFlagCore) != 0;
@@ -659,12 +780,12 @@ __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counte
           for (int r = 0; r < S; ++r) {
             double prod = cs[r] * ss[r];
             if (fabs(cs[r]) < 1.0e-12 && s.t_init && !((sflag >> r) & 1u) && !determ) {
-              bs.add(soff(F_NABORTED, r), fabs(ss[r]));
+              st_ab[r] += fabs(ss[r]);
               ss[r] = 0.0;
             }
             if (prod < 0.0) {
               double x = fabs(cs[r]), y = fabs(ss[r]);
-              bs.add(soff(F_NANNIHIL, r), 2.0 * (x < y ? x : y));
+              st_an[r] += 2.0 * (x < y ? x : y);
             }
           }
           bool occ = false;
@@ -673,13 +794,13 @@ __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counte
             wk.sgn[pos * S + r] = cs[r];
             occ = occ || !(fabs(cs[r]) < 1.e-12);
           }
-          if (!determ && !occ) remove_det<W>(wk, cnt, pos);
+          rm = !determ && !occ;
         }
       } else {
         if (s.t_init)
           for (int r = 0; r < S; ++r)
             if (!((sflag >> r) & 1u)) {
-              bs.add(soff(F_NABORTED, r), fabs(ss[r]));
+              st_ab[r] += fabs(ss[r]);
               ss[r] = 0.0;
             }
         bool occ = false;
@@ -693,10 +814,10 @@ __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counte
               double u, dummy;
               draw2<W>(s.seed, key, iter, r, 0, 4, u, dummy);
               if (pRemove > u) {
-                bs.add(soff(F_NREMOVED, r), fabs(ss[r]));
+                st_rm[r] += fabs(ss[r]);
                 ss[r] = 0.0;
               } else {
-                bs.add(soff(F_NBORN, r), thr - fabs(ss[r]));
+                st_bn[r] += thr - fabs(ss[r]);
                 ss[r] = ss[r] >= 0.0 ? thr : -thr;
               }
             }
@@ -707,6 +828,7 @@ __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counte
         }
       }
     }
+    remove_dets<W>(wk, cnt, rm, pos);
     long long q = wave_append(isnew, &cnt->n_new);
     if (isnew) {
 #pragma unroll
@@ -715,117 +837,142 @@ __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counte
       n_flag[q] = sflag & 0xFFFFu;
     }
   }
+  for (int r = 0; r < S; ++r) {
+    bs.add(soff(F_NABORTED, r), st_ab[r]);
+    bs.add(soff(F_NANNIHIL, r), st_an[r]);
+    bs.add(soff(F_NREMOVED, r), st_rm[r]);
+    bs.add(soff(F_NBORN, r), st_bn[r]);
+  }
   bs.flush(out);
 }
 
 // AddNewHashDet (load_balancer.fpp:514-629) for the new list
-template <int W>
+template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* cnt, const uint64_t* n_key,
                                                    const double* n_sgn, const uint32_t* n_flag,
                                                    const unsigned long long* n_ptr) {
-  const int S = s.S;
+  const int S = R == 1 ? 1 : s.S;
   int64_t nn = (int64_t)*n_ptr;
   unsigned long long tail = cnt->free_tail;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    uint64_t key[W];
+  int64_t trips = grid_trips(nn);
+  for (int64_t t = 0; t < trips; ++t) {
+    int64_t i = grid_elem(t);
+    bool act = i < nn;
+    long long q = wave_append(act, &cnt->free_head);
+    bool fresh = act && (unsigned long long)q >= tail;
+    long long ns = wave_append(fresh, &cnt->nslots);
+    int64_t slot = -1;
+    bool ok = false;
+    if (act) {
+      if (!fresh) slot = wk.freeq[(unsigned long long)q % (unsigned long long)wk.cap];
+      else if (ns < wk.cap) slot = ns;
+      else atomicOr(&cnt->err, 1u << NGPU_ERR_WALKERS_FULL);
+    }
+    if (slot >= 0) {
+      uint64_t key[W];
 #pragma unroll
-    for (int w = 0; w < W; ++w) key[w] = n_key[i * W + w];
-    unsigned long long q = atomicAdd(&cnt->free_head, 1ull);
-    int64_t slot;
-    if (q < tail) {
-      slot = wk.freeq[q % (unsigned long long)wk.cap];
-    } else {
-      unsigned long long ns = atomicAdd(&cnt->nslots, 1ull);
-      if ((int64_t)ns >= wk.cap) {
-        atomicOr(&cnt->err, 1u << NGPU_ERR_WALKERS_FULL);
-        atomicSub(&cnt->nslots, 1ull);
-        continue;
+      for (int w = 0; w < W; ++w) key[w] = n_key[i * W + w];
+      int64_t home = find_walker_hash<W>(key, s.rh, s.L) - 1;
+      int64_t hp = hash_insert<W>(wk, key, home, (int32_t)slot);
+      if (hp < 0) {
+        atomicOr(&cnt->err, 1u << NGPU_ERR_HASH_FULL);
+        wk.flags[slot] = kFlagRemoved;
+      } else {
+#pragma unroll
+        for (int w = 0; w < W; ++w) wk.keys[slot * W + w] = key[w];
+        for (int r = 0; r < S; ++r) wk.sgn[slot * S + r] = n_sgn[i * S + r];
+        wk.flags[slot] = n_flag[i] & ~kFlagRemoved;
+        wk.diagH[slot] = diag_helement<W>(s, key) - s.hii;
+        wk.offdiagH[slot] = offdiag_helement<W>(s, key);
+        wk.hpos[slot] = hp;
+        ok = true;
       }
-      slot = (int64_t)ns;
     }
-    int64_t home = find_walker_hash<W>(key, s.rh, s.L) - 1;
-    int64_t hp = hash_insert<W>(wk, key, home, (int32_t)slot);
-    if (hp < 0) {
-      atomicOr(&cnt->err, 1u << NGPU_ERR_HASH_FULL);
-      wk.flags[slot] = kFlagRemoved;
-      continue;
-    }
-#pragma unroll
-    for (int w = 0; w < W; ++w) wk.keys[slot * W + w] = key[w];
-    for (int r = 0; r < S; ++r) wk.sgn[slot * S + r] = n_sgn[i * S + r];
-    wk.flags[slot] = n_flag[i] & ~kFlagRemoved;
-    wk.diagH[slot] = diag_helement<W>(s, key) - s.hii;
-    wk.offdiagH[slot] = offdiag_helement<W>(s, key);
-    wk.hpos[slot] = hp;
-    if (s.t_real == 0 && s.t_init == 0) {}
-    atomicAdd(&cnt->n_live, 1ull);
+    unsigned long long c = wave_count(ok);
+    if (c && lane_id() == 0) atomicAdd(&cnt->n_live, c);
   }
 }
 
-// free-queue bookkeeping after inserts: pops beyond the tail appended instead
-__global__ void k_post_insert(Counters* cnt) {
+// free-queue bookkeeping after inserts: pops beyond the tail appended instead;
+// appends beyond the capacity were refused
+__global__ void k_post_insert(Counters* cnt, int64_t cap) {
   if (cnt->free_head > cnt->free_tail) cnt->free_head = cnt->free_tail;
+  if (cnt->nslots > (unsigned long long)cap) cnt->nslots = (unsigned long long)cap;
 }
 
 // CalcHashTableStats (load_balancer.fpp:646-805)
-template <int W>
+template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* cnt, ngpu_iter_out_t* out,
                                                   uint32_t iter) {
   __shared__ double acc_lds[S_NDOUBLE];
   BlockStats bs{acc_lds};
   bs.zero();
-  const int S = s.S;
+  const int S = R == 1 ? 1 : s.S;
   int64_t nslots = (int64_t)cnt->nslots;
   unsigned long long my_occ = 0;
-  for (int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < nslots;
-       slot += (int64_t)gridDim.x * blockDim.x) {
-    uint32_t f = wk.flags[slot];
-    if (f & kFlagRemoved) continue;
-    bool determ = (f & kFlagCore) != 0;
-    double sg[kMaxRep];
-    bool occ = false;
-    for (int r = 0; r < S; ++r) { sg[r] = wk.sgn[slot * S + r]; occ = occ || !(fabs(sg[r]) < 1.e-12); }
-    if (!occ && !determ) continue;
-    uint64_t key[W];
+  double tp[R], nq[R], rmv[R], bn[R];
+  for (int r = 0; r < S; ++r) tp[r] = nq[r] = rmv[r] = bn[r] = 0.0;
+  int64_t trips = grid_trips(nslots);
+  for (int64_t t = 0; t < trips; ++t) {
+    int64_t slot = grid_elem(t);
+    bool rm = false;
+    if (slot < nslots) {
+      uint32_t f = wk.flags[slot];
+      if (!(f & kFlagRemoved)) {
+        bool determ = (f & kFlagCore) != 0;
+        double sg[R];
+        bool occ = false;
+        for (int r = 0; r < S; ++r) { sg[r] = wk.sgn[slot * S + r]; occ = occ || !(fabs(sg[r]) < 1.e-12); }
+        if (occ || determ) {
+          uint64_t key[W];
 #pragma unroll
-    for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
-    if (s.t_real) {
-      double thr = s.occ_thresh;
-      bool changed = false;
-      for (int r = 0; r < S; ++r) {
-        if (!determ && fabs(sg[r]) > 1.e-12 && fabs(sg[r]) < thr) {
-          double pRemove = (thr - fabs(sg[r])) / thr;
-          double u, dummy;
-          draw2<W>(s.seed, key, iter, r, 0, 5, u, dummy);
-          if (pRemove > u) {
-            bs.add(soff(F_NREMOVED, r), fabs(sg[r]));
-            sg[r] = 0.0;
-          } else {
-            bs.add(soff(F_NBORN, r), thr - fabs(sg[r]));
-            sg[r] = sg[r] >= 0.0 ? thr : -thr;
+          for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
+          if (s.t_real) {
+            double thr = s.occ_thresh;
+            bool changed = false;
+            for (int r = 0; r < S; ++r) {
+              if (!determ && fabs(sg[r]) > 1.e-12 && fabs(sg[r]) < thr) {
+                double pRemove = (thr - fabs(sg[r])) / thr;
+                double u, dummy;
+                draw2<W>(s.seed, key, iter, r, 0, 5, u, dummy);
+                if (pRemove > u) {
+                  rmv[r] += fabs(sg[r]);
+                  sg[r] = 0.0;
+                } else {
+                  bn[r] += thr - fabs(sg[r]);
+                  sg[r] = sg[r] >= 0.0 ? thr : -thr;
+                }
+                changed = true;
+              }
+            }
+            if (changed) {
+              occ = false;
+              for (int r = 0; r < S; ++r) { wk.sgn[slot * S + r] = sg[r]; occ = occ || !(fabs(sg[r]) < 1.e-12); }
+              rm = !occ && !determ;
+            }
           }
-          changed = true;
-        }
-      }
-      if (changed) {
-        occ = false;
-        for (int r = 0; r < S; ++r) { wk.sgn[slot * S + r] = sg[r]; occ = occ || !(fabs(sg[r]) < 1.e-12); }
-        if (!occ && !determ) {
-          remove_det<W>(wk, cnt, slot);
-          continue;
+          if (!rm) {
+            my_occ++;
+            for (int r = 0; r < S; ++r) {
+              tp[r] += fabs(sg[r]);
+              nq[r] += sg[r] * sg[r];
+            }
+            if (det_eq<W>(key, s.ref))
+              for (int r = 0; r < S; ++r) bs.acc[soff(F_INSTHF, r)] = sg[r];
+          }
         }
       }
     }
-    my_occ++;
-    for (int r = 0; r < S; ++r) {
-      bs.add(soff(F_TOTPARTS, r), fabs(sg[r]));
-      bs.add(soff(F_NORMSQ, r), sg[r] * sg[r]);
-    }
-    if (det_eq<W>(key, s.ref))
-      for (int r = 0; r < S; ++r) bs.acc[soff(F_INSTHF, r)] = sg[r];
+    remove_dets<W>(wk, cnt, rm, slot);
   }
-  if (my_occ) atomicAdd(&cnt->n_occ, my_occ);
+  for (int r = 0; r < S; ++r) {
+    bs.add(soff(F_TOTPARTS, r), tp[r]);
+    bs.add(soff(F_NORMSQ, r), nq[r]);
+    bs.add(soff(F_NREMOVED, r), rmv[r]);
+    bs.add(soff(F_NBORN, r), bn[r]);
+  }
+  unsigned long long c = (unsigned long long)wave_sum((double)my_occ);
+  if (c && lane_id() == 0) atomicAdd(&cnt->n_occ, c);
   bs.flush(out);
 }
 
