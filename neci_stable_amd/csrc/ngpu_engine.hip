@@ -50,6 +50,7 @@ struct ngpu_engine {
   Walk wk{};
   Counters* cnt = nullptr;
   ngpu_iter_out_t* dout = nullptr;
+  double* dpart = nullptr;          // kStatCopies rows of block statistics
   ngpu_iter_out_t* hout = nullptr;  // pinned
   Counters* hcnt = nullptr;         // pinned
   // attempts
@@ -114,9 +115,10 @@ int err_from_mask(unsigned int m) {
   return NGPU_OK;
 }
 
-__global__ void k_begin_iter(ngpu_iter_out_t* out, Counters* cnt) {
+__global__ void k_begin_iter(ngpu_iter_out_t* out, double* part, Counters* cnt) {
   double* g = (double*)out;
   for (int i = threadIdx.x; i < S_NDOUBLE; i += blockDim.x) g[i] = 0.0;
+  for (int i = threadIdx.x; i < kStatCopies * kStatStride; i += blockDim.x) part[i] = 0.0;
   if (threadIdx.x == 0) {
     cnt->n_attempts = 0;
     cnt->n_spawned = 0;
@@ -124,6 +126,7 @@ __global__ void k_begin_iter(ngpu_iter_out_t* out, Counters* cnt) {
     cnt->n_new = 0;
     cnt->n_occ = 0;
     cnt->n_singles = 0;
+    cnt->n_fail = 0;
     cnt->err = 0;
     out->tot_walkers = 0;
     out->n_attempts = out->n_spawned = out->n_unique = out->n_new = 0;
@@ -131,10 +134,28 @@ __global__ void k_begin_iter(ngpu_iter_out_t* out, Counters* cnt) {
   }
 }
 
-// final counters of the iteration, then accumulation into the cycle sums
-// (ENumCyc, HFCyc, NoBorn, ... are cycle accumulators in the reference too)
-__global__ void k_end_iter(ngpu_iter_out_t* out, ngpu_iter_out_t* cyc, Counters* cnt, int64_t scap) {
-  __shared__ int done;
+// sum of the block-statistics rows, final counters of the iteration, then
+// accumulation into the cycle sums (ENumCyc, HFCyc, NoBorn, ... are cycle
+// accumulators in the reference too)
+__global__ void k_end_iter(ngpu_iter_out_t* out, const double* part, ngpu_iter_out_t* cyc, Counters* cnt,
+                           int64_t scap) {
+  double* g = (double*)out;
+  double* c = (double*)cyc;
+  for (int i = threadIdx.x; i < kStatStride; i += blockDim.x) {
+    bool is_max = stat_is_max(i);
+    double v = 0.0;
+    for (int q = 0; q < kStatCopies; ++q) {
+      double x = part[(size_t)q * kStatStride + i];
+      v = is_max ? (x > v ? x : v) : v + x;
+    }
+    if (i == X_NOCC) cnt->n_occ = (unsigned long long)v;
+    if (i >= S_NDOUBLE) continue;
+    g[i] = v;
+    if (is_max) c[i] = c[i] > v ? c[i] : v;
+    else if (i >= soff(F_INSTHF, 0) && i < soff(F_INSTHF, kMaxRep)) c[i] = v;
+    else c[i] += v;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     out->tot_walkers = (int64_t)cnt->n_occ;
     out->n_attempts = (int64_t)cnt->n_attempts;
@@ -143,18 +164,6 @@ __global__ void k_end_iter(ngpu_iter_out_t* out, ngpu_iter_out_t* cyc, Counters*
     out->n_unique = (int64_t)cnt->n_unique;
     out->n_new = (int64_t)cnt->n_new;
     out->err = (int32_t)cnt->err;
-    done = 1;
-  }
-  __syncthreads();
-  const double* g = (const double*)out;
-  double* c = (double*)cyc;
-  for (int i = threadIdx.x; i < S_NDOUBLE; i += blockDim.x) {
-    bool is_max = (i == S_MAXSPAWN) || (i >= S_BLOOMS && i < S_BLOOMS + 3);
-    if (is_max) c[i] = c[i] > g[i] ? c[i] : g[i];
-    else if (i >= soff(F_INSTHF, 0) && i < soff(F_INSTHF, kMaxRep)) c[i] = g[i];
-    else c[i] += g[i];
-  }
-  if (threadIdx.x == 0 && done) {
     cyc->tot_walkers = out->tot_walkers;
     cyc->n_attempts += out->n_attempts;
     cyc->n_spawned += out->n_spawned;
@@ -226,20 +235,20 @@ void annihilation_chain(ngpu_t* h, uint32_t iter) {
   rec(h, 3);
   int cur = radix_sort<W>(h, &h->cnt->n_spawned);
   rec(h, 4);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_merge<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->cnt, h->dout,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_merge<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->cnt, h->dpart,
                      h->sk[cur], h->si[cur], h->sp_val, h->sp_info, h->scap, h->u_key, h->u_sgn, h->u_flag);
   rec(h, 5);
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_annihilate<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
-                     h->dout, h->u_key, h->u_sgn, h->u_flag, h->n_key, h->n_sgn, h->n_flag, iter);
+                     h->dpart, h->u_key, h->u_sgn, h->u_flag, h->n_key, h->n_sgn, h->n_flag, iter);
   rec(h, 6);
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_insert<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
                      h->n_key, h->n_sgn, h->n_flag, &h->cnt->n_new);
-  hipLaunchKernelGGL(k_post_insert, dim3(1), dim3(1), 0, h->stream, h->cnt, h->wk.cap);
+  hipLaunchKernelGGL(k_post_insert, dim3(1), dim3(1), 0, h->stream, h->cnt, &h->cnt->n_new, h->wk.cap);
   rec(h, 7);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_stats<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->dout,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_stats<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->dpart,
                      iter);
   hash_upkeep<W>(h);
-  hipLaunchKernelGGL(k_end_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dcyc, h->cnt, h->scap);
+  hipLaunchKernelGGL(k_end_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dpart, h->dcyc, h->cnt, h->scap);
   rec(h, 8);
 }
 
@@ -248,19 +257,19 @@ void iterate_chain(ngpu_t* h, const ngpu_iter_in_t* in) {
   Shift sft;
   for (int r = 0; r < kMaxRep; ++r) sft.v[r] = r < h->S ? in->diag_sft[r] : 0.0;
   rec(h, 0);
-  hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->cnt);
+  hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dpart, h->cnt);
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_prepass<W, R>), dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
-                     h->dout, h->att_cnt, h->pinfo, h->blk_part, sft, in->tau, in->iter);
+                     h->dpart, h->att_cnt, h->pinfo, h->blk_part, sft, in->tau, in->iter);
   rec(h, 1);
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, h->stream, h->blk_part, h->nb_chunk, h->cnt);
   hipLaunchKernelGGL(k_scan_apply, dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, h->att_cnt, h->att_off,
                      h->blk_part, h->cnt, h->S);
   rec(h, 2);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spawn<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->dout,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spawn<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->dpart,
                      h->att_off, h->pinfo, h->sp_key, h->sp_val, h->sp_info, h->scap, h->sq, in->tau, in->iter);
   if (h->sys.model == NGPU_MODEL_MOLECULAR)
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spawn_singles<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
-                       h->dout, h->att_off, h->pinfo, h->sp_key, h->sp_val, h->sp_info, h->scap, h->sq, in->tau,
+                       h->dpart, h->att_off, h->pinfo, h->sp_key, h->sp_val, h->sp_info, h->scap, h->sq, in->tau,
                        in->iter);
   annihilation_chain<W, R>(h, in->iter);
 }
@@ -546,6 +555,7 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
   fail(dalloc(&h->n_flag, (size_t)ncap));
   fail(dalloc(&h->cnt, 1));
   fail(dalloc(&h->dout, 1));
+  fail(dalloc(&h->dpart, (size_t)kStatCopies * kStatStride));
   fail(dalloc(&h->dcyc, 1));
   fail(hipHostMalloc((void**)&h->hout, sizeof(ngpu_iter_out_t), hipHostMallocDefault));
   fail(hipHostMalloc((void**)&h->hcnt, sizeof(Counters), hipHostMallocDefault));
@@ -553,6 +563,7 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
     fail(hipMemsetAsync(wk.htab, 0xFF, (size_t)wk.L * 8 * W * sizeof(uint64_t), h->stream));
     fail(hipMemsetAsync(h->cnt, 0, sizeof(Counters), h->stream));
     fail(hipMemsetAsync(h->dout, 0, sizeof(ngpu_iter_out_t), h->stream));
+    fail(hipMemsetAsync(h->dpart, 0, sizeof(double) * kStatCopies * kStatStride, h->stream));
     fail(hipMemsetAsync(h->dcyc, 0, sizeof(ngpu_iter_out_t), h->stream));
     fail(hipMemsetAsync(wk.flags, 0, (size_t)cap * sizeof(uint32_t), h->stream));
     fail(hipStreamSynchronize(h->stream));
@@ -592,7 +603,7 @@ int ngpu_destroy(ngpu_t* h) {
   void* ptrs[] = {h->wk.keys, h->wk.sgn, h->wk.flags, h->wk.diagH, h->wk.offdiagH, h->wk.hpos, h->wk.freeq,
                   h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->sp_key, h->sp_val, h->sp_info,
                   h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->digit_tot, h->sq, h->u_key, h->u_sgn, h->u_flag, h->n_key,
-                  h->n_sgn, h->n_flag, h->cnt, h->dout, h->dcyc, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_ab_tab,
+                  h->n_sgn, h->n_flag, h->cnt, h->dout, h->dpart, h->dcyc, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_ab_tab,
                   h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -783,13 +794,14 @@ int ngpu_upload_walkers(ngpu_t* h, const int64_t* rows, int64_t n) {
   if (rc) return rc;
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_new, (unsigned long long)n);
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, (unsigned long long*)&h->cnt->err, 0ull);
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_fail, 0ull);
   {
     auto kern = h->W == 1 ? (h->S == 1 ? k_insert<1, 1> : k_insert<1, kMaxRep>)
                           : (h->S == 1 ? k_insert<2, 1> : k_insert<2, kMaxRep>);
     hipLaunchKernelGGL(kern, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->n_key,
                        h->n_sgn, h->n_flag, &h->cnt->n_new);
   }
-  hipLaunchKernelGGL(k_post_insert, dim3(1), dim3(1), 0, h->stream, h->cnt, h->wk.cap);
+  hipLaunchKernelGGL(k_post_insert, dim3(1), dim3(1), 0, h->stream, h->cnt, &h->cnt->n_new, h->wk.cap);
   if (h->W == 1) hash_upkeep<1>(h);
   else hash_upkeep<2>(h);
   HIP_OK(hipMemcpyAsync(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
@@ -1095,7 +1107,7 @@ int ngpu_annihilate(ngpu_t* h, uint32_t iter, const int64_t* sdet, const double*
     HIP_OK(hipMemcpyAsync(h->sp_val, sval, (size_t)n * 8, hipMemcpyHostToDevice, h->stream));
     HIP_OK(hipMemcpyAsync(h->sp_info, sinfo, (size_t)n * 4, hipMemcpyHostToDevice, h->stream));
   }
-  hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->cnt);
+  hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dpart, h->cnt);
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_spawned, (unsigned long long)n);
   bool p = h->prof;
   h->prof = false;

@@ -37,15 +37,36 @@ constexpr int S_NDOUBLE = S_NVALID + 9;
 __host__ __device__ constexpr int soff(int f, int r) { return f * kMaxRep + r; }
 static_assert(sizeof(double) * S_NDOUBLE == offsetof(ngpu_iter_out_t, tot_walkers),
               "statistics block of ngpu_iter_out_t must be S_NDOUBLE doubles");
+// block statistics beyond the output block: occupied determinants
+constexpr int X_NOCC = S_NDOUBLE;
+// Blocks flush their statistics into one of kStatCopies partial rows (chosen
+// by block index), so that no address takes more than 1/kStatCopies of the
+// device-scope atomics; the end-of-iteration kernel sums the rows.
+constexpr int kStatStride = 288;  // doubles per row (2304 B, 9 lines of 256 B)
+constexpr int kStatCopies = 32;
+static_assert(X_NOCC < kStatStride, "statistics row too short");
+__host__ __device__ constexpr bool stat_is_max(int i) {
+  return i == S_MAXSPAWN || (i >= S_BLOOMS && i < S_BLOOMS + 3);
+}
 
-struct Counters {
+// Device counters.  Appends reserve ranges of the global lists once per LDS
+// flush, never per lane or per wave: device-scope atomics on one address are
+// serialised, so their count, not their payload, bounds the kernels that issue
+// them.  The fields bumped in one kernel sit in separate 256-byte lines.
+struct alignas(256) Counters {
   unsigned long long nslots;
-  unsigned long long free_head, free_tail;
-  unsigned long long n_attempts, n_spawned, n_unique, n_new;
+  unsigned long long free_head;
+  unsigned long long free_tail0;  // free_tail at the start of the iteration
   unsigned long long n_live, n_tomb, n_occ;
+  unsigned long long n_attempts;
+  unsigned long long n_fail;      // inserts refused (capacity errors)
   unsigned int err;
-  unsigned int rebuild;  // set when tombstones must be reclaimed
-  unsigned long long n_singles;  // queued single-excitation attempts
+  unsigned int rebuild;           // set when tombstones must be reclaimed
+  alignas(256) unsigned long long free_tail;
+  alignas(256) unsigned long long n_spawned;
+  alignas(256) unsigned long long n_singles;  // queued single-excitation attempts
+  alignas(256) unsigned long long n_unique;
+  alignas(256) unsigned long long n_new;
 };
 
 // probe limit of the walker hash: the load guard (k_hash_check) keeps probe
@@ -107,9 +128,9 @@ __device__ inline unsigned long long wave_count(bool p) { return (unsigned long 
 // lanes, inactive lanes passing 0): the wave reduces with shuffles and one lane
 // updates LDS, so no LDS address is contended by more than one lane per wave.
 struct BlockStats {
-  double* acc;  // LDS, S_NDOUBLE entries
+  double* acc;  // LDS, kStatStride entries
   __device__ void zero() {
-    for (int i = threadIdx.x; i < S_NDOUBLE; i += blockDim.x) acc[i] = 0.0;
+    for (int i = threadIdx.x; i < kStatStride; i += blockDim.x) acc[i] = 0.0;
     __syncthreads();
   }
   __device__ void add(int i, double v) {
@@ -121,40 +142,95 @@ struct BlockStats {
     if (lane_id() == 0 && t > 0.0)
       atomicMax((unsigned long long*)&acc[i], (unsigned long long)__double_as_longlong(t));
   }
-  __device__ void flush(ngpu_iter_out_t* out) {
+  // into partial row blockIdx % kStatCopies (InstNoAtHF has a single writer,
+  // so summing the rows reproduces it)
+  __device__ void flush(double* part) {
     __syncthreads();
-    double* g = (double*)out;
-    for (int i = threadIdx.x; i < S_NDOUBLE; i += blockDim.x) {
+    double* g = part + (size_t)(blockIdx.x % kStatCopies) * kStatStride;
+    for (int i = threadIdx.x; i < kStatStride; i += blockDim.x) {
       double v = acc[i];
       if (v == 0.0) continue;
-      bool is_max = (i == S_MAXSPAWN) || (i >= S_BLOOMS && i < S_BLOOMS + 3);
-      if (is_max)
+      if (stat_is_max(i))
         atomicMax((unsigned long long*)&g[i], (unsigned long long)__double_as_longlong(v));
-      else if (i >= soff(F_INSTHF, 0) && i < soff(F_INSTHF, kMaxRep))
-        g[i] = v;  // InstNoAtHF: a single det writes it
       else
         atomicAdd(&g[i], v);
     }
   }
 };
 
+// ------------------------------------------------------ LDS-staged appends
+// A block stages the records it appends to a global list in LDS and reserves
+// space in the list once per flush (one device-scope atomic for up to CAP
+// records).  push() is wave-wide; flush_begin()/flush_end() are block-wide and
+// must be reached by every thread of the block in every trip.
+struct Stage {
+  uint32_t n;                 // staged records
+  unsigned long long base;    // reserved position of the flush in progress
+};
+
+// index in the LDS stage for lanes with pred (-1 otherwise)
+__device__ inline int stage_push(bool pred, Stage* st) {
+  uint64_t m = __ballot(pred);
+  if (m == 0) return -1;
+  int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t b = 0;
+  if (lane_id() == leader) b = atomicAdd(&st->n, (uint32_t)popc64(m));
+  b = __shfl(b, leader);
+  return pred ? (int)(b + (uint32_t)popc64(m & lanemask_lt())) : -1;
+}
+
+// true when the block must flush now (force, or fewer than blockDim free
+// entries left); then cnt = staged records and base = their reserved position
+__device__ inline bool stage_flush_begin(Stage* st, unsigned long long* gctr, int cap, bool force,
+                                         uint32_t& cnt, unsigned long long& base) {
+  __syncthreads();
+  cnt = st->n;
+  __syncthreads();  // every thread has read n before the next trip pushes again
+  if (cnt == 0 || (!force && cnt + blockDim.x <= (uint32_t)cap)) return false;
+  if (threadIdx.x == 0) st->base = atomicAdd(gctr, (unsigned long long)cnt);
+  __syncthreads();
+  base = st->base;
+  return true;
+}
+__device__ inline void stage_flush_end(Stage* st) {
+  __syncthreads();
+  if (threadIdx.x == 0) st->n = 0;
+  __syncthreads();
+}
+__device__ inline void stage_init(Stage* st) {
+  if (threadIdx.x == 0) st->n = 0;
+  __syncthreads();
+}
+
+// Freed slots staged for the free queue (a ring of cap entries).
+constexpr int kFreeStage = 512;
+struct FreeStage {
+  Stage st;
+  int64_t slot[kFreeStage];
+};
+
 // RemoveHashDet (load_balancer.fpp:631-644) for the lanes with rm set: the
-// hash entry becomes a tombstone, the slot joins the free queue.  Wave-wide call.
+// hash entry becomes a tombstone, the slot is staged for the free queue.
+// Wave-wide call; the tombstone/live counts follow from the free-queue tail.
 template <int W>
-__device__ inline void remove_dets(const Walk& wk, Counters* cnt, bool rm, int64_t slot) {
-  long long q = wave_append(rm, &cnt->free_tail);
+__device__ inline void remove_dets(const Walk& wk, FreeStage* fs, bool rm, int64_t slot) {
+  int q = stage_push(rm, &fs->st);
   if (rm) {
     int64_t hp = wk.hpos[slot];
     int32_t* sl = (int32_t*)(wk.htab + (size_t)(hp >> 2) * 8 * W + 4 * W);
     sl[hp & 3] = kSlotTomb;
     wk.flags[slot] |= kFlagRemoved;
-    wk.freeq[(unsigned long long)q % (unsigned long long)wk.cap] = slot;
+    fs->slot[q] = slot;
   }
-  unsigned long long c = wave_count(rm);
-  if (c && lane_id() == 0) {
-    atomicAdd(&cnt->n_tomb, c);
-    atomicSub(&cnt->n_live, c);
-  }
+}
+// block-wide: drain the staged slots into the free queue when due
+__device__ inline void free_flush(const Walk& wk, Counters* cnt, FreeStage* fs, bool force) {
+  uint32_t n;
+  unsigned long long base;
+  if (!stage_flush_begin(&fs->st, &cnt->free_tail, kFreeStage, force, n, base)) return;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+    wk.freeq[(base + i) % (unsigned long long)wk.cap] = fs->slot[i];
+  stage_flush_end(&fs->st);
 }
 
 // hash lookup: returns slot or -1
@@ -216,6 +292,30 @@ __device__ inline void chunk_of(int64_t n, int64_t& begin, int64_t& end) {
   if (begin > n) begin = n;
 }
 
+// Per-lane statistic accumulator of one field: a register when the kernel is
+// built for one replica (the field index is then the same in every trip), the
+// block's LDS row otherwise.  add() and done() are wave-wide calls.
+template <int R>
+struct LaneSum {
+  double v = 0.0;
+  __device__ void add(BlockStats& bs, int i, double x) {
+    if constexpr (R == 1) v += x; else bs.add(i, x);
+  }
+  __device__ void done(BlockStats& bs, int i) {
+    if constexpr (R == 1) bs.add(i, v);
+  }
+};
+template <int R>
+struct LaneMax {
+  double v = 0.0;
+  __device__ void add(BlockStats& bs, int i, double x) {
+    if constexpr (R == 1) v = x > v ? x : v; else bs.max(i, x);
+  }
+  __device__ void done(BlockStats& bs, int i) {
+    if constexpr (R == 1) bs.max(i, v);
+  }
+};
+
 // block exclusive scan of one int64 per thread (blockDim = kBlock)
 __device__ inline int64_t block_exclusive_scan(int64_t v, int64_t* lds, int64_t& total) {
   int lane = lane_id(), wv = threadIdx.x / kWave;
@@ -254,19 +354,22 @@ __device__ inline int64_t grid_elem(int64_t t) {
 // sums of attempt counts feed the scan directly.
 template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters* cnt,
-                                                    ngpu_iter_out_t* out, int32_t* att_cnt,
+                                                    double* part, int32_t* att_cnt,
                                                     uint8_t* pinfo, int64_t* blk_part, Shift sft,
                                                     double tau, uint32_t iter) {
-  __shared__ double acc_lds[S_NDOUBLE];
+  __shared__ double acc_lds[kStatStride];
   __shared__ int64_t red[kBlock / kWave];
+  __shared__ FreeStage fs;
   BlockStats bs{acc_lds};
   bs.zero();
+  stage_init(&fs.st);
   const int S = R == 1 ? 1 : s.S;
   int64_t nslots = (int64_t)cnt->nslots;
   int64_t begin, end;
   chunk_of(nslots, begin, end);
   int64_t my_att = 0;
   bool death_err = false;
+  LaneSum<R> a_idets, a_iwalk, a_ndets, a_nwalk, a_hf, a_doubs, a_enum, a_died, a_born, a_abort;
   for (int64_t base = begin; base < end; base += blockDim.x) {
     int64_t slot = base + threadIdx.x;
     bool valid = slot < end;
@@ -299,15 +402,15 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
       }
       if (s.t_init) {
         bool ini = occ && now, non = occ && !now;
-        bs.add(soff(F_NOINITDETS, r), ini ? 1.0 : 0.0);
-        bs.add(soff(F_NOINITWALK, r), ini ? fabs(sg[r]) : 0.0);
-        bs.add(soff(F_NONINITDETS, r), non ? 1.0 : 0.0);
-        bs.add(soff(F_NONINITWALK, r), non ? fabs(sg[r]) : 0.0);
+        a_idets.add(bs, soff(F_NOINITDETS, r), ini ? 1.0 : 0.0);
+        a_iwalk.add(bs, soff(F_NOINITWALK, r), ini ? fabs(sg[r]) : 0.0);
+        a_ndets.add(bs, soff(F_NONINITDETS, r), non ? 1.0 : 0.0);
+        a_nwalk.add(bs, soff(F_NONINITWALK, r), non ? fabs(sg[r]) : 0.0);
       }
       // SumEContrib
-      bs.add(soff(F_HFCYC, r), exlevel == 0 ? sg[r] : 0.0);
-      bs.add(soff(F_NOATDOUBS, r), exlevel == 2 ? fabs(sg[r]) : 0.0);
-      bs.add(soff(F_ENUM, r), occ ? hoff * sg[r] : 0.0);
+      a_hf.add(bs, soff(F_HFCYC, r), exlevel == 0 ? sg[r] : 0.0);
+      a_doubs.add(bs, soff(F_NOATDOUBS, r), exlevel == 2 ? fabs(sg[r]) : 0.0);
+      a_enum.add(bs, soff(F_ENUM, r), occ ? hoff * sg[r] : 0.0);
       // decide_num_to_spawn and attempt_die share one Philox block
       double died = 0.0, born = 0.0;
       cp[r] = sg[r];
@@ -336,8 +439,8 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
       } else if (valid) {
         att_cnt[slot * S + r] = 0;
       }
-      bs.add(soff(F_NDIED, r), died);
-      bs.add(soff(F_NBORN, r), born);
+      a_died.add(bs, soff(F_NDIED, r), died);
+      a_born.add(bs, soff(F_NBORN, r), born);
     }
     bool abort_anti = occ && s.t_init && any_left;
     if (abort_anti) any_left = false;
@@ -350,20 +453,32 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
         }
         any_left = any_left || (fabs(cp[r]) > 1.0e-12);
       }
-      bs.add(soff(F_NABORTED, r), ab);
+      a_abort.add(bs, soff(F_NABORTED, r), ab);
     }
     bool rm = occ && !(any_left || core);
     if (occ) {
       for (int r = 0; r < S; ++r) wk.sgn[slot * S + r] = rm ? 0.0 : cp[r];
       wk.flags[slot] = f;
     }
-    remove_dets<W>(wk, cnt, rm, slot);
+    remove_dets<W>(wk, &fs, rm, slot);
+    free_flush(wk, cnt, &fs, false);
   }
+  free_flush(wk, cnt, &fs, true);
   if (death_err) atomicOr(&cnt->err, 1u << NGPU_ERR_DEATH_PROB);
+  a_idets.done(bs, soff(F_NOINITDETS, 0));
+  a_iwalk.done(bs, soff(F_NOINITWALK, 0));
+  a_ndets.done(bs, soff(F_NONINITDETS, 0));
+  a_nwalk.done(bs, soff(F_NONINITWALK, 0));
+  a_hf.done(bs, soff(F_HFCYC, 0));
+  a_doubs.done(bs, soff(F_NOATDOUBS, 0));
+  a_enum.done(bs, soff(F_ENUM, 0));
+  a_died.done(bs, soff(F_NDIED, 0));
+  a_born.done(bs, soff(F_NBORN, 0));
+  a_abort.done(bs, soff(F_NABORTED, 0));
   int64_t tot;
   (void)block_exclusive_scan(my_att, red, tot);
   if (threadIdx.x == 0) blk_part[blockIdx.x] = tot;
-  bs.flush(out);
+  bs.flush(part);
 }
 
 // exclusive scan of the per-block partials (single block)
@@ -416,14 +531,85 @@ __device__ inline int64_t parent_pair(const int64_t* att_off, int64_t npairs, in
   return lo;
 }
 
+// LDS stages of the spawned list and of the queued single-excitation attempts
+constexpr int kSpawnStage = 768;
+constexpr int kQueueStage = 512;
+template <int W>
+struct SpawnStage {
+  Stage st;
+  uint64_t key[kSpawnStage * W];
+  double val[kSpawnStage];
+  uint32_t info[kSpawnStage];
+};
+struct QueueStage {
+  Stage st;
+  int64_t a[kQueueStage];
+};
+
+// block-wide: drain staged spawns into the spawned list when due
+template <int W>
+__device__ inline void spawn_flush(SpawnStage<W>* ss, Counters* cnt, uint64_t* sp_key, double* sp_val,
+                                   uint32_t* sp_info, int64_t scap, bool force, bool& overflow) {
+  uint32_t n;
+  unsigned long long base;
+  if (!stage_flush_begin(&ss->st, &cnt->n_spawned, kSpawnStage, force, n, base)) return;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    unsigned long long d = base + i;
+    if (d < (unsigned long long)scap) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) sp_key[d * W + w] = ss->key[i * W + w];
+      sp_val[d] = ss->val[i];
+      sp_info[d] = ss->info[i];
+    } else {
+      overflow = true;
+    }
+  }
+  stage_flush_end(&ss->st);
+}
+
+__device__ inline void queue_flush(QueueStage* qs, Counters* cnt, int64_t* sq, int64_t scap, bool force,
+                                   bool& overflow) {
+  uint32_t n;
+  unsigned long long base;
+  if (!stage_flush_begin(&qs->st, &cnt->n_singles, kQueueStage, force, n, base)) return;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    unsigned long long d = base + i;
+    if (d < (unsigned long long)scap) sq[d] = qs->a[i];
+    else overflow = true;
+  }
+  stage_flush_end(&qs->st);
+}
+
+// statistics of the spawning kernels, per lane until the end of the kernel
+template <int R>
+struct SpawnAcc {
+  LaneSum<1> valid, invalid, bloomc1, bloomc2;
+  LaneMax<1> maxspawn, blooms1, blooms2;
+  LaneSum<R> born, sing;
+  __device__ void done(BlockStats& bs, int mode) {
+    valid.done(bs, S_NVALID);
+    invalid.done(bs, S_NINVALID);
+    maxspawn.done(bs, S_MAXSPAWN);
+    born.done(bs, soff(F_NBORN, 0));
+    if (mode != 1) {
+      sing.done(bs, soff(F_SPAWNSING, 0));
+      bloomc1.done(bs, S_BLOOMC + 1);
+      blooms1.done(bs, S_BLOOMS + 1);
+    }
+    if (mode != 2) {
+      bloomc2.done(bs, S_BLOOMC + 2);
+      blooms2.done(bs, S_BLOOMS + 2);
+    }
+  }
+};
+
 // one spawning attempt (attempt_create_normal + create_particle); mode 0: any
 // excitation, 1: doubles only (singles are queued), 2: singles only.  Wave-wide.
 template <int W, int R>
-__device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* cnt, BlockStats& bs,
+__device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, BlockStats& bs, SpawnAcc<R>& acc,
                                      const int64_t* att_off, const uint8_t* pinfo, int64_t npairs,
-                                     bool active, int64_t a, int mode, uint64_t* sp_key, double* sp_val,
-                                     uint32_t* sp_info, int64_t scap, int64_t* sq, double tau,
-                                     uint32_t iter, bool& overflow) {
+                                     bool active, int64_t a, int mode, SpawnStage<W>* ss, QueueStage* qs,
+                                     double tau, uint32_t iter) {
   const int S = R == 1 ? 1 : s.S;
   bool emit = false, queue = false;
   uint64_t child[W];
@@ -476,37 +662,30 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* 
       }
     }
   }
-  long long qi = wave_append(queue, &cnt->n_singles);
-  if (queue) {
-    if (qi < scap) sq[qi] = a;
-    else overflow = true;
-  }
-  bs.add(S_NINVALID, invalid);
-  bs.add(S_NVALID, valid);
-  bs.max(S_MAXSPAWN, raw);
+  int qi = qs ? stage_push(queue, &qs->st) : -1;
+  if (queue) qs->a[qi] = a;
+  acc.invalid.add(bs, S_NINVALID, invalid);
+  acc.valid.add(bs, S_NVALID, valid);
+  acc.maxspawn.add(bs, S_MAXSPAWN, raw);
   for (int rr = 0; rr < S; ++rr) {
-    bs.add(soff(F_NBORN, rr), rr == r ? an : 0.0);
-    if (mode != 1) bs.add(soff(F_SPAWNSING, rr), (rr == r && ic == 1) ? an : 0.0);
+    acc.born.add(bs, soff(F_NBORN, rr), rr == r ? an : 0.0);
+    if (mode != 1) acc.sing.add(bs, soff(F_SPAWNSING, rr), (rr == r && ic == 1) ? an : 0.0);
   }
   bool bloom = emit && an > s.init_walk_no;
   if (mode != 2) {
-    bs.add(S_BLOOMC + 2, (bloom && ic == 2) ? 1.0 : 0.0);
-    bs.max(S_BLOOMS + 2, (bloom && ic == 2) ? an : 0.0);
+    acc.bloomc2.add(bs, S_BLOOMC + 2, (bloom && ic == 2) ? 1.0 : 0.0);
+    acc.blooms2.add(bs, S_BLOOMS + 2, (bloom && ic == 2) ? an : 0.0);
   }
   if (mode != 1) {
-    bs.add(S_BLOOMC + 1, (bloom && ic == 1) ? 1.0 : 0.0);
-    bs.max(S_BLOOMS + 1, (bloom && ic == 1) ? an : 0.0);
+    acc.bloomc1.add(bs, S_BLOOMC + 1, (bloom && ic == 1) ? 1.0 : 0.0);
+    acc.blooms1.add(bs, S_BLOOMS + 1, (bloom && ic == 1) ? an : 0.0);
   }
-  long long idx = wave_append(emit, &cnt->n_spawned);
+  int idx = stage_push(emit, &ss->st);
   if (emit) {
-    if (idx < scap) {
 #pragma unroll
-      for (int w = 0; w < W; ++w) sp_key[idx * W + w] = child[w];
-      sp_val[idx] = nspawn;
-      sp_info[idx] = info;
-    } else {
-      overflow = true;
-    }
+    for (int w = 0; w < W; ++w) ss->key[idx * W + w] = child[w];
+    ss->val[idx] = nspawn;
+    ss->info[idx] = info;
   }
 }
 
@@ -514,50 +693,64 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* 
 // singles (the heat-bath list is ~20x longer per attempt) go to a queue so that
 // waves do not idle on a few long lanes
 template <int W, int R>
-__global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* cnt, ngpu_iter_out_t* out,
+__global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* cnt, double* part,
                                                   const int64_t* att_off, const uint8_t* pinfo,
                                                   uint64_t* sp_key, double* sp_val, uint32_t* sp_info,
                                                   int64_t scap, int64_t* sq, double tau, uint32_t iter) {
-  __shared__ double acc_lds[S_NDOUBLE];
+  __shared__ double acc_lds[kStatStride];
+  __shared__ SpawnStage<W> ss;
+  __shared__ QueueStage qs;
   BlockStats bs{acc_lds};
   bs.zero();
+  stage_init(&ss.st);
+  stage_init(&qs.st);
   const int64_t total = (int64_t)cnt->n_attempts;
   const int64_t npairs = (int64_t)cnt->nslots * s.S;
   bool overflow = false;
   int mode = s.model == 1 ? 1 : 0;
+  SpawnAcc<R> acc;
   int64_t trips = grid_trips(total);
   for (int64_t t = 0; t < trips; ++t) {
     int64_t a = grid_elem(t);
-    spawn_attempt<W, R>(s, wk, cnt, bs, att_off, pinfo, npairs, a < total, a, mode, sp_key, sp_val, sp_info,
-                     scap, sq, tau, iter, overflow);
+    spawn_attempt<W, R>(s, wk, bs, acc, att_off, pinfo, npairs, a < total, a, mode, &ss, &qs, tau, iter);
+    spawn_flush<W>(&ss, cnt, sp_key, sp_val, sp_info, scap, false, overflow);
+    if (mode == 1) queue_flush(&qs, cnt, sq, scap, false, overflow);
   }
+  spawn_flush<W>(&ss, cnt, sp_key, sp_val, sp_info, scap, true, overflow);
+  if (mode == 1) queue_flush(&qs, cnt, sq, scap, true, overflow);
   if (overflow) atomicOr(&cnt->err, 1u << NGPU_ERR_SPAWN_FULL);
-  bs.flush(out);
+  acc.done(bs, mode);
+  bs.flush(part);
 }
 
 template <int W, int R>
-__global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Counters* cnt, ngpu_iter_out_t* out,
+__global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Counters* cnt, double* part,
                                                           const int64_t* att_off, const uint8_t* pinfo,
                                                           uint64_t* sp_key, double* sp_val, uint32_t* sp_info,
                                                           int64_t scap, const int64_t* sq, double tau,
                                                           uint32_t iter) {
-  __shared__ double acc_lds[S_NDOUBLE];
+  __shared__ double acc_lds[kStatStride];
+  __shared__ SpawnStage<W> ss;
   BlockStats bs{acc_lds};
   bs.zero();
+  stage_init(&ss.st);
   int64_t nq = (int64_t)cnt->n_singles;
   if (nq > scap) nq = scap;
   const int64_t npairs = (int64_t)cnt->nslots * s.S;
   bool overflow = false;
+  SpawnAcc<R> acc;
   int64_t trips = grid_trips(nq);
   for (int64_t t = 0; t < trips; ++t) {
     int64_t q = grid_elem(t);
     bool act = q < nq;
     int64_t a = act ? sq[q] : 0;
-    spawn_attempt<W, R>(s, wk, cnt, bs, att_off, pinfo, npairs, act, a, 2, sp_key, sp_val, sp_info, scap,
-                     nullptr, tau, iter, overflow);
+    spawn_attempt<W, R>(s, wk, bs, acc, att_off, pinfo, npairs, act, a, 2, &ss, nullptr, tau, iter);
+    spawn_flush<W>(&ss, cnt, sp_key, sp_val, sp_info, scap, false, overflow);
   }
+  spawn_flush<W>(&ss, cnt, sp_key, sp_val, sp_info, scap, true, overflow);
   if (overflow) atomicOr(&cnt->err, 1u << NGPU_ERR_SPAWN_FULL);
-  bs.flush(out);
+  acc.done(bs, 2);
+  bs.flush(part);
 }
 
 // ---------------------------------------------------------- radix sort
@@ -669,18 +862,60 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* kin, c
 }
 
 // ------------------------------------------------------------- merging
+// LDS stage of a determinant list (key, S signs, flag): the merged spawns and
+// the new determinants
+template <int W, int R>
+struct ListStage {
+  static constexpr int kCap = R == 1 ? 768 : 256;
+  Stage st;
+  uint64_t key[kCap * W];
+  double sgn[kCap * R];
+  uint32_t flag[kCap];
+};
+
+template <int W, int R>
+__device__ inline int list_push(ListStage<W, R>* ls, bool pred, const uint64_t* key, const double* sg,
+                                uint32_t flag, int S) {
+  int i = stage_push(pred, &ls->st);
+  if (pred) {
+#pragma unroll
+    for (int w = 0; w < W; ++w) ls->key[i * W + w] = key[w];
+    for (int r = 0; r < S; ++r) ls->sgn[i * R + r] = sg[r];
+    ls->flag[i] = flag;
+  }
+  return i;
+}
+
+template <int W, int R>
+__device__ inline void list_flush(ListStage<W, R>* ls, unsigned long long* gctr, uint64_t* okey, double* osgn,
+                                  uint32_t* oflag, int S, bool force) {
+  uint32_t n;
+  unsigned long long base;
+  if (!stage_flush_begin(&ls->st, gctr, ListStage<W, R>::kCap, force, n, base)) return;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    unsigned long long d = base + i;
+#pragma unroll
+    for (int w = 0; w < W; ++w) okey[d * W + w] = ls->key[i * W + w];
+    for (int r = 0; r < S; ++r) osgn[d * S + r] = ls->sgn[i * R + r];
+    oflag[d] = ls->flag[i];
+  }
+  stage_flush_end(&ls->st);
+}
+
 // CompressSpawnedList (Annihilation.F90:249-515) + FindResidualParticle
 // (:551-634): the head of each run of equal keys sums the run in sorted
 // (= emission, the sort is stable) order.
 template <int W, int R>
-__global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, ngpu_iter_out_t* out,
+__global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, double* part,
                                                   const uint64_t* skey, const uint32_t* sidx,
                                                   const double* sp_val, const uint32_t* sp_info,
                                                   int64_t scap, uint64_t* u_key, double* u_sgn,
                                                   uint32_t* u_flag) {
-  __shared__ double acc_lds[S_NDOUBLE];
+  __shared__ double acc_lds[kStatStride];
+  __shared__ ListStage<W, R> ls;
   BlockStats bs{acc_lds};
   bs.zero();
+  stage_init(&ls.st);
   const int S = R == 1 ? 1 : s.S;
   int64_t n = (int64_t)cnt->n_spawned;
   if (n > scap) n = scap;
@@ -729,29 +964,29 @@ __global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, ngpu_
         }
       }
     }
-    long long u = wave_append(keep, &cnt->n_unique);
-    if (keep) {
-#pragma unroll
-      for (int w = 0; w < W; ++w) u_key[u * W + w] = key[w];
-      for (int r = 0; r < S; ++r) u_sgn[u * S + r] = sum[r];
-      u_flag[u] = flag;
-    }
+    list_push<W, R>(&ls, keep, key, sum, flag, S);
+    list_flush<W, R>(&ls, &cnt->n_unique, u_key, u_sgn, u_flag, S, false);
   }
+  list_flush<W, R>(&ls, &cnt->n_unique, u_key, u_sgn, u_flag, S, true);
   for (int r = 0; r < S; ++r) bs.add(soff(F_NANNIHIL, r), ann[r]);
-  bs.flush(out);
+  bs.flush(part);
 }
 
 // ---------------------------------------------------------- annihilation
 // AnnihilateSpawnedParts (Annihilation.F90:965-1352): found dets are updated
 // in place (each unique key is owned by one lane); absent dets go to the new list.
 template <int W, int R>
-__global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counters* cnt, ngpu_iter_out_t* out,
+__global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counters* cnt, double* part,
                                                        const uint64_t* u_key, const double* u_sgn,
                                                        const uint32_t* u_flag, uint64_t* n_key,
                                                        double* n_sgn, uint32_t* n_flag, uint32_t iter) {
-  __shared__ double acc_lds[S_NDOUBLE];
+  __shared__ double acc_lds[kStatStride];
+  __shared__ ListStage<W, R> ls;
+  __shared__ FreeStage fs;
   BlockStats bs{acc_lds};
   bs.zero();
+  stage_init(&ls.st);
+  stage_init(&fs.st);
   const int S = R == 1 ? 1 : s.S;
   int64_t nu = (int64_t)cnt->n_unique;
   int64_t trips = grid_trips(nu);
@@ -828,22 +1063,20 @@ __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counte
         }
       }
     }
-    remove_dets<W>(wk, cnt, rm, pos);
-    long long q = wave_append(isnew, &cnt->n_new);
-    if (isnew) {
-#pragma unroll
-      for (int w = 0; w < W; ++w) n_key[q * W + w] = key[w];
-      for (int r = 0; r < S; ++r) n_sgn[q * S + r] = ss[r];
-      n_flag[q] = sflag & 0xFFFFu;
-    }
+    remove_dets<W>(wk, &fs, rm, pos);
+    list_push<W, R>(&ls, isnew, key, ss, sflag & 0xFFFFu, S);
+    free_flush(wk, cnt, &fs, false);
+    list_flush<W, R>(&ls, &cnt->n_new, n_key, n_sgn, n_flag, S, false);
   }
+  free_flush(wk, cnt, &fs, true);
+  list_flush<W, R>(&ls, &cnt->n_new, n_key, n_sgn, n_flag, S, true);
   for (int r = 0; r < S; ++r) {
     bs.add(soff(F_NABORTED, r), st_ab[r]);
     bs.add(soff(F_NANNIHIL, r), st_an[r]);
     bs.add(soff(F_NREMOVED, r), st_rm[r]);
     bs.add(soff(F_NBORN, r), st_bn[r]);
   }
-  bs.flush(out);
+  bs.flush(part);
 }
 
 // AddNewHashDet (load_balancer.fpp:514-629) for the new list
@@ -853,60 +1086,62 @@ __global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* 
                                                    const unsigned long long* n_ptr) {
   const int S = R == 1 ? 1 : s.S;
   int64_t nn = (int64_t)*n_ptr;
-  unsigned long long tail = cnt->free_tail;
-  int64_t trips = grid_trips(nn);
-  for (int64_t t = 0; t < trips; ++t) {
-    int64_t i = grid_elem(t);
-    bool act = i < nn;
-    long long q = wave_append(act, &cnt->free_head);
-    bool fresh = act && (unsigned long long)q >= tail;
-    long long ns = wave_append(fresh, &cnt->nslots);
-    int64_t slot = -1;
-    bool ok = false;
-    if (act) {
-      if (!fresh) slot = wk.freeq[(unsigned long long)q % (unsigned long long)wk.cap];
-      else if (ns < wk.cap) slot = ns;
-      else atomicOr(&cnt->err, 1u << NGPU_ERR_WALKERS_FULL);
+  // new det i takes the i-th queued free slot, past the queue a fresh slot
+  // at the end of the table: positions follow from i, no atomics
+  const unsigned long long head = cnt->free_head;
+  const int64_t avail = (int64_t)(cnt->free_tail - head);
+  const int64_t ns0 = (int64_t)cnt->nslots;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t slot = i < avail ? wk.freeq[(head + (unsigned long long)i) % (unsigned long long)wk.cap]
+                             : ns0 + (i - avail);
+    if (slot >= wk.cap) {
+      atomicOr(&cnt->err, 1u << NGPU_ERR_WALKERS_FULL);
+      atomicAdd(&cnt->n_fail, 1ull);
+      continue;
     }
-    if (slot >= 0) {
-      uint64_t key[W];
+    uint64_t key[W];
 #pragma unroll
-      for (int w = 0; w < W; ++w) key[w] = n_key[i * W + w];
-      int64_t home = find_walker_hash<W>(key, s.rh, s.L) - 1;
-      int64_t hp = hash_insert<W>(wk, key, home, (int32_t)slot);
-      if (hp < 0) {
-        atomicOr(&cnt->err, 1u << NGPU_ERR_HASH_FULL);
-        wk.flags[slot] = kFlagRemoved;
-      } else {
+    for (int w = 0; w < W; ++w) key[w] = n_key[i * W + w];
+    int64_t home = find_walker_hash<W>(key, s.rh, s.L) - 1;
+    int64_t hp = hash_insert<W>(wk, key, home, (int32_t)slot);
+    if (hp < 0) {
+      atomicOr(&cnt->err, 1u << NGPU_ERR_HASH_FULL);
+      atomicAdd(&cnt->n_fail, 1ull);
+      wk.flags[slot] = kFlagRemoved;
+    } else {
 #pragma unroll
-        for (int w = 0; w < W; ++w) wk.keys[slot * W + w] = key[w];
-        for (int r = 0; r < S; ++r) wk.sgn[slot * S + r] = n_sgn[i * S + r];
-        wk.flags[slot] = n_flag[i] & ~kFlagRemoved;
-        wk.diagH[slot] = diag_helement<W>(s, key) - s.hii;
-        wk.offdiagH[slot] = offdiag_helement<W>(s, key);
-        wk.hpos[slot] = hp;
-        ok = true;
-      }
+      for (int w = 0; w < W; ++w) wk.keys[slot * W + w] = key[w];
+      for (int r = 0; r < S; ++r) wk.sgn[slot * S + r] = n_sgn[i * S + r];
+      wk.flags[slot] = n_flag[i] & ~kFlagRemoved;
+      wk.diagH[slot] = diag_helement<W>(s, key) - s.hii;
+      wk.offdiagH[slot] = offdiag_helement<W>(s, key);
+      wk.hpos[slot] = hp;
     }
-    unsigned long long c = wave_count(ok);
-    if (c && lane_id() == 0) atomicAdd(&cnt->n_live, c);
   }
 }
 
-// free-queue bookkeeping after inserts: pops beyond the tail appended instead;
-// appends beyond the capacity were refused
-__global__ void k_post_insert(Counters* cnt, int64_t cap) {
-  if (cnt->free_head > cnt->free_tail) cnt->free_head = cnt->free_tail;
-  if (cnt->nslots > (unsigned long long)cap) cnt->nslots = (unsigned long long)cap;
+// bookkeeping of k_insert: queued slots used, fresh slots appended (beyond
+// the capacity refused), live entries added
+__global__ void k_post_insert(Counters* cnt, const unsigned long long* n_ptr, int64_t cap) {
+  unsigned long long nn = *n_ptr;
+  unsigned long long avail = cnt->free_tail - cnt->free_head;
+  unsigned long long used = nn < avail ? nn : avail;
+  cnt->free_head += used;
+  unsigned long long ns = cnt->nslots + (nn - used);
+  cnt->nslots = ns < (unsigned long long)cap ? ns : (unsigned long long)cap;
+  cnt->n_live += nn - cnt->n_fail;
 }
 
 // CalcHashTableStats (load_balancer.fpp:646-805)
 template <int W, int R>
-__global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* cnt, ngpu_iter_out_t* out,
+__global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* cnt, double* part,
                                                   uint32_t iter) {
-  __shared__ double acc_lds[S_NDOUBLE];
+  __shared__ double acc_lds[kStatStride];
+  __shared__ FreeStage fs;
   BlockStats bs{acc_lds};
   bs.zero();
+  stage_init(&fs.st);
   const int S = R == 1 ? 1 : s.S;
   int64_t nslots = (int64_t)cnt->nslots;
   unsigned long long my_occ = 0;
@@ -963,17 +1198,18 @@ __global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* c
         }
       }
     }
-    remove_dets<W>(wk, cnt, rm, slot);
+    remove_dets<W>(wk, &fs, rm, slot);
+    free_flush(wk, cnt, &fs, false);
   }
+  free_flush(wk, cnt, &fs, true);
   for (int r = 0; r < S; ++r) {
     bs.add(soff(F_TOTPARTS, r), tp[r]);
     bs.add(soff(F_NORMSQ, r), nq[r]);
     bs.add(soff(F_NREMOVED, r), rmv[r]);
     bs.add(soff(F_NBORN, r), bn[r]);
   }
-  unsigned long long c = (unsigned long long)wave_sum((double)my_occ);
-  if (c && lane_id() == 0) atomicAdd(&cnt->n_occ, c);
-  bs.flush(out);
+  bs.add(X_NOCC, (double)my_occ);
+  bs.flush(part);
 }
 
 // ---------------------------------------------------------- hash upkeep
@@ -983,6 +1219,11 @@ __global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* c
 // entries is a capacity error (MEMORYFACHASH too small), reported instead of
 // letting probe sequences grow without bound.
 __global__ void k_hash_check(Counters* cnt, int64_t entries) {
+  // every removal of the iteration left a tombstone and queued its slot
+  unsigned long long removed = cnt->free_tail - cnt->free_tail0;
+  cnt->n_live -= removed;
+  cnt->n_tomb += removed;
+  cnt->free_tail0 = cnt->free_tail;
   unsigned long long used = cnt->n_live + cnt->n_tomb;
   cnt->rebuild = (used * 10ull > (unsigned long long)entries * 6ull &&
                   cnt->n_tomb * 10ull > (unsigned long long)entries) ? 1u : 0u;
