@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """FCIQMC walker-iterations/s on one MI355X.
 
-Workload (config 1 of the measurement plan): synthetic C1 molecular
+Workload: synthetic C1 molecular
 Hamiltonian, 10 electrons in 20 spatial orbitals (seed 7, Cholesky rank 60),
 initiator FCIQMC (threshold 3), PCHB doubles (UNIF-UNIF particles, FAST-FAST
 spin-orbital-resolved holes) + on-the-fly heat-bath singles with pSingles 0.1,
@@ -23,38 +23,104 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "walker-iterations/s (spawn+annihilate+death) on 1 GPU; batched vs plain FP64"
-STAGES = ["prepass", "scan", "spawn", "sort", "merge", "annihilate", "insert", "stats"]
+STAGES = ["prepass", "scan", "spawn", "spawn_singles", "sort", "merge", "annihilate", "insert", "stats"]
+KERNELS = {"prepass": "k_prepass", "scan": "k_scan_apply", "spawn": "k_spawn", "spawn_singles": "k_spawn_singles",
+           "sort": "k_radix_scatter", "merge": "k_merge", "annihilate": "k_annihilate", "insert": "k_insert",
+           "stats": "k_stats"}
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(stage, c, W=1, S=1, nel=10, nsp=20, psingles=0.1, nbasis=40):
-    """Minimum compulsory bytes of one launch of a stage for per-iteration counts c.
-
-    N_d occupied dets, N_a attempts, N_s spawns, N_u unique spawned dets,
-    N_new inserted dets.  Records: key 8W, sign 8S, flags 4."""
-    K = 8 * W
+def algorithmic_bytes(stage, c, W=1, S=1, nbasis=40, psingles=0.1):
+    """Compulsory HBM bytes of one iteration's launch(es) of a stage: every input
+    read once and every output written once, for the per-iteration counts c
+    (Nd occupied dets, Na attempts, Ns spawns, Nu unique spawned dets, Nn new
+    dets).  Records: key 8W, signs 8S, flags 4.  The integral, alias and
+    neighbour tables (under 1 MB here) stay in L2 and count once, i.e. zero.
+    Queued single attempts and their share of the spawns are their expected
+    values psingles*Na and psingles*Ns."""
+    K, Sg = 8 * W, 8 * S
     Nd, Na, Ns, Nu, Nn = c["Nd"], c["Na"], c["Ns"], c["Nu"], c["Nn"]
-    if stage == "prepass":   # read key, signs, flags, diagH, offdiagH; write signs, flags, counts, pinfo
-        return Nd * (K + 8 * S + 4 + 16) + Nd * (8 * S + 4 + 4 * S + S)
-    if stage == "scan":
+    Nq, Nsq = psingles * Na, psingles * Ns
+    if stage == "prepass":     # read key, signs, flags, diagH, offdiagH; write signs, flags, counts, pinfo
+        return Nd * (K + Sg + 4 + 16) + Nd * (Sg + 4 + 4 * S + S)
+    if stage == "scan":        # read counts, write offsets
         return Nd * S * (4 + 8)
-    if stage == "spawn":
-        nvirt = nsp - nel // 2
-        single = nvirt * (8 + 2 * nel * 8) + (8 + 2 * nel * 8)   # heat-bath list + H_ij gathers
-        double = 16 + 8 + 8 + 16                                 # AB alias entry, prob, tgtOrbs, 2 UMAT gathers
-        per_attempt = K + 8 + 1 + psingles * single + (1 - psingles) * double
-        return Na * per_attempt + Ns * (K + 8 + 4)
-    if stage == "sort":      # one read+write pass of (key, index)
-        return 2 * Ns * (K + 4)
-    if stage == "merge":
-        return Ns * (K + 4 + 8 + 4) + Nu * (K + 8 * S + 4)
-    if stage == "annihilate":
-        return Nu * (K + 8 * S + 4) + Nu * 64 + Nu * 8 * S + Nn * (K + 8 * S + 4)
-    if stage == "insert":
-        return Nn * (K + 8 * S + 4) + Nn * (K + 8 * S + 4 + 24) + Nn * 64
-    if stage == "stats":
-        return Nd * (K + 8 * S + 4)
-    return 0
+    if stage == "spawn":       # parents once (key, pinfo, offset); queue (16 B) and double spawns written
+        return Nd * (K + S + 8 * S) + Nq * 16 + (Ns - Nsq) * (K + 12)
+    if stage == "spawn_singles":  # queue entry, parent key/pinfo/offset, single spawns written
+        return Nq * (16 + K + 1 + 8) + Nsq * (K + 12)
+    if stage == "sort":        # per 8-bit pass: histogram read of keys, scatter read+write of (key, index)
+        passes = (nbasis + 7) // 8 if W == 1 else (nbasis - 64 + 7) // 8 + 8
+        return passes * Ns * (K + 2 * (K + 4))
+    if stage == "merge":       # sorted keys + index, gathered value + info; unique list written
+        return Ns * (K + 4 + 8 + 4) + Nu * (K + Sg + 4)
+    if stage == "annihilate":  # unique list, one hash bucket (64W B), sign+flag read/write; new list written
+        return Nu * (K + Sg + 4) + Nu * 64 * W + Nu * (2 * Sg + 4) + Nn * (K + Sg + 4)
+    if stage == "insert":      # new list, free-queue entry, bucket line, walker row (key, signs, flags, 3x8)
+        return Nn * (K + Sg + 4) + Nn * 8 + Nn * 64 * W + Nn * (K + Sg + 4 + 24)
+    if stage == "stats":       # flags, signs, key per slot
+        return Nd * (4 + Sg + K)
+    return 0.0
+
+
+def _kernel_short(name):
+    """'void ngpu::k_spawn<1, 1>(...)' -> 'k_spawn<1, 1>'."""
+    n = name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").strip()
+    return n.split("::")[-1]
+
+
+def summarize_trace(path, last, per_iter_kernel="k_end_iter"):
+    """Steady-state table from a rocprofv3 kernel-trace CSV: per kernel, the
+    dispatches of the last `last` iterations (an iteration ends with one
+    dispatch of per_iter_kernel), average and per-iteration duration."""
+    import csv
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    ends = [i for i, r in enumerate(rows) if per_iter_kernel in r["Kernel_Name"]]
+    if len(ends) <= last:
+        raise SystemExit(f"only {len(ends)} iterations in {path}")
+    window = rows[ends[-last - 1] + 1: ends[-1] + 1]
+    agg = {}
+    for r in window:
+        k = _kernel_short(r["Kernel_Name"])
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        a = agg.setdefault(k, [0, 0.0])
+        a[0] += 1
+        a[1] += d
+    total = sum(v[1] for v in agg.values()) / last
+    print(f"| kernel | dispatches / iteration | average (us) | per iteration (us) | share |")
+    print("|---|---|---|---|---|")
+    for k, (n, t) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        print(f"| `{k}` | {n / last:g} | {t / n:.1f} | {t / last:.1f} | {100 * t / last / total:.1f} % |")
+    print(f"| all kernels | | | {total:.1f} | 100 % |")
+
+
+def summarize_pmc(path, last_dispatches):
+    """Average counter value per dispatch of each kernel over its last
+    `last_dispatches` dispatches, from a rocprofv3 counter-collection CSV."""
+    import csv
+    rows = list(csv.DictReader(open(path)))
+    per = {}
+    for r in rows:
+        key = (_kernel_short(r["Kernel_Name"]), r["Counter_Name"])
+        per.setdefault(key, []).append((int(r["Dispatch_Id"]), float(r["Counter_Value"])))
+    print("| kernel | counter | dispatches | average per dispatch |")
+    print("|---|---|---|---|")
+    for (k, c), v in sorted(per.items(), key=lambda kv: -sum(x[1] for x in kv[1][-last_dispatches:])):
+        v.sort()
+        tail = [x[1] for x in v[-last_dispatches:]]
+        print(f"| `{k}` | {c} | {len(tail)} | {sum(tail) / len(tail):.1f} |")
+
+
+def copy_calibration(nbytes, reps):
+    """A plain device copy of known size, for scaling FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
 def build_workload(args):
@@ -108,7 +174,21 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-growth-iters", type=int, default=6000)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run a device copy of this many bytes (PMC traffic scale)")
+    ap.add_argument("--summarize-trace", default=None, help="rocprofv3 kernel-trace CSV to summarise")
+    ap.add_argument("--summarize-pmc", default=None, help="rocprofv3 counter-collection CSV to summarise")
+    ap.add_argument("--last", type=int, default=60, help="iterations (or dispatches) summarised")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, 5)
+        return
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace, args.last)
+        return
+    if args.summarize_pmc:
+        summarize_pmc(args.summarize_pmc, args.last)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one GPU per process (one rank's walkers); "
                                    "multi-GPU runs are not part of this build", "n_gpus": args.gpus}))
@@ -149,10 +229,10 @@ def main():
                   Nn=float(sum(r["new"] for r in rows)) / K)
     per_stage = {STAGES[i]: ms[i] / max(nprof, 1) for i in range(len(STAGES))}
     dom = max(per_stage, key=per_stage.get)
-    b = algorithmic_bytes(dom, counts, psingles=spec.psingles)
+    b = algorithmic_bytes(dom, counts, W=eng.W, S=eng.S, nbasis=spec.nbasis, psingles=spec.psingles)
     achieved = b / (per_stage[dom] * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}_config1.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{KERNELS[dom]}_mol10e20o.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -167,9 +247,9 @@ def main():
                    "attempts_per_step": counts["Na"], "spawns_per_step": counts["Ns"],
                    "growth_iterations": drv.iter - K - args.warmup},
         "stage_ms_per_step": per_stage,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": b},
+                     "algorithmic_bytes_per_launch": b, "launch_ms": per_stage[dom]},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

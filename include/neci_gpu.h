@@ -157,7 +157,8 @@ int ngpu_get_cycle_stats(ngpu_t *h, ngpu_iter_out_t *sum, ngpu_iter_out_t *last,
 
 /* Stage timing with events on the engine stream (set_timer/halt_timer analogue,
    lib/timing.F90:122-188).  Stages: 0 prepass(initiator, proj-E, death),
-   1 attempt scan, 2 spawn, 3 sort, 4 merge, 5 annihilate, 6 insert, 7 stats. */
+   1 attempt scan, 2 spawn (all attempts; molecular: doubles, singles queued),
+   3 queued singles, 4 sort, 5 merge, 6 annihilate, 7 insert, 8 stats. */
 int ngpu_set_profiling(ngpu_t *h, int32_t on);
 int ngpu_get_stage_times(ngpu_t *h, double *ms_total, int32_t nstages, int64_t *n_iters);
 

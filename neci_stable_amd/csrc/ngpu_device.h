@@ -215,7 +215,9 @@ __host__ __device__ inline int spat(int orb) { return (orb + 1) >> 1; }
 
 struct AliasEnt {  // one alias-table entry (aliasSampling.F90 bias/alias pair)
   double bias;
-  int32_t alias;  // 0-based
+  double prob_self;   // probs(i) of this entry
+  double prob_alias;  // probs(alias(i)): a draw reads one 32-byte entry
+  int32_t alias;      // 0-based
   int32_t pad;
 };
 
@@ -242,6 +244,12 @@ struct DevSys {
   const double* j_prob;
   const uint8_t* j_ok;
   const int32_t* ab_tgt;       // tgtOrbs: ab_tgt[2*(ab-1)] = B <= A = ab_tgt[2*(ab-1)+1]
+  // heat-bath singles: (U(a,j,b,j), U(a,j,j,b)) at jk[2*(((a-1)*nsp + j-1)*nsp + b-1)]
+  // for spatial a, j, b; copies of UMAT entries, rows contiguous in b
+  const double* jk;
+  // diagonal elements: (U(i,j,i,j), U(i,j,j,i)) at jk0[2*((i-1)*nsp + j-1)]
+  const double* jk0;
+  int nsp;
 };
 
 __device__ __host__ inline double U(const DevSys& s, int i, int j, int k, int l) {
@@ -283,7 +291,7 @@ __device__ __host__ inline double sltcnd_0(const DevSys& s, const uint64_t* d) {
           int oj = 64 * wj + ctz64(xj) + 1;
           xj &= xj - 1;
           int idj = spat(oj);
-          t += U(s, idi, idj, idi, idj);
+          t += s.jk0[2 * ((idi - 1) * s.nsp + (idj - 1))];  // U(idi,idj,idi,idj)
           any = true;
         }
       }
@@ -296,13 +304,15 @@ __device__ __host__ inline double sltcnd_0(const DevSys& s, const uint64_t* d) {
     while (xi) {
       int oi = 64 * wi + ctz64(xi) + 1;
       xi &= xi - 1;
+      // exchange only between equal spins: bits of the other spin are masked
+      uint64_t same = is_beta(oi) ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
 #pragma unroll
       for (int wj = wi; wj < W; ++wj) {
-        uint64_t xj = (wj == wi) ? xi : d[wj];
+        uint64_t xj = ((wj == wi) ? xi : d[wj]) & same;
         while (xj) {
           int oj = 64 * wj + ctz64(xj) + 1;
           xj &= xj - 1;
-          if (is_beta(oi) == is_beta(oj)) hel_tmp -= U(s, spat(oi), spat(oj), spat(oj), spat(oi));
+          hel_tmp -= s.jk0[2 * ((spat(oi) - 1) * s.nsp + (spat(oj) - 1)) + 1];  // U(i,j,j,i)
         }
       }
     }
@@ -324,7 +334,7 @@ __device__ __host__ inline double sltcnd_1(const DevSys& s, const uint64_t* d, i
         x &= x - 1;
         if (o != src) {
           int id = spat(o);
-          hel += U(s, a, id, b, id);
+          hel += s.jk[2 * (((size_t)(a - 1) * s.nsp + (id - 1)) * s.nsp + (b - 1))];  // U(a,id,b,id)
         }
       }
     }
@@ -337,7 +347,7 @@ __device__ __host__ inline double sltcnd_1(const DevSys& s, const uint64_t* d, i
         x &= x - 1;
         if (o != src) {
           int id = spat(o);
-          hel -= U(s, a, id, id, b);
+          hel -= s.jk[2 * (((size_t)(a - 1) * s.nsp + (id - 1)) * s.nsp + (b - 1)) + 1];  // U(a,id,id,b)
         }
       }
     }
@@ -425,17 +435,21 @@ struct Excit {
   int ic;          // 0 = null
   int e0, e1, e2, e3;
   bool parity;
+  bool hel_ok;     // hel holds the unsigned-parity matrix element
   double pgen;
+  double hel;
 };
 
 // sample_AliasTable_t (aliasSampling.F90:310-332), 0-based
-__device__ __host__ inline int alias_sample(const AliasEnt* tab, int n, double r) {
+__device__ __host__ inline int alias_sample(const AliasEnt* tab, int n, double r, double& prob) {
   double x = (double)n * r;
   int pos = (int)x + 1;
   double b = x + 1.0 - (double)pos;
   if (b < 0.0) b = 0.0;
   AliasEnt e = tab[pos - 1];
-  return (b < e.bias) ? pos - 1 : e.alias;
+  bool self = b < e.bias;
+  prob = self ? e.prob_self : e.prob_alias;
+  return self ? pos - 1 : e.alias;
 }
 
 __device__ __host__ inline int fuse_index(int x, int y) {
@@ -504,12 +518,12 @@ __device__ __host__ inline Excit gen_single_heatbath(const DevSys& s, const uint
   }
   if (fabs(cum_sum) <= kEps) return ex;
   double r = dorb * cum_sum;
-  double cum = 0.0, cpt = 0.0;
+  double cum = 0.0, cpt = 0.0, v = 0.0;
   int orb = 0;
   for (int i = 1; i <= nsp; ++i) {
     orb = 2 * i - off;
-    cpt = 0.0;
-    if (!test_orb<W>(d, orb)) cpt = fabs(sltcnd_1<W>(s, d, src, orb));
+    v = test_orb<W>(d, orb) ? 0.0 : sltcnd_1<W>(s, d, src, orb);
+    cpt = fabs(v);
     cum += cpt;
     if (!(cum < r) || i == nsp) break;
   }
@@ -521,6 +535,8 @@ __device__ __host__ inline Excit gen_single_heatbath(const DevSys& s, const uint
   ex.e1 = orb;
   ex.parity = single_parity<W>(d, src, orb);
   ex.pgen = pg;
+  ex.hel = v;  // = sltcnd_1(src, orb), the element the spawn needs
+  ex.hel_ok = true;
   return ex;
 }
 
@@ -576,8 +592,8 @@ __device__ __host__ inline Excit gen_double_pchb(const DevSys& s, const uint64_t
     s1 = select_orb<W>(d, ~0ull, e1);
     if (!s.j_ok[s1 - 1]) return ex;
     const AliasEnt* row = s.j_tab + (size_t)(s1 - 1) * n;
-    int j = alias_sample(row, n, dr[2]);
-    double p_J_I = s.j_prob[(size_t)(s1 - 1) * n + j];
+    double p_J_I;
+    int j = alias_sample(row, n, dr[2], p_J_I);
     s2 = j + 1;
     if (!test_orb<W>(d, s2)) return ex;
     double p_I_J = s.j_ok[s2 - 1] ? s.j_prob[(size_t)(s2 - 1) * n + (s1 - 1)] : 0.0;
@@ -590,8 +606,8 @@ __device__ __host__ inline Excit gen_double_pchb(const DevSys& s, const uint64_t
   int ij = fuse_index(s1, s2);
   if (!s.ab_ok[ij - 1]) return ex;
   const AliasEnt* row = s.ab_tab + (size_t)(ij - 1) * s.ijmax;
-  int ab = alias_sample(row, s.ijmax, dr[3]) + 1;
-  double pgh = s.ab_prob[(size_t)(ij - 1) * s.ijmax + ab - 1];
+  double pgh;
+  int ab = alias_sample(row, s.ijmax, dr[3], pgh) + 1;
   int t1 = s.ab_tgt[2 * (ab - 1)], t2 = s.ab_tgt[2 * (ab - 1) + 1];
   if (t1 == t2 || test_orb<W>(d, t1) || test_orb<W>(d, t2)) return ex;
   if (fabs(pgh) <= kEps) return ex;

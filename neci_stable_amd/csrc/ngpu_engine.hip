@@ -1,6 +1,7 @@
 // ngpu_engine.hip -- engine state, set-up and the C ABI declared in
 // include/neci_gpu.h.  One engine = one rank's walkers on one GPU.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -13,7 +14,7 @@ using namespace ngpu;
 
 namespace {
 
-constexpr int kStages = 8;
+constexpr int kStages = 9;
 constexpr int kEvRing = 32;  // iterations whose stage events may be in flight
 
 #define HIP_OK(x)                                                        \
@@ -58,6 +59,8 @@ struct ngpu_engine {
   int64_t* att_off = nullptr;
   uint8_t* pinfo = nullptr;
   int64_t* blk_part = nullptr;
+  int64_t* tile_p = nullptr;  // parent pair of each spawn tile's first attempt
+  int64_t tcap = 0;
   // spawns, sort, merge, new
   int64_t scap = 0;
   uint64_t* sp_key = nullptr;
@@ -67,6 +70,11 @@ struct ngpu_engine {
   uint32_t* si[2] = {nullptr, nullptr};
   uint32_t* hist = nullptr;
   uint32_t* digit_tot = nullptr;
+  // sort by 32-bit colex rank when every determinant has one
+  bool rank_sort = false;
+  int rank_bits = 0;
+  uint32_t* d_binom = nullptr;  // C(p, i) at [p*(nel+1)+i]
+  uint32_t* rk = nullptr;       // ranks of the spawned list
   int64_t* sq = nullptr;  // queued single-excitation attempts
   uint64_t* u_key = nullptr;
   double* u_sgn = nullptr;
@@ -79,6 +87,8 @@ struct ngpu_engine {
   int32_t* d_nbr = nullptr;
   double* d_tmat = nullptr;
   double* d_umat = nullptr;
+  double* d_jk = nullptr;
+  double* d_jk0 = nullptr;
   AliasEnt* d_ab_tab = nullptr;
   double* d_ab_prob = nullptr;
   uint8_t* d_ab_ok = nullptr;
@@ -90,7 +100,8 @@ struct ngpu_engine {
   bool pending = false;
   // profiling
   bool prof = false;
-  hipEvent_t ev[kEvRing][kStages + 1];
+  hipEvent_t ev[kEvRing][kStages][2];  // start of a stage's first kernel, end of its last
+  uint32_t ev_mask[kEvRing] = {0};      // stages whose both events were recorded
   bool ev_pending[kEvRing] = {false};
   int ev_cur = 0;
   int64_t ev_next = 0;
@@ -186,23 +197,63 @@ __global__ void k_reset_cycle(ngpu_iter_out_t* cyc) {
 
 __global__ void k_set_count(unsigned long long* p, unsigned long long v) { *p = v; }
 
-void rec(ngpu_t* h, int stage) {
-  if (h->prof && h->ev_ok) (void)hipEventRecord(h->ev[h->ev_cur][stage], h->stream);
+// Kernel launch on the engine stream.  With profiling on, the launch that
+// opens stage `open` carries that stage's start event and the launch that
+// closes stage `close` its stop event (hipExtLaunchKernel records them at the
+// kernel's own start and end, not at a neighbouring command).
+template <typename F, typename... Args>
+void launch(ngpu_t* h, int open, int close, F kern, dim3 grid, dim3 block, Args... args) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->prof && h->ev_ok) {
+    if (open >= 0) e0 = h->ev[h->ev_cur][open][0];
+    if (close >= 0) {
+      e1 = h->ev[h->ev_cur][close][1];
+      h->ev_mask[h->ev_cur] |= 1u << close;
+    }
+  }
+  hipExtLaunchKernelGGL(kern, grid, block, 0, h->stream, e0, e1, 0, args...);
 }
 
 // tombstone reclamation and the load guard, launched unconditionally (the
 // kernels exit at once unless the device-side check asks for a rebuild)
 template <int W>
 void hash_upkeep(ngpu_t* h) {
-  hipLaunchKernelGGL(k_hash_check, dim3(1), dim3(1), 0, h->stream, h->cnt, h->wk.L * (int64_t)kBucketEntries);
-  hipLaunchKernelGGL(k_hash_clear<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->wk, h->cnt);
-  hipLaunchKernelGGL(k_hash_refill<W>, dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt);
-  hipLaunchKernelGGL(k_hash_done, dim3(1), dim3(1), 0, h->stream, h->cnt);
+  launch(h, -1, -1, k_hash_check, dim3(1), dim3(1), h->cnt, h->wk.L * (int64_t)kBucketEntries);
+  launch(h, -1, -1, k_hash_clear<W>, dim3(h->nb_grid), dim3(kBlock), h->wk, (const Counters*)h->cnt);
+  launch(h, -1, -1, k_hash_refill<W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt);
+  launch(h, -1, -1, k_hash_done, dim3(1), dim3(1), h->cnt);
 }
 
-// radix sort of the spawned keys; returns the buffer index holding the result
-template <int W>
-int radix_sort(ngpu_t* h, const unsigned long long* n_ptr) {
+// sort of the spawned list (stage 4) then the run-head merge (stage 5)
+template <int W, int R>
+void sort_and_merge(ngpu_t* h) {
+  const unsigned long long* n_ptr = &h->cnt->n_spawned;
+  if (h->rank_sort) {
+    launch(h, 4, -1, k_rank_hist<W>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_key, n_ptr, h->scap,
+           (const uint32_t*)h->d_binom, h->sys.nel + 1, h->rk, h->hist);
+    const int passes = (h->rank_bits + 7) / 8;
+    const uint32_t* kin = h->rk;
+    const uint32_t* iin = nullptr;
+    int cur = 0;
+    for (int p = 0; p < passes; ++p) {
+      int out = p & 1;
+      if (p > 0)
+        launch(h, -1, -1, k_radix_hist<uint32_t, 1, false>, dim3(h->nb_chunk), dim3(kBlock), kin, n_ptr, h->scap,
+               0, 8 * p, h->hist);
+      launch(h, -1, -1, k_radix_scan, dim3(kRadixBins), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
+      launch(h, -1, p + 1 == passes ? 4 : -1, k_radix_scatter<uint32_t, 1, false>, dim3(h->nb_chunk), dim3(kBlock),
+             kin, iin, (uint32_t*)h->sk[out], h->si[out], n_ptr, h->scap, 0, 8 * p, (const uint32_t*)h->hist,
+             (const uint32_t*)h->digit_tot);
+      kin = (const uint32_t*)h->sk[out];
+      iin = h->si[out];
+      cur = out;
+    }
+    launch(h, 5, 5, k_merge<W, R, uint32_t, 1>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->cnt, h->dpart,
+           (const uint32_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_key,
+           (const double*)h->sp_val, (const uint32_t*)h->sp_info, h->scap, h->u_key, h->u_sgn, h->u_flag);
+    return;
+  }
+  // determinant keys in ilut_lt order: least significant word first
   int nbits_hi = W == 1 ? h->sys.nbasis : 64;      // bits used in word 0
   int nbits_lo = W == 2 ? h->sys.nbasis - 64 : 0;  // bits used in word 1
   struct Pass { int word, shift; };
@@ -215,74 +266,71 @@ int radix_sort(ngpu_t* h, const unsigned long long* n_ptr) {
   const uint32_t* iin = nullptr;
   for (size_t p = 0; p < passes.size(); ++p) {
     int out = (int)(p & 1);
-    hipLaunchKernelGGL(k_radix_hist<W>, dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, kin, n_ptr, h->scap,
-                       passes[p].word, passes[p].shift, h->hist);
-    hipLaunchKernelGGL(k_radix_scan, dim3(kRadixBins), dim3(kBlock), 0, h->stream, h->hist, h->nb_chunk,
-                       h->digit_tot);
-    hipLaunchKernelGGL(k_radix_scatter<W>, dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, kin, iin,
-                       h->sk[out], h->si[out], n_ptr, h->scap, passes[p].word, passes[p].shift, h->hist,
-                       h->digit_tot);
+    bool first = p == 0, last = p + 1 == passes.size();
+    launch(h, first ? 4 : -1, -1, k_radix_hist<uint64_t, W, W == 2>, dim3(h->nb_chunk), dim3(kBlock), kin, n_ptr,
+           h->scap, passes[p].word, passes[p].shift, h->hist);
+    launch(h, -1, -1, k_radix_scan, dim3(kRadixBins), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
+    launch(h, -1, last ? 4 : -1, k_radix_scatter<uint64_t, W, W == 2>, dim3(h->nb_chunk), dim3(kBlock), kin, iin,
+           h->sk[out], h->si[out], n_ptr, h->scap, passes[p].word, passes[p].shift, (const uint32_t*)h->hist,
+           (const uint32_t*)h->digit_tot);
     kin = h->sk[out];
     iin = h->si[out];
     cur = out;
   }
-  return cur;
+  launch(h, 5, 5, k_merge<W, R, uint64_t, W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->cnt, h->dpart,
+         (const uint64_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_key,
+         (const double*)h->sp_val, (const uint32_t*)h->sp_info, h->scap, h->u_key, h->u_sgn, h->u_flag);
 }
 
 // merge -> annihilate -> insert -> stats, shared by iterate and the explicit-list entry
 template <int W, int R>
 void annihilation_chain(ngpu_t* h, uint32_t iter) {
-  rec(h, 3);
-  int cur = radix_sort<W>(h, &h->cnt->n_spawned);
-  rec(h, 4);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_merge<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->cnt, h->dpart,
-                     h->sk[cur], h->si[cur], h->sp_val, h->sp_info, h->scap, h->u_key, h->u_sgn, h->u_flag);
-  rec(h, 5);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_annihilate<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
-                     h->dpart, h->u_key, h->u_sgn, h->u_flag, h->n_key, h->n_sgn, h->n_flag, iter);
-  rec(h, 6);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_insert<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
-                     h->n_key, h->n_sgn, h->n_flag, &h->cnt->n_new);
-  hipLaunchKernelGGL(k_post_insert, dim3(1), dim3(1), 0, h->stream, h->cnt, &h->cnt->n_new, h->wk.cap);
-  rec(h, 7);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_stats<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->dpart,
-                     iter);
+  sort_and_merge<W, R>(h);
+  launch(h, 6, 6, k_annihilate<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
+         (const uint64_t*)h->u_key, (const double*)h->u_sgn, (const uint32_t*)h->u_flag, h->n_key, h->n_sgn,
+         h->n_flag, iter);
+  launch(h, 7, -1, k_insert<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt,
+         (const uint64_t*)h->n_key, (const double*)h->n_sgn, (const uint32_t*)h->n_flag,
+         (const unsigned long long*)&h->cnt->n_new);
+  launch(h, -1, 7, k_post_insert, dim3(1), dim3(1), h->cnt, (const unsigned long long*)&h->cnt->n_new,
+         h->wk.cap);
+  launch(h, 8, -1, k_stats<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart, iter);
   hash_upkeep<W>(h);
-  hipLaunchKernelGGL(k_end_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dpart, h->dcyc, h->cnt, h->scap);
-  rec(h, 8);
+  launch(h, -1, 8, k_end_iter, dim3(1), dim3(kBlock), h->dout, (const double*)h->dpart, h->dcyc, h->cnt,
+         h->scap);
 }
 
 template <int W, int R>
 void iterate_chain(ngpu_t* h, const ngpu_iter_in_t* in) {
   Shift sft;
   for (int r = 0; r < kMaxRep; ++r) sft.v[r] = r < h->S ? in->diag_sft[r] : 0.0;
-  rec(h, 0);
-  hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dpart, h->cnt);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_prepass<W, R>), dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
-                     h->dpart, h->att_cnt, h->pinfo, h->blk_part, sft, in->tau, in->iter);
-  rec(h, 1);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, h->stream, h->blk_part, h->nb_chunk, h->cnt);
-  hipLaunchKernelGGL(k_scan_apply, dim3(h->nb_chunk), dim3(kBlock), 0, h->stream, h->att_cnt, h->att_off,
-                     h->blk_part, h->cnt, h->S);
-  rec(h, 2);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spawn<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt, h->dpart,
-                     h->att_off, h->pinfo, h->sp_key, h->sp_val, h->sp_info, h->scap, h->sq, in->tau, in->iter);
+  launch(h, -1, -1, k_begin_iter, dim3(1), dim3(kBlock), h->dout, h->dpart, h->cnt);
+  launch(h, 0, 0, k_prepass<W, R>, dim3(h->nb_chunk), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart, h->att_cnt,
+         h->pinfo, h->blk_part, sft, in->tau, in->iter);
+  launch(h, 1, -1, k_scan_partials, dim3(1), dim3(kBlock), h->blk_part, h->nb_chunk, h->cnt);
+  launch(h, -1, 1, k_scan_apply, dim3(h->nb_chunk), dim3(kBlock), (const int32_t*)h->att_cnt, h->att_off,
+         (const int64_t*)h->blk_part, h->cnt, h->S, h->tile_p, h->tcap);
+  launch(h, 2, 2, k_spawn<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
+         (const int64_t*)h->att_off, (const int64_t*)h->tile_p, h->tcap, (const uint8_t*)h->pinfo, h->sp_key,
+         h->sp_val, h->sp_info, h->scap, h->sq, in->tau, in->iter);
   if (h->sys.model == NGPU_MODEL_MOLECULAR)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_spawn_singles<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
-                       h->dpart, h->att_off, h->pinfo, h->sp_key, h->sp_val, h->sp_info, h->scap, h->sq, in->tau,
-                       in->iter);
+    launch(h, 3, 3, k_spawn_singles<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
+           (const uint8_t*)h->pinfo, h->sp_key, h->sp_val, h->sp_info, h->scap, (const int64_t*)h->sq, in->tau,
+           in->iter);
   annihilation_chain<W, R>(h, in->iter);
 }
 
 void harvest(ngpu_t* h, int slot) {
   if (!h->ev_pending[slot]) return;
-  (void)hipEventSynchronize(h->ev[slot][kStages]);
   for (int s = 0; s < kStages; ++s) {
+    if (!(h->ev_mask[slot] & (1u << s))) continue;
+    (void)hipEventSynchronize(h->ev[slot][s][1]);
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, h->ev[slot][s], h->ev[slot][s + 1]) == hipSuccess) h->stage_ms[s] += ms;
+    if (hipEventElapsedTime(&ms, h->ev[slot][s][0], h->ev[slot][s][1]) == hipSuccess) h->stage_ms[s] += ms;
   }
   h->prof_iters++;
   h->ev_pending[slot] = false;
+  h->ev_mask[slot] = 0;
 }
 
 int finish(ngpu_t* h, ngpu_iter_out_t* out) {
@@ -318,11 +366,13 @@ void build_alias(const std::vector<double>& w, AliasEnt* tab, double* prob) {
   }
   for (int i : over) { bias[i] = 1.0; alias[i] = i; }
   for (int i : under) { bias[i] = 1.0; alias[i] = i; }
+  for (int i = 0; i < n; ++i) prob[i] = w[i] / sum;
   for (int i = 0; i < n; ++i) {
     tab[i].bias = bias[i];
+    tab[i].prob_self = prob[i];
+    tab[i].prob_alias = prob[alias[i]];
     tab[i].alias = alias[i];
     tab[i].pad = 0;
-    prob[i] = w[i] / sum;
   }
 }
 
@@ -536,6 +586,10 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
   fail(dalloc(&h->att_off, (size_t)cap * S + 1));
   fail(dalloc(&h->pinfo, (size_t)cap * S));
   fail(dalloc(&h->blk_part, (size_t)h->nb_chunk));
+  // tiles of attempts with a recorded first parent: attempts up to 4x the
+  // spawn capacity (beyond it the spawn kernel searches the offsets itself)
+  h->tcap = (4 * scap) / kBlock + 2;
+  fail(dalloc(&h->tile_p, (size_t)h->tcap));
   fail(dalloc(&h->sp_key, (size_t)scap * W));
   fail(dalloc(&h->sp_val, (size_t)scap));
   fail(dalloc(&h->sp_info, (size_t)scap));
@@ -544,8 +598,33 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
     fail(dalloc(&h->si[q], (size_t)scap));
   }
   fail(dalloc(&h->hist, (size_t)h->nb_chunk * kRadixBins));
+  {
+    // C(nbasis, nel) decides the sort key; the table holds C(p, i) for the
+    // colex rank of a determinant (p < nbasis, i <= nel)
+    const int nb = sys->nbasis, ne = sys->nel;
+    std::vector<unsigned __int128> c((size_t)(nb + 1) * (ne + 1), 0);
+    for (int p = 0; p <= nb; ++p) {
+      c[(size_t)p * (ne + 1)] = 1;
+      for (int i = 1; i <= ne && i <= p; ++i) {
+        unsigned __int128 v = c[(size_t)(p - 1) * (ne + 1) + i - 1] + c[(size_t)(p - 1) * (ne + 1) + i];
+        c[(size_t)p * (ne + 1) + i] = v > ((unsigned __int128)1 << 64) ? ((unsigned __int128)1 << 64) : v;
+      }
+    }
+    unsigned __int128 total = c[(size_t)nb * (ne + 1) + ne];
+    if (total <= ((unsigned __int128)1 << 32)) {
+      h->rank_sort = true;
+      while (h->rank_bits < 32 && ((unsigned __int128)1 << h->rank_bits) < total) ++h->rank_bits;
+      if (h->rank_bits == 0) h->rank_bits = 1;
+      std::vector<uint32_t> b((size_t)nb * (ne + 1));
+      for (int p = 0; p < nb; ++p)
+        for (int i = 0; i <= ne; ++i) b[(size_t)p * (ne + 1) + i] = (uint32_t)c[(size_t)p * (ne + 1) + i];
+      fail(dalloc(&h->d_binom, b.size()));
+      if (rc == NGPU_OK) fail(hipMemcpy(h->d_binom, b.data(), b.size() * 4, hipMemcpyHostToDevice));
+      fail(dalloc(&h->rk, (size_t)scap));
+    }
+  }
   fail(dalloc(&h->digit_tot, (size_t)kRadixBins));
-  fail(dalloc(&h->sq, (size_t)scap));
+  fail(dalloc(&h->sq, (size_t)scap * 2));  // (parent pair, attempt number)
   fail(dalloc(&h->u_key, (size_t)scap * W));
   fail(dalloc(&h->u_sgn, (size_t)scap * S));
   fail(dalloc(&h->u_flag, (size_t)scap));
@@ -586,8 +665,9 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
   if (rc == NGPU_OK) {
     h->ev_ok = true;
     for (int q = 0; q < kEvRing; ++q)
-      for (int s = 0; s <= kStages; ++s)
-        if (hipEventCreate(&h->ev[q][s]) != hipSuccess) h->ev_ok = false;
+      for (int s = 0; s < kStages; ++s)
+        for (int e = 0; e < 2; ++e)
+          if (hipEventCreate(&h->ev[q][s][e]) != hipSuccess) h->ev_ok = false;
   }
   if (rc != NGPU_OK) {
     ngpu_destroy(h);
@@ -603,7 +683,7 @@ int ngpu_destroy(ngpu_t* h) {
   void* ptrs[] = {h->wk.keys, h->wk.sgn, h->wk.flags, h->wk.diagH, h->wk.offdiagH, h->wk.hpos, h->wk.freeq,
                   h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->sp_key, h->sp_val, h->sp_info,
                   h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->digit_tot, h->sq, h->u_key, h->u_sgn, h->u_flag, h->n_key,
-                  h->n_sgn, h->n_flag, h->cnt, h->dout, h->dpart, h->dcyc, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_ab_tab,
+                  h->n_sgn, h->n_flag, h->cnt, h->dout, h->dpart, h->dcyc, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_binom, h->rk, h->tile_p, h->d_ab_tab,
                   h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -611,7 +691,8 @@ int ngpu_destroy(ngpu_t* h) {
   if (h->hcnt) (void)hipHostFree(h->hcnt);
   if (h->ev_ok)
     for (int q = 0; q < kEvRing; ++q)
-      for (int s = 0; s <= kStages; ++s) (void)hipEventDestroy(h->ev[q][s]);
+      for (int s = 0; s < kStages; ++s)
+        for (int e = 0; e < 2; ++e) (void)hipEventDestroy(h->ev[q][s][e]);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return NGPU_OK;
@@ -631,8 +712,38 @@ int ngpu_set_integrals(ngpu_t* h, const double* tmat2d, const double* umat, int6
   HIP_OK(dalloc(&h->d_umat, (size_t)umat_len));
   HIP_OK(hipMemcpy(h->d_tmat, tmat2d, nt * 8, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(h->d_umat, umat, (size_t)umat_len * 8, hipMemcpyHostToDevice));
+  {
+    // (Coulomb, exchange) rows of the heat-bath single-excitation weights
+    std::vector<double> jk((size_t)nsp * nsp * nsp * 2);
+    for (int a = 1; a <= nsp; ++a)
+      for (int j = 1; j <= nsp; ++j)
+        for (int b = 1; b <= nsp; ++b) {
+          size_t e = 2 * (((size_t)(a - 1) * nsp + (j - 1)) * nsp + (b - 1));
+          jk[e] = umat[umat_ind(a, j, b, j) - 1];
+          jk[e + 1] = umat[umat_ind(a, j, j, b) - 1];
+        }
+    if (h->d_jk) (void)hipFree(h->d_jk);
+    h->d_jk = nullptr;
+    HIP_OK(dalloc(&h->d_jk, jk.size()));
+    HIP_OK(hipMemcpy(h->d_jk, jk.data(), jk.size() * 8, hipMemcpyHostToDevice));
+    // (J, K) pairs of the diagonal elements
+    std::vector<double> jk0((size_t)nsp * nsp * 2);
+    for (int i = 1; i <= nsp; ++i)
+      for (int j = 1; j <= nsp; ++j) {
+        size_t e = 2 * ((size_t)(i - 1) * nsp + (j - 1));
+        jk0[e] = umat[umat_ind(i, j, i, j) - 1];
+        jk0[e + 1] = umat[umat_ind(i, j, j, i) - 1];
+      }
+    if (h->d_jk0) (void)hipFree(h->d_jk0);
+    h->d_jk0 = nullptr;
+    HIP_OK(dalloc(&h->d_jk0, jk0.size()));
+    HIP_OK(hipMemcpy(h->d_jk0, jk0.data(), jk0.size() * 8, hipMemcpyHostToDevice));
+  }
   h->ds.tmat = h->d_tmat;
   h->ds.umat = h->d_umat;
+  h->ds.jk = h->d_jk;
+  h->ds.jk0 = h->d_jk0;
+  h->ds.nsp = (int)nsp;
   h->ds.ecore = ecore;
   h->have_model = true;
   return NGPU_OK;
@@ -858,13 +969,12 @@ int ngpu_iterate(ngpu_t* h, const ngpu_iter_in_t* in, ngpu_iter_out_t* out) {
   int rc = ready(h);
   if (rc) return rc;
   if (!in || !(in->tau > 0.0)) return NGPU_ERR_ARG;
-  if (h->pending) {
-    rc = finish(h, nullptr);
-    if (rc) return rc;
-  }
+  // iterations queue on the stream back to back: nothing in the chain waits
+  // for the host; errors surface at the next synchronising call
   if (h->prof && h->ev_ok) {
     h->ev_cur = (int)(h->ev_next % kEvRing);
     harvest(h, h->ev_cur);
+    h->ev_mask[h->ev_cur] = 0;
   }
   if (h->W == 1) h->S == 1 ? iterate_chain<1, 1>(h, in) : iterate_chain<1, kMaxRep>(h, in);
   else h->S == 1 ? iterate_chain<2, 1>(h, in) : iterate_chain<2, kMaxRep>(h, in);

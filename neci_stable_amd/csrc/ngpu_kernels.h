@@ -60,6 +60,7 @@ struct alignas(256) Counters {
   unsigned long long n_live, n_tomb, n_occ;
   unsigned long long n_attempts;
   unsigned long long n_fail;      // inserts refused (capacity errors)
+  unsigned long long n_reused;    // inserts that took a tombstone's entry
   unsigned int err;
   unsigned int rebuild;           // set when tombstones must be reclaimed
   alignas(256) unsigned long long free_tail;
@@ -258,9 +259,11 @@ __device__ inline int64_t hash_lookup(const Walk& wk, const uint64_t* key, int64
   return -1;
 }
 
-// bucketed open-addressing insert of an absent key; returns hash position or -1
+// bucketed open-addressing insert of an absent key; returns hash position or
+// -1; reused is set when the entry claimed was a tombstone
 template <int W>
-__device__ inline int64_t hash_insert(const Walk& wk, const uint64_t* key, int64_t home, int32_t slot) {
+__device__ inline int64_t hash_insert(const Walk& wk, const uint64_t* key, int64_t home, int32_t slot,
+                                      bool& reused) {
   int64_t b = home;
   int64_t maxp = wk.L < kMaxProbe ? wk.L : kMaxProbe;
   for (int64_t probe = 0; probe < maxp; ++probe) {
@@ -272,6 +275,7 @@ __device__ inline int64_t hash_insert(const Walk& wk, const uint64_t* key, int64
       while (cur == kSlotEmpty || cur == kSlotTomb) {
         int32_t prev = atomicCAS(&sl[e], cur, slot);
         if (prev == cur) {
+          reused = cur == kSlotTomb;
 #pragma unroll
           for (int w = 0; w < W; ++w) bk[e * W + w] = key[w];
           return b * 4 + e;
@@ -500,9 +504,12 @@ __global__ void __launch_bounds__(kBlock) k_scan_partials(int64_t* part, int npa
   if (threadIdx.x == 0) cnt->n_attempts = (unsigned long long)tot;
 }
 
-// attempt offsets per (slot, replica) pair, same chunking as k_prepass
+// attempt offsets per (slot, replica) pair, same chunking as k_prepass; also
+// the parent pair of the first attempt of every kBlock-attempt tile (the
+// spawn kernel's tiles), written by the pair whose attempts contain it
 __global__ void __launch_bounds__(kBlock) k_scan_apply(const int32_t* att_cnt, int64_t* att_off,
-                                                       const int64_t* part, Counters* cnt, int S) {
+                                                       const int64_t* part, Counters* cnt, int S,
+                                                       int64_t* tile_p, int64_t tcap) {
   __shared__ int64_t red[kBlock / kWave];
   int64_t nslots = (int64_t)cnt->nslots;
   int64_t begin, end;
@@ -514,16 +521,20 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(const int32_t* att_cnt, i
     int64_t v = i < pe ? att_cnt[i] : 0;
     int64_t tot;
     int64_t pre = block_exclusive_scan(v, red, tot);
-    if (i < pe) att_off[i] = run + pre;
+    if (i < pe) {
+      int64_t o = run + pre;
+      att_off[i] = o;
+      for (int64_t t = (o + kBlock - 1) / kBlock; t * kBlock < o + v && t < tcap; ++t) tile_p[t] = i;
+    }
     run += tot;
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) att_off[nslots * S] = (int64_t)cnt->n_attempts;
 }
 
 // ------------------------------------------------------------- spawning
-// parent (slot, replica) of attempt a: last pair p with att_off[p] <= a
-__device__ inline int64_t parent_pair(const int64_t* att_off, int64_t npairs, int64_t a) {
-  int64_t lo = 0, hi = npairs - 1;
+// parent (slot, replica) of attempt a: last pair p in [lo, hi] with
+// att_off[p] <= a (the caller's bounds bracket the answer)
+__device__ inline int64_t parent_pair(const int64_t* att_off, int64_t lo, int64_t hi, int64_t a) {
   while (lo < hi) {
     int64_t mid = (lo + hi + 1) >> 1;
     if (att_off[mid] <= a) lo = mid; else hi = mid - 1;
@@ -533,6 +544,7 @@ __device__ inline int64_t parent_pair(const int64_t* att_off, int64_t npairs, in
 
 // LDS stages of the spawned list and of the queued single-excitation attempts
 constexpr int kSpawnStage = 768;
+constexpr int kParentWindow = 512;  // attempt offsets of one tile's parents staged in LDS
 constexpr int kQueueStage = 512;
 template <int W>
 struct SpawnStage {
@@ -541,9 +553,10 @@ struct SpawnStage {
   double val[kSpawnStage];
   uint32_t info[kSpawnStage];
 };
-struct QueueStage {
+struct QueueStage {  // queued attempts as (parent pair, attempt number)
   Stage st;
-  int64_t a[kQueueStage];
+  int64_t p[kQueueStage];
+  uint32_t k[kQueueStage];
 };
 
 // block-wide: drain staged spawns into the spawned list when due
@@ -574,8 +587,12 @@ __device__ inline void queue_flush(QueueStage* qs, Counters* cnt, int64_t* sq, i
   if (!stage_flush_begin(&qs->st, &cnt->n_singles, kQueueStage, force, n, base)) return;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
     unsigned long long d = base + i;
-    if (d < (unsigned long long)scap) sq[d] = qs->a[i];
-    else overflow = true;
+    if (d < (unsigned long long)scap) {
+      sq[2 * d] = qs->p[i];
+      sq[2 * d + 1] = (int64_t)qs->k[i];
+    } else {
+      overflow = true;
+    }
   }
   stage_flush_end(&qs->st);
 }
@@ -607,9 +624,8 @@ struct SpawnAcc {
 // excitation, 1: doubles only (singles are queued), 2: singles only.  Wave-wide.
 template <int W, int R>
 __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, BlockStats& bs, SpawnAcc<R>& acc,
-                                     const int64_t* att_off, const uint8_t* pinfo, int64_t npairs,
-                                     bool active, int64_t a, int mode, SpawnStage<W>* ss, QueueStage* qs,
-                                     double tau, uint32_t iter) {
+                                     const uint8_t* pinfo, bool active, int64_t p, uint32_t k, int mode,
+                                     SpawnStage<W>* ss, QueueStage* qs, double tau, uint32_t iter) {
   const int S = R == 1 ? 1 : s.S;
   bool emit = false, queue = false;
   uint64_t child[W];
@@ -618,10 +634,8 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, BlockStats
   int r = 0, ic = 0;
   double valid = 0.0, invalid = 0.0;
   if (active) {
-    int64_t p = parent_pair(att_off, npairs, a);
     int64_t slot = p / S;
     r = (int)(p - slot * S);
-    uint32_t k = (uint32_t)(a - att_off[p]);
     uint64_t key[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
@@ -648,7 +662,8 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, BlockStats
       } else {
         valid = 1.0;
         ic = ex.ic;
-        double hel = helement_ex<W>(s, key, ex.ic, ex.e0, ex.e1, ex.e2, ex.e3, ex.parity);
+        double hel = ex.hel_ok ? (ex.parity ? -ex.hel : ex.hel)
+                               : helement_ex<W>(s, key, ex.ic, ex.e0, ex.e1, ex.e2, ex.e3, ex.parity);
         uint8_t pi = pinfo[p];
         double wsign = (pi & 1) ? -1.0 : 1.0;
         nspawn = attempt_create(s, tau, hel, ex.pgen, wsign, dr[5], raw);
@@ -663,7 +678,10 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, BlockStats
     }
   }
   int qi = qs ? stage_push(queue, &qs->st) : -1;
-  if (queue) qs->a[qi] = a;
+  if (queue) {
+    qs->p[qi] = p;
+    qs->k[qi] = k;
+  }
   acc.invalid.add(bs, S_NINVALID, invalid);
   acc.valid.add(bs, S_NVALID, valid);
   acc.maxspawn.add(bs, S_MAXSPAWN, raw);
@@ -694,9 +712,10 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, BlockStats
 // waves do not idle on a few long lanes
 template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* cnt, double* part,
-                                                  const int64_t* att_off, const uint8_t* pinfo,
-                                                  uint64_t* sp_key, double* sp_val, uint32_t* sp_info,
-                                                  int64_t scap, int64_t* sq, double tau, uint32_t iter) {
+                                                  const int64_t* att_off, const int64_t* tile_p, int64_t tcap,
+                                                  const uint8_t* pinfo, uint64_t* sp_key, double* sp_val,
+                                                  uint32_t* sp_info, int64_t scap, int64_t* sq, double tau,
+                                                  uint32_t iter) {
   __shared__ double acc_lds[kStatStride];
   __shared__ SpawnStage<W> ss;
   __shared__ QueueStage qs;
@@ -709,10 +728,54 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
   bool overflow = false;
   int mode = s.model == 1 ? 1 : 0;
   SpawnAcc<R> acc;
+  __shared__ int64_t pb[2];
+  __shared__ int64_t woff[kParentWindow];
   int64_t trips = grid_trips(total);
   for (int64_t t = 0; t < trips; ++t) {
     int64_t a = grid_elem(t);
-    spawn_attempt<W, R>(s, wk, bs, acc, att_off, pinfo, npairs, a < total, a, mode, &ss, &qs, tau, iter);
+    // the block's attempts are the consecutive tile a0 .. a0+kBlock-1: its
+    // parents lie between the tile's first parent and the next tile's
+    int64_t a0 = grid_elem(t) - threadIdx.x;
+    int64_t tile = a0 / kBlock;
+    if (threadIdx.x == 0) {
+      int64_t lo = 0, hi = 0;
+      if (a0 < total) {
+        bool more = a0 + kBlock < total;
+        if (tile + 1 < tcap) {
+          lo = tile_p[tile];
+          hi = more ? tile_p[tile + 1] : npairs - 1;
+        } else {  // beyond the tile table: full searches
+          int64_t a1 = (more ? a0 + kBlock : total) - 1;
+          lo = parent_pair(att_off, 0, npairs - 1, a0);
+          hi = parent_pair(att_off, lo, npairs - 1, a1);
+        }
+      }
+      pb[0] = lo;
+      pb[1] = hi;
+    }
+    __syncthreads();
+    const int64_t lo = pb[0], hi = pb[1];
+    const int64_t nw = hi - lo + 1;
+    bool act = a < total;
+    int64_t p = 0;
+    uint32_t k = 0;
+    if (a0 < total && nw <= kParentWindow) {  // block-uniform: search the window in LDS
+      for (int64_t i = threadIdx.x; i < nw; i += blockDim.x) woff[i] = att_off[lo + i];
+      __syncthreads();
+      if (act) {
+        int64_t l = 0, h = nw - 1;
+        while (l < h) {
+          int64_t mid = (l + h + 1) >> 1;
+          if (woff[mid] <= a) l = mid; else h = mid - 1;
+        }
+        p = lo + l;
+        k = (uint32_t)(a - woff[l]);
+      }
+    } else if (act) {
+      p = parent_pair(att_off, lo, hi, a);
+      k = (uint32_t)(a - att_off[p]);
+    }
+    spawn_attempt<W, R>(s, wk, bs, acc, pinfo, act, p, k, mode, &ss, &qs, tau, iter);
     spawn_flush<W>(&ss, cnt, sp_key, sp_val, sp_info, scap, false, overflow);
     if (mode == 1) queue_flush(&qs, cnt, sq, scap, false, overflow);
   }
@@ -723,28 +786,172 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
   bs.flush(part);
 }
 
+// Queued single excitations (molecular, heat-bath singles), kHbGroup lanes per
+// attempt.  gen_single_heatbath's two sequential sweeps over the same-spin
+// orbitals b need |H(src -> b)| for every b; lane g of a group evaluates the
+// orbitals b = g+1, g+1+kHbGroup, ... from the (Coulomb, exchange) rows
+// jk[a][j][*], so the group's loads for one occupied orbital j are one
+// contiguous 16*kHbGroup-byte run instead of kHbGroup scattered UMAT entries.
+// Each value is summed in sltcnd_1's order (Coulomb over occupied orbitals,
+// then exchange over same-spin ones, then the one-body term), and both sweeps
+// run in ascending b on every lane with values fetched by shuffles, so the
+// selected orbital, pgen and H_ij are bit-identical to the one-lane generator.
+constexpr int kHbGroup = 16;
+constexpr int kHbPerLane = 4;  // spatial orbitals per lane: nsp <= 64
+
 template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Counters* cnt, double* part,
-                                                          const int64_t* att_off, const uint8_t* pinfo,
-                                                          uint64_t* sp_key, double* sp_val, uint32_t* sp_info,
-                                                          int64_t scap, const int64_t* sq, double tau,
-                                                          uint32_t iter) {
+                                                          const uint8_t* pinfo, uint64_t* sp_key, double* sp_val,
+                                                          uint32_t* sp_info, int64_t scap, const int64_t* sq,
+                                                          double tau, uint32_t iter) {
   __shared__ double acc_lds[kStatStride];
   __shared__ SpawnStage<W> ss;
   BlockStats bs{acc_lds};
   bs.zero();
   stage_init(&ss.st);
+  const int S = R == 1 ? 1 : s.S;
   int64_t nq = (int64_t)cnt->n_singles;
   if (nq > scap) nq = scap;
-  const int64_t npairs = (int64_t)cnt->nslots * s.S;
   bool overflow = false;
   SpawnAcc<R> acc;
-  int64_t trips = grid_trips(nq);
+  const int nsp = s.nbasis / 2;
+  const int g = threadIdx.x % kHbGroup;
+  const int gbase = lane_id() - lane_id() % kHbGroup;
+  const bool leader = g == 0;
+  const int64_t groups = (int64_t)gridDim.x * (blockDim.x / kHbGroup);
+  const int64_t grp = (int64_t)blockIdx.x * (blockDim.x / kHbGroup) + threadIdx.x / kHbGroup;
+  const int64_t trips = (nq + groups - 1) / groups;
   for (int64_t t = 0; t < trips; ++t) {
-    int64_t q = grid_elem(t);
+    int64_t q = t * groups + grp;
     bool act = q < nq;
-    int64_t a = act ? sq[q] : 0;
-    spawn_attempt<W, R>(s, wk, bs, acc, att_off, pinfo, npairs, act, a, 2, &ss, nullptr, tau, iter);
+    int64_t p = act ? sq[2 * q] : 0;
+    uint32_t k = act ? (uint32_t)sq[2 * q + 1] : 0u;
+    int64_t slot = p / S;
+    int r = (int)(p - slot * S);
+    uint64_t key[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) key[w] = act ? wk.keys[slot * W + w] : s.ref[w];  // idle groups: any valid det
+    double d0, d1, d2, d3, d4, d5;
+    draw2<W>(s.seed, key, iter, r, k, 1, d0, d1);
+    draw2<W>(s.seed, key, iter, r, k, 2, d2, d3);
+    draw2<W>(s.seed, key, iter, r, k, 3, d4, d5);
+    (void)d0; (void)d3; (void)d4;
+    int elec = 1 + (int)floor(d1 * (double)s.nel);
+    int src = select_orb<W>(key, ~0ull, elec);
+    int a = spat(src), off = is_beta(src) ? 1 : 0;
+    const double* row = s.jk + (size_t)2 * (a - 1) * nsp * nsp;
+    double v[kHbPerLane];
+#pragma unroll
+    for (int m = 0; m < kHbPerLane; ++m) v[m] = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      uint64_t x = key[w];
+      while (x) {
+        int o = 64 * w + ctz64(x) + 1;
+        x &= x - 1;
+        if (o == src) continue;
+        const double* rj = row + (size_t)2 * (spat(o) - 1) * nsp;
+#pragma unroll
+        for (int m = 0; m < kHbPerLane; ++m) {
+          int b0 = g + m * kHbGroup;
+          if (b0 < nsp) v[m] += rj[2 * b0];
+        }
+      }
+    }
+    uint64_t same = off ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      uint64_t x = key[w] & same;
+      while (x) {
+        int o = 64 * w + ctz64(x) + 1;
+        x &= x - 1;
+        if (o == src) continue;
+        const double* rj = row + (size_t)2 * (spat(o) - 1) * nsp;
+#pragma unroll
+        for (int m = 0; m < kHbPerLane; ++m) {
+          int b0 = g + m * kHbGroup;
+          if (b0 < nsp) v[m] -= rj[2 * b0 + 1];
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < kHbPerLane; ++m) {
+      int b = g + m * kHbGroup + 1;
+      if (b <= nsp) {
+        int orb = 2 * b - off;
+        v[m] = test_orb<W>(key, orb) ? 0.0 : v[m] + T(s, src, orb);
+      }
+    }
+    // sweep 1: normalisation; sweep 2: the first b whose running sum reaches r
+    double cum_sum = 0.0;
+    for (int b = 1; b <= nsp; ++b) {
+      int m = (b - 1) / kHbGroup;
+      double mine = m == 0 ? v[0] : m == 1 ? v[1] : m == 2 ? v[2] : v[3];
+      cum_sum += fabs(__shfl(mine, gbase + (b - 1) % kHbGroup));
+    }
+    double rr = d2 * cum_sum;
+    double cum = 0.0, cpt = 0.0, vsel = 0.0;
+    int bsel = 0;
+    bool found = false;
+    for (int b = 1; b <= nsp; ++b) {
+      int m = (b - 1) / kHbGroup;
+      double mine = m == 0 ? v[0] : m == 1 ? v[1] : m == 2 ? v[2] : v[3];
+      double vb = __shfl(mine, gbase + (b - 1) % kHbGroup);
+      if (!found) {
+        vsel = vb;
+        cpt = fabs(vb);
+        bsel = b;
+        cum += cpt;
+        found = !(cum < rr) || b == nsp;
+      }
+    }
+    bool mine_att = act && leader;
+    bool emit = false;
+    uint64_t child[W];
+    double nspawn = 0.0, an = 0.0, raw = 0.0, valid = 0.0, invalid = 0.0;
+    uint32_t info = 0;
+    if (mine_att) {
+      if (fabs(cum_sum) <= kEps || cpt == 0.0) {
+        invalid = 1.0;
+      } else {
+        valid = 1.0;
+        int tgt = 2 * bsel - off;
+        double pg = cpt / cum_sum;
+        pg = pg / (double)s.nel;
+        pg = pg * s.psingles;
+        double hel = single_parity<W>(key, src, tgt) ? -vsel : vsel;
+        uint8_t pi = pinfo[p];
+        double wsign = (pi & 1) ? -1.0 : 1.0;
+        nspawn = attempt_create(s, tau, hel, pg, wsign, d5, raw);
+        raw = fabs(raw);
+        if (fabs(nspawn) > kEps) {
+          emit = true;
+          an = fabs(nspawn);
+#pragma unroll
+          for (int w = 0; w < W; ++w) child[w] = key[w];
+          flip_orb<W>(child, src);
+          flip_orb<W>(child, tgt);
+          info = (uint32_t)r | ((s.t_init && (pi & 2)) ? kInfoInit : 0u);
+        }
+      }
+    }
+    acc.invalid.add(bs, S_NINVALID, invalid);
+    acc.valid.add(bs, S_NVALID, valid);
+    acc.maxspawn.add(bs, S_MAXSPAWN, raw);
+    for (int rr2 = 0; rr2 < S; ++rr2) {
+      acc.born.add(bs, soff(F_NBORN, rr2), rr2 == r ? an : 0.0);
+      acc.sing.add(bs, soff(F_SPAWNSING, rr2), rr2 == r ? an : 0.0);
+    }
+    bool bloom = emit && an > s.init_walk_no;
+    acc.bloomc1.add(bs, S_BLOOMC + 1, bloom ? 1.0 : 0.0);
+    acc.blooms1.add(bs, S_BLOOMS + 1, bloom ? an : 0.0);
+    int idx = stage_push(emit, &ss.st);
+    if (emit) {
+#pragma unroll
+      for (int w = 0; w < W; ++w) ss.key[idx * W + w] = child[w];
+      ss.val[idx] = nspawn;
+      ss.info[idx] = info;
+    }
     spawn_flush<W>(&ss, cnt, sp_key, sp_val, sp_info, scap, false, overflow);
   }
   spawn_flush<W>(&ss, cnt, sp_key, sp_val, sp_info, scap, true, overflow);
@@ -755,18 +962,25 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
 
 // ---------------------------------------------------------- radix sort
 // Stable LSD radix sort, 8-bit digits, reduce-then-scan: per-block digit
-// histograms, a per-digit scan over blocks, stable block-local scatter.
+// histograms, a per-digit scan over blocks, stable block-local scatter.  The
+// merge only needs equal determinants to be adjacent (and, within a run, the
+// spawn order, which a stable sort keeps), so the sort key may be any
+// injective function of the determinant.  When C(nbasis, nel) < 2^32 the key
+// is the determinant's 32-bit colex rank (sum over its i-th occupied bit at
+// position p of C(p, i)): fewer passes over narrower keys.  Otherwise the
+// key is the determinant itself, ordered as ilut_lt orders it.
 constexpr int kRadixBins = 256;
 
-template <int W>
-__device__ inline uint32_t radix_digit(const uint64_t* k, int word, int shift) {
-  uint64_t x = k[word];
-  if (W == 2 && word == 0) x ^= (1ull << 63);  // ilut_lt compares signed words
+// sort key: NW words of type K per element; FLIP compares word 0 as signed
+template <typename K, int NW, bool FLIP>
+__device__ inline uint32_t radix_digit(const K* k, int word, int shift) {
+  K x = k[word];
+  if (FLIP && word == 0) x ^= (K)1 << (sizeof(K) * 8 - 1);
   return (uint32_t)(x >> shift) & 0xFFu;
 }
 
-template <int W>
-__global__ void __launch_bounds__(kBlock) k_radix_hist(const uint64_t* keys, const unsigned long long* n_ptr,
+template <typename K, int NW, bool FLIP>
+__global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, const unsigned long long* n_ptr,
                                                        int64_t ncap, int word, int shift, uint32_t* hist) {
   __shared__ uint32_t h[kRadixBins];
   for (int i = threadIdx.x; i < kRadixBins; i += blockDim.x) h[i] = 0;
@@ -776,7 +990,40 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const uint64_t* keys, con
   int64_t begin, end;
   chunk_of(n, begin, end);
   for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x)
-    atomicAdd(&h[radix_digit<W>(keys + i * W, word, shift)], 1u);
+    atomicAdd(&h[radix_digit<K, NW, FLIP>(keys + i * NW, word, shift)], 1u);
+  __syncthreads();
+  for (int d = threadIdx.x; d < kRadixBins; d += blockDim.x) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
+}
+
+// colex rank of each spawned determinant (binom[p*(nel+1)+i] = C(p, i)),
+// fused with the histogram of its lowest digit
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, const unsigned long long* n_ptr,
+                                                      int64_t ncap, const uint32_t* binom, int nel1,
+                                                      uint32_t* rank, uint32_t* hist) {
+  __shared__ uint32_t h[kRadixBins];
+  for (int i = threadIdx.x; i < kRadixBins; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  int64_t n = (int64_t)*n_ptr;
+  if (n > ncap) n = ncap;
+  int64_t begin, end;
+  chunk_of(n, begin, end);
+  for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
+    uint32_t r = 0;
+    int c = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      uint64_t x = keys[i * W + w];
+      while (x) {
+        int p = 64 * w + ctz64(x);
+        x &= x - 1;
+        ++c;
+        r += binom[p * nel1 + c];
+      }
+    }
+    rank[i] = r;
+    atomicAdd(&h[r & 0xFFu], 1u);
+  }
   __syncthreads();
   for (int d = threadIdx.x; d < kRadixBins; d += blockDim.x) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
 }
@@ -801,11 +1048,10 @@ __global__ void __launch_bounds__(kBlock) k_radix_scan(uint32_t* hist, int nb, u
   if (threadIdx.x == 0) digit_tot[blockIdx.x] = (uint32_t)tot;
 }
 
-template <int W>
-__global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* kin, const uint32_t* iin,
-                                                          uint64_t* kout, uint32_t* iout,
-                                                          const unsigned long long* n_ptr, int64_t ncap,
-                                                          int word, int shift, const uint32_t* hist,
+template <typename K, int NW, bool FLIP>
+__global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, const uint32_t* iin, K* kout,
+                                                          uint32_t* iout, const unsigned long long* n_ptr,
+                                                          int64_t ncap, int word, int shift, const uint32_t* hist,
                                                           const uint32_t* digit_tot) {
   __shared__ uint32_t run[kRadixBins];
   __shared__ uint32_t wcnt[kBlock / kWave][kRadixBins];
@@ -827,13 +1073,13 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* kin, c
   for (int64_t base = begin; base < end; base += blockDim.x) {
     int64_t i = base + threadIdx.x;
     bool valid = i < end;
-    uint64_t k[W];
+    K k[NW];
     uint32_t idx = 0, d = 0;
     if (valid) {
 #pragma unroll
-      for (int w = 0; w < W; ++w) k[w] = kin[i * W + w];
+      for (int w = 0; w < NW; ++w) k[w] = kin[i * NW + w];
       idx = iin ? iin[i] : (uint32_t)i;
-      d = radix_digit<W>(k, word, shift);
+      d = radix_digit<K, NW, FLIP>(k, word, shift);
     }
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -848,7 +1094,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint64_t* kin, c
       uint32_t pos = run[d] + (uint32_t)rank;
       for (int q = 0; q < wv; ++q) pos += wcnt[q][d];
 #pragma unroll
-      for (int w = 0; w < W; ++w) kout[(size_t)pos * W + w] = k[w];
+      for (int w = 0; w < NW; ++w) kout[(size_t)pos * NW + w] = k[w];
       iout[pos] = idx;
     }
     __syncthreads();
@@ -905,9 +1151,17 @@ __device__ inline void list_flush(ListStage<W, R>* ls, unsigned long long* gctr,
 // CompressSpawnedList (Annihilation.F90:249-515) + FindResidualParticle
 // (:551-634): the head of each run of equal keys sums the run in sorted
 // (= emission, the sort is stable) order.
-template <int W, int R>
-__global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, double* part,
-                                                  const uint64_t* skey, const uint32_t* sidx,
+template <typename K, int NW>
+__device__ inline bool key_eq(const K* a, const K* b) {
+  bool e = true;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) e = e && (a[w] == b[w]);
+  return e;
+}
+
+template <int W, int R, typename K, int NW>
+__global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, double* part, const K* skey,
+                                                  const uint32_t* sidx, const uint64_t* sp_key,
                                                   const double* sp_val, const uint32_t* sp_info,
                                                   int64_t scap, uint64_t* u_key, double* u_sgn,
                                                   uint32_t* u_flag) {
@@ -929,13 +1183,15 @@ __global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, doubl
     uint32_t flag = 0;
     uint64_t key[W];
     if (i < n) {
-#pragma unroll
-      for (int w = 0; w < W; ++w) key[w] = skey[i * W + w];
+      const K* ki = skey + i * NW;
       bool head = true;
-      if (i > 0) head = !det_eq<W>(key, skey + (i - 1) * W);
+      if (i > 0) head = !key_eq<K, NW>(ki, skey + (i - 1) * NW);
       if (head) {
         int64_t j = i + 1;
-        while (j < n && det_eq<W>(key, skey + j * W)) ++j;
+        while (j < n && key_eq<K, NW>(ki, skey + j * NW)) ++j;
+        const uint64_t* kd = sp_key + (size_t)sidx[i] * W;
+#pragma unroll
+        for (int w = 0; w < W; ++w) key[w] = kd[w];
         for (int r = 0; r < S; ++r) sum[r] = 0.0;
         if (j == i + 1) {
           uint32_t inf = sp_info[sidx[i]];
@@ -1091,6 +1347,7 @@ __global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* 
   const unsigned long long head = cnt->free_head;
   const int64_t avail = (int64_t)(cnt->free_tail - head);
   const int64_t ns0 = (int64_t)cnt->nslots;
+  unsigned long long my_reused = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn;
        i += (int64_t)gridDim.x * blockDim.x) {
     int64_t slot = i < avail ? wk.freeq[(head + (unsigned long long)i) % (unsigned long long)wk.cap]
@@ -1104,7 +1361,9 @@ __global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* 
 #pragma unroll
     for (int w = 0; w < W; ++w) key[w] = n_key[i * W + w];
     int64_t home = find_walker_hash<W>(key, s.rh, s.L) - 1;
-    int64_t hp = hash_insert<W>(wk, key, home, (int32_t)slot);
+    bool reused = false;
+    int64_t hp = hash_insert<W>(wk, key, home, (int32_t)slot, reused);
+    my_reused += reused ? 1 : 0;
     if (hp < 0) {
       atomicOr(&cnt->err, 1u << NGPU_ERR_HASH_FULL);
       atomicAdd(&cnt->n_fail, 1ull);
@@ -1119,6 +1378,9 @@ __global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* 
       wk.hpos[slot] = hp;
     }
   }
+  // one no-return atomic per wave that reused entries
+  unsigned long long c = (unsigned long long)wave_sum((double)my_reused);
+  if (c && lane_id() == 0) atomicAdd(&cnt->n_reused, c);
 }
 
 // bookkeeping of k_insert: queued slots used, fresh slots appended (beyond
@@ -1131,6 +1393,8 @@ __global__ void k_post_insert(Counters* cnt, const unsigned long long* n_ptr, in
   unsigned long long ns = cnt->nslots + (nn - used);
   cnt->nslots = ns < (unsigned long long)cap ? ns : (unsigned long long)cap;
   cnt->n_live += nn - cnt->n_fail;
+  cnt->n_tomb -= cnt->n_reused;
+  cnt->n_reused = 0;
 }
 
 // CalcHashTableStats (load_balancer.fpp:646-805)
@@ -1251,7 +1515,8 @@ __global__ void __launch_bounds__(kBlock) k_hash_refill(DevSys s, Walk wk, Count
 #pragma unroll
     for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
     int64_t home = find_walker_hash<W>(key, s.rh, s.L) - 1;
-    int64_t hp = hash_insert<W>(wk, key, home, (int32_t)slot);
+    bool reused = false;
+    int64_t hp = hash_insert<W>(wk, key, home, (int32_t)slot, reused);
     if (hp < 0) atomicOr(&cnt->err, 1u << NGPU_ERR_HASH_FULL);
     else wk.hpos[slot] = hp;
   }

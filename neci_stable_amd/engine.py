@@ -144,9 +144,9 @@ class Engine:
         check(self.lib.ngpu_set_profiling(self.h, int(on)), "ngpu_set_profiling")
 
     def stage_times(self):
-        ms = np.zeros(8)
+        ms = np.zeros(9)
         n = ctypes.c_int64()
-        check(self.lib.ngpu_get_stage_times(self.h, ptr(ms), 8, ctypes.byref(n)), "ngpu_get_stage_times")
+        check(self.lib.ngpu_get_stage_times(self.h, ptr(ms), 9, ctypes.byref(n)), "ngpu_get_stage_times")
         return ms, n.value
 
     # -- batched plugin-surface entry points

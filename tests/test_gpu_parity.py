@@ -74,6 +74,10 @@ SPECS = {
     "mol8o6e_unif_fast": lambda **k: mol_spec(8, 6, **k),
     "mol8o6e_unif_unif": lambda **k: mol_spec(8, 6, particles=0, **k),
     "mol33o4e_w2": lambda **k: mol_spec(33, 4, rank=4, mseed=3, **k),
+    # C(nbasis, nel) > 2^32: the spawn sort runs on the determinant words
+    # themselves instead of 32-bit colex ranks
+    "hubbard18": lambda **k: hubbard_spec(18, 18, 4.0, **k),
+    "mol33o8e_w2": lambda **k: mol_spec(33, 8, rank=4, mseed=3, **k),
 }
 
 
@@ -175,7 +179,7 @@ def test_attempt_die_and_create():
         assert child[i] == exp
 
 
-@pytest.mark.parametrize("name", ["hubbard10", "mol8o6e_unif_fast", "mol33o4e_w2"])
+@pytest.mark.parametrize("name", ["hubbard10", "mol8o6e_unif_fast", "mol33o4e_w2", "hubbard18", "mol33o8e_w2"])
 def test_annihilate_explicit_list(name):
     spec = SPECS[name](init_walkers=3000)
     eng = engine(spec)
@@ -209,12 +213,14 @@ def test_annihilate_explicit_list(name):
 @pytest.mark.parametrize("name,nrep,real", [("hubbard10", 1, False), ("hubbard10", 2, False),
                                             ("mol8o6e_unif_fast", 1, False),
                                             ("mol8o6e_unif_unif", 1, False),
-                                            ("mol33o4e_w2", 1, False), ("hubbard10", 1, True),
+                                            ("mol33o4e_w2", 1, False), ("hubbard18", 1, False),
+                                            ("mol33o8e_w2", 1, False), ("hubbard10", 1, True),
                                             ("mol8o6e_unif_fast", 1, True)])
 def test_iterations_match_oracle(name, nrep, real):
     """Full iterations (spawn, merge, death, annihilation, stats) from the same
     walker set: identical walker sets after every iteration."""
-    tau = 0.01 if name.startswith("hub") else 0.02
+    # the 8-electron 2-word molecule grows ~1.4x per iteration at tau 0.02
+    tau = 0.01 if name.startswith("hub") else (0.005 if name == "mol33o8e_w2" else 0.02)
     spec = SPECS[name](init_walkers=4000, n_replicas=nrep, all_real_coeff=real, seed=77)
     eng = engine(spec)
     orc = oracle(spec, tau)
