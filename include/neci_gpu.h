@@ -142,6 +142,20 @@ int ngpu_upload_walkers(ngpu_t *h, const int64_t *rows, int64_t n);
 int ngpu_download_walkers(ngpu_t *h, int64_t *rows, double *diag_h, double *offdiag_h,
                           int64_t cap, int64_t *n);
 
+/* Semistochastic core space (init_semi_stochastic, semi_stoch_gen.F90:425-445;
+   semi_stoch_procs.F90:1095-1120).  The ncore determinants (W words each)
+   become core states: resident in the walker list even when empty, flagged
+   deterministic and, with the initiator approximation, initiators
+   (t_core_inits).  The engine builds the core Hamiltonian on the device
+   (calc_determ_hamil, fast_determ_hamil.F90:892-1548: H_ij of connected pairs,
+   diagonal H_ii - Hii, off-diagonal zeros dropped) as CSR, values in FP64 or,
+   with fp32_values, rounded to FP32 (accumulation stays FP64); *nnz receives
+   the number of stored elements.  Every later iteration projects the core
+   amplitudes deterministically (determ_projection, semi_stoch_procs.F90:105-241;
+   deterministic_annihilation, Annihilation.F90:930-963) and cancels spawns from
+   core to core dets (FciMCPar.F90:1659-1668).  One core space per engine. */
+int ngpu_set_core(ngpu_t *h, const int64_t *dets, int64_t ncore, int32_t fp32_values, int64_t *nnz);
+
 /* One FCIQMC iteration (PerformFCIMCycPar, FciMCPar.F90:1177-1920).  Runs
    asynchronously on the engine's stream when out == NULL; otherwise waits and
    fills out.  A failure inside the iteration is reported by the next call that

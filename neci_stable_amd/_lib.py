@@ -96,6 +96,7 @@ _SIGS = [
                                          ctypes.c_double]),
     ("ngpu_set_hash", ctypes.c_int, [_vp, _vp]),
     ("ngpu_set_reference", ctypes.c_int, [_vp, _vp, ctypes.c_double]),
+    ("ngpu_set_core", ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int32, P(ctypes.c_int64)]),
     ("ngpu_upload_walkers", ctypes.c_int, [_vp, _vp, ctypes.c_int64]),
     ("ngpu_download_walkers", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int64, P(ctypes.c_int64)]),
     ("ngpu_iterate", ctypes.c_int, [_vp, P(IterInT), P(IterOutT)]),

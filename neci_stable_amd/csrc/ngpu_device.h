@@ -226,6 +226,7 @@ struct DevSys {
   int nel, nbasis, S, model;
   int t_init, t_real, t_rsc, singles;
   int particles, nsites, maxnbr, ijmax;
+  int ncore;  // semistochastic core dets (0: none)
   double init_walk_no, rsc, occ_thresh, avmc;
   double hii, ecore, bhub, uhub;
   double psingles, pdoubles, pparallel;

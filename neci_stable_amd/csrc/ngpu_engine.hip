@@ -61,6 +61,16 @@ struct ngpu_engine {
   int64_t* blk_part = nullptr;
   int64_t* tile_p = nullptr;  // parent pair of each spawn tile's first attempt
   int64_t tcap = 0;
+  // semistochastic core space
+  int64_t ncore = 0, core_nnz = 0;
+  bool core_fp32 = false;
+  uint64_t* core_keys = nullptr;
+  int64_t* core_slot = nullptr;
+  int64_t* core_rowptr = nullptr;
+  int32_t* core_cols = nullptr;
+  void* core_vals = nullptr;  // double, or float when core_fp32
+  double* core_x = nullptr;
+  double* core_y = nullptr;
   // spawns, sort, merge, new
   int64_t scap = 0;
   uint64_t* sp_key = nullptr;
@@ -282,10 +292,30 @@ void sort_and_merge(ngpu_t* h) {
          (const double*)h->sp_val, (const uint32_t*)h->sp_info, h->scap, h->u_key, h->u_sgn, h->u_flag);
 }
 
+// semistochastic projection of the core amplitudes (before spawning)
+template <int R>
+void core_projection(ngpu_t* h, Shift sft, double tau) {
+  int nb = (int)std::min<int64_t>((h->ncore + kBlock - 1) / kBlock, h->nb_grid);
+  launch(h, -1, -1, k_core_gather<R>, dim3(nb), dim3(kBlock), (const int64_t*)h->core_slot, (const double*)h->wk.sgn,
+         h->S, h->ncore, h->core_x);
+  int nbw = (int)std::min<int64_t>((h->ncore + kBlock / kWave - 1) / (kBlock / kWave), 8 * h->nb_grid);
+  if (h->core_fp32)
+    launch(h, -1, -1, k_core_spmv<R, float>, dim3(nbw), dim3(kBlock), (const int64_t*)h->core_rowptr,
+           (const int32_t*)h->core_cols, (const float*)h->core_vals, (const double*)h->core_x, h->core_y, h->ncore,
+           h->S, sft, tau);
+  else
+    launch(h, -1, -1, k_core_spmv<R, double>, dim3(nbw), dim3(kBlock), (const int64_t*)h->core_rowptr,
+           (const int32_t*)h->core_cols, (const double*)h->core_vals, (const double*)h->core_x, h->core_y, h->ncore,
+           h->S, sft, tau);
+}
+
 // merge -> annihilate -> insert -> stats, shared by iterate and the explicit-list entry
 template <int W, int R>
-void annihilation_chain(ngpu_t* h, uint32_t iter) {
+void annihilation_chain(ngpu_t* h, uint32_t iter, bool with_core) {
   sort_and_merge<W, R>(h);
+  if (h->ncore && with_core)
+    launch(h, -1, -1, k_core_apply<R>, dim3((int)std::min<int64_t>((h->ncore + kBlock - 1) / kBlock, h->nb_grid)),
+           dim3(kBlock), h->ds, h->wk, (const int64_t*)h->core_slot, (const double*)h->core_y, h->ncore, h->dpart);
   launch(h, 6, 6, k_annihilate<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
          (const uint64_t*)h->u_key, (const double*)h->u_sgn, (const uint32_t*)h->u_flag, h->n_key, h->n_sgn,
          h->n_flag, iter);
@@ -307,6 +337,7 @@ void iterate_chain(ngpu_t* h, const ngpu_iter_in_t* in) {
   launch(h, -1, -1, k_begin_iter, dim3(1), dim3(kBlock), h->dout, h->dpart, h->cnt);
   launch(h, 0, 0, k_prepass<W, R>, dim3(h->nb_chunk), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart, h->att_cnt,
          h->pinfo, h->blk_part, sft, in->tau, in->iter);
+  if (h->ncore) core_projection<R>(h, sft, in->tau);
   launch(h, 1, -1, k_scan_partials, dim3(1), dim3(kBlock), h->blk_part, h->nb_chunk, h->cnt);
   launch(h, -1, 1, k_scan_apply, dim3(h->nb_chunk), dim3(kBlock), (const int32_t*)h->att_cnt, h->att_off,
          (const int64_t*)h->blk_part, h->cnt, h->S, h->tile_p, h->tcap);
@@ -317,7 +348,7 @@ void iterate_chain(ngpu_t* h, const ngpu_iter_in_t* in) {
     launch(h, 3, 3, k_spawn_singles<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
            (const uint8_t*)h->pinfo, h->sp_key, h->sp_val, h->sp_info, h->scap, (const int64_t*)h->sq, in->tau,
            in->iter);
-  annihilation_chain<W, R>(h, in->iter);
+  annihilation_chain<W, R>(h, in->iter, true);
 }
 
 void harvest(ngpu_t* h, int slot) {
@@ -683,7 +714,8 @@ int ngpu_destroy(ngpu_t* h) {
   void* ptrs[] = {h->wk.keys, h->wk.sgn, h->wk.flags, h->wk.diagH, h->wk.offdiagH, h->wk.hpos, h->wk.freeq,
                   h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->sp_key, h->sp_val, h->sp_info,
                   h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->digit_tot, h->sq, h->u_key, h->u_sgn, h->u_flag, h->n_key,
-                  h->n_sgn, h->n_flag, h->cnt, h->dout, h->dpart, h->dcyc, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_binom, h->rk, h->tile_p, h->d_ab_tab,
+                  h->n_sgn, h->n_flag, h->cnt, h->dout, h->dpart, h->dcyc, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_binom, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
+                  h->core_vals, h->core_x, h->core_y, h->d_ab_tab,
                   h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -918,6 +950,95 @@ int ngpu_upload_walkers(ngpu_t* h, const int64_t* rows, int64_t n) {
   HIP_OK(hipMemcpyAsync(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
   return err_from_mask(h->hcnt->err);
+}
+
+extern "C++" {
+namespace {
+template <int W, int R>
+int set_core_impl(ngpu_t* h, int64_t ncore, bool fp32, int64_t* nnz_out) {
+  const int S = h->S;
+  auto grid_rows = [&](int64_t n) { return dim3((unsigned)std::min<int64_t>((n + 3) / 4, 8 * (int64_t)h->nb_grid)); };
+  // core Hamiltonian: count, scan on the host, fill
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_core_rows<W, double>), grid_rows(ncore), dim3(kBlock), 0, h->stream, h->ds,
+                     (const uint64_t*)h->core_keys, ncore, 0, h->core_rowptr + 1, (const int64_t*)nullptr,
+                     (int32_t*)nullptr, (double*)nullptr);
+  std::vector<int64_t> rp((size_t)ncore + 1, 0);
+  HIP_OK(hipMemcpyAsync(rp.data() + 1, h->core_rowptr + 1, (size_t)ncore * 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  for (int64_t i = 0; i < ncore; ++i) rp[i + 1] += rp[i];
+  const int64_t nnz = rp[ncore];
+  HIP_OK(hipMemcpy(h->core_rowptr, rp.data(), rp.size() * 8, hipMemcpyHostToDevice));
+  HIP_OK(dalloc(&h->core_cols, (size_t)nnz));
+  if (fp32) {
+    float* v = nullptr;
+    HIP_OK(dalloc(&v, (size_t)nnz));
+    h->core_vals = v;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_core_rows<W, float>), grid_rows(ncore), dim3(kBlock), 0, h->stream, h->ds,
+                       (const uint64_t*)h->core_keys, ncore, 1, (int64_t*)nullptr, (const int64_t*)h->core_rowptr,
+                       h->core_cols, v);
+  } else {
+    double* v = nullptr;
+    HIP_OK(dalloc(&v, (size_t)nnz));
+    h->core_vals = v;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_core_rows<W, double>), grid_rows(ncore), dim3(kBlock), 0, h->stream, h->ds,
+                       (const uint64_t*)h->core_keys, ncore, 1, (int64_t*)nullptr, (const int64_t*)h->core_rowptr,
+                       h->core_cols, v);
+  }
+  // core dets resident, flagged deterministic (and initiator: t_core_inits)
+  uint32_t cflags = kFlagCore | (h->sys.t_trunc_initiator ? ((1u << S) - 1u) : 0u);
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_new, 0ull);
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_fail, 0ull);
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, (unsigned long long*)&h->cnt->err, 0ull);
+  dim3 gp((unsigned)std::min<int64_t>((ncore + kBlock - 1) / kBlock, h->nb_grid));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_core_place<W>), gp, dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
+                     (const uint64_t*)h->core_keys, ncore, cflags, h->core_slot, h->n_key, h->n_sgn, h->n_flag);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_insert<W, R>), dim3(h->nb_grid), dim3(kBlock), 0, h->stream, h->ds, h->wk,
+                     h->cnt, (const uint64_t*)h->n_key, (const double*)h->n_sgn, (const uint32_t*)h->n_flag,
+                     (const unsigned long long*)&h->cnt->n_new);
+  hipLaunchKernelGGL(k_post_insert, dim3(1), dim3(1), 0, h->stream, h->cnt, (const unsigned long long*)&h->cnt->n_new,
+                     h->wk.cap);
+  hash_upkeep<W>(h);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_core_place<W>), gp, dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
+                     (const uint64_t*)h->core_keys, ncore, cflags, h->core_slot, h->n_key, h->n_sgn, h->n_flag);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  if (h->hcnt->err) return err_from_mask(h->hcnt->err);
+  h->core_nnz = nnz;
+  if (nnz_out) *nnz_out = nnz;
+  return NGPU_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+int ngpu_set_core(ngpu_t* h, const int64_t* dets, int64_t ncore, int32_t fp32_values, int64_t* nnz) {
+  int rc = ready(h);
+  if (rc) return rc;
+  if (!dets || ncore < 1 || ncore > (int64_t)INT32_MAX || ncore > h->wk.cap) return NGPU_ERR_ARG;
+  if (h->ncore) return NGPU_ERR_STATE;  // one core space per engine
+  for (int64_t i = 0; i < ncore; ++i) {
+    int nel = 0;
+    for (int w = 0; w < h->W; ++w) nel += __builtin_popcountll((uint64_t)dets[i * h->W + w]);
+    if (nel != h->sys.nel) return NGPU_ERR_ARG;
+  }
+  if (h->pending) {
+    rc = finish(h, nullptr);
+    if (rc) return rc;
+  }
+  const int W = h->W, S = h->S;
+  HIP_OK(dalloc(&h->core_keys, (size_t)ncore * W));
+  HIP_OK(dalloc(&h->core_slot, (size_t)ncore));
+  HIP_OK(dalloc(&h->core_rowptr, (size_t)ncore + 1));
+  HIP_OK(dalloc(&h->core_x, (size_t)ncore * S));
+  HIP_OK(dalloc(&h->core_y, (size_t)ncore * S));
+  HIP_OK(hipMemcpy(h->core_keys, dets, (size_t)ncore * W * 8, hipMemcpyHostToDevice));
+  h->core_fp32 = fp32_values != 0;
+  if (W == 1) rc = S == 1 ? set_core_impl<1, 1>(h, ncore, h->core_fp32, nnz) : set_core_impl<1, kMaxRep>(h, ncore, h->core_fp32, nnz);
+  else rc = S == 1 ? set_core_impl<2, 1>(h, ncore, h->core_fp32, nnz) : set_core_impl<2, kMaxRep>(h, ncore, h->core_fp32, nnz);
+  if (rc) return rc;
+  h->ncore = ncore;
+  h->ds.ncore = 1;
+  return NGPU_OK;
 }
 
 int ngpu_download_walkers(ngpu_t* h, int64_t* rows, double* diag_h, double* offdiag_h, int64_t cap, int64_t* n) {
@@ -1221,8 +1342,9 @@ int ngpu_annihilate(ngpu_t* h, uint32_t iter, const int64_t* sdet, const double*
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_spawned, (unsigned long long)n);
   bool p = h->prof;
   h->prof = false;
-  if (W == 1) h->S == 1 ? annihilation_chain<1, 1>(h, iter) : annihilation_chain<1, kMaxRep>(h, iter);
-  else h->S == 1 ? annihilation_chain<2, 1>(h, iter) : annihilation_chain<2, kMaxRep>(h, iter);
+  // an explicit list carries no deterministic projection
+  if (W == 1) h->S == 1 ? annihilation_chain<1, 1>(h, iter, false) : annihilation_chain<1, kMaxRep>(h, iter, false);
+  else h->S == 1 ? annihilation_chain<2, 1>(h, iter, false) : annihilation_chain<2, kMaxRep>(h, iter, false);
   h->prof = p;
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(h->hout, h->dout, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));

@@ -430,7 +430,7 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
         }
         att_cnt[slot * S + r] = (int32_t)nsp;
         my_att += nsp;
-        pinfo[slot * S + r] = (uint8_t)((sg[r] < 0.0 ? 1 : 0) | (((f >> r) & 1u) ? 2 : 0));
+        pinfo[slot * S + r] = (uint8_t)((sg[r] < 0.0 ? 1 : 0) | (((f >> r) & 1u) ? 2 : 0) | (core ? 4 : 0));
         double fac = tau * (kii - sft.v[r]);
         if (fac > 2.0) death_err = true;
         double idie = core ? 0.0 : attempt_die(s.t_real != 0, fac, fabs(sg[r]), ub);
@@ -597,6 +597,18 @@ __device__ inline void queue_flush(QueueStage* qs, Counters* cnt, int64_t* sq, i
   stage_flush_end(&qs->st);
 }
 
+// is_core_state (semi_stoch_procs.F90:590-605) of the child of an excitation:
+// core dets are always resident, so the walker hash and its core flag answer it
+template <int W>
+__device__ inline bool core_child(const DevSys& s, const Walk& wk, Counters* cnt, const uint64_t* key,
+                                  const Excit& ex) {
+  uint64_t child[W];
+  apply_excit<W>(key, ex, child);
+  int64_t home = find_walker_hash<W>(child, s.rh, s.L) - 1;
+  int64_t pos = hash_lookup<W>(wk, child, home, cnt);
+  return pos >= 0 && (wk.flags[pos] & kFlagCore);
+}
+
 // statistics of the spawning kernels, per lane until the end of the kernel
 template <int R>
 struct SpawnAcc {
@@ -623,9 +635,9 @@ struct SpawnAcc {
 // one spawning attempt (attempt_create_normal + create_particle); mode 0: any
 // excitation, 1: doubles only (singles are queued), 2: singles only.  Wave-wide.
 template <int W, int R>
-__device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, BlockStats& bs, SpawnAcc<R>& acc,
-                                     const uint8_t* pinfo, bool active, int64_t p, uint32_t k, int mode,
-                                     SpawnStage<W>* ss, QueueStage* qs, double tau, uint32_t iter) {
+__device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* cnt, BlockStats& bs,
+                                     SpawnAcc<R>& acc, const uint8_t* pinfo, bool active, int64_t p, uint32_t k,
+                                     int mode, SpawnStage<W>* ss, QueueStage* qs, double tau, uint32_t iter) {
   const int S = R == 1 ? 1 : s.S;
   bool emit = false, queue = false;
   uint64_t child[W];
@@ -659,6 +671,8 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, BlockStats
       }
       if (ex.ic == 0) {
         invalid = 1.0;
+      } else if (s.ncore && (pinfo[p] & 4) && core_child<W>(s, wk, cnt, key, ex)) {
+        valid = 1.0;  // spawning from and to the core space is cancelled
       } else {
         valid = 1.0;
         ic = ex.ic;
@@ -775,7 +789,7 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
       p = parent_pair(att_off, lo, hi, a);
       k = (uint32_t)(a - att_off[p]);
     }
-    spawn_attempt<W, R>(s, wk, bs, acc, pinfo, act, p, k, mode, &ss, &qs, tau, iter);
+    spawn_attempt<W, R>(s, wk, cnt, bs, acc, pinfo, act, p, k, mode, &ss, &qs, tau, iter);
     spawn_flush<W>(&ss, cnt, sp_key, sp_val, sp_info, scap, false, overflow);
     if (mode == 1) queue_flush(&qs, cnt, sq, scap, false, overflow);
   }
@@ -910,9 +924,19 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
     uint64_t child[W];
     double nspawn = 0.0, an = 0.0, raw = 0.0, valid = 0.0, invalid = 0.0;
     uint32_t info = 0;
+    bool cancel = false;
+    if (mine_att && s.ncore && (pinfo[p] & 4) && !(fabs(cum_sum) <= kEps || cpt == 0.0)) {
+      Excit ex{};
+      ex.ic = 1;
+      ex.e0 = src;
+      ex.e1 = 2 * bsel - off;
+      cancel = core_child<W>(s, wk, cnt, key, ex);  // from and to the core space
+    }
     if (mine_att) {
       if (fabs(cum_sum) <= kEps || cpt == 0.0) {
         invalid = 1.0;
+      } else if (cancel) {
+        valid = 1.0;
       } else {
         valid = 1.0;
         int tgt = 2 * bsel - off;
@@ -1333,6 +1357,140 @@ __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counte
     bs.add(soff(F_NBORN, r), st_bn[r]);
   }
   bs.flush(part);
+}
+
+// ------------------------------------------------------ semistochastic core
+// Core amplitudes before the projection (no death acts on core dets, so the
+// signs after the prepass are the iteration's starting amplitudes).
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_core_gather(const int64_t* core_slot, const double* sgn, int S,
+                                                        int64_t ncore, double* x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncore; i += (int64_t)gridDim.x * blockDim.x)
+    for (int r = 0; r < S; ++r) x[i * S + r] = sgn[core_slot[i] * S + r];
+}
+
+// determ_projection (semi_stoch_procs.F90:105-241), one wave per core row:
+// lane l sums the row's nonzeros q = l, l+64, ... as acc = acc - H*x, the
+// butterfly acc = acc + shfl_xor(acc, o) for o = 32..1 combines the lanes, and
+// y = (acc + S*x_i) * tau.  The CPU restatement sums in exactly this order.
+template <int R, typename VT>
+__global__ void __launch_bounds__(kBlock) k_core_spmv(const int64_t* rowptr, const int32_t* cols, const VT* vals,
+                                                      const double* x, double* y, int64_t ncore, int S, Shift sft,
+                                                      double tau) {
+  const int lane = lane_id();
+  const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < ncore; i += nwaves) {
+    double acc[R];
+    for (int r = 0; r < S; ++r) acc[r] = 0.0;
+    const int64_t e = rowptr[i + 1];
+    for (int64_t q = rowptr[i] + lane; q < e; q += kWave) {
+      double v = (double)vals[q];
+      const double* xc = x + (size_t)cols[q] * S;
+      for (int r = 0; r < S; ++r) acc[r] = acc[r] - v * xc[r];
+    }
+    for (int r = 0; r < S; ++r) {
+#pragma unroll
+      for (int o = kWave / 2; o > 0; o >>= 1) acc[r] = acc[r] + __shfl_xor(acc[r], o);
+    }
+    if (lane == 0)
+      for (int r = 0; r < S; ++r) y[i * S + r] = (acc[r] + sft.v[r] * x[i * S + r]) * tau;
+  }
+}
+
+// deterministic_annihilation (Annihilation.F90:930-963): the projected
+// amplitudes join the core dets' signs (births and annihilations counted)
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_core_apply(DevSys s, Walk wk, const int64_t* core_slot, const double* y,
+                                                       int64_t ncore, double* part) {
+  __shared__ double acc_lds[kStatStride];
+  BlockStats bs{acc_lds};
+  bs.zero();
+  const int S = R == 1 ? 1 : s.S;
+  LaneSum<R> born, ann;
+  int64_t trips = grid_trips(ncore);
+  for (int64_t t = 0; t < trips; ++t) {
+    int64_t i = grid_elem(t);
+    bool act = i < ncore;
+    int64_t slot = act ? core_slot[i] : 0;
+    for (int r = 0; r < S; ++r) {
+      double b = 0.0, a = 0.0;
+      if (act) {
+        double yy = y[i * S + r], c = wk.sgn[slot * S + r];
+        wk.sgn[slot * S + r] = yy + c;
+        b = fabs(yy);
+        if (c * yy < 0.0) {
+          double x1 = fabs(c), x2 = fabs(yy);
+          a = 2.0 * (x1 < x2 ? x1 : x2);
+        }
+      }
+      born.add(bs, soff(F_NBORN, r), b);
+      ann.add(bs, soff(F_NANNIHIL, r), a);
+    }
+  }
+  born.done(bs, soff(F_NBORN, 0));
+  ann.done(bs, soff(F_NANNIHIL, 0));
+  bs.flush(part);
+}
+
+// core Hamiltonian rows (calc_determ_hamil, fast_determ_hamil.F90:892-1548):
+// one wave per row i scans every core det j in ascending order; H_ij for
+// connected pairs (diagonal H_ii - Hii), off-diagonal zeros dropped.  Pass 0
+// counts, pass 1 writes columns and values at rowptr[i].
+template <int W, typename VT>
+__global__ void __launch_bounds__(kBlock) k_core_rows(DevSys s, const uint64_t* keys, int64_t ncore, int pass,
+                                                      int64_t* count, const int64_t* rowptr, int32_t* cols,
+                                                      VT* vals) {
+  const int lane = lane_id();
+  const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < ncore; i += nwaves) {
+    const uint64_t* di = keys + i * W;
+    int64_t base = pass ? rowptr[i] : 0;
+    for (int64_t j0 = 0; j0 < ncore; j0 += kWave) {
+      int64_t j = j0 + lane;
+      bool keep = false;
+      double h = 0.0;
+      if (j < ncore) {
+        const uint64_t* dj = keys + j * W;
+        if (excit_level<W>(di, dj) <= 2) {
+          h = (i == j) ? diag_helement<W>(s, di) - s.hii : helement_pair<W>(s, di, dj);
+          keep = (i == j) || h != 0.0;
+        }
+      }
+      uint64_t m = __ballot(keep);
+      if (pass && keep) {
+        int64_t q = base + popc64(m & lanemask_lt());
+        cols[q] = (int32_t)j;
+        vals[q] = (VT)h;
+      }
+      base += popc64(m);
+    }
+    if (!pass && lane == 0) count[i] = base;
+  }
+}
+
+// place the core dets: resident ones get the core (and initiator) flags and
+// their slot; absent ones are appended to the new list with zero signs
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_core_place(DevSys s, Walk wk, Counters* cnt, const uint64_t* keys,
+                                                       int64_t ncore, uint32_t cflags, int64_t* core_slot,
+                                                       uint64_t* n_key, double* n_sgn, uint32_t* n_flag) {
+  const int S = s.S;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncore; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t* d = keys + i * W;
+    int64_t home = find_walker_hash<W>(d, s.rh, s.L) - 1;
+    int64_t pos = hash_lookup<W>(wk, d, home, cnt);
+    if (pos >= 0) {
+      wk.flags[pos] |= cflags;
+      core_slot[i] = pos;
+    } else {
+      unsigned long long q = atomicAdd(&cnt->n_new, 1ull);
+#pragma unroll
+      for (int w = 0; w < W; ++w) n_key[q * W + w] = d[w];
+      for (int r = 0; r < S; ++r) n_sgn[q * S + r] = 0.0;
+      n_flag[q] = cflags;
+      core_slot[i] = -1;
+    }
+  }
 }
 
 // AddNewHashDet (load_balancer.fpp:514-629) for the new list

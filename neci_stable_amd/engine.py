@@ -140,6 +140,15 @@ class Engine:
               "ngpu_get_cycle_stats")
         return sm.to_dict(self.S), self._out.to_dict(self.S), n.value
 
+    def set_core(self, dets, fp32=False):
+        """Semistochastic core space (ngpu_set_core); returns the number of stored H_ij."""
+        dets = np.ascontiguousarray(dets, dtype=np.int64).reshape(-1, self.W)
+        nnz = ctypes.c_int64()
+        check(self.lib.ngpu_set_core(self.h, ptr(dets), dets.shape[0], int(bool(fp32)), ctypes.byref(nnz)),
+              "ngpu_set_core")
+        self.ncore = dets.shape[0]
+        return nnz.value
+
     def set_profiling(self, on=True):
         check(self.lib.ngpu_set_profiling(self.h, int(on)), "ngpu_set_profiling")
 

@@ -810,6 +810,14 @@ struct ostate {
     uint64_t *spawned, *spawned2;
     int64_t nspawned;
     int64_t *perm, *perm_tmp;
+    /* semistochastic core space (cs_replicas: core dets, sparse_core_ham) */
+    int64_t ncore;
+    uint64_t *core_det;   /* ncore * W */
+    int64_t *core_row;    /* CurrentDets row of each core det */
+    int64_t *rowptr;      /* CSR of H_ij (diagonal H_ii - Hii), columns ascending */
+    int32_t *cols;
+    double *vals;         /* FP64 values, or FP64 copies of FP32-rounded values */
+    double *cx, *cy;      /* core amplitudes and their projection, ncore * S */
 };
 
 static inline double *row_sign(ostate_t *st, uint64_t *row) { (void)st; return (double *)(row + st->W); }
@@ -838,11 +846,21 @@ ostate_t *o_create(const osys_t *sys)
     return st;
 }
 
+static void free_core(ostate_t *st)
+{
+    free(st->core_det); free(st->core_row); free(st->rowptr); free(st->cols); free(st->vals);
+    free(st->cx); free(st->cy);
+    st->core_det = NULL; st->core_row = NULL; st->rowptr = NULL; st->cols = NULL; st->vals = NULL;
+    st->cx = NULL; st->cy = NULL;
+    st->ncore = 0;
+}
+
 void o_destroy(ostate_t *st)
 {
     if (!st) return;
     free(st->cur); free(st->diagH); free(st->offdiagH); free(st->head); free(st->next);
     free(st->freeslot); free(st->spawned); free(st->spawned2); free(st->perm); free(st->perm_tmp);
+    free_core(st);
     free(st);
 }
 
@@ -953,6 +971,139 @@ int64_t o_get_walkers(ostate_t *st, uint64_t *dets, double *sgns, uint32_t *flag
         n++;
     }
     return n;
+}
+
+/* ------------------------------------------------------- core space */
+
+/* init_semi_stochastic (semi_stoch_gen.F90:425-445, semi_stoch_procs.F90:1095-1120)
+   for a given core space: every core det is in CurrentDets (absent ones with zero
+   sign), flagged deterministic and, with the initiator approximation
+   (t_core_inits), initiator in every replica.  The core Hamiltonian
+   (calc_determ_hamil, fast_determ_hamil.F90:892-1548: H_ij, diagonal H_ii - Hii)
+   is stored as CSR, columns ascending, off-diagonal zeros dropped; fp32 keeps
+   FP32-rounded values. */
+int32_t o_set_core(ostate_t *st, const uint64_t *dets, int64_t n, int32_t fp32, int64_t *nnz_out)
+{
+    const int W = st->W, S = st->S, nel = st->s.nel;
+    free_core(st);
+    if (n <= 0) { if (nnz_out) *nnz_out = 0; return 0; }
+    st->ncore = n;
+    st->core_det = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n * W);
+    memcpy(st->core_det, dets, sizeof(uint64_t) * (size_t)n * W);
+    st->core_row = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    st->rowptr = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+    st->cx = (double *)calloc((size_t)n * S, sizeof(double));
+    st->cy = (double *)calloc((size_t)n * S, sizeof(double));
+    int32_t *occ = (int32_t *)malloc(sizeof(int32_t) * (size_t)n * nel);
+    for (int64_t i = 0; i < n; ++i) o_decode(dets + i * W, W, occ + i * nel);
+    for (int pass = 0; pass < 2; ++pass) {
+        int64_t k = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            const uint64_t *di = dets + i * W;
+            for (int64_t j = 0; j < n; ++j) {
+                const uint64_t *dj = dets + j * W;
+                if (excit_level(di, dj, W) > 2) continue;
+                double h = (i == j) ? o_diag_helement(&st->s, occ + i * nel) - st->s.hii
+                                    : o_get_helement(&st->s, occ + i * nel, occ + j * nel, nel);
+                if (i != j && h == 0.0) continue;
+                if (pass == 1) {
+                    st->cols[k] = (int32_t)j;
+                    st->vals[k] = fp32 ? (double)(float)h : h;
+                }
+                k++;
+            }
+            if (pass == 0) st->rowptr[i + 1] = k;
+        }
+        if (pass == 0) {
+            st->cols = (int32_t *)malloc(sizeof(int32_t) * (size_t)(k ? k : 1));
+            st->vals = (double *)malloc(sizeof(double) * (size_t)(k ? k : 1));
+        }
+    }
+    free(occ);
+    if (nnz_out) *nnz_out = st->rowptr[n];
+    /* place the core dets in CurrentDets */
+    st->ifree_start = st->ifree_end = 0;
+    uint64_t cflags = FLAG_CORE;
+    if (st->s.t_trunc_init)
+        for (int r = 0; r < S; ++r) cflags |= 1ull << r;
+    double zero[O_MAX_REPLICAS] = {0};
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t *d = dets + i * W;
+        int64_t b = bucket_of(st, d);
+        int64_t pos = hash_lookup(st, d, b);
+        if (pos < 0) {
+            int32_t nI[O_MAX_NEL];
+            o_decode(d, W, nI);
+            int e = add_new_hash_det(st, d, zero, cflags, o_diag_helement(&st->s, nI),
+                                     o_offdiag_helement(&st->s, d), b);
+            if (e) return e;
+            pos = hash_lookup(st, d, b);
+        } else {
+            *row_flag(st, cur_row(st, pos)) |= cflags;
+        }
+        st->core_row[i] = pos;
+    }
+    return 0;
+}
+
+/* is_core_state: the det is in CurrentDets with the deterministic flag */
+static int is_core_det(ostate_t *st, const uint64_t *det)
+{
+    int64_t pos = hash_lookup(st, det, bucket_of(st, det));
+    return pos >= 0 && (*row_flag(st, cur_row(st, pos)) & FLAG_CORE);
+}
+
+/* determ_projection (semi_stoch_procs.F90:105-241): y_i = tau*(S*x_i - sum_j H_ij x_j).
+   Summation order (shared with the GPU engine): the nonzeros of row i are dealt
+   to 64 partial sums by their position q in the row (partial q mod 64, each
+   summed in ascending q as partial = partial - H*x), the partials are combined
+   by the butterfly p[l] = p[l] + p[l xor o] for o = 32,16,8,4,2,1, then
+   y = (p[0] + S*x_i) * tau. */
+void o_core_project(const ostate_t *st, const double *x, const double *shift, double tau, double *y)
+{
+    const int S = st->S;
+    double p[64], t[64];
+    for (int64_t i = 0; i < st->ncore; ++i) {
+        for (int r = 0; r < S; ++r) {
+            for (int l = 0; l < 64; ++l) p[l] = 0.0;
+            for (int64_t q = st->rowptr[i]; q < st->rowptr[i + 1]; ++q) {
+                int l = (int)((q - st->rowptr[i]) & 63);
+                p[l] = p[l] - st->vals[q] * x[(size_t)st->cols[q] * S + r];
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                for (int l = 0; l < 64; ++l) t[l] = p[l] + p[l ^ o];
+                memcpy(p, t, sizeof(p));
+            }
+            y[(size_t)i * S + r] = (p[0] + shift[r] * x[(size_t)i * S + r]) * tau;
+        }
+    }
+}
+
+int64_t o_core_csr(const ostate_t *st, int64_t *rowptr, int32_t *cols, double *vals)
+{
+    if (rowptr) memcpy(rowptr, st->rowptr, sizeof(int64_t) * (size_t)(st->ncore + 1));
+    int64_t nnz = st->ncore ? st->rowptr[st->ncore] : 0;
+    if (cols) memcpy(cols, st->cols, sizeof(int32_t) * (size_t)nnz);
+    if (vals) memcpy(vals, st->vals, sizeof(double) * (size_t)nnz);
+    return nnz;
+}
+
+/* deterministic_annihilation (Annihilation.F90:930-963) */
+static void core_annihilation(ostate_t *st, ostats_t *out)
+{
+    const int S = st->S;
+    for (int64_t i = 0; i < st->ncore; ++i) {
+        double *cs = row_sign(st, cur_row(st, st->core_row[i]));
+        for (int r = 0; r < S; ++r) {
+            double y = st->cy[(size_t)i * S + r], c = cs[r];
+            cs[r] = y + c;
+            out->nborn[r] += fabs(y);
+            if (c * y < 0.0) {
+                double a = fabs(c), b = fabs(y);
+                out->nannihil[r] += 2.0 * (a < b ? a : b);
+            }
+        }
+    }
 }
 
 /* stochastic_round_r (lib/util_mod.fpp:207-224) */
@@ -1298,6 +1449,8 @@ int32_t o_iterate(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_t 
                     continue;
                 }
                 out->n_valid_excits += 1.0;
+                /* spawning from and to the core space is cancelled (FciMCPar.F90:1659-1668) */
+                if (core && st->ncore && is_core_det(st, ilutJ)) continue;
                 int32_t nI[O_MAX_NEL];
                 o_decode(row, W, nI);
                 /* attempt_create_normal (fcimc_pointed_fns.F90:178-491) */
@@ -1334,10 +1487,17 @@ int32_t o_iterate(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_t 
     }
     out->n_spawned = st->nspawned;
     if (err) { out->err = err; return err; }
+    /* determ_projection on the core amplitudes (unchanged by the loop: no death) */
+    if (st->ncore) {
+        for (int64_t i = 0; i < st->ncore; ++i)
+            memcpy(st->cx + (size_t)i * S, row_sign(st, cur_row(st, st->core_row[i])), sizeof(double) * S);
+        o_core_project(st, st->cx, diag_sft, s->tau, st->cy);
+    }
     /* communicate_and_merge_spawns -> CompressSpawnedList */
     int64_t nu = compress_spawned(st, st->nspawned, out);
     out->n_unique = nu;
-    /* DirectAnnihilation -> AnnihilateSpawnedParts + CalcHashTableStats */
+    /* DirectAnnihilation: deterministic_annihilation, AnnihilateSpawnedParts, CalcHashTableStats */
+    if (st->ncore) core_annihilation(st, out);
     err = annihilate_spawned(st, nu, iter, out);
     if (err) { out->err = err; return err; }
     hash_table_stats(st, iter, out);

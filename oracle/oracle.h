@@ -123,6 +123,11 @@ int32_t o_add_walkers(ostate_t *st, const uint64_t *dets, const double *sgns, co
 int64_t o_get_walkers(ostate_t *st, uint64_t *dets, double *sgns, uint32_t *flags,
                       double *diagh, double *offdiagh, int64_t cap);
 int32_t o_iterate(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_t *out);
+/* semistochastic core space: n dets (W words each); CSR of H_ij with diagonal
+   H_ii - Hii; fp32 = values rounded to FP32 */
+int32_t o_set_core(ostate_t *st, const uint64_t *dets, int64_t n, int32_t fp32, int64_t *nnz);
+void o_core_project(const ostate_t *st, const double *x, const double *shift, double tau, double *y);
+int64_t o_core_csr(const ostate_t *st, int64_t *rowptr, int32_t *cols, double *vals);
 /* annihilation of an explicit spawned list only (CompressSpawnedList + death-free
    AnnihilateSpawnedParts); spawned rows are (det words | sign | info) */
 int32_t o_annihilate_list(ostate_t *st, uint32_t iter, const uint64_t *sdet, const double *sval,

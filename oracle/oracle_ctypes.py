@@ -91,6 +91,9 @@ def lib():
             "o_iterate": (ctypes.c_int32, [_vp, ctypes.c_uint32, _vp, _vp]),
             "o_annihilate_list": (ctypes.c_int32, [_vp, ctypes.c_uint32, _vp, _vp, _vp, ctypes.c_int64, _vp,
                                                    _vp, _vp, _vp, _vp]),
+            "o_set_core": (ctypes.c_int32, [_vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp]),
+            "o_core_project": (None, [_vp, _vp, _vp, ctypes.c_double, _vp]),
+            "o_core_csr": (ctypes.c_int64, [_vp, _vp, _vp, _vp]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -227,6 +230,31 @@ class Oracle:
         d["rc"] = rc
         k = nm.value
         return d, md[:k].view(np.int64), ms[:k], mf[:k]
+
+    def set_core(self, dets, fp32=False):
+        """Semistochastic core space; returns the number of stored H_ij."""
+        dets = np.ascontiguousarray(dets, dtype=np.int64).reshape(-1, self.W).view(np.uint64)
+        self.ncore = dets.shape[0]
+        nnz = ctypes.c_int64()
+        rc = self.L.o_set_core(self.st, p(dets), self.ncore, int(bool(fp32)), ctypes.byref(nnz))
+        if rc:
+            raise RuntimeError(f"o_set_core {rc}")
+        return nnz.value
+
+    def core_csr(self):
+        nnz = self.L.o_core_csr(self.st, None, None, None)
+        rp = np.zeros(self.ncore + 1, dtype=np.int64)
+        cols = np.zeros(max(nnz, 1), dtype=np.int32)
+        vals = np.zeros(max(nnz, 1))
+        self.L.o_core_csr(self.st, p(rp), p(cols), p(vals))
+        return rp, cols[:nnz], vals[:nnz]
+
+    def core_project(self, x, shift, tau):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(self.ncore, self.S)
+        sh = np.ascontiguousarray(np.broadcast_to(np.asarray(shift, dtype=np.float64), (self.S,)))
+        y = np.zeros_like(x)
+        self.L.o_core_project(self.st, p(x), p(sh), float(tau), p(y))
+        return y
 
     def gen_excit(self, det, draws):
         det = np.ascontiguousarray(det, dtype=np.int64).view(np.uint64)

@@ -42,6 +42,32 @@ def random_dets(spec, n, rng):
     return out
 
 
+def hopping_neighbourhood(spec, ref, depth):
+    """Dets reachable from ref by up to `depth` nearest-neighbour hops (Hubbard)."""
+    nbr = spec.system.nbr
+    seen = {tuple(ref)}
+    front = [np.asarray(ref)]
+    for _ in range(depth):
+        nxt = []
+        for d in front:
+            orbs = decode_det(d.reshape(1, -1))
+            occ = set(orbs)
+            for o in orbs:
+                site = (o + 1) // 2
+                for t in nbr[site - 1]:
+                    if t == 0:
+                        continue
+                    to = 2 * t - 1 if o % 2 == 1 else 2 * t
+                    if to in occ:
+                        continue
+                    nd = encode_det(sorted((occ - {o}) | {to}), spec.nbasis)
+                    if tuple(nd) not in seen:
+                        seen.add(tuple(nd))
+                        nxt.append(nd)
+        front = nxt
+    return np.array(sorted(seen), dtype=np.int64)
+
+
 def assert_stats(a, b, S, exact=True):
     for f in EXACT_FIELDS:
         np.testing.assert_array_equal(a[f][:S], b[f][:S], err_msg=f)
@@ -278,3 +304,40 @@ def test_hash_full_is_an_error():
     with pytest.raises(NgpuError) as e:
         eng.upload(eng.rows(dets, np.ones((len(dets), 1))))
     assert e.value.code == 6
+
+
+@pytest.mark.parametrize("name,nrep,fp32", [("hubbard10", 1, False), ("hubbard10", 2, False),
+                                             ("mol8o6e_unif_fast", 1, False), ("mol8o6e_unif_fast", 1, True),
+                                             ("mol33o4e_w2", 1, False)])
+def test_semistochastic_iterations_match_oracle(name, nrep, fp32):
+    """Iterations with a semistochastic core space (deterministic projection of
+    the core amplitudes, core-to-core spawns cancelled): identical walker sets
+    and signs after every iteration.  The core projection sums in the same
+    fixed order on both sides, so amplitudes agree bit for bit; statistics that
+    sum real amplitudes in a different order agree to 1e-10."""
+    tau = 0.01 if name.startswith("hub") else 0.02
+    spec = SPECS[name](init_walkers=4000, n_replicas=nrep, seed=31)
+    eng = engine(spec)
+    orc = oracle(spec, tau)
+    n0 = 30.0 if name.startswith("hub") else 400.0
+    eng.start_single_part(n0)
+    orc.add_walkers(orc.ref[None, :], np.full((1, nrep), n0), [(1 << nrep) - 1])
+    rng = np.random.default_rng(8)
+    if name.startswith("hub"):
+        core = hopping_neighbourhood(spec, orc.ref, 2)
+    else:
+        core = np.unique(np.concatenate([orc.ref[None, :], random_dets(spec, 150, rng)]), axis=0)
+    nnz = eng.set_core(core, fp32=fp32)
+    assert nnz == orc.set_core(core, fp32=fp32) and nnz > 2 * len(core)
+    shift = np.linspace(0.2, 0.4, nrep)
+    for it in range(1, 31):
+        a = eng.iterate(it, tau, shift)
+        b = orc.iterate(it, shift)
+        assert b["rc"] == 0
+        for f in EXACT_FIELDS + ["enum_cyc"]:
+            np.testing.assert_allclose(a[f][:nrep], b[f][:nrep], rtol=1e-10, atol=1e-9, err_msg=f)
+        for f in ["n_valid_excits", "n_invalid_excits", "tot_walkers", "n_attempts", "n_spawned",
+                  "n_unique", "n_new"]:
+            assert a[f] == b[f], (it, f, a[f], b[f])
+    n = assert_same_walkers(eng, orc)
+    assert n > len(core)

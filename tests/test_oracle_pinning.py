@@ -356,3 +356,41 @@ def test_merge_and_annihilation_against_groupby(nrep):
             if not (int(f) >> r) & 1:
                 aborted[r] += abs(s[r])
     np.testing.assert_array_equal(st["naborted"][:nrep], aborted)
+
+
+# ------------------------------------------------ semistochastic core space
+@pytest.mark.parametrize("fp32", [False, True])
+def test_core_hamiltonian_and_projection_against_dense(fp32):
+    """The oracle's core Hamiltonian (CSR, diagonal H_ii - Hii) against the
+    second-quantised matrix of the same determinants, and one deterministic
+    projection y = tau*(S*x - (H - Hii)*x) (semi_stoch_procs.F90:105-241)
+    against a dense numpy product."""
+    sysm = HubbardChain(6, 6, 3.0, bhub=-1.0)
+    spec = Spec(sysm, init_walkers=1000, n_replicas=2)
+    o = oc.Oracle(spec, 0.01)
+    dets = ed.hubbard_dets(6, 3, 3)
+    H = ed.hubbard_hamiltonian(6, sysm.nbr, 3.0, -1.0, dets).toarray()
+    rng = np.random.default_rng(4)
+    pick = np.sort(rng.choice(len(dets), 120, replace=False))
+    core = np.array([encode_det(decode_det(np.array([dets[k]], dtype=np.int64)), 12) for k in pick])
+    nnz = o.set_core(core, fp32=fp32)
+    rp, cols, vals = o.core_csr()
+    assert nnz == len(cols)
+    hc = H[np.ix_(pick, pick)] - o.hii * np.eye(len(pick))
+    dense = np.zeros_like(hc)
+    for i in range(len(pick)):
+        assert np.all(np.diff(cols[rp[i]:rp[i + 1]]) > 0)    # ascending columns
+        dense[i, cols[rp[i]:rp[i + 1]]] = vals[rp[i]:rp[i + 1]]
+    expect = hc.astype(np.float32).astype(np.float64) if fp32 else hc
+    np.testing.assert_array_equal(dense, expect)
+    x = rng.normal(size=(len(pick), 2))
+    shift = np.array([0.3, -0.2])
+    y = o.core_project(x, shift, 0.01)
+    ref = 0.01 * (x * shift[None, :] - dense @ x)
+    np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-14)
+    # the core dets are resident, flagged deterministic and initiator in every replica
+    d, s, f, _, _ = o.walkers()
+    have = {tuple(r) for r in d}
+    assert all(tuple(c) in have for c in core.reshape(-1, 1))
+    fl = {tuple(r): int(x) for r, x in zip(d, f)}
+    assert all(fl[tuple(c)] & (1 << 16) and fl[tuple(c)] & 3 == 3 for c in core.reshape(-1, 1))
