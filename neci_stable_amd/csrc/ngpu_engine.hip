@@ -73,9 +73,7 @@ struct ngpu_engine {
   double* core_y = nullptr;
   // spawns, sort, merge, new
   int64_t scap = 0;
-  uint64_t* sp_key = nullptr;
-  double* sp_val = nullptr;
-  uint32_t* sp_info = nullptr;
+  uint64_t* sp_rec = nullptr;  // spawned list, kRecWords words per record
   uint64_t* sk[2] = {nullptr, nullptr};
   uint32_t* si[2] = {nullptr, nullptr};
   uint32_t* hist = nullptr;
@@ -239,7 +237,7 @@ template <int W, int R>
 void sort_and_merge(ngpu_t* h) {
   const unsigned long long* n_ptr = &h->cnt->n_spawned;
   if (h->rank_sort) {
-    launch(h, 4, -1, k_rank_hist<W>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_key, n_ptr, h->scap,
+    launch(h, 4, -1, k_rank_hist<W>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec, n_ptr, h->scap,
            (const uint32_t*)h->d_binom, h->sys.nel + 1, h->rk, h->hist);
     const int passes = (h->rank_bits + 7) / 8;
     const uint32_t* kin = h->rk;
@@ -248,19 +246,19 @@ void sort_and_merge(ngpu_t* h) {
     for (int p = 0; p < passes; ++p) {
       int out = p & 1;
       if (p > 0)
-        launch(h, -1, -1, k_radix_hist<uint32_t, 1, false>, dim3(h->nb_chunk), dim3(kBlock), kin, n_ptr, h->scap,
+        launch(h, -1, -1, k_radix_hist<uint32_t, 1, false>, dim3(h->nb_chunk), dim3(kBlock), kin, 1, n_ptr, h->scap,
                0, 8 * p, h->hist);
       launch(h, -1, -1, k_radix_scan, dim3(kRadixBins), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
       launch(h, -1, p + 1 == passes ? 4 : -1, k_radix_scatter<uint32_t, 1, false>, dim3(h->nb_chunk), dim3(kBlock),
-             kin, iin, (uint32_t*)h->sk[out], h->si[out], n_ptr, h->scap, 0, 8 * p, (const uint32_t*)h->hist,
+             kin, 1, iin, (uint32_t*)h->sk[out], h->si[out], n_ptr, h->scap, 0, 8 * p, (const uint32_t*)h->hist,
              (const uint32_t*)h->digit_tot);
       kin = (const uint32_t*)h->sk[out];
       iin = h->si[out];
       cur = out;
     }
     launch(h, 5, 5, k_merge<W, R, uint32_t, 1>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->cnt, h->dpart,
-           (const uint32_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_key,
-           (const double*)h->sp_val, (const uint32_t*)h->sp_info, h->scap, h->u_key, h->u_sgn, h->u_flag);
+           (const uint32_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_rec, h->scap, h->u_key,
+           h->u_sgn, h->u_flag);
     return;
   }
   // determinant keys in ilut_lt order: least significant word first
@@ -272,24 +270,26 @@ void sort_and_merge(ngpu_t* h) {
     for (int sh = 0; sh < nbits_lo; sh += 8) passes.push_back({1, sh});
   for (int sh = 0; sh < nbits_hi; sh += 8) passes.push_back({0, sh});
   int cur = 0;
-  const uint64_t* kin = h->sp_key;
+  const uint64_t* kin = h->sp_rec;
+  int kstride = kRecWords;  // the first pass reads the keys inside the records
   const uint32_t* iin = nullptr;
   for (size_t p = 0; p < passes.size(); ++p) {
     int out = (int)(p & 1);
     bool first = p == 0, last = p + 1 == passes.size();
-    launch(h, first ? 4 : -1, -1, k_radix_hist<uint64_t, W, W == 2>, dim3(h->nb_chunk), dim3(kBlock), kin, n_ptr,
-           h->scap, passes[p].word, passes[p].shift, h->hist);
+    launch(h, first ? 4 : -1, -1, k_radix_hist<uint64_t, W, W == 2>, dim3(h->nb_chunk), dim3(kBlock), kin, kstride,
+           n_ptr, h->scap, passes[p].word, passes[p].shift, h->hist);
     launch(h, -1, -1, k_radix_scan, dim3(kRadixBins), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
-    launch(h, -1, last ? 4 : -1, k_radix_scatter<uint64_t, W, W == 2>, dim3(h->nb_chunk), dim3(kBlock), kin, iin,
+    launch(h, -1, last ? 4 : -1, k_radix_scatter<uint64_t, W, W == 2>, dim3(h->nb_chunk), dim3(kBlock), kin, kstride, iin,
            h->sk[out], h->si[out], n_ptr, h->scap, passes[p].word, passes[p].shift, (const uint32_t*)h->hist,
            (const uint32_t*)h->digit_tot);
     kin = h->sk[out];
+    kstride = W;
     iin = h->si[out];
     cur = out;
   }
   launch(h, 5, 5, k_merge<W, R, uint64_t, W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->cnt, h->dpart,
-         (const uint64_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_key,
-         (const double*)h->sp_val, (const uint32_t*)h->sp_info, h->scap, h->u_key, h->u_sgn, h->u_flag);
+         (const uint64_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_rec, h->scap, h->u_key,
+         h->u_sgn, h->u_flag);
 }
 
 // semistochastic projection of the core amplitudes (before spawning)
@@ -342,11 +342,11 @@ void iterate_chain(ngpu_t* h, const ngpu_iter_in_t* in) {
   launch(h, -1, 1, k_scan_apply, dim3(h->nb_chunk), dim3(kBlock), (const int32_t*)h->att_cnt, h->att_off,
          (const int64_t*)h->blk_part, h->cnt, h->S, h->tile_p, h->tcap);
   launch(h, 2, 2, k_spawn<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
-         (const int64_t*)h->att_off, (const int64_t*)h->tile_p, h->tcap, (const uint8_t*)h->pinfo, h->sp_key,
-         h->sp_val, h->sp_info, h->scap, h->sq, in->tau, in->iter);
+         (const int64_t*)h->att_off, (const int64_t*)h->tile_p, h->tcap, (const uint8_t*)h->pinfo, h->sp_rec,
+         h->scap, h->sq, in->tau, in->iter);
   if (h->sys.model == NGPU_MODEL_MOLECULAR)
     launch(h, 3, 3, k_spawn_singles<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
-           (const uint8_t*)h->pinfo, h->sp_key, h->sp_val, h->sp_info, h->scap, (const int64_t*)h->sq, in->tau,
+           (const uint8_t*)h->pinfo, h->sp_rec, h->scap, (const int64_t*)h->sq, in->tau,
            in->iter);
   annihilation_chain<W, R>(h, in->iter, true);
 }
@@ -621,9 +621,7 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
   // spawn capacity (beyond it the spawn kernel searches the offsets itself)
   h->tcap = (4 * scap) / kBlock + 2;
   fail(dalloc(&h->tile_p, (size_t)h->tcap));
-  fail(dalloc(&h->sp_key, (size_t)scap * W));
-  fail(dalloc(&h->sp_val, (size_t)scap));
-  fail(dalloc(&h->sp_info, (size_t)scap));
+  fail(dalloc(&h->sp_rec, (size_t)scap * kRecWords));
   for (int q = 0; q < 2; ++q) {
     fail(dalloc(&h->sk[q], (size_t)scap * W));
     fail(dalloc(&h->si[q], (size_t)scap));
@@ -712,7 +710,7 @@ int ngpu_destroy(ngpu_t* h) {
   if (!h) return NGPU_ERR_ARG;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void* ptrs[] = {h->wk.keys, h->wk.sgn, h->wk.flags, h->wk.diagH, h->wk.offdiagH, h->wk.hpos, h->wk.freeq,
-                  h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->sp_key, h->sp_val, h->sp_info,
+                  h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->sp_rec,
                   h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->digit_tot, h->sq, h->u_key, h->u_sgn, h->u_flag, h->n_key,
                   h->n_sgn, h->n_flag, h->cnt, h->dout, h->dpart, h->dcyc, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_binom, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
                   h->core_vals, h->core_x, h->core_y, h->d_ab_tab,
@@ -1334,9 +1332,15 @@ int ngpu_annihilate(ngpu_t* h, uint32_t iter, const int64_t* sdet, const double*
   }
   const int W = h->W, S = h->S;
   if (n) {
-    HIP_OK(hipMemcpyAsync(h->sp_key, sdet, (size_t)n * W * 8, hipMemcpyHostToDevice, h->stream));
-    HIP_OK(hipMemcpyAsync(h->sp_val, sval, (size_t)n * 8, hipMemcpyHostToDevice, h->stream));
-    HIP_OK(hipMemcpyAsync(h->sp_info, sinfo, (size_t)n * 4, hipMemcpyHostToDevice, h->stream));
+    std::vector<uint64_t> rec((size_t)n * kRecWords, 0);
+    for (int64_t i = 0; i < n; ++i) {
+      for (int w = 0; w < W; ++w) rec[(size_t)i * kRecWords + w] = (uint64_t)sdet[i * W + w];
+      uint64_t vb;
+      memcpy(&vb, &sval[i], 8);
+      rec[(size_t)i * kRecWords + 2] = vb;
+      rec[(size_t)i * kRecWords + 3] = sinfo[i];
+    }
+    HIP_OK(hipMemcpy(h->sp_rec, rec.data(), rec.size() * 8, hipMemcpyHostToDevice));
   }
   hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dpart, h->cnt);
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_spawned, (unsigned long long)n);

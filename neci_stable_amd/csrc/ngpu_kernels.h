@@ -559,20 +559,30 @@ struct QueueStage {  // queued attempts as (parent pair, attempt number)
   uint32_t k[kQueueStage];
 };
 
+// A spawned-list record is kRecWords 64-bit words (32 bytes, so a record
+// never straddles a cache line): the determinant's words, the child amplitude
+// (FP64 bits) in word 2 and the info word (bits 0-7 replica, bit 8 parent
+// initiator) in word 3.  The merge reads one record per spawn.
+constexpr int kRecWords = 4;
+__device__ inline double rec_val(const uint64_t* r) { return __longlong_as_double((long long)r[2]); }
+__device__ inline uint32_t rec_info(const uint64_t* r) { return (uint32_t)r[3]; }
+
 // block-wide: drain staged spawns into the spawned list when due
 template <int W>
-__device__ inline void spawn_flush(SpawnStage<W>* ss, Counters* cnt, uint64_t* sp_key, double* sp_val,
-                                   uint32_t* sp_info, int64_t scap, bool force, bool& overflow) {
+__device__ inline void spawn_flush(SpawnStage<W>* ss, Counters* cnt, uint64_t* sp_rec, int64_t scap, bool force,
+                                   bool& overflow) {
   uint32_t n;
   unsigned long long base;
   if (!stage_flush_begin(&ss->st, &cnt->n_spawned, kSpawnStage, force, n, base)) return;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
     unsigned long long d = base + i;
     if (d < (unsigned long long)scap) {
+      uint64_t* r = sp_rec + d * kRecWords;
 #pragma unroll
-      for (int w = 0; w < W; ++w) sp_key[d * W + w] = ss->key[i * W + w];
-      sp_val[d] = ss->val[i];
-      sp_info[d] = ss->info[i];
+      for (int w = 0; w < W; ++w) r[w] = ss->key[i * W + w];
+      if (W == 1) r[1] = 0;
+      r[2] = (uint64_t)__double_as_longlong(ss->val[i]);
+      r[3] = ss->info[i];
     } else {
       overflow = true;
     }
@@ -727,8 +737,8 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* 
 template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* cnt, double* part,
                                                   const int64_t* att_off, const int64_t* tile_p, int64_t tcap,
-                                                  const uint8_t* pinfo, uint64_t* sp_key, double* sp_val,
-                                                  uint32_t* sp_info, int64_t scap, int64_t* sq, double tau,
+                                                  const uint8_t* pinfo, uint64_t* sp_rec, int64_t scap,
+                                                  int64_t* sq, double tau,
                                                   uint32_t iter) {
   __shared__ double acc_lds[kStatStride];
   __shared__ SpawnStage<W> ss;
@@ -790,10 +800,10 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
       k = (uint32_t)(a - att_off[p]);
     }
     spawn_attempt<W, R>(s, wk, cnt, bs, acc, pinfo, act, p, k, mode, &ss, &qs, tau, iter);
-    spawn_flush<W>(&ss, cnt, sp_key, sp_val, sp_info, scap, false, overflow);
+    spawn_flush<W>(&ss, cnt, sp_rec, scap, false, overflow);
     if (mode == 1) queue_flush(&qs, cnt, sq, scap, false, overflow);
   }
-  spawn_flush<W>(&ss, cnt, sp_key, sp_val, sp_info, scap, true, overflow);
+  spawn_flush<W>(&ss, cnt, sp_rec, scap, true, overflow);
   if (mode == 1) queue_flush(&qs, cnt, sq, scap, true, overflow);
   if (overflow) atomicOr(&cnt->err, 1u << NGPU_ERR_SPAWN_FULL);
   acc.done(bs, mode);
@@ -804,22 +814,24 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
 // attempt.  gen_single_heatbath's two sequential sweeps over the same-spin
 // orbitals b need |H(src -> b)| for every b; lane g of a group evaluates the
 // orbitals b = g+1, g+1+kHbGroup, ... from the (Coulomb, exchange) rows
-// jk[a][j][*], so the group's loads for one occupied orbital j are one
-// contiguous 16*kHbGroup-byte run instead of kHbGroup scattered UMAT entries.
-// Each value is summed in sltcnd_1's order (Coulomb over occupied orbitals,
-// then exchange over same-spin ones, then the one-body term), and both sweeps
-// run in ascending b on every lane with values fetched by shuffles, so the
-// selected orbital, pgen and H_ij are bit-identical to the one-lane generator.
-constexpr int kHbGroup = 16;
-constexpr int kHbPerLane = 4;  // spatial orbitals per lane: nsp <= 64
+// jk[a][j][*] (contiguous in b, so a group's loads for one occupied orbital j
+// are one contiguous run).  Each value is summed in sltcnd_1's order (Coulomb
+// over the occupied orbitals, then exchange over the same-spin ones, then the
+// one-body term) and stored in LDS; both sweeps then run in ascending b over
+// LDS on every lane of the group, so the selected orbital, pgen and H_ij are
+// bit-identical to the one-lane generator.  The three Philox blocks of the
+// attempt are computed by three different lanes of the group and shared.
+constexpr int kHbGroup = 4;
+constexpr int kHbPerLane = 16;  // spatial orbitals per lane: nsp <= 64
+constexpr int kHbGroups = kBlock / kHbGroup;
 
 template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Counters* cnt, double* part,
-                                                          const uint8_t* pinfo, uint64_t* sp_key, double* sp_val,
-                                                          uint32_t* sp_info, int64_t scap, const int64_t* sq,
-                                                          double tau, uint32_t iter) {
+                                                          const uint8_t* pinfo, uint64_t* sp_rec, int64_t scap,
+                                                          const int64_t* sq, double tau, uint32_t iter) {
   __shared__ double acc_lds[kStatStride];
   __shared__ SpawnStage<W> ss;
+  __shared__ double hv[kHbGroups][64];  // signed H(src -> b) of each group's attempt
   BlockStats bs{acc_lds};
   bs.zero();
   stage_init(&ss.st);
@@ -829,11 +841,14 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
   bool overflow = false;
   SpawnAcc<R> acc;
   const int nsp = s.nbasis / 2;
+  const int mcount = (nsp + kHbGroup - 1) / kHbGroup;  // wave-uniform
   const int g = threadIdx.x % kHbGroup;
+  const int gl = threadIdx.x / kHbGroup;                // group within the block
   const int gbase = lane_id() - lane_id() % kHbGroup;
   const bool leader = g == 0;
-  const int64_t groups = (int64_t)gridDim.x * (blockDim.x / kHbGroup);
-  const int64_t grp = (int64_t)blockIdx.x * (blockDim.x / kHbGroup) + threadIdx.x / kHbGroup;
+  double* myv = hv[gl];
+  const int64_t groups = (int64_t)gridDim.x * kHbGroups;
+  const int64_t grp = (int64_t)blockIdx.x * kHbGroups + gl;
   const int64_t trips = (nq + groups - 1) / groups;
   for (int64_t t = 0; t < trips; ++t) {
     int64_t q = t * groups + grp;
@@ -845,11 +860,12 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
     uint64_t key[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) key[w] = act ? wk.keys[slot * W + w] : s.ref[w];  // idle groups: any valid det
-    double d0, d1, d2, d3, d4, d5;
-    draw2<W>(s.seed, key, iter, r, k, 1, d0, d1);
-    draw2<W>(s.seed, key, iter, r, k, 2, d2, d3);
-    draw2<W>(s.seed, key, iter, r, k, 3, d4, d5);
-    (void)d0; (void)d3; (void)d4;
+    // lane g < 3 draws Philox block (purpose g+1); d1 = dsel, d2 = dorb, d5 = spawn rounding
+    double ua = 0.0, ub = 0.0;
+    if (g < 3) draw2<W>(s.seed, key, iter, r, k, (uint32_t)(g + 1), ua, ub);
+    const double d1 = __shfl(ub, gbase + 0);
+    const double d2 = __shfl(ua, gbase + 1);
+    const double d5 = __shfl(ub, gbase + 2);
     int elec = 1 + (int)floor(d1 * (double)s.nel);
     int src = select_orb<W>(key, ~0ull, elec);
     int a = spat(src), off = is_beta(src) ? 1 : 0;
@@ -864,12 +880,10 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
         int o = 64 * w + ctz64(x) + 1;
         x &= x - 1;
         if (o == src) continue;
-        const double* rj = row + (size_t)2 * (spat(o) - 1) * nsp;
+        const double* rj = row + (size_t)2 * (spat(o) - 1) * nsp + 2 * g;
 #pragma unroll
-        for (int m = 0; m < kHbPerLane; ++m) {
-          int b0 = g + m * kHbGroup;
-          if (b0 < nsp) v[m] += rj[2 * b0];
-        }
+        for (int m = 0; m < kHbPerLane; ++m)
+          if (m < mcount && g + m * kHbGroup < nsp) v[m] += rj[2 * m * kHbGroup];
       }
     }
     uint64_t same = off ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
@@ -880,44 +894,34 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
         int o = 64 * w + ctz64(x) + 1;
         x &= x - 1;
         if (o == src) continue;
-        const double* rj = row + (size_t)2 * (spat(o) - 1) * nsp;
+        const double* rj = row + (size_t)2 * (spat(o) - 1) * nsp + 2 * g + 1;
 #pragma unroll
-        for (int m = 0; m < kHbPerLane; ++m) {
-          int b0 = g + m * kHbGroup;
-          if (b0 < nsp) v[m] -= rj[2 * b0 + 1];
-        }
+        for (int m = 0; m < kHbPerLane; ++m)
+          if (m < mcount && g + m * kHbGroup < nsp) v[m] -= rj[2 * m * kHbGroup];
       }
     }
 #pragma unroll
     for (int m = 0; m < kHbPerLane; ++m) {
       int b = g + m * kHbGroup + 1;
-      if (b <= nsp) {
+      if (m < mcount && b <= nsp) {
         int orb = 2 * b - off;
-        v[m] = test_orb<W>(key, orb) ? 0.0 : v[m] + T(s, src, orb);
+        myv[b - 1] = test_orb<W>(key, orb) ? 0.0 : v[m] + T(s, src, orb);
       }
     }
+    __syncthreads();
     // sweep 1: normalisation; sweep 2: the first b whose running sum reaches r
     double cum_sum = 0.0;
-    for (int b = 1; b <= nsp; ++b) {
-      int m = (b - 1) / kHbGroup;
-      double mine = m == 0 ? v[0] : m == 1 ? v[1] : m == 2 ? v[2] : v[3];
-      cum_sum += fabs(__shfl(mine, gbase + (b - 1) % kHbGroup));
-    }
+    for (int b = 0; b < nsp; ++b) cum_sum += fabs(myv[b]);
     double rr = d2 * cum_sum;
     double cum = 0.0, cpt = 0.0, vsel = 0.0;
     int bsel = 0;
-    bool found = false;
     for (int b = 1; b <= nsp; ++b) {
-      int m = (b - 1) / kHbGroup;
-      double mine = m == 0 ? v[0] : m == 1 ? v[1] : m == 2 ? v[2] : v[3];
-      double vb = __shfl(mine, gbase + (b - 1) % kHbGroup);
-      if (!found) {
-        vsel = vb;
-        cpt = fabs(vb);
-        bsel = b;
-        cum += cpt;
-        found = !(cum < rr) || b == nsp;
-      }
+      double vb = myv[b - 1];
+      vsel = vb;
+      cpt = fabs(vb);
+      bsel = b;
+      cum += cpt;
+      if (!(cum < rr) || b == nsp) break;
     }
     bool mine_att = act && leader;
     bool emit = false;
@@ -976,9 +980,9 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
       ss.val[idx] = nspawn;
       ss.info[idx] = info;
     }
-    spawn_flush<W>(&ss, cnt, sp_key, sp_val, sp_info, scap, false, overflow);
+    spawn_flush<W>(&ss, cnt, sp_rec, scap, false, overflow);  // its barriers also order the reuse of hv
   }
-  spawn_flush<W>(&ss, cnt, sp_key, sp_val, sp_info, scap, true, overflow);
+  spawn_flush<W>(&ss, cnt, sp_rec, scap, true, overflow);
   if (overflow) atomicOr(&cnt->err, 1u << NGPU_ERR_SPAWN_FULL);
   acc.done(bs, 2);
   bs.flush(part);
@@ -1004,7 +1008,7 @@ __device__ inline uint32_t radix_digit(const K* k, int word, int shift) {
 }
 
 template <typename K, int NW, bool FLIP>
-__global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, const unsigned long long* n_ptr,
+__global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, int kstride, const unsigned long long* n_ptr,
                                                        int64_t ncap, int word, int shift, uint32_t* hist) {
   __shared__ uint32_t h[kRadixBins];
   for (int i = threadIdx.x; i < kRadixBins; i += blockDim.x) h[i] = 0;
@@ -1014,13 +1018,13 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, const unsi
   int64_t begin, end;
   chunk_of(n, begin, end);
   for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x)
-    atomicAdd(&h[radix_digit<K, NW, FLIP>(keys + i * NW, word, shift)], 1u);
+    atomicAdd(&h[radix_digit<K, NW, FLIP>(keys + i * kstride, word, shift)], 1u);
   __syncthreads();
   for (int d = threadIdx.x; d < kRadixBins; d += blockDim.x) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
 }
 
-// colex rank of each spawned determinant (binom[p*(nel+1)+i] = C(p, i)),
-// fused with the histogram of its lowest digit
+// colex rank of each spawned record's determinant (binom[p*(nel+1)+i] =
+// C(p, i)), fused with the histogram of its lowest digit
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, const unsigned long long* n_ptr,
                                                       int64_t ncap, const uint32_t* binom, int nel1,
@@ -1037,7 +1041,7 @@ __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, cons
     int c = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      uint64_t x = keys[i * W + w];
+      uint64_t x = keys[i * kRecWords + w];
       while (x) {
         int p = 64 * w + ctz64(x);
         x &= x - 1;
@@ -1072,20 +1076,32 @@ __global__ void __launch_bounds__(kBlock) k_radix_scan(uint32_t* hist, int nb, u
   if (threadIdx.x == 0) digit_tot[blockIdx.x] = (uint32_t)tot;
 }
 
+// Stable scatter of one digit, in tiles of kRadixItems*kBlock elements: the
+// tile is ranked by digit (ballot peers within a wave, LDS counts across waves
+// and rounds, in element order), reordered by digit in LDS, then written out
+// so that consecutive threads write consecutive positions of each digit's run.
+constexpr int kRadixItems = 8;
+constexpr int kRadixTile = kRadixItems * kBlock;
+
 template <typename K, int NW, bool FLIP>
-__global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, const uint32_t* iin, K* kout,
+__global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstride, const uint32_t* iin, K* kout,
                                                           uint32_t* iout, const unsigned long long* n_ptr,
                                                           int64_t ncap, int word, int shift, const uint32_t* hist,
                                                           const uint32_t* digit_tot) {
-  __shared__ uint32_t run[kRadixBins];
+  __shared__ uint32_t run[kRadixBins];     // next global position of each digit (this block)
+  __shared__ uint32_t cnt[kRadixBins];     // digit counts of the tile so far
+  __shared__ uint32_t tstart[kRadixBins];  // start of each digit inside the reordered tile
   __shared__ uint32_t wcnt[kBlock / kWave][kRadixBins];
+  __shared__ K lk[kRadixTile * NW];
+  __shared__ uint32_t lv[kRadixTile];
   __shared__ int64_t red[kBlock / kWave];
-  int wv = threadIdx.x / kWave;
+  const int wv = threadIdx.x / kWave;
   {
     // digit bases: exclusive scan of the digit totals (blockDim == kRadixBins)
     int64_t tot;
     int64_t base = block_exclusive_scan((int64_t)digit_tot[threadIdx.x], red, tot);
     run[threadIdx.x] = (uint32_t)base + hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x];
+    cnt[threadIdx.x] = 0;
     for (int q = 0; q < kBlock / kWave; ++q) wcnt[q][threadIdx.x] = 0;
   }
   __syncthreads();
@@ -1093,40 +1109,77 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, const ui
   if (n > ncap) n = ncap;
   int64_t begin, end;
   chunk_of(n, begin, end);
-  uint64_t ltmask = lanemask_lt();
-  for (int64_t base = begin; base < end; base += blockDim.x) {
-    int64_t i = base + threadIdx.x;
-    bool valid = i < end;
-    K k[NW];
-    uint32_t idx = 0, d = 0;
-    if (valid) {
+  const uint64_t ltmask = lanemask_lt();
+  for (int64_t tb = begin; tb < end; tb += kRadixTile) {
+    K k[kRadixItems][NW];
+    uint32_t v[kRadixItems], d[kRadixItems], off[kRadixItems];
 #pragma unroll
-      for (int w = 0; w < NW; ++w) k[w] = kin[i * NW + w];
-      idx = iin ? iin[i] : (uint32_t)i;
-      d = radix_digit<K, NW, FLIP>(k, word, shift);
-    }
-    uint64_t peers = __ballot(valid);
+    for (int j = 0; j < kRadixItems; ++j) {
+      int64_t e = tb + (int64_t)j * kBlock + threadIdx.x;
+      bool valid = e < end;
+      d[j] = 0;
+      v[j] = 0;
+      if (valid) {
 #pragma unroll
-    for (int bit = 0; bit < 8; ++bit) {
-      uint64_t m = __ballot(valid && ((d >> bit) & 1u));
-      peers &= ((d >> bit) & 1u) ? m : ~m;
+        for (int w = 0; w < NW; ++w) k[j][w] = kin[e * kstride + w];
+        v[j] = iin ? iin[e] : (uint32_t)e;
+        d[j] = radix_digit<K, NW, FLIP>(k[j], word, shift);
+      }
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (int bit = 0; bit < 8; ++bit) {
+        uint64_t m = __ballot(valid && ((d[j] >> bit) & 1u));
+        peers &= ((d[j] >> bit) & 1u) ? m : ~m;
+      }
+      int rank = popc64(peers & ltmask);
+      if (valid && rank == 0) wcnt[wv][d[j]] = (uint32_t)popc64(peers);
+      __syncthreads();
+      if (valid) {
+        uint32_t o = cnt[d[j]] + (uint32_t)rank;
+        for (int q = 0; q < wv; ++q) o += wcnt[q][d[j]];
+        off[j] = o;
+      } else {
+        off[j] = 0xFFFFFFFFu;
+      }
+      __syncthreads();
+      {
+        uint32_t sum = 0;
+        for (int q = 0; q < kBlock / kWave; ++q) { sum += wcnt[q][threadIdx.x]; wcnt[q][threadIdx.x] = 0; }
+        cnt[threadIdx.x] += sum;
+      }
+      __syncthreads();
     }
-    int rank = popc64(peers & ltmask);
-    if (valid && rank == 0) wcnt[wv][d] = (uint32_t)popc64(peers);
+    // digit starts inside the tile
+    {
+      int64_t tot;
+      int64_t st = block_exclusive_scan((int64_t)cnt[threadIdx.x], red, tot);
+      tstart[threadIdx.x] = (uint32_t)st;
+    }
     __syncthreads();
-    if (valid) {
-      uint32_t pos = run[d] + (uint32_t)rank;
-      for (int q = 0; q < wv; ++q) pos += wcnt[q][d];
 #pragma unroll
-      for (int w = 0; w < NW; ++w) kout[(size_t)pos * NW + w] = k[w];
-      iout[pos] = idx;
+    for (int j = 0; j < kRadixItems; ++j) {
+      if (off[j] != 0xFFFFFFFFu) {
+        uint32_t lp = tstart[d[j]] + off[j];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) lk[lp * NW + w] = k[j][w];
+        lv[lp] = v[j];
+      }
     }
     __syncthreads();
-    for (int dd = threadIdx.x; dd < kRadixBins; dd += blockDim.x) {
-      uint32_t sum = 0;
-      for (int q = 0; q < kBlock / kWave; ++q) { sum += wcnt[q][dd]; wcnt[q][dd] = 0; }
-      run[dd] += sum;
+    const int tn = (int)((end - tb) < kRadixTile ? (end - tb) : kRadixTile);
+    for (int i = threadIdx.x; i < tn; i += blockDim.x) {
+      K kk[NW];
+#pragma unroll
+      for (int w = 0; w < NW; ++w) kk[w] = lk[i * NW + w];
+      uint32_t dd = radix_digit<K, NW, FLIP>(kk, word, shift);
+      uint32_t gp = run[dd] + (uint32_t)i - tstart[dd];
+#pragma unroll
+      for (int w = 0; w < NW; ++w) kout[(size_t)gp * NW + w] = kk[w];
+      iout[gp] = lv[i];
     }
+    __syncthreads();
+    run[threadIdx.x] += cnt[threadIdx.x];
+    cnt[threadIdx.x] = 0;
     __syncthreads();
   }
 }
@@ -1185,10 +1238,8 @@ __device__ inline bool key_eq(const K* a, const K* b) {
 
 template <int W, int R, typename K, int NW>
 __global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, double* part, const K* skey,
-                                                  const uint32_t* sidx, const uint64_t* sp_key,
-                                                  const double* sp_val, const uint32_t* sp_info,
-                                                  int64_t scap, uint64_t* u_key, double* u_sgn,
-                                                  uint32_t* u_flag) {
+                                                  const uint32_t* sidx, const uint64_t* sp_rec, int64_t scap,
+                                                  uint64_t* u_key, double* u_sgn, uint32_t* u_flag) {
   __shared__ double acc_lds[kStatStride];
   __shared__ ListStage<W, R> ls;
   BlockStats bs{acc_lds};
@@ -1213,21 +1264,22 @@ __global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, doubl
       if (head) {
         int64_t j = i + 1;
         while (j < n && key_eq<K, NW>(ki, skey + j * NW)) ++j;
-        const uint64_t* kd = sp_key + (size_t)sidx[i] * W;
+        const uint64_t* kd = sp_rec + (size_t)sidx[i] * kRecWords;
 #pragma unroll
         for (int w = 0; w < W; ++w) key[w] = kd[w];
         for (int r = 0; r < S; ++r) sum[r] = 0.0;
         if (j == i + 1) {
-          uint32_t inf = sp_info[sidx[i]];
+          uint32_t inf = rec_info(kd);
           int r = (int)(inf & 0xFFu);
-          sum[r] = sp_val[sidx[i]];
+          sum[r] = rec_val(kd);
           if (inf & kInfoInit) flag |= 1u << r;
           keep = !(fabs(sum[r]) < 1.e-12);
         } else {
           for (int64_t q = i; q < j; ++q) {
-            uint32_t inf = sp_info[sidx[q]];
+            const uint64_t* rq = sp_rec + (size_t)sidx[q] * kRecWords;
+            uint32_t inf = rec_info(rq);
             int r = (int)(inf & 0xFFu);
-            double nv = sp_val[sidx[q]];
+            double nv = rec_val(rq);
             double cv = sum[r];
             if (s.t_init) {
               if ((fabs(cv) > 1.e-12 && fabs(nv) > 1.e-12) || (inf & kInfoInit)) flag |= 1u << r;
