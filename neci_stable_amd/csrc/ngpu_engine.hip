@@ -210,6 +210,19 @@ __global__ void k_set_count(unsigned long long* p, unsigned long long v) { *p = 
 // closes stage `close` its stop event (hipExtLaunchKernel records them at the
 // kernel's own start and end, not at a neighbouring command).
 template <typename F, typename... Args>
+void launch_lds(ngpu_t* h, int open, int close, F kern, dim3 grid, dim3 block, size_t lds, Args... args) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->prof && h->ev_ok) {
+    if (open >= 0) e0 = h->ev[h->ev_cur][open][0];
+    if (close >= 0) {
+      e1 = h->ev[h->ev_cur][close][1];
+      h->ev_mask[h->ev_cur] |= 1u << close;
+    }
+  }
+  hipExtLaunchKernelGGL(kern, grid, block, (uint32_t)lds, h->stream, e0, e1, 0, args...);
+}
+
+template <typename F, typename... Args>
 void launch(ngpu_t* h, int open, int close, F kern, dim3 grid, dim3 block, Args... args) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (h->prof && h->ev_ok) {
@@ -344,10 +357,15 @@ void iterate_chain(ngpu_t* h, const ngpu_iter_in_t* in) {
   launch(h, 2, 2, k_spawn<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
          (const int64_t*)h->att_off, (const int64_t*)h->tile_p, h->tcap, (const uint8_t*)h->pinfo, h->sp_rec,
          h->scap, h->sq, in->tau, in->iter);
-  if (h->sys.model == NGPU_MODEL_MOLECULAR)
-    launch(h, 3, 3, k_spawn_singles<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
-           (const uint8_t*)h->pinfo, h->sp_rec, h->scap, (const int64_t*)h->sq, in->tau,
-           in->iter);
+  if (h->sys.model == NGPU_MODEL_MOLECULAR) {
+    const int nsp = h->sys.nbasis / 2;
+    const size_t lds = sizeof(double) * (size_t)kHbGroups * nsp;
+    auto kern = nsp <= 4 * kHbGroup ? k_spawn_singles<W, R, 4>
+              : nsp <= 5 * kHbGroup ? k_spawn_singles<W, R, 5>
+              : nsp <= 8 * kHbGroup ? k_spawn_singles<W, R, 8> : k_spawn_singles<W, R, 16>;
+    launch_lds(h, 3, 3, kern, dim3(h->nb_grid), dim3(kBlock), lds, h->ds, h->wk, h->cnt, h->dpart,
+               (const uint8_t*)h->pinfo, h->sp_rec, h->scap, (const int64_t*)h->sq, in->tau, in->iter);
+  }
   annihilation_chain<W, R>(h, in->iter, true);
 }
 

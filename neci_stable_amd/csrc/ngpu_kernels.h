@@ -822,16 +822,48 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
 // bit-identical to the one-lane generator.  The three Philox blocks of the
 // attempt are computed by three different lanes of the group and shared.
 constexpr int kHbGroup = 4;
-constexpr int kHbPerLane = 16;  // spatial orbitals per lane: nsp <= 64
 constexpr int kHbGroups = kBlock / kHbGroup;
 
-template <int W, int R>
+// Accumulates sign * rows[b] into v for the occupied orbitals of x (ascending,
+// src removed), two orbitals per step so that both rows' loads are in flight
+// before the ordered additions.
+template <int PL>
+__device__ inline void hb_rows(uint64_t x, int base, const double* row, int nsp, int g, int mcount, double sign,
+                               double* v) {
+  while (x) {
+    int o1 = base + ctz64(x) + 1;
+    x &= x - 1;
+    bool two = x != 0;
+    int o2 = two ? base + ctz64(x) + 1 : o1;
+    if (two) x &= x - 1;
+    const double* r1 = row + (size_t)2 * (spat(o1) - 1) * nsp + 2 * g;
+    const double* r2 = row + (size_t)2 * (spat(o2) - 1) * nsp + 2 * g;
+    double a1[PL], a2[PL];
+#pragma unroll
+    for (int m = 0; m < PL; ++m) {
+      bool in = m < mcount && g + m * kHbGroup < nsp;
+      a1[m] = in ? r1[2 * m * kHbGroup] : 0.0;
+      a2[m] = (in && two) ? r2[2 * m * kHbGroup] : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < PL; ++m) {
+      if (m < mcount && g + m * kHbGroup < nsp) {
+        v[m] = sign > 0.0 ? v[m] + a1[m] : v[m] - a1[m];
+        if (two) v[m] = sign > 0.0 ? v[m] + a2[m] : v[m] - a2[m];
+      }
+    }
+  }
+}
+
+// PL: spatial orbitals per lane (nsp <= kHbGroup*PL); dynamic LDS holds the
+// kHbGroups*nsp values of the block's attempts
+template <int W, int R, int PL>
 __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Counters* cnt, double* part,
                                                           const uint8_t* pinfo, uint64_t* sp_rec, int64_t scap,
                                                           const int64_t* sq, double tau, uint32_t iter) {
   __shared__ double acc_lds[kStatStride];
   __shared__ SpawnStage<W> ss;
-  __shared__ double hv[kHbGroups][64];  // signed H(src -> b) of each group's attempt
+  extern __shared__ double hv_dyn[];  // signed H(src -> b) of each group's attempt
   BlockStats bs{acc_lds};
   bs.zero();
   stage_init(&ss.st);
@@ -846,7 +878,7 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
   const int gl = threadIdx.x / kHbGroup;                // group within the block
   const int gbase = lane_id() - lane_id() % kHbGroup;
   const bool leader = g == 0;
-  double* myv = hv[gl];
+  double* myv = hv_dyn + (size_t)gl * nsp;
   const int64_t groups = (int64_t)gridDim.x * kHbGroups;
   const int64_t grp = (int64_t)blockIdx.x * kHbGroups + gl;
   const int64_t trips = (nq + groups - 1) / groups;
@@ -870,38 +902,25 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
     int src = select_orb<W>(key, ~0ull, elec);
     int a = spat(src), off = is_beta(src) ? 1 : 0;
     const double* row = s.jk + (size_t)2 * (a - 1) * nsp * nsp;
-    double v[kHbPerLane];
+    double v[PL];
 #pragma unroll
-    for (int m = 0; m < kHbPerLane; ++m) v[m] = 0.0;
+    for (int m = 0; m < PL; ++m) v[m] = 0.0;
+    uint64_t same = off ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
+    // Coulomb terms over every occupied orbital but src, then exchange over the same-spin ones
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       uint64_t x = key[w];
-      while (x) {
-        int o = 64 * w + ctz64(x) + 1;
-        x &= x - 1;
-        if (o == src) continue;
-        const double* rj = row + (size_t)2 * (spat(o) - 1) * nsp + 2 * g;
-#pragma unroll
-        for (int m = 0; m < kHbPerLane; ++m)
-          if (m < mcount && g + m * kHbGroup < nsp) v[m] += rj[2 * m * kHbGroup];
-      }
+      if ((src - 1) >> 6 == w) x &= ~(1ull << ((src - 1) & 63));
+      hb_rows<PL>(x, 64 * w, row, nsp, g, mcount, 1.0, v);
     }
-    uint64_t same = off ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       uint64_t x = key[w] & same;
-      while (x) {
-        int o = 64 * w + ctz64(x) + 1;
-        x &= x - 1;
-        if (o == src) continue;
-        const double* rj = row + (size_t)2 * (spat(o) - 1) * nsp + 2 * g + 1;
-#pragma unroll
-        for (int m = 0; m < kHbPerLane; ++m)
-          if (m < mcount && g + m * kHbGroup < nsp) v[m] -= rj[2 * m * kHbGroup];
-      }
+      if ((src - 1) >> 6 == w) x &= ~(1ull << ((src - 1) & 63));
+      hb_rows<PL>(x, 64 * w, row + 1, nsp, g, mcount, -1.0, v);
     }
 #pragma unroll
-    for (int m = 0; m < kHbPerLane; ++m) {
+    for (int m = 0; m < PL; ++m) {
       int b = g + m * kHbGroup + 1;
       if (m < mcount && b <= nsp) {
         int orb = 2 * b - off;
