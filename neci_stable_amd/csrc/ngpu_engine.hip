@@ -332,14 +332,14 @@ void annihilation_chain(ngpu_t* h, uint32_t iter, bool with_core) {
   launch(h, 6, 6, k_annihilate<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
          (const uint64_t*)h->u_key, (const double*)h->u_sgn, (const uint32_t*)h->u_flag, h->n_key, h->n_sgn,
          h->n_flag, iter);
-  launch(h, 7, -1, k_insert<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt,
+  launch(h, 7, 7, k_insert<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt,
          (const uint64_t*)h->n_key, (const double*)h->n_sgn, (const uint32_t*)h->n_flag,
          (const unsigned long long*)&h->cnt->n_new);
-  launch(h, -1, 7, k_post_insert, dim3(1), dim3(1), h->cnt, (const unsigned long long*)&h->cnt->n_new,
+  launch(h, -1, -1, k_post_insert, dim3(1), dim3(1), h->cnt, (const unsigned long long*)&h->cnt->n_new,
          h->wk.cap);
-  launch(h, 8, -1, k_stats<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart, iter);
+  launch(h, 8, 8, k_stats<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart, iter);
   hash_upkeep<W>(h);
-  launch(h, -1, 8, k_end_iter, dim3(1), dim3(kBlock), h->dout, (const double*)h->dpart, h->dcyc, h->cnt,
+  launch(h, -1, -1, k_end_iter, dim3(1), dim3(kBlock), h->dout, (const double*)h->dpart, h->dcyc, h->cnt,
          h->scap);
 }
 
