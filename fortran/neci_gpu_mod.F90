@@ -44,7 +44,7 @@ module neci_gpu_mod
     public :: ngpu_create, ngpu_destroy, ngpu_set_integrals, ngpu_set_lattice, ngpu_set_excitgen
     public :: ngpu_set_hash, ngpu_set_reference, ngpu_upload_walkers, ngpu_download_walkers
     public :: ngpu_iterate, ngpu_get_stats, ngpu_get_cycle_stats, ngpu_find_walker_hash_batch
-    public :: ngpu_diag_helement_batch
+    public :: ngpu_diag_helement_batch, ngpu_set_excitgen_uniform, ngpu_set_core
 
     interface
         integer(c_int) function ngpu_create(h, sys) bind(c, name="ngpu_create")
@@ -78,6 +78,20 @@ module neci_gpu_mod
             type(c_ptr), value :: h
             integer(c_int32_t), value :: singles, particles
             real(c_double), value :: psingles, pparallel
+        end function
+        integer(c_int) function ngpu_set_excitgen_uniform(h, psingles) &
+            bind(c, name="ngpu_set_excitgen_uniform")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), value :: psingles
+        end function
+        integer(c_int) function ngpu_set_core(h, dets, ncore, fp32_values, nnz) bind(c, name="ngpu_set_core")
+            import :: c_int, c_ptr, c_int64_t, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int64_t), intent(in) :: dets(*)
+            integer(c_int64_t), value :: ncore
+            integer(c_int32_t), value :: fp32_values
+            integer(c_int64_t), intent(out) :: nnz
         end function
         integer(c_int) function ngpu_set_hash(h, random_hash2) bind(c, name="ngpu_set_hash")
             import :: c_int, c_ptr, c_int32_t

@@ -43,7 +43,9 @@ typedef enum {
     NGPU_ERR_ARG = 4,
     NGPU_ERR_HIP = 5,
     NGPU_ERR_HASH_FULL = 6,    /* every bucket of the walker hash is taken (MEMORYFACHASH) */
-    NGPU_ERR_STATE = 7         /* call out of order (e.g. iterate before set_reference) */
+    NGPU_ERR_STATE = 7,        /* call out of order (e.g. iterate before set_reference) */
+    NGPU_ERR_EXCITGEN = 8      /* uniform generator found no allowed orbital within the reference's
+                                  attempt bounds (Stop_All sites, GenRandSymExcitNUMod.F90:724-870) */
 } ngpu_status;
 
 enum { NGPU_MODEL_HUBBARD = 0, NGPU_MODEL_MOLECULAR = 1 };
@@ -126,6 +128,13 @@ int ngpu_set_lattice(ngpu_t *h, const int32_t *nbr, int32_t nsites, int32_t maxn
    on the host and uploads them.  Requires ngpu_set_integrals first. */
 int ngpu_set_excitgen(ngpu_t *h, int32_t singles, int32_t particles, double psingles,
                       double pparallel);
+
+/* Molecular excitation generator: NECI's uniform random excitation generator
+   gen_rand_excit (GenRandSymExcitNUMod.F90:72-153, the default binding of
+   generate_excitation, fcimc_initialisation.fpp:2138) for a C1 system: singles
+   and doubles chosen uniformly among the spin-allowed ones, doubles with
+   probability 1 - psingles.  Replaces ngpu_set_excitgen's binding. */
+int ngpu_set_excitgen_uniform(ngpu_t *h, double psingles);
 
 /* RandomHash2(1:nbasis) (fcimc_initialisation.fpp:890-915) for FindWalkerHash. */
 int ngpu_set_hash(ngpu_t *h, const int32_t *random_hash2);

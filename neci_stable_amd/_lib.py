@@ -30,6 +30,7 @@ STATUS = {
     5: "NGPU_ERR_HIP",
     6: "NGPU_ERR_HASH_FULL (increase MEMORYFACHASH)",
     7: "NGPU_ERR_STATE",
+    8: "NGPU_ERR_EXCITGEN (uniform generator found no allowed orbital)",
 }
 
 
@@ -92,6 +93,7 @@ _SIGS = [
     ("ngpu_set_integrals", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_double]),
     ("ngpu_set_lattice", ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                         ctypes.c_double]),
+    ("ngpu_set_excitgen_uniform", ctypes.c_int, [_vp, ctypes.c_double]),
     ("ngpu_set_excitgen", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                          ctypes.c_double]),
     ("ngpu_set_hash", ctypes.c_int, [_vp, _vp]),

@@ -36,6 +36,7 @@ class Spec:
     n_hash_buckets: Optional[int] = None
     singles: int = 1
     particles: int = PART_UNIF_FAST
+    excitgen: str = "pchb"   # molecular: "pchb" (PCHB doubles + heat-bath singles) or "uniform" (gen_rand_excit)
     psingles: Optional[float] = None
     pparallel: float = 0.5
     ref_orbs: Optional[list] = None

@@ -227,6 +227,7 @@ struct DevSys {
   int t_init, t_real, t_rsc, singles;
   int particles, nsites, maxnbr, ijmax;
   int ncore;  // semistochastic core dets (0: none)
+  int uniform;  // molecular generator: 0 PCHB + heat-bath singles, 1 gen_rand_excit
   double init_walk_no, rsc, occ_thresh, avmc;
   double hii, ecore, bhub, uhub;
   double psingles, pdoubles, pparallel;
@@ -619,11 +620,223 @@ __device__ __host__ inline Excit gen_double_pchb(const DevSys& s, const uint64_t
   return ex;
 }
 
+// Uniform draws for generators with rejection loops (a varying number of
+// draws): u_1..u_4 are the attempt's d1..d4 (d5 stays the spawn's rounding
+// draw); u_5, u_6, ... come from Philox blocks with purpose 6 and key word 1 =
+// replica | (block + 1) << 8, two uniforms per block.
+template <int W>
+struct DrawSeq {
+  const double* dr;
+  uint32_t seed;
+  const uint64_t* key;
+  uint32_t iter;
+  int rep;
+  uint32_t k;
+  int n;
+  int blk;
+  double c0, c1;
+  __device__ __host__ DrawSeq(const double* d, uint32_t sd, const uint64_t* ky, uint32_t it, int r, uint32_t kk)
+      : dr(d), seed(sd), key(ky), iter(it), rep(r), k(kk), n(0), blk(-1), c0(0.0), c1(0.0) {}
+  __device__ __host__ double next() {
+    ++n;
+    if (n <= 4) return dr[n];
+    int j = n - 5, b = j >> 1;
+    if (b != blk) {
+      uint64_t dh = det_hash64<W>(key);
+      U4 c{(uint32_t)dh, (uint32_t)(dh >> 32), iter, (k << 3) | 6u};
+      U4 o = philox4x32_10(c, seed, (uint32_t)rep | ((uint32_t)(b + 1) << 8));
+      c0 = u01(o.x, o.y);
+      c1 = u01(o.z, o.w);
+      blk = b;
+    }
+    return (j & 1) ? c1 : c0;
+  }
+};
+
+// gen_rand_excit (GenRandSymExcitNUMod.F90:72-153) for C1 systems (one
+// irrep, no Lz): CreateDoubExcit (:155-185: PickElecPair :1021, the forbidden
+// count :385-412, PickAOrb :672-870, IsAOrbSymAllowed :873-949, PickBOrb
+// :193-306, FindDoubleProb :187) and CreateSingleExcit (:1118-1286,
+// CheckIfSingleExcits :1060).  Class counts are the occupied/unoccupied
+// counts per spin (construct_class_counts :1288); the orbitals of one spin in
+// symmetry-label order are the orbitals of that spin in index order.  A double
+// is chosen when d0 < pDoubles.  The reference stops the run when a rejection
+// loop exceeds its bound; here the excitation is null and fail is set.
+template <int W>
+__device__ __host__ inline Excit gen_uniform(const DevSys& s, const uint64_t* d, const double* dr, DrawSeq<W>& u,
+                                             bool& fail) {
+  Excit ex{};
+  const uint64_t kA = 0xAAAAAAAAAAAAAAAAull, kB = 0x5555555555555555ull;
+  const int nel = s.nel, nb = s.nbasis, nsp = nb / 2;
+  int na = 0, nbe = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) { na += popc64(d[w] & kA); nbe += popc64(d[w] & kB); }
+  const int ua = nsp - na, ub = nsp - nbe;  // unoccupied alpha / beta
+  if (dr[0] < s.pdoubles) {
+    const int pairs = nel * (nel - 1) / 2;
+    int ind = (int)((double)pairs * u.next()) + 1;
+    int X = pairs - ind;
+    int K = 0;  // INT((SQRT(8X+1)-1)/2): the largest K with K(K+1)/2 <= X
+    while ((K + 1) * (K + 2) / 2 <= X) ++K;
+    int e1 = nel - 1 - K, e2 = nel - X + K * (K + 1) / 2;
+    int o1 = select_orb<W>(d, ~0ull, e1), o2 = select_orb<W>(d, ~0ull, e2);
+    bool b1 = is_beta(o1), b2 = is_beta(o2);
+    int ispn = (b1 != b2) ? 2 : (!b1 ? 3 : 1);
+    int forb = 0;
+    if (ispn == 2) {
+      if (ua == 0) forb += ub;
+      if (ub == 0) forb += ua;
+    } else if (ispn == 1) {
+      if (ub == 1) forb = 1;
+    } else {
+      if (ua == 1) forb = 1;
+    }
+    const int nexa = ispn == 2 ? nb - nel : (ispn == 1 ? ub : ua);
+    if (nexa == forb) return ex;
+    int orbA = 0;
+    for (int overall = 0;; ++overall) {
+      if (nexa - forb <= 3) {
+        int chosen = (int)((double)(nexa - forb) * u.next()) + 1;
+        int z = 1;
+        if (ispn == 2) {
+          for (int o = 1; o <= nb && !orbA; ++o) {
+            if (test_orb<W>(d, o)) continue;
+            if (is_beta(o) ? ua != 0 : ub != 0) {
+              if (z == chosen) orbA = o;
+              z++;
+            }
+          }
+        } else {
+          for (int i = 1; i <= nsp && !orbA; ++i) {
+            int o = ispn == 1 ? 2 * i - 1 : 2 * i;
+            if (test_orb<W>(d, o)) continue;
+            if (ispn == 1 ? ub != 1 : ua != 1) {
+              if (z == chosen) orbA = o;
+              z++;
+            }
+          }
+        }
+        if (!orbA) { fail = true; return ex; }
+        break;  // the enumerating branch returns without the final check
+      }
+      for (int att = 0;; ++att) {
+        int o = ispn == 2 ? (int)((double)nb * u.next()) + 1
+                          : 2 * ((int)((double)nsp * u.next()) + 1) - (ispn == 1 ? 1 : 0);
+        if (!test_orb<W>(d, o)) { orbA = o; break; }
+        if (att > 250) { fail = true; return ex; }
+      }
+      bool allowed = ispn == 2 ? (is_beta(orbA) ? ua != 0 : ub != 0) : (ispn == 1 ? ub != 1 : ua != 1);
+      if (allowed) break;
+      if (overall > nb * 20) { fail = true; return ex; }
+    }
+    int nexb, other;
+    bool bbeta;
+    if (ispn == 2) {
+      if (is_beta(orbA)) { nexb = ua; other = ub; bbeta = false; }
+      else { nexb = ub; other = ua; bbeta = true; }
+    } else if (ispn == 1) {
+      nexb = ub; other = ub; bbeta = true;
+    } else {
+      nexb = ua; other = ua; bbeta = false;
+    }
+    if (ispn != 2) { nexb -= 1; other -= 1; }
+    int orbB = 0;
+    for (int att = 0;; ++att) {
+      int o = 2 * ((int)((double)nsp * u.next()) + 1) - (bbeta ? 1 : 0);
+      if (!test_orb<W>(d, o) && o != orbA) { orbB = o; break; }
+      if (att > 1000) { fail = true; return ex; }
+    }
+    ex.ic = 2;
+    ex.e0 = o1; ex.e1 = o2;
+    ex.e2 = orbA < orbB ? orbA : orbB;
+    ex.e3 = orbA < orbB ? orbB : orbA;
+    ex.parity = double_parity<W>(d, ex.e0, ex.e1, ex.e2, ex.e3);
+    ex.pgen = s.pdoubles * ((1.0 / (double)nexb) + (1.0 / (double)other)) / ((double)(pairs * (nexa - forb)));
+    return ex;
+  }
+  int nowex = 0;
+  if (na != 0 && ua == 0) nowex += na;
+  if (nbe != 0 && ub == 0) nowex += nbe;
+  if (nowex == nel) return ex;
+  int src = 0, nexc = 0;
+  for (int att = 0;; ++att) {
+    int e = (int)((double)nel * u.next()) + 1;
+    src = select_orb<W>(d, ~0ull, e);
+    nexc = is_beta(src) ? ub : ua;
+    if (nexc != 0) break;
+    if (att > 250) { fail = true; return ex; }
+  }
+  int tgt = 0;
+  for (int att = 0;; ++att) {
+    int o = 2 * ((int)((double)nsp * u.next()) + 1) - (is_beta(src) ? 1 : 0);
+    if (!test_orb<W>(d, o)) { tgt = o; break; }
+    if (att > 250) { fail = true; return ex; }
+  }
+  ex.ic = 1;
+  ex.e0 = src;
+  ex.e1 = tgt;
+  ex.parity = single_parity<W>(d, src, tgt);
+  ex.pgen = (1.0 - s.pdoubles) / (double)(nexc * (nel - nowex));
+  return ex;
+}
+
+// calc_pgen_symrandexcit2 (GenRandSymExcitNUMod.F90:1322-1470) for C1 systems
+template <int W>
+__device__ __host__ inline double pgen_uniform(const DevSys& s, const uint64_t* d, int ic, const int32_t* ex) {
+  const uint64_t kA = 0xAAAAAAAAAAAAAAAAull, kB = 0x5555555555555555ull;
+  const int nel = s.nel, nb = s.nbasis, nsp = nb / 2;
+  int na = 0, nbe = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) { na += popc64(d[w] & kA); nbe += popc64(d[w] & kB); }
+  const int ua = nsp - na, ub = nsp - nbe;
+  if (ic == 1) {
+    int nowex = 0;
+    if (na != 0 && ua == 0) nowex += na;
+    if (nbe != 0 && ub == 0) nowex += nbe;
+    int nexa = is_beta(ex[0]) ? ub : ua;
+    return (1.0 - s.pdoubles) / (double)(nexa * (nel - nowex));
+  }
+  if (ic != 2) return 0.0;
+  bool b1 = is_beta(ex[0]), b2 = is_beta(ex[1]);
+  int ispn = (b1 != b2) ? 2 : (!b1 ? 3 : 1);
+  int nexa = ispn == 2 ? nb - nel : (ispn == 1 ? ub : ua);
+  int forb = 0;
+  if (ispn == 2) {
+    if (ua == 0) forb += ub;
+    if (ub == 0) forb += ua;
+  } else if (ispn == 1) {
+    if (ub == 1) forb = 1;
+  } else {
+    if (ua == 1) forb = 1;
+  }
+  int nexb, other;
+  if (ispn == 2) {
+    if (is_beta(ex[2])) { nexb = ua; other = ub; }
+    else { nexb = ub; other = ua; }
+  } else if (ispn == 1) {
+    nexb = ub; other = ub;
+  } else {
+    nexb = ua; other = ua;
+  }
+  if (ispn != 2) { nexb -= 1; other -= 1; }
+  const int pairs = nel * (nel - 1) / 2;
+  return s.pdoubles * ((1.0 / (double)nexb) + (1.0 / (double)other)) / ((double)(pairs * (nexa - forb)));
+}
+
 // generate_excitation as bound by init_fcimc_fn_pointers
 template <int W>
-__device__ __host__ inline Excit gen_excit(const DevSys& s, const uint64_t* d, const double* dr) {
+__device__ __host__ inline Excit gen_excit(const DevSys& s, const uint64_t* d, const double* dr,
+                                           uint32_t iter = 0xFFFFFFFFu, int rep = 0, uint32_t k = 0,
+                                           bool* fail = nullptr) {
   if (s.model == 0) return gen_hubbard<W>(s, d, dr);
   Excit ex;
+  if (s.uniform) {
+    DrawSeq<W> u(dr, s.seed, d, iter, rep, k);
+    bool f = false;
+    ex = gen_uniform<W>(s, d, dr, u, f);
+    if (fail) *fail = f;
+    return ex;
+  }
   if (dr[0] < s.psingles) {  // gen_exc_sd (excitation_generators.F90:112-138)
     ex = gen_single_heatbath<W>(s, d, dr[1], dr[2]);
     ex.pgen = ex.pgen * s.psingles;

@@ -357,7 +357,7 @@ void iterate_chain(ngpu_t* h, const ngpu_iter_in_t* in) {
   launch(h, 2, 2, k_spawn<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
          (const int64_t*)h->att_off, (const int64_t*)h->tile_p, h->tcap, (const uint8_t*)h->pinfo, h->sp_rec,
          h->scap, h->sq, in->tau, in->iter);
-  if (h->sys.model == NGPU_MODEL_MOLECULAR) {
+  if (h->sys.model == NGPU_MODEL_MOLECULAR && !h->ds.uniform) {
     const int nsp = h->sys.nbasis / 2;
     const size_t lds = sizeof(double) * (size_t)kHbGroups * nsp;
     auto kern = nsp <= 4 * kHbGroup ? k_spawn_singles<W, R, 4>
@@ -482,6 +482,7 @@ __device__ double calc_pgen(const DevSys& s, const uint64_t* d, int ic, const in
     if (cum_sum < kEps) return 0.0;
     return (1.0 / (double)s.nel) * (cpt / cum_sum);
   }
+  if (s.uniform) return pgen_uniform<W>(s, d, ic, ex);
   if (ic == 1) {
     int src = ex[0], tgt = ex[1];
     double pgen = 1.0 / (double)s.nel;
@@ -823,6 +824,7 @@ int ngpu_set_excitgen(ngpu_t* h, int32_t singles, int32_t particles, double psin
   if (singles != NGPU_SINGLES_HEATBATH) return NGPU_ERR_ARG;
   if (particles != NGPU_PART_UNIF_UNIF && particles != NGPU_PART_UNIF_FAST) return NGPU_ERR_ARG;
   if (!(psingles >= 0.0 && psingles <= 1.0) || !(pparallel >= 0.0 && pparallel <= 1.0)) return NGPU_ERR_ARG;
+  h->ds.uniform = 0;
   const int n = h->sys.nbasis;
   const int ijmax = fuse_index(n, n);
   std::vector<double> tmat((size_t)n * n);
@@ -910,6 +912,17 @@ int ngpu_set_excitgen(ngpu_t* h, int32_t singles, int32_t particles, double psin
   h->ds.psingles = psingles;
   h->ds.pdoubles = 1.0 - psingles;
   h->ds.pparallel = pparallel;
+  h->have_excitgen = true;
+  return NGPU_OK;
+}
+
+int ngpu_set_excitgen_uniform(ngpu_t* h, double psingles) {
+  if (!h || h->sys.model != NGPU_MODEL_MOLECULAR || !h->d_umat) return NGPU_ERR_ARG;
+  if (!(psingles >= 0.0 && psingles <= 1.0)) return NGPU_ERR_ARG;
+  if ((h->sys.nbasis & 1) || h->sys.nel < 2) return NGPU_ERR_ARG;
+  h->ds.uniform = 1;
+  h->ds.psingles = psingles;
+  h->ds.pdoubles = 1.0 - psingles;
   h->have_excitgen = true;
   return NGPU_OK;
 }

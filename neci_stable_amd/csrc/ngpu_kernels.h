@@ -663,7 +663,7 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* 
     for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
     double dr[6];
     draw2<W>(s.seed, key, iter, r, k, 1, dr[0], dr[1]);
-    bool single = s.model == 1 && dr[0] < s.psingles;
+    bool single = s.model == 1 && !s.uniform && dr[0] < s.psingles;
     if (mode == 1 && single) {
       queue = true;
     } else {
@@ -672,6 +672,10 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* 
       Excit ex;
       if (s.model == 0) {
         ex = gen_hubbard<W>(s, key, dr);
+      } else if (s.uniform) {
+        bool fail = false;
+        ex = gen_excit<W>(s, key, dr, iter, r, k, &fail);
+        if (fail) atomicOr(&cnt->err, 1u << NGPU_ERR_EXCITGEN);
       } else if (single) {
         ex = gen_single_heatbath<W>(s, key, dr[1], dr[2]);
         ex.pgen = ex.pgen * s.psingles;
@@ -750,7 +754,7 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
   const int64_t total = (int64_t)cnt->n_attempts;
   const int64_t npairs = (int64_t)cnt->nslots * s.S;
   bool overflow = false;
-  int mode = s.model == 1 ? 1 : 0;
+  int mode = (s.model == 1 && !s.uniform) ? 1 : 0;
   SpawnAcc<R> acc;
   __shared__ int64_t pb[2];
   __shared__ int64_t woff[kParentWindow];

@@ -47,8 +47,11 @@ class Engine:
             um = np.ascontiguousarray(system.umat, dtype=np.float64)
             check(self.lib.ngpu_set_integrals(h, ptr(tm), ptr(um), um.size, system.ecore),
                   "ngpu_set_integrals")
-            check(self.lib.ngpu_set_excitgen(h, spec.singles, spec.particles, spec.psingles, spec.pparallel),
-                  "ngpu_set_excitgen")
+            if spec.excitgen == "uniform":
+                check(self.lib.ngpu_set_excitgen_uniform(h, spec.psingles), "ngpu_set_excitgen_uniform")
+            else:
+                check(self.lib.ngpu_set_excitgen(h, spec.singles, spec.particles, spec.psingles, spec.pparallel),
+                      "ngpu_set_excitgen")
         self.random_hash = spec.random_hash
         check(self.lib.ngpu_set_hash(h, ptr(self.random_hash)), "ngpu_set_hash")
         self.ref_orbs = spec.ref_orbs

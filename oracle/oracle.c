@@ -700,15 +700,245 @@ static int gen_double_pchb(const osys_t *s, const uint64_t *ilut, const int32_t 
 }
 
 /* One excitation: returns 0 for a null excitation, else fills outputs. */
+/* ---------------------------------------------- uniform molecular generator */
+
+/* The stream of uniforms a generator with rejection loops consumes: d[1..4] of
+   the attempt, then purpose-6 Philox blocks (two uniforms each) keyed by
+   (seed, replica | (block + 1) << 8). */
+typedef struct {
+    const osys_t *s;
+    const double *d;
+    const uint64_t *det;
+    uint32_t iter, k;
+    int rep, n, blk;
+    double c[2];
+} draw_seq_t;
+
+static double seq_next(draw_seq_t *q)
+{
+    q->n++;
+    if (q->n <= 4) return q->d[q->n];
+    int j = q->n - 5, b = j / 2;
+    if (b != q->blk) {
+        int W = nwords(q->s->nbasis);
+        uint64_t dh = det_hash64(q->det, W);
+        uint32_t ctr[4] = {(uint32_t)dh, (uint32_t)(dh >> 32), q->iter, (q->k << 3) | 6u};
+        uint32_t key[2] = {q->s->seed, (uint32_t)q->rep | ((uint32_t)(b + 1) << 8)};
+        uint32_t o[4];
+        o_philox4x32_10(ctr, key, o);
+        q->c[0] = u01(o[0], o[1]);
+        q->c[1] = u01(o[2], o[3]);
+        q->blk = b;
+    }
+    return q->c[j & 1];
+}
+
+/* spin class of a spin orbital: 1 alpha (even), 2 beta (odd), as ClassCountInd(spin, 0, 0) */
+static inline int cls(int orb) { return is_beta(orb) ? 2 : 1; }
+
+/* gen_rand_excit (GenRandSymExcitNUMod.F90:72-153) for a C1 system (one irrep, no Lz).
+   ClassCountOcc/Unocc (construct_class_counts :1288) reduce to per-spin counts; the
+   orbitals of one spin in symmetry-label order are taken in index order.
+   Returns 0 for a null excitation; *fail marks a rejection loop past the bound at
+   which the reference stops the run. */
+static int gen_uniform(const osys_t *s, const uint64_t *ilut, const int32_t *nI, const double *d,
+                       draw_seq_t *q, int32_t *ic, int32_t *ex, int32_t *nJ, double *pgen, int32_t *fail)
+{
+    const int nel = s->nel, nb = s->nbasis, nsp = nb / 2;
+    int occ[3] = {0, 0, 0}, unocc[3];
+    for (int i = 0; i < nel; ++i) occ[cls(nI[i])]++;
+    unocc[1] = nsp - occ[1];
+    unocc[2] = nsp - occ[2];
+    double pdoub = 1.0 - s->psingles;
+    if (d[0] < pdoub) {
+        /* CreateDoubExcit (:155-185) */
+        *ic = 2;
+        int elecpairs = nel * (nel - 1) / 2;
+        /* PickElecPair (:1021-1058) */
+        int ind = (int)(elecpairs * seq_next(q)) + 1;
+        int X = elecpairs - ind;
+        int K = (int)((sqrt(8.0 * (double)X + 1.0) - 1.0) / 2.0);
+        int e1 = nel - 1 - K, e2 = nel - X + (K * (K + 1)) / 2;
+        int orb1 = nI[e1 - 1], orb2 = nI[e2 - 1];
+        int ms1 = is_beta(orb1) ? -1 : 1, ms2 = is_beta(orb2) ? -1 : 1;
+        int ispn = (ms1 * ms2 == -1) ? 2 : (ms1 == 1 ? 3 : 1);
+        /* FindNumForbiddenOrbs (:385-412, no Lz, one irrep) */
+        int forbidden = 0;
+        if (ispn == 2) {
+            if (unocc[1] == 0) forbidden += unocc[2];
+            if (unocc[2] == 0) forbidden += unocc[1];
+        } else if (ispn == 1) {
+            if (unocc[2] == 1) forbidden = 1;
+        } else {
+            if (unocc[1] == 1) forbidden = 1;
+        }
+        /* PickAOrb (:672-870) with IsAOrbSymAllowed (:873-949) */
+        int nexcit_a = ispn == 2 ? nb - nel : (ispn == 1 ? nsp - occ[2] : nsp - occ[1]);
+        if (nexcit_a == forbidden) return 0;
+        int orbA = 0, spinA = 0;
+        for (int overall = 0;; ++overall) {
+            if (nexcit_a - forbidden <= 3) {
+                int chosen = (int)((nexcit_a - forbidden) * seq_next(q)) + 1;
+                int z = 1;
+                if (ispn == 2) {
+                    for (int i = 0; i < nb && !orbA; ++i) {
+                        if (test_orb(ilut, i + 1)) continue;
+                        int sa = is_beta(i + 1) ? -1 : 1;
+                        int allowed = sa == 1 ? unocc[2] != 0 : unocc[1] != 0;
+                        if (allowed) {
+                            if (z == chosen) { orbA = i + 1; spinA = sa; }
+                            z++;
+                        }
+                    }
+                } else {
+                    for (int i = 1; i <= nsp && !orbA; ++i) {
+                        int o = ispn == 1 ? 2 * i - 1 : 2 * i;
+                        if (test_orb(ilut, o)) continue;
+                        int allowed = ispn == 1 ? unocc[2] != 1 : unocc[1] != 1;
+                        if (allowed) {
+                            if (z == chosen) orbA = o;
+                            z++;
+                        }
+                    }
+                }
+                if (!orbA) { *fail = 1; return 0; }
+                break;
+            }
+            int attempts = 0;
+            for (;;) {
+                int o;
+                if (ispn == 2) {
+                    o = (int)(nb * seq_next(q)) + 1;
+                } else {
+                    o = (int)(nsp * seq_next(q)) + 1;
+                    o = ispn == 1 ? 2 * o - 1 : 2 * o;
+                }
+                if (!test_orb(ilut, o)) { orbA = o; break; }
+                if (attempts > 250) { *fail = 1; return 0; }
+                attempts++;
+            }
+            spinA = is_beta(orbA) ? -1 : 1;
+            int allowed;
+            if (ispn == 2) allowed = spinA == 1 ? unocc[2] != 0 : unocc[1] != 0;
+            else if (ispn == 1) allowed = unocc[2] != 1;
+            else allowed = unocc[1] != 1;
+            if (allowed) break;
+            if (overall > nb * 20) { *fail = 1; return 0; }
+        }
+        /* PickBOrb (:193-306), tCycleOrbs off */
+        int nexcit_b, other, spin_b;
+        if (ispn == 2) {
+            if (is_beta(orbA)) { nexcit_b = unocc[1]; other = unocc[2]; spin_b = 1; }
+            else { nexcit_b = unocc[2]; other = unocc[1]; spin_b = 2; }
+        } else if (ispn == 1) {
+            nexcit_b = unocc[2]; other = unocc[2]; spin_b = 2;
+        } else {
+            nexcit_b = unocc[1]; other = unocc[1]; spin_b = 1;
+        }
+        if (ispn != 2) { nexcit_b -= 1; other -= 1; }
+        int orbB = 0;
+        for (int attempts = 0;; ++attempts) {
+            int c = (int)(nsp * seq_next(q));
+            int o = 2 * (c + 1) - (spin_b - 1);
+            if (!test_orb(ilut, o) && o != orbA) { orbB = o; break; }
+            if (attempts > 1000) { *fail = 1; return 0; }
+        }
+        int par = o_make_double(nI, nel, e1, e2, orbA, orbB, nJ, ex);
+        /* FindDoubleProb (:187-191) */
+        *pgen = pdoub * ((1.0 / (double)nexcit_b) + (1.0 / (double)other)) /
+                (double)(elecpairs * (nexcit_a - forbidden));
+        return 1 | (par ? 2 : 0);
+    }
+    /* CreateSingleExcit (:1118-1286) with CheckIfSingleExcits (:1060-1085) */
+    *ic = 1;
+    int nowex = 0;
+    for (int c = 1; c <= 2; ++c)
+        if (occ[c] != 0 && unocc[c] == 0) nowex += occ[c];
+    if (nowex == nel) return 0;
+    int eleci = 0, nexcit = 0, spin = 0;
+    for (int attempts = 0;; ++attempts) {
+        eleci = (int)(nel * seq_next(q)) + 1;
+        spin = cls(nI[eleci - 1]);
+        nexcit = unocc[spin];
+        if (nexcit != 0) break;
+        if (attempts > 250) { *fail = 1; return 0; }
+    }
+    int orb = 0;
+    for (int attempts = 0;; ++attempts) {
+        int c = (int)(nsp * seq_next(q));
+        int o = 2 * (c + 1) - (spin - 1);
+        if (!test_orb(ilut, o)) { orb = o; break; }
+        if (attempts > 250) { *fail = 1; return 0; }
+    }
+    int par = o_make_single(nI, nel, eleci, orb, nJ, ex);
+    *pgen = (1.0 - pdoub) / (double)(nexcit * (nel - nowex));
+    return 1 | (par ? 2 : 0);
+}
+
+/* calc_pgen_symrandexcit2 (GenRandSymExcitNUMod.F90:1322-1470) for a C1 system */
+static double pgen_uniform(const osys_t *s, const int32_t *nI, int32_t ic, const int32_t *ex)
+{
+    const int nel = s->nel, nb = s->nbasis, nsp = nb / 2;
+    int occ[3] = {0, 0, 0}, unocc[3];
+    for (int i = 0; i < nel; ++i) occ[cls(nI[i])]++;
+    unocc[1] = nsp - occ[1];
+    unocc[2] = nsp - occ[2];
+    double pdoub = 1.0 - s->psingles;
+    if (ic == 1) {
+        int nowex = 0;
+        for (int c = 1; c <= 2; ++c)
+            if (occ[c] != 0 && unocc[c] == 0) nowex += occ[c];
+        int nexcit_a = unocc[cls(ex[0])];
+        return (1.0 - pdoub) / (double)(nexcit_a * (nel - nowex));
+    }
+    int msi = is_beta(ex[0]) ? -1 : 1, msj = is_beta(ex[1]) ? -1 : 1;
+    int ispn, nexcit_a;
+    if (msi * msj == -1) { ispn = 2; nexcit_a = nb - nel; }
+    else if (msi == 1) { ispn = 3; nexcit_a = nsp - occ[1]; }
+    else { ispn = 1; nexcit_a = nsp - occ[2]; }
+    int forbidden = 0;
+    if (ispn == 2) {
+        if (unocc[1] == 0) forbidden += unocc[2];
+        if (unocc[2] == 0) forbidden += unocc[1];
+    } else if (ispn == 1) {
+        if (unocc[2] == 1) forbidden = 1;
+    } else {
+        if (unocc[1] == 1) forbidden = 1;
+    }
+    int nexcit_b, other;
+    if (ispn == 2) {
+        if (is_beta(ex[2])) { nexcit_b = unocc[1]; other = unocc[2]; }
+        else { nexcit_b = unocc[2]; other = unocc[1]; }
+    } else if (ispn == 1) {
+        nexcit_b = unocc[2]; other = unocc[2];
+    } else {
+        nexcit_b = unocc[1]; other = unocc[1];
+    }
+    if (ispn != 2) { nexcit_b -= 1; other -= 1; }
+    int elecpairs = nel * (nel - 1) / 2;
+    return pdoub * ((1.0 / (double)nexcit_b) + (1.0 / (double)other)) / (double)(elecpairs * (nexcit_a - forbidden));
+}
+
 int32_t o_gen_excit(const osys_t *s, const uint64_t *ilut, const double *d, int32_t *nJ,
                     uint64_t *ilutJ, int32_t *ic, int32_t *ex, int32_t *tparity, double *pgen)
+{
+    int32_t fail = 0;
+    return o_gen_excit_ctx(s, ilut, d, 0xFFFFFFFFu, 0, 0, nJ, ilutJ, ic, ex, tparity, pgen, &fail);
+}
+
+int32_t o_gen_excit_ctx(const osys_t *s, const uint64_t *ilut, const double *d, uint32_t iter, int32_t rep,
+                        uint32_t k, int32_t *nJ, uint64_t *ilutJ, int32_t *ic, int32_t *ex, int32_t *tparity,
+                        double *pgen, int32_t *fail)
 {
     int W = nwords(s->nbasis);
     int32_t nI[O_MAX_NEL];
     o_decode(ilut, W, nI);
     int res;
     double pg = 0.0;
-    if (s->model == O_MODEL_HUBBARD) {
+    if (s->model == O_MODEL_MOLECULAR && s->uniform) {
+        draw_seq_t q = {s, d, ilut, iter, k, rep, 0, -1, {0.0, 0.0}};
+        res = gen_uniform(s, ilut, nI, d, &q, ic, ex, nJ, &pg, fail);
+    } else if (s->model == O_MODEL_HUBBARD) {
         *ic = 1;
         res = gen_hubbard(s, ilut, nI, d, ex, nJ, &pg);
     } else if (d[0] < s->psingles) {
@@ -737,6 +967,11 @@ int32_t o_gen_excit(const osys_t *s, const uint64_t *ilut, const double *d, int3
 double o_calc_pgen(const osys_t *s, const uint64_t *ilut, int32_t ic, const int32_t *ex)
 {
     int W = nwords(s->nbasis);
+    if (s->model == O_MODEL_MOLECULAR && s->uniform) {
+        int32_t nI[O_MAX_NEL];
+        o_decode(ilut, W, nI);
+        return pgen_uniform(s, nI, ic, ex);
+    }
     int32_t nI[O_MAX_NEL];
     o_decode(ilut, W, nI);
     int nel = s->nel;
@@ -1442,9 +1677,10 @@ int32_t o_iterate(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_t 
                 draw2(s, row, W, iter, r, (uint32_t)p, 1, &d[0], &d[1]);
                 draw2(s, row, W, iter, r, (uint32_t)p, 2, &d[2], &d[3]);
                 draw2(s, row, W, iter, r, (uint32_t)p, 3, &d[4], &d[5]);
-                int32_t ic, tpar;
+                int32_t ic, tpar, gfail = 0;
                 double pgen;
-                if (!o_gen_excit(s, row, d, nJ, ilutJ, &ic, ex, &tpar, &pgen)) {
+                if (!o_gen_excit_ctx(s, row, d, iter, r, (uint32_t)p, nJ, ilutJ, &ic, ex, &tpar, &pgen, &gfail)) {
+                    if (gfail) err = 8;
                     out->n_invalid_excits += 1.0;
                     continue;
                 }

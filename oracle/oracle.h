@@ -23,6 +23,8 @@
  *   purpose 1,2,3: spawn attempt k: d0,d1 / d2,d3 / d4,d5
  *   purpose 4: per new det+replica: stochRoundSpawn
  *   purpose 5: per det+replica: CalcHashTableStats rounding
+ *   purpose 6: spawn attempt k, draws of generators with rejection loops
+ *     beyond d1..d4: block b uses key = (seed, replica | (b + 1) << 8)
  */
 #ifndef NECI_ORACLE_H
 #define NECI_ORACLE_H
@@ -62,7 +64,8 @@ typedef struct {
     int32_t singles, particles;
     double psingles, pparallel;
     /* PCHB tables, built by o_build_pchb */
-    int32_t pchb_ready, pad1;
+    int32_t pchb_ready;
+    int32_t uniform;                    /* 1: gen_rand_excit instead of PCHB + heat-bath singles */
     /* walker hash */
     const int32_t *random_hash2;        /* RandomHash2(1:nbasis) */
     int64_t nbuckets;
@@ -113,6 +116,11 @@ int32_t o_get_pchb_row(int32_t which, int32_t row, double *bias, int32_t *alias,
 int32_t o_gen_excit(const osys_t *s, const uint64_t *ilut, const double *d, int32_t *nJ,
                     uint64_t *ilutJ, int32_t *ic, int32_t *ex, int32_t *tparity, double *pgen);
 double o_calc_pgen(const osys_t *s, const uint64_t *ilut, int32_t ic, const int32_t *ex);
+/* as o_gen_excit with the draw context of spawn attempt k of (det, iter, replica):
+   generators with rejection loops draw beyond d[1..4] from that context */
+int32_t o_gen_excit_ctx(const osys_t *s, const uint64_t *ilut, const double *d, uint32_t iter, int32_t rep,
+                        uint32_t k, int32_t *nJ, uint64_t *ilutJ, int32_t *ic, int32_t *ex, int32_t *tparity,
+                        double *pgen, int32_t *fail);
 
 /* --- iteration engine (the restatement of PerformFCIMCycPar) --- */
 ostate_t *o_create(const osys_t *sys);

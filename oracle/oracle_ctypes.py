@@ -25,7 +25,7 @@ class OSys(ctypes.Structure):
         ("t_real_coeff", ctypes.c_int32), ("t_real_spawn_cutoff", ctypes.c_int32),
         ("pad0", ctypes.c_int32), ("hii", ctypes.c_double), ("ref", ctypes.c_uint64 * 2),
         ("singles", ctypes.c_int32), ("particles", ctypes.c_int32), ("psingles", ctypes.c_double),
-        ("pparallel", ctypes.c_double), ("pchb_ready", ctypes.c_int32), ("pad1", ctypes.c_int32),
+        ("pparallel", ctypes.c_double), ("pchb_ready", ctypes.c_int32), ("uniform", ctypes.c_int32),
         ("random_hash2", ctypes.c_void_p), ("nbuckets", ctypes.c_int64), ("seed", ctypes.c_uint32),
         ("pad2", ctypes.c_uint32), ("max_walkers", ctypes.c_int64), ("max_spawned", ctypes.c_int64),
     ]
@@ -94,6 +94,8 @@ def lib():
             "o_set_core": (ctypes.c_int32, [_vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp]),
             "o_core_project": (None, [_vp, _vp, _vp, ctypes.c_double, _vp]),
             "o_core_csr": (ctypes.c_int64, [_vp, _vp, _vp, _vp]),
+            "o_gen_excit_ctx": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32,
+                                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -137,6 +139,7 @@ class Oracle:
             o.tmat, o.umat, o.umat_len, o.ecore = tm.ctypes.data, um.ctypes.data, um.size, s.ecore
             o.singles, o.particles = 1, spec.particles
             o.psingles, o.pparallel = spec.psingles, spec.pparallel
+            o.uniform = int(getattr(spec, "excitgen", "pchb") == "uniform")
         o.tau = float(tau)
         o.avmc = spec.av_mc_excits
         o.init_walk_no = spec.init_walk_no
@@ -160,7 +163,7 @@ class Oracle:
         o.max_walkers = int(max_walkers or spec.max_walkers)
         o.max_spawned = int(max_spawned or spec.max_spawned)
         self.sys = o
-        if spec.model == 1:
+        if spec.model == 1 and not o.uniform:
             L.o_build_pchb(ctypes.byref(o))
         o.hii = 0.0
         self.hii = self.diag(spec.ref_orbs)
@@ -270,6 +273,24 @@ class Oracle:
         if not ok:
             return None
         return dj.view(np.int64), ic.value, ex.copy(), tp.value, pg.value
+
+    def gen_excit_ctx(self, det, draws, it, rep, k):
+        """gen_excit with the draw context of spawn attempt k (rejection-loop draws); adds fail."""
+        det = np.ascontiguousarray(det, dtype=np.int64).view(np.uint64)
+        draws = np.ascontiguousarray(draws, dtype=np.float64)
+        nJ = np.zeros(64, dtype=np.int32)
+        dj = np.zeros(self.W, dtype=np.uint64)
+        ic = ctypes.c_int32()
+        ex = np.zeros(4, dtype=np.int32)
+        tp = ctypes.c_int32()
+        pg = ctypes.c_double()
+        fail = ctypes.c_int32()
+        ok = self.L.o_gen_excit_ctx(ctypes.byref(self.sys), p(det), p(draws), int(it) & 0xFFFFFFFF, int(rep),
+                                    int(k) & 0xFFFFFFFF, p(nJ), p(dj), ctypes.byref(ic), p(ex), ctypes.byref(tp),
+                                    ctypes.byref(pg), ctypes.byref(fail))
+        if not ok:
+            return None, fail.value
+        return (dj.view(np.int64), ic.value, ex.copy(), tp.value, pg.value), fail.value
 
     def calc_pgen(self, det, ic, ex):
         det = np.ascontiguousarray(det, dtype=np.int64).view(np.uint64)

@@ -104,6 +104,9 @@ SPECS = {
     # themselves instead of 32-bit colex ranks
     "hubbard18": lambda **k: hubbard_spec(18, 18, 4.0, **k),
     "mol33o8e_w2": lambda **k: mol_spec(33, 8, rank=4, mseed=3, **k),
+    # uniform random excitations (gen_rand_excit) with rejection loops
+    "mol8o6e_uniform": lambda **k: mol_spec(8, 6, excitgen="uniform", psingles=0.2, **k),
+    "mol33o4e_w2_uniform": lambda **k: mol_spec(33, 4, rank=4, mseed=3, excitgen="uniform", **k),
 }
 
 
@@ -241,7 +244,8 @@ def test_annihilate_explicit_list(name):
                                             ("mol8o6e_unif_unif", 1, False),
                                             ("mol33o4e_w2", 1, False), ("hubbard18", 1, False),
                                             ("mol33o8e_w2", 1, False), ("hubbard10", 1, True),
-                                            ("mol8o6e_unif_fast", 1, True)])
+                                            ("mol8o6e_unif_fast", 1, True), ("mol8o6e_uniform", 1, False),
+                                            ("mol8o6e_uniform", 2, False), ("mol33o4e_w2_uniform", 1, False)])
 def test_iterations_match_oracle(name, nrep, real):
     """Full iterations (spawn, merge, death, annihilation, stats) from the same
     walker set: identical walker sets after every iteration."""
@@ -308,7 +312,7 @@ def test_hash_full_is_an_error():
 
 @pytest.mark.parametrize("name,nrep,fp32", [("hubbard10", 1, False), ("hubbard10", 2, False),
                                              ("mol8o6e_unif_fast", 1, False), ("mol8o6e_unif_fast", 1, True),
-                                             ("mol33o4e_w2", 1, False)])
+                                             ("mol33o4e_w2", 1, False), ("mol8o6e_uniform", 1, False)])
 def test_semistochastic_iterations_match_oracle(name, nrep, fp32):
     """Iterations with a semistochastic core space (deterministic projection of
     the core amplitudes, core-to-core spawns cancelled): identical walker sets

@@ -394,3 +394,70 @@ def test_core_hamiltonian_and_projection_against_dense(fp32):
     assert all(tuple(c) in have for c in core.reshape(-1, 1))
     fl = {tuple(r): int(x) for r, x in zip(d, f)}
     assert all(fl[tuple(c)] & (1 << 16) and fl[tuple(c)] & 3 == 3 for c in core.reshape(-1, 1))
+
+
+# ------------------------------------------------- uniform molecular generator
+def _all_excitations(orbs, nbasis):
+    """Every spin-conserving single and double excitation of a determinant:
+    (ic, ex=[src.., tgt..] sorted, child orbitals)."""
+    occ = set(orbs)
+    virt = [o for o in range(1, nbasis + 1) if o not in occ]
+    out = []
+    for a in orbs:
+        for t in virt:
+            if (a & 1) == (t & 1):
+                out.append((1, [a, t, 0, 0], sorted((occ - {a}) | {t})))
+    for i, a in enumerate(orbs):
+        for b in orbs[i + 1:]:
+            for j, t in enumerate(virt):
+                for u in virt[j + 1:]:
+                    if sorted([a & 1, b & 1]) == sorted([t & 1, u & 1]):
+                        out.append((2, [a, b, t, u], sorted((occ - {a, b}) | {t, u})))
+    return out
+
+
+@pytest.mark.parametrize("nspat,nel,which", [(5, 4, "ref"), (5, 4, "random"), (5, 8, "ref"), (6, 7, "random")])
+def test_uniform_generator_pgen_and_frequencies(nspat, nel, which):
+    """gen_rand_excit restated for C1 systems: the returned pgen equals
+    calc_pgen_symrandexcit2 bit for bit, pgen summed over every spin-allowed
+    excitation is 1 (or below 1 where the reference's forbidden-orbital rules
+    exclude electron pairs), and sampled frequencies match pgen within the 15%
+    criterion of run_excit_gen_tester (pchb_excitgen_test_helper.F90:105-111).
+    The cases cover the rejection loops, the enumerating branch (<= 3 allowed
+    orbitals) and forbidden orbitals (one unoccupied orbital per spin)."""
+    mol = molecule(nspat, nel, rank=6, seed=11)
+    spec = Spec(mol, init_walkers=100, excitgen="uniform", psingles=0.3)
+    o = oc.Oracle(spec, 0.01)
+    rng = np.random.default_rng(7)
+    nb = 2 * nspat
+    if which == "ref":
+        orbs = list(spec.ref_orbs)
+    else:
+        na = sum(1 for x in spec.ref_orbs if x % 2 == 0)
+        orbs = sorted(list(rng.choice(np.arange(2, nb + 1, 2), na, replace=False)) +
+                      list(rng.choice(np.arange(1, nb + 1, 2), nel - na, replace=False)))
+    det = encode_det([int(x) for x in orbs], nb)
+    exc = _all_excitations([int(x) for x in orbs], nb)
+    pg_enum = {}
+    for ic, ex, child in exc:
+        pg_enum[tuple(encode_det(child, nb))] = o.calc_pgen(det, ic, ex)
+    total = sum(pg_enum.values())
+    assert total <= 1.0 + 1e-12
+    n = 200000
+    draws = rng.random((n, 6))
+    counts = {}
+    for i in range(n):
+        r, fail = o.gen_excit_ctx(det, draws[i], 3, 0, i)
+        assert not fail
+        if r is None:
+            continue
+        dj, ic, ex, tp, pg = r
+        key = tuple(dj)
+        assert key in pg_enum
+        assert pg == pg_enum[key]
+        counts[key] = counts.get(key, 0) + 1
+    for key, pgv in pg_enum.items():
+        if pgv * n > 400:
+            assert abs(counts.get(key, 0) / n - pgv) / pgv < 0.15, (key, counts.get(key, 0) / n, pgv)
+    generated = sum(counts.values()) / n
+    assert abs(generated - total) < 0.01
