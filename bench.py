@@ -112,6 +112,39 @@ def summarize_pmc(path, last_dispatches):
         print(f"| `{k}` | {c} | {len(tail)} | {sum(tail) / len(tail):.1f} |")
 
 
+def _pmc_means(path, last_dispatches):
+    import csv
+    per = {}
+    for r in csv.DictReader(open(path)):
+        per.setdefault(_kernel_short(r["Kernel_Name"]), []).append((int(r["Dispatch_Id"]),
+                                                                     float(r["Counter_Value"])))
+    out = {}
+    for k, v in per.items():
+        v.sort()
+        tail = [x[1] for x in v[-last_dispatches:]]
+        out[k] = sum(tail) / len(tail)
+    return out
+
+
+def traffic_table(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, copy_bytes, last_dispatches):
+    """HBM bytes per launch from FETCH_SIZE and WRITE_SIZE (KiB per dispatch),
+    each scaled by what the counter reads for a device copy of copy_bytes
+    (read copy_bytes, write copy_bytes)."""
+    cf = _pmc_means(cal_fetch_csv, 5)
+    cw = _pmc_means(cal_write_csv, 5)
+    kc = max(cf, key=cf.get)  # the copy kernel dominates its own run
+    sf = copy_bytes / (cf[kc] * 1024.0)
+    sw = copy_bytes / (cw[kc] * 1024.0)
+    f = _pmc_means(fetch_csv, last_dispatches)
+    w = _pmc_means(write_csv, last_dispatches)
+    table = {"fetch_scale": sf, "write_scale": sw, "copy_bytes": copy_bytes, "kernels": {}}
+    for k in sorted(set(f) & set(w)):
+        name = k.split("<")[0]
+        table["kernels"][name] = {"instance": k, "fetch_kib": f[k], "write_kib": w[k],
+                                  "hbm_bytes_per_launch": sf * f[k] * 1024.0 + sw * w[k] * 1024.0}
+    return table
+
+
 def copy_calibration(nbytes, reps):
     """A plain device copy of known size, for scaling FETCH_SIZE/WRITE_SIZE."""
     import torch
@@ -179,12 +212,19 @@ def main():
     ap.add_argument("--summarize-trace", default=None, help="rocprofv3 kernel-trace CSV to summarise")
     ap.add_argument("--summarize-pmc", default=None, help="rocprofv3 counter-collection CSV to summarise")
     ap.add_argument("--last", type=int, default=60, help="iterations (or dispatches) summarised")
+    ap.add_argument("--traffic", nargs=4, default=None,
+                    metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="HBM bytes per launch from counter runs, scaled by a copy calibration")
     args = ap.parse_args()
-    if args.copy_calibration:
+    if args.copy_calibration and not args.traffic:
         copy_calibration(args.copy_calibration, 5)
         return
     if args.summarize_trace:
         summarize_trace(args.summarize_trace, args.last)
+        return
+    if args.traffic:
+        print(json.dumps(traffic_table(*args.traffic, copy_bytes=args.copy_calibration or 268435456,
+                                       last_dispatches=args.last), indent=1))
         return
     if args.summarize_pmc:
         summarize_pmc(args.summarize_pmc, args.last)
@@ -233,10 +273,10 @@ def main():
     b = algorithmic_bytes(dom, counts, W=eng.W, S=eng.S, nbasis=spec.nbasis, psingles=spec.psingles)
     achieved = b / (per_stage[dom] * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{KERNELS[dom]}_mol10e20o.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_mol10e20o.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            traffic = json.load(f)["kernels"].get(KERNELS[dom], {}).get("hbm_bytes_per_launch")
     line = {
         "metric": METRIC, "value": walker_iters / dt, "unit": "walker-iterations/s", "n_gpus": 1,
         "steps": K, "warmup": args.warmup, "ms_per_step": dt * 1e3 / K, "higher_is_better": True,

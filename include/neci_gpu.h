@@ -108,6 +108,22 @@ typedef struct {
 int ngpu_create(ngpu_t **h, const ngpu_sys_t *sys);
 int ngpu_destroy(ngpu_t *h);
 
+/* FCIDUMP reader (host code, no device work): the &FCI namelist that
+   READFCIINT reads (readint.F90:614-636).  orbsym receives up to orbsym_cap
+   ORBSYM labels (may be NULL).  Unrestricted (UHF, IUHF) and relativistic
+   (TREL) dumps are refused with NGPU_ERR_ARG, as is a missing file. */
+int ngpu_fcidump_header(const char *path, int32_t *norb, int32_t *nelec, int32_t *ms2, int32_t *isym,
+                        int32_t *orbsym, int32_t orbsym_cap);
+/* FCIDUMP integrals (READFCIINT body, readint.F90:700-898) into the arrays
+   ngpu_set_integrals takes: tmat2d (nbasis x nbasis, nbasis = 2*NORB), umat
+   (umat_len = P(P+1)/2 with P = NORB(NORB+1)/2) and ecore.  Lines with
+   |Z| < umat_eps and I > 0 are skipped (UMatEps, 1e-8 by default,
+   System_neci.F90:110); *n_two receives the number of two-electron integrals
+   stored (the reference's NonZeroInt).  A one-electron pair given twice with
+   values more than 1e-7 apart is NGPU_ERR_ARG, as the reference stops there. */
+int ngpu_read_fcidump(const char *path, double umat_eps, double *tmat2d, int64_t nbasis, double *umat,
+                      int64_t umat_len, double *ecore, int64_t *n_two);
+
 /* Integrals as READFCIINT leaves them (readint.F90:585-950): TMAT2D(nbasis,nbasis)
    column major; UMAT packed by UMatInd_base (UMatCache.F90:257) over spatial
    orbitals, UMAT(1) at umat[0]; ECORE.  Selects get_spawn_helement =

@@ -90,6 +90,10 @@ _vp = ctypes.c_void_p
 _SIGS = [
     ("ngpu_create", ctypes.c_int, [P(_vp), P(SysT)]),
     ("ngpu_destroy", ctypes.c_int, [_vp]),
+    ("ngpu_fcidump_header", ctypes.c_int, [ctypes.c_char_p, P(ctypes.c_int32), P(ctypes.c_int32),
+                                           P(ctypes.c_int32), P(ctypes.c_int32), _vp, ctypes.c_int32]),
+    ("ngpu_read_fcidump", ctypes.c_int, [ctypes.c_char_p, ctypes.c_double, _vp, ctypes.c_int64, _vp,
+                                         ctypes.c_int64, P(ctypes.c_double), P(ctypes.c_int64)]),
     ("ngpu_set_integrals", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_double]),
     ("ngpu_set_lattice", ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                         ctypes.c_double]),

@@ -12,6 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc", "ngpu_engine.hip")
+HOST_SRC = [os.path.join(HERE, "csrc", "ngpu_fcidump.cpp")]   # host-only parts of the library
 OUT = os.path.join(HERE, "libneci_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
@@ -20,7 +21,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp
 
 
 def build_library(verbose=False):
-    cmd = [HIPCC] + FLAGS + ["-o", OUT + ".tmp", SRC]
+    cmd = [HIPCC] + FLAGS + ["-o", OUT + ".tmp", SRC] + HOST_SRC
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -354,7 +354,9 @@ void iterate_chain(ngpu_t* h, const ngpu_iter_in_t* in) {
   launch(h, 1, -1, k_scan_partials, dim3(1), dim3(kBlock), h->blk_part, h->nb_chunk, h->cnt);
   launch(h, -1, 1, k_scan_apply, dim3(h->nb_chunk), dim3(kBlock), (const int32_t*)h->att_cnt, h->att_off,
          (const int64_t*)h->blk_part, h->cnt, h->S, h->tile_p, h->tcap);
-  launch(h, 2, 2, k_spawn<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
+  auto spawn = h->sys.model == NGPU_MODEL_HUBBARD ? k_spawn<W, R, kGenHubbard>
+             : h->ds.uniform ? k_spawn<W, R, kGenUniform> : k_spawn<W, R, kGenPchb>;
+  launch(h, 2, 2, spawn, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
          (const int64_t*)h->att_off, (const int64_t*)h->tile_p, h->tcap, (const uint8_t*)h->pinfo, h->sp_rec,
          h->scap, h->sq, in->tau, in->iter);
   if (h->sys.model == NGPU_MODEL_MOLECULAR && !h->ds.uniform) {

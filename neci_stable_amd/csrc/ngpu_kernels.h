@@ -642,9 +642,14 @@ struct SpawnAcc {
   }
 };
 
+// generator bound at compile time, one k_spawn instantiation each, so that the
+// code of one generator (the uniform one's private arrays in particular) costs
+// the others nothing
+constexpr int kGenHubbard = 0, kGenPchb = 1, kGenUniform = 2;
+
 // one spawning attempt (attempt_create_normal + create_particle); mode 0: any
 // excitation, 1: doubles only (singles are queued), 2: singles only.  Wave-wide.
-template <int W, int R>
+template <int W, int R, int G>
 __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* cnt, BlockStats& bs,
                                      SpawnAcc<R>& acc, const uint8_t* pinfo, bool active, int64_t p, uint32_t k,
                                      int mode, SpawnStage<W>* ss, QueueStage* qs, double tau, uint32_t iter) {
@@ -663,16 +668,16 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* 
     for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
     double dr[6];
     draw2<W>(s.seed, key, iter, r, k, 1, dr[0], dr[1]);
-    bool single = s.model == 1 && !s.uniform && dr[0] < s.psingles;
+    bool single = G == kGenPchb && dr[0] < s.psingles;
     if (mode == 1 && single) {
       queue = true;
     } else {
       draw2<W>(s.seed, key, iter, r, k, 2, dr[2], dr[3]);
       draw2<W>(s.seed, key, iter, r, k, 3, dr[4], dr[5]);
       Excit ex;
-      if (s.model == 0) {
+      if constexpr (G == kGenHubbard) {
         ex = gen_hubbard<W>(s, key, dr);
-      } else if (s.uniform) {
+      } else if constexpr (G == kGenUniform) {
         bool fail = false;
         ex = gen_excit<W>(s, key, dr, iter, r, k, &fail);
         if (fail) atomicOr(&cnt->err, 1u << NGPU_ERR_EXCITGEN);
@@ -738,7 +743,7 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* 
 // all attempts: Hubbard spawns directly; molecular doubles spawn here while
 // singles (the heat-bath list is ~20x longer per attempt) go to a queue so that
 // waves do not idle on a few long lanes
-template <int W, int R>
+template <int W, int R, int G>
 __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* cnt, double* part,
                                                   const int64_t* att_off, const int64_t* tile_p, int64_t tcap,
                                                   const uint8_t* pinfo, uint64_t* sp_rec, int64_t scap,
@@ -754,7 +759,7 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
   const int64_t total = (int64_t)cnt->n_attempts;
   const int64_t npairs = (int64_t)cnt->nslots * s.S;
   bool overflow = false;
-  int mode = (s.model == 1 && !s.uniform) ? 1 : 0;
+  constexpr int mode = G == kGenPchb ? 1 : 0;
   SpawnAcc<R> acc;
   __shared__ int64_t pb[2];
   __shared__ int64_t woff[kParentWindow];
@@ -803,7 +808,7 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
       p = parent_pair(att_off, lo, hi, a);
       k = (uint32_t)(a - att_off[p]);
     }
-    spawn_attempt<W, R>(s, wk, cnt, bs, acc, pinfo, act, p, k, mode, &ss, &qs, tau, iter);
+    spawn_attempt<W, R, G>(s, wk, cnt, bs, acc, pinfo, act, p, k, mode, &ss, &qs, tau, iter);
     spawn_flush<W>(&ss, cnt, sp_rec, scap, false, overflow);
     if (mode == 1) queue_flush(&qs, cnt, sq, scap, false, overflow);
   }

@@ -4,7 +4,7 @@ These are the host responsibilities NECI keeps outside the iteration
 (SysInit/IntInit/DetInit, NECICore.F90:292-365): reading integrals, building
 the lattice, choosing the reference, RandomHash2 and pSingles.
 """
-import re
+import os
 
 import numpy as np
 
@@ -198,57 +198,36 @@ def write_fcidump(path, mol, thresh=1e-12):
 
 
 def read_fcidump(path, umat_eps=1e-8):
-    """READFCIINT (readint.F90:585-950), spin-restricted, real integrals.
-
-    Returns a MolecularSystem whose tmat/umat are filled line by line exactly
-    as the reference does: UMAT(UMatInd(I,K,J,L)) = Z, TMAT2D(2I-s, 2J-s) = Z for
-    both spins (ISPINS = 2), ECORE from the 0 0 0 0 line; |Z| < UMatEps ignored.
-    """
-    with open(path) as f:
-        text = f.read()
-    m = re.search(r"&FCI(.*?)(&END|/)", text, re.S | re.I)
-    if not m:
-        raise ValueError("FCIDUMP: no &FCI namelist")
-    nl = m.group(1)
-
-    def nlval(key, default=None):
-        mm = re.search(key + r"\s*=\s*(-?\d+)", nl, re.I)
-        return int(mm.group(1)) if mm else default
-
-    norb = nlval("NORB")
-    nelec = nlval("NELEC")
-    ms2 = nlval("MS2", 0)
-    if nlval("UHF", 0):
-        raise ValueError("FCIDUMP: UHF integrals are not supported by this reader")
-    body = text[m.end():]
-    nbasis = 2 * norb
+    """READFCIINT (readint.F90:585-950) through the engine library's reader
+    (ngpu_fcidump_header + ngpu_read_fcidump): spin-restricted real integrals
+    as the reference stores them, UMAT(UMatInd(I,K,J,L)) = Z, TMAT2D(2I-s, 2J-s) = Z
+    for both spins, ECORE from the 0 0 0 0 line, |Z| < UMatEps skipped.  Returns
+    a MolecularSystem with tmat/umat filled, plus .orbsym, .isym and .n_two
+    (the reference's NonZeroInt)."""
+    import ctypes
+    from ._lib import check, load_library, ptr
+    lib = load_library()
+    bpath = os.fsencode(path)
+    norb, nelec, ms2, isym = (ctypes.c_int32() for _ in range(4))
+    check(lib.ngpu_fcidump_header(bpath, ctypes.byref(norb), ctypes.byref(nelec), ctypes.byref(ms2),
+                                  ctypes.byref(isym), None, 0), "ngpu_fcidump_header")
+    n = norb.value
+    orbsym = np.zeros(n, dtype=np.int32)
+    check(lib.ngpu_fcidump_header(bpath, None, None, None, None, ptr(orbsym), n), "ngpu_fcidump_header")
+    nbasis = 2 * n
     tmat = np.zeros((nbasis, nbasis))
-    umat = np.zeros(umat_len(norb))
-    ecore = 0.0
-    h1 = np.zeros((norb, norb))
-    for line in body.splitlines():
-        parts = line.replace(",", " ").split()
-        if len(parts) < 5:
-            continue
-        z = float(parts[0].replace("D", "E").replace("d", "e"))
-        i, j, k, l = (int(x) for x in parts[1:5])
-        if abs(z) < umat_eps and i > 0:
-            continue
-        if i == 0:
-            ecore = z
-        elif j == 0:
-            continue
-        elif k == 0:
-            for ispn in (1, 2):
-                tmat[2 * i - ispn, 2 * j - ispn] = z
-                tmat[2 * j - ispn, 2 * i - ispn] = z
-            h1[i - 1, j - 1] = z
-            h1[j - 1, i - 1] = z
-        else:
-            umat[umat_ind(i, k, j, l) - 1] = z
-    mol = MolecularSystem(norb, nelec, h1, None, ecore=ecore, ms2=ms2)
+    umat = np.zeros(umat_len(n))
+    ecore = ctypes.c_double()
+    n_two = ctypes.c_int64()
+    check(lib.ngpu_read_fcidump(bpath, umat_eps, ptr(tmat), nbasis, ptr(umat), len(umat), ctypes.byref(ecore),
+                                ctypes.byref(n_two)), "ngpu_read_fcidump")
+    h1 = tmat[0::2, 0::2].copy()
+    mol = MolecularSystem(n, nelec.value, h1, None, ecore=ecore.value, ms2=ms2.value)
     mol.tmat = tmat
     mol.umat = umat
+    mol.orbsym = orbsym
+    mol.isym = isym.value
+    mol.n_two = n_two.value
     return mol
 
 
