@@ -34,6 +34,7 @@ extern "C" {
 #define NGPU_MAX_REPLICAS 16
 
 typedef struct ngpu_engine ngpu_t;
+typedef struct ngpu_batch ngpu_batch_t;
 
 typedef enum {
     NGPU_OK = 0,
@@ -186,6 +187,29 @@ int ngpu_set_core(ngpu_t *h, const int64_t *dets, int64_t ncore, int32_t fp32_va
    fills out.  A failure inside the iteration is reported by the next call that
    waits. */
 int ngpu_iterate(ngpu_t *h, const ngpu_iter_in_t *in, ngpu_iter_out_t *out);
+
+/* Replay one captured graph of the iteration's launches instead of issuing
+   them one by one (on != 0).  The kernels read tau, DiagSft and Iter from
+   device memory, written by one small launch before each replay, so the graph
+   is captured once and recaptured only after a set-up call changes the chain
+   (ngpu_set_core, ngpu_set_excitgen, ...).  No counterpart in the reference
+   (its iteration is a host loop); results are identical either way. */
+int ngpu_set_graph(ngpu_t *h, int32_t on);
+
+/* Batches: several independent engines (different systems, sizes, replica
+   counts; one W for all) iterated together as ONE graph per iteration, one
+   branch per engine, launched on one stream.  No counterpart in the
+   reference, which runs one system per process; each engine's results are
+   those of iterating it alone.  While batched, an engine is iterated only
+   through ngpu_batch_iterate (ngpu_iterate returns NGPU_ERR_STATE); its
+   statistics, downloads and set-up calls work as before and are ordered
+   after the batch's iterations.  ngpu_batch_destroy returns the engines to
+   independent use; destroy the batch before its engines. */
+int ngpu_batch_create(ngpu_batch_t **b, ngpu_t *const *hs, int32_t n);
+/* One iteration of every engine of the batch; ins holds one input per engine,
+   in the order given to ngpu_batch_create.  Asynchronous. */
+int ngpu_batch_iterate(ngpu_batch_t *b, const ngpu_iter_in_t *ins);
+int ngpu_batch_destroy(ngpu_batch_t *b);
 /* Wait for the last iteration and return its statistics. */
 int ngpu_get_stats(ngpu_t *h, ngpu_iter_out_t *out);
 /* Update cycle (calculate_new_shift_wrapper, fcimc_iter_utilities.F90:1403):

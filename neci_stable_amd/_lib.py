@@ -109,6 +109,10 @@ _SIGS = [
     ("ngpu_get_stats", ctypes.c_int, [_vp, P(IterOutT)]),
     ("ngpu_get_cycle_stats", ctypes.c_int, [_vp, P(IterOutT), P(IterOutT), P(ctypes.c_int32)]),
     ("ngpu_set_profiling", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("ngpu_set_graph", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("ngpu_batch_create", ctypes.c_int, [P(_vp), _vp, ctypes.c_int32]),
+    ("ngpu_batch_iterate", ctypes.c_int, [_vp, _vp]),
+    ("ngpu_batch_destroy", ctypes.c_int, [_vp]),
     ("ngpu_get_stage_times", ctypes.c_int, [_vp, _vp, ctypes.c_int32, P(ctypes.c_int64)]),
     ("ngpu_find_walker_hash_batch", ctypes.c_int, [_vp, _vp, ctypes.c_int64, _vp]),
     ("ngpu_get_spawn_helement_batch", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, _vp]),

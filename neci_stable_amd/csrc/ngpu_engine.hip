@@ -117,6 +117,31 @@ struct ngpu_engine {
   double stage_ms[kStages] = {0};
   int64_t prof_iters = 0;
   ngpu_iter_out_t* dcyc = nullptr;  // statistics summed over the update cycle
+  // per-iteration inputs on the device; while batched this points into the
+  // batch's array and own_params keeps the engine's allocation
+  IterParams* dparams = nullptr;
+  IterParams* own_params = nullptr;
+  // replay of the captured iteration chain
+  bool use_graph = false;
+  bool graph_dirty = true;
+  hipGraphExec_t gexec = nullptr;
+  ngpu_batch* batch = nullptr;
+  hipStream_t own_stream = nullptr;
+};
+
+struct ngpu_batch {
+  std::vector<ngpu_t*> hs;
+  hipStream_t stream = nullptr;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  IterParams* dparams = nullptr;                 // one entry per engine
+  static constexpr int kRing = 4;
+  IterParams* hparams[kRing] = {nullptr};        // pinned staging slots
+  hipEvent_t slot_done[kRing] = {nullptr};
+  bool slot_used[kRing] = {false};
+  int next_slot = 0;
 };
 
 namespace {
@@ -219,7 +244,8 @@ void launch_lds(ngpu_t* h, int open, int close, F kern, dim3 grid, dim3 block, s
       h->ev_mask[h->ev_cur] |= 1u << close;
     }
   }
-  hipExtLaunchKernelGGL(kern, grid, block, (uint32_t)lds, h->stream, e0, e1, 0, args...);
+  if (e0 || e1) hipExtLaunchKernelGGL(kern, grid, block, (uint32_t)lds, h->stream, e0, e1, 0, args...);
+  else hipLaunchKernelGGL(kern, grid, block, (uint32_t)lds, h->stream, args...);
 }
 
 template <typename F, typename... Args>
@@ -232,7 +258,8 @@ void launch(ngpu_t* h, int open, int close, F kern, dim3 grid, dim3 block, Args.
       h->ev_mask[h->ev_cur] |= 1u << close;
     }
   }
-  hipExtLaunchKernelGGL(kern, grid, block, 0, h->stream, e0, e1, 0, args...);
+  if (e0 || e1) hipExtLaunchKernelGGL(kern, grid, block, 0, h->stream, e0, e1, 0, args...);
+  else hipLaunchKernelGGL(kern, grid, block, 0, h->stream, args...);
 }
 
 // tombstone reclamation and the load guard, launched unconditionally (the
@@ -307,7 +334,7 @@ void sort_and_merge(ngpu_t* h) {
 
 // semistochastic projection of the core amplitudes (before spawning)
 template <int R>
-void core_projection(ngpu_t* h, Shift sft, double tau) {
+void core_projection(ngpu_t* h) {
   int nb = (int)std::min<int64_t>((h->ncore + kBlock - 1) / kBlock, h->nb_grid);
   launch(h, -1, -1, k_core_gather<R>, dim3(nb), dim3(kBlock), (const int64_t*)h->core_slot, (const double*)h->wk.sgn,
          h->S, h->ncore, h->core_x);
@@ -315,42 +342,41 @@ void core_projection(ngpu_t* h, Shift sft, double tau) {
   if (h->core_fp32)
     launch(h, -1, -1, k_core_spmv<R, float>, dim3(nbw), dim3(kBlock), (const int64_t*)h->core_rowptr,
            (const int32_t*)h->core_cols, (const float*)h->core_vals, (const double*)h->core_x, h->core_y, h->ncore,
-           h->S, sft, tau);
+           h->S, (const IterParams*)h->dparams);
   else
     launch(h, -1, -1, k_core_spmv<R, double>, dim3(nbw), dim3(kBlock), (const int64_t*)h->core_rowptr,
            (const int32_t*)h->core_cols, (const double*)h->core_vals, (const double*)h->core_x, h->core_y, h->ncore,
-           h->S, sft, tau);
+           h->S, (const IterParams*)h->dparams);
 }
 
 // merge -> annihilate -> insert -> stats, shared by iterate and the explicit-list entry
 template <int W, int R>
-void annihilation_chain(ngpu_t* h, uint32_t iter, bool with_core) {
+void annihilation_chain(ngpu_t* h, bool with_core) {
   sort_and_merge<W, R>(h);
   if (h->ncore && with_core)
     launch(h, -1, -1, k_core_apply<R>, dim3((int)std::min<int64_t>((h->ncore + kBlock - 1) / kBlock, h->nb_grid)),
            dim3(kBlock), h->ds, h->wk, (const int64_t*)h->core_slot, (const double*)h->core_y, h->ncore, h->dpart);
   launch(h, 6, 6, k_annihilate<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
          (const uint64_t*)h->u_key, (const double*)h->u_sgn, (const uint32_t*)h->u_flag, h->n_key, h->n_sgn,
-         h->n_flag, iter);
+         h->n_flag, (const IterParams*)h->dparams);
   launch(h, 7, 7, k_insert<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt,
          (const uint64_t*)h->n_key, (const double*)h->n_sgn, (const uint32_t*)h->n_flag,
          (const unsigned long long*)&h->cnt->n_new);
   launch(h, -1, -1, k_post_insert, dim3(1), dim3(1), h->cnt, (const unsigned long long*)&h->cnt->n_new,
          h->wk.cap);
-  launch(h, 8, 8, k_stats<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart, iter);
+  launch(h, 8, 8, k_stats<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
+         (const IterParams*)h->dparams);
   hash_upkeep<W>(h);
   launch(h, -1, -1, k_end_iter, dim3(1), dim3(kBlock), h->dout, (const double*)h->dpart, h->dcyc, h->cnt,
          h->scap);
 }
 
 template <int W, int R>
-void iterate_chain(ngpu_t* h, const ngpu_iter_in_t* in) {
-  Shift sft;
-  for (int r = 0; r < kMaxRep; ++r) sft.v[r] = r < h->S ? in->diag_sft[r] : 0.0;
+void iterate_chain(ngpu_t* h) {
   launch(h, -1, -1, k_begin_iter, dim3(1), dim3(kBlock), h->dout, h->dpart, h->cnt);
   launch(h, 0, 0, k_prepass<W, R>, dim3(h->nb_chunk), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart, h->att_cnt,
-         h->pinfo, h->blk_part, sft, in->tau, in->iter);
-  if (h->ncore) core_projection<R>(h, sft, in->tau);
+         h->pinfo, h->blk_part, (const IterParams*)h->dparams);
+  if (h->ncore) core_projection<R>(h);
   launch(h, 1, -1, k_scan_partials, dim3(1), dim3(kBlock), h->blk_part, h->nb_chunk, h->cnt);
   launch(h, -1, 1, k_scan_apply, dim3(h->nb_chunk), dim3(kBlock), (const int32_t*)h->att_cnt, h->att_off,
          (const int64_t*)h->blk_part, h->cnt, h->S, h->tile_p, h->tcap);
@@ -358,7 +384,7 @@ void iterate_chain(ngpu_t* h, const ngpu_iter_in_t* in) {
              : h->ds.uniform ? k_spawn<W, R, kGenUniform> : k_spawn<W, R, kGenPchb>;
   launch(h, 2, 2, spawn, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
          (const int64_t*)h->att_off, (const int64_t*)h->tile_p, h->tcap, (const uint8_t*)h->pinfo, h->sp_rec,
-         h->scap, h->sq, in->tau, in->iter);
+         h->scap, h->sq, (const IterParams*)h->dparams);
   if (h->sys.model == NGPU_MODEL_MOLECULAR && !h->ds.uniform) {
     const int nsp = h->sys.nbasis / 2;
     const size_t lds = sizeof(double) * (size_t)kHbGroups * nsp;
@@ -366,9 +392,9 @@ void iterate_chain(ngpu_t* h, const ngpu_iter_in_t* in) {
               : nsp <= 5 * kHbGroup ? k_spawn_singles<W, R, 5>
               : nsp <= 8 * kHbGroup ? k_spawn_singles<W, R, 8> : k_spawn_singles<W, R, 16>;
     launch_lds(h, 3, 3, kern, dim3(h->nb_grid), dim3(kBlock), lds, h->ds, h->wk, h->cnt, h->dpart,
-               (const uint8_t*)h->pinfo, h->sp_rec, h->scap, (const int64_t*)h->sq, in->tau, in->iter);
+               (const uint8_t*)h->pinfo, h->sp_rec, h->scap, (const int64_t*)h->sq, (const IterParams*)h->dparams);
   }
-  annihilation_chain<W, R>(h, in->iter, true);
+  annihilation_chain<W, R>(h, true);
 }
 
 void harvest(ngpu_t* h, int slot) {
@@ -686,6 +712,8 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
   fail(dalloc(&h->dout, 1));
   fail(dalloc(&h->dpart, (size_t)kStatCopies * kStatStride));
   fail(dalloc(&h->dcyc, 1));
+  fail(dalloc(&h->own_params, 1));
+  h->dparams = h->own_params;
   fail(hipHostMalloc((void**)&h->hout, sizeof(ngpu_iter_out_t), hipHostMallocDefault));
   fail(hipHostMalloc((void**)&h->hcnt, sizeof(Counters), hipHostMallocDefault));
   if (rc == NGPU_OK) {
@@ -729,11 +757,12 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
 
 int ngpu_destroy(ngpu_t* h) {
   if (!h) return NGPU_ERR_ARG;
+  if (h->batch) return NGPU_ERR_STATE;  // ngpu_batch_destroy first
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void* ptrs[] = {h->wk.keys, h->wk.sgn, h->wk.flags, h->wk.diagH, h->wk.offdiagH, h->wk.hpos, h->wk.freeq,
                   h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->sp_rec,
                   h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->digit_tot, h->sq, h->u_key, h->u_sgn, h->u_flag, h->n_key,
-                  h->n_sgn, h->n_flag, h->cnt, h->dout, h->dpart, h->dcyc, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_binom, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
+                  h->n_sgn, h->n_flag, h->cnt, h->dout, h->dpart, h->dcyc, h->own_params, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_binom, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
                   h->core_vals, h->core_x, h->core_y, h->d_ab_tab,
                   h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt};
   for (void* p : ptrs)
@@ -744,12 +773,14 @@ int ngpu_destroy(ngpu_t* h) {
     for (int q = 0; q < kEvRing; ++q)
       for (int s = 0; s < kStages; ++s)
         for (int e = 0; e < 2; ++e) (void)hipEventDestroy(h->ev[q][s][e]);
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return NGPU_OK;
 }
 
 int ngpu_set_integrals(ngpu_t* h, const double* tmat2d, const double* umat, int64_t umat_len, double ecore) {
+  if (h) h->graph_dirty = true;  // kernel arguments of the captured chain change
   if (!h || !tmat2d || !umat || umat_len < 1 || h->sys.model != NGPU_MODEL_MOLECULAR) return NGPU_ERR_ARG;
   int64_t nsp = h->sys.nbasis / 2;
   int64_t npair = nsp * (nsp + 1) / 2;
@@ -801,6 +832,7 @@ int ngpu_set_integrals(ngpu_t* h, const double* tmat2d, const double* umat, int6
 }
 
 int ngpu_set_lattice(ngpu_t* h, const int32_t* nbr, int32_t nsites, int32_t maxnbr, double bhub, double uhub) {
+  if (h) h->graph_dirty = true;  // kernel arguments of the captured chain change
   if (!h || !nbr || h->sys.model != NGPU_MODEL_HUBBARD) return NGPU_ERR_ARG;
   if (nsites * 2 != h->sys.nbasis || maxnbr < 1 || maxnbr > 16) return NGPU_ERR_ARG;
   for (int64_t i = 0; i < (int64_t)nsites * maxnbr; ++i)
@@ -822,6 +854,7 @@ int ngpu_set_lattice(ngpu_t* h, const int32_t* nbr, int32_t nsites, int32_t maxn
 // for the FCI case and the J sampler of init_PC_WeightedParticles_t
 // (gasci_pchb_doubles_select_particles.fpp:254-301)
 int ngpu_set_excitgen(ngpu_t* h, int32_t singles, int32_t particles, double psingles, double pparallel) {
+  if (h) h->graph_dirty = true;  // kernel arguments of the captured chain change
   if (!h || h->sys.model != NGPU_MODEL_MOLECULAR || !h->have_model) return NGPU_ERR_ARG;
   if (singles != NGPU_SINGLES_HEATBATH) return NGPU_ERR_ARG;
   if (particles != NGPU_PART_UNIF_UNIF && particles != NGPU_PART_UNIF_FAST) return NGPU_ERR_ARG;
@@ -919,6 +952,7 @@ int ngpu_set_excitgen(ngpu_t* h, int32_t singles, int32_t particles, double psin
 }
 
 int ngpu_set_excitgen_uniform(ngpu_t* h, double psingles) {
+  if (h) h->graph_dirty = true;  // kernel arguments of the captured chain change
   if (!h || h->sys.model != NGPU_MODEL_MOLECULAR || !h->d_umat) return NGPU_ERR_ARG;
   if (!(psingles >= 0.0 && psingles <= 1.0)) return NGPU_ERR_ARG;
   if ((h->sys.nbasis & 1) || h->sys.nel < 2) return NGPU_ERR_ARG;
@@ -930,6 +964,7 @@ int ngpu_set_excitgen_uniform(ngpu_t* h, double psingles) {
 }
 
 int ngpu_set_hash(ngpu_t* h, const int32_t* random_hash2) {
+  if (h) h->graph_dirty = true;  // kernel arguments of the captured chain change
   if (!h || !random_hash2) return NGPU_ERR_ARG;
   for (int i = 0; i < h->sys.nbasis; ++i)
     if (random_hash2[i] < 1 || random_hash2[i] > 1000 * h->sys.nbasis) return NGPU_ERR_ARG;
@@ -941,6 +976,7 @@ int ngpu_set_hash(ngpu_t* h, const int32_t* random_hash2) {
 }
 
 int ngpu_set_reference(ngpu_t* h, const int64_t* ilut_ref, double hii) {
+  if (h) h->graph_dirty = true;  // kernel arguments of the captured chain change
   if (!h || !ilut_ref) return NGPU_ERR_ARG;
   int nel = 0;
   for (int w = 0; w < h->W; ++w) nel += __builtin_popcountll((uint64_t)ilut_ref[w]);
@@ -1043,6 +1079,7 @@ int set_core_impl(ngpu_t* h, int64_t ncore, bool fp32, int64_t* nnz_out) {
 }  // extern "C++"
 
 int ngpu_set_core(ngpu_t* h, const int64_t* dets, int64_t ncore, int32_t fp32_values, int64_t* nnz) {
+  if (h) h->graph_dirty = true;  // kernel arguments of the captured chain change
   int rc = ready(h);
   if (rc) return rc;
   if (!dets || ncore < 1 || ncore > (int64_t)INT32_MAX || ncore > h->wk.cap) return NGPU_ERR_ARG;
@@ -1117,27 +1154,221 @@ int ngpu_download_walkers(ngpu_t* h, int64_t* rows, double* diag_h, double* offd
   return m > cap ? NGPU_ERR_ARG : NGPU_OK;
 }
 
+namespace {
+
+__global__ void k_set_params(IterParams* p, IterParams v) { *p = v; }
+
+IterParams make_params(const ngpu_t* h, double tau, const double* sft, uint32_t iter) {
+  IterParams v{};
+  v.tau = tau;
+  for (int r = 0; r < kMaxRep; ++r) v.sft.v[r] = (sft && r < h->S) ? sft[r] : 0.0;
+  v.iter = iter;
+  return v;
+}
+
+// the iteration's inputs, stream-ordered before the chain that reads them
+void set_params(ngpu_t* h, double tau, const double* sft, uint32_t iter) {
+  hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, h->stream, h->dparams, make_params(h, tau, sft, iter));
+}
+
+void run_chain(ngpu_t* h) {
+  if (h->W == 1) h->S == 1 ? iterate_chain<1, 1>(h) : iterate_chain<1, kMaxRep>(h);
+  else h->S == 1 ? iterate_chain<2, 1>(h) : iterate_chain<2, kMaxRep>(h);
+}
+
+// one iteration's launches captured from the engine stream into an
+// executable graph; the kernels read tau, shifts and the iteration number
+// from h->dparams, so the graph replays every iteration unchanged
+int capture_engine(ngpu_t* h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  h->gexec = nullptr;
+  hipGraph_t g = nullptr;
+  HIP_OK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  bool prof = h->prof;
+  h->prof = false;
+  run_chain(h);
+  h->prof = prof;
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (e != hipSuccess) return NGPU_ERR_HIP;
+  e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    h->gexec = nullptr;
+    return NGPU_ERR_HIP;
+  }
+  h->graph_dirty = false;
+  return NGPU_OK;
+}
+
+}  // namespace
+
 int ngpu_iterate(ngpu_t* h, const ngpu_iter_in_t* in, ngpu_iter_out_t* out) {
   int rc = ready(h);
   if (rc) return rc;
   if (!in || !(in->tau > 0.0)) return NGPU_ERR_ARG;
+  if (h->batch) return NGPU_ERR_STATE;  // batched engines iterate through ngpu_batch_iterate
   // iterations queue on the stream back to back: nothing in the chain waits
   // for the host; errors surface at the next synchronising call
-  if (h->prof && h->ev_ok) {
-    h->ev_cur = (int)(h->ev_next % kEvRing);
-    harvest(h, h->ev_cur);
-    h->ev_mask[h->ev_cur] = 0;
+  set_params(h, in->tau, in->diag_sft, in->iter);
+  if (h->use_graph && !h->prof) {
+    if (h->graph_dirty || !h->gexec) {
+      rc = capture_engine(h);
+      if (rc) return rc;
+    }
+    HIP_OK(hipGraphLaunch(h->gexec, h->stream));
+  } else {
+    if (h->prof && h->ev_ok) {
+      h->ev_cur = (int)(h->ev_next % kEvRing);
+      harvest(h, h->ev_cur);
+      h->ev_mask[h->ev_cur] = 0;
+    }
+    run_chain(h);
+    if (h->prof && h->ev_ok) {
+      h->ev_pending[h->ev_cur] = true;
+      h->ev_next++;
+    }
   }
-  if (h->W == 1) h->S == 1 ? iterate_chain<1, 1>(h, in) : iterate_chain<1, kMaxRep>(h, in);
-  else h->S == 1 ? iterate_chain<2, 1>(h, in) : iterate_chain<2, kMaxRep>(h, in);
   HIP_OK(hipGetLastError());
-  if (h->prof && h->ev_ok) {
-    h->ev_pending[h->ev_cur] = true;
-    h->ev_next++;
-  }
   if (out) HIP_OK(hipMemcpyAsync(h->hout, h->dout, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));
   h->pending = true;
   if (out) return finish(h, out);
+  return NGPU_OK;
+}
+
+int ngpu_set_graph(ngpu_t* h, int32_t on) {
+  if (!h) return NGPU_ERR_ARG;
+  h->use_graph = on != 0;
+  return NGPU_OK;
+}
+
+// ---- batches: several engines, one captured graph per iteration ----
+namespace {
+
+int capture_batch(ngpu_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  hipGraph_t g = nullptr;
+  HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  HIP_OK(hipEventRecord(b->fork, b->stream));
+  for (size_t i = 0; i < b->hs.size(); ++i) {
+    ngpu_t* h = b->hs[i];
+    HIP_OK(hipStreamWaitEvent(b->side[i], b->fork, 0));
+    h->stream = b->side[i];
+    bool prof = h->prof;
+    h->prof = false;
+    run_chain(h);
+    h->prof = prof;
+    h->stream = b->stream;
+    HIP_OK(hipEventRecord(b->join[i], b->side[i]));
+    HIP_OK(hipStreamWaitEvent(b->stream, b->join[i], 0));
+  }
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess) return NGPU_ERR_HIP;
+  e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    b->gexec = nullptr;
+    return NGPU_ERR_HIP;
+  }
+  for (ngpu_t* h : b->hs) h->graph_dirty = false;
+  return NGPU_OK;
+}
+
+void batch_release(ngpu_batch* b) {
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (ngpu_t* h : b->hs) {
+    h->stream = h->own_stream;
+    h->dparams = h->own_params;
+    h->batch = nullptr;
+    h->graph_dirty = true;
+  }
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  for (hipStream_t st : b->side) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : b->join) (void)hipEventDestroy(ev);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  for (int q = 0; q < ngpu_batch::kRing; ++q) {
+    if (b->hparams[q]) (void)hipHostFree(b->hparams[q]);
+    if (b->slot_done[q]) (void)hipEventDestroy(b->slot_done[q]);
+  }
+  if (b->dparams) (void)hipFree(b->dparams);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+}  // namespace
+
+int ngpu_batch_create(ngpu_batch** out, ngpu_t* const* hs, int32_t n) {
+  if (!out || !hs || n < 1) return NGPU_ERR_ARG;
+  *out = nullptr;
+  for (int32_t i = 0; i < n; ++i) {
+    int rc = ready(hs[i]);
+    if (rc) return rc;
+    if (hs[i]->batch) return NGPU_ERR_STATE;
+    for (int32_t j = 0; j < i; ++j)
+      if (hs[j] == hs[i]) return NGPU_ERR_ARG;
+  }
+  auto* b = new ngpu_batch;
+  bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
+            hipMalloc((void**)&b->dparams, sizeof(IterParams) * (size_t)n) == hipSuccess;
+  for (int q = 0; ok && q < ngpu_batch::kRing; ++q)
+    ok = hipHostMalloc((void**)&b->hparams[q], sizeof(IterParams) * (size_t)n, hipHostMallocDefault) == hipSuccess &&
+         hipEventCreateWithFlags(&b->slot_done[q], hipEventDisableTiming) == hipSuccess;
+  for (int32_t i = 0; ok && i < n; ++i) {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+    if (ok) b->side.push_back(st);
+    ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+    if (ok) b->join.push_back(ev);
+  }
+  if (!ok) {
+    batch_release(b);
+    return NGPU_ERR_HIP;
+  }
+  for (int32_t i = 0; i < n; ++i) {
+    ngpu_t* h = hs[i];
+    if (h->pending) (void)finish(h, nullptr);
+    (void)hipStreamSynchronize(h->stream);
+    h->own_stream = h->stream;
+    h->stream = b->stream;
+    h->dparams = b->dparams + i;
+    h->batch = b;
+    h->graph_dirty = true;
+    b->hs.push_back(h);
+  }
+  *out = b;
+  return NGPU_OK;
+}
+
+int ngpu_batch_iterate(ngpu_batch* b, const ngpu_iter_in_t* ins) {
+  if (!b || !ins) return NGPU_ERR_ARG;
+  const size_t n = b->hs.size();
+  for (size_t i = 0; i < n; ++i)
+    if (!(ins[i].tau > 0.0)) return NGPU_ERR_ARG;
+  // the staging slot is rewritten only after its previous copy has run
+  const int q = b->next_slot;
+  if (b->slot_used[q]) HIP_OK(hipEventSynchronize(b->slot_done[q]));
+  for (size_t i = 0; i < n; ++i)
+    b->hparams[q][i] = make_params(b->hs[i], ins[i].tau, ins[i].diag_sft, ins[i].iter);
+  HIP_OK(hipMemcpyAsync(b->dparams, b->hparams[q], sizeof(IterParams) * n, hipMemcpyHostToDevice, b->stream));
+  HIP_OK(hipEventRecord(b->slot_done[q], b->stream));
+  b->slot_used[q] = true;
+  b->next_slot = (q + 1) % ngpu_batch::kRing;
+  bool stale = !b->gexec;
+  for (ngpu_t* h : b->hs) stale = stale || h->graph_dirty;
+  if (stale) {
+    int rc = capture_batch(b);
+    if (rc) return rc;
+  }
+  HIP_OK(hipGraphLaunch(b->gexec, b->stream));
+  for (ngpu_t* h : b->hs) h->pending = true;
+  return NGPU_OK;
+}
+
+int ngpu_batch_destroy(ngpu_batch* b) {
+  if (!b) return NGPU_ERR_ARG;
+  batch_release(b);
   return NGPU_OK;
 }
 
@@ -1162,6 +1393,7 @@ int ngpu_get_cycle_stats(ngpu_t* h, ngpu_iter_out_t* sum, ngpu_iter_out_t* last,
 
 int ngpu_set_profiling(ngpu_t* h, int32_t on) {
   if (!h) return NGPU_ERR_ARG;
+  if (h->batch) return NGPU_ERR_STATE;
   if (h->pending) (void)finish(h, nullptr);
   h->prof = on != 0;
   for (int s = 0; s < kStages; ++s) h->stage_ms[s] = 0.0;
@@ -1380,8 +1612,9 @@ int ngpu_annihilate(ngpu_t* h, uint32_t iter, const int64_t* sdet, const double*
   bool p = h->prof;
   h->prof = false;
   // an explicit list carries no deterministic projection
-  if (W == 1) h->S == 1 ? annihilation_chain<1, 1>(h, iter, false) : annihilation_chain<1, kMaxRep>(h, iter, false);
-  else h->S == 1 ? annihilation_chain<2, 1>(h, iter, false) : annihilation_chain<2, kMaxRep>(h, iter, false);
+  set_params(h, 1.0, nullptr, iter);
+  if (W == 1) h->S == 1 ? annihilation_chain<1, 1>(h, false) : annihilation_chain<1, kMaxRep>(h, false);
+  else h->S == 1 ? annihilation_chain<2, 1>(h, false) : annihilation_chain<2, kMaxRep>(h, false);
   h->prof = p;
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(h->hout, h->dout, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));

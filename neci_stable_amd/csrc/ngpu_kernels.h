@@ -88,6 +88,14 @@ struct Walk {
 };
 
 struct Shift { double v[kMaxRep]; };
+// per-iteration inputs (time step, shifts, iteration number), read by the
+// kernels from device memory so that a captured iteration replays unchanged
+struct IterParams {
+  double tau;
+  Shift sft;
+  uint32_t iter;
+  uint32_t pad;
+};
 
 // ----------------------------------------------------------- wave utils
 __device__ inline int lane_id() { return __lane_id(); }
@@ -359,8 +367,10 @@ __device__ inline int64_t grid_elem(int64_t t) {
 template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters* cnt,
                                                     double* part, int32_t* att_cnt,
-                                                    uint8_t* pinfo, int64_t* blk_part, Shift sft,
-                                                    double tau, uint32_t iter) {
+                                                    uint8_t* pinfo, int64_t* blk_part, const IterParams* prm) {
+  const double tau = prm->tau;
+  const uint32_t iter = prm->iter;
+  const double* sftv = prm->sft.v;
   __shared__ double acc_lds[kStatStride];
   __shared__ int64_t red[kBlock / kWave];
   __shared__ FreeStage fs;
@@ -431,7 +441,7 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
         att_cnt[slot * S + r] = (int32_t)nsp;
         my_att += nsp;
         pinfo[slot * S + r] = (uint8_t)((sg[r] < 0.0 ? 1 : 0) | (((f >> r) & 1u) ? 2 : 0) | (core ? 4 : 0));
-        double fac = tau * (kii - sft.v[r]);
+        double fac = tau * (kii - sftv[r]);
         if (fac > 2.0) death_err = true;
         double idie = core ? 0.0 : attempt_die(s.t_real != 0, fac, fabs(sg[r]), ub);
         double a = fabs(sg[r]);
@@ -747,8 +757,9 @@ template <int W, int R, int G>
 __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* cnt, double* part,
                                                   const int64_t* att_off, const int64_t* tile_p, int64_t tcap,
                                                   const uint8_t* pinfo, uint64_t* sp_rec, int64_t scap,
-                                                  int64_t* sq, double tau,
-                                                  uint32_t iter) {
+                                                  int64_t* sq, const IterParams* prm) {
+  const double tau = prm->tau;
+  const uint32_t iter = prm->iter;
   __shared__ double acc_lds[kStatStride];
   __shared__ SpawnStage<W> ss;
   __shared__ QueueStage qs;
@@ -869,7 +880,9 @@ __device__ inline void hb_rows(uint64_t x, int base, const double* row, int nsp,
 template <int W, int R, int PL>
 __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Counters* cnt, double* part,
                                                           const uint8_t* pinfo, uint64_t* sp_rec, int64_t scap,
-                                                          const int64_t* sq, double tau, uint32_t iter) {
+                                                          const int64_t* sq, const IterParams* prm) {
+  const double tau = prm->tau;
+  const uint32_t iter = prm->iter;
   __shared__ double acc_lds[kStatStride];
   __shared__ SpawnStage<W> ss;
   extern __shared__ double hv_dyn[];  // signed H(src -> b) of each group's attempt
@@ -1339,7 +1352,8 @@ template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counters* cnt, double* part,
                                                        const uint64_t* u_key, const double* u_sgn,
                                                        const uint32_t* u_flag, uint64_t* n_key,
-                                                       double* n_sgn, uint32_t* n_flag, uint32_t iter) {
+                                                       double* n_sgn, uint32_t* n_flag, const IterParams* prm) {
+  const uint32_t iter = prm->iter;
   __shared__ double acc_lds[kStatStride];
   __shared__ ListStage<W, R> ls;
   __shared__ FreeStage fs;
@@ -1455,8 +1469,10 @@ __global__ void __launch_bounds__(kBlock) k_core_gather(const int64_t* core_slot
 // y = (acc + S*x_i) * tau.  The CPU restatement sums in exactly this order.
 template <int R, typename VT>
 __global__ void __launch_bounds__(kBlock) k_core_spmv(const int64_t* rowptr, const int32_t* cols, const VT* vals,
-                                                      const double* x, double* y, int64_t ncore, int S, Shift sft,
-                                                      double tau) {
+                                                      const double* x, double* y, int64_t ncore, int S,
+                                                      const IterParams* prm) {
+  const double tau = prm->tau;
+  const double* sftv = prm->sft.v;
   const int lane = lane_id();
   const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
   for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < ncore; i += nwaves) {
@@ -1473,7 +1489,7 @@ __global__ void __launch_bounds__(kBlock) k_core_spmv(const int64_t* rowptr, con
       for (int o = kWave / 2; o > 0; o >>= 1) acc[r] = acc[r] + __shfl_xor(acc[r], o);
     }
     if (lane == 0)
-      for (int r = 0; r < S; ++r) y[i * S + r] = (acc[r] + sft.v[r] * x[i * S + r]) * tau;
+      for (int r = 0; r < S; ++r) y[i * S + r] = (acc[r] + sftv[r] * x[i * S + r]) * tau;
   }
 }
 
@@ -1638,7 +1654,8 @@ __global__ void k_post_insert(Counters* cnt, const unsigned long long* n_ptr, in
 // CalcHashTableStats (load_balancer.fpp:646-805)
 template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* cnt, double* part,
-                                                  uint32_t iter) {
+                                                  const IterParams* prm) {
+  const uint32_t iter = prm->iter;
   __shared__ double acc_lds[kStatStride];
   __shared__ FreeStage fs;
   BlockStats bs{acc_lds};

@@ -64,6 +64,8 @@ class Engine:
 
     # -- lifetime
     def close(self):
+        if getattr(self, "batch", None) is not None:
+            self.batch.close()
         if getattr(self, "h", None):
             self.lib.ngpu_destroy(self.h)
             self.h = None
@@ -152,6 +154,10 @@ class Engine:
         self.ncore = dets.shape[0]
         return nnz.value
 
+    def set_graph(self, on=True):
+        """Replay one captured graph per iteration (ngpu_set_graph)."""
+        check(self.lib.ngpu_set_graph(self.h, int(on)), "ngpu_set_graph")
+
     def set_profiling(self, on=True):
         check(self.lib.ngpu_set_profiling(self.h, int(on)), "ngpu_set_profiling")
 
@@ -236,3 +242,44 @@ class Engine:
                                        ptr(md), ptr(ms), ptr(mf), ctypes.byref(nm)), "ngpu_annihilate")
         k = nm.value
         return out.to_dict(self.S), md[:k], ms[:k], mf[:k]
+
+
+class Batch:
+    """Several engines iterated as one captured graph per iteration
+    (ngpu_batch_create / ngpu_batch_iterate).  Engines keep their own
+    statistics and walkers; each one's results are those of running it alone."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        self.lib = _lib.load_library()
+        hs = (ctypes.c_void_p * len(self.engines))(*[e.h.value for e in self.engines])
+        b = ctypes.c_void_p()
+        check(self.lib.ngpu_batch_create(ctypes.byref(b), hs, len(self.engines)), "ngpu_batch_create")
+        self.b = b
+        self._ins = (IterInT * len(self.engines))()
+        for e in self.engines:
+            e.batch = self
+
+    def iterate(self, its, taus, shifts):
+        """One iteration of every engine: its[i], taus[i], shifts[i] (scalar or per replica)."""
+        for i, e in enumerate(self.engines):
+            inp = self._ins[i]
+            inp.tau = float(taus[i])
+            sh = np.broadcast_to(np.asarray(shifts[i], dtype=np.float64), (e.S,))
+            for r in range(e.S):
+                inp.diag_sft[r] = float(sh[r])
+            inp.iter = int(its[i]) & 0xFFFFFFFF
+        check(self.lib.ngpu_batch_iterate(self.b, self._ins), "ngpu_batch_iterate")
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.ngpu_batch_destroy(self.b)
+            self.b = None
+            for e in self.engines:
+                e.batch = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

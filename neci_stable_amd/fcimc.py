@@ -145,6 +145,44 @@ class FciMCPar:
             return self.sum_enum / self.sum_noathf + self.hii
 
 
+class BatchedFciMCPar:
+    """FciMCPar for several engines at once: one ngpu_batch_iterate per
+    iteration runs every engine's iteration; each driver keeps its own shift
+    and update cycle (all drivers share StepsSft)."""
+
+    def __init__(self, batch, drivers):
+        self.batch = batch
+        self.drivers = list(drivers)
+        steps = {d.steps_shift for d in self.drivers}
+        if len(steps) != 1:
+            raise ValueError("batched drivers must share StepsSft")
+        self.steps_shift = steps.pop()
+
+    def run(self, n_iter):
+        t0 = time.perf_counter()
+        done = 0
+        while done < n_iter:
+            d0 = self.drivers[0]
+            k = min(self.steps_shift - d0.iter % self.steps_shift, n_iter - done)
+            for _ in range(k):
+                for d in self.drivers:
+                    d.iter += 1
+                self.batch.iterate([d.iter for d in self.drivers], [d.tau for d in self.drivers],
+                                   [d.shift for d in self.drivers])
+            done += k
+            for d in self.drivers:
+                sm, last, n = d.b.cycle_stats()
+                d.walker_iters += float(np.sum(sm["tot_parts"]))
+                d.last = last
+                if d.iter % d.steps_shift == 0:
+                    d._update_shift(sm, last, n)
+        return time.perf_counter() - t0
+
+    @property
+    def walker_iters(self):
+        return sum(d.walker_iters for d in self.drivers)
+
+
 def reblock(x):
     """Flyvbjerg-Petersen reblocking (the method of ErrorAnalysis.F90:374-1145):
     returns (mean, error) taking the plateau of the blocked standard error."""
