@@ -198,6 +198,98 @@ def cpu_baseline(spec, eng, drv, seconds):
                        f"single-threaded oracle, {dt:.1f} s")
 
 
+def batched_workload(n_mol_replicas=16, n_hub=16, seed=2024, mol_walkers=5e6, ncore=50000):
+    """32 independent problems of uneven size: the 14e/30o molecular
+    Hamiltonian (seed 7, rank 60) with n_mol_replicas replicas of distinct
+    random streams and a core of the ncore most populated determinants, and
+    n_hub half-filled Hubbard chains of 12-18 sites, U/t uniform in [2, 8],
+    walker targets log-uniform in [1e4, 2e6]."""
+    import numpy as np
+    from neci_stable_amd import HubbardChain, Spec, synthetic_molecule
+    rng = np.random.default_rng(seed)
+    hubs = []
+    for i in range(n_hub):
+        L = int(rng.integers(12, 19))
+        u = float(rng.uniform(2.0, 8.0))
+        target = float(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+        hubs.append(dict(L=L, u=u, target=target, seed=seed + 1 + i))
+    mol = synthetic_molecule(30, 14, 60, 7)
+    hub_specs = [(Spec(HubbardChain(h["L"], h["L"], h["u"], bhub=-1.0), init_walkers=h["target"], seed=h["seed"],
+                       memory_fac_part=4.0, memory_fac_spawn=4.0, memory_fac_hash=2.0), 0.01, h) for h in hubs]
+    mol_kw = dict(particles=0, psingles=0.1, memory_fac_spawn=4.0, memory_fac_part=3.0, memory_fac_hash=2.0)
+    return mol, mol_kw, hub_specs, mol_walkers, ncore
+
+
+def run_batched(args):
+    """The same engines iterated two ways, from the same grown populations:
+    plain (each problem its own engine, iterated alone with one launch per
+    kernel, one problem after the other) and batched (all engines as the
+    branches of one captured graph per iteration).  The molecular replicas
+    are engines of one replica each with distinct seeds."""
+    import numpy as np
+    from neci_stable_amd import Spec
+    from neci_stable_amd.engine import Batch, Engine
+    from neci_stable_amd.fcimc import BatchedFciMCPar, FciMCPar
+
+    mol, mol_kw, hub_specs, mol_walkers, ncore = batched_workload(args.mol_replicas, args.hub_systems,
+                                                                  mol_walkers=args.mol_walkers)
+    tau_mol = 0.001
+
+    def grown(spec, tau, target, n0):
+        e = Engine(spec)
+        e.start_single_part(n0)
+        d = FciMCPar(e, tau=tau, target_walkers=target, steps_shift=10, shift_damp=0.5)
+        while d.single_part.any() and d.iter < args.max_growth_iters:
+            d.run(10)
+        d.run(args.warmup)
+        return e, d
+
+    def with_core(e, d):
+        dets, sg, _, _, _ = e.download()
+        o = np.argsort(-np.abs(sg).sum(axis=1), kind="stable")[:ncore]
+        e.set_core(np.ascontiguousarray(dets[np.sort(o)]), fp32=True)
+        d.run(10)
+
+    def timed(fn, drivers, k):
+        w0 = sum(d.walker_iters for d in drivers)
+        t0 = time.perf_counter()
+        fn(k)
+        dt = time.perf_counter() - t0
+        return (sum(d.walker_iters for d in drivers) - w0) / dt, dt
+
+    K = args.steps
+    engines = []
+    for r in range(args.mol_replicas):
+        spec = Spec(mol, init_walkers=mol_walkers, seed=7000 + r, **mol_kw)
+        e, d = grown(spec, tau_mol, mol_walkers, 1e3)
+        with_core(e, d)
+        engines.append((e, d))
+    for spec, tau, h in hub_specs:
+        engines.append(grown(spec, tau, h["target"], 30.0))
+    print(f"# grown: {len(engines)} engines", file=sys.stderr, flush=True)
+    drivers = [d for _, d in engines]
+    p_rate, p_dt = timed(lambda k: [d.run(k) for d in drivers], drivers, K)
+    batch = Batch([e for e, _ in engines])
+    runner = BatchedFciMCPar(batch, drivers)
+    runner.run(10)  # capture and first replays
+    b_rate, b_dt = timed(runner.run, drivers, K)
+    batch.close()
+    walkers = [float(np.sum(d.last["tot_parts"])) for d in drivers]
+    for e, _ in engines:
+        e.close()
+    line = {"metric": "walker-iterations/s on 1 GPU, batched vs plain FP64", "unit": "walker-iterations/s",
+            "n_gpus": 1, "steps": K, "batched": b_rate, "plain": p_rate, "ratio": b_rate / p_rate,
+            "ms_per_step": {"batched": b_dt * 1e3 / K, "plain": p_dt * 1e3 / K},
+            "dtype": "f64 walkers, f32-stored core Hamiltonian", "data": "synthetic",
+            "config": {"workload": f"{args.mol_replicas} replicas of molecular 14e/30o (seed 7, rank 60, "
+                                   f"{ncore}-det core, one engine each) + {len(hub_specs)} Hubbard chains "
+                                   f"of 12-18 sites",
+                       "hubbard": [dict(sites=h["L"], u=round(h["u"], 3), target=round(h["target"]))
+                                   for _, _, h in hub_specs],
+                       "walkers_per_problem": walkers}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +304,11 @@ def main():
     ap.add_argument("--summarize-trace", default=None, help="rocprofv3 kernel-trace CSV to summarise")
     ap.add_argument("--summarize-pmc", default=None, help="rocprofv3 counter-collection CSV to summarise")
     ap.add_argument("--last", type=int, default=60, help="iterations (or dispatches) summarised")
+    ap.add_argument("--batched", action="store_true",
+                    help="batched (one graph per iteration) vs plain form on the 32-problem workload")
+    ap.add_argument("--mol-replicas", type=int, default=16)
+    ap.add_argument("--hub-systems", type=int, default=16)
+    ap.add_argument("--mol-walkers", type=float, default=5e6)
     ap.add_argument("--traffic", nargs=4, default=None,
                     metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="HBM bytes per launch from counter runs, scaled by a copy calibration")
@@ -233,6 +330,9 @@ def main():
         print(json.dumps({"error": "this engine runs one GPU per process (one rank's walkers); "
                                    "multi-GPU runs are not part of this build", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.batched:
+        run_batched(args)
+        return
     import numpy as np
     from neci_stable_amd.engine import Engine
     from neci_stable_amd.fcimc import FciMCPar
@@ -286,6 +386,7 @@ def main():
                    "target_walkers": args.walkers,
                    "mean_tot_parts": walker_iters / K, "mean_determinants": counts["Nd"],
                    "attempts_per_step": counts["Na"], "spawns_per_step": counts["Ns"],
+                   "unique_spawned_per_step": counts["Nu"], "new_dets_per_step": counts["Nn"],
                    "growth_iterations": drv.iter - K - args.warmup},
         "stage_ms_per_step": per_stage,
         "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,

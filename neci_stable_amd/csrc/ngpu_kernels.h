@@ -60,7 +60,6 @@ struct alignas(256) Counters {
   unsigned long long n_live, n_tomb, n_occ;
   unsigned long long n_attempts;
   unsigned long long n_fail;      // inserts refused (capacity errors)
-  unsigned long long n_reused;    // inserts that took a tombstone's entry
   unsigned int err;
   unsigned int rebuild;           // set when tombstones must be reclaimed
   alignas(256) unsigned long long free_tail;
@@ -68,6 +67,11 @@ struct alignas(256) Counters {
   alignas(256) unsigned long long n_singles;  // queued single-excitation attempts
   alignas(256) unsigned long long n_unique;
   alignas(256) unsigned long long n_new;
+  // per-block counts of tombstone entries taken by inserts, spread over
+  // kSpread lines (one add per block)
+  struct alignas(256) Spread { unsigned long long v; };
+  static constexpr int kSpread = 32;
+  Spread reuse_part[kSpread];
 };
 
 // probe limit of the walker hash: the load guard (k_hash_check) keeps probe
@@ -220,7 +224,11 @@ struct FreeStage {
 
 // RemoveHashDet (load_balancer.fpp:631-644) for the lanes with rm set: the
 // hash entry becomes a tombstone, the slot is staged for the free queue.
-// Wave-wide call; the tombstone/live counts follow from the free-queue tail.
+// Wave-wide call; the live and tombstone counts follow from the free-queue
+// tail.  (Releasing the entry as empty where the
+// bucket already holds an empty entry would also be valid -- such a bucket
+// ends every probe -- but it makes each removal read its bucket and, measured,
+// cost more in the removing kernels than the rebuilds it saves.)
 template <int W>
 __device__ inline void remove_dets(const Walk& wk, FreeStage* fs, bool rm, int64_t slot) {
   int q = stage_push(rm, &fs->st);
@@ -242,7 +250,10 @@ __device__ inline void free_flush(const Walk& wk, Counters* cnt, FreeStage* fs, 
   stage_flush_end(&fs->st);
 }
 
-// hash lookup: returns slot or -1
+
+// hash lookup: returns slot or -1.  Entries of a bucket fill in order and
+// become empty only when the table is rebuilt, so the first empty entry ends
+// the probe sequence.
 template <int W>
 __device__ inline int64_t hash_lookup(const Walk& wk, const uint64_t* key, int64_t home, Counters* cnt) {
   int64_t b = home;
@@ -1633,8 +1644,15 @@ __global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* 
     }
   }
   // one no-return atomic per wave that reused entries
-  unsigned long long c = (unsigned long long)wave_sum((double)my_reused);
-  if (c && lane_id() == 0) atomicAdd(&cnt->n_reused, c);
+  // one add per block, spread over kSpread lines
+  __shared__ unsigned int blk_reused;
+  if (threadIdx.x == 0) blk_reused = 0;
+  __syncthreads();
+  unsigned int c = (unsigned int)wave_sum((double)my_reused);
+  if (c && lane_id() == 0) atomicAdd(&blk_reused, c);
+  __syncthreads();
+  if (threadIdx.x == 0 && blk_reused)
+    atomicAdd(&cnt->reuse_part[blockIdx.x % Counters::kSpread].v, (unsigned long long)blk_reused);
 }
 
 // bookkeeping of k_insert: queued slots used, fresh slots appended (beyond
@@ -1647,8 +1665,12 @@ __global__ void k_post_insert(Counters* cnt, const unsigned long long* n_ptr, in
   unsigned long long ns = cnt->nslots + (nn - used);
   cnt->nslots = ns < (unsigned long long)cap ? ns : (unsigned long long)cap;
   cnt->n_live += nn - cnt->n_fail;
-  cnt->n_tomb -= cnt->n_reused;
-  cnt->n_reused = 0;
+  unsigned long long reused = 0;
+  for (int q = 0; q < Counters::kSpread; ++q) {
+    reused += cnt->reuse_part[q].v;
+    cnt->reuse_part[q].v = 0;
+  }
+  cnt->n_tomb -= reused;
 }
 
 // CalcHashTableStats (load_balancer.fpp:646-805)
