@@ -45,6 +45,8 @@ module neci_gpu_mod
     public :: ngpu_set_hash, ngpu_set_reference, ngpu_upload_walkers, ngpu_download_walkers
     public :: ngpu_iterate, ngpu_get_stats, ngpu_get_cycle_stats, ngpu_find_walker_hash_batch
     public :: ngpu_diag_helement_batch, ngpu_set_excitgen_uniform, ngpu_set_core
+    public :: ngpu_set_graph, ngpu_batch_create, ngpu_batch_iterate, ngpu_batch_destroy
+    public :: ngpu_fcidump_header, ngpu_read_fcidump
 
     interface
         integer(c_int) function ngpu_create(h, sys) bind(c, name="ngpu_create")
@@ -92,6 +94,46 @@ module neci_gpu_mod
             integer(c_int64_t), value :: ncore
             integer(c_int32_t), value :: fp32_values
             integer(c_int64_t), intent(out) :: nnz
+        end function
+        integer(c_int) function ngpu_set_graph(h, on) bind(c, name="ngpu_set_graph")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: on
+        end function
+        integer(c_int) function ngpu_batch_create(b, hs, n) bind(c, name="ngpu_batch_create")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), intent(out) :: b
+            type(c_ptr), intent(in) :: hs(*)
+            integer(c_int32_t), value :: n
+        end function
+        integer(c_int) function ngpu_batch_iterate(b, ins) bind(c, name="ngpu_batch_iterate")
+            import :: c_int, c_ptr, ngpu_iter_in_t
+            type(c_ptr), value :: b
+            type(ngpu_iter_in_t), intent(in) :: ins(*)
+        end function
+        integer(c_int) function ngpu_batch_destroy(b) bind(c, name="ngpu_batch_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b
+        end function
+        integer(c_int) function ngpu_fcidump_header(path, norb, nelec, ms2, isym, orbsym, orbsym_cap) &
+            bind(c, name="ngpu_fcidump_header")
+            import :: c_int, c_char, c_int32_t
+            character(kind=c_char), intent(in) :: path(*)
+            integer(c_int32_t), intent(out) :: norb, nelec, ms2, isym
+            integer(c_int32_t), intent(out) :: orbsym(*)
+            integer(c_int32_t), value :: orbsym_cap
+        end function
+        integer(c_int) function ngpu_read_fcidump(path, umat_eps, tmat2d, nbasis, umat, umat_len, ecore, n_two) &
+            bind(c, name="ngpu_read_fcidump")
+            import :: c_int, c_char, c_double, c_int64_t
+            character(kind=c_char), intent(in) :: path(*)
+            real(c_double), value :: umat_eps
+            real(c_double), intent(out) :: tmat2d(*)
+            integer(c_int64_t), value :: nbasis
+            real(c_double), intent(out) :: umat(*)
+            integer(c_int64_t), value :: umat_len
+            real(c_double), intent(out) :: ecore
+            integer(c_int64_t), intent(out) :: n_two
         end function
         integer(c_int) function ngpu_set_hash(h, random_hash2) bind(c, name="ngpu_set_hash")
             import :: c_int, c_ptr, c_int32_t
