@@ -74,7 +74,7 @@ typedef struct {
     double av_mc_excits;         /* AvMCExcits (default 1) */
     int64_t max_walkers;         /* MaxWalkersPart = MEMORYFACPART * InitWalkers */
     int64_t max_spawned;         /* MaxSpawned = MEMORYFACSPAWN * InitWalkers * inum_runs */
-    int64_t n_hash_buckets;      /* nWalkerHashes = nint(MEMORYFACHASH * InitWalkers) */
+    int64_t n_hash_buckets;      /* nWalkerHashes = nint(MEMORYFACHASH * InitWalkers), at most 2^29 */
     uint32_t seed;               /* SEED (keys the counter-based RNG) */
     int32_t pad;
 } ngpu_sys_t;

@@ -252,6 +252,9 @@ struct DevSys {
   // diagonal elements: (U(i,j,i,j), U(i,j,j,i)) at jk0[2*((i-1)*nsp + j-1)]
   const double* jk0;
   int nsp;
+  // C(p, i) at binom[p*nel1 + i] when spawns are sorted by colex rank, else null
+  const uint32_t* binom;
+  int nel1;
 };
 
 __device__ __host__ inline double U(const DevSys& s, int i, int j, int k, int l) {

@@ -272,13 +272,18 @@ void hash_upkeep(ngpu_t* h) {
   launch(h, -1, -1, k_hash_done, dim3(1), dim3(1), h->cnt);
 }
 
-// sort of the spawned list (stage 4) then the run-head merge (stage 5)
+// sort of the spawned list (stage 4) then the run-head merge (stage 5);
+// ranked: the spawn kernels already wrote each record's colex rank
 template <int W, int R>
-void sort_and_merge(ngpu_t* h) {
+void sort_and_merge(ngpu_t* h, bool ranked) {
   const unsigned long long* n_ptr = &h->cnt->n_spawned;
   if (h->rank_sort) {
-    launch(h, 4, -1, k_rank_hist<W>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec, n_ptr, h->scap,
-           (const uint32_t*)h->d_binom, h->sys.nel + 1, h->rk, h->hist);
+    if (ranked)
+      launch(h, 4, -1, k_radix_hist<uint32_t, 1, false>, dim3(h->nb_chunk), dim3(kBlock), (const uint32_t*)h->rk, 1,
+             n_ptr, h->scap, 0, 0, h->hist);
+    else
+      launch(h, 4, -1, k_rank_hist<W>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec, n_ptr,
+             h->scap, (const uint32_t*)h->d_binom, h->sys.nel + 1, h->rk, h->hist);
     const int passes = (h->rank_bits + 7) / 8;
     const uint32_t* kin = h->rk;
     const uint32_t* iin = nullptr;
@@ -336,7 +341,7 @@ void sort_and_merge(ngpu_t* h) {
 template <int R>
 void core_projection(ngpu_t* h) {
   int nb = (int)std::min<int64_t>((h->ncore + kBlock - 1) / kBlock, h->nb_grid);
-  launch(h, -1, -1, k_core_gather<R>, dim3(nb), dim3(kBlock), (const int64_t*)h->core_slot, (const double*)h->wk.sgn,
+  launch(h, -1, -1, k_core_gather<R>, dim3(nb), dim3(kBlock), (const int64_t*)h->core_slot, h->wk,
          h->S, h->ncore, h->core_x);
   int nbw = (int)std::min<int64_t>((h->ncore + kBlock / kWave - 1) / (kBlock / kWave), 8 * h->nb_grid);
   if (h->core_fp32)
@@ -352,7 +357,7 @@ void core_projection(ngpu_t* h) {
 // merge -> annihilate -> insert -> stats, shared by iterate and the explicit-list entry
 template <int W, int R>
 void annihilation_chain(ngpu_t* h, bool with_core) {
-  sort_and_merge<W, R>(h);
+  sort_and_merge<W, R>(h, with_core);  // spawns of the iteration chain carry their ranks
   if (h->ncore && with_core)
     launch(h, -1, -1, k_core_apply<R>, dim3((int)std::min<int64_t>((h->ncore + kBlock - 1) / kBlock, h->nb_grid)),
            dim3(kBlock), h->ds, h->wk, (const int64_t*)h->core_slot, (const double*)h->core_y, h->ncore, h->dpart);
@@ -384,7 +389,7 @@ void iterate_chain(ngpu_t* h) {
              : h->ds.uniform ? k_spawn<W, R, kGenUniform> : k_spawn<W, R, kGenPchb>;
   launch(h, 2, 2, spawn, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
          (const int64_t*)h->att_off, (const int64_t*)h->tile_p, h->tcap, (const uint8_t*)h->pinfo, h->sp_rec,
-         h->scap, h->sq, (const IterParams*)h->dparams);
+         h->rank_sort ? h->rk : nullptr, h->scap, h->sq, (const IterParams*)h->dparams);
   if (h->sys.model == NGPU_MODEL_MOLECULAR && !h->ds.uniform) {
     const int nsp = h->sys.nbasis / 2;
     const size_t lds = sizeof(double) * (size_t)kHbGroups * nsp;
@@ -392,7 +397,8 @@ void iterate_chain(ngpu_t* h) {
               : nsp <= 5 * kHbGroup ? k_spawn_singles<W, R, 5>
               : nsp <= 8 * kHbGroup ? k_spawn_singles<W, R, 8> : k_spawn_singles<W, R, 16>;
     launch_lds(h, 3, 3, kern, dim3(h->nb_grid), dim3(kBlock), lds, h->ds, h->wk, h->cnt, h->dpart,
-               (const uint8_t*)h->pinfo, h->sp_rec, h->scap, (const int64_t*)h->sq, (const IterParams*)h->dparams);
+               (const uint8_t*)h->pinfo, h->sp_rec, h->rank_sort ? h->rk : nullptr, h->scap, (const int64_t*)h->sq,
+               (const IterParams*)h->dparams);
   }
   annihilation_chain<W, R>(h, true);
 }
@@ -623,7 +629,8 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
     return NGPU_ERR_ARG;
   if (sys->n_replicas < 1 || sys->n_replicas > NGPU_MAX_REPLICAS) return NGPU_ERR_ARG;
   if (sys->max_walkers < 1 || sys->max_walkers > (int64_t)INT32_MAX || sys->max_spawned < 1 ||
-      sys->max_spawned > (int64_t)UINT32_MAX || sys->n_hash_buckets < 1)
+      sys->max_spawned > (int64_t)UINT32_MAX || sys->n_hash_buckets < 1 ||
+      sys->n_hash_buckets > (int64_t)INT32_MAX / kBucketEntries)  // entries are stored as int32
     return NGPU_ERR_ARG;
   if (sys->model != NGPU_MODEL_HUBBARD && sys->model != NGPU_MODEL_MOLECULAR) return NGPU_ERR_ARG;
   ngpu_t* h = new ngpu_engine();
@@ -652,12 +659,11 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
   Walk& wk = h->wk;
   wk.cap = cap;
   wk.L = sys->n_hash_buckets;
+  wk.srow = S + 1;
   fail(dalloc(&wk.keys, (size_t)cap * W));
-  fail(dalloc(&wk.sgn, (size_t)cap * S));
-  fail(dalloc(&wk.flags, (size_t)cap));
+  fail(dalloc(&wk.st, (size_t)cap * wk.srow));
   fail(dalloc(&wk.diagH, (size_t)cap));
   fail(dalloc(&wk.offdiagH, (size_t)cap));
-  fail(dalloc(&wk.hpos, (size_t)cap));
   fail(dalloc(&wk.freeq, (size_t)cap));
   fail(dalloc(&wk.htab, (size_t)wk.L * 8 * W));
   fail(dalloc(&h->att_cnt, (size_t)cap * S));
@@ -687,7 +693,7 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
       }
     }
     unsigned __int128 total = c[(size_t)nb * (ne + 1) + ne];
-    if (total <= ((unsigned __int128)1 << 32)) {
+    if (total <= ((unsigned __int128)1 << 32) && nb * (ne + 1) <= kBinomLds) {
       h->rank_sort = true;
       while (h->rank_bits < 32 && ((unsigned __int128)1 << h->rank_bits) < total) ++h->rank_bits;
       if (h->rank_bits == 0) h->rank_bits = 1;
@@ -696,6 +702,8 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
         for (int i = 0; i <= ne; ++i) b[(size_t)p * (ne + 1) + i] = (uint32_t)c[(size_t)p * (ne + 1) + i];
       fail(dalloc(&h->d_binom, b.size()));
       if (rc == NGPU_OK) fail(hipMemcpy(h->d_binom, b.data(), b.size() * 4, hipMemcpyHostToDevice));
+      h->ds.binom = h->d_binom;
+      h->ds.nel1 = sys->nel + 1;
       fail(dalloc(&h->rk, (size_t)scap));
     }
   }
@@ -722,7 +730,7 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
     fail(hipMemsetAsync(h->dout, 0, sizeof(ngpu_iter_out_t), h->stream));
     fail(hipMemsetAsync(h->dpart, 0, sizeof(double) * kStatCopies * kStatStride, h->stream));
     fail(hipMemsetAsync(h->dcyc, 0, sizeof(ngpu_iter_out_t), h->stream));
-    fail(hipMemsetAsync(wk.flags, 0, (size_t)cap * sizeof(uint32_t), h->stream));
+    fail(hipMemsetAsync(wk.st, 0, (size_t)cap * wk.srow * sizeof(double), h->stream));
     fail(hipStreamSynchronize(h->stream));
     memset(h->hout, 0, sizeof(ngpu_iter_out_t));
   }
@@ -759,7 +767,7 @@ int ngpu_destroy(ngpu_t* h) {
   if (!h) return NGPU_ERR_ARG;
   if (h->batch) return NGPU_ERR_STATE;  // ngpu_batch_destroy first
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  void* ptrs[] = {h->wk.keys, h->wk.sgn, h->wk.flags, h->wk.diagH, h->wk.offdiagH, h->wk.hpos, h->wk.freeq,
+  void* ptrs[] = {h->wk.keys, h->wk.st, h->wk.diagH, h->wk.offdiagH, h->wk.freeq,
                   h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->sp_rec,
                   h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->digit_tot, h->sq, h->u_key, h->u_sgn, h->u_flag, h->n_key,
                   h->n_sgn, h->n_flag, h->cnt, h->dout, h->dpart, h->dcyc, h->own_params, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_binom, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
@@ -1119,11 +1127,18 @@ int ngpu_download_walkers(ngpu_t* h, int64_t* rows, double* diag_h, double* offd
   std::vector<double> sg((size_t)ns * S), dh((size_t)ns), oh((size_t)ns);
   std::vector<uint32_t> fl((size_t)ns);
   if (ns > 0) {
+    const int64_t sr = h->wk.srow;
+    std::vector<double> st((size_t)ns * sr);
     HIP_OK(hipMemcpy(key.data(), h->wk.keys, key.size() * 8, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(sg.data(), h->wk.sgn, sg.size() * 8, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(fl.data(), h->wk.flags, fl.size() * 4, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(st.data(), h->wk.st, st.size() * 8, hipMemcpyDeviceToHost));
     HIP_OK(hipMemcpy(dh.data(), h->wk.diagH, dh.size() * 8, hipMemcpyDeviceToHost));
     HIP_OK(hipMemcpy(oh.data(), h->wk.offdiagH, oh.size() * 8, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < ns; ++i) {
+      for (int r = 0; r < S; ++r) sg[(size_t)i * S + r] = st[(size_t)i * sr + r];
+      uint64_t fw;
+      memcpy(&fw, &st[(size_t)i * sr + S], 8);
+      fl[(size_t)i] = (uint32_t)fw;
+    }
   }
   const int* init_bits = h->sys.flag_layout == NGPU_FLAGS_NECI ? kNeciInit : kMneciInit;
   const int* det_bits = h->sys.flag_layout == NGPU_FLAGS_NECI ? kNeciDeterm : kMneciDeterm;

@@ -78,17 +78,23 @@ struct alignas(256) Counters {
 // sequences short, the cap turns a saturated table into an error, not a stall
 constexpr int64_t kMaxProbe = 4096;
 
+// Walker table.  The per-slot state that the random-access kernels update
+// together -- the S signs, the flag word and the hash entry -- is one row of
+// S+1 doubles (signs, then flags in the low and the hash entry in the high
+// 32 bits of the last word), so that a lookup-and-update touches one line.
 struct Walk {
   uint64_t* keys;     // cap * W
-  double* sgn;        // cap * S
-  uint32_t* flags;    // cap
+  double* st;         // cap * srow: S signs | flags, hash entry (bucket*4 + e)
+  int64_t srow;       // S + 1
   double* diagH;      // H_ii - Hii
   double* offdiagH;   // H(det, ref)
-  int64_t* hpos;      // hash entry (bucket*4 + e)
   int64_t cap;
   uint64_t* htab;     // L buckets of 8W words
   int64_t L;
   int64_t* freeq;     // free-slot queue (ring, cap entries)
+  __device__ double& sgn(int64_t slot, int r) const { return st[slot * srow + r]; }
+  __device__ uint32_t& flags(int64_t slot) const { return *(uint32_t*)(st + slot * srow + srow - 1); }
+  __device__ int32_t& hpos(int64_t slot) const { return *((int32_t*)(st + slot * srow + srow - 1) + 1); }
 };
 
 struct Shift { double v[kMaxRep]; };
@@ -233,10 +239,10 @@ template <int W>
 __device__ inline void remove_dets(const Walk& wk, FreeStage* fs, bool rm, int64_t slot) {
   int q = stage_push(rm, &fs->st);
   if (rm) {
-    int64_t hp = wk.hpos[slot];
+    int64_t hp = wk.hpos(slot);
     int32_t* sl = (int32_t*)(wk.htab + (size_t)(hp >> 2) * 8 * W + 4 * W);
     sl[hp & 3] = kSlotTomb;
-    wk.flags[slot] |= kFlagRemoved;
+    wk.flags(slot) |= kFlagRemoved;
     fs->slot[q] = slot;
   }
 }
@@ -398,7 +404,7 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
   for (int64_t base = begin; base < end; base += blockDim.x) {
     int64_t slot = base + threadIdx.x;
     bool valid = slot < end;
-    uint32_t f = valid ? wk.flags[slot] : kFlagRemoved;
+    uint32_t f = valid ? wk.flags(slot) : kFlagRemoved;
     bool live = !(f & kFlagRemoved);
     uint64_t key[W];
     double sg[R];
@@ -406,7 +412,7 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
 #pragma unroll
     for (int w = 0; w < W; ++w) key[w] = live ? wk.keys[slot * W + w] : 0ull;
     for (int r = 0; r < S; ++r) {
-      sg[r] = live ? wk.sgn[slot * S + r] : 0.0;
+      sg[r] = live ? wk.sgn(slot, r) : 0.0;
       occ = occ || !(fabs(sg[r]) < 1.e-12);
     }
     bool core = (f & kFlagCore) != 0;
@@ -482,8 +488,8 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
     }
     bool rm = occ && !(any_left || core);
     if (occ) {
-      for (int r = 0; r < S; ++r) wk.sgn[slot * S + r] = rm ? 0.0 : cp[r];
-      wk.flags[slot] = f;
+      for (int r = 0; r < S; ++r) wk.sgn(slot, r) = rm ? 0.0 : cp[r];
+      wk.flags(slot) = f;
     }
     remove_dets<W>(wk, &fs, rm, slot);
     free_flush(wk, cnt, &fs, false);
@@ -564,7 +570,7 @@ __device__ inline int64_t parent_pair(const int64_t* att_off, int64_t lo, int64_
 }
 
 // LDS stages of the spawned list and of the queued single-excitation attempts
-constexpr int kSpawnStage = 768;
+constexpr int kSpawnStage = 576;
 constexpr int kParentWindow = 512;  // attempt offsets of one tile's parents staged in LDS
 constexpr int kQueueStage = 512;
 template <int W>
@@ -588,10 +594,42 @@ constexpr int kRecWords = 4;
 __device__ inline double rec_val(const uint64_t* r) { return __longlong_as_double((long long)r[2]); }
 __device__ inline uint32_t rec_info(const uint64_t* r) { return (uint32_t)r[3]; }
 
-// block-wide: drain staged spawns into the spawned list when due
+// colex rank of a determinant: sum over its i-th occupied orbital (bit p,
+// ascending) of C(p, i); injective on determinants of nel electrons
+template <int W>
+__device__ inline uint32_t colex_rank(const uint32_t* binom, int nel1, const uint64_t* d) {
+  uint32_t r = 0;
+  int c = 0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    uint64_t x = d[w];
+    while (x) {
+      int p = 64 * w + ctz64(x);
+      x &= x - 1;
+      ++c;
+      r += binom[p * nel1 + c];
+    }
+  }
+  return r;
+}
+
+// binomial table of the colex rank, staged in LDS by the spawn kernels (the
+// engine ranks only when nbasis*(nel+1) fits)
+constexpr int kBinomLds = 512;
+struct BinomLds {
+  uint32_t c[kBinomLds];
+  __device__ void load(const DevSys& s, const uint32_t* rk) {
+    if (rk)
+      for (int i = threadIdx.x; i < s.nbasis * s.nel1; i += blockDim.x) c[i] = s.binom[i];
+    __syncthreads();
+  }
+};
+
+// block-wide: drain staged spawns into the spawned list when due; with rk,
+// also each spawn's colex rank (the first sort key) at the same index
 template <int W>
 __device__ inline void spawn_flush(SpawnStage<W>* ss, Counters* cnt, uint64_t* sp_rec, int64_t scap, bool force,
-                                   bool& overflow) {
+                                   bool& overflow, const DevSys& s, const BinomLds& bl, uint32_t* rk) {
   uint32_t n;
   unsigned long long base;
   if (!stage_flush_begin(&ss->st, &cnt->n_spawned, kSpawnStage, force, n, base)) return;
@@ -604,6 +642,7 @@ __device__ inline void spawn_flush(SpawnStage<W>* ss, Counters* cnt, uint64_t* s
       if (W == 1) r[1] = 0;
       r[2] = (uint64_t)__double_as_longlong(ss->val[i]);
       r[3] = ss->info[i];
+      if (rk) rk[d] = colex_rank<W>(bl.c, s.nel1, ss->key + i * W);
     } else {
       overflow = true;
     }
@@ -637,7 +676,7 @@ __device__ inline bool core_child(const DevSys& s, const Walk& wk, Counters* cnt
   apply_excit<W>(key, ex, child);
   int64_t home = find_walker_hash<W>(child, s.rh, s.L) - 1;
   int64_t pos = hash_lookup<W>(wk, child, home, cnt);
-  return pos >= 0 && (wk.flags[pos] & kFlagCore);
+  return pos >= 0 && (wk.flags(pos) & kFlagCore);
 }
 
 // statistics of the spawning kernels, per lane until the end of the kernel
@@ -767,12 +806,14 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* 
 template <int W, int R, int G>
 __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* cnt, double* part,
                                                   const int64_t* att_off, const int64_t* tile_p, int64_t tcap,
-                                                  const uint8_t* pinfo, uint64_t* sp_rec, int64_t scap,
-                                                  int64_t* sq, const IterParams* prm) {
+                                                  const uint8_t* pinfo, uint64_t* sp_rec, uint32_t* rk,
+                                                  int64_t scap, int64_t* sq, const IterParams* prm) {
   const double tau = prm->tau;
   const uint32_t iter = prm->iter;
   __shared__ double acc_lds[kStatStride];
   __shared__ SpawnStage<W> ss;
+  __shared__ BinomLds bl;
+  bl.load(s, rk);
   __shared__ QueueStage qs;
   BlockStats bs{acc_lds};
   bs.zero();
@@ -831,10 +872,10 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
       k = (uint32_t)(a - att_off[p]);
     }
     spawn_attempt<W, R, G>(s, wk, cnt, bs, acc, pinfo, act, p, k, mode, &ss, &qs, tau, iter);
-    spawn_flush<W>(&ss, cnt, sp_rec, scap, false, overflow);
+    spawn_flush<W>(&ss, cnt, sp_rec, scap, false, overflow, s, bl, rk);
     if (mode == 1) queue_flush(&qs, cnt, sq, scap, false, overflow);
   }
-  spawn_flush<W>(&ss, cnt, sp_rec, scap, true, overflow);
+  spawn_flush<W>(&ss, cnt, sp_rec, scap, true, overflow, s, bl, rk);
   if (mode == 1) queue_flush(&qs, cnt, sq, scap, true, overflow);
   if (overflow) atomicOr(&cnt->err, 1u << NGPU_ERR_SPAWN_FULL);
   acc.done(bs, mode);
@@ -890,12 +931,14 @@ __device__ inline void hb_rows(uint64_t x, int base, const double* row, int nsp,
 // kHbGroups*nsp values of the block's attempts
 template <int W, int R, int PL>
 __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Counters* cnt, double* part,
-                                                          const uint8_t* pinfo, uint64_t* sp_rec, int64_t scap,
-                                                          const int64_t* sq, const IterParams* prm) {
+                                                          const uint8_t* pinfo, uint64_t* sp_rec, uint32_t* rk,
+                                                          int64_t scap, const int64_t* sq, const IterParams* prm) {
   const double tau = prm->tau;
   const uint32_t iter = prm->iter;
   __shared__ double acc_lds[kStatStride];
   __shared__ SpawnStage<W> ss;
+  __shared__ BinomLds bl;
+  bl.load(s, rk);
   extern __shared__ double hv_dyn[];  // signed H(src -> b) of each group's attempt
   BlockStats bs{acc_lds};
   bs.zero();
@@ -1032,9 +1075,9 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
       ss.val[idx] = nspawn;
       ss.info[idx] = info;
     }
-    spawn_flush<W>(&ss, cnt, sp_rec, scap, false, overflow);  // its barriers also order the reuse of hv
+    spawn_flush<W>(&ss, cnt, sp_rec, scap, false, overflow, s, bl, rk);  // its barriers also order the reuse of hv
   }
-  spawn_flush<W>(&ss, cnt, sp_rec, scap, true, overflow);
+  spawn_flush<W>(&ss, cnt, sp_rec, scap, true, overflow, s, bl, rk);
   if (overflow) atomicOr(&cnt->err, 1u << NGPU_ERR_SPAWN_FULL);
   acc.done(bs, 2);
   bs.flush(part);
@@ -1089,18 +1132,7 @@ __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, cons
   int64_t begin, end;
   chunk_of(n, begin, end);
   for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
-    uint32_t r = 0;
-    int c = 0;
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-      uint64_t x = keys[i * kRecWords + w];
-      while (x) {
-        int p = 64 * w + ctz64(x);
-        x &= x - 1;
-        ++c;
-        r += binom[p * nel1 + c];
-      }
-    }
+    uint32_t r = colex_rank<W>(binom, nel1, keys + i * kRecWords);
     rank[i] = r;
     atomicAdd(&h[r & 0xFFu], 1u);
   }
@@ -1394,8 +1426,8 @@ __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counte
       if (pos >= 0) {
         double cs[R];
         double tot = 0.0;
-        for (int r = 0; r < S; ++r) { cs[r] = wk.sgn[pos * S + r]; tot += fabs(cs[r]); }
-        bool determ = (wk.flags[pos] & kFlagCore) != 0;
+        for (int r = 0; r < S; ++r) { cs[r] = wk.sgn(pos, r); tot += fabs(cs[r]); }
+        bool determ = (wk.flags(pos) & kFlagCore) != 0;
         if (tot >= 1.e-12 || determ) {
           for (int r = 0; r < S; ++r) {
             double prod = cs[r] * ss[r];
@@ -1411,7 +1443,7 @@ __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counte
           bool occ = false;
           for (int r = 0; r < S; ++r) {
             cs[r] = ss[r] + cs[r];
-            wk.sgn[pos * S + r] = cs[r];
+            wk.sgn(pos, r) = cs[r];
             occ = occ || !(fabs(cs[r]) < 1.e-12);
           }
           rm = !determ && !occ;
@@ -1468,10 +1500,10 @@ __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counte
 // Core amplitudes before the projection (no death acts on core dets, so the
 // signs after the prepass are the iteration's starting amplitudes).
 template <int R>
-__global__ void __launch_bounds__(kBlock) k_core_gather(const int64_t* core_slot, const double* sgn, int S,
+__global__ void __launch_bounds__(kBlock) k_core_gather(const int64_t* core_slot, Walk wk, int S,
                                                         int64_t ncore, double* x) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncore; i += (int64_t)gridDim.x * blockDim.x)
-    for (int r = 0; r < S; ++r) x[i * S + r] = sgn[core_slot[i] * S + r];
+    for (int r = 0; r < S; ++r) x[i * S + r] = wk.sgn(core_slot[i], r);
 }
 
 // determ_projection (semi_stoch_procs.F90:105-241), one wave per core row:
@@ -1522,8 +1554,8 @@ __global__ void __launch_bounds__(kBlock) k_core_apply(DevSys s, Walk wk, const 
     for (int r = 0; r < S; ++r) {
       double b = 0.0, a = 0.0;
       if (act) {
-        double yy = y[i * S + r], c = wk.sgn[slot * S + r];
-        wk.sgn[slot * S + r] = yy + c;
+        double yy = y[i * S + r], c = wk.sgn(slot, r);
+        wk.sgn(slot, r) = yy + c;
         b = fabs(yy);
         if (c * yy < 0.0) {
           double x1 = fabs(c), x2 = fabs(yy);
@@ -1587,7 +1619,7 @@ __global__ void __launch_bounds__(kBlock) k_core_place(DevSys s, Walk wk, Counte
     int64_t home = find_walker_hash<W>(d, s.rh, s.L) - 1;
     int64_t pos = hash_lookup<W>(wk, d, home, cnt);
     if (pos >= 0) {
-      wk.flags[pos] |= cflags;
+      wk.flags(pos) |= cflags;
       core_slot[i] = pos;
     } else {
       unsigned long long q = atomicAdd(&cnt->n_new, 1ull);
@@ -1632,15 +1664,15 @@ __global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* 
     if (hp < 0) {
       atomicOr(&cnt->err, 1u << NGPU_ERR_HASH_FULL);
       atomicAdd(&cnt->n_fail, 1ull);
-      wk.flags[slot] = kFlagRemoved;
+      wk.flags(slot) = kFlagRemoved;
     } else {
 #pragma unroll
       for (int w = 0; w < W; ++w) wk.keys[slot * W + w] = key[w];
-      for (int r = 0; r < S; ++r) wk.sgn[slot * S + r] = n_sgn[i * S + r];
-      wk.flags[slot] = n_flag[i] & ~kFlagRemoved;
+      for (int r = 0; r < S; ++r) wk.sgn(slot, r) = n_sgn[i * S + r];
+      wk.flags(slot) = n_flag[i] & ~kFlagRemoved;
       wk.diagH[slot] = diag_helement<W>(s, key) - s.hii;
       wk.offdiagH[slot] = offdiag_helement<W>(s, key);
-      wk.hpos[slot] = hp;
+      wk.hpos(slot) = hp;
     }
   }
   // one no-return atomic per wave that reused entries
@@ -1693,12 +1725,12 @@ __global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* c
     int64_t slot = grid_elem(t);
     bool rm = false;
     if (slot < nslots) {
-      uint32_t f = wk.flags[slot];
+      uint32_t f = wk.flags(slot);
       if (!(f & kFlagRemoved)) {
         bool determ = (f & kFlagCore) != 0;
         double sg[R];
         bool occ = false;
-        for (int r = 0; r < S; ++r) { sg[r] = wk.sgn[slot * S + r]; occ = occ || !(fabs(sg[r]) < 1.e-12); }
+        for (int r = 0; r < S; ++r) { sg[r] = wk.sgn(slot, r); occ = occ || !(fabs(sg[r]) < 1.e-12); }
         if (occ || determ) {
           uint64_t key[W];
 #pragma unroll
@@ -1723,7 +1755,7 @@ __global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* c
             }
             if (changed) {
               occ = false;
-              for (int r = 0; r < S; ++r) { wk.sgn[slot * S + r] = sg[r]; occ = occ || !(fabs(sg[r]) < 1.e-12); }
+              for (int r = 0; r < S; ++r) { wk.sgn(slot, r) = sg[r]; occ = occ || !(fabs(sg[r]) < 1.e-12); }
               rm = !occ && !determ;
             }
           }
@@ -1787,7 +1819,7 @@ __global__ void __launch_bounds__(kBlock) k_hash_refill(DevSys s, Walk wk, Count
   int64_t nslots = (int64_t)cnt->nslots;
   for (int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < nslots;
        slot += (int64_t)gridDim.x * blockDim.x) {
-    if (wk.flags[slot] & kFlagRemoved) continue;
+    if (wk.flags(slot) & kFlagRemoved) continue;
     uint64_t key[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
@@ -1795,7 +1827,7 @@ __global__ void __launch_bounds__(kBlock) k_hash_refill(DevSys s, Walk wk, Count
     bool reused = false;
     int64_t hp = hash_insert<W>(wk, key, home, (int32_t)slot, reused);
     if (hp < 0) atomicOr(&cnt->err, 1u << NGPU_ERR_HASH_FULL);
-    else wk.hpos[slot] = hp;
+    else wk.hpos(slot) = hp;
   }
 }
 
