@@ -24,8 +24,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "walker-iterations/s (spawn+annihilate+death) on 1 GPU; batched vs plain FP64"
 STAGES = ["prepass", "scan", "spawn", "spawn_singles", "sort", "merge", "annihilate", "insert", "stats"]
-KERNELS = {"prepass": "k_prepass", "spawn": "k_spawn", "spawn_singles": "k_spawn_singles", "merge": "k_merge",
-           "annihilate": "k_annihilate", "insert": "k_insert", "stats": "k_stats"}
+KERNELS = {"prepass": "k_prepass", "spawn": "k_spawn", "spawn_singles": "k_spawn_singles",
+           "annihilate": "k_merge_annihilate", "insert": "k_insert", "stats": "k_stats"}
 SINGLE_KERNEL_STAGES = set(KERNELS)
 HBM_PEAK_GBS = 8000.0
 
@@ -52,10 +52,12 @@ def algorithmic_bytes(stage, c, W=1, S=1, nbasis=40, psingles=0.1):
     if stage == "sort":        # per 8-bit pass: histogram read of keys, scatter read+write of (key, index)
         passes = (nbasis + 7) // 8 if W == 1 else (nbasis - 64 + 7) // 8 + 8
         return passes * Ns * (K + 2 * (K + 4))
-    if stage == "merge":       # sorted keys + index, gathered value + info; unique list written
-        return Ns * (K + 4 + 8 + 4) + Nu * (K + Sg + 4)
-    if stage == "annihilate":  # unique list, one hash bucket (64W B), sign+flag read/write; new list written
-        return Nu * (K + Sg + 4) + Nu * 64 * W + Nu * (2 * Sg + 4) + Nn * (K + Sg + 4)
+    if stage == "merge":       # fused into the annihilate stage
+        return 0.0
+    if stage == "annihilate":  # merge and annihilation in one pass: sorted keys + index, gathered value + info
+        #                        per spawn; per unique det one hash bucket (64W B) and its state row
+        #                        (S signs + flag word) read and written; new list written
+        return Ns * (K + 4 + 8 + 4) + Nu * 64 * W + Nu * 2 * (Sg + 8) + Nn * (K + Sg + 4)
     if stage == "insert":      # new list, free-queue entry, bucket line, walker row (key, signs, flags, 3x8)
         return Nn * (K + Sg + 4) + Nn * 8 + Nn * 64 * W + Nn * (K + Sg + 4 + 24)
     if stage == "stats":       # flags, signs, key per slot

@@ -84,9 +84,11 @@ struct ngpu_engine {
   uint32_t* d_binom = nullptr;  // C(p, i) at [p*(nel+1)+i]
   uint32_t* rk = nullptr;       // ranks of the spawned list
   int64_t* sq = nullptr;  // queued single-excitation attempts
+  // merged spawned list, kept only for the explicit-list entry point
   uint64_t* u_key = nullptr;
   double* u_sgn = nullptr;
   uint32_t* u_flag = nullptr;
+  unsigned long long* u_count = nullptr;
   uint64_t* n_key = nullptr;
   double* n_sgn = nullptr;
   uint32_t* n_flag = nullptr;
@@ -205,7 +207,13 @@ __global__ void k_end_iter(ngpu_iter_out_t* out, const double* part, ngpu_iter_o
     out->n_attempts = (int64_t)cnt->n_attempts;
     int64_t ns = (int64_t)cnt->n_spawned;
     out->n_spawned = ns < scap ? ns : scap;
-    out->n_unique = (int64_t)cnt->n_unique;
+    unsigned long long nu = 0;
+    for (int q = 0; q < Counters::kSpread; ++q) {
+      nu += cnt->uniq_part[q].v;
+      cnt->uniq_part[q].v = 0;
+    }
+    cnt->n_unique = nu;
+    out->n_unique = (int64_t)nu;
     out->n_new = (int64_t)cnt->n_new;
     out->err = (int32_t)cnt->err;
     cyc->tot_walkers = out->tot_walkers;
@@ -272,10 +280,12 @@ void hash_upkeep(ngpu_t* h) {
   launch(h, -1, -1, k_hash_done, dim3(1), dim3(1), h->cnt);
 }
 
-// sort of the spawned list (stage 4) then the run-head merge (stage 5);
-// ranked: the spawn kernels already wrote each record's colex rank
+// sort of the spawned list (stage 4) then the merge of equal determinants
+// fused with their annihilation into the walker list (stage 6; stage 5, the
+// merge alone, is empty since the fusion); ranked: the spawn kernels already
+// wrote each record's colex rank
 template <int W, int R>
-void sort_and_merge(ngpu_t* h, bool ranked) {
+void sort_and_merge(ngpu_t* h, bool ranked, bool list_out) {
   const unsigned long long* n_ptr = &h->cnt->n_spawned;
   if (h->rank_sort) {
     if (ranked)
@@ -301,9 +311,13 @@ void sort_and_merge(ngpu_t* h, bool ranked) {
       iin = h->si[out];
       cur = out;
     }
-    launch(h, 5, 5, k_merge<W, R, uint32_t, 1>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->cnt, h->dpart,
-           (const uint32_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_rec, h->scap, h->u_key,
-           h->u_sgn, h->u_flag);
+    if (list_out)
+      launch(h, -1, -1, k_merge_list<W, R, uint32_t, 1>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->cnt,
+             (const uint32_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_rec, h->scap, h->u_key,
+             h->u_sgn, h->u_flag, h->u_count);
+    launch(h, 6, 6, k_merge_annihilate<W, R, uint32_t, 1>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt,
+           h->dpart, (const uint32_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_rec, h->scap,
+           h->n_key, h->n_sgn, h->n_flag, (const IterParams*)h->dparams);
     return;
   }
   // determinant keys in ilut_lt order: least significant word first
@@ -332,9 +346,13 @@ void sort_and_merge(ngpu_t* h, bool ranked) {
     iin = h->si[out];
     cur = out;
   }
-  launch(h, 5, 5, k_merge<W, R, uint64_t, W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->cnt, h->dpart,
-         (const uint64_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_rec, h->scap, h->u_key,
-         h->u_sgn, h->u_flag);
+  if (list_out)
+    launch(h, -1, -1, k_merge_list<W, R, uint64_t, W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->cnt,
+           (const uint64_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_rec, h->scap, h->u_key,
+           h->u_sgn, h->u_flag, h->u_count);
+  launch(h, 6, 6, k_merge_annihilate<W, R, uint64_t, W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt,
+         h->dpart, (const uint64_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_rec, h->scap,
+         h->n_key, h->n_sgn, h->n_flag, (const IterParams*)h->dparams);
 }
 
 // semistochastic projection of the core amplitudes (before spawning)
@@ -356,14 +374,13 @@ void core_projection(ngpu_t* h) {
 
 // merge -> annihilate -> insert -> stats, shared by iterate and the explicit-list entry
 template <int W, int R>
-void annihilation_chain(ngpu_t* h, bool with_core) {
-  sort_and_merge<W, R>(h, with_core);  // spawns of the iteration chain carry their ranks
+void annihilation_chain(ngpu_t* h, bool with_core, bool list_out = false) {
+  // the projected core amplitudes join the walkers before the spawns do, as
+  // deterministic_annihilation precedes AnnihilateSpawnedParts
   if (h->ncore && with_core)
     launch(h, -1, -1, k_core_apply<R>, dim3((int)std::min<int64_t>((h->ncore + kBlock - 1) / kBlock, h->nb_grid)),
            dim3(kBlock), h->ds, h->wk, (const int64_t*)h->core_slot, (const double*)h->core_y, h->ncore, h->dpart);
-  launch(h, 6, 6, k_annihilate<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
-         (const uint64_t*)h->u_key, (const double*)h->u_sgn, (const uint32_t*)h->u_flag, h->n_key, h->n_sgn,
-         h->n_flag, (const IterParams*)h->dparams);
+  sort_and_merge<W, R>(h, with_core, list_out);  // spawns of the iteration chain carry their ranks
   launch(h, 7, 7, k_insert<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt,
          (const uint64_t*)h->n_key, (const double*)h->n_sgn, (const uint32_t*)h->n_flag,
          (const unsigned long long*)&h->cnt->n_new);
@@ -709,9 +726,6 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
   }
   fail(dalloc(&h->digit_tot, (size_t)kRadixBins));
   fail(dalloc(&h->sq, (size_t)scap * 2));  // (parent pair, attempt number)
-  fail(dalloc(&h->u_key, (size_t)scap * W));
-  fail(dalloc(&h->u_sgn, (size_t)scap * S));
-  fail(dalloc(&h->u_flag, (size_t)scap));
   int64_t ncap = std::max<int64_t>(scap, cap);
   fail(dalloc(&h->n_key, (size_t)ncap * W));
   fail(dalloc(&h->n_sgn, (size_t)ncap * S));
@@ -769,8 +783,8 @@ int ngpu_destroy(ngpu_t* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void* ptrs[] = {h->wk.keys, h->wk.st, h->wk.diagH, h->wk.offdiagH, h->wk.freeq,
                   h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->sp_rec,
-                  h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->digit_tot, h->sq, h->u_key, h->u_sgn, h->u_flag, h->n_key,
-                  h->n_sgn, h->n_flag, h->cnt, h->dout, h->dpart, h->dcyc, h->own_params, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_binom, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
+                  h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->digit_tot, h->sq, h->n_key,
+                  h->n_sgn, h->n_flag, h->u_key, h->u_sgn, h->u_flag, h->u_count, h->cnt, h->dout, h->dpart, h->dcyc, h->own_params, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_binom, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
                   h->core_vals, h->core_x, h->core_y, h->d_ab_tab,
                   h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt};
   for (void* p : ptrs)
@@ -1611,6 +1625,12 @@ int ngpu_annihilate(ngpu_t* h, uint32_t iter, const int64_t* sdet, const double*
     if (rc) return rc;
   }
   const int W = h->W, S = h->S;
+  if (!h->u_key) {  // the merged list is kept only for this entry point
+    HIP_OK(hipMalloc((void**)&h->u_key, (size_t)h->scap * W * 8));
+    HIP_OK(hipMalloc((void**)&h->u_sgn, (size_t)h->scap * S * 8));
+    HIP_OK(hipMalloc((void**)&h->u_flag, (size_t)h->scap * 4));
+    HIP_OK(hipMalloc((void**)&h->u_count, 8));
+  }
   if (n) {
     std::vector<uint64_t> rec((size_t)n * kRecWords, 0);
     for (int64_t i = 0; i < n; ++i) {
@@ -1624,19 +1644,22 @@ int ngpu_annihilate(ngpu_t* h, uint32_t iter, const int64_t* sdet, const double*
   }
   hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dpart, h->cnt);
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_spawned, (unsigned long long)n);
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, h->u_count, 0ull);
   bool p = h->prof;
   h->prof = false;
   // an explicit list carries no deterministic projection
   set_params(h, 1.0, nullptr, iter);
-  if (W == 1) h->S == 1 ? annihilation_chain<1, 1>(h, false) : annihilation_chain<1, kMaxRep>(h, false);
-  else h->S == 1 ? annihilation_chain<2, 1>(h, false) : annihilation_chain<2, kMaxRep>(h, false);
+  if (W == 1) h->S == 1 ? annihilation_chain<1, 1>(h, false, true) : annihilation_chain<1, kMaxRep>(h, false, true);
+  else h->S == 1 ? annihilation_chain<2, 1>(h, false, true) : annihilation_chain<2, kMaxRep>(h, false, true);
   h->prof = p;
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(h->hout, h->dout, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipMemcpyAsync(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
   h->pending = true;
   rc = finish(h, out);
-  int64_t nu = (int64_t)h->hcnt->n_unique;
+  unsigned long long nu_dev = 0;
+  HIP_OK(hipMemcpy(&nu_dev, h->u_count, 8, hipMemcpyDeviceToHost));
+  int64_t nu = (int64_t)nu_dev;
   if (n_merged) *n_merged = nu;
   if (nu > 0 && merged_det && merged_sgn && merged_flags) {
     HIP_OK(hipMemcpy(merged_det, h->u_key, (size_t)nu * W * 8, hipMemcpyDeviceToHost));

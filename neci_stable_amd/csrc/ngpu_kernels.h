@@ -72,6 +72,7 @@ struct alignas(256) Counters {
   struct alignas(256) Spread { unsigned long long v; };
   static constexpr int kSpread = 32;
   Spread reuse_part[kSpread];
+  Spread uniq_part[kSpread];  // unique spawned determinants, per block
 };
 
 // probe limit of the walker hash: the load guard (k_hash_check) keeps probe
@@ -1320,21 +1321,20 @@ __device__ inline bool key_eq(const K* a, const K* b) {
   return e;
 }
 
+// The merged (compressed) spawned list alone, for the explicit-list entry
+// point: run heads sum their runs as k_merge_annihilate does, and the list
+// goes to memory; no statistics (k_merge_annihilate counts them).
 template <int W, int R, typename K, int NW>
-__global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, double* part, const K* skey,
-                                                  const uint32_t* sidx, const uint64_t* sp_rec, int64_t scap,
-                                                  uint64_t* u_key, double* u_sgn, uint32_t* u_flag) {
-  __shared__ double acc_lds[kStatStride];
+__global__ void __launch_bounds__(kBlock) k_merge_list(DevSys s, Counters* cnt, const K* skey, const uint32_t* sidx,
+                                                       const uint64_t* sp_rec, int64_t scap, uint64_t* u_key,
+                                                       double* u_sgn, uint32_t* u_flag,
+                                                       unsigned long long* u_count) {
   __shared__ ListStage<W, R> ls;
-  BlockStats bs{acc_lds};
-  bs.zero();
   stage_init(&ls.st);
   const int S = R == 1 ? 1 : s.S;
   int64_t n = (int64_t)cnt->n_spawned;
   if (n > scap) n = scap;
   int64_t trips = grid_trips(n);
-  double ann[R];
-  for (int r = 0; r < S; ++r) ann[r] = 0.0;
   for (int64_t t = 0; t < trips; ++t) {
     int64_t i = grid_elem(t);
     bool keep = false;
@@ -1352,28 +1352,22 @@ __global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, doubl
 #pragma unroll
         for (int w = 0; w < W; ++w) key[w] = kd[w];
         for (int r = 0; r < S; ++r) sum[r] = 0.0;
-        if (j == i + 1) {
-          uint32_t inf = rec_info(kd);
+        for (int64_t q = i; q < j; ++q) {
+          const uint64_t* rq = sp_rec + (size_t)sidx[q] * kRecWords;
+          uint32_t inf = rec_info(rq);
           int r = (int)(inf & 0xFFu);
-          sum[r] = rec_val(kd);
-          if (inf & kInfoInit) flag |= 1u << r;
-          keep = !(fabs(sum[r]) < 1.e-12);
-        } else {
-          for (int64_t q = i; q < j; ++q) {
-            const uint64_t* rq = sp_rec + (size_t)sidx[q] * kRecWords;
-            uint32_t inf = rec_info(rq);
-            int r = (int)(inf & 0xFFu);
-            double nv = rec_val(rq);
-            double cv = sum[r];
-            if (s.t_init) {
-              if ((fabs(cv) > 1.e-12 && fabs(nv) > 1.e-12) || (inf & kInfoInit)) flag |= 1u << r;
-            }
-            if (cv * nv < 0.0) {
-              double x = fabs(cv), y = fabs(nv);
-              ann[r] += 2.0 * (x < y ? x : y);
-            }
-            sum[r] = cv + nv;
+          double nv = rec_val(rq);
+          double cv = sum[r];
+          if (j == i + 1) {
+            if (inf & kInfoInit) flag |= 1u << r;
+          } else if (s.t_init) {
+            if ((fabs(cv) > 1.e-12 && fabs(nv) > 1.e-12) || (inf & kInfoInit)) flag |= 1u << r;
           }
+          sum[r] = cv + nv;
+        }
+        if (j == i + 1) {
+          keep = !(fabs(sum[rec_info(kd) & 0xFFu]) < 1.e-12);
+        } else {
           double tot = 0.0;
           for (int r = 0; r < S; ++r) tot += fabs(sum[r]);
           keep = tot > 1.e-12;
@@ -1381,46 +1375,89 @@ __global__ void __launch_bounds__(kBlock) k_merge(DevSys s, Counters* cnt, doubl
       }
     }
     list_push<W, R>(&ls, keep, key, sum, flag, S);
-    list_flush<W, R>(&ls, &cnt->n_unique, u_key, u_sgn, u_flag, S, false);
+    list_flush<W, R>(&ls, u_count, u_key, u_sgn, u_flag, S, false);
   }
-  list_flush<W, R>(&ls, &cnt->n_unique, u_key, u_sgn, u_flag, S, true);
-  for (int r = 0; r < S; ++r) bs.add(soff(F_NANNIHIL, r), ann[r]);
-  bs.flush(part);
+  list_flush<W, R>(&ls, u_count, u_key, u_sgn, u_flag, S, true);
 }
 
-// ---------------------------------------------------------- annihilation
-// AnnihilateSpawnedParts (Annihilation.F90:965-1352): found dets are updated
-// in place (each unique key is owned by one lane); absent dets go to the new list.
-template <int W, int R>
-__global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counters* cnt, double* part,
-                                                       const uint64_t* u_key, const double* u_sgn,
-                                                       const uint32_t* u_flag, uint64_t* n_key,
-                                                       double* n_sgn, uint32_t* n_flag, const IterParams* prm) {
+// CompressSpawnedList (Annihilation.F90:249-515) + FindResidualParticle
+// (:551-634), then AnnihilateSpawnedParts (:965-1352), in one pass over the
+// sorted spawns: the head of each run of equal keys sums the run in sorted
+// (= emission, the sort is stable) order, then looks its determinant up in the
+// walker hash; found determinants are updated in place (each key is owned by
+// one lane), absent ones go to the new list.  The lookups of one wave overlap
+// the record gathers of the others, and the merged list never goes to memory.
+template <int W, int R, typename K, int NW>
+__global__ void __launch_bounds__(kBlock) k_merge_annihilate(DevSys s, Walk wk, Counters* cnt, double* part,
+                                                             const K* skey, const uint32_t* sidx,
+                                                             const uint64_t* sp_rec, int64_t scap, uint64_t* n_key,
+                                                             double* n_sgn, uint32_t* n_flag, const IterParams* prm) {
   const uint32_t iter = prm->iter;
   __shared__ double acc_lds[kStatStride];
   __shared__ ListStage<W, R> ls;
   __shared__ FreeStage fs;
+  __shared__ unsigned int blk_unique;
   BlockStats bs{acc_lds};
   bs.zero();
   stage_init(&ls.st);
   stage_init(&fs.st);
+  if (threadIdx.x == 0) blk_unique = 0;
   const int S = R == 1 ? 1 : s.S;
-  int64_t nu = (int64_t)cnt->n_unique;
-  int64_t trips = grid_trips(nu);
-  double st_ab[R], st_an[R], st_rm[R], st_bn[R];
-  for (int r = 0; r < S; ++r) st_ab[r] = st_an[r] = st_rm[r] = st_bn[r] = 0.0;
+  int64_t n = (int64_t)cnt->n_spawned;
+  if (n > scap) n = scap;
+  int64_t trips = grid_trips(n);
+  double ann[R], st_ab[R], st_an[R], st_rm[R], st_bn[R];
+  for (int r = 0; r < S; ++r) ann[r] = st_ab[r] = st_an[r] = st_rm[r] = st_bn[r] = 0.0;
   for (int64_t t = 0; t < trips; ++t) {
     int64_t i = grid_elem(t);
-    bool isnew = false, rm = false;
-    int64_t pos = -1;
-    uint64_t key[W];
+    bool keep = false;
     double ss[R];
     uint32_t sflag = 0;
-    if (i < nu) {
+    uint64_t key[W];
+    if (i < n) {
+      const K* ki = skey + i * NW;
+      bool head = true;
+      if (i > 0) head = !key_eq<K, NW>(ki, skey + (i - 1) * NW);
+      if (head) {
+        int64_t j = i + 1;
+        while (j < n && key_eq<K, NW>(ki, skey + j * NW)) ++j;
+        const uint64_t* kd = sp_rec + (size_t)sidx[i] * kRecWords;
 #pragma unroll
-      for (int w = 0; w < W; ++w) key[w] = u_key[i * W + w];
-      for (int r = 0; r < S; ++r) ss[r] = u_sgn[i * S + r];
-      sflag = u_flag[i];
+        for (int w = 0; w < W; ++w) key[w] = kd[w];
+        for (int r = 0; r < S; ++r) ss[r] = 0.0;
+        if (j == i + 1) {
+          uint32_t inf = rec_info(kd);
+          int r = (int)(inf & 0xFFu);
+          ss[r] = rec_val(kd);
+          if (inf & kInfoInit) sflag |= 1u << r;
+          keep = !(fabs(ss[r]) < 1.e-12);
+        } else {
+          for (int64_t q = i; q < j; ++q) {
+            const uint64_t* rq = sp_rec + (size_t)sidx[q] * kRecWords;
+            uint32_t inf = rec_info(rq);
+            int r = (int)(inf & 0xFFu);
+            double nv = rec_val(rq);
+            double cv = ss[r];
+            if (s.t_init) {
+              if ((fabs(cv) > 1.e-12 && fabs(nv) > 1.e-12) || (inf & kInfoInit)) sflag |= 1u << r;
+            }
+            if (cv * nv < 0.0) {
+              double x = fabs(cv), y = fabs(nv);
+              ann[r] += 2.0 * (x < y ? x : y);
+            }
+            ss[r] = cv + nv;
+          }
+          double tot = 0.0;
+          for (int r = 0; r < S; ++r) tot += fabs(ss[r]);
+          keep = tot > 1.e-12;
+        }
+      }
+    }
+    unsigned int nk = (unsigned int)wave_count(keep);
+    if (nk && lane_id() == 0) atomicAdd(&blk_unique, nk);
+    bool isnew = false, rm = false;
+    int64_t pos = -1;
+    if (keep) {
       int64_t home = find_walker_hash<W>(key, s.rh, s.L) - 1;
       pos = hash_lookup<W>(wk, key, home, cnt);
       if (pos >= 0) {
@@ -1489,10 +1526,13 @@ __global__ void __launch_bounds__(kBlock) k_annihilate(DevSys s, Walk wk, Counte
   list_flush<W, R>(&ls, &cnt->n_new, n_key, n_sgn, n_flag, S, true);
   for (int r = 0; r < S; ++r) {
     bs.add(soff(F_NABORTED, r), st_ab[r]);
-    bs.add(soff(F_NANNIHIL, r), st_an[r]);
+    bs.add(soff(F_NANNIHIL, r), st_an[r] + ann[r]);
     bs.add(soff(F_NREMOVED, r), st_rm[r]);
     bs.add(soff(F_NBORN, r), st_bn[r]);
   }
+  __syncthreads();
+  if (threadIdx.x == 0 && blk_unique)
+    atomicAdd(&cnt->uniq_part[blockIdx.x % Counters::kSpread].v, (unsigned long long)blk_unique);
   bs.flush(part);
 }
 
@@ -1791,6 +1831,10 @@ __global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* c
 // pass 10%; more than 80% live
 // entries is a capacity error (MEMORYFACHASH too small), reported instead of
 // letting probe sequences grow without bound.
+// the table is rebuilt once live + tombstone entries pass this share (and
+// tombstones pass 10 %): later rebuilds against longer probe sequences
+constexpr unsigned long long kRebuildUsedPct = 60;
+
 __global__ void k_hash_check(Counters* cnt, int64_t entries) {
   // every removal of the iteration left a tombstone and queued its slot
   unsigned long long removed = cnt->free_tail - cnt->free_tail0;
@@ -1798,7 +1842,7 @@ __global__ void k_hash_check(Counters* cnt, int64_t entries) {
   cnt->n_tomb += removed;
   cnt->free_tail0 = cnt->free_tail;
   unsigned long long used = cnt->n_live + cnt->n_tomb;
-  cnt->rebuild = (used * 10ull > (unsigned long long)entries * 6ull &&
+  cnt->rebuild = (used * 100ull > (unsigned long long)entries * kRebuildUsedPct &&
                   cnt->n_tomb * 10ull > (unsigned long long)entries) ? 1u : 0u;
   if (cnt->n_live * 10ull > (unsigned long long)entries * 8ull) cnt->err |= 1u << NGPU_ERR_HASH_FULL;
 }
