@@ -9,7 +9,8 @@ spin-orbital-resolved holes) + on-the-fly heat-bath singles with pSingles 0.1,
 
 A step is one FCIQMC iteration (spawn, merge, death, annihilation, hash
 statistics) over the resident walker list; every 10 steps the host runs the
-shift update (NECI's update cycle).  The walker population is grown to the
+shift update (NECI's update cycle, with TARGET-SHIFTDAMP so that the
+population settles at TOTALWALKERS).  The walker population is grown to the
 target untimed, then W warm-up steps, then K timed steps.
 value = sum over timed steps of TotParts / wall time of those steps.
 """
@@ -343,10 +344,16 @@ def main():
     tau = 0.005
     eng = Engine(spec)
     eng.start_single_part(100.0)
-    drv = FciMCPar(eng, tau=tau, target_walkers=args.walkers, steps_shift=10, shift_damp=0.5)
+    # TARGET-SHIFTDAMP holds the population at TOTALWALKERS; the run is grown
+    # until five consecutive update cycles average within 10 % of it
+    drv = FciMCPar(eng, tau=tau, target_walkers=args.walkers, steps_shift=10, shift_damp=0.5,
+                   target_shiftdamp=True)
     tg = time.perf_counter()
-    while drv.single_part.any() and drv.iter < args.max_growth_iters:
+    settled = 0
+    while settled < 5 and drv.iter < args.max_growth_iters:
         drv.run(10)
+        av = float(np.sum(drv.rows[-1]["tot_parts"]))
+        settled = settled + 1 if abs(av / args.walkers - 1.0) < 0.1 else 0
     t_growth = time.perf_counter() - tg
     print(f"# growth: {drv.iter} iterations, {t_growth:.2f} s, TotParts {drv.rows[-1]['tot_parts']}",
           file=sys.stderr, flush=True)
@@ -384,7 +391,8 @@ def main():
         "steps": K, "warmup": args.warmup, "ms_per_step": dt * 1e3 / K, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "synthetic molecular 10e/20o (seed 7, rank 60), initiator 3, "
-                               "PCHB unif-unif:fast-fast + heat-bath singles (psingles 0.1), tau 0.005, plain FP64",
+                               "PCHB unif-unif:fast-fast + heat-bath singles (psingles 0.1), tau 0.005, "
+                               "target-shiftdamp 0.5, plain FP64",
                    "target_walkers": args.walkers,
                    "mean_tot_parts": walker_iters / K, "mean_determinants": counts["Nd"],
                    "attempts_per_step": counts["Na"], "spawns_per_step": counts["Ns"],

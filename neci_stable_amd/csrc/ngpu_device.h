@@ -62,8 +62,10 @@ __host__ __device__ inline double u01(uint32_t lo, uint32_t hi) {
 }
 
 // Two uniforms for (det, iteration, replica, k, purpose).  Purposes:
-// 0 per-det (num-to-spawn, death), 1-3 spawn attempt k, 4 new-det rounding,
-// 5 statistics rounding.
+// 0 per-det (num-to-spawn, death); 1-3 spawn attempt k: 1 gives d0, d1, 2
+// gives d3 (PCHB hole pair) and d5 (spawn rounding), 3 gives d2 and d4 (read
+// only by FAST particle selection, heat-bath singles and the uniform
+// generator); 4 new-det rounding; 5 statistics rounding.
 template <int W>
 __host__ __device__ inline void draw2(uint32_t seed, const uint64_t* det, uint32_t iter, int rep,
                                       uint32_t k, uint32_t purpose, double& ua, double& ub) {

@@ -733,8 +733,12 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* 
     if (mode == 1 && single) {
       queue = true;
     } else {
-      draw2<W>(s.seed, key, iter, r, k, 2, dr[2], dr[3]);
-      draw2<W>(s.seed, key, iter, r, k, 3, dr[4], dr[5]);
+      // purpose 2: the PCHB hole pair (d3) and the spawn rounding (d5); purpose
+      // 3 (d2, d4) only for the generators that read them
+      draw2<W>(s.seed, key, iter, r, k, 2, dr[3], dr[5]);
+      dr[2] = dr[4] = 0.0;
+      if (G == kGenUniform || (G == kGenPchb && (s.particles == 1 || single)))
+        draw2<W>(s.seed, key, iter, r, k, 3, dr[2], dr[4]);
       Excit ex;
       if constexpr (G == kGenHubbard) {
         ex = gen_hubbard<W>(s, key, dr);
@@ -969,12 +973,13 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
     uint64_t key[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) key[w] = act ? wk.keys[slot * W + w] : s.ref[w];  // idle groups: any valid det
-    // lane g < 3 draws Philox block (purpose g+1); d1 = dsel, d2 = dorb, d5 = spawn rounding
+    // lane g < 3 draws Philox block (purpose g+1); d1 = dsel (purpose 1), d5 =
+    // spawn rounding (purpose 2, second value), d2 = dorb (purpose 3, first)
     double ua = 0.0, ub = 0.0;
     if (g < 3) draw2<W>(s.seed, key, iter, r, k, (uint32_t)(g + 1), ua, ub);
     const double d1 = __shfl(ub, gbase + 0);
-    const double d2 = __shfl(ua, gbase + 1);
-    const double d5 = __shfl(ub, gbase + 2);
+    const double d5 = __shfl(ub, gbase + 1);
+    const double d2 = __shfl(ua, gbase + 2);
     int elec = 1 + (int)floor(d1 * (double)s.nel);
     int src = select_orb<W>(key, ~0ull, elec);
     int a = spat(src), off = is_beta(src) ? 1 : 0;

@@ -11,6 +11,13 @@ in FP64 from the cycle-accumulated device statistics:
 
 During the single-particle phase the shift is held until TotParts exceeds
 TOTALWALKERS, then jumps to the cycle projected energy (tJumpShift, :1063-1083).
+With TARGET-SHIFTDAMP (tTargetShiftdamp, Calc.F90:1532-1550) there is no
+single-particle phase and the update gains a term that pulls the population
+to TOTALWALKERS (:1147-1154):
+
+  DiagSft -= (ln(growth) * SftDamp + ln(TotParts / TOTALWALKERS) * SftDamp2) / (tau * StepsSft)
+
+with SftDamp2 = SftDamp^2 / 4 unless given.
 """
 import math
 import time
@@ -64,7 +71,8 @@ class FciMCPar:
     tau, shift, wait=False) and cycle_stats() -> (sums, last, n_iters)."""
 
     def __init__(self, backend, *, tau, target_walkers, steps_shift=10, shift_damp=0.1,
-                 diag_shift=0.0, jump_shift=True, hii=None, log=None):
+                 diag_shift=0.0, jump_shift=True, hii=None, log=None, target_shiftdamp=False,
+                 shift_damp2=None):
         self.b = backend
         self.tau = float(tau)
         self.target = float(target_walkers)
@@ -72,7 +80,12 @@ class FciMCPar:
         self.shift_damp = float(shift_damp)
         self.S = backend.S
         self.shift = np.full(self.S, float(diag_shift))
-        self.single_part = np.ones(self.S, dtype=bool)
+        self.target_shiftdamp = bool(target_shiftdamp)
+        if self.target_shiftdamp:
+            self.shift_damp2 = self.shift_damp ** 2 / 4.0 if shift_damp2 is None else float(shift_damp2)
+            if not (0.0 < self.shift_damp < 1.0 and 0.0 < self.shift_damp2 < self.shift_damp):
+                raise ValueError("shift damping factors must lie in (0, 1) with SftDamp2 < SftDamp")
+        self.single_part = np.full(self.S, not self.target_shiftdamp)
         self.vary_shift_iter = np.zeros(self.S, dtype=np.int64)
         self.jump_shift = jump_shift
         self.hii = backend.hii if hii is None else hii
@@ -105,7 +118,10 @@ class FciMCPar:
                     self.shift[r] = proje[r]
                     defer[r] = True
             if not self.single_part[r] and not defer[r] and growth is not None and growth[r] > 0:
-                self.shift[r] -= math.log(growth[r]) * self.shift_damp / (self.tau * self.steps_shift)
+                d = math.log(growth[r]) * self.shift_damp
+                if self.target_shiftdamp and av[r] > 0:
+                    d += math.log(av[r] / self.target) * self.shift_damp2
+                self.shift[r] -= d / (self.tau * self.steps_shift)
         self.old_av_walkers = av
         if not np.all(self.single_part):
             self.sum_enum += en

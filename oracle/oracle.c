@@ -1673,10 +1673,13 @@ int32_t o_iterate(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_t 
             double wsign = sg[r] >= 0.0 ? 1.0 : -1.0;
             int parent_init = (int)((*fl >> r) & 1);
             for (int64_t p = 0; p < nsp; ++p) {
+                /* d0, d1 from purpose 1; the PCHB hole pair d3 and the spawn rounding d5
+                   from purpose 2; d2, d4 (FAST particles, heat-bath singles, the uniform
+                   generator) from purpose 3 */
                 double d[6];
                 draw2(s, row, W, iter, r, (uint32_t)p, 1, &d[0], &d[1]);
-                draw2(s, row, W, iter, r, (uint32_t)p, 2, &d[2], &d[3]);
-                draw2(s, row, W, iter, r, (uint32_t)p, 3, &d[4], &d[5]);
+                draw2(s, row, W, iter, r, (uint32_t)p, 2, &d[3], &d[5]);
+                draw2(s, row, W, iter, r, (uint32_t)p, 3, &d[2], &d[4]);
                 int32_t ic, tpar, gfail = 0;
                 double pgen;
                 if (!o_gen_excit_ctx(s, row, d, iter, r, (uint32_t)p, nJ, ilutJ, &ic, ex, &tpar, &pgen, &gfail)) {

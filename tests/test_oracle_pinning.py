@@ -259,6 +259,27 @@ def test_oracle_fciqmc_energy_vs_exact_diagonalisation():
     assert abs(mean + o.hii - e_ed) < 3 * err + 1e-3, (mean, err, e_ed)
 
 
+def test_target_shiftdamp_holds_population_and_energy():
+    """TARGET-SHIFTDAMP (fcimc_iter_utilities.F90:1147-1154): the shift varies
+    from the start and its extra term ln(N/N_target)*SftDamp2 pulls the
+    population to TOTALWALKERS; the shift still averages to the ED energy."""
+    from neci_stable_amd.fcimc import FciMCPar, OracleCycleAdapter, reblock
+    sysm = HubbardChain(6, 6, 4.0, bhub=-1.0)
+    spec = Spec(sysm, init_walkers=3000, seed=7, trunc_initiator=False)
+    o = oc.Oracle(spec, 0.02)
+    o.add_walkers(o.ref[None, :], np.array([[10.0]]), [1])
+    drv = FciMCPar(OracleCycleAdapter(o, 1), tau=0.02, target_walkers=3000, shift_damp=0.3,
+                   target_shiftdamp=True)
+    assert not drv.single_part.any()
+    drv.run(6000)
+    pops = np.array([r["tot_parts"][0] for r in drv.rows])
+    assert abs(pops[-300:].mean() / 3000.0 - 1.0) < 0.1, pops[-300:].mean()
+    sh = np.array([r["shift"][0] for r in drv.rows])
+    mean, err = reblock(sh[len(sh) // 4:])
+    e_ed = ed.ground_state(ed.hubbard_hamiltonian(6, sysm.nbr, 4.0, -1.0, ed.hubbard_dets(6, 3, 3)))
+    assert abs(mean + o.hii - e_ed) < 3 * err + 1e-3, (mean, err, e_ed)
+
+
 # ------------------------------------------------ C ABI of the engine
 
 
