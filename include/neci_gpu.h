@@ -222,8 +222,11 @@ int ngpu_get_cycle_stats(ngpu_t *h, ngpu_iter_out_t *sum, ngpu_iter_out_t *last,
    lib/timing.F90:122-188).  Each stage is timed from the start of its first
    kernel to the end of its last.  Stages: 0 prepass (initiator, proj-E, death),
    1 attempt scan (2 kernels), 2 spawn (all attempts; molecular: doubles, singles
-   queued), 3 queued singles, 4 sort (all radix passes), 5 merge, 6 annihilate,
-   7 insert, 8 hash statistics.  Stages 0, 2, 3, 5, 6, 7 and 8 are one kernel each. */
+   queued), 3 queued singles, 4 sort (all radix passes), 5 merge (empty: fused
+   into 6), 6 merge and annihilation, 7 insert, 8 hash statistics.  Stages 0, 2,
+   3, 6, 7 and 8 are one kernel each.  on is a bit mask of the stages to time
+   (bit s for stage s; 0 switches timing off); each timed stage adds two events
+   to its launches, so time only the stages needed. */
 int ngpu_set_profiling(ngpu_t *h, int32_t on);
 int ngpu_get_stage_times(ngpu_t *h, double *ms_total, int32_t nstages, int64_t *n_iters);
 

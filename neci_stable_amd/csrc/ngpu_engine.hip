@@ -110,6 +110,7 @@ struct ngpu_engine {
   bool pending = false;
   // profiling
   bool prof = false;
+  uint32_t prof_mask = 0;  // stages timed while prof is set
   hipEvent_t ev[kEvRing][kStages][2];  // start of a stage's first kernel, end of its last
   uint32_t ev_mask[kEvRing] = {0};      // stages whose both events were recorded
   bool ev_pending[kEvRing] = {false};
@@ -246,8 +247,8 @@ template <typename F, typename... Args>
 void launch_lds(ngpu_t* h, int open, int close, F kern, dim3 grid, dim3 block, size_t lds, Args... args) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (h->prof && h->ev_ok) {
-    if (open >= 0) e0 = h->ev[h->ev_cur][open][0];
-    if (close >= 0) {
+    if (open >= 0 && ((h->prof_mask >> open) & 1u)) e0 = h->ev[h->ev_cur][open][0];
+    if (close >= 0 && ((h->prof_mask >> close) & 1u)) {
       e1 = h->ev[h->ev_cur][close][1];
       h->ev_mask[h->ev_cur] |= 1u << close;
     }
@@ -260,8 +261,8 @@ template <typename F, typename... Args>
 void launch(ngpu_t* h, int open, int close, F kern, dim3 grid, dim3 block, Args... args) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (h->prof && h->ev_ok) {
-    if (open >= 0) e0 = h->ev[h->ev_cur][open][0];
-    if (close >= 0) {
+    if (open >= 0 && ((h->prof_mask >> open) & 1u)) e0 = h->ev[h->ev_cur][open][0];
+    if (close >= 0 && ((h->prof_mask >> close) & 1u)) {
       e1 = h->ev[h->ev_cur][close][1];
       h->ev_mask[h->ev_cur] |= 1u << close;
     }
@@ -1425,6 +1426,7 @@ int ngpu_set_profiling(ngpu_t* h, int32_t on) {
   if (h->batch) return NGPU_ERR_STATE;
   if (h->pending) (void)finish(h, nullptr);
   h->prof = on != 0;
+  h->prof_mask = (uint32_t)on & ((1u << kStages) - 1u);
   for (int s = 0; s < kStages; ++s) h->stage_ms[s] = 0.0;
   h->prof_iters = 0;
   return NGPU_OK;

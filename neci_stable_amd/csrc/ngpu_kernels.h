@@ -1099,6 +1099,23 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
 // position p of C(p, i)): fewer passes over narrower keys.  Otherwise the
 // key is the determinant itself, ordered as ilut_lt orders it.
 constexpr int kRadixBins = 256;
+constexpr int kRadixItems = 8;
+constexpr int kRadixTile = kRadixItems * kBlock;
+
+// element range of this block in the histogram and scatter passes: whole
+// tiles, and no more blocks than there are tiles (blocks past the last tile
+// get an empty range), so that a small spawned list is not spread over more
+// blocks than it fills
+__device__ inline void radix_chunk(int64_t n, int64_t& begin, int64_t& end) {
+  int64_t tiles = (n + kRadixTile - 1) / kRadixTile;
+  int64_t nb = tiles < (int64_t)gridDim.x ? tiles : (int64_t)gridDim.x;
+  if (nb < 1) nb = 1;
+  int64_t per = (tiles + nb - 1) / nb * kRadixTile;
+  begin = (int64_t)blockIdx.x * per;
+  end = begin + per;
+  if (end > n) end = n;
+  if (begin > n) begin = n;
+}
 
 // sort key: NW words of type K per element; FLIP compares word 0 as signed
 template <typename K, int NW, bool FLIP>
@@ -1117,7 +1134,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, int kstrid
   int64_t n = (int64_t)*n_ptr;
   if (n > ncap) n = ncap;
   int64_t begin, end;
-  chunk_of(n, begin, end);
+  radix_chunk(n, begin, end);
   for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x)
     atomicAdd(&h[radix_digit<K, NW, FLIP>(keys + i * kstride, word, shift)], 1u);
   __syncthreads();
@@ -1136,7 +1153,7 @@ __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, cons
   int64_t n = (int64_t)*n_ptr;
   if (n > ncap) n = ncap;
   int64_t begin, end;
-  chunk_of(n, begin, end);
+  radix_chunk(n, begin, end);
   for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
     uint32_t r = colex_rank<W>(binom, nel1, keys + i * kRecWords);
     rank[i] = r;
@@ -1170,9 +1187,6 @@ __global__ void __launch_bounds__(kBlock) k_radix_scan(uint32_t* hist, int nb, u
 // tile is ranked by digit (ballot peers within a wave, LDS counts across waves
 // and rounds, in element order), reordered by digit in LDS, then written out
 // so that consecutive threads write consecutive positions of each digit's run.
-constexpr int kRadixItems = 8;
-constexpr int kRadixTile = kRadixItems * kBlock;
-
 template <typename K, int NW, bool FLIP>
 __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstride, const uint32_t* iin, K* kout,
                                                           uint32_t* iout, const unsigned long long* n_ptr,
@@ -1198,7 +1212,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
   int64_t n = (int64_t)*n_ptr;
   if (n > ncap) n = ncap;
   int64_t begin, end;
-  chunk_of(n, begin, end);
+  radix_chunk(n, begin, end);
   const uint64_t ltmask = lanemask_lt();
   for (int64_t tb = begin; tb < end; tb += kRadixTile) {
     K k[kRadixItems][NW];

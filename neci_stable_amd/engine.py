@@ -158,8 +158,12 @@ class Engine:
         """Replay one captured graph per iteration (ngpu_set_graph)."""
         check(self.lib.ngpu_set_graph(self.h, int(on)), "ngpu_set_graph")
 
+    ALL_STAGES = (1 << 9) - 1
+
     def set_profiling(self, on=True):
-        check(self.lib.ngpu_set_profiling(self.h, int(on)), "ngpu_set_profiling")
+        """Stage timing: True for every stage, False for none, or a bit mask of stages."""
+        mask = self.ALL_STAGES if on is True else (0 if on is False else int(on))
+        check(self.lib.ngpu_set_profiling(self.h, mask), "ngpu_set_profiling")
 
     def stage_times(self):
         ms = np.zeros(9)
