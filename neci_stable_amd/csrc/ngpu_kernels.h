@@ -1099,7 +1099,7 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
 // position p of C(p, i)): fewer passes over narrower keys.  Otherwise the
 // key is the determinant itself, ordered as ilut_lt orders it.
 constexpr int kRadixBins = 256;
-constexpr int kRadixItems = 8;
+constexpr int kRadixItems = 4;
 constexpr int kRadixTile = kRadixItems * kBlock;
 
 // element range of this block in the histogram and scatter passes: whole
