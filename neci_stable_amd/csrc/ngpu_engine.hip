@@ -124,6 +124,10 @@ struct ngpu_engine {
   // batch's array and own_params keeps the engine's allocation
   IterParams* dparams = nullptr;
   IterParams* own_params = nullptr;
+  // inputs of the next iteration when its first kernel writes them (launches
+  // issued one by one); a captured graph reads what k_set_params wrote
+  IterParams next_params{};
+  bool chain_sets_params = false;
   // replay of the captured iteration chain
   bool use_graph = false;
   bool graph_dirty = true;
@@ -162,7 +166,9 @@ int err_from_mask(unsigned int m) {
   return NGPU_OK;
 }
 
-__global__ void k_begin_iter(ngpu_iter_out_t* out, double* part, Counters* cnt) {
+__global__ void k_begin_iter(ngpu_iter_out_t* out, double* part, Counters* cnt, IterParams* prm, IterParams v,
+                             int write_params) {
+  if (write_params && threadIdx.x == 0) *prm = v;  // the iteration's inputs (not in a captured graph)
   double* g = (double*)out;
   for (int i = threadIdx.x; i < S_NDOUBLE; i += blockDim.x) g[i] = 0.0;
   for (int i = threadIdx.x; i < kStatCopies * kStatStride; i += blockDim.x) part[i] = 0.0;
@@ -185,7 +191,8 @@ __global__ void k_begin_iter(ngpu_iter_out_t* out, double* part, Counters* cnt) 
 // accumulation into the cycle sums (ENumCyc, HFCyc, NoBorn, ... are cycle
 // accumulators in the reference too)
 __global__ void k_end_iter(ngpu_iter_out_t* out, const double* part, ngpu_iter_out_t* cyc, Counters* cnt,
-                           int64_t scap) {
+                           int64_t scap, int64_t hash_entries) {
+  if (threadIdx.x == 0) hash_done(cnt, hash_entries);  // after the rebuild kernels, if they ran
   double* g = (double*)out;
   double* c = (double*)cyc;
   for (int i = threadIdx.x; i < kStatStride; i += blockDim.x) {
@@ -272,13 +279,13 @@ void launch(ngpu_t* h, int open, int close, F kern, dim3 grid, dim3 block, Args.
 }
 
 // tombstone reclamation and the load guard, launched unconditionally (the
-// kernels exit at once unless the device-side check asks for a rebuild)
+// kernels exit at once unless the counters call for a rebuild); with_done:
+// also the bookkeeping kernel (in an iteration k_end_iter does it)
 template <int W>
-void hash_upkeep(ngpu_t* h) {
-  launch(h, -1, -1, k_hash_check, dim3(1), dim3(1), h->cnt, h->wk.L * (int64_t)kBucketEntries);
+void hash_upkeep(ngpu_t* h, bool with_done = true) {
   launch(h, -1, -1, k_hash_clear<W>, dim3(h->nb_grid), dim3(kBlock), h->wk, (const Counters*)h->cnt);
   launch(h, -1, -1, k_hash_refill<W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt);
-  launch(h, -1, -1, k_hash_done, dim3(1), dim3(1), h->cnt);
+  if (with_done) launch(h, -1, -1, k_hash_done, dim3(1), dim3(1), h->cnt, h->wk.L * (int64_t)kBucketEntries);
 }
 
 // sort of the spawned list (stage 4) then the merge of equal determinants
@@ -389,19 +396,19 @@ void annihilation_chain(ngpu_t* h, bool with_core, bool list_out = false) {
          h->wk.cap);
   launch(h, 8, 8, k_stats<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
          (const IterParams*)h->dparams);
-  hash_upkeep<W>(h);
+  hash_upkeep<W>(h, false);
   launch(h, -1, -1, k_end_iter, dim3(1), dim3(kBlock), h->dout, (const double*)h->dpart, h->dcyc, h->cnt,
-         h->scap);
+         h->scap, h->wk.L * (int64_t)kBucketEntries);
 }
 
 template <int W, int R>
 void iterate_chain(ngpu_t* h) {
-  launch(h, -1, -1, k_begin_iter, dim3(1), dim3(kBlock), h->dout, h->dpart, h->cnt);
+  launch(h, -1, -1, k_begin_iter, dim3(1), dim3(kBlock), h->dout, h->dpart, h->cnt, h->dparams, h->next_params,
+         h->chain_sets_params ? 1 : 0);
   launch(h, 0, 0, k_prepass<W, R>, dim3(h->nb_chunk), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart, h->att_cnt,
          h->pinfo, h->blk_part, (const IterParams*)h->dparams);
   if (h->ncore) core_projection<R>(h);
-  launch(h, 1, -1, k_scan_partials, dim3(1), dim3(kBlock), h->blk_part, h->nb_chunk, h->cnt);
-  launch(h, -1, 1, k_scan_apply, dim3(h->nb_chunk), dim3(kBlock), (const int32_t*)h->att_cnt, h->att_off,
+  launch(h, 1, 1, k_scan_apply, dim3(h->nb_chunk), dim3(kBlock), (const int32_t*)h->att_cnt, h->att_off,
          (const int64_t*)h->blk_part, h->cnt, h->S, h->tile_p, h->tcap);
   auto spawn = h->sys.model == NGPU_MODEL_HUBBARD ? k_spawn<W, R, kGenHubbard>
              : h->ds.uniform ? k_spawn<W, R, kGenUniform> : k_spawn<W, R, kGenPchb>;
@@ -1239,20 +1246,23 @@ int ngpu_iterate(ngpu_t* h, const ngpu_iter_in_t* in, ngpu_iter_out_t* out) {
   if (h->batch) return NGPU_ERR_STATE;  // batched engines iterate through ngpu_batch_iterate
   // iterations queue on the stream back to back: nothing in the chain waits
   // for the host; errors surface at the next synchronising call
-  set_params(h, in->tau, in->diag_sft, in->iter);
   if (h->use_graph && !h->prof) {
     if (h->graph_dirty || !h->gexec) {
       rc = capture_engine(h);
       if (rc) return rc;
     }
+    set_params(h, in->tau, in->diag_sft, in->iter);
     HIP_OK(hipGraphLaunch(h->gexec, h->stream));
   } else {
+    h->next_params = make_params(h, in->tau, in->diag_sft, in->iter);
+    h->chain_sets_params = true;
     if (h->prof && h->ev_ok) {
       h->ev_cur = (int)(h->ev_next % kEvRing);
       harvest(h, h->ev_cur);
       h->ev_mask[h->ev_cur] = 0;
     }
     run_chain(h);
+    h->chain_sets_params = false;
     if (h->prof && h->ev_ok) {
       h->ev_pending[h->ev_cur] = true;
       h->ev_next++;
@@ -1644,7 +1654,8 @@ int ngpu_annihilate(ngpu_t* h, uint32_t iter, const int64_t* sdet, const double*
     }
     HIP_OK(hipMemcpy(h->sp_rec, rec.data(), rec.size() * 8, hipMemcpyHostToDevice));
   }
-  hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dpart, h->cnt);
+  hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dpart, h->cnt, h->dparams,
+                     IterParams{}, 0);
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_spawned, (unsigned long long)n);
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, h->u_count, 0ull);
   bool p = h->prof;

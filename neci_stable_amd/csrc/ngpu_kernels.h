@@ -61,7 +61,6 @@ struct alignas(256) Counters {
   unsigned long long n_attempts;
   unsigned long long n_fail;      // inserts refused (capacity errors)
   unsigned int err;
-  unsigned int rebuild;           // set when tombstones must be reclaimed
   alignas(256) unsigned long long free_tail;
   alignas(256) unsigned long long n_spawned;
   alignas(256) unsigned long long n_singles;  // queued single-excitation attempts
@@ -513,24 +512,6 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
   bs.flush(part);
 }
 
-// exclusive scan of the per-block partials (single block)
-__global__ void __launch_bounds__(kBlock) k_scan_partials(int64_t* part, int nparts, Counters* cnt) {
-  __shared__ int64_t red[kBlock / kWave];
-  int per = (nparts + kBlock - 1) / kBlock;
-  int b0 = threadIdx.x * per;
-  int64_t loc = 0;
-  for (int i = 0; i < per; ++i)
-    if (b0 + i < nparts) loc += part[b0 + i];
-  int64_t tot;
-  int64_t pre = block_exclusive_scan(loc, red, tot);
-  for (int i = 0; i < per; ++i)
-    if (b0 + i < nparts) {
-      int64_t v = part[b0 + i];
-      part[b0 + i] = pre;
-      pre += v;
-    }
-  if (threadIdx.x == 0) cnt->n_attempts = (unsigned long long)tot;
-}
 
 // attempt offsets per (slot, replica) pair, same chunking as k_prepass; also
 // the parent pair of the first attempt of every kBlock-attempt tile (the
@@ -543,7 +524,11 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(const int32_t* att_cnt, i
   int64_t begin, end;
   chunk_of(nslots, begin, end);
   int64_t pb = begin * S, pe = end * S;
-  int64_t run = part[blockIdx.x];
+  // this block's first offset: the attempt counts of the blocks before it
+  int64_t loc = 0;
+  for (int q = threadIdx.x; q < (int)blockIdx.x; q += blockDim.x) loc += part[q];
+  int64_t run;
+  (void)block_exclusive_scan(loc, red, run);
   for (int64_t base = pb; base < pe; base += blockDim.x) {
     int64_t i = base + threadIdx.x;
     int64_t v = i < pe ? att_cnt[i] : 0;
@@ -556,7 +541,10 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(const int32_t* att_cnt, i
     }
     run += tot;
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) att_off[nslots * S] = (int64_t)cnt->n_attempts;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    att_off[nslots * S] = run;
+    cnt->n_attempts = (unsigned long long)run;
+  }
 }
 
 // ------------------------------------------------------------- spawning
@@ -1854,21 +1842,30 @@ __global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* c
 // tombstones pass 10 %): later rebuilds against longer probe sequences
 constexpr unsigned long long kRebuildUsedPct = 60;
 
-__global__ void k_hash_check(Counters* cnt, int64_t entries) {
-  // every removal of the iteration left a tombstone and queued its slot
+// the rebuild decision, a pure function of the counters after the
+// iteration's removals (nothing changes them between k_stats and the end of
+// the iteration), so the clear, refill and bookkeeping kernels each evaluate
+// it and no kernel has to publish it
+__device__ inline bool hash_rebuild_due(const Counters* cnt, int64_t entries) {
+  unsigned long long removed = cnt->free_tail - cnt->free_tail0;  // each left a tombstone
+  unsigned long long live = cnt->n_live - removed, tomb = cnt->n_tomb + removed;
+  return (live + tomb) * 100ull > (unsigned long long)entries * kRebuildUsedPct &&
+         tomb * 10ull > (unsigned long long)entries;
+}
+
+// bookkeeping of the iteration's removals and of a rebuild (one thread)
+__device__ inline void hash_done(Counters* cnt, int64_t entries) {
+  bool rebuilt = hash_rebuild_due(cnt, entries);
   unsigned long long removed = cnt->free_tail - cnt->free_tail0;
   cnt->n_live -= removed;
-  cnt->n_tomb += removed;
+  cnt->n_tomb = rebuilt ? 0ull : cnt->n_tomb + removed;
   cnt->free_tail0 = cnt->free_tail;
-  unsigned long long used = cnt->n_live + cnt->n_tomb;
-  cnt->rebuild = (used * 100ull > (unsigned long long)entries * kRebuildUsedPct &&
-                  cnt->n_tomb * 10ull > (unsigned long long)entries) ? 1u : 0u;
   if (cnt->n_live * 10ull > (unsigned long long)entries * 8ull) cnt->err |= 1u << NGPU_ERR_HASH_FULL;
 }
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_hash_clear(Walk wk, const Counters* cnt) {
-  if (!cnt->rebuild) return;
+  if (!hash_rebuild_due(cnt, wk.L * (int64_t)kBucketEntries)) return;
   for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < wk.L; b += (int64_t)gridDim.x * blockDim.x) {
     int32_t* sl = (int32_t*)(wk.htab + (size_t)b * 8 * W + 4 * W);
 #pragma unroll
@@ -1878,7 +1875,7 @@ __global__ void __launch_bounds__(kBlock) k_hash_clear(Walk wk, const Counters* 
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_hash_refill(DevSys s, Walk wk, Counters* cnt) {
-  if (!cnt->rebuild) return;
+  if (!hash_rebuild_due(cnt, wk.L * (int64_t)kBucketEntries)) return;
   int64_t nslots = (int64_t)cnt->nslots;
   for (int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < nslots;
        slot += (int64_t)gridDim.x * blockDim.x) {
@@ -1894,11 +1891,6 @@ __global__ void __launch_bounds__(kBlock) k_hash_refill(DevSys s, Walk wk, Count
   }
 }
 
-__global__ void k_hash_done(Counters* cnt) {
-  if (cnt->rebuild) {
-    cnt->n_tomb = 0;
-    cnt->rebuild = 0;
-  }
-}
+__global__ void k_hash_done(Counters* cnt, int64_t entries) { hash_done(cnt, entries); }
 
 }  // namespace ngpu
