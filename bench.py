@@ -164,7 +164,7 @@ def build_workload(args):
     mol = synthetic_molecule(20, 10, 60, 7)
     spec = Spec(mol, init_walkers=args.walkers, seed=1234, trunc_initiator=True, init_walk_no=3.0,
                 particles=0, psingles=0.1, pparallel=0.5, memory_fac_spawn=8.0,
-                memory_fac_part=10.0, memory_fac_hash=2.0)
+                memory_fac_part=10.0, memory_fac_hash=args.memory_fac_hash)
     return spec
 
 
@@ -299,7 +299,10 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--profile-steps", type=int, default=50, help="steps of the all-stage timing pass")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--walkers", type=float, default=1e6)
+    ap.add_argument("--memory-fac-hash", type=float, default=0.7,
+                    help="MEMORYFACHASH: walker hash buckets per TOTALWALKERS (the reference's default 0.7)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-growth-iters", type=int, default=6000)
@@ -356,11 +359,17 @@ def main():
         av = float(np.sum(drv.rows[-1]["tot_parts"]))
         settled = settled + 1 if abs(av / args.walkers - 1.0) < 0.1 else 0
     t_growth = time.perf_counter() - tg
+    growth_iters = drv.iter
     print(f"# growth: {drv.iter} iterations, {t_growth:.2f} s, TotParts {drv.rows[-1]['tot_parts']}",
           file=sys.stderr, flush=True)
     drv.run(args.warmup)
     print(f"# warmup done: TotParts {drv.rows[-1]['tot_parts']}, dets {drv.rows[-1]['tot_walkers']}",
           file=sys.stderr, flush=True)
+    tuned = None
+    if not args.no_autotune:
+        from neci_stable_amd.autotune import autotune
+        tuned = autotune(drv)
+        print(f"# autotuned launch shape: {tuned['shape']}", file=sys.stderr, flush=True)
     # every stage timed over a short pass first (the breakdown, and the choice
     # of the roofline kernel); the timed region then carries events on that
     # kernel's launches only, since events on every launch slow the iteration
@@ -407,7 +416,9 @@ def main():
                    "mean_tot_parts": walker_iters / K, "mean_determinants": counts["Nd"],
                    "attempts_per_step": counts["Na"], "spawns_per_step": counts["Ns"],
                    "unique_spawned_per_step": counts["Nu"], "new_dets_per_step": counts["Nn"],
-                   "growth_iterations": drv.iter - K - args.warmup},
+                   "growth_iterations": growth_iters,
+                   "launch_shape": eng.launch_shape(),
+                   "autotuned": tuned is not None},
         "stage_ms_per_step": per_stage,
         "stage_ms_note": f"every stage timed over {args.profile_steps} steps before the timed region",
         "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,

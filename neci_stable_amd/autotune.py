@@ -1,0 +1,51 @@
+"""Launch-shape autotuner.
+
+The engine's kernels split their work over a number of blocks fixed at launch
+and the radix sort ranks tiles of a fixed number of elements per thread
+(ngpu_set_launch_shape).  The best values depend on the walker count and the
+system, so they are measured on the running calculation: one knob at a time
+(radix tile, then the chunked kernels' blocks, then the grid-stride kernels'
+blocks), each candidate timed over a few iterations of the driver, the
+fastest kept.  The walker dynamics do not depend on the shape, so tuning
+while the calculation runs changes nothing but its speed.
+"""
+import time
+
+
+def _time_iterations(driver, steps):
+    t0 = time.perf_counter()
+    driver.run(steps)
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
+def autotune(driver, steps=20, warm=5, repeats=2, log=None):
+    """Tune the launch shape of driver's engine (driver: FciMCPar over an
+    Engine).  Runs (warm + steps) * repeats iterations per candidate, about
+    25 candidates' worth in all.  Returns {"shape": {...}, "trials": [...]}."""
+    eng = driver.b
+    base = eng.launch_shape()
+    cu = base["n_cu"]
+    knobs = [
+        ("radix_items", [2, 4, 8]),
+        ("chunk_blocks", [2 * cu, 4 * cu, 8 * cu]),
+        ("grid_blocks", [4 * cu, 8 * cu, 16 * cu]),
+    ]
+    best = {k: base[k] for k in ("chunk_blocks", "grid_blocks", "radix_items")}
+    trials = []
+    for name, values in knobs:
+        scores = {}
+        for v in values:
+            shape = dict(best)
+            shape[name] = v
+            eng.set_launch_shape(**shape)
+            ms = []
+            for _ in range(repeats):
+                driver.run(warm)
+                ms.append(_time_iterations(driver, steps))
+            scores[v] = min(ms)
+            trials.append(dict(shape, ms_per_iteration=scores[v]))
+            if log:
+                log(f"autotune {shape}: {scores[v]:.4f} ms/iteration")
+        best[name] = min(scores, key=scores.get)
+    eng.set_launch_shape(**best)
+    return {"shape": best, "trials": trials}

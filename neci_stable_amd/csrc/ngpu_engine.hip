@@ -46,6 +46,8 @@ struct ngpu_engine {
   int W = 1, S = 1;
   int nb_chunk = 1024;  // blocks of the chunked kernels (prepass, scans, radix)
   int nb_grid = 2048;   // blocks of the grid-stride kernels
+  int radix_items = 4;  // elements per thread of a radix tile (2, 4 or 8)
+  int n_cu = 256, max_chunk = 4096;
   hipStream_t stream = nullptr;
   DevSys ds{};
   Walk wk{};
@@ -292,15 +294,15 @@ void hash_upkeep(ngpu_t* h, bool with_done = true) {
 // fused with their annihilation into the walker list (stage 6; stage 5, the
 // merge alone, is empty since the fusion); ranked: the spawn kernels already
 // wrote each record's colex rank
-template <int W, int R>
-void sort_and_merge(ngpu_t* h, bool ranked, bool list_out) {
+template <int W, int R, int IT>
+void sort_and_merge_it(ngpu_t* h, bool ranked, bool list_out) {
   const unsigned long long* n_ptr = &h->cnt->n_spawned;
   if (h->rank_sort) {
     if (ranked)
-      launch(h, 4, -1, k_radix_hist<uint32_t, 1, false>, dim3(h->nb_chunk), dim3(kBlock), (const uint32_t*)h->rk, 1,
+      launch(h, 4, -1, k_radix_hist<uint32_t, 1, false, IT>, dim3(h->nb_chunk), dim3(kBlock), (const uint32_t*)h->rk, 1,
              n_ptr, h->scap, 0, 0, h->hist);
     else
-      launch(h, 4, -1, k_rank_hist<W>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec, n_ptr,
+      launch(h, 4, -1, k_rank_hist<W, IT>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec, n_ptr,
              h->scap, (const uint32_t*)h->d_binom, h->sys.nel + 1, h->rk, h->hist);
     const int passes = (h->rank_bits + 7) / 8;
     const uint32_t* kin = h->rk;
@@ -309,10 +311,10 @@ void sort_and_merge(ngpu_t* h, bool ranked, bool list_out) {
     for (int p = 0; p < passes; ++p) {
       int out = p & 1;
       if (p > 0)
-        launch(h, -1, -1, k_radix_hist<uint32_t, 1, false>, dim3(h->nb_chunk), dim3(kBlock), kin, 1, n_ptr, h->scap,
+        launch(h, -1, -1, k_radix_hist<uint32_t, 1, false, IT>, dim3(h->nb_chunk), dim3(kBlock), kin, 1, n_ptr, h->scap,
                0, 8 * p, h->hist);
       launch(h, -1, -1, k_radix_scan, dim3(kRadixBins), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
-      launch(h, -1, p + 1 == passes ? 4 : -1, k_radix_scatter<uint32_t, 1, false>, dim3(h->nb_chunk), dim3(kBlock),
+      launch(h, -1, p + 1 == passes ? 4 : -1, k_radix_scatter<uint32_t, 1, false, IT>, dim3(h->nb_chunk), dim3(kBlock),
              kin, 1, iin, (uint32_t*)h->sk[out], h->si[out], n_ptr, h->scap, 0, 8 * p, (const uint32_t*)h->hist,
              (const uint32_t*)h->digit_tot);
       kin = (const uint32_t*)h->sk[out];
@@ -343,10 +345,10 @@ void sort_and_merge(ngpu_t* h, bool ranked, bool list_out) {
   for (size_t p = 0; p < passes.size(); ++p) {
     int out = (int)(p & 1);
     bool first = p == 0, last = p + 1 == passes.size();
-    launch(h, first ? 4 : -1, -1, k_radix_hist<uint64_t, W, W == 2>, dim3(h->nb_chunk), dim3(kBlock), kin, kstride,
+    launch(h, first ? 4 : -1, -1, k_radix_hist<uint64_t, W, W == 2, IT>, dim3(h->nb_chunk), dim3(kBlock), kin, kstride,
            n_ptr, h->scap, passes[p].word, passes[p].shift, h->hist);
     launch(h, -1, -1, k_radix_scan, dim3(kRadixBins), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
-    launch(h, -1, last ? 4 : -1, k_radix_scatter<uint64_t, W, W == 2>, dim3(h->nb_chunk), dim3(kBlock), kin, kstride, iin,
+    launch(h, -1, last ? 4 : -1, k_radix_scatter<uint64_t, W, W == 2, IT>, dim3(h->nb_chunk), dim3(kBlock), kin, kstride, iin,
            h->sk[out], h->si[out], n_ptr, h->scap, passes[p].word, passes[p].shift, (const uint32_t*)h->hist,
            (const uint32_t*)h->digit_tot);
     kin = h->sk[out];
@@ -361,6 +363,14 @@ void sort_and_merge(ngpu_t* h, bool ranked, bool list_out) {
   launch(h, 6, 6, k_merge_annihilate<W, R, uint64_t, W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt,
          h->dpart, (const uint64_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_rec, h->scap,
          h->n_key, h->n_sgn, h->n_flag, (const IterParams*)h->dparams);
+}
+
+// the sort with the engine's tile size (elements per thread of a radix tile)
+template <int W, int R>
+void sort_and_merge(ngpu_t* h, bool ranked, bool list_out) {
+  if (h->radix_items == 2) sort_and_merge_it<W, R, 2>(h, ranked, list_out);
+  else if (h->radix_items == 8) sort_and_merge_it<W, R, 8>(h, ranked, list_out);
+  else sort_and_merge_it<W, R, 4>(h, ranked, list_out);
 }
 
 // semistochastic projection of the core amplitudes (before spawning)
@@ -670,6 +680,8 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
   }
   h->nb_chunk = std::max(256, 4 * prop.multiProcessorCount);
   h->nb_grid = std::max(512, 8 * prop.multiProcessorCount);
+  h->n_cu = std::max(1, prop.multiProcessorCount);
+  h->max_chunk = 16 * h->n_cu;  // buffers sized by the chunk count allow up to this many blocks
   int rc = NGPU_OK;
   auto fail = [&](hipError_t e) {
     if (e != hipSuccess && rc == NGPU_OK) {
@@ -694,7 +706,7 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
   fail(dalloc(&h->att_cnt, (size_t)cap * S));
   fail(dalloc(&h->att_off, (size_t)cap * S + 1));
   fail(dalloc(&h->pinfo, (size_t)cap * S));
-  fail(dalloc(&h->blk_part, (size_t)h->nb_chunk));
+  fail(dalloc(&h->blk_part, (size_t)h->max_chunk));
   // tiles of attempts with a recorded first parent: attempts up to 4x the
   // spawn capacity (beyond it the spawn kernel searches the offsets itself)
   h->tcap = (4 * scap) / kBlock + 2;
@@ -704,7 +716,7 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
     fail(dalloc(&h->sk[q], (size_t)scap * W));
     fail(dalloc(&h->si[q], (size_t)scap));
   }
-  fail(dalloc(&h->hist, (size_t)h->nb_chunk * kRadixBins));
+  fail(dalloc(&h->hist, (size_t)h->max_chunk * kRadixBins));
   {
     // C(nbasis, nel) decides the sort key; the table holds C(p, i) for the
     // colex rank of a determinant (p < nbasis, i <= nel)
@@ -1272,6 +1284,33 @@ int ngpu_iterate(ngpu_t* h, const ngpu_iter_in_t* in, ngpu_iter_out_t* out) {
   if (out) HIP_OK(hipMemcpyAsync(h->hout, h->dout, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));
   h->pending = true;
   if (out) return finish(h, out);
+  return NGPU_OK;
+}
+
+int ngpu_set_launch_shape(ngpu_t* h, int32_t chunk_blocks, int32_t grid_blocks, int32_t radix_items) {
+  if (!h) return NGPU_ERR_ARG;
+  if (chunk_blocks < 0 || chunk_blocks > h->max_chunk || (chunk_blocks && chunk_blocks < 64) || grid_blocks < 0 ||
+      (grid_blocks && grid_blocks < 64) || grid_blocks > 64 * h->n_cu ||
+      !(radix_items == 0 || radix_items == 2 || radix_items == 4 || radix_items == 8))
+    return NGPU_ERR_ARG;
+  if (h->pending) {
+    int rc = finish(h, nullptr);
+    if (rc) return rc;
+  }
+  if (chunk_blocks) h->nb_chunk = chunk_blocks;
+  if (grid_blocks) h->nb_grid = grid_blocks;
+  if (radix_items) h->radix_items = radix_items;
+  h->graph_dirty = true;
+  return NGPU_OK;
+}
+
+int ngpu_get_launch_shape(ngpu_t* h, int32_t* chunk_blocks, int32_t* grid_blocks, int32_t* radix_items,
+                          int32_t* n_cu) {
+  if (!h) return NGPU_ERR_ARG;
+  if (chunk_blocks) *chunk_blocks = h->nb_chunk;
+  if (grid_blocks) *grid_blocks = h->nb_grid;
+  if (radix_items) *radix_items = h->radix_items;
+  if (n_cu) *n_cu = h->n_cu;
   return NGPU_OK;
 }
 

@@ -1087,14 +1087,16 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
 // position p of C(p, i)): fewer passes over narrower keys.  Otherwise the
 // key is the determinant itself, ordered as ilut_lt orders it.
 constexpr int kRadixBins = 256;
-constexpr int kRadixItems = 4;
-constexpr int kRadixTile = kRadixItems * kBlock;
+// radix tiles: IT elements per thread, IT*kBlock per tile (IT = 2, 4 or 8,
+// chosen per engine at run time)
 
 // element range of this block in the histogram and scatter passes: whole
 // tiles, and no more blocks than there are tiles (blocks past the last tile
 // get an empty range), so that a small spawned list is not spread over more
 // blocks than it fills
+template <int IT>
 __device__ inline void radix_chunk(int64_t n, int64_t& begin, int64_t& end) {
+  constexpr int kRadixTile = IT * kBlock;
   int64_t tiles = (n + kRadixTile - 1) / kRadixTile;
   int64_t nb = tiles < (int64_t)gridDim.x ? tiles : (int64_t)gridDim.x;
   if (nb < 1) nb = 1;
@@ -1113,7 +1115,7 @@ __device__ inline uint32_t radix_digit(const K* k, int word, int shift) {
   return (uint32_t)(x >> shift) & 0xFFu;
 }
 
-template <typename K, int NW, bool FLIP>
+template <typename K, int NW, bool FLIP, int IT>
 __global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, int kstride, const unsigned long long* n_ptr,
                                                        int64_t ncap, int word, int shift, uint32_t* hist) {
   __shared__ uint32_t h[kRadixBins];
@@ -1122,7 +1124,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, int kstrid
   int64_t n = (int64_t)*n_ptr;
   if (n > ncap) n = ncap;
   int64_t begin, end;
-  radix_chunk(n, begin, end);
+  radix_chunk<IT>(n, begin, end);
   for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x)
     atomicAdd(&h[radix_digit<K, NW, FLIP>(keys + i * kstride, word, shift)], 1u);
   __syncthreads();
@@ -1131,7 +1133,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, int kstrid
 
 // colex rank of each spawned record's determinant (binom[p*(nel+1)+i] =
 // C(p, i)), fused with the histogram of its lowest digit
-template <int W>
+template <int W, int IT>
 __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, const unsigned long long* n_ptr,
                                                       int64_t ncap, const uint32_t* binom, int nel1,
                                                       uint32_t* rank, uint32_t* hist) {
@@ -1141,7 +1143,7 @@ __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, cons
   int64_t n = (int64_t)*n_ptr;
   if (n > ncap) n = ncap;
   int64_t begin, end;
-  radix_chunk(n, begin, end);
+  radix_chunk<IT>(n, begin, end);
   for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
     uint32_t r = colex_rank<W>(binom, nel1, keys + i * kRecWords);
     rank[i] = r;
@@ -1171,11 +1173,11 @@ __global__ void __launch_bounds__(kBlock) k_radix_scan(uint32_t* hist, int nb, u
   if (threadIdx.x == 0) digit_tot[blockIdx.x] = (uint32_t)tot;
 }
 
-// Stable scatter of one digit, in tiles of kRadixItems*kBlock elements: the
+// Stable scatter of one digit, in tiles of IT*kBlock elements: the
 // tile is ranked by digit (ballot peers within a wave, LDS counts across waves
 // and rounds, in element order), reordered by digit in LDS, then written out
 // so that consecutive threads write consecutive positions of each digit's run.
-template <typename K, int NW, bool FLIP>
+template <typename K, int NW, bool FLIP, int IT>
 __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstride, const uint32_t* iin, K* kout,
                                                           uint32_t* iout, const unsigned long long* n_ptr,
                                                           int64_t ncap, int word, int shift, const uint32_t* hist,
@@ -1184,8 +1186,8 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
   __shared__ uint32_t cnt[kRadixBins];     // digit counts of the tile so far
   __shared__ uint32_t tstart[kRadixBins];  // start of each digit inside the reordered tile
   __shared__ uint32_t wcnt[kBlock / kWave][kRadixBins];
-  __shared__ K lk[kRadixTile * NW];
-  __shared__ uint32_t lv[kRadixTile];
+  __shared__ K lk[(IT * kBlock) * NW];
+  __shared__ uint32_t lv[(IT * kBlock)];
   __shared__ int64_t red[kBlock / kWave];
   const int wv = threadIdx.x / kWave;
   {
@@ -1200,13 +1202,13 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
   int64_t n = (int64_t)*n_ptr;
   if (n > ncap) n = ncap;
   int64_t begin, end;
-  radix_chunk(n, begin, end);
+  radix_chunk<IT>(n, begin, end);
   const uint64_t ltmask = lanemask_lt();
-  for (int64_t tb = begin; tb < end; tb += kRadixTile) {
-    K k[kRadixItems][NW];
-    uint32_t v[kRadixItems], d[kRadixItems], off[kRadixItems];
+  for (int64_t tb = begin; tb < end; tb += (IT * kBlock)) {
+    K k[IT][NW];
+    uint32_t v[IT], d[IT], off[IT];
 #pragma unroll
-    for (int j = 0; j < kRadixItems; ++j) {
+    for (int j = 0; j < IT; ++j) {
       int64_t e = tb + (int64_t)j * kBlock + threadIdx.x;
       bool valid = e < end;
       d[j] = 0;
@@ -1249,7 +1251,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kRadixItems; ++j) {
+    for (int j = 0; j < IT; ++j) {
       if (off[j] != 0xFFFFFFFFu) {
         uint32_t lp = tstart[d[j]] + off[j];
 #pragma unroll
@@ -1258,7 +1260,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
       }
     }
     __syncthreads();
-    const int tn = (int)((end - tb) < kRadixTile ? (end - tb) : kRadixTile);
+    const int tn = (int)((end - tb) < (IT * kBlock) ? (end - tb) : (IT * kBlock));
     for (int i = threadIdx.x; i < tn; i += blockDim.x) {
       K kk[NW];
 #pragma unroll

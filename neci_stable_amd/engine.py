@@ -154,6 +154,15 @@ class Engine:
         self.ncore = dets.shape[0]
         return nnz.value
 
+    def set_launch_shape(self, chunk_blocks=0, grid_blocks=0, radix_items=0):
+        check(self.lib.ngpu_set_launch_shape(self.h, int(chunk_blocks), int(grid_blocks), int(radix_items)),
+              "ngpu_set_launch_shape")
+
+    def launch_shape(self):
+        v = [ctypes.c_int32() for _ in range(4)]
+        check(self.lib.ngpu_get_launch_shape(self.h, *[ctypes.byref(x) for x in v]), "ngpu_get_launch_shape")
+        return dict(chunk_blocks=v[0].value, grid_blocks=v[1].value, radix_items=v[2].value, n_cu=v[3].value)
+
     def set_graph(self, on=True):
         """Replay one captured graph per iteration (ngpu_set_graph)."""
         check(self.lib.ngpu_set_graph(self.h, int(on)), "ngpu_set_graph")
