@@ -1086,7 +1086,7 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
 // is the determinant's 32-bit colex rank (sum over its i-th occupied bit at
 // position p of C(p, i)): fewer passes over narrower keys.  Otherwise the
 // key is the determinant itself, ordered as ilut_lt orders it.
-constexpr int kRadixBins = 256;
+constexpr int kRadixBins = 1024;  // bins of the widest digit (10 bits)
 // radix tiles: IT elements per thread, IT*kBlock per tile (IT = 2, 4 or 8,
 // chosen per engine at run time)
 
@@ -1108,37 +1108,40 @@ __device__ inline void radix_chunk(int64_t n, int64_t& begin, int64_t& end) {
 }
 
 // sort key: NW words of type K per element; FLIP compares word 0 as signed
-template <typename K, int NW, bool FLIP>
+// DB-bit digit (DB = 8 or 10; kRadixBins is the largest digit range)
+template <typename K, int NW, bool FLIP, int DB = 8>
 __device__ inline uint32_t radix_digit(const K* k, int word, int shift) {
   K x = k[word];
   if (FLIP && word == 0) x ^= (K)1 << (sizeof(K) * 8 - 1);
-  return (uint32_t)(x >> shift) & 0xFFu;
+  return (uint32_t)(x >> shift) & ((1u << DB) - 1u);
 }
 
-template <typename K, int NW, bool FLIP, int IT>
+template <typename K, int NW, bool FLIP, int IT, int DB>
 __global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, int kstride, const unsigned long long* n_ptr,
                                                        int64_t ncap, int word, int shift, uint32_t* hist) {
-  __shared__ uint32_t h[kRadixBins];
-  for (int i = threadIdx.x; i < kRadixBins; i += blockDim.x) h[i] = 0;
+  constexpr int RB = 1 << DB;
+  __shared__ uint32_t h[RB];
+  for (int i = threadIdx.x; i < RB; i += blockDim.x) h[i] = 0;
   __syncthreads();
   int64_t n = (int64_t)*n_ptr;
   if (n > ncap) n = ncap;
   int64_t begin, end;
   radix_chunk<IT>(n, begin, end);
   for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x)
-    atomicAdd(&h[radix_digit<K, NW, FLIP>(keys + i * kstride, word, shift)], 1u);
+    atomicAdd(&h[radix_digit<K, NW, FLIP, DB>(keys + i * kstride, word, shift)], 1u);
   __syncthreads();
-  for (int d = threadIdx.x; d < kRadixBins; d += blockDim.x) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
+  for (int d = threadIdx.x; d < RB; d += blockDim.x) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
 }
 
 // colex rank of each spawned record's determinant (binom[p*(nel+1)+i] =
 // C(p, i)), fused with the histogram of its lowest digit
-template <int W, int IT>
+template <int W, int IT, int DB>
 __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, const unsigned long long* n_ptr,
                                                       int64_t ncap, const uint32_t* binom, int nel1,
                                                       uint32_t* rank, uint32_t* hist) {
-  __shared__ uint32_t h[kRadixBins];
-  for (int i = threadIdx.x; i < kRadixBins; i += blockDim.x) h[i] = 0;
+  constexpr int RB = 1 << DB;
+  __shared__ uint32_t h[RB];
+  for (int i = threadIdx.x; i < RB; i += blockDim.x) h[i] = 0;
   __syncthreads();
   int64_t n = (int64_t)*n_ptr;
   if (n > ncap) n = ncap;
@@ -1147,10 +1150,10 @@ __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, cons
   for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
     uint32_t r = colex_rank<W>(binom, nel1, keys + i * kRecWords);
     rank[i] = r;
-    atomicAdd(&h[r & 0xFFu], 1u);
+    atomicAdd(&h[r & (RB - 1u)], 1u);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < kRadixBins; d += blockDim.x) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
+  for (int d = threadIdx.x; d < RB; d += blockDim.x) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
 }
 
 // one block per digit: exclusive scan of that digit's per-block counts
@@ -1177,26 +1180,38 @@ __global__ void __launch_bounds__(kBlock) k_radix_scan(uint32_t* hist, int nb, u
 // tile is ranked by digit (ballot peers within a wave, LDS counts across waves
 // and rounds, in element order), reordered by digit in LDS, then written out
 // so that consecutive threads write consecutive positions of each digit's run.
-template <typename K, int NW, bool FLIP, int IT>
+template <typename K, int NW, bool FLIP, int IT, int DB>
 __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstride, const uint32_t* iin, K* kout,
                                                           uint32_t* iout, const unsigned long long* n_ptr,
                                                           int64_t ncap, int word, int shift, const uint32_t* hist,
                                                           const uint32_t* digit_tot) {
-  __shared__ uint32_t run[kRadixBins];     // next global position of each digit (this block)
-  __shared__ uint32_t cnt[kRadixBins];     // digit counts of the tile so far
-  __shared__ uint32_t tstart[kRadixBins];  // start of each digit inside the reordered tile
-  __shared__ uint32_t wcnt[kBlock / kWave][kRadixBins];
-  __shared__ K lk[(IT * kBlock) * NW];
-  __shared__ uint32_t lv[(IT * kBlock)];
+  constexpr int RB = 1 << DB;          // digit values
+  constexpr int DPT = RB / kBlock;     // digits owned by each thread (consecutive)
+  constexpr int TILE = IT * kBlock;
+  __shared__ uint32_t run[RB];     // next global position of each digit (this block)
+  __shared__ uint32_t cnt[RB];     // digit counts of the tile so far
+  __shared__ uint32_t tstart[RB];  // start of each digit inside the reordered tile
+  __shared__ uint32_t wcnt[kBlock / kWave][RB];
+  __shared__ K lk[TILE * NW];
+  __shared__ uint32_t lv[TILE];
   __shared__ int64_t red[kBlock / kWave];
   const int wv = threadIdx.x / kWave;
+  const int d0 = threadIdx.x * DPT;
   {
-    // digit bases: exclusive scan of the digit totals (blockDim == kRadixBins)
+    // digit bases: exclusive scan of the digit totals in digit order
+    int64_t loc = 0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) loc += digit_tot[d0 + q];
     int64_t tot;
-    int64_t base = block_exclusive_scan((int64_t)digit_tot[threadIdx.x], red, tot);
-    run[threadIdx.x] = (uint32_t)base + hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x];
-    cnt[threadIdx.x] = 0;
-    for (int q = 0; q < kBlock / kWave; ++q) wcnt[q][threadIdx.x] = 0;
+    int64_t base = block_exclusive_scan(loc, red, tot);
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      const int d = d0 + q;
+      run[d] = (uint32_t)base + hist[(size_t)d * gridDim.x + blockIdx.x];
+      base += digit_tot[d];
+      cnt[d] = 0;
+      for (int w = 0; w < kBlock / kWave; ++w) wcnt[w][d] = 0;
+    }
   }
   __syncthreads();
   int64_t n = (int64_t)*n_ptr;
@@ -1204,7 +1219,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
   int64_t begin, end;
   radix_chunk<IT>(n, begin, end);
   const uint64_t ltmask = lanemask_lt();
-  for (int64_t tb = begin; tb < end; tb += (IT * kBlock)) {
+  for (int64_t tb = begin; tb < end; tb += TILE) {
     K k[IT][NW];
     uint32_t v[IT], d[IT], off[IT];
 #pragma unroll
@@ -1217,11 +1232,12 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
 #pragma unroll
         for (int w = 0; w < NW; ++w) k[j][w] = kin[e * kstride + w];
         v[j] = iin ? iin[e] : (uint32_t)e;
-        d[j] = radix_digit<K, NW, FLIP>(k[j], word, shift);
+        d[j] = radix_digit<K, NW, FLIP, DB>(k[j], word, shift);
       }
+      // lanes of this wave with the same digit, by one ballot per digit bit
       uint64_t peers = __ballot(valid);
 #pragma unroll
-      for (int bit = 0; bit < 8; ++bit) {
+      for (int bit = 0; bit < DB; ++bit) {
         uint64_t m = __ballot(valid && ((d[j] >> bit) & 1u));
         peers &= ((d[j] >> bit) & 1u) ? m : ~m;
       }
@@ -1236,18 +1252,27 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
         off[j] = 0xFFFFFFFFu;
       }
       __syncthreads();
-      {
+#pragma unroll
+      for (int q = 0; q < DPT; ++q) {
+        const int dd = d0 + q;
         uint32_t sum = 0;
-        for (int q = 0; q < kBlock / kWave; ++q) { sum += wcnt[q][threadIdx.x]; wcnt[q][threadIdx.x] = 0; }
-        cnt[threadIdx.x] += sum;
+        for (int w = 0; w < kBlock / kWave; ++w) { sum += wcnt[w][dd]; wcnt[w][dd] = 0; }
+        cnt[dd] += sum;
       }
       __syncthreads();
     }
     // digit starts inside the tile
     {
+      int64_t loc = 0;
+#pragma unroll
+      for (int q = 0; q < DPT; ++q) loc += cnt[d0 + q];
       int64_t tot;
-      int64_t st = block_exclusive_scan((int64_t)cnt[threadIdx.x], red, tot);
-      tstart[threadIdx.x] = (uint32_t)st;
+      int64_t st = block_exclusive_scan(loc, red, tot);
+#pragma unroll
+      for (int q = 0; q < DPT; ++q) {
+        tstart[d0 + q] = (uint32_t)st;
+        st += cnt[d0 + q];
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -1260,20 +1285,23 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
       }
     }
     __syncthreads();
-    const int tn = (int)((end - tb) < (IT * kBlock) ? (end - tb) : (IT * kBlock));
+    const int tn = (int)((end - tb) < TILE ? (end - tb) : TILE);
     for (int i = threadIdx.x; i < tn; i += blockDim.x) {
       K kk[NW];
 #pragma unroll
       for (int w = 0; w < NW; ++w) kk[w] = lk[i * NW + w];
-      uint32_t dd = radix_digit<K, NW, FLIP>(kk, word, shift);
+      uint32_t dd = radix_digit<K, NW, FLIP, DB>(kk, word, shift);
       uint32_t gp = run[dd] + (uint32_t)i - tstart[dd];
 #pragma unroll
       for (int w = 0; w < NW; ++w) kout[(size_t)gp * NW + w] = kk[w];
       iout[gp] = lv[i];
     }
     __syncthreads();
-    run[threadIdx.x] += cnt[threadIdx.x];
-    cnt[threadIdx.x] = 0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+      run[d0 + q] += cnt[d0 + q];
+      cnt[d0 + q] = 0;
+    }
     __syncthreads();
   }
 }
