@@ -250,8 +250,10 @@ def run_batched(args):
     def with_core(e, d):
         dets, sg, _, _, _ = e.download()
         o = np.argsort(-np.abs(sg).sum(axis=1), kind="stable")[:ncore]
-        e.set_core(np.ascontiguousarray(dets[np.sort(o)]), fp32=True)
+        core = np.ascontiguousarray(dets[np.sort(o)])
+        e.set_core(core, fp32=False)
         d.run(10)
+        return core
 
     def timed(fn, drivers, k):
         w0 = sum(d.walker_iters for d in drivers)
@@ -262,16 +264,24 @@ def run_batched(args):
 
     K = args.steps
     engines = []
+    cores = []
     for r in range(args.mol_replicas):
         spec = Spec(mol, init_walkers=mol_walkers, seed=7000 + r, **mol_kw)
         e, d = grown(spec, tau_mol, mol_walkers, 1e3)
-        with_core(e, d)
+        cores.append(with_core(e, d))
         engines.append((e, d))
     for spec, tau, h in hub_specs:
         engines.append(grown(spec, tau, h["target"], 30.0))
     print(f"# grown: {len(engines)} engines", file=sys.stderr, flush=True)
     drivers = [d for _, d in engines]
+    # plain FP64: FP64 core Hamiltonian and spawned amplitudes
     p_rate, p_dt = timed(lambda k: [d.run(k) for d in drivers], drivers, K)
+    # batched FP32/FP64: core Hamiltonian and spawned amplitudes stored in FP32,
+    # sums, energies and the shift in FP64
+    for i, (e, _) in enumerate(engines):
+        if i < len(cores):
+            e.set_core(cores[i], fp32=True)
+        e.set_spawn_precision(True)
     batch = Batch([e for e, _ in engines])
     runner = BatchedFciMCPar(batch, drivers)
     runner.run(10)  # capture and first replays
@@ -283,7 +293,9 @@ def run_batched(args):
     line = {"metric": "walker-iterations/s on 1 GPU, batched vs plain FP64", "unit": "walker-iterations/s",
             "n_gpus": 1, "steps": K, "batched": b_rate, "plain": p_rate, "ratio": b_rate / p_rate,
             "ms_per_step": {"batched": b_dt * 1e3 / K, "plain": p_dt * 1e3 / K},
-            "dtype": "f64 walkers, f32-stored core Hamiltonian", "data": "synthetic",
+            "dtype": {"plain": "f64 walkers, core Hamiltonian and spawned amplitudes",
+                      "batched": "f64 walkers and sums; f32-stored core Hamiltonian and spawned amplitudes"},
+            "data": "synthetic",
             "config": {"workload": f"{args.mol_replicas} replicas of molecular 14e/30o (seed 7, rank 60, "
                                    f"{ncore}-det core, one engine each) + {len(hub_specs)} Hubbard chains "
                                    f"of 12-18 sites",

@@ -153,6 +153,16 @@ int ngpu_set_excitgen(ngpu_t *h, int32_t singles, int32_t particles, double psin
    probability 1 - psingles.  Replaces ngpu_set_excitgen's binding. */
 int ngpu_set_excitgen_uniform(ngpu_t *h, double psingles);
 
+/* Storage precision of the spawned amplitudes.  create_particle
+   (fcimc_helper.F90:197) writes each child's sign into SpawnedParts as FP64
+   (encode_sign, BitReps.F90:454); fp32 = 1 stores it as FP32 instead (the
+   value attempt_create returns is rounded to nearest, so ngpu_attempt_create_batch
+   returns it rounded too), and the merge and annihilation still sum in FP64.
+   fp32 = 0 (the default) is the reference's FP64.  With FP32, a spawned record
+   takes W + 1 words instead of 4 (16 bytes for one-word determinants), and
+   ngpu_annihilate rounds its input amplitudes to FP32.  Not while batched. */
+int ngpu_set_spawn_precision(ngpu_t *h, int32_t fp32);
+
 /* RandomHash2(1:nbasis) (fcimc_initialisation.fpp:890-915) for FindWalkerHash. */
 int ngpu_set_hash(ngpu_t *h, const int32_t *random_hash2);
 

@@ -98,6 +98,7 @@ _SIGS = [
     ("ngpu_set_lattice", ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                         ctypes.c_double]),
     ("ngpu_set_excitgen_uniform", ctypes.c_int, [_vp, ctypes.c_double]),
+    ("ngpu_set_spawn_precision", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("ngpu_set_excitgen", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                          ctypes.c_double]),
     ("ngpu_set_hash", ctypes.c_int, [_vp, _vp]),

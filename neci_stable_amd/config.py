@@ -42,6 +42,7 @@ class Spec:
     ref_orbs: Optional[list] = None
     random_hash: Optional[np.ndarray] = None
     flag_layout: int = 0
+    spawn_fp32: bool = False  # spawned amplitudes stored in FP32 (sums stay FP64)
     model: int = field(init=False)
     nel: int = field(init=False)
     nbasis: int = field(init=False)

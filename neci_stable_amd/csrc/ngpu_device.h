@@ -257,6 +257,10 @@ struct DevSys {
   // C(p, i) at binom[p*nel1 + i] when spawns are sorted by colex rank, else null
   const uint32_t* binom;
   int nel1;
+  // spawned-list records: FP64 amplitudes in 4 words (key, key or 0, value,
+  // info), or FP32 amplitudes in W + 1 words (key words, then info << 32 |
+  // FP32 bits); sums over spawns stay FP64 either way
+  int spawn_fp32, rec_words;
 };
 
 __device__ __host__ inline double U(const DevSys& s, int i, int j, int k, int l) {
@@ -888,6 +892,8 @@ __device__ __host__ inline double attempt_create(const DevSys& s, double tau, do
   } else {
     nspawn = stochastic_round(nspawn, r);
   }
+  // FP32 spawned amplitudes: the stored value, rounded to nearest
+  if (s.spawn_fp32) nspawn = (double)(float)nspawn;
   return nspawn;
 }
 

@@ -304,7 +304,7 @@ int sort_ranks(ngpu_t* h, bool ranked) {
            (const uint32_t*)h->rk, 1, n_ptr, h->scap, 0, 0, h->hist);
   else
     launch(h, 4, -1, k_rank_hist<W, IT, DB>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec, n_ptr,
-           h->scap, (const uint32_t*)h->d_binom, h->sys.nel + 1, h->rk, h->hist);
+           h->scap, h->ds.rec_words, (const uint32_t*)h->d_binom, h->sys.nel + 1, h->rk, h->hist);
   const int passes = (h->rank_bits + DB - 1) / DB;
   const uint32_t* kin = h->rk;
   const uint32_t* iin = nullptr;
@@ -351,7 +351,7 @@ void sort_and_merge_it(ngpu_t* h, bool ranked, bool list_out) {
   for (int sh = 0; sh < nbits_hi; sh += 8) passes.push_back({0, sh});
   int cur = 0;
   const uint64_t* kin = h->sp_rec;
-  int kstride = kRecWords;  // the first pass reads the keys inside the records
+  int kstride = h->ds.rec_words;  // the first pass reads the keys inside the records
   const uint32_t* iin = nullptr;
   for (size_t p = 0; p < passes.size(); ++p) {
     int out = (int)(p & 1);
@@ -793,6 +793,8 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
   h->ds.seed = sys->seed;
   h->ds.L = sys->n_hash_buckets;
   h->ds.pdoubles = 1.0;
+  h->ds.spawn_fp32 = 0;
+  h->ds.rec_words = kRecWords;
   if (rc == NGPU_OK) {
     h->ev_ok = true;
     for (int q = 0; q < kEvRing; ++q)
@@ -1013,6 +1015,15 @@ int ngpu_set_excitgen_uniform(ngpu_t* h, double psingles) {
   h->ds.psingles = psingles;
   h->ds.pdoubles = 1.0 - psingles;
   h->have_excitgen = true;
+  return NGPU_OK;
+}
+
+int ngpu_set_spawn_precision(ngpu_t* h, int32_t fp32) {
+  if (!h || (fp32 != 0 && fp32 != 1)) return NGPU_ERR_ARG;
+  if (h->batch) return NGPU_ERR_STATE;  // the batch's graph holds the record format
+  h->graph_dirty = true;  // kernel arguments of the captured chain change
+  h->ds.spawn_fp32 = fp32;
+  h->ds.rec_words = fp32 ? h->W + 1 : kRecWords;
   return NGPU_OK;
 }
 
@@ -1694,13 +1705,22 @@ int ngpu_annihilate(ngpu_t* h, uint32_t iter, const int64_t* sdet, const double*
     HIP_OK(hipMalloc((void**)&h->u_count, 8));
   }
   if (n) {
-    std::vector<uint64_t> rec((size_t)n * kRecWords, 0);
+    const int rw = h->ds.rec_words;
+    std::vector<uint64_t> rec((size_t)n * rw, 0);
     for (int64_t i = 0; i < n; ++i) {
-      for (int w = 0; w < W; ++w) rec[(size_t)i * kRecWords + w] = (uint64_t)sdet[i * W + w];
-      uint64_t vb;
-      memcpy(&vb, &sval[i], 8);
-      rec[(size_t)i * kRecWords + 2] = vb;
-      rec[(size_t)i * kRecWords + 3] = sinfo[i];
+      uint64_t* r = rec.data() + (size_t)i * rw;
+      for (int w = 0; w < W; ++w) r[w] = (uint64_t)sdet[i * W + w];
+      if (h->ds.spawn_fp32) {  // FP32 amplitudes: each value rounded to nearest on entry
+        float f = (float)sval[i];
+        uint32_t fb;
+        memcpy(&fb, &f, 4);
+        r[W] = ((uint64_t)sinfo[i] << 32) | fb;
+      } else {
+        uint64_t vb;
+        memcpy(&vb, &sval[i], 8);
+        r[2] = vb;
+        r[3] = sinfo[i];
+      }
     }
     HIP_OK(hipMemcpy(h->sp_rec, rec.data(), rec.size() * 8, hipMemcpyHostToDevice));
   }

@@ -575,13 +575,22 @@ struct QueueStage {  // queued attempts as (parent pair, attempt number)
   uint32_t k[kQueueStage];
 };
 
-// A spawned-list record is kRecWords 64-bit words (32 bytes, so a record
-// never straddles a cache line): the determinant's words, the child amplitude
-// (FP64 bits) in word 2 and the info word (bits 0-7 replica, bit 8 parent
-// initiator) in word 3.  The merge reads one record per spawn.
-constexpr int kRecWords = 4;
-__device__ inline double rec_val(const uint64_t* r) { return __longlong_as_double((long long)r[2]); }
-__device__ inline uint32_t rec_info(const uint64_t* r) { return (uint32_t)r[3]; }
+// A spawned-list record with FP64 amplitudes is kRecWords 64-bit words (32
+// bytes, so a record never straddles a cache line): the determinant's words,
+// the child amplitude (FP64 bits) in word 2 and the info word (bits 0-7
+// replica, bit 8 parent initiator) in word 3.  With FP32 amplitudes it is
+// W + 1 words: the determinant's words, then info << 32 | the amplitude's FP32
+// bits (16 bytes for one-word determinants).  The merge reads one record per
+// spawn and sums the amplitudes in FP64.
+constexpr int kRecWords = 4;  // the larger of the two record sizes
+template <int W>
+__device__ inline double rec_val(const DevSys& s, const uint64_t* r) {
+  return s.spawn_fp32 ? (double)__uint_as_float((uint32_t)r[W]) : __longlong_as_double((long long)r[2]);
+}
+template <int W>
+__device__ inline uint32_t rec_info(const DevSys& s, const uint64_t* r) {
+  return s.spawn_fp32 ? (uint32_t)(r[W] >> 32) : (uint32_t)r[3];
+}
 
 // colex rank of a determinant: sum over its i-th occupied orbital (bit p,
 // ascending) of C(p, i); injective on determinants of nel electrons
@@ -625,12 +634,16 @@ __device__ inline void spawn_flush(SpawnStage<W>* ss, Counters* cnt, uint64_t* s
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
     unsigned long long d = base + i;
     if (d < (unsigned long long)scap) {
-      uint64_t* r = sp_rec + d * kRecWords;
+      uint64_t* r = sp_rec + d * (unsigned long long)s.rec_words;
 #pragma unroll
       for (int w = 0; w < W; ++w) r[w] = ss->key[i * W + w];
-      if (W == 1) r[1] = 0;
-      r[2] = (uint64_t)__double_as_longlong(ss->val[i]);
-      r[3] = ss->info[i];
+      if (s.spawn_fp32) {  // the value is already FP32-representable (attempt_create)
+        r[W] = ((uint64_t)ss->info[i] << 32) | __float_as_uint((float)ss->val[i]);
+      } else {
+        if (W == 1) r[1] = 0;
+        r[2] = (uint64_t)__double_as_longlong(ss->val[i]);
+        r[3] = ss->info[i];
+      }
       if (rk) rk[d] = colex_rank<W>(bl.c, s.nel1, ss->key + i * W);
     } else {
       overflow = true;
@@ -1137,8 +1150,8 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, int kstrid
 // C(p, i)), fused with the histogram of its lowest digit
 template <int W, int IT, int DB>
 __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, const unsigned long long* n_ptr,
-                                                      int64_t ncap, const uint32_t* binom, int nel1,
-                                                      uint32_t* rank, uint32_t* hist) {
+                                                      int64_t ncap, int rstride, const uint32_t* binom,
+                                                      int nel1, uint32_t* rank, uint32_t* hist) {
   constexpr int RB = 1 << DB;
   __shared__ uint32_t h[RB];
   for (int i = threadIdx.x; i < RB; i += blockDim.x) h[i] = 0;
@@ -1148,7 +1161,7 @@ __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, cons
   int64_t begin, end;
   radix_chunk<IT>(n, begin, end);
   for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
-    uint32_t r = colex_rank<W>(binom, nel1, keys + i * kRecWords);
+    uint32_t r = colex_rank<W>(binom, nel1, keys + i * rstride);
     rank[i] = r;
     atomicAdd(&h[r & (RB - 1u)], 1u);
   }
@@ -1385,15 +1398,15 @@ __global__ void __launch_bounds__(kBlock) k_merge_list(DevSys s, Counters* cnt, 
       if (head) {
         int64_t j = i + 1;
         while (j < n && key_eq<K, NW>(ki, skey + j * NW)) ++j;
-        const uint64_t* kd = sp_rec + (size_t)sidx[i] * kRecWords;
+        const uint64_t* kd = sp_rec + (size_t)sidx[i] * s.rec_words;
 #pragma unroll
         for (int w = 0; w < W; ++w) key[w] = kd[w];
         for (int r = 0; r < S; ++r) sum[r] = 0.0;
         for (int64_t q = i; q < j; ++q) {
-          const uint64_t* rq = sp_rec + (size_t)sidx[q] * kRecWords;
-          uint32_t inf = rec_info(rq);
+          const uint64_t* rq = sp_rec + (size_t)sidx[q] * s.rec_words;
+          uint32_t inf = rec_info<W>(s, rq);
           int r = (int)(inf & 0xFFu);
-          double nv = rec_val(rq);
+          double nv = rec_val<W>(s, rq);
           double cv = sum[r];
           if (j == i + 1) {
             if (inf & kInfoInit) flag |= 1u << r;
@@ -1403,7 +1416,7 @@ __global__ void __launch_bounds__(kBlock) k_merge_list(DevSys s, Counters* cnt, 
           sum[r] = cv + nv;
         }
         if (j == i + 1) {
-          keep = !(fabs(sum[rec_info(kd) & 0xFFu]) < 1.e-12);
+          keep = !(fabs(sum[rec_info<W>(s, kd) & 0xFFu]) < 1.e-12);
         } else {
           double tot = 0.0;
           for (int r = 0; r < S; ++r) tot += fabs(sum[r]);
@@ -1458,22 +1471,22 @@ __global__ void __launch_bounds__(kBlock) k_merge_annihilate(DevSys s, Walk wk, 
       if (head) {
         int64_t j = i + 1;
         while (j < n && key_eq<K, NW>(ki, skey + j * NW)) ++j;
-        const uint64_t* kd = sp_rec + (size_t)sidx[i] * kRecWords;
+        const uint64_t* kd = sp_rec + (size_t)sidx[i] * s.rec_words;
 #pragma unroll
         for (int w = 0; w < W; ++w) key[w] = kd[w];
         for (int r = 0; r < S; ++r) ss[r] = 0.0;
         if (j == i + 1) {
-          uint32_t inf = rec_info(kd);
+          uint32_t inf = rec_info<W>(s, kd);
           int r = (int)(inf & 0xFFu);
-          ss[r] = rec_val(kd);
+          ss[r] = rec_val<W>(s, kd);
           if (inf & kInfoInit) sflag |= 1u << r;
           keep = !(fabs(ss[r]) < 1.e-12);
         } else {
           for (int64_t q = i; q < j; ++q) {
-            const uint64_t* rq = sp_rec + (size_t)sidx[q] * kRecWords;
-            uint32_t inf = rec_info(rq);
+            const uint64_t* rq = sp_rec + (size_t)sidx[q] * s.rec_words;
+            uint32_t inf = rec_info<W>(s, rq);
             int r = (int)(inf & 0xFFu);
-            double nv = rec_val(rq);
+            double nv = rec_val<W>(s, rq);
             double cv = ss[r];
             if (s.t_init) {
               if ((fabs(cv) > 1.e-12 && fabs(nv) > 1.e-12) || (inf & kInfoInit)) sflag |= 1u << r;

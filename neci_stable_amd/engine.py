@@ -52,6 +52,8 @@ class Engine:
             else:
                 check(self.lib.ngpu_set_excitgen(h, spec.singles, spec.particles, spec.psingles, spec.pparallel),
                       "ngpu_set_excitgen")
+        if spec.spawn_fp32:
+            check(self.lib.ngpu_set_spawn_precision(h, 1), "ngpu_set_spawn_precision")
         self.random_hash = spec.random_hash
         check(self.lib.ngpu_set_hash(h, ptr(self.random_hash)), "ngpu_set_hash")
         self.ref_orbs = spec.ref_orbs
@@ -153,6 +155,11 @@ class Engine:
               "ngpu_set_core")
         self.ncore = dets.shape[0]
         return nnz.value
+
+    def set_spawn_precision(self, fp32):
+        """Spawned amplitudes stored in FP32 (True) or FP64 (ngpu_set_spawn_precision)."""
+        check(self.lib.ngpu_set_spawn_precision(self.h, int(bool(fp32))), "ngpu_set_spawn_precision")
+        self.spec.spawn_fp32 = bool(fp32)
 
     def set_launch_shape(self, chunk_blocks=0, grid_blocks=0, radix_items=0):
         check(self.lib.ngpu_set_launch_shape(self.h, int(chunk_blocks), int(grid_blocks), int(radix_items)),

@@ -1703,6 +1703,9 @@ int32_t o_iterate(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_t 
                 } else {
                     nspawn = stochastic_round(nspawn, d[5]);
                 }
+                /* FP32 storage of the spawned amplitude: C's double -> float conversion
+                   rounds to nearest even, as the store does */
+                if (s->spawn_fp32) nspawn = (double)(float)nspawn;
                 if (fabs(nspawn) <= EPS) continue;
                 /* new_child_stats_normal (fcimc_pointed_fns.F90:506-571) */
                 out->nborn[r] += fabs(nspawn);

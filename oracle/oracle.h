@@ -57,7 +57,8 @@ typedef struct {
     double ecore;
     /* run parameters */
     double tau, avmc, init_walk_no, real_spawn_cutoff, occ_thresh;
-    int32_t t_trunc_init, t_real_coeff, t_real_spawn_cutoff, pad0;
+    int32_t t_trunc_init, t_real_coeff, t_real_spawn_cutoff;
+    int32_t spawn_fp32;                 /* 1: spawned amplitudes stored in FP32 (rounded to nearest) */
     double hii;                         /* reference diagonal element (total energy) */
     uint64_t ref[O_MAX_W];              /* reference determinant */
     /* excitation generator */

@@ -23,7 +23,7 @@ class OSys(ctypes.Structure):
         ("init_walk_no", ctypes.c_double), ("real_spawn_cutoff", ctypes.c_double),
         ("occ_thresh", ctypes.c_double), ("t_trunc_init", ctypes.c_int32),
         ("t_real_coeff", ctypes.c_int32), ("t_real_spawn_cutoff", ctypes.c_int32),
-        ("pad0", ctypes.c_int32), ("hii", ctypes.c_double), ("ref", ctypes.c_uint64 * 2),
+        ("spawn_fp32", ctypes.c_int32), ("hii", ctypes.c_double), ("ref", ctypes.c_uint64 * 2),
         ("singles", ctypes.c_int32), ("particles", ctypes.c_int32), ("psingles", ctypes.c_double),
         ("pparallel", ctypes.c_double), ("pchb_ready", ctypes.c_int32), ("uniform", ctypes.c_int32),
         ("random_hash2", ctypes.c_void_p), ("nbuckets", ctypes.c_int64), ("seed", ctypes.c_uint32),
@@ -148,6 +148,7 @@ class Oracle:
         o.t_trunc_init = int(spec.trunc_initiator)
         o.t_real_coeff = int(spec.all_real_coeff)
         o.t_real_spawn_cutoff = int(spec.t_real_spawn_cutoff)
+        o.spawn_fp32 = int(getattr(spec, "spawn_fp32", False))
         W = spec.W
         ref = np.zeros(W, dtype=np.uint64)
         for orb in spec.ref_orbs:

@@ -239,6 +239,40 @@ def test_annihilate_explicit_list(name):
     assert_same_walkers(eng, orc)
 
 
+@pytest.mark.parametrize("name", ["hubbard10", "mol33o4e_w2", "mol33o8e_w2"])
+def test_annihilate_explicit_list_fp32(name):
+    """An explicit list with real amplitudes under FP32 spawn storage: the
+    engine rounds each input amplitude to FP32, so the oracle, given the same
+    values rounded by numpy, merges and annihilates to identical results (the
+    list's order fixes the order of each merge sum)."""
+    spec = SPECS[name](init_walkers=3000, all_real_coeff=True, spawn_fp32=True)
+    eng = engine(spec)
+    orc = oracle(spec, 0.01)
+    rng = np.random.default_rng(23)
+    dets = np.unique(random_dets(spec, 300, rng), axis=0)
+    sg = np.round(rng.normal(0, 4, (len(dets), 1)), 3)
+    sg[sg == 0] = 1.0
+    fl = (np.abs(sg[:, 0]) > 3).astype(np.uint32)
+    eng.upload(eng.rows(dets, sg, internal_to_neci_flags(fl, 1)))
+    orc.add_walkers(dets, sg, fl)
+    pool = np.concatenate([dets[:150], random_dets(spec, 150, rng)])
+    sdet = pool[rng.integers(0, len(pool), 2500)]
+    sval = rng.normal(0, 1.5, 2500) + 0.1
+    sinfo = (rng.random(2500) < 0.5).astype(np.uint32) << 8
+    a, md, ms, mf = eng.annihilate(7, sdet, sval, sinfo)
+    b, md2, ms2, mf2 = orc.annihilate(7, sdet, sval.astype(np.float32).astype(np.float64), sinfo)
+    assert b["rc"] == 0
+    m1 = sort_walkers(md, ms, mf)
+    m2 = sort_walkers(md2, ms2, mf2)
+    for x, y in zip(m1, m2):
+        np.testing.assert_array_equal(x, y)
+    assert_same_walkers(eng, orc)
+    # and the FP64 list differs: the rounding took place
+    b64, md3, ms3, mf3 = oracle(SPECS[name](init_walkers=3000, all_real_coeff=True), 0.01).annihilate(
+        7, sdet, sval, sinfo)
+    assert not np.array_equal(sort_walkers(md3, ms3, mf3)[1], m2[1])
+
+
 @pytest.mark.parametrize("name,nrep,real", [("hubbard10", 1, False), ("hubbard10", 2, False),
                                             ("mol8o6e_unif_fast", 1, False),
                                             ("mol8o6e_unif_unif", 1, False),
@@ -249,9 +283,23 @@ def test_annihilate_explicit_list(name):
 def test_iterations_match_oracle(name, nrep, real):
     """Full iterations (spawn, merge, death, annihilation, stats) from the same
     walker set: identical walker sets after every iteration."""
+    run_iterations(name, nrep, real)
+
+
+@pytest.mark.parametrize("name,nrep", [("hubbard10", 1), ("hubbard10", 2), ("mol8o6e_unif_fast", 1),
+                                       ("mol33o4e_w2", 1), ("mol8o6e_uniform", 1), ("hubbard18", 1)])
+def test_iterations_fp32_spawns_match_oracle(name, nrep):
+    """Spawned amplitudes stored in FP32 with real coefficients: the engine and
+    the oracle round each spawn to FP32 at the same point, so walker signs and
+    statistics agree to 1e-10 (an FP64 spawn on one side would differ by
+    ~1e-8 relative)."""
+    run_iterations(name, nrep, True, spawn_fp32=True)
+
+
+def run_iterations(name, nrep, real, spawn_fp32=False):
     # the 8-electron 2-word molecule grows ~1.4x per iteration at tau 0.02
     tau = 0.01 if name.startswith("hub") else (0.005 if name == "mol33o8e_w2" else 0.02)
-    spec = SPECS[name](init_walkers=4000, n_replicas=nrep, all_real_coeff=real, seed=77)
+    spec = SPECS[name](init_walkers=4000, n_replicas=nrep, all_real_coeff=real, seed=77, spawn_fp32=spawn_fp32)
     eng = engine(spec)
     orc = oracle(spec, tau)
     n0 = 30.0 if name.startswith("hub") else 400.0

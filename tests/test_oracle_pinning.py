@@ -379,6 +379,32 @@ def test_merge_and_annihilation_against_groupby(nrep):
     np.testing.assert_array_equal(st["naborted"][:nrep], aborted)
 
 
+def test_fp32_spawn_storage_rounds_each_spawn():
+    """FP32 spawned amplitudes (real coefficients): one iteration from the
+    same start with FP64 and with FP32 spawn storage makes the same spawning
+    decisions (the rounding comes after them), so both runs hold the same
+    determinants; each amplitude then differs by FP32 rounding only, some do
+    differ, and NoBorn (the sum of |spawn|) differs by at most half an FP32
+    ulp per spawn."""
+    res = {}
+    for fp32 in (False, True):
+        spec = mol_spec(8, 6, init_walkers=4000, all_real_coeff=True, seed=19, spawn_fp32=fp32)
+        o = oc.Oracle(spec, 0.02)
+        o.add_walkers(o.ref[None, :], np.full((1, 1), 3000.0), [1])
+        st = o.iterate(1, 0.3)
+        assert st["rc"] == 0
+        d, sg, _, _, _ = o.walkers()
+        order = np.lexsort(d.T[::-1])
+        res[fp32] = (st, d[order], sg[order, 0])
+    (a64, d64, s64), (a32, d32, s32) = res[False], res[True]
+    np.testing.assert_array_equal(d64, d32)
+    assert a64["n_spawned"] == a32["n_spawned"] > 100
+    assert np.all(np.abs(s32 - s64) <= 1e-6 * (np.abs(s64) + 1.0))
+    assert np.count_nonzero(s32 != s64) > 10
+    nb64, nb32 = a64["nborn"][0], a32["nborn"][0]
+    assert nb32 != nb64 and abs(nb32 - nb64) <= 2.0 ** -24 * nb64
+
+
 # ------------------------------------------------ semistochastic core space
 @pytest.mark.parametrize("fp32", [False, True])
 def test_core_hamiltonian_and_projection_against_dense(fp32):
