@@ -189,7 +189,10 @@ int ngpu_download_walkers(ngpu_t *h, int64_t *rows, double *diag_h, double *offd
    the number of stored elements.  Every later iteration projects the core
    amplitudes deterministically (determ_projection, semi_stoch_procs.F90:105-241;
    deterministic_annihilation, Annihilation.F90:930-963) and cancels spawns from
-   core to core dets (FciMCPar.F90:1659-1668).  One core space per engine. */
+   core to core dets (FciMCPar.F90:1659-1668).  One core space per engine: a
+   second call with the same determinants (same order) only stores the core
+   Hamiltonian again in the precision fp32_values asks for; any other second
+   call returns NGPU_ERR_STATE. */
 int ngpu_set_core(ngpu_t *h, const int64_t *dets, int64_t ncore, int32_t fp32_values, int64_t *nnz);
 
 /* One FCIQMC iteration (PerformFCIMCycPar, FciMCPar.F90:1177-1920).  Runs

@@ -1139,6 +1139,36 @@ int set_core_impl(ngpu_t* h, int64_t ncore, bool fp32, int64_t* nnz_out) {
   if (nnz_out) *nnz_out = nnz;
   return NGPU_OK;
 }
+
+// the resident core's Hamiltonian values stored again in the other precision
+// (same rows and columns, each value recomputed from the integrals)
+template <int W>
+int restore_core_values(ngpu_t* h, bool fp32) {
+  const int64_t ncore = h->ncore;
+  dim3 grid((unsigned)std::min<int64_t>((ncore + 3) / 4, 8 * (int64_t)h->nb_grid));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  HIP_OK(hipFree(h->core_vals));
+  h->core_vals = nullptr;
+  if (fp32) {
+    float* v = nullptr;
+    HIP_OK(dalloc(&v, (size_t)h->core_nnz));
+    h->core_vals = v;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_core_rows<W, float>), grid, dim3(kBlock), 0, h->stream, h->ds,
+                       (const uint64_t*)h->core_keys, ncore, 1, (int64_t*)nullptr, (const int64_t*)h->core_rowptr,
+                       h->core_cols, v);
+  } else {
+    double* v = nullptr;
+    HIP_OK(dalloc(&v, (size_t)h->core_nnz));
+    h->core_vals = v;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_core_rows<W, double>), grid, dim3(kBlock), 0, h->stream, h->ds,
+                       (const uint64_t*)h->core_keys, ncore, 1, (int64_t*)nullptr, (const int64_t*)h->core_rowptr,
+                       h->core_cols, v);
+  }
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(h->stream));
+  h->core_fp32 = fp32;
+  return NGPU_OK;
+}
 }  // namespace
 }  // extern "C++"
 
@@ -1147,7 +1177,24 @@ int ngpu_set_core(ngpu_t* h, const int64_t* dets, int64_t ncore, int32_t fp32_va
   int rc = ready(h);
   if (rc) return rc;
   if (!dets || ncore < 1 || ncore > (int64_t)INT32_MAX || ncore > h->wk.cap) return NGPU_ERR_ARG;
-  if (h->ncore) return NGPU_ERR_STATE;  // one core space per engine
+  if (h->ncore) {
+    // one core space per engine; called again with the same determinants it
+    // changes the storage precision of the core Hamiltonian
+    if (ncore != h->ncore || h->batch) return NGPU_ERR_STATE;
+    if (h->pending) {
+      rc = finish(h, nullptr);
+      if (rc) return rc;
+    }
+    std::vector<int64_t> cur((size_t)ncore * h->W);
+    HIP_OK(hipMemcpy(cur.data(), h->core_keys, cur.size() * 8, hipMemcpyDeviceToHost));
+    if (memcmp(cur.data(), dets, cur.size() * 8) != 0) return NGPU_ERR_STATE;
+    if ((fp32_values != 0) != h->core_fp32) {
+      rc = h->W == 1 ? restore_core_values<1>(h, fp32_values != 0) : restore_core_values<2>(h, fp32_values != 0);
+      if (rc) return rc;
+    }
+    if (nnz) *nnz = h->core_nnz;
+    return NGPU_OK;
+  }
   for (int64_t i = 0; i < ncore; ++i) {
     int nel = 0;
     for (int w = 0; w < h->W; ++w) nel += __builtin_popcountll((uint64_t)dets[i * h->W + w]);

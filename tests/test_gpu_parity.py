@@ -358,15 +358,21 @@ def test_hash_full_is_an_error():
     assert e.value.code == 6
 
 
-@pytest.mark.parametrize("name,nrep,fp32", [("hubbard10", 1, False), ("hubbard10", 2, False),
-                                             ("mol8o6e_unif_fast", 1, False), ("mol8o6e_unif_fast", 1, True),
-                                             ("mol33o4e_w2", 1, False), ("mol8o6e_uniform", 1, False)])
-def test_semistochastic_iterations_match_oracle(name, nrep, fp32):
+@pytest.mark.parametrize("name,nrep,fp32,switch", [("hubbard10", 1, False, False), ("hubbard10", 2, False, False),
+                                                    ("mol8o6e_unif_fast", 1, False, False),
+                                                    ("mol8o6e_unif_fast", 1, True, False),
+                                                    ("mol8o6e_unif_fast", 1, True, True),
+                                                    ("mol33o4e_w2", 1, False, False),
+                                                    ("mol33o4e_w2", 1, False, True),
+                                                    ("mol8o6e_uniform", 1, False, False)])
+def test_semistochastic_iterations_match_oracle(name, nrep, fp32, switch):
     """Iterations with a semistochastic core space (deterministic projection of
     the core amplitudes, core-to-core spawns cancelled): identical walker sets
     and signs after every iteration.  The core projection sums in the same
     fixed order on both sides, so amplitudes agree bit for bit; statistics that
-    sum real amplitudes in a different order agree to 1e-10."""
+    sum real amplitudes in a different order agree to 1e-10.  switch: the core
+    is first set in the other precision, then set again with the same
+    determinants (only the stored values change); a different core is refused."""
     tau = 0.01 if name.startswith("hub") else 0.02
     spec = SPECS[name](init_walkers=4000, n_replicas=nrep, seed=31)
     eng = engine(spec)
@@ -379,6 +385,12 @@ def test_semistochastic_iterations_match_oracle(name, nrep, fp32):
         core = hopping_neighbourhood(spec, orc.ref, 2)
     else:
         core = np.unique(np.concatenate([orc.ref[None, :], random_dets(spec, 150, rng)]), axis=0)
+    if switch:
+        from neci_stable_amd._lib import NgpuError
+        eng.set_core(core, fp32=not fp32)
+        with pytest.raises(NgpuError) as e:
+            eng.set_core(core[1:], fp32=fp32)
+        assert e.value.code == 7
     nnz = eng.set_core(core, fp32=fp32)
     assert nnz == orc.set_core(core, fp32=fp32) and nnz > 2 * len(core)
     shift = np.linspace(0.2, 0.4, nrep)
