@@ -2,11 +2,12 @@
 
 The engine's kernels split their work over a number of blocks fixed at launch
 and the radix sort ranks tiles of a fixed number of elements per thread
-(ngpu_set_launch_shape).  The best values depend on the walker count and the
-system, so they are measured on the running calculation: one knob at a time
-(radix tile, then the chunked kernels' blocks, then the grid-stride kernels'
-blocks), each candidate timed over a few iterations of the driver, the
-fastest kept.  The walker dynamics do not depend on the shape, so tuning
+(ngpu_set_launch_shape); the heat-bath singles run 4, 8 or 16 lanes per
+attempt.  The best values depend on the walker count and the system, so they
+are measured on the running calculation: one knob at a time (radix tile, the
+chunked kernels' blocks, the grid-stride kernels' blocks, then the lanes per
+single attempt), each candidate timed over a few iterations of the driver,
+the fastest kept.  The walker dynamics do not depend on the shape, so tuning
 while the calculation runs changes nothing but its speed.
 """
 import time
@@ -29,8 +30,9 @@ def autotune(driver, steps=20, warm=5, repeats=2, log=None):
         ("radix_items", [2, 4, 8]),
         ("chunk_blocks", [2 * cu, 4 * cu, 8 * cu]),
         ("grid_blocks", [4 * cu, 8 * cu, 16 * cu]),
+        ("single_lanes", [4, 8, 16]),
     ]
-    best = {k: base[k] for k in ("chunk_blocks", "grid_blocks", "radix_items")}
+    best = {k: base[k] for k in ("chunk_blocks", "grid_blocks", "radix_items", "single_lanes")}
     trials = []
     for name, values in knobs:
         scores = {}

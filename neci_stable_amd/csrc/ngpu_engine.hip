@@ -47,6 +47,7 @@ struct ngpu_engine {
   int nb_chunk = 1024;  // blocks of the chunked kernels (prepass, scans, radix)
   int nb_grid = 2048;   // blocks of the grid-stride kernels
   int radix_items = 4;  // elements per thread of a radix tile (2, 4 or 8)
+  int single_lanes = 4;  // lanes per heat-bath single attempt (4, 8 or 16)
   int n_cu = 256, max_chunk = 4096;
   hipStream_t stream = nullptr;
   DevSys ds{};
@@ -438,10 +439,15 @@ void iterate_chain(ngpu_t* h) {
          h->rank_sort ? h->rk : nullptr, h->scap, h->sq, (const IterParams*)h->dparams);
   if (h->sys.model == NGPU_MODEL_MOLECULAR && !h->ds.uniform) {
     const int nsp = h->sys.nbasis / 2;
-    const size_t lds = sizeof(double) * (size_t)kHbGroups * nsp;
-    auto kern = nsp <= 4 * kHbGroup ? k_spawn_singles<W, R, 4>
-              : nsp <= 5 * kHbGroup ? k_spawn_singles<W, R, 5>
-              : nsp <= 8 * kHbGroup ? k_spawn_singles<W, R, 8> : k_spawn_singles<W, R, 16>;
+    // G lanes per attempt, PL >= nsp / G orbitals per lane
+    const int G = h->single_lanes;
+    const size_t lds = sizeof(double) * (size_t)(kBlock / G) * nsp;
+    auto kern = G == 16 ? (nsp <= 16 ? k_spawn_singles<W, R, 1, 16> : nsp <= 32 ? k_spawn_singles<W, R, 2, 16>
+                           : k_spawn_singles<W, R, 4, 16>)
+              : G == 8 ? (nsp <= 16 ? k_spawn_singles<W, R, 2, 8> : nsp <= 24 ? k_spawn_singles<W, R, 3, 8>
+                          : nsp <= 32 ? k_spawn_singles<W, R, 4, 8> : k_spawn_singles<W, R, 8, 8>)
+              : nsp <= 16 ? k_spawn_singles<W, R, 4, 4> : nsp <= 20 ? k_spawn_singles<W, R, 5, 4>
+              : nsp <= 32 ? k_spawn_singles<W, R, 8, 4> : k_spawn_singles<W, R, 16, 4>;
     launch_lds(h, 3, 3, kern, dim3(h->nb_grid), dim3(kBlock), lds, h->ds, h->wk, h->cnt, h->dpart,
                (const uint8_t*)h->pinfo, h->sp_rec, h->rank_sort ? h->rk : nullptr, h->scap, (const int64_t*)h->sq,
                (const IterParams*)h->dparams);
@@ -1356,11 +1362,13 @@ int ngpu_iterate(ngpu_t* h, const ngpu_iter_in_t* in, ngpu_iter_out_t* out) {
   return NGPU_OK;
 }
 
-int ngpu_set_launch_shape(ngpu_t* h, int32_t chunk_blocks, int32_t grid_blocks, int32_t radix_items) {
+int ngpu_set_launch_shape(ngpu_t* h, int32_t chunk_blocks, int32_t grid_blocks, int32_t radix_items,
+                          int32_t single_lanes) {
   if (!h) return NGPU_ERR_ARG;
   if (chunk_blocks < 0 || chunk_blocks > h->max_chunk || (chunk_blocks && chunk_blocks < 64) || grid_blocks < 0 ||
       (grid_blocks && grid_blocks < 64) || grid_blocks > 64 * h->n_cu ||
-      !(radix_items == 0 || radix_items == 2 || radix_items == 4 || radix_items == 8))
+      !(radix_items == 0 || radix_items == 2 || radix_items == 4 || radix_items == 8) ||
+      !(single_lanes == 0 || single_lanes == 4 || single_lanes == 8 || single_lanes == 16))
     return NGPU_ERR_ARG;
   if (h->pending) {
     int rc = finish(h, nullptr);
@@ -1369,16 +1377,18 @@ int ngpu_set_launch_shape(ngpu_t* h, int32_t chunk_blocks, int32_t grid_blocks, 
   if (chunk_blocks) h->nb_chunk = chunk_blocks;
   if (grid_blocks) h->nb_grid = grid_blocks;
   if (radix_items) h->radix_items = radix_items;
+  if (single_lanes) h->single_lanes = single_lanes;
   h->graph_dirty = true;
   return NGPU_OK;
 }
 
 int ngpu_get_launch_shape(ngpu_t* h, int32_t* chunk_blocks, int32_t* grid_blocks, int32_t* radix_items,
-                          int32_t* n_cu) {
+                          int32_t* single_lanes, int32_t* n_cu) {
   if (!h) return NGPU_ERR_ARG;
   if (chunk_blocks) *chunk_blocks = h->nb_chunk;
   if (grid_blocks) *grid_blocks = h->nb_grid;
   if (radix_items) *radix_items = h->radix_items;
+  if (single_lanes) *single_lanes = h->single_lanes;
   if (n_cu) *n_cu = h->n_cu;
   return NGPU_OK;
 }

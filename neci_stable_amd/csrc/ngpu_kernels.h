@@ -888,24 +888,25 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
   bs.flush(part);
 }
 
-// Queued single excitations (molecular, heat-bath singles), kHbGroup lanes per
-// attempt.  gen_single_heatbath's two sequential sweeps over the same-spin
-// orbitals b need |H(src -> b)| for every b; lane g of a group evaluates the
-// orbitals b = g+1, g+1+kHbGroup, ... from the (Coulomb, exchange) rows
-// jk[a][j][*] (contiguous in b, so a group's loads for one occupied orbital j
-// are one contiguous run).  Each value is summed in sltcnd_1's order (Coulomb
-// over the occupied orbitals, then exchange over the same-spin ones, then the
-// one-body term) and stored in LDS; both sweeps then run in ascending b over
-// LDS on every lane of the group, so the selected orbital, pgen and H_ij are
-// bit-identical to the one-lane generator.  The three Philox blocks of the
-// attempt are computed by three different lanes of the group and shared.
-constexpr int kHbGroup = 4;
-constexpr int kHbGroups = kBlock / kHbGroup;
+// Queued single excitations (molecular, heat-bath singles), G lanes per
+// attempt (G = 4, 8 or 16, a launch-shape choice).  gen_single_heatbath's two
+// sequential sweeps over the same-spin orbitals b need |H(src -> b)| for every
+// b; lane g of a group evaluates the orbitals b = g+1, g+1+G, ... from the
+// (Coulomb, exchange) rows jk[a][j][*] (contiguous in b, so a group's loads for
+// one occupied orbital j are one contiguous run).  Each value is summed in
+// sltcnd_1's order (Coulomb over the occupied orbitals, then exchange over the
+// same-spin ones, then the one-body term) and stored in LDS; both sweeps then
+// run in ascending b over LDS on every lane of the group, so the selected
+// orbital, pgen and H_ij are bit-identical to the one-lane generator.  The
+// attempt's three Philox blocks are spread over the group's lanes and shared.
+// More lanes per attempt shorten each attempt's chain of loads (measured at
+// 20 spatial orbitals: 1, 2 and 4 lanes took 116, 64 and 46 us for 1e5
+// attempts); fewer lanes would repeat the sequential sweeps on fewer lanes.
 
 // Accumulates sign * rows[b] into v for the occupied orbitals of x (ascending,
 // src removed), two orbitals per step so that both rows' loads are in flight
 // before the ordered additions.
-template <int PL>
+template <int PL, int G>
 __device__ inline void hb_rows(uint64_t x, int base, const double* row, int nsp, int g, int mcount, double sign,
                                double* v) {
   while (x) {
@@ -919,13 +920,13 @@ __device__ inline void hb_rows(uint64_t x, int base, const double* row, int nsp,
     double a1[PL], a2[PL];
 #pragma unroll
     for (int m = 0; m < PL; ++m) {
-      bool in = m < mcount && g + m * kHbGroup < nsp;
-      a1[m] = in ? r1[2 * m * kHbGroup] : 0.0;
-      a2[m] = (in && two) ? r2[2 * m * kHbGroup] : 0.0;
+      bool in = m < mcount && g + m * G < nsp;
+      a1[m] = in ? r1[2 * m * G] : 0.0;
+      a2[m] = (in && two) ? r2[2 * m * G] : 0.0;
     }
 #pragma unroll
     for (int m = 0; m < PL; ++m) {
-      if (m < mcount && g + m * kHbGroup < nsp) {
+      if (m < mcount && g + m * G < nsp) {
         v[m] = sign > 0.0 ? v[m] + a1[m] : v[m] - a1[m];
         if (two) v[m] = sign > 0.0 ? v[m] + a2[m] : v[m] - a2[m];
       }
@@ -933,12 +934,13 @@ __device__ inline void hb_rows(uint64_t x, int base, const double* row, int nsp,
   }
 }
 
-// PL: spatial orbitals per lane (nsp <= kHbGroup*PL); dynamic LDS holds the
-// kHbGroups*nsp values of the block's attempts
-template <int W, int R, int PL>
+// PL: spatial orbitals per lane (nsp <= G*PL); dynamic LDS holds the
+// (kBlock/G)*nsp values of the block's attempts
+template <int W, int R, int PL, int G>
 __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Counters* cnt, double* part,
                                                           const uint8_t* pinfo, uint64_t* sp_rec, uint32_t* rk,
                                                           int64_t scap, const int64_t* sq, const IterParams* prm) {
+  constexpr int kGroups = kBlock / G;
   const double tau = prm->tau;
   const uint32_t iter = prm->iter;
   __shared__ double acc_lds[kStatStride];
@@ -955,14 +957,14 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
   bool overflow = false;
   SpawnAcc<R> acc;
   const int nsp = s.nbasis / 2;
-  const int mcount = (nsp + kHbGroup - 1) / kHbGroup;  // wave-uniform
-  const int g = threadIdx.x % kHbGroup;
-  const int gl = threadIdx.x / kHbGroup;                // group within the block
-  const int gbase = lane_id() - lane_id() % kHbGroup;
+  const int mcount = (nsp + G - 1) / G;  // wave-uniform
+  const int g = threadIdx.x % G;
+  const int gl = threadIdx.x / G;        // group within the block
+  const int gbase = lane_id() - lane_id() % G;
   const bool leader = g == 0;
   double* myv = hv_dyn + (size_t)gl * nsp;
-  const int64_t groups = (int64_t)gridDim.x * kHbGroups;
-  const int64_t grp = (int64_t)blockIdx.x * kHbGroups + gl;
+  const int64_t groups = (int64_t)gridDim.x * kGroups;
+  const int64_t grp = (int64_t)blockIdx.x * kGroups + gl;
   const int64_t trips = (nq + groups - 1) / groups;
   for (int64_t t = 0; t < trips; ++t) {
     int64_t q = t * groups + grp;
@@ -974,13 +976,15 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
     uint64_t key[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) key[w] = act ? wk.keys[slot * W + w] : s.ref[w];  // idle groups: any valid det
-    // lane g < 3 draws Philox block (purpose g+1); d1 = dsel (purpose 1), d5 =
-    // spawn rounding (purpose 2, second value), d2 = dorb (purpose 3, first)
-    double ua = 0.0, ub = 0.0;
-    if (g < 3) draw2<W>(s.seed, key, iter, r, k, (uint32_t)(g + 1), ua, ub);
-    const double d1 = __shfl(ub, gbase + 0);
-    const double d5 = __shfl(ub, gbase + 1);
-    const double d2 = __shfl(ua, gbase + 2);
+    // lane q % G draws Philox block q+1 (q = 0, 1, 2); d1 = dsel (purpose 1),
+    // d5 = spawn rounding (purpose 2, second value), d2 = dorb (purpose 3, first)
+    double ua[3] = {0.0, 0.0, 0.0}, ub[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q3 = 0; q3 < 3; ++q3)
+      if (q3 % G == g) draw2<W>(s.seed, key, iter, r, k, (uint32_t)(q3 + 1), ua[q3], ub[q3]);
+    const double d1 = G == 1 ? ub[0] : __shfl(ub[0], gbase + 0 % G);
+    const double d5 = G == 1 ? ub[1] : __shfl(ub[1], gbase + 1 % G);
+    const double d2 = G == 1 ? ua[2] : __shfl(ua[2], gbase + 2 % G);
     int elec = 1 + (int)floor(d1 * (double)s.nel);
     int src = select_orb<W>(key, ~0ull, elec);
     int a = spat(src), off = is_beta(src) ? 1 : 0;
@@ -994,17 +998,17 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
     for (int w = 0; w < W; ++w) {
       uint64_t x = key[w];
       if ((src - 1) >> 6 == w) x &= ~(1ull << ((src - 1) & 63));
-      hb_rows<PL>(x, 64 * w, row, nsp, g, mcount, 1.0, v);
+      hb_rows<PL, G>(x, 64 * w, row, nsp, g, mcount, 1.0, v);
     }
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       uint64_t x = key[w] & same;
       if ((src - 1) >> 6 == w) x &= ~(1ull << ((src - 1) & 63));
-      hb_rows<PL>(x, 64 * w, row + 1, nsp, g, mcount, -1.0, v);
+      hb_rows<PL, G>(x, 64 * w, row + 1, nsp, g, mcount, -1.0, v);
     }
 #pragma unroll
     for (int m = 0; m < PL; ++m) {
-      int b = g + m * kHbGroup + 1;
+      int b = g + m * G + 1;
       if (m < mcount && b <= nsp) {
         int orb = 2 * b - off;
         myv[b - 1] = test_orb<W>(key, orb) ? 0.0 : v[m] + T(s, src, orb);
