@@ -194,8 +194,11 @@ __global__ void k_begin_iter(ngpu_iter_out_t* out, double* part, Counters* cnt, 
 // accumulation into the cycle sums (ENumCyc, HFCyc, NoBorn, ... are cycle
 // accumulators in the reference too)
 __global__ void k_end_iter(ngpu_iter_out_t* out, const double* part, ngpu_iter_out_t* cyc, Counters* cnt,
-                           int64_t scap, int64_t hash_entries) {
-  if (threadIdx.x == 0) hash_done(cnt, hash_entries);  // after the rebuild kernels, if they ran
+                           int64_t scap, int64_t cap, int64_t hash_entries) {
+  if (threadIdx.x == 0) {
+    post_insert(cnt, cnt->n_new, cap);  // the slots k_insert filled
+    hash_account(cnt, hash_entries);
+  }
   double* g = (double*)out;
   double* c = (double*)cyc;
   for (int i = threadIdx.x; i < kStatStride; i += blockDim.x) {
@@ -281,14 +284,15 @@ void launch(ngpu_t* h, int open, int close, F kern, dim3 grid, dim3 block, Args.
   else hipLaunchKernelGGL(kern, grid, block, 0, h->stream, args...);
 }
 
-// tombstone reclamation and the load guard, launched unconditionally (the
-// kernels exit at once unless the counters call for a rebuild); with_done:
-// also the bookkeeping kernel (in an iteration k_end_iter does it)
+// tombstone reclamation and the load guard after a set-up path (upload, core
+// placement): the kernels exit at once unless the counters call for a
+// rebuild.  Iterations launch none of them: the end-of-iteration accounting
+// flags a due rebuild and the next iteration's prepass and attempt scan do it.
 template <int W>
-void hash_upkeep(ngpu_t* h, bool with_done = true) {
+void hash_upkeep(ngpu_t* h) {
   launch(h, -1, -1, k_hash_clear<W>, dim3(h->nb_grid), dim3(kBlock), h->wk, (const Counters*)h->cnt);
   launch(h, -1, -1, k_hash_refill<W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt);
-  if (with_done) launch(h, -1, -1, k_hash_done, dim3(1), dim3(1), h->cnt, h->wk.L * (int64_t)kBucketEntries);
+  launch(h, -1, -1, k_hash_done, dim3(1), dim3(1), h->cnt, h->wk.L * (int64_t)kBucketEntries);
 }
 
 // sort of the spawned list (stage 4) then the merge of equal determinants
@@ -414,13 +418,10 @@ void annihilation_chain(ngpu_t* h, bool with_core, bool list_out = false) {
   launch(h, 7, 7, k_insert<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt,
          (const uint64_t*)h->n_key, (const double*)h->n_sgn, (const uint32_t*)h->n_flag,
          (const unsigned long long*)&h->cnt->n_new);
-  launch(h, -1, -1, k_post_insert, dim3(1), dim3(1), h->cnt, (const unsigned long long*)&h->cnt->n_new,
-         h->wk.cap);
   launch(h, 8, 8, k_stats<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
          (const IterParams*)h->dparams);
-  hash_upkeep<W>(h, false);
   launch(h, -1, -1, k_end_iter, dim3(1), dim3(kBlock), h->dout, (const double*)h->dpart, h->dcyc, h->cnt,
-         h->scap, h->wk.L * (int64_t)kBucketEntries);
+         h->scap, h->wk.cap, h->wk.L * (int64_t)kBucketEntries);
 }
 
 template <int W, int R>
@@ -430,8 +431,8 @@ void iterate_chain(ngpu_t* h) {
   launch(h, 0, 0, k_prepass<W, R>, dim3(h->nb_chunk), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart, h->att_cnt,
          h->pinfo, h->blk_part, (const IterParams*)h->dparams);
   if (h->ncore) core_projection<R>(h);
-  launch(h, 1, 1, k_scan_apply, dim3(h->nb_chunk), dim3(kBlock), (const int32_t*)h->att_cnt, h->att_off,
-         (const int64_t*)h->blk_part, h->cnt, h->S, h->tile_p, h->tcap);
+  launch(h, 1, 1, k_scan_apply<W>, dim3(h->nb_chunk), dim3(kBlock), h->ds, h->wk, (const int32_t*)h->att_cnt,
+         h->att_off, (const int64_t*)h->blk_part, h->cnt, h->S, h->tile_p, h->tcap);
   auto spawn = h->sys.model == NGPU_MODEL_HUBBARD ? k_spawn<W, R, kGenHubbard>
              : h->ds.uniform ? k_spawn<W, R, kGenUniform> : k_spawn<W, R, kGenPchb>;
   launch(h, 2, 2, spawn, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,

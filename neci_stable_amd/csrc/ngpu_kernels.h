@@ -60,7 +60,12 @@ struct alignas(256) Counters {
   unsigned long long n_live, n_tomb, n_occ;
   unsigned long long n_attempts;
   unsigned long long n_fail;      // inserts refused (capacity errors)
+  unsigned long long free_tail_ins;  // free_tail as k_insert saw it
   unsigned int err;
+  // the walker hash is rebuilt at the start of the next iteration (set by the
+  // end-of-iteration accounting; the prepass clears the table, the attempt
+  // scan refills it)
+  unsigned int rebuild;
   alignas(256) unsigned long long free_tail;
   alignas(256) unsigned long long n_spawned;
   alignas(256) unsigned long long n_singles;  // queued single-excitation attempts
@@ -235,13 +240,17 @@ struct FreeStage {
 // bucket already holds an empty entry would also be valid -- such a bucket
 // ends every probe -- but it makes each removal read its bucket and, measured,
 // cost more in the removing kernels than the rebuilds it saves.)
+// tomb = false: the table is being cleared for a rebuild, so the entry is left
+// alone (the rebuild reinserts live slots only)
 template <int W>
-__device__ inline void remove_dets(const Walk& wk, FreeStage* fs, bool rm, int64_t slot) {
+__device__ inline void remove_dets(const Walk& wk, FreeStage* fs, bool rm, int64_t slot, bool tomb = true) {
   int q = stage_push(rm, &fs->st);
   if (rm) {
-    int64_t hp = wk.hpos(slot);
-    int32_t* sl = (int32_t*)(wk.htab + (size_t)(hp >> 2) * 8 * W + 4 * W);
-    sl[hp & 3] = kSlotTomb;
+    if (tomb) {
+      int64_t hp = wk.hpos(slot);
+      int32_t* sl = (int32_t*)(wk.htab + (size_t)(hp >> 2) * 8 * W + 4 * W);
+      sl[hp & 3] = kSlotTomb;
+    }
     wk.flags(slot) |= kFlagRemoved;
     fs->slot[q] = slot;
   }
@@ -311,6 +320,32 @@ __device__ inline int64_t hash_insert(const Walk& wk, const uint64_t* key, int64
     b = (b + 1 == wk.L) ? 0 : b + 1;
   }
   return -1;
+}
+
+// rebuild of the walker hash in two grid-wide phases (a kernel boundary
+// between them): every entry emptied, then every live slot reinserted
+template <int W>
+__device__ inline void hash_clear_part(const Walk& wk) {
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < wk.L; b += (int64_t)gridDim.x * blockDim.x) {
+    int32_t* sl = (int32_t*)(wk.htab + (size_t)b * 8 * W + 4 * W);
+#pragma unroll
+    for (int e = 0; e < kBucketEntries; ++e) sl[e] = kSlotEmpty;
+  }
+}
+template <int W>
+__device__ inline void hash_refill_part(const DevSys& s, const Walk& wk, Counters* cnt, int64_t nslots) {
+  for (int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < nslots;
+       slot += (int64_t)gridDim.x * blockDim.x) {
+    if (wk.flags(slot) & kFlagRemoved) continue;
+    uint64_t key[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
+    int64_t home = find_walker_hash<W>(key, s.rh, s.L) - 1;
+    bool reused = false;
+    int64_t hp = hash_insert<W>(wk, key, home, (int32_t)slot, reused);
+    if (hp < 0) atomicOr(&cnt->err, 1u << NGPU_ERR_HASH_FULL);
+    else wk.hpos(slot) = hp;
+  }
 }
 
 __device__ inline void chunk_of(int64_t n, int64_t& begin, int64_t& end) {
@@ -396,6 +431,8 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
   stage_init(&fs.st);
   const int S = R == 1 ? 1 : s.S;
   int64_t nslots = (int64_t)cnt->nslots;
+  const bool rebuild = cnt->rebuild != 0;
+  if (rebuild) hash_clear_part<W>(wk);  // the attempt scan refills the table
   int64_t begin, end;
   chunk_of(nslots, begin, end);
   int64_t my_att = 0;
@@ -491,7 +528,7 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
       for (int r = 0; r < S; ++r) wk.sgn(slot, r) = rm ? 0.0 : cp[r];
       wk.flags(slot) = f;
     }
-    remove_dets<W>(wk, &fs, rm, slot);
+    remove_dets<W>(wk, &fs, rm, slot, !rebuild);
     free_flush(wk, cnt, &fs, false);
   }
   free_flush(wk, cnt, &fs, true);
@@ -515,12 +552,25 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
 
 // attempt offsets per (slot, replica) pair, same chunking as k_prepass; also
 // the parent pair of the first attempt of every kBlock-attempt tile (the
-// spawn kernel's tiles), written by the pair whose attempts contain it
-__global__ void __launch_bounds__(kBlock) k_scan_apply(const int32_t* att_cnt, int64_t* att_off,
+// spawn kernel's tiles), written by the pair whose attempts contain it.  When
+// a rebuild is due, also the refill of the walker hash that the prepass
+// cleared, before any lookup of the iteration.
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_scan_apply(DevSys s, Walk wk, const int32_t* att_cnt, int64_t* att_off,
                                                        const int64_t* part, Counters* cnt, int S,
                                                        int64_t* tile_p, int64_t tcap) {
   __shared__ int64_t red[kBlock / kWave];
   int64_t nslots = (int64_t)cnt->nslots;
+  if (cnt->rebuild) {
+    hash_refill_part<W>(s, wk, cnt, nslots);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      // the prepass's removals left no tombstones; the table now holds the
+      // live slots only
+      cnt->n_live -= cnt->free_tail - cnt->free_tail0;
+      cnt->free_tail0 = cnt->free_tail;
+      cnt->n_tomb = 0;
+    }
+  }
   int64_t begin, end;
   chunk_of(nslots, begin, end);
   int64_t pb = begin * S, pe = end * S;
@@ -1736,8 +1786,10 @@ __global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* 
   // new det i takes the i-th queued free slot, past the queue a fresh slot
   // at the end of the table: positions follow from i, no atomics
   const unsigned long long head = cnt->free_head;
-  const int64_t avail = (int64_t)(cnt->free_tail - head);
+  const unsigned long long tail = cnt->free_tail;
+  const int64_t avail = (int64_t)(tail - head);
   const int64_t ns0 = (int64_t)cnt->nslots;
+  if (blockIdx.x == 0 && threadIdx.x == 0) cnt->free_tail_ins = tail;  // for the slot bookkeeping
   unsigned long long my_reused = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -1783,13 +1835,23 @@ __global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* 
 
 // bookkeeping of k_insert: queued slots used, fresh slots appended (beyond
 // the capacity refused), live entries added
-__global__ void k_post_insert(Counters* cnt, const unsigned long long* n_ptr, int64_t cap) {
-  unsigned long long nn = *n_ptr;
-  unsigned long long avail = cnt->free_tail - cnt->free_head;
-  unsigned long long used = nn < avail ? nn : avail;
+// slots after k_insert placed nn new determinants: the first ones took the
+// queued free slots, the rest fresh slots at the end of the table.  A pure
+// function of counters that nothing changes between k_insert and the
+// bookkeeping, so k_stats evaluates it for its loop bound while the
+// bookkeeping (k_end_iter, or k_post_insert on the set-up paths) applies it.
+__device__ inline void slots_after_insert(const Counters* cnt, unsigned long long nn, int64_t cap,
+                                          unsigned long long& used, unsigned long long& ns) {
+  unsigned long long avail = cnt->free_tail_ins - cnt->free_head;
+  used = nn < avail ? nn : avail;
+  ns = cnt->nslots + (nn - used);
+  if (ns > (unsigned long long)cap) ns = (unsigned long long)cap;
+}
+__device__ inline void post_insert(Counters* cnt, unsigned long long nn, int64_t cap) {
+  unsigned long long used, ns;
+  slots_after_insert(cnt, nn, cap, used, ns);
   cnt->free_head += used;
-  unsigned long long ns = cnt->nslots + (nn - used);
-  cnt->nslots = ns < (unsigned long long)cap ? ns : (unsigned long long)cap;
+  cnt->nslots = ns;
   cnt->n_live += nn - cnt->n_fail;
   unsigned long long reused = 0;
   for (int q = 0; q < Counters::kSpread; ++q) {
@@ -1797,6 +1859,9 @@ __global__ void k_post_insert(Counters* cnt, const unsigned long long* n_ptr, in
     cnt->reuse_part[q].v = 0;
   }
   cnt->n_tomb -= reused;
+}
+__global__ void k_post_insert(Counters* cnt, const unsigned long long* n_ptr, int64_t cap) {
+  post_insert(cnt, *n_ptr, cap);
 }
 
 // CalcHashTableStats (load_balancer.fpp:646-805)
@@ -1810,7 +1875,9 @@ __global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* c
   bs.zero();
   stage_init(&fs.st);
   const int S = R == 1 ? 1 : s.S;
-  int64_t nslots = (int64_t)cnt->nslots;
+  unsigned long long used, ns;
+  slots_after_insert(cnt, cnt->n_new, wk.cap, used, ns);  // the bookkeeping applies it later
+  const int64_t nslots = (int64_t)ns;
   unsigned long long my_occ = 0;
   double tp[R], nq[R], rmv[R], bn[R];
   for (int r = 0; r < S; ++r) tp[r] = nq[r] = rmv[r] = bn[r] = 0.0;
@@ -1889,10 +1956,11 @@ __global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* c
 // tombstones pass 10 %): later rebuilds against longer probe sequences
 constexpr unsigned long long kRebuildUsedPct = 60;
 
-// the rebuild decision, a pure function of the counters after the
-// iteration's removals (nothing changes them between k_stats and the end of
-// the iteration), so the clear, refill and bookkeeping kernels each evaluate
-// it and no kernel has to publish it
+// the rebuild decision, a pure function of the counters after the removals
+// since the last accounting (nothing changes them between the removing
+// kernels and the accounting), so the in-place clear, refill and bookkeeping
+// kernels each evaluate it; an iteration's accounting publishes it for the
+// next iteration instead
 __device__ inline bool hash_rebuild_due(const Counters* cnt, int64_t entries) {
   unsigned long long removed = cnt->free_tail - cnt->free_tail0;  // each left a tombstone
   unsigned long long live = cnt->n_live - removed, tomb = cnt->n_tomb + removed;
@@ -1900,42 +1968,40 @@ __device__ inline bool hash_rebuild_due(const Counters* cnt, int64_t entries) {
          tomb * 10ull > (unsigned long long)entries;
 }
 
-// bookkeeping of the iteration's removals and of a rebuild (one thread)
+// bookkeeping of the removals since the last call and of an in-place rebuild
+// by k_hash_clear/k_hash_refill (one thread; set-up paths)
 __device__ inline void hash_done(Counters* cnt, int64_t entries) {
   bool rebuilt = hash_rebuild_due(cnt, entries);
   unsigned long long removed = cnt->free_tail - cnt->free_tail0;
   cnt->n_live -= removed;
   cnt->n_tomb = rebuilt ? 0ull : cnt->n_tomb + removed;
   cnt->free_tail0 = cnt->free_tail;
+  cnt->rebuild = 0;
   if (cnt->n_live * 10ull > (unsigned long long)entries * 8ull) cnt->err |= 1u << NGPU_ERR_HASH_FULL;
 }
 
+// end of an iteration (one thread): the removals' tombstones are counted, and
+// a rebuild that has become due is left to the next iteration's prepass and
+// attempt scan, which run before any other access to the table
+__device__ inline void hash_account(Counters* cnt, int64_t entries) {
+  unsigned long long removed = cnt->free_tail - cnt->free_tail0;
+  cnt->n_live -= removed;
+  cnt->n_tomb += removed;
+  cnt->free_tail0 = cnt->free_tail;
+  cnt->rebuild = hash_rebuild_due(cnt, entries) ? 1u : 0u;
+  if (cnt->n_live * 10ull > (unsigned long long)entries * 8ull) cnt->err |= 1u << NGPU_ERR_HASH_FULL;
+}
+
+// in-place rebuild for the set-up paths (upload, core placement, explicit
+// lists); iterations rebuild inside k_prepass and k_scan_apply instead
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_hash_clear(Walk wk, const Counters* cnt) {
-  if (!hash_rebuild_due(cnt, wk.L * (int64_t)kBucketEntries)) return;
-  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < wk.L; b += (int64_t)gridDim.x * blockDim.x) {
-    int32_t* sl = (int32_t*)(wk.htab + (size_t)b * 8 * W + 4 * W);
-#pragma unroll
-    for (int e = 0; e < kBucketEntries; ++e) sl[e] = kSlotEmpty;
-  }
+  if (hash_rebuild_due(cnt, wk.L * (int64_t)kBucketEntries)) hash_clear_part<W>(wk);
 }
 
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_hash_refill(DevSys s, Walk wk, Counters* cnt) {
-  if (!hash_rebuild_due(cnt, wk.L * (int64_t)kBucketEntries)) return;
-  int64_t nslots = (int64_t)cnt->nslots;
-  for (int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < nslots;
-       slot += (int64_t)gridDim.x * blockDim.x) {
-    if (wk.flags(slot) & kFlagRemoved) continue;
-    uint64_t key[W];
-#pragma unroll
-    for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
-    int64_t home = find_walker_hash<W>(key, s.rh, s.L) - 1;
-    bool reused = false;
-    int64_t hp = hash_insert<W>(wk, key, home, (int32_t)slot, reused);
-    if (hp < 0) atomicOr(&cnt->err, 1u << NGPU_ERR_HASH_FULL);
-    else wk.hpos(slot) = hp;
-  }
+  if (hash_rebuild_due(cnt, wk.L * (int64_t)kBucketEntries)) hash_refill_part<W>(s, wk, cnt, (int64_t)cnt->nslots);
 }
 
 __global__ void k_hash_done(Counters* cnt, int64_t entries) { hash_done(cnt, entries); }
