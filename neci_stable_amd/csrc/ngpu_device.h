@@ -270,20 +270,41 @@ __device__ __host__ inline double T(const DevSys& s, int i, int j) {
   return s.tmat[(size_t)(i - 1) + (size_t)(j - 1) * (size_t)s.nbasis];
 }
 
+// Accumulates the values at(o) over the set bits o of x (1-based orbital
+// base + bit + 1, ascending), adding (sub = false) or subtracting them in bit
+// order; at(o) returns a pointer, or null to skip o.  Four values are loaded
+// before any of them is used, so a chain of n loads takes n/4 round trips
+// while the sum stays bit-identical to one bit at a time.
+template <typename F>
+__device__ __host__ inline double accum_bits4(uint64_t x, int base, double acc, bool sub, F at) {
+  while (x) {
+    const double* p[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      p[q] = nullptr;
+      if (x) {
+        p[q] = at(base + ctz64(x) + 1);
+        x &= x - 1;
+      }
+    }
+    double v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = p[q] ? *p[q] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (p[q]) acc = sub ? acc - v[q] : acc + v[q];
+  }
+  return acc;
+}
+
 // sltcnd_0_base (sltcnd.fpp:585-622): ordering of the FP sums follows the
 // reference loops exactly (inner partial sum per i, then exchange, then one-body)
 template <int W>
 __device__ __host__ inline double sltcnd_0(const DevSys& s, const uint64_t* d) {
   double hel_sing = 0.0;
 #pragma unroll
-  for (int w = 0; w < W; ++w) {
-    uint64_t x = d[w];
-    while (x) {
-      int o = 64 * w + ctz64(x) + 1;
-      x &= x - 1;
-      hel_sing += T(s, o, o);
-    }
-  }
+  for (int w = 0; w < W; ++w)
+    hel_sing = accum_bits4(d[w], 64 * w, hel_sing, false, [&](int o) { return &s.tmat[(size_t)(o - 1) * (s.nbasis + 1)]; });
   double hel_doub = 0.0, hel_tmp = 0.0;
   // i runs over electrons; the inner loops run over electrons above i
 #pragma unroll
@@ -292,19 +313,14 @@ __device__ __host__ inline double sltcnd_0(const DevSys& s, const uint64_t* d) {
     while (xi) {
       int oi = 64 * wi + ctz64(xi) + 1;
       xi &= xi - 1;
-      int idi = spat(oi);
+      const double* row = s.jk0 + 2 * (size_t)(spat(oi) - 1) * s.nsp;
       double t = 0.0;
       bool any = false;
 #pragma unroll
       for (int wj = wi; wj < W; ++wj) {
         uint64_t xj = (wj == wi) ? xi : d[wj];
-        while (xj) {
-          int oj = 64 * wj + ctz64(xj) + 1;
-          xj &= xj - 1;
-          int idj = spat(oj);
-          t += s.jk0[2 * ((idi - 1) * s.nsp + (idj - 1))];  // U(idi,idj,idi,idj)
-          any = true;
-        }
+        any = any || xj != 0;
+        t = accum_bits4(xj, 64 * wj, t, false, [&](int oj) { return row + 2 * (spat(oj) - 1); });  // U(i,j,i,j)
       }
       if (any) hel_doub += t;
     }
@@ -315,16 +331,13 @@ __device__ __host__ inline double sltcnd_0(const DevSys& s, const uint64_t* d) {
     while (xi) {
       int oi = 64 * wi + ctz64(xi) + 1;
       xi &= xi - 1;
+      const double* row = s.jk0 + 2 * (size_t)(spat(oi) - 1) * s.nsp + 1;
       // exchange only between equal spins: bits of the other spin are masked
       uint64_t same = is_beta(oi) ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
 #pragma unroll
       for (int wj = wi; wj < W; ++wj) {
         uint64_t xj = ((wj == wi) ? xi : d[wj]) & same;
-        while (xj) {
-          int oj = 64 * wj + ctz64(xj) + 1;
-          xj &= xj - 1;
-          hel_tmp -= s.jk0[2 * ((spat(oi) - 1) * s.nsp + (spat(oj) - 1)) + 1];  // U(i,j,j,i)
-        }
+        hel_tmp = accum_bits4(xj, 64 * wj, hel_tmp, true, [&](int oj) { return row + 2 * (spat(oj) - 1); });  // U(i,j,j,i)
       }
     }
   }
@@ -337,31 +350,15 @@ __device__ __host__ inline double sltcnd_1(const DevSys& s, const uint64_t* d, i
   int a = spat(src), b = spat(tgt);
   double hel = 0.0;
   if (is_beta(src) == is_beta(tgt)) {
+    const double* base = s.jk + 2 * ((size_t)(a - 1) * s.nsp * s.nsp + (size_t)(b - 1));
+    const size_t step = 2 * (size_t)s.nsp;
+    auto coul = [&](int o) -> const double* { return o == src ? nullptr : base + step * (size_t)(spat(o) - 1); };
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-      uint64_t x = d[w];
-      while (x) {
-        int o = 64 * w + ctz64(x) + 1;
-        x &= x - 1;
-        if (o != src) {
-          int id = spat(o);
-          hel += s.jk[2 * (((size_t)(a - 1) * s.nsp + (id - 1)) * s.nsp + (b - 1))];  // U(a,id,b,id)
-        }
-      }
-    }
+    for (int w = 0; w < W; ++w) hel = accum_bits4(d[w], 64 * w, hel, false, coul);  // U(a,id,b,id)
     uint64_t same = is_beta(src) ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
+    auto exch = [&](int o) -> const double* { return o == src ? nullptr : base + step * (size_t)(spat(o) - 1) + 1; };
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-      uint64_t x = d[w] & same;
-      while (x) {
-        int o = 64 * w + ctz64(x) + 1;
-        x &= x - 1;
-        if (o != src) {
-          int id = spat(o);
-          hel -= s.jk[2 * (((size_t)(a - 1) * s.nsp + (id - 1)) * s.nsp + (b - 1)) + 1];  // U(a,id,id,b)
-        }
-      }
-    }
+    for (int w = 0; w < W; ++w) hel = accum_bits4(d[w] & same, 64 * w, hel, true, exch);  // U(a,id,id,b)
   }
   hel += T(s, src, tgt);
   return hel;
