@@ -31,7 +31,19 @@ SINGLE_KERNEL_STAGES = set(KERNELS)
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(stage, c, W=1, S=1, nbasis=40, psingles=0.1):
+def sort_key(nbasis, nel):
+    """(bytes per key, 8-bit passes) of the spawn sort: the 32-bit colex rank
+    when C(nbasis, nel) fits 32 bits (and its binomial table the engine's LDS
+    copy), else the determinant words."""
+    import math
+    if nel is not None and math.comb(nbasis, nel) <= 2 ** 32 and nbasis * (nel + 1) <= 512:
+        bits = max(1, (math.comb(nbasis, nel) - 1).bit_length())
+        return 4, (bits + 7) // 8
+    W = (nbasis + 63) // 64
+    return 8 * W, ((nbasis + 7) // 8 if W == 1 else (nbasis - 64 + 7) // 8 + 8)
+
+
+def algorithmic_bytes(stage, c, W=1, S=1, nbasis=40, psingles=0.1, nel=None):
     """Compulsory HBM bytes of one iteration's launch(es) of a stage: every input
     read once and every output written once, for the per-iteration counts c
     (Nd occupied dets, Na attempts, Ns spawns, Nu unique spawned dets, Nn new
@@ -51,8 +63,8 @@ def algorithmic_bytes(stage, c, W=1, S=1, nbasis=40, psingles=0.1):
     if stage == "spawn_singles":  # queue entry, parent key/pinfo/offset, single spawns written
         return Nq * (16 + K + 1 + 8) + Nsq * (K + 12)
     if stage == "sort":        # per 8-bit pass: histogram read of keys, scatter read+write of (key, index)
-        passes = (nbasis + 7) // 8 if W == 1 else (nbasis - 64 + 7) // 8 + 8
-        return passes * Ns * (K + 2 * (K + 4))
+        kb, passes = sort_key(nbasis, nel)
+        return passes * Ns * (kb + 2 * (kb + 4))
     if stage == "merge":       # fused into the annihilate stage
         return 0.0
     if stage == "annihilate":  # merge and annihilation in one pass: sorted keys + index, gathered value + info
@@ -64,6 +76,15 @@ def algorithmic_bytes(stage, c, W=1, S=1, nbasis=40, psingles=0.1):
     if stage == "stats":       # flags, signs, key per slot
         return Nd * (4 + Sg + K)
     return 0.0
+
+
+def iteration_bytes(c, W=1, S=1, nbasis=40, psingles=0.1, nel=None):
+    """Compulsory HBM bytes of one whole iteration: the stages' figures of
+    algorithmic_bytes summed, except that the sort counts one read and one
+    write of (key, index) per spawn, the algorithmic minimum, not one per
+    radix pass."""
+    tot = sum(algorithmic_bytes(st, c, W, S, nbasis, psingles, nel) for st in STAGES if st != "sort")
+    return tot + 2 * c["Ns"] * (sort_key(nbasis, nel)[0] + 4)
 
 
 def _kernel_short(name):
@@ -410,8 +431,9 @@ def main():
                   Nn=float(sum(r["new"] for r in rows)) / K)
     # the roofline is that of one kernel: the slowest stage made of a single
     # kernel, its launches timed in the timed region itself
-    b = algorithmic_bytes(dom, counts, W=eng.W, S=eng.S, nbasis=spec.nbasis, psingles=spec.psingles)
+    b = algorithmic_bytes(dom, counts, W=eng.W, S=eng.S, nbasis=spec.nbasis, psingles=spec.psingles, nel=spec.nel)
     achieved = b / (dom_ms * 1e-3) / 1e9
+    it_bytes = iteration_bytes(counts, W=eng.W, S=eng.S, nbasis=spec.nbasis, psingles=spec.psingles, nel=spec.nel)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_mol10e20o.json")
     if os.path.exists(tpath):
@@ -436,6 +458,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": b, "launch_ms": dom_ms},
+        # the whole iteration against the same peak: compulsory bytes of every
+        # stage per iteration over the measured time per iteration
+        "iteration_roofline": {"bound": "hbm", "achieved": it_bytes / (dt / K) / 1e9, "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": it_bytes / (dt / K) / 1e9 / HBM_PEAK_GBS,
+                               "algorithmic_bytes_per_iteration": it_bytes},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

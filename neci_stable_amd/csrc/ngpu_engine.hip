@@ -62,6 +62,7 @@ struct ngpu_engine {
   int64_t* att_off = nullptr;
   uint8_t* pinfo = nullptr;
   int64_t* blk_part = nullptr;
+  int64_t* blk_rm = nullptr;  // per prepass block: slots freed
   int64_t* tile_p = nullptr;  // parent pair of each spawn tile's first attempt
   int64_t tcap = 0;
   // semistochastic core space
@@ -429,10 +430,11 @@ void iterate_chain(ngpu_t* h) {
   launch(h, -1, -1, k_begin_iter, dim3(1), dim3(kBlock), h->dout, h->dpart, h->cnt, h->dparams, h->next_params,
          h->chain_sets_params ? 1 : 0);
   launch(h, 0, 0, k_prepass<W, R>, dim3(h->nb_chunk), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart, h->att_cnt,
-         h->pinfo, h->blk_part, (const IterParams*)h->dparams);
+         h->pinfo, h->blk_part, h->blk_rm, (const IterParams*)h->dparams);
   if (h->ncore) core_projection<R>(h);
   launch(h, 1, 1, k_scan_apply<W>, dim3(h->nb_chunk), dim3(kBlock), h->ds, h->wk, (const int32_t*)h->att_cnt,
-         h->att_off, (const int64_t*)h->blk_part, h->cnt, h->S, h->tile_p, h->tcap);
+         h->att_off, (const int64_t*)h->blk_part, (const int64_t*)h->blk_rm, (const uint8_t*)h->pinfo, h->cnt,
+         h->S, h->tile_p, h->tcap);
   auto spawn = h->sys.model == NGPU_MODEL_HUBBARD ? k_spawn<W, R, kGenHubbard>
              : h->ds.uniform ? k_spawn<W, R, kGenUniform> : k_spawn<W, R, kGenPchb>;
   launch(h, 2, 2, spawn, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
@@ -725,6 +727,7 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
   fail(dalloc(&h->att_off, (size_t)cap * S + 1));
   fail(dalloc(&h->pinfo, (size_t)cap * S));
   fail(dalloc(&h->blk_part, (size_t)h->max_chunk));
+  fail(dalloc(&h->blk_rm, (size_t)h->max_chunk));
   // tiles of attempts with a recorded first parent: attempts up to 4x the
   // spawn capacity (beyond it the spawn kernel searches the offsets itself)
   h->tcap = (4 * scap) / kBlock + 2;
@@ -822,7 +825,7 @@ int ngpu_destroy(ngpu_t* h) {
   if (h->batch) return NGPU_ERR_STATE;  // ngpu_batch_destroy first
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void* ptrs[] = {h->wk.keys, h->wk.st, h->wk.diagH, h->wk.offdiagH, h->wk.freeq,
-                  h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->sp_rec,
+                  h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->blk_rm, h->sp_rec,
                   h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->digit_tot, h->sq, h->n_key,
                   h->n_sgn, h->n_flag, h->u_key, h->u_sgn, h->u_flag, h->u_count, h->cnt, h->dout, h->dpart, h->dcyc, h->own_params, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_binom, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
                   h->core_vals, h->core_x, h->core_y, h->d_ab_tab,

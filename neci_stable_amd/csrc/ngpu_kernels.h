@@ -61,11 +61,13 @@ struct alignas(256) Counters {
   unsigned long long n_attempts;
   unsigned long long n_fail;      // inserts refused (capacity errors)
   unsigned long long free_tail_ins;  // free_tail as k_insert saw it
+  unsigned long long rm_pre;         // slots the iteration's prepass freed
   unsigned int err;
   // the walker hash is rebuilt at the start of the next iteration (set by the
   // end-of-iteration accounting; the prepass clears the table, the attempt
   // scan refills it)
   unsigned int rebuild;
+  unsigned int rebuilt;  // set by the attempt scan that refilled the table
   alignas(256) unsigned long long free_tail;
   alignas(256) unsigned long long n_spawned;
   alignas(256) unsigned long long n_singles;  // queued single-excitation attempts
@@ -225,6 +227,9 @@ __device__ inline void stage_init(Stage* st) {
   if (threadIdx.x == 0) st->n = 0;
   __syncthreads();
 }
+
+// pinfo bit of a slot the prepass freed (first replica's byte)
+constexpr uint8_t kPinfoFreed = 8;
 
 // Freed slots staged for the free queue (a ring of cap entries).
 constexpr int kFreeStage = 512;
@@ -419,23 +424,22 @@ __device__ inline int64_t grid_elem(int64_t t) {
 template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters* cnt,
                                                     double* part, int32_t* att_cnt,
-                                                    uint8_t* pinfo, int64_t* blk_part, const IterParams* prm) {
+                                                    uint8_t* pinfo, int64_t* blk_part, int64_t* blk_rm,
+                                                    const IterParams* prm) {
   const double tau = prm->tau;
   const uint32_t iter = prm->iter;
   const double* sftv = prm->sft.v;
   __shared__ double acc_lds[kStatStride];
   __shared__ int64_t red[kBlock / kWave];
-  __shared__ FreeStage fs;
   BlockStats bs{acc_lds};
   bs.zero();
-  stage_init(&fs.st);
   const int S = R == 1 ? 1 : s.S;
   int64_t nslots = (int64_t)cnt->nslots;
   const bool rebuild = cnt->rebuild != 0;
   if (rebuild) hash_clear_part<W>(wk);  // the attempt scan refills the table
   int64_t begin, end;
   chunk_of(nslots, begin, end);
-  int64_t my_att = 0;
+  int64_t my_att = 0, my_rm = 0;
   bool death_err = false;
   LaneSum<R> a_idets, a_iwalk, a_ndets, a_nwalk, a_hf, a_doubs, a_enum, a_died, a_born, a_abort;
   for (int64_t base = begin; base < end; base += blockDim.x) {
@@ -453,6 +457,7 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
       occ = occ || !(fabs(sg[r]) < 1.e-12);
     }
     bool core = (f & kFlagCore) != 0;
+    uint8_t pi0 = 0;
     int exlevel = occ ? excit_level<W>(s.ref, key) : -1;
     double hoff = occ ? wk.offdiagH[slot] : 0.0;
     double kii = occ ? wk.diagH[slot] : 0.0;
@@ -494,7 +499,9 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
         }
         att_cnt[slot * S + r] = (int32_t)nsp;
         my_att += nsp;
-        pinfo[slot * S + r] = (uint8_t)((sg[r] < 0.0 ? 1 : 0) | (((f >> r) & 1u) ? 2 : 0) | (core ? 4 : 0));
+        uint8_t pi = (uint8_t)((sg[r] < 0.0 ? 1 : 0) | (((f >> r) & 1u) ? 2 : 0) | (core ? 4 : 0));
+        if (r == 0) pi0 = pi;  // written with the removal bit below
+        else pinfo[slot * S + r] = pi;
         double fac = tau * (kii - sftv[r]);
         if (fac > 2.0) death_err = true;
         double idie = core ? 0.0 : attempt_die(s.t_real != 0, fac, fabs(sg[r]), ub);
@@ -526,12 +533,18 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
     bool rm = occ && !(any_left || core);
     if (occ) {
       for (int r = 0; r < S; ++r) wk.sgn(slot, r) = rm ? 0.0 : cp[r];
-      wk.flags(slot) = f;
+      wk.flags(slot) = rm ? (f | kFlagRemoved) : f;
+      if (rm && !rebuild) {  // RemoveHashDet: the entry becomes a tombstone
+        int64_t hp = wk.hpos(slot);
+        int32_t* sl = (int32_t*)(wk.htab + (size_t)(hp >> 2) * 8 * W + 4 * W);
+        sl[hp & 3] = kSlotTomb;
+      }
     }
-    remove_dets<W>(wk, &fs, rm, slot, !rebuild);
-    free_flush(wk, cnt, &fs, false);
+    // bit 3 of the first replica's byte: freed here; the attempt scan queues
+    // these slots in the free list (in slot order, no atomics here)
+    if (valid) pinfo[slot * S] = (uint8_t)(pi0 | (rm ? kPinfoFreed : 0));
+    my_rm += rm ? 1 : 0;
   }
-  free_flush(wk, cnt, &fs, true);
   if (death_err) atomicOr(&cnt->err, 1u << NGPU_ERR_DEATH_PROB);
   a_idets.done(bs, soff(F_NOINITDETS, 0));
   a_iwalk.done(bs, soff(F_NOINITWALK, 0));
@@ -546,6 +559,8 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
   int64_t tot;
   (void)block_exclusive_scan(my_att, red, tot);
   if (threadIdx.x == 0) blk_part[blockIdx.x] = tot;
+  (void)block_exclusive_scan(my_rm, red, tot);
+  if (threadIdx.x == 0) blk_rm[blockIdx.x] = tot;
   bs.flush(part);
 }
 
@@ -557,43 +572,85 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
 // cleared, before any lookup of the iteration.
 template <int W>
 __global__ void __launch_bounds__(kBlock) k_scan_apply(DevSys s, Walk wk, const int32_t* att_cnt, int64_t* att_off,
-                                                       const int64_t* part, Counters* cnt, int S,
+                                                       const int64_t* part, const int64_t* part_rm,
+                                                       const uint8_t* pinfo, Counters* cnt, int S,
                                                        int64_t* tile_p, int64_t tcap) {
   __shared__ int64_t red[kBlock / kWave];
   int64_t nslots = (int64_t)cnt->nslots;
   if (cnt->rebuild) {
     hash_refill_part<W>(s, wk, cnt, nslots);
+    // the prepass's removals left no tombstones: the table now holds the live
+    // slots only (the end-of-iteration accounting counts the later removals)
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-      // the prepass's removals left no tombstones; the table now holds the
-      // live slots only
-      cnt->n_live -= cnt->free_tail - cnt->free_tail0;
-      cnt->free_tail0 = cnt->free_tail;
       cnt->n_tomb = 0;
+      cnt->rebuilt = 1;
     }
   }
   int64_t begin, end;
   chunk_of(nslots, begin, end);
-  int64_t pb = begin * S, pe = end * S;
-  // this block's first offset: the attempt counts of the blocks before it
-  int64_t loc = 0;
-  for (int q = threadIdx.x; q < (int)blockIdx.x; q += blockDim.x) loc += part[q];
-  int64_t run;
+  // this block's first attempt offset and first free-queue position: the
+  // counts of the blocks before it.  The slots the prepass freed go, in slot
+  // order, after the free queue's tail (the prepass appends nothing itself,
+  // so the tail is still free_tail0, the value at the last accounting).
+  // With one replica both scans run as one over packed counts (attempts in
+  // the low 40 bits, freed slots above).
+  const unsigned long long tail0 = cnt->free_tail0;
+  constexpr int kPackShift = 40;
+  int64_t loc = 0, loc_rm = 0;
+  for (int q = threadIdx.x; q < (int)blockIdx.x; q += blockDim.x) {
+    loc += part[q];
+    loc_rm += part_rm[q];
+  }
+  int64_t run, run_rm;
   (void)block_exclusive_scan(loc, red, run);
-  for (int64_t base = pb; base < pe; base += blockDim.x) {
-    int64_t i = base + threadIdx.x;
-    int64_t v = i < pe ? att_cnt[i] : 0;
-    int64_t tot;
-    int64_t pre = block_exclusive_scan(v, red, tot);
-    if (i < pe) {
-      int64_t o = run + pre;
-      att_off[i] = o;
-      for (int64_t t = (o + kBlock - 1) / kBlock; t * kBlock < o + v && t < tcap; ++t) tile_p[t] = i;
+  (void)block_exclusive_scan(loc_rm, red, run_rm);
+  if (S == 1) {
+    for (int64_t b0 = begin; b0 < end; b0 += blockDim.x) {
+      int64_t i = b0 + threadIdx.x;
+      int64_t v = 0, f = 0;
+      if (i < end) {
+        v = att_cnt[i];
+        f = (pinfo[i] & kPinfoFreed) ? 1 : 0;
+      }
+      int64_t tot;
+      int64_t pre = block_exclusive_scan(v | (f << kPackShift), red, tot);
+      if (i < end) {
+        int64_t o = run + (pre & ((1ll << kPackShift) - 1));
+        att_off[i] = o;
+        for (int64_t t = (o + kBlock - 1) / kBlock; t * kBlock < o + v && t < tcap; ++t) tile_p[t] = i;
+        if (f) wk.freeq[(tail0 + (unsigned long long)(run_rm + (pre >> kPackShift))) % (unsigned long long)wk.cap] = i;
+      }
+      run += tot & ((1ll << kPackShift) - 1);
+      run_rm += tot >> kPackShift;
     }
-    run += tot;
+  } else {
+    for (int64_t b0 = begin; b0 < end; b0 += blockDim.x) {
+      int64_t slot = b0 + threadIdx.x;
+      int64_t f = (slot < end && (pinfo[slot * S] & kPinfoFreed)) ? 1 : 0;
+      int64_t tot;
+      int64_t pre = block_exclusive_scan(f, red, tot);
+      if (f) wk.freeq[(tail0 + (unsigned long long)(run_rm + pre)) % (unsigned long long)wk.cap] = slot;
+      run_rm += tot;
+    }
+    int64_t pb = begin * S, pe = end * S;
+    for (int64_t base = pb; base < pe; base += blockDim.x) {
+      int64_t i = base + threadIdx.x;
+      int64_t v = i < pe ? att_cnt[i] : 0;
+      int64_t tot;
+      int64_t pre = block_exclusive_scan(v, red, tot);
+      if (i < pe) {
+        int64_t o = run + pre;
+        att_off[i] = o;
+        for (int64_t t = (o + kBlock - 1) / kBlock; t * kBlock < o + v && t < tcap; ++t) tile_p[t] = i;
+      }
+      run += tot;
+    }
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
     att_off[nslots * S] = run;
     cnt->n_attempts = (unsigned long long)run;
+    cnt->free_tail = tail0 + (unsigned long long)run_rm;
+    cnt->rm_pre = (unsigned long long)run_rm;
   }
 }
 
@@ -1986,7 +2043,10 @@ __device__ inline void hash_done(Counters* cnt, int64_t entries) {
 __device__ inline void hash_account(Counters* cnt, int64_t entries) {
   unsigned long long removed = cnt->free_tail - cnt->free_tail0;
   cnt->n_live -= removed;
-  cnt->n_tomb += removed;
+  // after a rebuild in this iteration only the removals after the attempt
+  // scan's refill left tombstones; the refill zeroed n_tomb
+  cnt->n_tomb += cnt->rebuilt ? removed - cnt->rm_pre : removed;
+  cnt->rebuilt = 0;
   cnt->free_tail0 = cnt->free_tail;
   cnt->rebuild = hash_rebuild_due(cnt, entries) ? 1u : 0u;
   if (cnt->n_live * 10ull > (unsigned long long)entries * 8ull) cnt->err |= 1u << NGPU_ERR_HASH_FULL;
