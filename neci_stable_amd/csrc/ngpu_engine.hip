@@ -170,6 +170,9 @@ int err_from_mask(unsigned int m) {
   return NGPU_OK;
 }
 
+// the state an iteration starts from: statistics rows and append counters
+// empty.  Iterations leave it behind themselves (k_end_iter); this kernel
+// restores it after the set-up paths and starts an explicit-list annihilation.
 __global__ void k_begin_iter(ngpu_iter_out_t* out, double* part, Counters* cnt, IterParams* prm, IterParams v,
                              int write_params) {
   if (write_params && threadIdx.x == 0) *prm = v;  // the iteration's inputs (not in a captured graph)
@@ -194,7 +197,7 @@ __global__ void k_begin_iter(ngpu_iter_out_t* out, double* part, Counters* cnt, 
 // sum of the block-statistics rows, final counters of the iteration, then
 // accumulation into the cycle sums (ENumCyc, HFCyc, NoBorn, ... are cycle
 // accumulators in the reference too)
-__global__ void k_end_iter(ngpu_iter_out_t* out, const double* part, ngpu_iter_out_t* cyc, Counters* cnt,
+__global__ void k_end_iter(ngpu_iter_out_t* out, double* part, ngpu_iter_out_t* cyc, Counters* cnt,
                            int64_t scap, int64_t cap, int64_t hash_entries) {
   if (threadIdx.x == 0) {
     post_insert(cnt, cnt->n_new, cap);  // the slots k_insert filled
@@ -207,6 +210,7 @@ __global__ void k_end_iter(ngpu_iter_out_t* out, const double* part, ngpu_iter_o
     double v = 0.0;
     for (int q = 0; q < kStatCopies; ++q) {
       double x = part[(size_t)q * kStatStride + i];
+      part[(size_t)q * kStatStride + i] = 0.0;  // empty for the next iteration
       v = is_max ? (x > v ? x : v) : v + x;
     }
     if (i == X_NOCC) cnt->n_occ = (unsigned long long)v;
@@ -238,6 +242,13 @@ __global__ void k_end_iter(ngpu_iter_out_t* out, const double* part, ngpu_iter_o
     cyc->n_new += out->n_new;
     cyc->err |= out->err;
     cyc->pad += 1;  // iterations in the cycle
+    // the counters the next iteration appends to start from zero (the chain
+    // has no separate reset launch)
+    cnt->n_spawned = 0;
+    cnt->n_singles = 0;
+    cnt->n_new = 0;
+    cnt->n_fail = 0;
+    cnt->err = 0;
   }
 }
 
@@ -252,6 +263,7 @@ __global__ void k_reset_cycle(ngpu_iter_out_t* cyc) {
 }
 
 __global__ void k_set_count(unsigned long long* p, unsigned long long v) { *p = v; }
+__global__ void k_set_u32(unsigned int* p, unsigned int v) { *p = v; }
 
 // Kernel launch on the engine stream.  With profiling on, the launch that
 // opens stage `open` carries that stage's start event and the launch that
@@ -421,16 +433,14 @@ void annihilation_chain(ngpu_t* h, bool with_core, bool list_out = false) {
          (const unsigned long long*)&h->cnt->n_new);
   launch(h, 8, 8, k_stats<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
          (const IterParams*)h->dparams);
-  launch(h, -1, -1, k_end_iter, dim3(1), dim3(kBlock), h->dout, (const double*)h->dpart, h->dcyc, h->cnt,
+  launch(h, -1, -1, k_end_iter, dim3(1), dim3(kBlock), h->dout, h->dpart, h->dcyc, h->cnt,
          h->scap, h->wk.cap, h->wk.L * (int64_t)kBucketEntries);
 }
 
 template <int W, int R>
 void iterate_chain(ngpu_t* h) {
-  launch(h, -1, -1, k_begin_iter, dim3(1), dim3(kBlock), h->dout, h->dpart, h->cnt, h->dparams, h->next_params,
-         h->chain_sets_params ? 1 : 0);
   launch(h, 0, 0, k_prepass<W, R>, dim3(h->nb_chunk), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart, h->att_cnt,
-         h->pinfo, h->blk_part, h->blk_rm, (const IterParams*)h->dparams);
+         h->pinfo, h->blk_part, h->blk_rm, h->dparams, h->next_params, h->chain_sets_params ? 1 : 0);
   if (h->ncore) core_projection<R>(h);
   launch(h, 1, 1, k_scan_apply<W>, dim3(h->nb_chunk), dim3(kBlock), h->ds, h->wk, (const int32_t*)h->att_cnt,
          h->att_off, (const int64_t*)h->blk_part, (const int64_t*)h->blk_rm, (const uint8_t*)h->pinfo, h->cnt,
@@ -1077,7 +1087,7 @@ int ngpu_upload_walkers(ngpu_t* h, const int64_t* rows, int64_t n) {
   rc = upload_rows_to_new_list(h, rows, n);
   if (rc) return rc;
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_new, (unsigned long long)n);
-  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, (unsigned long long*)&h->cnt->err, 0ull);
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, h->stream, &h->cnt->err, 0u);
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_fail, 0ull);
   {
     auto kern = h->W == 1 ? (h->S == 1 ? k_insert<1, 1> : k_insert<1, kMaxRep>)
@@ -1090,6 +1100,10 @@ int ngpu_upload_walkers(ngpu_t* h, const int64_t* rows, int64_t n) {
   else hash_upkeep<2>(h);
   HIP_OK(hipMemcpyAsync(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
+  // iterations start from empty append counters and statistics rows
+  hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dpart, h->cnt, h->dparams,
+                     IterParams{}, 0);
+  HIP_OK(hipGetLastError());
   return err_from_mask(h->hcnt->err);
 }
 
@@ -1129,7 +1143,7 @@ int set_core_impl(ngpu_t* h, int64_t ncore, bool fp32, int64_t* nnz_out) {
   uint32_t cflags = kFlagCore | (h->sys.t_trunc_initiator ? ((1u << S) - 1u) : 0u);
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_new, 0ull);
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_fail, 0ull);
-  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, (unsigned long long*)&h->cnt->err, 0ull);
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, h->stream, &h->cnt->err, 0u);
   dim3 gp((unsigned)std::min<int64_t>((ncore + kBlock - 1) / kBlock, h->nb_grid));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_core_place<W>), gp, dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
                      (const uint64_t*)h->core_keys, ncore, cflags, h->core_slot, h->n_key, h->n_sgn, h->n_flag);
@@ -1144,6 +1158,10 @@ int set_core_impl(ngpu_t* h, int64_t ncore, bool fp32, int64_t* nnz_out) {
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
+  // iterations start from empty append counters and statistics rows
+  hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dpart, h->cnt, h->dparams,
+                     IterParams{}, 0);
+  HIP_OK(hipGetLastError());
   if (h->hcnt->err) return err_from_mask(h->hcnt->err);
   h->core_nnz = nnz;
   if (nnz_out) *nnz_out = nnz;

@@ -425,10 +425,13 @@ template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters* cnt,
                                                     double* part, int32_t* att_cnt,
                                                     uint8_t* pinfo, int64_t* blk_part, int64_t* blk_rm,
-                                                    const IterParams* prm) {
-  const double tau = prm->tau;
-  const uint32_t iter = prm->iter;
-  const double* sftv = prm->sft.v;
+                                                    IterParams* prm, IterParams v, int use_v) {
+  // the iteration's inputs: from the launch when launches are issued one by
+  // one (block 0 publishes them for the later kernels), from device memory
+  // in a replayed graph
+  const double tau = use_v ? v.tau : prm->tau;
+  const uint32_t iter = use_v ? v.iter : prm->iter;
+  if (use_v && blockIdx.x == 0 && threadIdx.x == 0) *prm = v;
   __shared__ double acc_lds[kStatStride];
   __shared__ int64_t red[kBlock / kWave];
   BlockStats bs{acc_lds};
@@ -502,7 +505,7 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
         uint8_t pi = (uint8_t)((sg[r] < 0.0 ? 1 : 0) | (((f >> r) & 1u) ? 2 : 0) | (core ? 4 : 0));
         if (r == 0) pi0 = pi;  // written with the removal bit below
         else pinfo[slot * S + r] = pi;
-        double fac = tau * (kii - sftv[r]);
+        double fac = tau * (kii - (use_v ? v.sft.v[r] : prm->sft.v[r]));
         if (fac > 2.0) death_err = true;
         double idie = core ? 0.0 : attempt_die(s.t_real != 0, fac, fabs(sg[r]), ub);
         double a = fabs(sg[r]);
