@@ -199,9 +199,27 @@ __global__ void k_begin_iter(ngpu_iter_out_t* out, double* part, Counters* cnt, 
 // accumulators in the reference too)
 __global__ void k_end_iter(ngpu_iter_out_t* out, double* part, ngpu_iter_out_t* cyc, Counters* cnt,
                            int64_t scap, int64_t cap, int64_t hash_entries) {
-  if (threadIdx.x == 0) {
-    post_insert(cnt, cnt->n_new, cap);  // the slots k_insert filled
-    hash_account(cnt, hash_entries);
+  __shared__ unsigned long long n_unique;
+  if (threadIdx.x < kWave) {
+    // the spread per-block counts, one line per lane, summed across the wave
+    const int l = threadIdx.x;
+    long long r = 0, u = 0;
+    if (l < Counters::kSpread) {
+      r = (long long)cnt->reuse_part[l].v;
+      u = (long long)cnt->uniq_part[l].v;
+      cnt->reuse_part[l].v = 0;
+      cnt->uniq_part[l].v = 0;
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+      r += __shfl_xor(r, o);
+      u += __shfl_xor(u, o);
+    }
+    if (l == 0) {
+      post_insert(cnt, cnt->n_new, cap, (unsigned long long)r);  // the slots k_insert filled
+      hash_account(cnt, hash_entries);
+      n_unique = (unsigned long long)u;
+    }
   }
   double* g = (double*)out;
   double* c = (double*)cyc;
@@ -226,11 +244,7 @@ __global__ void k_end_iter(ngpu_iter_out_t* out, double* part, ngpu_iter_out_t* 
     out->n_attempts = (int64_t)cnt->n_attempts;
     int64_t ns = (int64_t)cnt->n_spawned;
     out->n_spawned = ns < scap ? ns : scap;
-    unsigned long long nu = 0;
-    for (int q = 0; q < Counters::kSpread; ++q) {
-      nu += cnt->uniq_part[q].v;
-      cnt->uniq_part[q].v = 0;
-    }
+    const unsigned long long nu = n_unique;
     cnt->n_unique = nu;
     out->n_unique = (int64_t)nu;
     out->n_new = (int64_t)cnt->n_new;

@@ -1907,21 +1907,23 @@ __device__ inline void slots_after_insert(const Counters* cnt, unsigned long lon
   ns = cnt->nslots + (nn - used);
   if (ns > (unsigned long long)cap) ns = (unsigned long long)cap;
 }
-__device__ inline void post_insert(Counters* cnt, unsigned long long nn, int64_t cap) {
+// reused: tombstone entries the inserts took (the sum of reuse_part, which
+// the caller has emptied)
+__device__ inline void post_insert(Counters* cnt, unsigned long long nn, int64_t cap, unsigned long long reused) {
   unsigned long long used, ns;
   slots_after_insert(cnt, nn, cap, used, ns);
   cnt->free_head += used;
   cnt->nslots = ns;
   cnt->n_live += nn - cnt->n_fail;
+  cnt->n_tomb -= reused;
+}
+__global__ void k_post_insert(Counters* cnt, const unsigned long long* n_ptr, int64_t cap) {
   unsigned long long reused = 0;
   for (int q = 0; q < Counters::kSpread; ++q) {
     reused += cnt->reuse_part[q].v;
     cnt->reuse_part[q].v = 0;
   }
-  cnt->n_tomb -= reused;
-}
-__global__ void k_post_insert(Counters* cnt, const unsigned long long* n_ptr, int64_t cap) {
-  post_insert(cnt, *n_ptr, cap);
+  post_insert(cnt, *n_ptr, cap, reused);
 }
 
 // CalcHashTableStats (load_balancer.fpp:646-805)
