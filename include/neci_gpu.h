@@ -245,6 +245,13 @@ int ngpu_get_stats(ngpu_t *h, ngpu_iter_out_t *out);
    inst_no_at_hf) plus the last iteration's, and start a new cycle. */
 int ngpu_get_cycle_stats(ngpu_t *h, ngpu_iter_out_t *sum, ngpu_iter_out_t *last, int32_t *n_iters);
 
+/* Walker-hash diagnostics (waits for queued work): live entries, tombstones
+   (entries of removed determinants not yet reclaimed), total entries (4 per
+   bucket) and the number of in-place rebuilds since creation.  No counterpart
+   in the reference, whose chained buckets (hash.F90) have no tombstones; any
+   argument may be NULL. */
+int ngpu_get_hash_stats(ngpu_t *h, int64_t *live, int64_t *tombstones, int64_t *entries, int64_t *rebuilds);
+
 /* Stage timing with events on the engine stream (set_timer/halt_timer analogue,
    lib/timing.F90:122-188).  Each stage is timed from the start of its first
    kernel to the end of its last.  Stages: 0 prepass (initiator, proj-E, death),

@@ -1572,6 +1572,17 @@ int ngpu_get_stats(ngpu_t* h, ngpu_iter_out_t* out) {
   return finish(h, out);
 }
 
+int ngpu_get_hash_stats(ngpu_t* h, int64_t* live, int64_t* tombstones, int64_t* entries, int64_t* rebuilds) {
+  if (!h) return NGPU_ERR_ARG;
+  HIP_OK(hipStreamSynchronize(h->stream));
+  HIP_OK(hipMemcpy(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost));
+  if (live) *live = (int64_t)h->hcnt->n_live;
+  if (tombstones) *tombstones = (int64_t)h->hcnt->n_tomb;
+  if (entries) *entries = h->wk.L * (int64_t)kBucketEntries;
+  if (rebuilds) *rebuilds = (int64_t)h->hcnt->n_rebuilds;
+  return NGPU_OK;
+}
+
 int ngpu_get_cycle_stats(ngpu_t* h, ngpu_iter_out_t* sum, ngpu_iter_out_t* last, int32_t* n_iters) {
   if (!h) return NGPU_ERR_ARG;
   HIP_OK(hipMemcpyAsync(h->hout, h->dout, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));

@@ -68,6 +68,7 @@ struct alignas(256) Counters {
   // scan refills it)
   unsigned int rebuild;
   unsigned int rebuilt;  // set by the attempt scan that refilled the table
+  unsigned long long n_rebuilds;  // walker-hash rebuilds since creation
   alignas(256) unsigned long long free_tail;
   alignas(256) unsigned long long n_spawned;
   alignas(256) unsigned long long n_singles;  // queued single-excitation attempts
@@ -587,6 +588,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(DevSys s, Walk wk, const 
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       cnt->n_tomb = 0;
       cnt->rebuilt = 1;
+      cnt->n_rebuilds += 1;
     }
   }
   int64_t begin, end;
@@ -2034,6 +2036,7 @@ __device__ inline bool hash_rebuild_due(const Counters* cnt, int64_t entries) {
 // by k_hash_clear/k_hash_refill (one thread; set-up paths)
 __device__ inline void hash_done(Counters* cnt, int64_t entries) {
   bool rebuilt = hash_rebuild_due(cnt, entries);
+  if (rebuilt) cnt->n_rebuilds += 1;
   unsigned long long removed = cnt->free_tail - cnt->free_tail0;
   cnt->n_live -= removed;
   cnt->n_tomb = rebuilt ? 0ull : cnt->n_tomb + removed;

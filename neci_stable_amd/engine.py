@@ -156,6 +156,12 @@ class Engine:
         self.ncore = dets.shape[0]
         return nnz.value
 
+    def hash_stats(self):
+        """Walker-hash diagnostics: dict(live, tombstones, entries, rebuilds)."""
+        v = [ctypes.c_int64() for _ in range(4)]
+        check(self.lib.ngpu_get_hash_stats(self.h, *[ctypes.byref(x) for x in v]), "ngpu_get_hash_stats")
+        return dict(live=v[0].value, tombstones=v[1].value, entries=v[2].value, rebuilds=v[3].value)
+
     def set_spawn_precision(self, fp32):
         """Spawned amplitudes stored in FP32 (True) or FP64 (ngpu_set_spawn_precision)."""
         check(self.lib.ngpu_set_spawn_precision(self.h, int(bool(fp32))), "ngpu_set_spawn_precision")

@@ -329,21 +329,32 @@ def run_iterations(name, nrep, real, spawn_fp32=False):
 
 def test_hash_rebuild_keeps_parity():
     """A walker hash small enough that tombstones trigger in-place rebuilds:
-    the walker set must still follow the oracle exactly."""
+    the walker set must still follow the oracle exactly, rebuilds do happen,
+    and after every iteration the live-entry count equals the number of
+    determinants in the table, with the tombstones within the entries.
+    110 buckets = 440 entries: growth at shift 0.6 fills ~62 % of them (new
+    determinants mostly reuse the tombstones that removals leave); a shift of
+    -30 then makes deaths outrun births, so tombstones pass 10 % of the
+    entries while live plus tombstone entries stay above 60 %."""
     tau = 0.01
-    # 120 buckets = 480 entries: ~280 live determinants after 32 iterations (below the
-    # 80% guard) while ~110 removals leave enough tombstones to pass the rebuild trigger
-    spec = SPECS["hubbard10"](init_walkers=4000, seed=5, n_hash_buckets=120)
+    spec = SPECS["hubbard10"](init_walkers=4000, seed=5, n_hash_buckets=110)
     eng = engine(spec)
     orc = oracle(spec, tau)
     eng.start_single_part(40.0)
     orc.add_walkers(orc.ref[None, :], np.full((1, 1), 40.0), [1])
-    for it in range(1, 33):
-        a = eng.iterate(it, tau, 0.6)
-        b = orc.iterate(it, 0.6)
+    for it in range(1, 41):
+        shift = 0.6 if it <= 31 else -30.0
+        a = eng.iterate(it, tau, shift)
+        b = orc.iterate(it, shift)
         assert b["rc"] == 0
         assert_stats(a, b, 1)
-    assert assert_same_walkers(eng, orc) > 200
+        hs = eng.hash_stats()
+        assert hs["live"] == a["tot_walkers"], (it, hs, a["tot_walkers"])
+        assert 0 <= hs["tombstones"] and hs["live"] + hs["tombstones"] <= hs["entries"], (it, hs)
+    hs = eng.hash_stats()
+    print(f"hash after 40 iterations: {hs}", flush=True)
+    assert hs["rebuilds"] >= 1, hs
+    assert assert_same_walkers(eng, orc) > 20
 
 
 def test_hash_full_is_an_error():
