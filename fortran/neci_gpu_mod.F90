@@ -46,7 +46,7 @@ module neci_gpu_mod
     public :: ngpu_iterate, ngpu_get_stats, ngpu_get_cycle_stats, ngpu_find_walker_hash_batch
     public :: ngpu_diag_helement_batch, ngpu_set_excitgen_uniform, ngpu_set_core
     public :: ngpu_set_graph, ngpu_batch_create, ngpu_batch_iterate, ngpu_batch_destroy
-    public :: ngpu_fcidump_header, ngpu_read_fcidump, ngpu_set_spawn_precision
+    public :: ngpu_fcidump_header, ngpu_read_fcidump, ngpu_set_spawn_precision, ngpu_get_hash_stats
 
     interface
         integer(c_int) function ngpu_create(h, sys) bind(c, name="ngpu_create")
@@ -91,6 +91,12 @@ module neci_gpu_mod
             import :: c_int, c_ptr, c_int32_t
             type(c_ptr), value :: h
             integer(c_int32_t), value :: fp32
+        end function
+        integer(c_int) function ngpu_get_hash_stats(h, live, tombstones, entries, rebuilds) &
+            bind(c, name="ngpu_get_hash_stats")
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: h
+            integer(c_int64_t), intent(out) :: live, tombstones, entries, rebuilds
         end function
         integer(c_int) function ngpu_set_core(h, dets, ncore, fp32_values, nnz) bind(c, name="ngpu_set_core")
             import :: c_int, c_ptr, c_int64_t, c_int32_t
