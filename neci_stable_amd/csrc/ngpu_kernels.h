@@ -1,17 +1,21 @@
 // ngpu_kernels.h -- kernels of one FCIQMC iteration on the GPU.
 //
 // Iteration chain (all on the engine stream, no host round trip):
-//   k_prepass   per slot: initiator flag, proj-E sums, number of spawn
-//               attempts, death (fused: spawning only needs the pre-death sign
-//               bit and initiator bit, saved in pinfo)
-//   k_scan_*    exclusive scan of attempt counts -> attempt offsets
-//   k_spawn     per attempt: parent by binary search, excitation, H_ij,
-//               stochastic rounding, wave-aggregated append of spawn records
-//   k_sort_*    stable LSD radix sort of (key, spawn index), ilut_lt order
-//   k_merge     per run of equal keys: CompressSpawnedList/FindResidualParticle
-//   k_annihilate per unique spawned det: hash lookup, add/abort/remove
-//   k_insert    new dets: slot from the free queue, bucketed open-addressing insert
-//   k_stats     per slot: occupied-threshold rounding, TotParts, norm, HF pop
+//   k_prepass    per slot: initiator flag, proj-E sums, number of spawn
+//                attempts, death (fused: spawning only needs the pre-death
+//                sign bit and initiator bit, saved in pinfo); marks the slots
+//                it frees; clears the walker hash when a rebuild is due
+//   k_scan_apply exclusive scan of attempt counts -> attempt offsets; the
+//                freed slots into the free queue; the hash refill of a rebuild
+//   k_spawn      per attempt: parent by binary search, excitation, H_ij,
+//                stochastic rounding, LDS-staged append of spawn records;
+//                heat-bath singles are queued for k_spawn_singles
+//   k_radix_*    stable LSD radix sort of (colex rank or ilut_lt key, spawn index)
+//   k_merge_annihilate per run of equal keys: CompressSpawnedList /
+//                FindResidualParticle, then hash lookup, add/abort/remove
+//   k_insert     new dets: slot from the free queue, bucketed open-addressing insert
+//   k_stats      per slot: occupied-threshold rounding, TotParts, norm, HF pop
+//   (k_end_iter in the engine: slot and hash bookkeeping, statistics sums)
 #pragma once
 #include "ngpu_device.h"
 #include "../../include/neci_gpu.h"
@@ -245,18 +249,15 @@ struct FreeStage {
 // tail.  (Releasing the entry as empty where the
 // bucket already holds an empty entry would also be valid -- such a bucket
 // ends every probe -- but it makes each removal read its bucket and, measured,
-// cost more in the removing kernels than the rebuilds it saves.)
-// tomb = false: the table is being cleared for a rebuild, so the entry is left
-// alone (the rebuild reinserts live slots only)
+// cost more in the removing kernels than the rebuilds it saves.)  The prepass
+// removes without this helper: it marks the slot, the attempt scan queues it.
 template <int W>
-__device__ inline void remove_dets(const Walk& wk, FreeStage* fs, bool rm, int64_t slot, bool tomb = true) {
+__device__ inline void remove_dets(const Walk& wk, FreeStage* fs, bool rm, int64_t slot) {
   int q = stage_push(rm, &fs->st);
   if (rm) {
-    if (tomb) {
-      int64_t hp = wk.hpos(slot);
-      int32_t* sl = (int32_t*)(wk.htab + (size_t)(hp >> 2) * 8 * W + 4 * W);
-      sl[hp & 3] = kSlotTomb;
-    }
+    int64_t hp = wk.hpos(slot);
+    int32_t* sl = (int32_t*)(wk.htab + (size_t)(hp >> 2) * 8 * W + 4 * W);
+    sl[hp & 3] = kSlotTomb;
     wk.flags(slot) |= kFlagRemoved;
     fs->slot[q] = slot;
   }
