@@ -1,0 +1,70 @@
+"""Host logic of bench.py (CPU only): the spawn-sort key and the compulsory
+byte model behind the roofline figures, the kernel-trace summary, and the
+refusal of more than one GPU."""
+import csv
+import json
+import math
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+COUNTS = dict(Nd=5.45e5, Na=9.97e5, Ns=5.46e5, Nu=5.0e5, Nn=1.0e5)
+
+
+@pytest.mark.parametrize("nbasis,nel,expect", [
+    (40, 10, (4, 4)),    # C(40,10) = 847 660 528 < 2^32: 30-bit colex ranks, 4 passes
+    (20, 10, (4, 3)),    # C(20,10) = 184 756: 18 bits
+    (36, 18, (8, 5)),    # C(36,18) > 2^32: determinant word, 36 bits in 5 passes
+    (80, 26, (16, 10)),  # two words: 16 bits of the second (2 passes) + 8 passes of the first
+    (60, 14, (8, 8)),    # C(60,14) > 2^32, one word
+])
+def test_sort_key_matches_the_engine_rule(nbasis, nel, expect):
+    assert bench.sort_key(nbasis, nel) == expect
+    if expect[0] == 4:
+        assert math.comb(nbasis, nel) <= 2 ** 32
+        assert 2 ** (8 * expect[1] - 8) < math.comb(nbasis, nel) <= 2 ** (8 * expect[1])
+
+
+def test_sort_and_iteration_bytes():
+    kb, passes = bench.sort_key(40, 10)
+    assert bench.algorithmic_bytes("sort", COUNTS, nel=10) == passes * COUNTS["Ns"] * (kb + 2 * (kb + 4))
+    stages = sum(bench.algorithmic_bytes(s, COUNTS, nel=10) for s in bench.STAGES if s != "sort")
+    assert bench.iteration_bytes(COUNTS, nel=10) == pytest.approx(stages + 2 * COUNTS["Ns"] * (kb + 4))
+    # the merge is fused into the annihilation stage
+    assert bench.algorithmic_bytes("merge", COUNTS) == 0.0
+    # prepass: 53 bytes per determinant for one word and one replica
+    assert bench.algorithmic_bytes("prepass", COUNTS) == pytest.approx(53 * COUNTS["Nd"])
+
+
+def test_summarize_trace(tmp_path, capsys):
+    """Per-iteration windows end at k_end_iter; averages over the last ones."""
+    path = tmp_path / "trace.csv"
+    rows = []
+    t = 0
+    for it in range(4):
+        for name, dur in (("void ngpu::k_spawn<1, 1, 1>(ngpu::DevSys)", 50_000 + it),
+                          ("void ngpu::k_end_iter(ngpu_iter_out_t*)", 5_000)):
+            rows.append({"Kernel_Name": name, "Start_Timestamp": t, "End_Timestamp": t + dur})
+            t += dur + 1000
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        w.writeheader()
+        w.writerows(rows)
+    bench.summarize_trace(str(path), 2)
+    out = capsys.readouterr().out
+    assert "`k_spawn<1, 1, 1>` | 1 | 50.0 |" in out  # (50002 + 50003) / 2 ns
+    assert "`k_end_iter` | 1 | 5.0 |" in out
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert msg["n_gpus"] == 2 and "one GPU" in msg["error"]
