@@ -1089,15 +1089,15 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
     uint64_t key[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) key[w] = act ? wk.keys[slot * W + w] : s.ref[w];  // idle groups: any valid det
-    // lane q % G draws Philox block q+1 (q = 0, 1, 2); d1 = dsel (purpose 1),
-    // d5 = spawn rounding (purpose 2, second value), d2 = dorb (purpose 3, first)
-    double ua[3] = {0.0, 0.0, 0.0}, ub[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int q3 = 0; q3 < 3; ++q3)
-      if (q3 % G == g) draw2<W>(s.seed, key, iter, r, k, (uint32_t)(q3 + 1), ua[q3], ub[q3]);
-    const double d1 = G == 1 ? ub[0] : __shfl(ub[0], gbase + 0 % G);
-    const double d5 = G == 1 ? ub[1] : __shfl(ub[1], gbase + 1 % G);
-    const double d2 = G == 1 ? ua[2] : __shfl(ua[2], gbase + 2 % G);
+    // lane g < 3 draws Philox block g+1, all three in one pass of the wave;
+    // d1 = dsel (purpose 1), d5 = spawn rounding (purpose 2, second value),
+    // d2 = dorb (purpose 3, first)
+    static_assert(G >= 3, "three lanes of a group draw the attempt's Philox blocks");
+    double ua = 0.0, ub = 0.0;
+    if (g < 3) draw2<W>(s.seed, key, iter, r, k, (uint32_t)(g + 1), ua, ub);
+    const double d1 = __shfl(ub, gbase + 0);
+    const double d5 = __shfl(ub, gbase + 1);
+    const double d2 = __shfl(ua, gbase + 2);
     int elec = 1 + (int)floor(d1 * (double)s.nel);
     int src = select_orb<W>(key, ~0ull, elec);
     int a = spat(src), off = is_beta(src) ? 1 : 0;
