@@ -1080,124 +1080,128 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
   const int64_t grp = (int64_t)blockIdx.x * kGroups + gl;
   const int64_t trips = (nq + groups - 1) / groups;
   for (int64_t t = 0; t < trips; ++t) {
-    int64_t q = t * groups + grp;
-    bool act = q < nq;
-    int64_t p = act ? sq[2 * q] : 0;
-    uint32_t k = act ? (uint32_t)sq[2 * q + 1] : 0u;
-    int64_t slot = p / S;
-    int r = (int)(p - slot * S);
-    uint64_t key[W];
+    // blocks whose groups are all past the queue skip the attempt (block-
+    // uniform, so the body's barrier is reached by all of the block or none)
+    if (t * groups + (int64_t)blockIdx.x * kGroups < nq) {
+      int64_t q = t * groups + grp;
+      bool act = q < nq;
+      int64_t p = act ? sq[2 * q] : 0;
+      uint32_t k = act ? (uint32_t)sq[2 * q + 1] : 0u;
+      int64_t slot = p / S;
+      int r = (int)(p - slot * S);
+      uint64_t key[W];
 #pragma unroll
-    for (int w = 0; w < W; ++w) key[w] = act ? wk.keys[slot * W + w] : s.ref[w];  // idle groups: any valid det
-    // lane g < 3 draws Philox block g+1, all three in one pass of the wave;
-    // d1 = dsel (purpose 1), d5 = spawn rounding (purpose 2, second value),
-    // d2 = dorb (purpose 3, first)
-    static_assert(G >= 3, "three lanes of a group draw the attempt's Philox blocks");
-    double ua = 0.0, ub = 0.0;
-    if (g < 3) draw2<W>(s.seed, key, iter, r, k, (uint32_t)(g + 1), ua, ub);
-    const double d1 = __shfl(ub, gbase + 0);
-    const double d5 = __shfl(ub, gbase + 1);
-    const double d2 = __shfl(ua, gbase + 2);
-    int elec = 1 + (int)floor(d1 * (double)s.nel);
-    int src = select_orb<W>(key, ~0ull, elec);
-    int a = spat(src), off = is_beta(src) ? 1 : 0;
-    const double* row = s.jk + (size_t)2 * (a - 1) * nsp * nsp;
-    double v[PL];
+      for (int w = 0; w < W; ++w) key[w] = act ? wk.keys[slot * W + w] : s.ref[w];  // idle groups: any valid det
+      // lane g < 3 draws Philox block g+1, all three in one pass of the wave;
+      // d1 = dsel (purpose 1), d5 = spawn rounding (purpose 2, second value),
+      // d2 = dorb (purpose 3, first)
+      static_assert(G >= 3, "three lanes of a group draw the attempt's Philox blocks");
+      double ua = 0.0, ub = 0.0;
+      if (g < 3) draw2<W>(s.seed, key, iter, r, k, (uint32_t)(g + 1), ua, ub);
+      const double d1 = __shfl(ub, gbase + 0);
+      const double d5 = __shfl(ub, gbase + 1);
+      const double d2 = __shfl(ua, gbase + 2);
+      int elec = 1 + (int)floor(d1 * (double)s.nel);
+      int src = select_orb<W>(key, ~0ull, elec);
+      int a = spat(src), off = is_beta(src) ? 1 : 0;
+      const double* row = s.jk + (size_t)2 * (a - 1) * nsp * nsp;
+      double v[PL];
 #pragma unroll
-    for (int m = 0; m < PL; ++m) v[m] = 0.0;
-    uint64_t same = off ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
-    // Coulomb terms over every occupied orbital but src, then exchange over the same-spin ones
+      for (int m = 0; m < PL; ++m) v[m] = 0.0;
+      uint64_t same = off ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
+      // Coulomb terms over every occupied orbital but src, then exchange over the same-spin ones
 #pragma unroll
-    for (int w = 0; w < W; ++w) {
-      uint64_t x = key[w];
-      if ((src - 1) >> 6 == w) x &= ~(1ull << ((src - 1) & 63));
-      hb_rows<PL, G>(x, 64 * w, row, nsp, g, mcount, 1.0, v);
-    }
-#pragma unroll
-    for (int w = 0; w < W; ++w) {
-      uint64_t x = key[w] & same;
-      if ((src - 1) >> 6 == w) x &= ~(1ull << ((src - 1) & 63));
-      hb_rows<PL, G>(x, 64 * w, row + 1, nsp, g, mcount, -1.0, v);
-    }
-#pragma unroll
-    for (int m = 0; m < PL; ++m) {
-      int b = g + m * G + 1;
-      if (m < mcount && b <= nsp) {
-        int orb = 2 * b - off;
-        myv[b - 1] = test_orb<W>(key, orb) ? 0.0 : v[m] + T(s, src, orb);
+      for (int w = 0; w < W; ++w) {
+        uint64_t x = key[w];
+        if ((src - 1) >> 6 == w) x &= ~(1ull << ((src - 1) & 63));
+        hb_rows<PL, G>(x, 64 * w, row, nsp, g, mcount, 1.0, v);
       }
-    }
-    __syncthreads();
-    // sweep 1: normalisation; sweep 2: the first b whose running sum reaches r
-    double cum_sum = 0.0;
-    for (int b = 0; b < nsp; ++b) cum_sum += fabs(myv[b]);
-    double rr = d2 * cum_sum;
-    double cum = 0.0, cpt = 0.0, vsel = 0.0;
-    int bsel = 0;
-    for (int b = 1; b <= nsp; ++b) {
-      double vb = myv[b - 1];
-      vsel = vb;
-      cpt = fabs(vb);
-      bsel = b;
-      cum += cpt;
-      if (!(cum < rr) || b == nsp) break;
-    }
-    bool mine_att = act && leader;
-    bool emit = false;
-    uint64_t child[W];
-    double nspawn = 0.0, an = 0.0, raw = 0.0, valid = 0.0, invalid = 0.0;
-    uint32_t info = 0;
-    bool cancel = false;
-    if (mine_att && s.ncore && (pinfo[p] & 4) && !(fabs(cum_sum) <= kEps || cpt == 0.0)) {
-      Excit ex{};
-      ex.ic = 1;
-      ex.e0 = src;
-      ex.e1 = 2 * bsel - off;
-      cancel = core_child<W>(s, wk, cnt, key, ex);  // from and to the core space
-    }
-    if (mine_att) {
-      if (fabs(cum_sum) <= kEps || cpt == 0.0) {
-        invalid = 1.0;
-      } else if (cancel) {
-        valid = 1.0;
-      } else {
-        valid = 1.0;
-        int tgt = 2 * bsel - off;
-        double pg = cpt / cum_sum;
-        pg = pg / (double)s.nel;
-        pg = pg * s.psingles;
-        double hel = single_parity<W>(key, src, tgt) ? -vsel : vsel;
-        uint8_t pi = pinfo[p];
-        double wsign = (pi & 1) ? -1.0 : 1.0;
-        nspawn = attempt_create(s, tau, hel, pg, wsign, d5, raw);
-        raw = fabs(raw);
-        if (fabs(nspawn) > kEps) {
-          emit = true;
-          an = fabs(nspawn);
 #pragma unroll
-          for (int w = 0; w < W; ++w) child[w] = key[w];
-          flip_orb<W>(child, src);
-          flip_orb<W>(child, tgt);
-          info = (uint32_t)r | ((s.t_init && (pi & 2)) ? kInfoInit : 0u);
+      for (int w = 0; w < W; ++w) {
+        uint64_t x = key[w] & same;
+        if ((src - 1) >> 6 == w) x &= ~(1ull << ((src - 1) & 63));
+        hb_rows<PL, G>(x, 64 * w, row + 1, nsp, g, mcount, -1.0, v);
+      }
+#pragma unroll
+      for (int m = 0; m < PL; ++m) {
+        int b = g + m * G + 1;
+        if (m < mcount && b <= nsp) {
+          int orb = 2 * b - off;
+          myv[b - 1] = test_orb<W>(key, orb) ? 0.0 : v[m] + T(s, src, orb);
         }
       }
-    }
-    acc.invalid.add(bs, S_NINVALID, invalid);
-    acc.valid.add(bs, S_NVALID, valid);
-    acc.maxspawn.add(bs, S_MAXSPAWN, raw);
-    for (int rr2 = 0; rr2 < S; ++rr2) {
-      acc.born.add(bs, soff(F_NBORN, rr2), rr2 == r ? an : 0.0);
-      acc.sing.add(bs, soff(F_SPAWNSING, rr2), rr2 == r ? an : 0.0);
-    }
-    bool bloom = emit && an > s.init_walk_no;
-    acc.bloomc1.add(bs, S_BLOOMC + 1, bloom ? 1.0 : 0.0);
-    acc.blooms1.add(bs, S_BLOOMS + 1, bloom ? an : 0.0);
-    int idx = stage_push(emit, &ss.st);
-    if (emit) {
+      __syncthreads();
+      // sweep 1: normalisation; sweep 2: the first b whose running sum reaches r
+      double cum_sum = 0.0;
+      for (int b = 0; b < nsp; ++b) cum_sum += fabs(myv[b]);
+      double rr = d2 * cum_sum;
+      double cum = 0.0, cpt = 0.0, vsel = 0.0;
+      int bsel = 0;
+      for (int b = 1; b <= nsp; ++b) {
+        double vb = myv[b - 1];
+        vsel = vb;
+        cpt = fabs(vb);
+        bsel = b;
+        cum += cpt;
+        if (!(cum < rr) || b == nsp) break;
+      }
+      bool mine_att = act && leader;
+      bool emit = false;
+      uint64_t child[W];
+      double nspawn = 0.0, an = 0.0, raw = 0.0, valid = 0.0, invalid = 0.0;
+      uint32_t info = 0;
+      bool cancel = false;
+      if (mine_att && s.ncore && (pinfo[p] & 4) && !(fabs(cum_sum) <= kEps || cpt == 0.0)) {
+        Excit ex{};
+        ex.ic = 1;
+        ex.e0 = src;
+        ex.e1 = 2 * bsel - off;
+        cancel = core_child<W>(s, wk, cnt, key, ex);  // from and to the core space
+      }
+      if (mine_att) {
+        if (fabs(cum_sum) <= kEps || cpt == 0.0) {
+          invalid = 1.0;
+        } else if (cancel) {
+          valid = 1.0;
+        } else {
+          valid = 1.0;
+          int tgt = 2 * bsel - off;
+          double pg = cpt / cum_sum;
+          pg = pg / (double)s.nel;
+          pg = pg * s.psingles;
+          double hel = single_parity<W>(key, src, tgt) ? -vsel : vsel;
+          uint8_t pi = pinfo[p];
+          double wsign = (pi & 1) ? -1.0 : 1.0;
+          nspawn = attempt_create(s, tau, hel, pg, wsign, d5, raw);
+          raw = fabs(raw);
+          if (fabs(nspawn) > kEps) {
+            emit = true;
+            an = fabs(nspawn);
 #pragma unroll
-      for (int w = 0; w < W; ++w) ss.key[idx * W + w] = child[w];
-      ss.val[idx] = nspawn;
-      ss.info[idx] = info;
+            for (int w = 0; w < W; ++w) child[w] = key[w];
+            flip_orb<W>(child, src);
+            flip_orb<W>(child, tgt);
+            info = (uint32_t)r | ((s.t_init && (pi & 2)) ? kInfoInit : 0u);
+          }
+        }
+      }
+      acc.invalid.add(bs, S_NINVALID, invalid);
+      acc.valid.add(bs, S_NVALID, valid);
+      acc.maxspawn.add(bs, S_MAXSPAWN, raw);
+      for (int rr2 = 0; rr2 < S; ++rr2) {
+        acc.born.add(bs, soff(F_NBORN, rr2), rr2 == r ? an : 0.0);
+        acc.sing.add(bs, soff(F_SPAWNSING, rr2), rr2 == r ? an : 0.0);
+      }
+      bool bloom = emit && an > s.init_walk_no;
+      acc.bloomc1.add(bs, S_BLOOMC + 1, bloom ? 1.0 : 0.0);
+      acc.blooms1.add(bs, S_BLOOMS + 1, bloom ? an : 0.0);
+      int idx = stage_push(emit, &ss.st);
+      if (emit) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) ss.key[idx * W + w] = child[w];
+        ss.val[idx] = nspawn;
+        ss.info[idx] = info;
+      }
     }
     spawn_flush<W>(&ss, cnt, sp_rec, scap, false, overflow, s, bl, rk);  // its barriers also order the reuse of hv
   }
