@@ -29,7 +29,7 @@ def autotune(driver, steps=20, warm=5, repeats=2, log=None):
     knobs = [
         ("radix_items", [2, 4, 8]),
         ("chunk_blocks", [2 * cu, 4 * cu, 8 * cu]),
-        ("grid_blocks", [4 * cu, 8 * cu, 16 * cu]),
+        ("grid_blocks", [3 * cu, 4 * cu, 8 * cu]),
         ("single_lanes", [4, 8, 16]),
     ]
     best = {k: base[k] for k in ("chunk_blocks", "grid_blocks", "radix_items", "single_lanes")}

@@ -45,7 +45,7 @@ struct ngpu_engine {
   ngpu_sys_t sys{};
   int W = 1, S = 1;
   int nb_chunk = 1024;  // blocks of the chunked kernels (prepass, scans, radix)
-  int nb_grid = 2048;   // blocks of the grid-stride kernels
+  int nb_grid = 1024;   // blocks of the grid-stride kernels
   int radix_items = 4;  // elements per thread of a radix tile (2, 4 or 8)
   int single_lanes = 4;  // lanes per heat-bath single attempt (4, 8 or 16)
   int n_cu = 256, max_chunk = 4096;
@@ -723,7 +723,7 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
     return NGPU_ERR_HIP;
   }
   h->nb_chunk = std::max(256, 4 * prop.multiProcessorCount);
-  h->nb_grid = std::max(512, 8 * prop.multiProcessorCount);
+  h->nb_grid = std::max(512, 4 * prop.multiProcessorCount);
   h->n_cu = std::max(1, prop.multiProcessorCount);
   h->max_chunk = 16 * h->n_cu;  // buffers sized by the chunk count allow up to this many blocks
   int rc = NGPU_OK;
