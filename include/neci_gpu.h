@@ -213,15 +213,16 @@ int ngpu_set_graph(ngpu_t *h, int32_t on);
    blocks of the chunked kernels (prepass, attempt scan, radix passes; 64 to
    16 per compute unit), blocks of the grid-stride kernels (spawn, merge and
    annihilation, insert, statistics; 64 to 64 per compute unit), elements
-   per thread of a radix tile (2, 4 or 8) and lanes per heat-bath single
-   attempt (4, 8 or 16).  0 keeps a value.  Results do not
+   per thread of a radix tile (2, 4 or 8), lanes per heat-bath single attempt
+   (4, 8 or 16) and blocks of the merge-annihilation kernel (64 to 64 per
+   compute unit).  0 keeps a value.  Results do not
    depend on the shape (the kernels' outputs are independent of how work is
    split; only the storage order of new determinants may differ).  No
    counterpart in the reference. */
 int ngpu_set_launch_shape(ngpu_t *h, int32_t chunk_blocks, int32_t grid_blocks, int32_t radix_items,
-                          int32_t single_lanes);
+                          int32_t single_lanes, int32_t merge_blocks);
 int ngpu_get_launch_shape(ngpu_t *h, int32_t *chunk_blocks, int32_t *grid_blocks, int32_t *radix_items,
-                          int32_t *single_lanes, int32_t *n_cu);
+                          int32_t *single_lanes, int32_t *merge_blocks, int32_t *n_cu);
 
 /* Batches: several independent engines (different systems, sizes, replica
    counts; one W for all) iterated together as ONE graph per iteration, one

@@ -113,9 +113,10 @@ _SIGS = [
                                            P(ctypes.c_int64)]),
     ("ngpu_set_profiling", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("ngpu_set_graph", ctypes.c_int, [_vp, ctypes.c_int32]),
-    ("ngpu_set_launch_shape", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    ("ngpu_set_launch_shape", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_int32]),
     ("ngpu_get_launch_shape", ctypes.c_int, [_vp, P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int32),
-                                             P(ctypes.c_int32), P(ctypes.c_int32)]),
+                                             P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int32)]),
     ("ngpu_batch_create", ctypes.c_int, [P(_vp), _vp, ctypes.c_int32]),
     ("ngpu_batch_iterate", ctypes.c_int, [_vp, _vp]),
     ("ngpu_batch_destroy", ctypes.c_int, [_vp]),

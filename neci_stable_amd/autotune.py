@@ -5,9 +5,9 @@ and the radix sort ranks tiles of a fixed number of elements per thread
 (ngpu_set_launch_shape); the heat-bath singles run 4, 8 or 16 lanes per
 attempt.  The best values depend on the walker count and the system, so they
 are measured on the running calculation: one knob at a time (radix tile, the
-chunked kernels' blocks, the grid-stride kernels' blocks, then the lanes per
-single attempt), each candidate timed over a few iterations of the driver,
-the fastest kept.  The walker dynamics do not depend on the shape, so tuning
+chunked kernels' blocks, the grid-stride kernels' blocks, the lanes per
+single attempt, then the merge-annihilation kernel's blocks), each candidate
+timed over a few iterations of the driver, the fastest kept.  The walker dynamics do not depend on the shape, so tuning
 while the calculation runs changes nothing but its speed.
 """
 import time
@@ -31,8 +31,9 @@ def autotune(driver, steps=20, warm=5, repeats=2, log=None):
         ("chunk_blocks", [2 * cu, 4 * cu, 8 * cu]),
         ("grid_blocks", [3 * cu, 4 * cu, 8 * cu]),
         ("single_lanes", [4, 8, 16]),
+        ("merge_blocks", [2 * cu, 3 * cu, 4 * cu]),
     ]
-    best = {k: base[k] for k in ("chunk_blocks", "grid_blocks", "radix_items", "single_lanes")}
+    best = {k: base[k] for k in ("chunk_blocks", "grid_blocks", "radix_items", "single_lanes", "merge_blocks")}
     trials = []
     for name, values in knobs:
         scores = {}

@@ -48,6 +48,7 @@ struct ngpu_engine {
   int nb_grid = 1024;   // blocks of the grid-stride kernels
   int radix_items = 4;  // elements per thread of a radix tile (2, 4 or 8)
   int single_lanes = 4;  // lanes per heat-bath single attempt (4, 8 or 16)
+  int nb_merge = 0;      // blocks of the merge-annihilation kernel (0: nb_grid)
   int n_cu = 256, max_chunk = 4096;
   hipStream_t stream = nullptr;
   DevSys ds{};
@@ -368,7 +369,7 @@ void sort_and_merge_it(ngpu_t* h, bool ranked, bool list_out) {
       launch(h, -1, -1, k_merge_list<W, R, uint32_t, 1>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->cnt,
              (const uint32_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_rec, h->scap, h->u_key,
              h->u_sgn, h->u_flag, h->u_count);
-    launch(h, 6, 6, k_merge_annihilate<W, R, uint32_t, 1>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt,
+    launch(h, 6, 6, k_merge_annihilate<W, R, uint32_t, 1>, dim3(h->nb_merge ? h->nb_merge : h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt,
            h->dpart, (const uint32_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_rec, h->scap,
            h->n_key, h->n_sgn, h->n_flag, (const IterParams*)h->dparams);
     return;
@@ -403,7 +404,7 @@ void sort_and_merge_it(ngpu_t* h, bool ranked, bool list_out) {
     launch(h, -1, -1, k_merge_list<W, R, uint64_t, W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->cnt,
            (const uint64_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_rec, h->scap, h->u_key,
            h->u_sgn, h->u_flag, h->u_count);
-  launch(h, 6, 6, k_merge_annihilate<W, R, uint64_t, W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt,
+  launch(h, 6, 6, k_merge_annihilate<W, R, uint64_t, W>, dim3(h->nb_merge ? h->nb_merge : h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt,
          h->dpart, (const uint64_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_rec, h->scap,
          h->n_key, h->n_sgn, h->n_flag, (const IterParams*)h->dparams);
 }
@@ -724,6 +725,7 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
   }
   h->nb_chunk = std::max(256, 4 * prop.multiProcessorCount);
   h->nb_grid = std::max(512, 4 * prop.multiProcessorCount);
+  h->nb_merge = std::max(384, 3 * prop.multiProcessorCount);
   h->n_cu = std::max(1, prop.multiProcessorCount);
   h->max_chunk = 16 * h->n_cu;  // buffers sized by the chunk count allow up to this many blocks
   int rc = NGPU_OK;
@@ -1399,10 +1401,11 @@ int ngpu_iterate(ngpu_t* h, const ngpu_iter_in_t* in, ngpu_iter_out_t* out) {
 }
 
 int ngpu_set_launch_shape(ngpu_t* h, int32_t chunk_blocks, int32_t grid_blocks, int32_t radix_items,
-                          int32_t single_lanes) {
+                          int32_t single_lanes, int32_t merge_blocks) {
   if (!h) return NGPU_ERR_ARG;
   if (chunk_blocks < 0 || chunk_blocks > h->max_chunk || (chunk_blocks && chunk_blocks < 64) || grid_blocks < 0 ||
-      (grid_blocks && grid_blocks < 64) || grid_blocks > 64 * h->n_cu ||
+      (grid_blocks && grid_blocks < 64) || grid_blocks > 64 * h->n_cu || merge_blocks < 0 ||
+      (merge_blocks && merge_blocks < 64) || merge_blocks > 64 * h->n_cu ||
       !(radix_items == 0 || radix_items == 2 || radix_items == 4 || radix_items == 8) ||
       !(single_lanes == 0 || single_lanes == 4 || single_lanes == 8 || single_lanes == 16))
     return NGPU_ERR_ARG;
@@ -1414,13 +1417,15 @@ int ngpu_set_launch_shape(ngpu_t* h, int32_t chunk_blocks, int32_t grid_blocks, 
   if (grid_blocks) h->nb_grid = grid_blocks;
   if (radix_items) h->radix_items = radix_items;
   if (single_lanes) h->single_lanes = single_lanes;
+  if (merge_blocks) h->nb_merge = merge_blocks;
   h->graph_dirty = true;
   return NGPU_OK;
 }
 
 int ngpu_get_launch_shape(ngpu_t* h, int32_t* chunk_blocks, int32_t* grid_blocks, int32_t* radix_items,
-                          int32_t* single_lanes, int32_t* n_cu) {
+                          int32_t* single_lanes, int32_t* merge_blocks, int32_t* n_cu) {
   if (!h) return NGPU_ERR_ARG;
+  if (merge_blocks) *merge_blocks = h->nb_merge ? h->nb_merge : h->nb_grid;
   if (chunk_blocks) *chunk_blocks = h->nb_chunk;
   if (grid_blocks) *grid_blocks = h->nb_grid;
   if (radix_items) *radix_items = h->radix_items;

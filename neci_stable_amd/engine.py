@@ -167,15 +167,15 @@ class Engine:
         check(self.lib.ngpu_set_spawn_precision(self.h, int(bool(fp32))), "ngpu_set_spawn_precision")
         self.spec.spawn_fp32 = bool(fp32)
 
-    def set_launch_shape(self, chunk_blocks=0, grid_blocks=0, radix_items=0, single_lanes=0):
+    def set_launch_shape(self, chunk_blocks=0, grid_blocks=0, radix_items=0, single_lanes=0, merge_blocks=0):
         check(self.lib.ngpu_set_launch_shape(self.h, int(chunk_blocks), int(grid_blocks), int(radix_items),
-                                             int(single_lanes)), "ngpu_set_launch_shape")
+                                             int(single_lanes), int(merge_blocks)), "ngpu_set_launch_shape")
 
     def launch_shape(self):
-        v = [ctypes.c_int32() for _ in range(5)]
+        v = [ctypes.c_int32() for _ in range(6)]
         check(self.lib.ngpu_get_launch_shape(self.h, *[ctypes.byref(x) for x in v]), "ngpu_get_launch_shape")
         return dict(chunk_blocks=v[0].value, grid_blocks=v[1].value, radix_items=v[2].value,
-                    single_lanes=v[3].value, n_cu=v[4].value)
+                    single_lanes=v[3].value, merge_blocks=v[4].value, n_cu=v[5].value)
 
     def set_graph(self, on=True):
         """Replay one captured graph per iteration (ngpu_set_graph)."""

@@ -13,8 +13,8 @@ def test_launch_shape_does_not_change_results():
     from neci_stable_amd.engine import Engine
     spec = mol_spec(8, 6, init_walkers=4000, seed=12)
     shapes = [dict(chunk_blocks=0, grid_blocks=0, radix_items=4),
-              dict(chunk_blocks=64, grid_blocks=64, radix_items=2, single_lanes=16),
-              dict(chunk_blocks=256, grid_blocks=4096, radix_items=8, single_lanes=8)]
+              dict(chunk_blocks=64, grid_blocks=64, radix_items=2, single_lanes=16, merge_blocks=512),
+              dict(chunk_blocks=256, grid_blocks=4096, radix_items=8, single_lanes=8, merge_blocks=64)]
     engines = []
     for sh in shapes:
         e = Engine(spec)
@@ -43,7 +43,7 @@ def test_launch_shape_rejects_out_of_range():
     from neci_stable_amd.engine import Engine
     e = Engine(mol_spec(8, 6, init_walkers=1000))
     for bad in (dict(radix_items=3), dict(chunk_blocks=10), dict(grid_blocks=-1), dict(chunk_blocks=10 ** 7),
-                dict(single_lanes=3)):
+                dict(single_lanes=3), dict(merge_blocks=10)):
         with pytest.raises(NgpuError):
             e.set_launch_shape(**bad)
     e.close()
