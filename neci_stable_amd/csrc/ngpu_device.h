@@ -184,6 +184,8 @@ __host__ __device__ inline bool double_parity(const uint64_t* d, int s1, int s2,
 // ----------------------------------------------------------- hashing
 // FindWalkerHash (hash.F90:23-37): wrapping int64 accumulation over the
 // occupied orbitals in ascending order, bucket = |mod(h, L)| + 1 (1-based).
+// The RandomHash2 values of four orbitals are loaded before the four steps
+// that use them (the steps themselves stay in orbital order).
 template <int W>
 __host__ __device__ inline int64_t find_walker_hash(const uint64_t* d, const int32_t* rh, int64_t L) {
   uint64_t h = 0;
@@ -192,11 +194,27 @@ __host__ __device__ inline int64_t find_walker_hash(const uint64_t* d, const int
   for (int w = 0; w < W; ++w) {
     uint64_t x = d[w];
     while (x) {
-      int orb = 64 * w + ctz64(x) + 1;
-      x &= x - 1;
-      ++i;
-      int32_t term = rh[orb - 1] * i;
-      h = 1099511628211ull * h + (uint64_t)(int64_t)term;
+      int orb[4], k = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        orb[q] = 0;
+        if (x) {
+          orb[q] = 64 * w + ctz64(x) + 1;
+          x &= x - 1;
+          k = q + 1;
+        }
+      }
+      int32_t r[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) r[q] = q < k ? rh[orb[q] - 1] : 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q < k) {
+          ++i;
+          int32_t term = r[q] * i;
+          h = 1099511628211ull * h + (uint64_t)(int64_t)term;
+        }
+      }
     }
   }
   int64_t m = (int64_t)h % L;
