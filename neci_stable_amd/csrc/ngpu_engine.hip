@@ -327,44 +327,32 @@ void hash_upkeep(ngpu_t* h) {
 // fused with their annihilation into the walker list (stage 6; stage 5, the
 // merge alone, is empty since the fusion); ranked: the spawn kernels already
 // wrote each record's colex rank
-// the rank path: DB-bit digits (ceil(rank_bits / DB) passes)
-template <int W, int R, int IT, int DB>
+// the rank path: one stable pass on the keys' top byte, then one block per
+// top-byte segment sorts it by the low 24 bits (k_seg_sort); the sorted
+// (key, spawn index) pairs end in buffer 0
+template <int W, int R, int IT>
 int sort_ranks(ngpu_t* h, bool ranked) {
   const unsigned long long* n_ptr = &h->cnt->n_spawned;
-  constexpr int RB = 1 << DB;
   if (ranked)
-    launch(h, 4, -1, k_radix_hist<uint32_t, 1, false, IT, DB>, dim3(h->nb_chunk), dim3(kBlock),
-           (const uint32_t*)h->rk, 1, n_ptr, h->scap, 0, 0, h->hist);
+    launch(h, 4, -1, k_radix_hist<uint32_t, 1, false, IT, 8>, dim3(h->nb_chunk), dim3(kBlock),
+           (const uint32_t*)h->rk, 1, n_ptr, h->scap, 0, 24, h->hist);
   else
-    launch(h, 4, -1, k_rank_hist<W, IT, DB>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec, n_ptr,
-           h->scap, h->ds.rec_words, (const uint32_t*)h->d_binom, h->sys.nel + 1, h->rk, h->hist);
-  const int passes = (h->rank_bits + DB - 1) / DB;
-  const uint32_t* kin = h->rk;
-  const uint32_t* iin = nullptr;
-  int cur = 0;
-  for (int p = 0; p < passes; ++p) {
-    int out = p & 1;
-    if (p > 0)
-      launch(h, -1, -1, k_radix_hist<uint32_t, 1, false, IT, DB>, dim3(h->nb_chunk), dim3(kBlock), kin, 1, n_ptr,
-             h->scap, 0, DB * p, h->hist);
-    launch(h, -1, -1, k_radix_scan, dim3(RB), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
-    launch(h, -1, p + 1 == passes ? 4 : -1, k_radix_scatter<uint32_t, 1, false, IT, DB>, dim3(h->nb_chunk),
-           dim3(kBlock), kin, 1, iin, (uint32_t*)h->sk[out], h->si[out], n_ptr, h->scap, 0, DB * p,
-           (const uint32_t*)h->hist, (const uint32_t*)h->digit_tot);
-    kin = (const uint32_t*)h->sk[out];
-    iin = h->si[out];
-    cur = out;
-  }
-  return cur;
+    launch(h, 4, -1, k_rank_hist<W, IT, 8>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec, n_ptr,
+           h->scap, h->ds.rec_words, (const uint32_t*)h->d_binom, h->sys.nel + 1, h->rk, h->hist, 24);
+  launch(h, -1, -1, k_radix_scan, dim3(256), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
+  launch(h, -1, -1, k_radix_scatter<uint32_t, 1, false, IT, 8>, dim3(h->nb_chunk), dim3(kBlock),
+         (const uint32_t*)h->rk, 1, (const uint32_t*)nullptr, (uint32_t*)h->sk[1], h->si[1], n_ptr, h->scap, 0, 24,
+         (const uint32_t*)h->hist, (const uint32_t*)h->digit_tot);
+  launch(h, -1, 4, k_seg_sort, dim3(256), dim3(kBlock), (uint32_t*)h->sk[1], h->si[1], (uint32_t*)h->sk[0],
+         h->si[0], (const uint32_t*)h->digit_tot);
+  return 0;
 }
 
 template <int W, int R, int IT>
 void sort_and_merge_it(ngpu_t* h, bool ranked, bool list_out) {
   const unsigned long long* n_ptr = &h->cnt->n_spawned;
   if (h->rank_sort) {
-    // 8-bit digits: 10-bit digits save a pass for 30-bit ranks but make each
-    // scatter pass twice as slow (measured 20.4 vs 10.8 us at 5.5e5 spawns)
-    const int cur = sort_ranks<W, R, IT, 8>(h, ranked);
+    const int cur = sort_ranks<W, R, IT>(h, ranked);
     if (list_out)
       launch(h, -1, -1, k_merge_list<W, R, uint32_t, 1>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->cnt,
              (const uint32_t*)h->sk[cur], (const uint32_t*)h->si[cur], (const uint64_t*)h->sp_rec, h->scap, h->u_key,

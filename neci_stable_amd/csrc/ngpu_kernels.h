@@ -705,6 +705,10 @@ __device__ inline uint32_t rec_info(const DevSys& s, const uint64_t* r) {
   return s.spawn_fp32 ? (uint32_t)(r[W] >> 32) : (uint32_t)r[3];
 }
 
+// the spawn sort's 32-bit key is the colex rank times this odd constant: a
+// bijection of 32-bit values that spreads the keys' top byte evenly
+constexpr uint32_t kRankMul = 0x9E3779B1u;
+
 // colex rank of a determinant: sum over its i-th occupied orbital (bit p,
 // ascending) of C(p, i); injective on determinants of nel electrons
 template <int W>
@@ -757,7 +761,7 @@ __device__ inline void spawn_flush(SpawnStage<W>* ss, Counters* cnt, uint64_t* s
         r[2] = (uint64_t)__double_as_longlong(ss->val[i]);
         r[3] = ss->info[i];
       }
-      if (rk) rk[d] = colex_rank<W>(bl.c, s.nel1, ss->key + i * W);
+      if (rk) rk[d] = colex_rank<W>(bl.c, s.nel1, ss->key + i * W) * kRankMul;
     } else {
       overflow = true;
     }
@@ -1267,12 +1271,13 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, int kstrid
   for (int d = threadIdx.x; d < RB; d += blockDim.x) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
 }
 
-// colex rank of each spawned record's determinant (binom[p*(nel+1)+i] =
-// C(p, i)), fused with the histogram of its lowest digit
+// sort key of each spawned record's determinant: its colex rank
+// (binom[p*(nel+1)+i] = C(p, i)) times kRankMul, fused with the histogram of
+// the digit at shift
 template <int W, int IT, int DB>
 __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, const unsigned long long* n_ptr,
                                                       int64_t ncap, int rstride, const uint32_t* binom,
-                                                      int nel1, uint32_t* rank, uint32_t* hist) {
+                                                      int nel1, uint32_t* rank, uint32_t* hist, int shift) {
   constexpr int RB = 1 << DB;
   __shared__ uint32_t h[RB];
   for (int i = threadIdx.x; i < RB; i += blockDim.x) h[i] = 0;
@@ -1282,9 +1287,9 @@ __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, cons
   int64_t begin, end;
   radix_chunk<IT>(n, begin, end);
   for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
-    uint32_t r = colex_rank<W>(binom, nel1, keys + i * rstride);
+    uint32_t r = colex_rank<W>(binom, nel1, keys + i * rstride) * kRankMul;
     rank[i] = r;
-    atomicAdd(&h[r & (RB - 1u)], 1u);
+    atomicAdd(&h[(r >> shift) & (RB - 1u)], 1u);
   }
   __syncthreads();
   for (int d = threadIdx.x; d < RB; d += blockDim.x) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
@@ -1437,6 +1442,111 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
       cnt[d0 + q] = 0;
     }
     __syncthreads();
+  }
+}
+
+// ------------------------------------------------------- partition sort
+// Sort of 32-bit keys in two launches after one stable radix pass on the
+// keys' top byte: that pass leaves segment b (the keys whose top byte is b)
+// contiguous and in input order, and block b of k_seg_sort sorts segment b
+// by the low 24 bits with three stable 8-bit passes, so the whole list ends
+// sorted and stable.  The passes run in LDS for segments of up to kSegCap
+// keys and through global memory (the input buffers serve as scratch) for
+// larger ones.  The keys are colex ranks times an odd constant (a bijection
+// of 32-bit values), so the top byte spreads the keys evenly over segments.
+constexpr int kSegCap = 4096;
+
+struct SegScratch {
+  uint32_t cnt[256];   // digit counts, then each digit's next position
+  uint32_t wcnt[kBlock / kWave][256];
+  int64_t red[kBlock / kWave];
+};
+
+// one stable 8-bit pass of block-sized tiles, in order, src -> dst (LDS or
+// global); every thread of the block calls it
+__device__ inline void seg_pass(const uint32_t* sk, const uint32_t* si, uint32_t* dk, uint32_t* di, int64_t n,
+                                int shift, SegScratch* sc) {
+  const int wv = threadIdx.x / kWave;
+  const uint64_t ltmask = lanemask_lt();
+  sc->cnt[threadIdx.x] = 0;
+  for (int q = 0; q < kBlock / kWave; ++q) sc->wcnt[q][threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&sc->cnt[(sk[i] >> shift) & 255u], 1u);
+  __syncthreads();
+  {
+    int64_t tot;
+    int64_t pre = block_exclusive_scan((int64_t)sc->cnt[threadIdx.x], sc->red, tot);
+    sc->cnt[threadIdx.x] = (uint32_t)pre;
+  }
+  __syncthreads();
+  for (int64_t t0 = 0; t0 < n; t0 += blockDim.x) {
+    const int64_t i = t0 + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t k = valid ? sk[i] : 0u, v = valid ? si[i] : 0u;
+    const uint32_t d = (k >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      uint64_t m = __ballot(valid && ((d >> bit) & 1u));
+      peers &= ((d >> bit) & 1u) ? m : ~m;
+    }
+    const int rank = popc64(peers & ltmask);
+    if (valid && rank == 0) sc->wcnt[wv][d] = (uint32_t)popc64(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t o = sc->cnt[d] + (uint32_t)rank;
+      for (int q = 0; q < wv; ++q) o += sc->wcnt[q][d];
+      dk[o] = k;
+      di[o] = v;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+    for (int q = 0; q < kBlock / kWave; ++q) {
+      add += sc->wcnt[q][threadIdx.x];
+      sc->wcnt[q][threadIdx.x] = 0;
+    }
+    sc->cnt[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+// block b sorts segment b (256 blocks); digit_tot holds the segment sizes,
+// the counts of the top-byte pass
+__global__ void __launch_bounds__(kBlock) k_seg_sort(uint32_t* kin, uint32_t* iin, uint32_t* kout, uint32_t* iout,
+                                                     const uint32_t* digit_tot) {
+  static_assert(kBlock == 256, "one thread per digit");
+  __shared__ uint32_t ka[kSegCap], ia[kSegCap], kb[kSegCap], ib[kSegCap];
+  __shared__ SegScratch sc;
+  __shared__ int64_t seg[2];
+  {
+    int64_t sz = digit_tot[threadIdx.x], tot;
+    int64_t pre = block_exclusive_scan(sz, sc.red, tot);
+    if ((int)threadIdx.x == (int)blockIdx.x) {
+      seg[0] = pre;
+      seg[1] = sz;
+    }
+  }
+  __syncthreads();
+  const int64_t start = seg[0], n = seg[1];
+  if (n == 0) return;  // block-uniform
+  uint32_t *gk = kin + start, *gi = iin + start, *ok = kout + start, *oi = iout + start;
+  if (n <= kSegCap) {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      ka[i] = gk[i];
+      ia[i] = gi[i];
+    }
+    __syncthreads();
+    seg_pass(ka, ia, kb, ib, n, 0, &sc);
+    seg_pass(kb, ib, ka, ia, n, 8, &sc);
+    seg_pass(ka, ia, kb, ib, n, 16, &sc);
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      ok[i] = kb[i];
+      oi[i] = ib[i];
+    }
+  } else {
+    seg_pass(gk, gi, ok, oi, n, 0, &sc);
+    seg_pass(ok, oi, gk, gi, n, 8, &sc);
+    seg_pass(gk, gi, ok, oi, n, 16, &sc);
   }
 }
 
