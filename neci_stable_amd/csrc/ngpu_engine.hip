@@ -343,7 +343,7 @@ int sort_ranks(ngpu_t* h, bool ranked) {
   launch(h, -1, -1, k_radix_scatter<uint32_t, 1, false, IT, 8>, dim3(h->nb_chunk), dim3(kBlock),
          (const uint32_t*)h->rk, 1, (const uint32_t*)nullptr, (uint32_t*)h->sk[1], h->si[1], n_ptr, h->scap, 0, 24,
          (const uint32_t*)h->hist, (const uint32_t*)h->digit_tot);
-  launch(h, -1, 4, k_seg_sort, dim3(256), dim3(kBlock), (uint32_t*)h->sk[1], h->si[1], (uint32_t*)h->sk[0],
+  launch(h, -1, 4, k_seg_sort, dim3(256), dim3(kSegThreads), (uint32_t*)h->sk[1], h->si[1], (uint32_t*)h->sk[0],
          h->si[0], (const uint32_t*)h->digit_tot);
   return 0;
 }

@@ -1456,28 +1456,46 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
 // of 32-bit values), so the top byte spreads the keys evenly over segments.
 constexpr int kSegCap = 4096;
 
+constexpr int kSegThreads = 1024;  // threads of a k_seg_sort block
+constexpr int kSegWaves = kSegThreads / kWave;
+
 struct SegScratch {
-  uint32_t cnt[256];   // digit counts, then each digit's next position
-  uint32_t wcnt[kBlock / kWave][256];
-  int64_t red[kBlock / kWave];
+  uint32_t cnt[256];  // digit counts, then each digit's next position
+  uint32_t wcnt[kSegWaves][256];
 };
 
-// one stable 8-bit pass of block-sized tiles, in order, src -> dst (LDS or
-// global); every thread of the block calls it
+// exclusive scan of the 256 digit counts by the first wave (4 per lane)
+__device__ inline void seg_scan_counts(SegScratch* sc) {
+  if (threadIdx.x < kWave) {
+    const int l = threadIdx.x;
+    uint32_t c0 = sc->cnt[4 * l], c1 = sc->cnt[4 * l + 1], c2 = sc->cnt[4 * l + 2], c3 = sc->cnt[4 * l + 3];
+    uint32_t s = c0 + c1 + c2 + c3, x = s;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      uint32_t y = __shfl_up(x, o);
+      if (l >= o) x += y;
+    }
+    uint32_t pre = x - s;
+    sc->cnt[4 * l] = pre;
+    sc->cnt[4 * l + 1] = pre + c0;
+    sc->cnt[4 * l + 2] = pre + c0 + c1;
+    sc->cnt[4 * l + 3] = pre + c0 + c1 + c2;
+  }
+}
+
+// one stable 8-bit pass, src -> dst (LDS or global), in tiles of one key per
+// thread: ballots rank the keys of a digit within each wave, the waves'
+// counts in LDS order the waves.  Every thread of the block calls it.
 __device__ inline void seg_pass(const uint32_t* sk, const uint32_t* si, uint32_t* dk, uint32_t* di, int64_t n,
                                 int shift, SegScratch* sc) {
   const int wv = threadIdx.x / kWave;
   const uint64_t ltmask = lanemask_lt();
-  sc->cnt[threadIdx.x] = 0;
-  for (int q = 0; q < kBlock / kWave; ++q) sc->wcnt[q][threadIdx.x] = 0;
+  for (int t = threadIdx.x; t < 256; t += blockDim.x) sc->cnt[t] = 0;
+  for (int t = threadIdx.x; t < kSegWaves * 256; t += blockDim.x) (&sc->wcnt[0][0])[t] = 0;
   __syncthreads();
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&sc->cnt[(sk[i] >> shift) & 255u], 1u);
   __syncthreads();
-  {
-    int64_t tot;
-    int64_t pre = block_exclusive_scan((int64_t)sc->cnt[threadIdx.x], sc->red, tot);
-    sc->cnt[threadIdx.x] = (uint32_t)pre;
-  }
+  seg_scan_counts(sc);
   __syncthreads();
   for (int64_t t0 = 0; t0 < n; t0 += blockDim.x) {
     const int64_t i = t0 + threadIdx.x;
@@ -1500,31 +1518,32 @@ __device__ inline void seg_pass(const uint32_t* sk, const uint32_t* si, uint32_t
       di[o] = v;
     }
     __syncthreads();
-    uint32_t add = 0;
-    for (int q = 0; q < kBlock / kWave; ++q) {
-      add += sc->wcnt[q][threadIdx.x];
-      sc->wcnt[q][threadIdx.x] = 0;
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) {
+      uint32_t add = 0;
+      for (int q = 0; q < kSegWaves; ++q) {
+        add += sc->wcnt[q][t];
+        sc->wcnt[q][t] = 0;
+      }
+      sc->cnt[t] += add;
     }
-    sc->cnt[threadIdx.x] += add;
     __syncthreads();
   }
 }
 
-// block b sorts segment b (256 blocks); digit_tot holds the segment sizes,
-// the counts of the top-byte pass
-__global__ void __launch_bounds__(kBlock) k_seg_sort(uint32_t* kin, uint32_t* iin, uint32_t* kout, uint32_t* iout,
-                                                     const uint32_t* digit_tot) {
-  static_assert(kBlock == 256, "one thread per digit");
+// block b sorts segment b (256 blocks of kSegThreads); digit_tot holds the
+// segment sizes, the counts of the top-byte pass
+__global__ void __launch_bounds__(kSegThreads) k_seg_sort(uint32_t* kin, uint32_t* iin, uint32_t* kout,
+                                                          uint32_t* iout, const uint32_t* digit_tot) {
   __shared__ uint32_t ka[kSegCap], ia[kSegCap], kb[kSegCap], ib[kSegCap];
   __shared__ SegScratch sc;
   __shared__ int64_t seg[2];
-  {
-    int64_t sz = digit_tot[threadIdx.x], tot;
-    int64_t pre = block_exclusive_scan(sz, sc.red, tot);
-    if ((int)threadIdx.x == (int)blockIdx.x) {
-      seg[0] = pre;
-      seg[1] = sz;
-    }
+  if (threadIdx.x < 256) sc.cnt[threadIdx.x] = digit_tot[threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x == 0) {  // this segment's start: the sizes of the segments before it
+    int64_t st = 0;
+    for (int b = 0; b < (int)blockIdx.x; ++b) st += sc.cnt[b];
+    seg[0] = st;
+    seg[1] = sc.cnt[blockIdx.x];
   }
   __syncthreads();
   const int64_t start = seg[0], n = seg[1];
