@@ -32,13 +32,14 @@ HBM_PEAK_GBS = 8000.0
 
 
 def sort_key(nbasis, nel):
-    """(bytes per key, 8-bit passes) of the spawn sort: the 32-bit colex rank
-    when C(nbasis, nel) fits 32 bits (and its binomial table the engine's LDS
-    copy), else the determinant words."""
+    """(bytes per key, 8-bit passes through global memory) of the spawn sort:
+    for the 32-bit colex-rank key (C(nbasis, nel) fits 32 bits and its
+    binomial table the engine's LDS copy) the top-byte pass and the segment
+    sort's read and write, counted as two passes; else the determinant words,
+    one LSD pass per 8 bits."""
     import math
     if nel is not None and math.comb(nbasis, nel) <= 2 ** 32 and nbasis * (nel + 1) <= 512:
-        bits = max(1, (math.comb(nbasis, nel) - 1).bit_length())
-        return 4, (bits + 7) // 8
+        return 4, 2
     W = (nbasis + 63) // 64
     return 8 * W, ((nbasis + 7) // 8 if W == 1 else (nbasis - 64 + 7) // 8 + 8)
 
@@ -62,7 +63,7 @@ def algorithmic_bytes(stage, c, W=1, S=1, nbasis=40, psingles=0.1, nel=None):
         return Nd * (K + S + 8 * S) + Nq * 16 + (Ns - Nsq) * (K + 12)
     if stage == "spawn_singles":  # queue entry, parent key/pinfo/offset, single spawns written
         return Nq * (16 + K + 1 + 8) + Nsq * (K + 12)
-    if stage == "sort":        # per 8-bit pass: histogram read of keys, scatter read+write of (key, index)
+    if stage == "sort":        # per pass: histogram read of keys, scatter read+write of (key, index)
         kb, passes = sort_key(nbasis, nel)
         return passes * Ns * (kb + 2 * (kb + 4))
     if stage == "merge":       # fused into the annihilate stage

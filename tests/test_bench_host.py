@@ -18,17 +18,15 @@ COUNTS = dict(Nd=5.45e5, Na=9.97e5, Ns=5.46e5, Nu=5.0e5, Nn=1.0e5)
 
 
 @pytest.mark.parametrize("nbasis,nel,expect", [
-    (40, 10, (4, 4)),    # C(40,10) = 847 660 528 < 2^32: 30-bit colex ranks, 4 passes
-    (20, 10, (4, 3)),    # C(20,10) = 184 756: 18 bits
+    (40, 10, (4, 2)),    # C(40,10) = 847 660 528 < 2^32: colex-rank key, partition sort
+    (20, 10, (4, 2)),    # C(20,10) = 184 756
     (36, 18, (8, 5)),    # C(36,18) > 2^32: determinant word, 36 bits in 5 passes
     (80, 26, (16, 10)),  # two words: 16 bits of the second (2 passes) + 8 passes of the first
     (60, 14, (8, 8)),    # C(60,14) > 2^32, one word
 ])
 def test_sort_key_matches_the_engine_rule(nbasis, nel, expect):
     assert bench.sort_key(nbasis, nel) == expect
-    if expect[0] == 4:
-        assert math.comb(nbasis, nel) <= 2 ** 32
-        assert 2 ** (8 * expect[1] - 8) < math.comb(nbasis, nel) <= 2 ** (8 * expect[1])
+    assert (expect[0] == 4) == (math.comb(nbasis, nel) <= 2 ** 32)
 
 
 def test_sort_and_iteration_bytes():
