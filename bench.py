@@ -48,10 +48,13 @@ def algorithmic_bytes(stage, c, W=1, S=1, nbasis=40, psingles=0.1, nel=None):
     """Compulsory HBM bytes of one iteration's launch(es) of a stage: every input
     read once and every output written once, for the per-iteration counts c
     (Nd occupied dets, Na attempts, Ns spawns, Nu unique spawned dets, Nn new
-    dets).  Records: key 8W, signs 8S, flags 4.  The integral, alias and
-    neighbour tables (under 1 MB here) stay in L2 and count once, i.e. zero.
-    Queued single attempts and their share of the spawns are their expected
-    values psingles*Na and psingles*Ns."""
+    dets).  Records: key 8W, signs 8S, flags 4.  The integral and heat-bath
+    tables (under 1 MB here) stay in L2 and count once, i.e. zero; the PCHB
+    alias rows (21 MB at 40 spin orbitals) do not fit one XCD's 4 MB L2, so
+    every double attempt counts its alias entry as the minimum an alias
+    draw reads: bias 8 + alias 4 + probability 8 bytes (T_gen of a PCHB
+    double).  Queued single attempts and their share of the spawns are their
+    expected values psingles*Na and psingles*Ns."""
     K, Sg = 8 * W, 8 * S
     Nd, Na, Ns, Nu, Nn = c["Nd"], c["Na"], c["Ns"], c["Nu"], c["Nn"]
     Nq, Nsq = psingles * Na, psingles * Ns
@@ -59,8 +62,9 @@ def algorithmic_bytes(stage, c, W=1, S=1, nbasis=40, psingles=0.1, nel=None):
         return Nd * (K + Sg + 4 + 16) + Nd * (Sg + 4 + 4 * S + S)
     if stage == "scan":        # read counts, write offsets
         return Nd * S * (4 + 8)
-    if stage == "spawn":       # parents once (key, pinfo, offset); queue (16 B) and double spawns written
-        return Nd * (K + S + 8 * S) + Nq * 16 + (Ns - Nsq) * (K + 12)
+    if stage == "spawn":       # parents once (key, pinfo, offset); one alias entry per double attempt;
+        #                        queue (16 B) and double spawns written
+        return Nd * (K + S + 8 * S) + (Na - Nq) * 20 + Nq * 16 + (Ns - Nsq) * (K + 12)
     if stage == "spawn_singles":  # queue entry, parent key/pinfo/offset, single spawns written
         return Nq * (16 + K + 1 + 8) + Nsq * (K + 12)
     if stage == "sort":        # per pass: histogram read of keys, scatter read+write of (key, index)

@@ -38,6 +38,11 @@ def test_sort_and_iteration_bytes():
     assert bench.algorithmic_bytes("merge", COUNTS) == 0.0
     # prepass: 53 bytes per determinant for one word and one replica
     assert bench.algorithmic_bytes("prepass", COUNTS) == pytest.approx(53 * COUNTS["Nd"])
+    # spawn: parents (17 B), one 20-byte alias entry per double attempt, the
+    # queued singles (16 B) and the double spawns (20 B) written
+    nq, nsq = 0.1 * COUNTS["Na"], 0.1 * COUNTS["Ns"]
+    assert bench.algorithmic_bytes("spawn", COUNTS) == pytest.approx(
+        17 * COUNTS["Nd"] + 20 * (COUNTS["Na"] - nq) + 16 * nq + 20 * (COUNTS["Ns"] - nsq))
 
 
 def test_summarize_trace(tmp_path, capsys):
