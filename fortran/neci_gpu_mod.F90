@@ -47,6 +47,7 @@ module neci_gpu_mod
     public :: ngpu_diag_helement_batch, ngpu_set_excitgen_uniform, ngpu_set_core
     public :: ngpu_set_graph, ngpu_batch_create, ngpu_batch_iterate, ngpu_batch_destroy
     public :: ngpu_fcidump_header, ngpu_read_fcidump, ngpu_set_spawn_precision, ngpu_get_hash_stats
+    public :: ngpu_get_error
 
     interface
         integer(c_int) function ngpu_create(h, sys) bind(c, name="ngpu_create")
@@ -91,6 +92,11 @@ module neci_gpu_mod
             import :: c_int, c_ptr, c_int32_t
             type(c_ptr), value :: h
             integer(c_int32_t), value :: fp32
+        end function
+        integer(c_int) function ngpu_get_error(h, status, iter) bind(c, name="ngpu_get_error")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), intent(out) :: status, iter
         end function
         integer(c_int) function ngpu_get_hash_stats(h, live, tombstones, entries, rebuilds) &
             bind(c, name="ngpu_get_hash_stats")

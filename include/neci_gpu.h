@@ -198,7 +198,7 @@ int ngpu_set_core(ngpu_t *h, const int64_t *dets, int64_t ncore, int32_t fp32_va
 /* One FCIQMC iteration (PerformFCIMCycPar, FciMCPar.F90:1177-1920).  Runs
    asynchronously on the engine's stream when out == NULL; otherwise waits and
    fills out.  A failure inside the iteration is reported by the next call that
-   waits. */
+   waits, and halts the engine (ngpu_get_error). */
 int ngpu_iterate(ngpu_t *h, const ngpu_iter_in_t *in, ngpu_iter_out_t *out);
 
 /* Replay one captured graph of the iteration's launches instead of issuing
@@ -245,6 +245,17 @@ int ngpu_get_stats(ngpu_t *h, ngpu_iter_out_t *out);
    call (maxima for max_cyc_spawn/bloom_size, last value for tot_walkers and
    inst_no_at_hf) plus the last iteration's, and start a new cycle. */
 int ngpu_get_cycle_stats(ngpu_t *h, ngpu_iter_out_t *sum, ngpu_iter_out_t *last, int32_t *n_iters);
+
+/* Sticky error of the iteration chain (waits for queued work).  The first
+   iteration that reports a status (capacity, stability) halts the engine:
+   every later iteration returns at once, so the walker list stays as that
+   iteration left it, the point at which the reference stops (stop_all).
+   ngpu_iterate with an output block, ngpu_get_stats and ngpu_get_cycle_stats
+   report the status from then on; this call also gives the number (Iter) of
+   the iteration that failed.  *status = NGPU_OK and *iter = 0 while the
+   engine runs.  A halted engine refuses uploads, core spaces and explicit
+   lists with the same status; destroy it.  Either pointer may be NULL. */
+int ngpu_get_error(ngpu_t *h, int32_t *status, uint32_t *iter);
 
 /* Walker-hash diagnostics (waits for queued work): live entries, tombstones
    (entries of removed determinants not yet reclaimed), total entries (4 per

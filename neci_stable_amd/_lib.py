@@ -109,6 +109,7 @@ _SIGS = [
     ("ngpu_iterate", ctypes.c_int, [_vp, P(IterInT), P(IterOutT)]),
     ("ngpu_get_stats", ctypes.c_int, [_vp, P(IterOutT)]),
     ("ngpu_get_cycle_stats", ctypes.c_int, [_vp, P(IterOutT), P(IterOutT), P(ctypes.c_int32)]),
+    ("ngpu_get_error", ctypes.c_int, [_vp, P(ctypes.c_int32), P(ctypes.c_uint32)]),
     ("ngpu_get_hash_stats", ctypes.c_int, [_vp, P(ctypes.c_int64), P(ctypes.c_int64), P(ctypes.c_int64),
                                            P(ctypes.c_int64)]),
     ("ngpu_set_profiling", ctypes.c_int, [_vp, ctypes.c_int32]),

@@ -199,7 +199,10 @@ __global__ void k_begin_iter(ngpu_iter_out_t* out, double* part, Counters* cnt, 
 // accumulation into the cycle sums (ENumCyc, HFCyc, NoBorn, ... are cycle
 // accumulators in the reference too)
 __global__ void k_end_iter(ngpu_iter_out_t* out, double* part, ngpu_iter_out_t* cyc, Counters* cnt,
-                           int64_t scap, int64_t cap, int64_t hash_entries) {
+                           int64_t scap, int64_t cap, int64_t hash_entries, const IterParams* prm) {
+  // halted: the output block keeps the failing iteration's statistics and
+  // error, and nothing is added to the cycle
+  if (cnt->halt) return;
   __shared__ unsigned long long n_unique;
   if (threadIdx.x < kWave) {
     // the spread per-block counts, one line per lane, summed across the wave
@@ -257,6 +260,10 @@ __global__ void k_end_iter(ngpu_iter_out_t* out, double* part, ngpu_iter_out_t* 
     cyc->n_new += out->n_new;
     cyc->err |= out->err;
     cyc->pad += 1;  // iterations in the cycle
+    if (cnt->err) {  // the chain halts from the next iteration on
+      cnt->halt = cnt->err;
+      cnt->err_iter = prm->iter;
+    }
     // the counters the next iteration appends to start from zero (the chain
     // has no separate reset launch)
     cnt->n_spawned = 0;
@@ -429,7 +436,8 @@ void annihilation_chain(ngpu_t* h, bool with_core, bool list_out = false) {
   // deterministic_annihilation precedes AnnihilateSpawnedParts
   if (h->ncore && with_core)
     launch(h, -1, -1, k_core_apply<R>, dim3((int)std::min<int64_t>((h->ncore + kBlock - 1) / kBlock, h->nb_grid)),
-           dim3(kBlock), h->ds, h->wk, (const int64_t*)h->core_slot, (const double*)h->core_y, h->ncore, h->dpart);
+           dim3(kBlock), h->ds, h->wk, (const Counters*)h->cnt, (const int64_t*)h->core_slot, (const double*)h->core_y,
+           h->ncore, h->dpart);
   sort_and_merge<W, R>(h, with_core, list_out);  // spawns of the iteration chain carry their ranks
   launch(h, 7, 7, k_insert<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt,
          (const uint64_t*)h->n_key, (const double*)h->n_sgn, (const uint32_t*)h->n_flag,
@@ -437,7 +445,7 @@ void annihilation_chain(ngpu_t* h, bool with_core, bool list_out = false) {
   launch(h, 8, 8, k_stats<W, R>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
          (const IterParams*)h->dparams);
   launch(h, -1, -1, k_end_iter, dim3(1), dim3(kBlock), h->dout, h->dpart, h->dcyc, h->cnt,
-         h->scap, h->wk.cap, h->wk.L * (int64_t)kBucketEntries);
+         h->scap, h->wk.cap, h->wk.L * (int64_t)kBucketEntries, (const IterParams*)h->dparams);
 }
 
 template <int W, int R>
@@ -1108,6 +1116,7 @@ int ngpu_upload_walkers(ngpu_t* h, const int64_t* rows, int64_t n) {
   hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dpart, h->cnt, h->dparams,
                      IterParams{}, 0);
   HIP_OK(hipGetLastError());
+  if (h->hcnt->halt) return err_from_mask(h->hcnt->halt);  // halted: nothing was inserted
   return err_from_mask(h->hcnt->err);
 }
 
@@ -1166,6 +1175,7 @@ int set_core_impl(ngpu_t* h, int64_t ncore, bool fp32, int64_t* nnz_out) {
   hipLaunchKernelGGL(k_begin_iter, dim3(1), dim3(kBlock), 0, h->stream, h->dout, h->dpart, h->cnt, h->dparams,
                      IterParams{}, 0);
   HIP_OK(hipGetLastError());
+  if (h->hcnt->halt) return err_from_mask(h->hcnt->halt);
   if (h->hcnt->err) return err_from_mask(h->hcnt->err);
   h->core_nnz = nnz;
   if (nnz_out) *nnz_out = nnz;
@@ -1582,11 +1592,22 @@ int ngpu_get_cycle_stats(ngpu_t* h, ngpu_iter_out_t* sum, ngpu_iter_out_t* last,
   ngpu_iter_out_t tmp;
   HIP_OK(hipMemcpyAsync(&tmp, h->dcyc, sizeof(ngpu_iter_out_t), hipMemcpyDeviceToHost, h->stream));
   hipLaunchKernelGGL(k_reset_cycle, dim3(1), dim3(kBlock), 0, h->stream, h->dcyc);
+  HIP_OK(hipMemcpyAsync(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
   int rc = finish(h, last);
   if (sum) *sum = tmp;
   if (n_iters) *n_iters = tmp.pad;
   if (rc == NGPU_OK) rc = err_from_mask((unsigned)tmp.err);
+  if (rc == NGPU_OK) rc = err_from_mask(h->hcnt->halt);  // halted in an earlier cycle
   return rc;
+}
+
+int ngpu_get_error(ngpu_t* h, int32_t* status, uint32_t* iter) {
+  if (!h) return NGPU_ERR_ARG;
+  HIP_OK(hipStreamSynchronize(h->stream));
+  HIP_OK(hipMemcpy(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost));
+  if (status) *status = err_from_mask(h->hcnt->halt);
+  if (iter) *iter = h->hcnt->halt ? h->hcnt->err_iter : 0u;
+  return NGPU_OK;
 }
 
 int ngpu_set_profiling(ngpu_t* h, int32_t on) {
@@ -1794,6 +1815,8 @@ int ngpu_annihilate(ngpu_t* h, uint32_t iter, const int64_t* sdet, const double*
     rc = finish(h, nullptr);
     if (rc) return rc;
   }
+  HIP_OK(hipMemcpy(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost));
+  if (h->hcnt->halt) return err_from_mask(h->hcnt->halt);
   const int W = h->W, S = h->S;
   if (!h->u_key) {  // the merged list is kept only for this entry point
     HIP_OK(hipMalloc((void**)&h->u_key, (size_t)h->scap * W * 8));

@@ -72,6 +72,12 @@ struct alignas(256) Counters {
   // scan refills it)
   unsigned int rebuild;
   unsigned int rebuilt;  // set by the attempt scan that refilled the table
+  // sticky error: the error mask of the first iteration that reported one,
+  // and that iteration's number.  While halt is set every kernel of the
+  // iteration chain returns at once, so the walker list stays as the failing
+  // iteration left it (the reference stops there, stop_all)
+  unsigned int halt;
+  unsigned int err_iter;
   unsigned long long n_rebuilds;  // walker-hash rebuilds since creation
   alignas(256) unsigned long long free_tail;
   alignas(256) unsigned long long n_spawned;
@@ -428,6 +434,7 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
                                                     double* part, int32_t* att_cnt,
                                                     uint8_t* pinfo, int64_t* blk_part, int64_t* blk_rm,
                                                     IterParams* prm, IterParams v, int use_v) {
+  if (cnt->halt) return;
   // the iteration's inputs: from the launch when launches are issued one by
   // one (block 0 publishes them for the later kernels), from device memory
   // in a replayed graph
@@ -580,6 +587,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(DevSys s, Walk wk, const 
                                                        const int64_t* part, const int64_t* part_rm,
                                                        const uint8_t* pinfo, Counters* cnt, int S,
                                                        int64_t* tile_p, int64_t tcap) {
+  if (cnt->halt) return;
   __shared__ int64_t red[kBlock / kWave];
   int64_t nslots = (int64_t)cnt->nslots;
   if (cnt->rebuild) {
@@ -931,6 +939,7 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
                                                   const int64_t* att_off, const int64_t* tile_p, int64_t tcap,
                                                   const uint8_t* pinfo, uint64_t* sp_rec, uint32_t* rk,
                                                   int64_t scap, int64_t* sq, const IterParams* prm) {
+  if (cnt->halt) return;
   const double tau = prm->tau;
   const uint32_t iter = prm->iter;
   __shared__ double acc_lds[kStatStride];
@@ -1057,6 +1066,7 @@ template <int W, int R, int PL, int G>
 __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Counters* cnt, double* part,
                                                           const uint8_t* pinfo, uint64_t* sp_rec, uint32_t* rk,
                                                           int64_t scap, const int64_t* sq, const IterParams* prm) {
+  if (cnt->halt) return;
   constexpr int kGroups = kBlock / G;
   const double tau = prm->tau;
   const uint32_t iter = prm->iter;
@@ -1629,6 +1639,7 @@ __global__ void __launch_bounds__(kBlock) k_merge_list(DevSys s, Counters* cnt, 
                                                        const uint64_t* sp_rec, int64_t scap, uint64_t* u_key,
                                                        double* u_sgn, uint32_t* u_flag,
                                                        unsigned long long* u_count) {
+  if (cnt->halt) return;
   __shared__ ListStage<W, R> ls;
   stage_init(&ls.st);
   const int S = R == 1 ? 1 : s.S;
@@ -1692,6 +1703,7 @@ __global__ void __launch_bounds__(kBlock) k_merge_annihilate(DevSys s, Walk wk, 
                                                              const K* skey, const uint32_t* sidx,
                                                              const uint64_t* sp_rec, int64_t scap, uint64_t* n_key,
                                                              double* n_sgn, uint32_t* n_flag, const IterParams* prm) {
+  if (cnt->halt) return;
   const uint32_t iter = prm->iter;
   __shared__ double acc_lds[kStatStride];
   __shared__ ListStage<W, R> ls;
@@ -1879,8 +1891,9 @@ __global__ void __launch_bounds__(kBlock) k_core_spmv(const int64_t* rowptr, con
 // deterministic_annihilation (Annihilation.F90:930-963): the projected
 // amplitudes join the core dets' signs (births and annihilations counted)
 template <int R>
-__global__ void __launch_bounds__(kBlock) k_core_apply(DevSys s, Walk wk, const int64_t* core_slot, const double* y,
-                                                       int64_t ncore, double* part) {
+__global__ void __launch_bounds__(kBlock) k_core_apply(DevSys s, Walk wk, const Counters* cnt, const int64_t* core_slot,
+                                                       const double* y, int64_t ncore, double* part) {
+  if (cnt->halt) return;
   __shared__ double acc_lds[kStatStride];
   BlockStats bs{acc_lds};
   bs.zero();
@@ -1977,6 +1990,7 @@ template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* cnt, const uint64_t* n_key,
                                                    const double* n_sgn, const uint32_t* n_flag,
                                                    const unsigned long long* n_ptr) {
+  if (cnt->halt) return;
   const int S = R == 1 ? 1 : s.S;
   int64_t nn = (int64_t)*n_ptr;
   // new det i takes the i-th queued free slot, past the queue a fresh slot
@@ -2066,6 +2080,7 @@ __global__ void k_post_insert(Counters* cnt, const unsigned long long* n_ptr, in
 template <int W, int R>
 __global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* cnt, double* part,
                                                   const IterParams* prm) {
+  if (cnt->halt) return;
   const uint32_t iter = prm->iter;
   __shared__ double acc_lds[kStatStride];
   __shared__ FreeStage fs;

@@ -156,6 +156,14 @@ class Engine:
         self.ncore = dets.shape[0]
         return nnz.value
 
+    def error(self):
+        """Sticky error of the iteration chain: (status, iteration that failed),
+        (0, 0) while the engine runs (ngpu_get_error)."""
+        st = ctypes.c_int32()
+        it = ctypes.c_uint32()
+        check(self.lib.ngpu_get_error(self.h, ctypes.byref(st), ctypes.byref(it)), "ngpu_get_error")
+        return st.value, it.value
+
     def hash_stats(self):
         """Walker-hash diagnostics: dict(live, tombstones, entries, rebuilds)."""
         v = [ctypes.c_int64() for _ in range(4)]

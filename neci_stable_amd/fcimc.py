@@ -24,6 +24,30 @@ import time
 
 import numpy as np
 
+# statistics that the cycle keeps as a maximum, or as the last iteration's value
+_CYCLE_MAX = ("max_cyc_spawn", "bloom_size")
+_CYCLE_LAST = ("tot_walkers", "inst_no_at_hf", "rc")
+
+
+def merge_cycle_sums(a, b):
+    """Cycle sums of two consecutive runs of iterations (a, then b) as one:
+    sums add, maxima take the larger, the last iteration's values come from b
+    (the accumulation rules of the engine's cycle block)."""
+    out = {}
+    for k, v in b.items():
+        if k not in a:
+            out[k] = v
+        elif k in _CYCLE_MAX:
+            out[k] = np.maximum(a[k], v) if isinstance(v, np.ndarray) else max(a[k], v)
+        elif k in _CYCLE_LAST:
+            out[k] = v
+        elif k == "err":
+            out[k] = int(a[k]) | int(v)
+        else:
+            out[k] = a[k] + v
+    return out
+
+
 class OracleCycleAdapter:
     """Gives a synchronous backend (iterate(it, shift) -> stats) the cycle API
     of the engine: asynchronous iterate + cycle_stats() sums."""
@@ -49,15 +73,7 @@ class OracleCycleAdapter:
             self.sum = {k: (np.array(v, dtype=float).copy() if isinstance(v, np.ndarray) else v)
                         for k, v in st.items()}
         else:
-            for k, v in st.items():
-                if k in ("max_cyc_spawn",):
-                    self.sum[k] = max(self.sum[k], v)
-                elif k in ("bloom_size",):
-                    self.sum[k] = np.maximum(self.sum[k], v)
-                elif k in ("tot_walkers", "inst_no_at_hf", "rc", "err"):
-                    self.sum[k] = v
-                else:
-                    self.sum[k] = self.sum[k] + v
+            self.sum = merge_cycle_sums(self.sum, st)
         return st if wait else None
 
     def cycle_stats(self):
@@ -96,6 +112,10 @@ class FciMCPar:
         self.sum_enum = np.zeros(self.S)
         self.sum_noathf = np.zeros(self.S)
         self.walker_iters = 0.0
+        # running totals over every iteration read so far (per-step figures
+        # of any window of iterations are differences of these)
+        self.totals = dict(iters=0, walker_iters=0.0, attempts=0, spawned=0, unique=0, new=0, det_iters=0.0)
+        self._pending = None  # sums of the current update cycle's iterations read so far
         self.last = None
         self.log = log
 
@@ -127,7 +147,7 @@ class FciMCPar:
             self.sum_enum += en
             self.sum_noathf += hf
         row = dict(iter=self.iter, shift=self.shift.copy(), tot_parts=tot_parts.copy(),
-                   tot_walkers=last["tot_walkers"], proj_e=proje, hf=hf / n,
+                   tot_walkers=last["tot_walkers"], proj_e=proje, hf=hf / n, enum=en / n,
                    nborn=sm["nborn"], ndied=sm["ndied"], nannihil=sm["nannihil"],
                    naborted=sm["naborted"], attempts=sm["n_attempts"], spawned=sm["n_spawned"],
                    unique=sm["n_unique"], new=sm["n_new"],
@@ -137,6 +157,30 @@ class FciMCPar:
             self.shift_hist.append((self.iter, self.shift.copy(), proje.copy()))
         if self.log:
             self.log(row)
+
+    def _chunk_done(self, sm, last, n):
+        """Statistics of the iterations run since the last read.  The update
+        cycle (calculate_new_shift_wrapper) averages over all StepsSft
+        iterations of the cycle (fcimc_iter_utilities.F90:929-933), so a
+        chunk that ends inside a cycle is kept and merged with the rest of
+        the cycle; the running totals count every iteration as it is read."""
+        self.walker_iters += float(np.sum(sm["tot_parts"]))
+        t = self.totals
+        t["iters"] += n
+        t["walker_iters"] += float(np.sum(sm["tot_parts"]))
+        t["attempts"] += int(sm["n_attempts"])
+        t["spawned"] += int(sm["n_spawned"])
+        t["unique"] += int(sm["n_unique"])
+        t["new"] += int(sm["n_new"])
+        t["det_iters"] += float(last["tot_walkers"]) * n
+        self.last = last
+        if self._pending is not None:
+            sm, n = merge_cycle_sums(self._pending[0], sm), self._pending[1] + n
+            self._pending = None
+        if self.iter % self.steps_shift == 0:
+            self._update_shift(sm, last, n)
+        else:
+            self._pending = (sm, n)
 
     def run(self, n_iter):
         """n_iter iterations, update cycle every StepsSft; returns wall seconds."""
@@ -148,11 +192,7 @@ class FciMCPar:
                 self.iter += 1
                 self.b.iterate(self.iter, self.tau, self.shift, wait=False)
             done += k
-            sm, last, n = self.b.cycle_stats()
-            self.walker_iters += float(np.sum(sm["tot_parts"]))
-            self.last = last
-            if self.iter % self.steps_shift == 0:
-                self._update_shift(sm, last, n)
+            self._chunk_done(*self.b.cycle_stats())
         return time.perf_counter() - t0
 
     def projected_energy(self):
@@ -187,11 +227,7 @@ class BatchedFciMCPar:
                                    [d.shift for d in self.drivers])
             done += k
             for d in self.drivers:
-                sm, last, n = d.b.cycle_stats()
-                d.walker_iters += float(np.sum(sm["tot_parts"]))
-                d.last = last
-                if d.iter % d.steps_shift == 0:
-                    d._update_shift(sm, last, n)
+                d._chunk_done(*d.b.cycle_stats())
         return time.perf_counter() - t0
 
     @property
@@ -225,3 +261,54 @@ def reblock(x):
             best = err
             break
     return mean, float(best)
+
+
+def _plateau(errs):
+    """Index of the first blocking level whose error no later level exceeds
+    by more than twice its own uncertainty (the plateau of Flyvbjerg-Petersen
+    blocking); the last level when none qualifies."""
+    for i, (m, err) in enumerate(errs):
+        e_err = err / np.sqrt(2.0 * (m - 1))
+        if all(e <= err + 2 * e_err for _, e in errs[i + 1:]):
+            return i
+    return len(errs) - 1
+
+
+def _blocks(x):
+    """Blocking levels of x: each level halves the previous one by averaging
+    neighbours (reblock_data, ErrorAnalysis.F90:944-984, block length 2)."""
+    out = []
+    y = np.asarray(x, dtype=np.float64)
+    while y.size >= 4:
+        out.append(y)
+        if y.size % 2:
+            y = y[:-1]
+        y = 0.5 * (y[0::2] + y[1::2])
+    return out
+
+
+def proje_error(num, den):
+    """Projected energy and its error from the per-cycle numerator and
+    denominator series (error_analysis, ErrorAnalysis.F90:250-300): both series
+    are reblocked to the longer of their two plateau block lengths, the
+    means and errors taken there (analyze_data, :912-942), and the ratio's
+    error combines the relative errors with the covariance correction:
+    E = <N>/<D>, err = |E| sqrt((sD/<D>)^2 + (sN/<N>)^2 - 2 cov(N, D)/(n <D> <N>))."""
+    bn, bd = _blocks(num), _blocks(den)
+    if not bn:
+        return float(np.mean(num) / np.mean(den)), float("nan")
+
+    def errs(levels):
+        return [(y.size, float(np.sqrt(max(np.mean(y * y) - np.mean(y) ** 2, 0.0) / (y.size - 1))))
+                for y in levels]
+    lvl = max(_plateau(errs(bn)), _plateau(errs(bd)))
+    yn, yd = bn[lvl], bd[lvl]
+    n = yn.size
+    mn, md = float(np.mean(yn)), float(np.mean(yd))
+    sn = float(np.sqrt(max(np.mean(yn * yn) - mn * mn, 0.0) / (n - 1)))
+    sd = float(np.sqrt(max(np.mean(yd * yd) - md * md, 0.0) / (n - 1)))
+    cov = float(np.sum((yn - mn) * (yd - md)) / (n - 1))
+    e = mn / md
+    var = (sd / md) ** 2 + (sn / mn) ** 2 - 2.0 * cov / (n * md * mn)
+    return e, abs(e) * float(np.sqrt(max(var, 0.0)))
+

@@ -25,7 +25,9 @@ def encode_det(orbs, nbasis):
 
 
 def decode_det(words):
-    u = np.asarray(words).view(np.uint64)
+    """Occupied orbitals (1-based, ascending) of one determinant given as its
+    W words, in any shape that holds exactly W words ((W,) or (1, W))."""
+    u = np.ascontiguousarray(words).view(np.uint64).ravel()
     out = []
     for wi, x in enumerate(u):
         x = int(x)

@@ -70,3 +70,45 @@ def test_walker_list_full_from_upload():
         eng.upload(eng.rows(dets, np.ones((len(dets), 1))))
     assert e.value.code == 2
     eng.close()
+
+
+def test_async_iterations_halt_at_the_failing_iteration():
+    """Iterations queued without waiting (as the host drivers run them): the
+    status arrives at the next read of the cycle statistics, the engine
+    reports the oracle's failing iteration (ngpu_get_error), and the walker
+    list is the one the failing iteration left, however many iterations were
+    queued after it."""
+    from neci_stable_amd._lib import NgpuError
+    spec = SPECS["hubbard10"](init_walkers=4000, seed=9, max_walkers=40)
+    eng, orc = _start(spec, 3000.0)
+    ref = engine(spec)
+    ref.start_single_part(3000.0)
+    it_fail = None
+    for it in range(1, 11):
+        if orc.iterate(it, 0.0)["rc"]:
+            it_fail = it
+            break
+    assert it_fail is not None
+    for it in range(1, it_fail + 1):  # the same engine state, run to the failure with waits
+        if it < it_fail:
+            ref.iterate(it, TAU, 0.0)
+        else:
+            with pytest.raises(NgpuError):
+                ref.iterate(it, TAU, 0.0)
+    assert eng.error() == (0, 0)
+    for it in range(1, it_fail + 6):
+        eng.iterate(it, TAU, 0.0, wait=False)
+    with pytest.raises(NgpuError) as e:
+        eng.cycle_stats()
+    assert e.value.code == 2
+    assert eng.error() == (2, it_fail)
+    with pytest.raises(NgpuError):  # still halted in the next cycle
+        eng.iterate(it_fail + 6, TAU, 0.0)
+    a = eng.download()
+    b = ref.download()
+    from common import sort_walkers
+    for x, y in zip(sort_walkers(*a[:3]), sort_walkers(*b[:3])):
+        np.testing.assert_array_equal(x, y)
+    eng.close()
+    ref.close()
+

@@ -107,6 +107,9 @@ SPECS = {
     # uniform random excitations (gen_rand_excit) with rejection loops
     "mol8o6e_uniform": lambda **k: mol_spec(8, 6, excitgen="uniform", psingles=0.2, **k),
     "mol33o4e_w2_uniform": lambda **k: mol_spec(33, 4, rank=4, mseed=3, excitgen="uniform", **k),
+    # plain FCIQMC: no initiator approximation (TRUNCINITIATOR off)
+    "hubbard10_plain": lambda **k: hubbard_spec(10, 10, 4.0, trunc_initiator=False, **k),
+    "mol8o6e_plain": lambda **k: mol_spec(8, 6, particles=0, trunc_initiator=False, **k),
 }
 
 
@@ -279,7 +282,9 @@ def test_annihilate_explicit_list_fp32(name):
                                             ("mol33o4e_w2", 1, False), ("hubbard18", 1, False),
                                             ("mol33o8e_w2", 1, False), ("hubbard10", 1, True),
                                             ("mol8o6e_unif_fast", 1, True), ("mol8o6e_uniform", 1, False),
-                                            ("mol8o6e_uniform", 2, False), ("mol33o4e_w2_uniform", 1, False)])
+                                            ("mol8o6e_uniform", 2, False), ("mol33o4e_w2_uniform", 1, False),
+                                            ("hubbard10_plain", 1, False), ("hubbard10_plain", 2, False),
+                                            ("mol8o6e_plain", 1, False)])
 def test_iterations_match_oracle(name, nrep, real):
     """Full iterations (spawn, merge, death, annihilation, stats) from the same
     walker set: identical walker sets after every iteration."""
@@ -325,6 +330,40 @@ def run_iterations(name, nrep, real, spawn_fp32=False):
     else:
         n = assert_same_walkers(eng, orc)
         assert n > 50
+
+
+def test_config1_molecule_matches_oracle():
+    """The 10e/20o synthetic molecule (seed 7, Cholesky rank 60): PCHB doubles
+    with UNIF-UNIF particle selection and heat-bath singles (psingles 0.1),
+    initiator threshold 3, tau 0.005.  The engine grows the population to
+    about 1e5 walkers (TARGET-SHIFTDAMP); from that walker set the engine
+    and the oracle run 20 more iterations with identical statistics and
+    identical walker sets."""
+    from neci_stable_amd import Spec
+    from neci_stable_amd.fcimc import FciMCPar
+    from common import molecule
+    tau = 0.005
+    spec = Spec(molecule(20, 10, 60, 7), init_walkers=1e5, seed=1234, particles=0, psingles=0.1,
+                pparallel=0.5, memory_fac_spawn=8.0)
+    eng = engine(spec)
+    eng.start_single_part(100.0)
+    drv = FciMCPar(eng, tau=tau, target_walkers=1e5, steps_shift=10, shift_damp=0.5, target_shiftdamp=True)
+    while drv.rows == [] or drv.rows[-1]["tot_parts"][0] < 0.9e5:
+        drv.run(10)
+        assert drv.iter < 3000
+    dets, sg, fl, _, _ = eng.download()
+    print(f"config-1 molecule: {sg[:, 0].__abs__().sum():.0f} walkers on {len(dets)} determinants "
+          f"after {drv.iter} iterations", flush=True)
+    orc = oracle(spec, tau)
+    orc.add_walkers_bulk(dets, sg, neci_to_internal_flags(fl, 1))
+    shift = drv.shift.copy()
+    for it in range(drv.iter + 1, drv.iter + 21):
+        a = eng.iterate(it, tau, shift)
+        b = orc.iterate(it, shift)
+        assert b["rc"] == 0
+        assert_stats(a, b, 1)
+    assert assert_same_walkers(eng, orc) > 5e4
+    eng.close()
 
 
 def test_hash_rebuild_keeps_parity():
