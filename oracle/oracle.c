@@ -15,8 +15,11 @@
  * bit 16 core (deterministic) det, bit 17 removed slot.
  * Spawn info word: bits 0-7 replica, bit 8 parent initiator, bit 9 core parent.
  */
+#define _POSIX_C_SOURCE 200809L  /* pthread barriers under -std=c11 */
 #include "oracle.h"
 #include <math.h>
+#include <stddef.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1607,14 +1610,34 @@ static int walker_death(ostate_t *st, int64_t j, const double *diag_sft, const d
     return err;
 }
 
-int32_t o_iterate(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_t *out)
+/* is_core_state through a sorted list of the core determinants (the lookup of
+   the sharded iteration, where a child's owner may be another shard) */
+typedef struct {
+    const uint64_t *dets;  /* ncore * W, ascending in ilut_cmp order */
+    int64_t n;
+    int W;
+} core_index_t;
+
+static int core_index_has(const core_index_t *ci, const uint64_t *det)
+{
+    int64_t lo = 0, hi = ci->n - 1;
+    while (lo <= hi) {
+        int64_t mid = (lo + hi) / 2;
+        int c = ilut_cmp(ci->dets + (size_t)mid * ci->W, det, ci->W);
+        if (c == 0) return 1;
+        if (c < 0) lo = mid + 1; else hi = mid - 1;
+    }
+    return 0;
+}
+
+/* The loop over determinants of PerformFCIMCycPar (FciMCPar.F90:1294-1758):
+   initiator flags, SumEContrib, spawning into st->spawned, death.  Children
+   that are core dets are recognised through ci when given (sharded runs),
+   else through the state's own walker hash. */
+static int det_loop(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_t *out, const core_index_t *ci)
 {
     const osys_t *s = &st->s;
-    const int W = st->W, S = st->S, RL = st->rowlen, nel = s->nel;
-    memset(out, 0, sizeof(*out));
-    st->nspawned = 0;
-    st->ifree_start = 0;
-    st->ifree_end = 0;
+    const int W = st->W, S = st->S, RL = st->rowlen;
     int err = 0;
     int32_t nJ[O_MAX_NEL], ex[4];
     uint64_t ilutJ[O_MAX_W];
@@ -1689,7 +1712,7 @@ int32_t o_iterate(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_t 
                 }
                 out->n_valid_excits += 1.0;
                 /* spawning from and to the core space is cancelled (FciMCPar.F90:1659-1668) */
-                if (core && st->ncore && is_core_det(st, ilutJ)) continue;
+                if (core && (ci ? core_index_has(ci, ilutJ) : (st->ncore && is_core_det(st, ilutJ)))) continue;
                 int32_t nI[O_MAX_NEL];
                 o_decode(row, W, nI);
                 /* attempt_create_normal (fcimc_pointed_fns.F90:178-491) */
@@ -1728,6 +1751,18 @@ int32_t o_iterate(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_t 
         if (e) err = e;
     }
     out->n_spawned = st->nspawned;
+    return err;
+}
+
+int32_t o_iterate(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_t *out)
+{
+    const osys_t *s = &st->s;
+    const int S = st->S;
+    memset(out, 0, sizeof(*out));
+    st->nspawned = 0;
+    st->ifree_start = 0;
+    st->ifree_end = 0;
+    int err = det_loop(st, iter, diag_sft, out, NULL);
     if (err) { out->err = err; return err; }
     /* determ_projection on the core amplitudes (unchanged by the loop: no death) */
     if (st->ncore) {
@@ -1743,7 +1778,6 @@ int32_t o_iterate(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_t 
     err = annihilate_spawned(st, nu, iter, out);
     if (err) { out->err = err; return err; }
     hash_table_stats(st, iter, out);
-    (void)nel;
     return 0;
 }
 
@@ -1783,4 +1817,378 @@ int32_t o_annihilate_list(ostate_t *st, uint32_t iter, const uint64_t *sdet, con
     if (err) { out->err = err; return err; }
     hash_table_stats(st, iter, out);
     return 0;
+}
+
+/* ===================================================== hash-sharded runs
+ *
+ * The reference runs one MPI rank per core; each rank owns the determinants
+ * that DetermineDetNode (load_balance_calcnodes.F90:25-117) assigns to it by
+ * a hash of the occupied orbitals, spawns into per-destination sections of
+ * SpawnedParts, exchanges them (SendProcNewParts, Annihilation.F90:150-247)
+ * and merges and annihilates its own part.  Here the ranks are P threads of
+ * one process sharing memory: shard q is an ostate_t holding the
+ * determinants it owns, the exchange is a copy between shards, and the
+ * semistochastic core vector is shared (the allgather of
+ * semi_stoch_procs.F90:127).  With integer walkers the result is that of the
+ * one-state iteration, determinant for determinant: every random number is
+ * keyed by determinant, and merged sums of integers do not depend on order.
+ */
+#include <pthread.h>
+
+struct omt {
+    osys_t s;
+    int P, W, S, RL;
+    ostate_t **sh;
+    int64_t *sendoff;  /* P x (P+1): start of each destination's section in a sender's buffer */
+    /* shared core space */
+    int64_t ncore;
+    uint64_t *core_det;
+    int64_t *rowptr;
+    int32_t *cols;
+    double *vals, *cx, *cy;
+    int32_t *core_owner;
+    int64_t *core_row;
+    uint64_t *core_sorted;
+    core_index_t ci;
+    /* iteration arguments and per-shard results */
+    uint32_t iter;
+    double sft[O_MAX_REPLICAS];
+    ostats_t *st_out;
+    int *st_err;
+    int phase;  /* 0 iterate, 1 build core rows, -1 exit */
+    int core_pass, core_fp32;
+    pthread_t *th;
+    pthread_barrier_t bar;
+};
+
+/* owner shard of a determinant: the walker-hash accumulation of hash.F90:30-35
+   over RandomHash2, mixed by the splitmix64 finaliser so that the owner is
+   independent of the shard's own bucket index */
+static int owner_of(const omt_t *m, const uint64_t *det)
+{
+    int32_t nI[O_MAX_NEL];
+    o_decode(det, m->W, nI);
+    uint64_t h = 0;
+    for (int i = 0; i < m->s.nel; ++i)
+        h = 1099511628211ull * h + (uint64_t)(int64_t)(m->s.random_hash2[nI[i] - 1] * (i + 1));
+    return (int)(mix64(h) % (uint64_t)m->P);
+}
+
+static int cmp_det_w1(const void *a, const void *b)
+{
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return ilut_cmp(&x, &y, 1);
+}
+static int cmp_det_w2(const void *a, const void *b) { return ilut_cmp((const uint64_t *)a, (const uint64_t *)b, 2); }
+
+/* rows [r0, r1) of the core Hamiltonian (the loop of o_set_core, one pass
+   with a per-row count) */
+static void core_rows_range(omt_t *m, int64_t r0, int64_t r1, int pass, int fp32)
+{
+    const int W = m->W, nel = m->s.nel;
+    int32_t oi[O_MAX_NEL], oj[O_MAX_NEL];
+    for (int64_t i = r0; i < r1; ++i) {
+        const uint64_t *di = m->core_det + (size_t)i * W;
+        o_decode(di, W, oi);
+        int64_t k = pass ? m->rowptr[i] : 0;
+        for (int64_t j = 0; j < m->ncore; ++j) {
+            const uint64_t *dj = m->core_det + (size_t)j * W;
+            if (excit_level(di, dj, W) > 2) continue;
+            double h;
+            if (i == j) {
+                h = o_diag_helement(&m->s, oi) - m->s.hii;
+            } else {
+                o_decode(dj, W, oj);
+                h = o_get_helement(&m->s, oi, oj, nel);
+            }
+            if (i != j && h == 0.0) continue;
+            if (pass) {
+                m->cols[k] = (int32_t)j;
+                m->vals[k] = fp32 ? (double)(float)h : h;
+            }
+            k++;
+        }
+        if (!pass) m->rowptr[i + 1] = k;
+    }
+}
+
+static void range_of(int64_t n, int P, int q, int64_t *a, int64_t *b)
+{
+    int64_t per = (n + P - 1) / P;
+    *a = per * q < n ? per * q : n;
+    *b = per * (q + 1) < n ? per * (q + 1) : n;
+}
+
+/* one iteration on shard q (the three phases, separated by barriers) */
+static void shard_iterate(omt_t *m, int q)
+{
+    ostate_t *st = m->sh[q];
+    ostats_t *out = &m->st_out[q];
+    const int P = m->P, RL = m->RL, S = m->S;
+    memset(out, 0, sizeof(*out));
+    st->nspawned = 0;
+    st->ifree_start = st->ifree_end = 0;
+    int err = det_loop(st, m->iter, m->sft, out, m->ncore ? &m->ci : NULL);
+    /* spawns into sections by destination (a stable counting sort), into spawned2 */
+    int64_t *off = m->sendoff + (size_t)q * (P + 1);
+    int32_t *own = (int32_t *)malloc(sizeof(int32_t) * (size_t)(st->nspawned ? st->nspawned : 1));
+    for (int d = 0; d <= P; ++d) off[d] = 0;
+    for (int64_t i = 0; i < st->nspawned; ++i) {
+        own[i] = owner_of(m, st->spawned + (size_t)i * RL);
+        off[own[i] + 1]++;
+    }
+    for (int d = 0; d < P; ++d) off[d + 1] += off[d];
+    {
+        int64_t pos[256];  /* P <= 256 (o_mt_create) */
+        for (int d = 0; d < P; ++d) pos[d] = off[d];
+        for (int64_t i = 0; i < st->nspawned; ++i)
+            memcpy(st->spawned2 + (size_t)(pos[own[i]]++) * RL, st->spawned + (size_t)i * RL, sizeof(uint64_t) * RL);
+    }
+    free(own);
+    /* the core amplitudes this shard holds */
+    for (int64_t i = 0; i < m->ncore; ++i)
+        if (m->core_owner[i] == q)
+            memcpy(m->cx + (size_t)i * S, row_sign(st, cur_row(st, m->core_row[i])), sizeof(double) * S);
+    pthread_barrier_wait(&m->bar);
+    /* determ_projection, rows split over the shards */
+    if (m->ncore) {
+        /* rows a..b of y from the whole x, in o_core_project's summation order */
+        int64_t a, b;
+        range_of(m->ncore, P, q, &a, &b);
+        for (int64_t i = a; i < b; ++i) {
+            double p[64], t[64];
+            for (int r = 0; r < S; ++r) {
+                for (int l = 0; l < 64; ++l) p[l] = 0.0;
+                for (int64_t k = m->rowptr[i]; k < m->rowptr[i + 1]; ++k) {
+                    int l = (int)((k - m->rowptr[i]) & 63);
+                    p[l] = p[l] - m->vals[k] * m->cx[(size_t)m->cols[k] * S + r];
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    for (int l = 0; l < 64; ++l) t[l] = p[l] + p[l ^ o];
+                    memcpy(p, t, sizeof(p));
+                }
+                m->cy[(size_t)i * S + r] = (p[0] + m->sft[r] * m->cx[(size_t)i * S + r]) * m->s.tau;
+            }
+        }
+    }
+    /* receive: the sections addressed to q, in sender order */
+    int64_t n = 0;
+    for (int src = 0; src < P; ++src) n += m->sendoff[(size_t)src * (P + 1) + q + 1] - m->sendoff[(size_t)src * (P + 1) + q];
+    if (n > st->maxs) err = err ? err : 1;
+    pthread_barrier_wait(&m->bar);  /* every shard has finished its det loop before spawned is reused */
+    if (!err) {
+        int64_t k = 0;
+        for (int src = 0; src < P; ++src) {
+            const int64_t *o = m->sendoff + (size_t)src * (P + 1);
+            int64_t c = o[q + 1] - o[q];
+            memcpy(st->spawned + (size_t)k * RL, m->sh[src]->spawned2 + (size_t)o[q] * RL, sizeof(uint64_t) * RL * (size_t)c);
+            k += c;
+        }
+        st->nspawned = n;
+    }
+    pthread_barrier_wait(&m->bar);  /* every section is copied before spawned2 is overwritten */
+    if (!err) {
+        int64_t nu = compress_spawned(st, st->nspawned, out);
+        out->n_unique = nu;
+        /* deterministic_annihilation for the core dets of this shard */
+        for (int64_t i = 0; i < m->ncore; ++i) {
+            if (m->core_owner[i] != q) continue;
+            double *cs = row_sign(st, cur_row(st, m->core_row[i]));
+            for (int r = 0; r < S; ++r) {
+                double y = m->cy[(size_t)i * S + r], c = cs[r];
+                cs[r] = y + c;
+                out->nborn[r] += fabs(y);
+                if (c * y < 0.0) {
+                    double a = fabs(c), b = fabs(y);
+                    out->nannihil[r] += 2.0 * (a < b ? a : b);
+                }
+            }
+        }
+        err = annihilate_spawned(st, nu, m->iter, out);
+        if (!err) hash_table_stats(st, m->iter, out);
+    }
+    m->st_err[q] = err;
+}
+
+static void *shard_main(void *arg)
+{
+    void **a = (void **)arg;
+    omt_t *m = (omt_t *)a[0];
+    int q = (int)(intptr_t)a[1];
+    free(arg);
+    for (;;) {
+        pthread_barrier_wait(&m->bar);  /* start */
+        if (m->phase < 0) break;
+        if (m->phase == 0) shard_iterate(m, q);
+        if (m->phase == 1) {
+            int64_t a, b;
+            range_of(m->ncore, m->P, q, &a, &b);
+            core_rows_range(m, a, b, m->core_pass, m->core_fp32);
+        }
+        pthread_barrier_wait(&m->bar);  /* done */
+    }
+    return NULL;
+}
+
+omt_t *o_mt_create(const osys_t *sys, int32_t nshards)
+{
+    if (nshards < 1 || nshards > 256) return NULL;
+    omt_t *m = (omt_t *)calloc(1, sizeof(omt_t));
+    m->s = *sys;
+    m->P = nshards;
+    m->W = nwords(sys->nbasis);
+    m->S = sys->nrep;
+    m->RL = m->W + m->S + 1;
+    m->sh = (ostate_t **)calloc((size_t)nshards, sizeof(ostate_t *));
+    osys_t ss = *sys;
+    ss.nbuckets = (sys->nbuckets + nshards - 1) / nshards;
+    if (ss.nbuckets < 1) ss.nbuckets = 1;
+    ss.max_walkers = sys->max_walkers / nshards * 2 + 4096;
+    ss.max_spawned = sys->max_spawned / nshards * 2 + 4096;
+    for (int q = 0; q < nshards; ++q) m->sh[q] = o_create(&ss);
+    m->sendoff = (int64_t *)calloc((size_t)nshards * (nshards + 1), sizeof(int64_t));
+    m->st_out = (ostats_t *)calloc((size_t)nshards, sizeof(ostats_t));
+    m->st_err = (int *)calloc((size_t)nshards, sizeof(int));
+    m->th = (pthread_t *)calloc((size_t)nshards, sizeof(pthread_t));
+    pthread_barrier_init(&m->bar, NULL, (unsigned)nshards);
+    for (int q = 1; q < nshards; ++q) {
+        void **arg = (void **)malloc(2 * sizeof(void *));
+        arg[0] = m;
+        arg[1] = (void *)(intptr_t)q;
+        pthread_create(&m->th[q], NULL, shard_main, arg);
+    }
+    return m;
+}
+
+void o_mt_destroy(omt_t *m)
+{
+    if (!m) return;
+    m->phase = -1;
+    pthread_barrier_wait(&m->bar);
+    for (int q = 1; q < m->P; ++q) pthread_join(m->th[q], NULL);
+    pthread_barrier_destroy(&m->bar);
+    for (int q = 0; q < m->P; ++q) o_destroy(m->sh[q]);
+    free(m->sh); free(m->sendoff); free(m->st_out); free(m->st_err); free(m->th);
+    free(m->core_det); free(m->rowptr); free(m->cols); free(m->vals); free(m->cx); free(m->cy);
+    free(m->core_owner); free(m->core_row); free(m->core_sorted);
+    free(m);
+}
+
+/* the rows of the core Hamiltonian split over the shards' threads */
+static void core_rows_all(omt_t *m, int pass, int fp32)
+{
+    m->core_pass = pass;
+    m->core_fp32 = fp32;
+    m->phase = 1;
+    pthread_barrier_wait(&m->bar);  /* start */
+    int64_t a, b;
+    range_of(m->ncore, m->P, 0, &a, &b);
+    core_rows_range(m, a, b, pass, fp32);
+    pthread_barrier_wait(&m->bar);  /* done */
+}
+
+int32_t o_mt_add_walkers(omt_t *m, const uint64_t *dets, const double *sgns, const uint32_t *flags, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t *d = dets + i * m->W;
+        int rc = o_add_walker(m->sh[owner_of(m, d)], d, sgns + i * m->S, flags[i]);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int64_t o_mt_get_walkers(omt_t *m, uint64_t *dets, double *sgns, uint32_t *flags, int64_t cap)
+{
+    int64_t n = 0;
+    for (int q = 0; q < m->P; ++q) {
+        int64_t c = o_get_walkers(m->sh[q], n < cap ? dets + (size_t)n * m->W : dets, n < cap ? sgns + (size_t)n * m->S : sgns,
+                                  n < cap ? flags + n : flags, NULL, NULL, cap - n > 0 ? cap - n : 0);
+        n += c;
+    }
+    return n;
+}
+
+/* the core space, shared by the shards: the Hamiltonian rows are built by
+   the calling thread (o_set_core's rule), each core det is placed in its
+   owner shard */
+int32_t o_mt_set_core(omt_t *m, const uint64_t *dets, int64_t n, int32_t fp32, int64_t *nnz_out)
+{
+    const int W = m->W, S = m->S;
+    m->ncore = n;
+    m->core_det = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n * W);
+    memcpy(m->core_det, dets, sizeof(uint64_t) * (size_t)n * W);
+    m->rowptr = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+    core_rows_all(m, 0, fp32);
+    for (int64_t i = 0; i < n; ++i) m->rowptr[i + 1] += m->rowptr[i];
+    int64_t nnz = m->rowptr[n];
+    m->cols = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+    m->vals = (double *)malloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
+    core_rows_all(m, 1, fp32);
+    m->cx = (double *)calloc((size_t)n * S, sizeof(double));
+    m->cy = (double *)calloc((size_t)n * S, sizeof(double));
+    m->core_owner = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    m->core_row = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    uint64_t cflags = FLAG_CORE;
+    if (m->s.t_trunc_init)
+        for (int r = 0; r < S; ++r) cflags |= 1ull << r;
+    double zero[O_MAX_REPLICAS] = {0};
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t *d = dets + (size_t)i * W;
+        int q = owner_of(m, d);
+        ostate_t *st = m->sh[q];
+        st->ifree_start = st->ifree_end = 0;
+        int64_t b = bucket_of(st, d);
+        int64_t pos = hash_lookup(st, d, b);
+        if (pos < 0) {
+            int32_t nI[O_MAX_NEL];
+            o_decode(d, W, nI);
+            int e = add_new_hash_det(st, d, zero, cflags, o_diag_helement(&st->s, nI), o_offdiag_helement(&st->s, d), b);
+            if (e) return e;
+            pos = hash_lookup(st, d, b);
+        } else {
+            *row_flag(st, cur_row(st, pos)) |= cflags;
+        }
+        m->core_owner[i] = q;
+        m->core_row[i] = pos;
+    }
+    m->core_sorted = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n * W);
+    memcpy(m->core_sorted, dets, sizeof(uint64_t) * (size_t)n * W);
+    qsort(m->core_sorted, (size_t)n, sizeof(uint64_t) * W, W == 1 ? cmp_det_w1 : cmp_det_w2);
+    m->ci.dets = m->core_sorted;
+    m->ci.n = n;
+    m->ci.W = W;
+    if (nnz_out) *nnz_out = nnz;
+    return 0;
+}
+
+int32_t o_mt_iterate(omt_t *m, uint32_t iter, const double *diag_sft, ostats_t *out)
+{
+    m->iter = iter;
+    for (int r = 0; r < m->S; ++r) m->sft[r] = diag_sft[r];
+    m->phase = 0;
+    pthread_barrier_wait(&m->bar);  /* start */
+    shard_iterate(m, 0);
+    pthread_barrier_wait(&m->bar);  /* done */
+    memset(out, 0, sizeof(*out));
+    int err = 0;
+    for (int q = 0; q < m->P; ++q) {
+        const ostats_t *o = &m->st_out[q];
+        const double *a = (const double *)o;
+        double *b = (double *)out;
+        const int nd = (int)(offsetof(ostats_t, tot_walkers) / sizeof(double));
+        const int i_max = (int)(offsetof(ostats_t, max_cyc_spawn) / sizeof(double));
+        const int i_bs = (int)(offsetof(ostats_t, bloom_size) / sizeof(double));
+        for (int i = 0; i < nd; ++i) {
+            if (i == i_max || (i >= i_bs && i < i_bs + 3)) b[i] = a[i] > b[i] ? a[i] : b[i];
+            else b[i] += a[i];
+        }
+        out->tot_walkers += o->tot_walkers;
+        out->n_attempts += o->n_attempts;
+        out->n_spawned += o->n_spawned;
+        out->n_unique += o->n_unique;
+        out->n_new += o->n_new;
+        if (m->st_err[q] && !err) err = m->st_err[q];
+    }
+    out->err = err;
+    return err;
 }

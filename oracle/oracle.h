@@ -144,6 +144,15 @@ int32_t o_annihilate_list(ostate_t *st, uint32_t iter, const uint64_t *sdet, con
                           uint64_t *merged_det, double *merged_sgn, uint32_t *merged_flags,
                           int64_t *n_merged);
 
+/* --- hash-sharded iteration over P threads (the reference's one rank per core) --- */
+typedef struct omt omt_t;
+omt_t *o_mt_create(const osys_t *sys, int32_t nshards);
+void o_mt_destroy(omt_t *m);
+int32_t o_mt_add_walkers(omt_t *m, const uint64_t *dets, const double *sgns, const uint32_t *flags, int64_t n);
+int64_t o_mt_get_walkers(omt_t *m, uint64_t *dets, double *sgns, uint32_t *flags, int64_t cap);
+int32_t o_mt_set_core(omt_t *m, const uint64_t *dets, int64_t n, int32_t fp32, int64_t *nnz);
+int32_t o_mt_iterate(omt_t *m, uint32_t iter, const double *diag_sft, ostats_t *out);
+
 #ifdef __cplusplus
 }
 #endif

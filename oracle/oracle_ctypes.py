@@ -96,6 +96,12 @@ def lib():
             "o_core_csr": (ctypes.c_int64, [_vp, _vp, _vp, _vp]),
             "o_gen_excit_ctx": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32,
                                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+            "o_mt_create": (_vp, [_vp, ctypes.c_int32]),
+            "o_mt_destroy": (None, [_vp]),
+            "o_mt_add_walkers": (ctypes.c_int32, [_vp, _vp, _vp, _vp, ctypes.c_int64]),
+            "o_mt_get_walkers": (ctypes.c_int64, [_vp, _vp, _vp, _vp, ctypes.c_int64]),
+            "o_mt_set_core": (ctypes.c_int32, [_vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp]),
+            "o_mt_iterate": (ctypes.c_int32, [_vp, ctypes.c_uint32, _vp, _vp]),
         }
         for n, (r, a) in sig.items():
             f = getattr(L, n)
@@ -350,3 +356,65 @@ def make_single(nI, elec, tgt):
     ex = np.zeros(2, dtype=np.int32)
     par = lib().o_make_single(p(a), len(a), elec, tgt, p(nJ), p(ex))
     return nJ, ex, par
+
+
+class ShardedOracle:
+    """The oracle's iteration over P hash-sharded states on P threads (one
+    per host core, as the reference runs one MPI rank per core).  Same
+    system set-up as Oracle; walkers are routed to their owner shard."""
+
+    def __init__(self, spec, tau, nshards, max_walkers=None, max_spawned=None):
+        base = Oracle(spec, tau, max_walkers, max_spawned)  # system tables and PCHB build
+        self.base = base
+        self.L = base.L
+        self.W, self.S, self.ref, self.hii = base.W, base.S, base.ref, base.hii
+        self.P = int(nshards)
+        self.m = self.L.o_mt_create(ctypes.byref(base.sys), self.P)
+        if not self.m:
+            raise RuntimeError("o_mt_create failed")
+        self.out = OStats()
+        self.ncore = 0
+
+    def close(self):
+        if getattr(self, "m", None):
+            self.L.o_mt_destroy(self.m)
+            self.m = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_walkers_bulk(self, dets, signs, flags):
+        dets = np.ascontiguousarray(dets, dtype=np.int64).reshape(-1, self.W).view(np.uint64)
+        signs = np.ascontiguousarray(signs, dtype=np.float64).reshape(-1, self.S)
+        flags = np.ascontiguousarray(flags, dtype=np.uint32)
+        rc = self.L.o_mt_add_walkers(self.m, p(dets), p(signs), p(flags), dets.shape[0])
+        if rc:
+            raise RuntimeError(f"o_mt_add_walkers {rc}")
+
+    def set_core(self, dets, fp32=False):
+        dets = np.ascontiguousarray(dets, dtype=np.int64).reshape(-1, self.W).view(np.uint64)
+        nnz = ctypes.c_int64()
+        rc = self.L.o_mt_set_core(self.m, p(dets), dets.shape[0], int(bool(fp32)), ctypes.byref(nnz))
+        if rc:
+            raise RuntimeError(f"o_mt_set_core {rc}")
+        self.ncore = dets.shape[0]
+        return nnz.value
+
+    def walkers(self, cap=20_000_000):
+        n = self.L.o_mt_get_walkers(self.m, None, None, None, 0)
+        d = np.zeros((max(n, 1), self.W), dtype=np.uint64)
+        s = np.zeros((max(n, 1), self.S))
+        f = np.zeros(max(n, 1), dtype=np.uint32)
+        n = self.L.o_mt_get_walkers(self.m, p(d), p(s), p(f), n)
+        return d[:n].view(np.int64), s[:n], f[:n]
+
+    def iterate(self, it, shift):
+        sh = np.ascontiguousarray(np.broadcast_to(np.asarray(shift, dtype=np.float64), (self.S,)))
+        rc = self.L.o_mt_iterate(self.m, int(it) & 0xFFFFFFFF, p(sh), ctypes.byref(self.out))
+        d = self.out.to_dict(self.S)
+        d["rc"] = rc
+        return d
+
