@@ -177,6 +177,21 @@ int ngpu_upload_walkers(ngpu_t *h, const int64_t *rows, int64_t n);
    det_diagH (= H_ii - Hii) and det_offdiagH; *n = number of occupied rows. */
 int ngpu_download_walkers(ngpu_t *h, int64_t *rows, double *diag_h, double *offdiag_h,
                           int64_t cap, int64_t *n);
+/* Allocation-free form of ngpu_download_walkers: the same rows, in the same
+   order, compacted on the device into caller-owned DEVICE memory (rows:
+   cap rows of nbasis/64+1+n_replicas+1 int64; diag_h, offdiag_h: cap doubles
+   or NULL), with the number of occupied rows written to the device int64 *n.
+   No counterpart in the reference; it is for callers that cannot allocate
+   (code under HIP graph capture, checkpoint writers inside a serving loop):
+   it calls no hipMalloc/hipFree and no host synchronisation, and runs
+   asynchronously on the engine's stream, ordered after queued iterations.
+   Rows past cap are counted in *n but not written.  Its scratch is a fixed
+   128 KiB array per GPU, held until the process ends and shared by every
+   call on that GPU: calls that overlap on one GPU race.  An engine whose
+   walker table holds more than 67,108,864 slots (MaxWalkersPart) is refused
+   with NGPU_ERR_ARG. */
+int ngpu_download_walkers_dev(ngpu_t *h, int64_t *rows, double *diag_h, double *offdiag_h,
+                              int64_t cap, int64_t *n);
 
 /* Semistochastic core space (init_semi_stochastic, semi_stoch_gen.F90:425-445;
    semi_stoch_procs.F90:1095-1120).  The ncore determinants (W words each)

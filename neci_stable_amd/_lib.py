@@ -106,6 +106,7 @@ _SIGS = [
     ("ngpu_set_core", ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int32, P(ctypes.c_int64)]),
     ("ngpu_upload_walkers", ctypes.c_int, [_vp, _vp, ctypes.c_int64]),
     ("ngpu_download_walkers", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int64, P(ctypes.c_int64)]),
+    ("ngpu_download_walkers_dev", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int64, _vp]),
     ("ngpu_iterate", ctypes.c_int, [_vp, P(IterInT), P(IterOutT)]),
     ("ngpu_get_stats", ctypes.c_int, [_vp, P(IterOutT)]),
     ("ngpu_get_cycle_stats", ctypes.c_int, [_vp, P(IterOutT), P(IterOutT), P(ctypes.c_int32)]),

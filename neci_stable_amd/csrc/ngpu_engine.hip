@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ngpu_kernels.h"
@@ -41,8 +42,42 @@ const int kNeciDeterm[2] = {19, 20};
 
 }  // namespace
 
+// Scratch of the allocation-free download (ngpu_download_walkers_dev): the
+// rows to keep per tile of kDlTile walker slots, scanned in place into each
+// tile's output offset, then the total at [kDlMaxTiles].  One copy per GPU,
+// held until the process ends and shared by every call on that GPU, so calls
+// that overlap on one GPU race on it.  Host code never passes this name to a
+// kernel: the address each device holds it at comes from hipGetSymbolAddress
+// (scratch_setup, called by ngpu_create with the device current).
+constexpr int64_t kDlTile = 4096;                       // walker slots per tile (one block)
+constexpr int64_t kDlMaxTiles = 16384;                  // tiles the scratch holds (128 KiB)
+constexpr int64_t kDlMaxSlots = kDlTile * kDlMaxTiles;  // 67,108,864 walker slots
+static __device__ __align__(256) unsigned long long scratch[kDlMaxTiles + 32];
+
+namespace {
+constexpr int kMaxDevices = 64;
+unsigned long long* g_scratch[kMaxDevices] = {nullptr};  // device address of scratch, per device
+std::mutex g_scratch_mu;
+
+// the current device's address of scratch, looked up once per device
+int scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return NGPU_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[dev]) return NGPU_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    fprintf(stderr, "[neci_gpu] hipGetSymbolAddress(scratch) on device %d: %s\n", dev, hipGetErrorString(e));
+    return NGPU_ERR_HIP;
+  }
+  g_scratch[dev] = (unsigned long long*)p;
+  return NGPU_OK;
+}
+}  // namespace
+
 struct ngpu_engine {
   ngpu_sys_t sys{};
+  int dev = 0;  // the device the engine's memory and stream live on
   int W = 1, S = 1;
   int nb_chunk = 1024;  // blocks of the chunked kernels (prepass, scans, radix)
   int nb_grid = 1024;   // blocks of the grid-stride kernels
@@ -719,6 +754,11 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
     delete h;
     return NGPU_ERR_HIP;
   }
+  h->dev = dev;
+  if (int src = scratch_setup(dev)) {  // the allocation-free download's scratch on this device
+    delete h;
+    return src;
+  }
   h->nb_chunk = std::max(256, 4 * prop.multiProcessorCount);
   h->nb_grid = std::max(512, 4 * prop.multiProcessorCount);
   h->nb_merge = std::max(384, 3 * prop.multiProcessorCount);
@@ -1312,6 +1352,135 @@ int ngpu_download_walkers(ngpu_t* h, int64_t* rows, double* diag_h, double* offd
   *n = m;
   if (!rows) return NGPU_OK;  // size query
   return m > cap ? NGPU_ERR_ARG : NGPU_OK;
+}
+
+namespace {
+
+// the row test of ngpu_download_walkers: a slot in use that holds a nonzero
+// sign in some replica or is a core determinant
+__device__ inline bool dl_keep(const Walk& wk, int64_t slot, int S) {
+  const uint32_t f = wk.flags(slot);
+  if (f & kFlagRemoved) return false;
+  if (f & kFlagCore) return true;
+  for (int r = 0; r < S; ++r)
+    if (!(fabs(wk.sgn(slot, r)) < 1e-12)) return true;
+  return false;
+}
+
+// engine flag bits -> NECI flag bit numbers of the boundary layout
+struct FlagMap {
+  int init[16], determ[16], maxr;
+};
+
+// rows kept per tile of kDlTile slots (one block per tile; tiles past the
+// used slots count zero)
+__global__ void __launch_bounds__(kBlock) k_dl_count(Walk wk, const Counters* cnt, int S, unsigned long long* counts) {
+  __shared__ int64_t lds[kBlock / kWave];
+  const int64_t ns = (int64_t)cnt->nslots;
+  const int64_t b0 = (int64_t)blockIdx.x * kDlTile;
+  const int64_t b1 = std::min<int64_t>(b0 + kDlTile, ns);
+  int64_t c = 0;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += kBlock) c += dl_keep(wk, i, S) ? 1 : 0;
+  int64_t tot = 0;
+  (void)block_exclusive_scan(c, lds, tot);
+  if (threadIdx.x == 0) counts[blockIdx.x] = (unsigned long long)tot;
+}
+
+// exclusive scan of the ntiles counts in place (one block of 1024 threads,
+// ntiles <= kDlMaxTiles: 16 tiles per thread); the total goes to
+// counts[kDlMaxTiles] and to *n_out
+__global__ void __launch_bounds__(1024) k_dl_scan(unsigned long long* counts, int64_t ntiles, int64_t* n_out) {
+  constexpr int kT = 1024, kPer = (int)(kDlMaxTiles / kT);
+  __shared__ unsigned long long part[kT];
+  const int64_t lo = (int64_t)threadIdx.x * kPer;
+  unsigned long long v[kPer], s = 0;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    v[q] = lo + q < ntiles ? counts[lo + q] : 0ull;
+    s += v[q];
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < kT; o <<= 1) {  // inclusive Hillis-Steele scan of the per-thread sums
+    unsigned long long y = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0ull;
+    __syncthreads();
+    part[threadIdx.x] += y;
+    __syncthreads();
+  }
+  unsigned long long run = part[threadIdx.x] - s;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    if (lo + q < ntiles) counts[lo + q] = run;
+    run += v[q];
+  }
+  if (threadIdx.x == kT - 1) {
+    counts[kDlMaxTiles] = part[kT - 1];
+    *n_out = (int64_t)part[kT - 1];
+  }
+}
+
+// the kept rows of each tile, in slot order, from the tile's offset on; rows
+// at positions >= cap are counted but not written
+__global__ void __launch_bounds__(kBlock) k_dl_write(Walk wk, const Counters* cnt, int W, int S, FlagMap fm,
+                                                     const unsigned long long* offsets, int64_t* rows, double* diag_h,
+                                                     double* offdiag_h, int64_t cap) {
+  __shared__ int64_t lds[kBlock / kWave];
+  const int64_t ns = (int64_t)cnt->nslots;
+  const int64_t b0 = (int64_t)blockIdx.x * kDlTile;
+  const int RL = W + S + 1;
+  int64_t pos0 = (int64_t)offsets[blockIdx.x];
+  for (int64_t t = 0; t < kDlTile && b0 + t < ns; t += kBlock) {  // block-uniform trip count
+    const int64_t i = b0 + t + threadIdx.x;
+    const bool keep = i < ns && dl_keep(wk, i, S);
+    int64_t tot = 0;
+    const int64_t pos = pos0 + block_exclusive_scan(keep ? 1 : 0, lds, tot);
+    if (keep && pos < cap) {
+      int64_t* row = rows + pos * RL;
+      for (int w = 0; w < W; ++w) row[w] = (int64_t)wk.keys[i * W + w];
+      for (int r = 0; r < S; ++r) row[W + r] = __double_as_longlong(wk.sgn(i, r));
+      const uint32_t f = wk.flags(i);
+      uint64_t nf = 0;
+      for (int r = 0; r < S && r < fm.maxr; ++r) {
+        if ((f >> r) & 1u) nf |= 1ull << fm.init[r];
+        if (f & kFlagCore) nf |= 1ull << fm.determ[r];
+      }
+      row[W + S] = (int64_t)nf;
+      if (diag_h) diag_h[pos] = wk.diagH[i];
+      if (offdiag_h) offdiag_h[pos] = wk.offdiagH[i];
+    }
+    pos0 += tot;
+  }
+}
+
+}  // namespace
+
+int ngpu_download_walkers_dev(ngpu_t* h, int64_t* rows, double* diag_h, double* offdiag_h, int64_t cap, int64_t* n) {
+  if (!h || !n || cap < 0 || (cap > 0 && !rows)) return NGPU_ERR_ARG;
+  if (h->wk.cap > kDlMaxSlots) {
+    fprintf(stderr,
+            "[neci_gpu] ngpu_download_walkers_dev: the walker table has %lld slots, more than the %lld its "
+            "fixed scratch covers; use ngpu_download_walkers\n",
+            (long long)h->wk.cap, (long long)kDlMaxSlots);
+    return NGPU_ERR_ARG;
+  }
+  unsigned long long* sp = (h->dev >= 0 && h->dev < kMaxDevices) ? g_scratch[h->dev] : nullptr;
+  if (!sp) return NGPU_ERR_STATE;
+  FlagMap fm{};
+  const bool neci = h->sys.flag_layout == NGPU_FLAGS_NECI;
+  fm.maxr = neci ? 2 : 16;
+  for (int r = 0; r < fm.maxr; ++r) {
+    fm.init[r] = neci ? kNeciInit[r] : kMneciInit[r];
+    fm.determ[r] = neci ? kNeciDeterm[r] : kMneciDeterm[r];
+  }
+  // every slot in use lies below the table's capacity, so its tiles cover them
+  const int64_t ntiles = std::max<int64_t>(1, (h->wk.cap + kDlTile - 1) / kDlTile);
+  hipLaunchKernelGGL(k_dl_count, dim3((unsigned)ntiles), dim3(kBlock), 0, h->stream, h->wk, (const Counters*)h->cnt,
+                     h->S, sp);
+  hipLaunchKernelGGL(k_dl_scan, dim3(1), dim3(1024), 0, h->stream, sp, ntiles, n);
+  hipLaunchKernelGGL(k_dl_write, dim3((unsigned)ntiles), dim3(kBlock), 0, h->stream, h->wk, (const Counters*)h->cnt,
+                     h->W, h->S, fm, (const unsigned long long*)sp, rows, diag_h, offdiag_h, cap);
+  HIP_OK(hipGetLastError());
+  return NGPU_OK;
 }
 
 namespace {

@@ -121,6 +121,18 @@ class Engine:
         flags = rows[:m, -1].copy()
         return dets, signs, flags, dh[:m].copy(), oh[:m].copy()
 
+    def download_dev(self, rows_ptr, diag_ptr, offdiag_ptr, cap, n_ptr):
+        """ngpu_download_walkers_dev: rows compacted into device memory at the
+        given addresses (int or None), asynchronous on the engine's stream."""
+        check(self.lib.ngpu_download_walkers_dev(self.h, rows_ptr, diag_ptr, offdiag_ptr, int(cap), n_ptr),
+              "ngpu_download_walkers_dev")
+
+    def sync(self):
+        """Wait for the engine's queued work (ngpu_get_error waits on its stream)."""
+        st = ctypes.c_int32()
+        check(self.lib.ngpu_get_error(self.h, ctypes.byref(st), None), "ngpu_get_error")
+        return st.value
+
     # -- iteration
     def iterate(self, it, tau, shift, wait=True):
         inp = IterInT()
