@@ -57,3 +57,48 @@ def sort_walkers(dets, *arrs):
     keys = [dets[:, w] for w in range(dets.shape[1] - 1, -1, -1)]
     o = np.lexsort(keys)
     return (dets[o],) + tuple(np.asarray(a)[o] for a in arrs)
+
+
+class DeviceBuffer:
+    """Device memory from the HIP runtime the engine library itself links
+    (/opt/rocm/lib/libamdhip64.so.7), for tests of the entry points that take
+    device pointers.  (torch carries its own HIP runtime; two runtimes in one
+    process do not share the device, so tests avoid torch here.)"""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        import ctypes
+        if cls._hip is None:
+            h = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+            h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+            h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+            h.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+            h.hipFree.argtypes = [ctypes.c_void_p]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, nbytes, fill_byte=None):
+        import ctypes
+        self.nbytes = max(int(nbytes), 8)
+        self.ptr = ctypes.c_void_p()
+        assert self.hip().hipMalloc(ctypes.byref(self.ptr), self.nbytes) == 0
+        if fill_byte is not None:
+            assert self.hip().hipMemset(self.ptr, int(fill_byte), self.nbytes) == 0
+            assert self.hip().hipDeviceSynchronize() == 0
+
+    @property
+    def addr(self):
+        return self.ptr.value
+
+    def to_host(self, dtype, count):
+        out = np.zeros(max(int(count), 1), dtype=dtype)
+        assert self.hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) == 0
+        return out[:count]
+
+    def free(self):
+        if self.ptr:
+            self.hip().hipFree(self.ptr)
+            self.ptr = None

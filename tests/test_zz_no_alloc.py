@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from common import hubbard_spec, mol_spec
+from common import DeviceBuffer, hubbard_spec, mol_spec
 
 pytestmark = pytest.mark.gpu
 
@@ -33,16 +33,18 @@ def _grown(spec, iters, tau, n0, core=0):
 
 
 def _device_download(eng, cap):
-    import torch
     RL = eng.W + eng.S + 1
-    rows = torch.full((max(cap, 1), RL), -7, dtype=torch.int64, device="cuda")
-    dh = torch.full((max(cap, 1),), -7.0, dtype=torch.float64, device="cuda")
-    oh = torch.full((max(cap, 1),), -7.0, dtype=torch.float64, device="cuda")
-    n = torch.full((1,), -1, dtype=torch.int64, device="cuda")
-    torch.cuda.synchronize()  # the fills are on torch's stream, the download on the engine's
-    eng.download_dev(rows.data_ptr(), dh.data_ptr(), oh.data_ptr(), cap, n.data_ptr())
+    rows = DeviceBuffer(max(cap, 1) * RL * 8, fill_byte=0xA5)
+    dh = DeviceBuffer(max(cap, 1) * 8, fill_byte=0xA5)
+    oh = DeviceBuffer(max(cap, 1) * 8, fill_byte=0xA5)
+    n = DeviceBuffer(8, fill_byte=0xFF)
+    eng.download_dev(rows.addr, dh.addr, oh.addr, cap, n.addr)
     assert eng.sync() == 0
-    return rows.cpu().numpy(), dh.cpu().numpy(), oh.cpu().numpy(), int(n.cpu()[0])
+    out = (rows.to_host(np.int64, max(cap, 1) * RL).reshape(-1, RL), dh.to_host(np.float64, max(cap, 1)),
+           oh.to_host(np.float64, max(cap, 1)), int(n.to_host(np.int64, 1)[0]))
+    for b in (rows, dh, oh, n):
+        b.free()
+    return out
 
 
 def _host_download(eng):
@@ -71,8 +73,6 @@ def _check_same(eng, label):
 
 
 def test_download_dev_matches_host_download_twice():
-    import torch
-    assert torch.cuda.is_available()
     # a two-replica two-word molecule with a core space (core flags, NECI's
     # multi-replica flag bits), then a small Hubbard chain
     big = _grown(mol_spec(33, 4, rank=4, mseed=3, n_replicas=2, init_walkers=20000), 60, 0.01, 200.0, core=40)
@@ -95,39 +95,38 @@ def test_download_dev_matches_host_download_twice():
 
 def test_download_dev_after_iterations_without_sync():
     """Queued asynchronous iterations are ordered before the download."""
-    import torch
     eng = _grown(hubbard_spec(12, 12, 4.0, init_walkers=3000), 5, 0.01, 300.0)
     try:
         for it in range(6, 26):
             eng.iterate(it, 0.01, np.zeros(1), wait=False)
         RL = eng.W + eng.S + 1
         cap = int(eng.max_walkers)
-        rows = torch.zeros((cap, RL), dtype=torch.int64, device="cuda")
-        n = torch.zeros((1,), dtype=torch.int64, device="cuda")
-        torch.cuda.synchronize()
-        eng.download_dev(rows.data_ptr(), None, None, cap, n.data_ptr())
+        rows = DeviceBuffer(cap * RL * 8)
+        n = DeviceBuffer(8, fill_byte=0)
+        eng.download_dev(rows.addr, None, None, cap, n.addr)
         assert eng.sync() == 0
         hr, _, _ = _host_download(eng)
-        m = int(n.cpu()[0])
+        m = int(n.to_host(np.int64, 1)[0])
         assert m == len(hr) > 100
-        assert np.array_equal(rows[:m].cpu().numpy(), hr)
+        assert np.array_equal(rows.to_host(np.int64, m * RL).reshape(m, RL), hr)
+        rows.free()
+        n.free()
     finally:
         eng.close()
 
 
 def test_download_dev_refuses_oversized_tables():
-    import torch
     from neci_stable_amd._lib import NgpuError
     from neci_stable_amd.engine import Engine
     spec = hubbard_spec(10, 10, 4.0, init_walkers=500, max_walkers=(1 << 26) + 1)
     eng = Engine(spec)
-    n = torch.full((1,), -1, dtype=torch.int64, device="cuda")
-    torch.cuda.synchronize()
+    n = DeviceBuffer(8, fill_byte=0xFF)
     try:
         with pytest.raises(NgpuError) as e:
-            eng.download_dev(None, None, None, 0, n.data_ptr())
+            eng.download_dev(None, None, None, 0, n.addr)
         assert e.value.code == 4
         assert eng.sync() == 0
-        assert int(n.cpu()[0]) == -1  # nothing was launched
+        assert int(n.to_host(np.int64, 1)[0]) == -1  # nothing was launched
     finally:
+        n.free()
         eng.close()
