@@ -445,13 +445,22 @@ static inline int alias_sample(const double *bias, const int32_t *alias, int n, 
 
 /* PCHB tables: AB rows per IJ (fuseIndex over spin orbitals) and J rows per I */
 static struct {
-    int nbi, ijmax;
+    int nbi, ijmax, holes;
     double *ab_bias, *ab_prob;
     int32_t *ab_alias;
     uint8_t *ab_ok;
     double *j_bias, *j_prob;
     int32_t *j_alias;
     uint8_t *j_ok;
+    /* FULL-FULL particles: first particle I over all spin orbitals */
+    double *i_bias, *i_prob;
+    int32_t *i_alias;
+    uint8_t i_ok;
+    /* FULL-FULL holes: first hole A per IJ, row (IJ-1); second hole B per
+       (IJ, A), row (IJ-1)*nbi + (A-1); each row over the nbi spin orbitals */
+    double *a_bias, *a_prob, *b_bias, *b_prob;
+    int32_t *a_alias, *b_alias;
+    uint8_t *a_ok, *b_ok;
 } P;
 
 static inline int fuse(int x, int y) { return x < y ? x + y * (y - 1) / 2 : y + x * (x - 1) / 2; }
@@ -460,7 +469,78 @@ void o_free_pchb(void)
 {
     free(P.ab_bias); free(P.ab_prob); free(P.ab_alias); free(P.ab_ok);
     free(P.j_bias); free(P.j_prob); free(P.j_alias); free(P.j_ok);
+    free(P.i_bias); free(P.i_prob); free(P.i_alias);
+    free(P.a_bias); free(P.a_prob); free(P.a_alias); free(P.a_ok);
+    free(P.b_bias); free(P.b_prob); free(P.b_alias); free(P.b_ok);
     memset(&P, 0, sizeof(P));
+}
+
+/* alias table and probabilities of one sampler row; returns 0 (row left
+   empty, every probability 0) when the weights sum to zero: init_AliasSampler_t
+   (aliasSampling.F90:376-398) then leaves the sampler unassociated */
+static int alias_row(const double *w, int n, double *bias, int32_t *alias, double *prob)
+{
+    double sw = 0.0;
+    for (int q = 0; q < n; ++q) sw += w[q];
+    if (fabs(sw) <= EPS) return 0;
+    o_alias_build(w, n, bias, alias, prob);
+    return 1;
+}
+
+/* PCHB weight |<IJ||AB>| of a spin-allowed double, 0 otherwise
+   (get_PCHB_weight, gasci_pchb_doubles_select_particles.fpp:596-609) */
+static double pchb_weight(const osys_t *s, int I, int J, int A, int B)
+{
+    int t1 = A < B ? A : B, t2 = A < B ? B : A;
+    int ms_s = (is_beta(I) ? -1 : 1) + (is_beta(J) ? -1 : 1);
+    int ms_t = (is_beta(t1) ? -1 : 1) + (is_beta(t2) ? -1 : 1);
+    return ms_s == ms_t ? fabs(sltcnd_2(s, I, J, t1, t2)) : 0.0;
+}
+
+/* GAS_doubles_PCHB_compute_samplers of the fully weighted holes
+   (gasci_pchb_doubles_spinorb_fullyweighted.fpp:317-391): per (I < J), per A,
+   the B row w_B(B) = |<IJ||AB>|; the A row w_A(A) = sum(w_B); the particle
+   weight IJ_weights(I, J) = sum(w_A). */
+static void build_full_holes(const osys_t *s, double *ijw)
+{
+    int n = P.nbi, ijmax = P.ijmax;
+    P.a_bias = (double *)calloc((size_t)ijmax * n, sizeof(double));
+    P.a_prob = (double *)calloc((size_t)ijmax * n, sizeof(double));
+    P.a_alias = (int32_t *)calloc((size_t)ijmax * n, sizeof(int32_t));
+    P.a_ok = (uint8_t *)calloc((size_t)ijmax, 1);
+    P.b_bias = (double *)calloc((size_t)ijmax * n * n, sizeof(double));
+    P.b_prob = (double *)calloc((size_t)ijmax * n * n, sizeof(double));
+    P.b_alias = (int32_t *)calloc((size_t)ijmax * n * n, sizeof(int32_t));
+    P.b_ok = (uint8_t *)calloc((size_t)ijmax * n, 1);
+    double *wa = (double *)malloc(sizeof(double) * (size_t)n);
+    double *wb = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int I = 1; I <= n; ++I)
+        for (int J = I + 1; J <= n; ++J) {
+            int IJ = fuse(I, J);
+            for (int q = 0; q < n; ++q) wa[q] = 0.0;
+            for (int A = 1; A <= n; ++A) {
+                if (A == I || A == J) continue;
+                for (int q = 0; q < n; ++q) wb[q] = 0.0;
+                for (int B = 1; B <= n; ++B) {
+                    if (B == I || B == J || B == A) continue;
+                    wb[B - 1] = pchb_weight(s, I, J, A, B);
+                }
+                size_t row = (size_t)(IJ - 1) * n + (A - 1);
+                P.b_ok[row] = (uint8_t)alias_row(wb, n, P.b_bias + row * n, P.b_alias + row * n,
+                                                 P.b_prob + row * n);
+                double sw = 0.0;
+                for (int q = 0; q < n; ++q) sw += wb[q];
+                wa[A - 1] = sw;
+            }
+            size_t row = (size_t)(IJ - 1);
+            P.a_ok[row] = (uint8_t)alias_row(wa, n, P.a_bias + row * n, P.a_alias + row * n, P.a_prob + row * n);
+            double sw = 0.0;
+            for (int q = 0; q < n; ++q) sw += wa[q];
+            ijw[(I - 1) + (size_t)(J - 1) * n] = sw;
+            ijw[(J - 1) + (size_t)(I - 1) * n] = sw;
+        }
+    free(wa);
+    free(wb);
 }
 
 /* GAS_doubles_PCHB_spinorb_compute_samplers (gasci_pchb_doubles_spinorb_fastweighted.fpp:280-347)
@@ -473,6 +553,7 @@ int32_t o_build_pchb(osys_t *s)
     int ijmax = fuse(n, n);
     P.nbi = n;
     P.ijmax = ijmax;
+    P.holes = s->holes;
     size_t sz = (size_t)ijmax * (size_t)ijmax;
     P.ab_bias = (double *)calloc(sz, sizeof(double));
     P.ab_prob = (double *)calloc(sz, sizeof(double));
@@ -484,7 +565,8 @@ int32_t o_build_pchb(osys_t *s)
     P.j_ok = (uint8_t *)calloc((size_t)n, 1);
     double *w = (double *)malloc(sizeof(double) * (size_t)ijmax);
     double *ijw = (double *)calloc((size_t)n * n, sizeof(double));
-    for (int I = 1; I <= n; ++I)
+    if (s->holes == O_HOLES_FULL_FULL) build_full_holes(s, ijw);
+    else for (int I = 1; I <= n; ++I)
         for (int J = I + 1; J <= n; ++J) {
             for (int q = 0; q < ijmax; ++q) w[q] = 0.0;
             for (int A = 1; A <= n; ++A) {
@@ -519,6 +601,21 @@ int32_t o_build_pchb(osys_t *s)
             o_alias_build(row, n, P.j_bias + off, P.j_alias + off, P.j_prob + off);
             P.j_ok[I - 1] = 1;
         }
+    }
+    /* the I sampler of the fully weighted particles: sum(weights(:, :), dim=1)
+       (init_PC_WeightedParticles_t, gasci_pchb_doubles_select_particles.fpp:287-297) */
+    P.i_bias = (double *)calloc((size_t)n, sizeof(double));
+    P.i_prob = (double *)calloc((size_t)n, sizeof(double));
+    P.i_alias = (int32_t *)calloc((size_t)n, sizeof(int32_t));
+    {
+        double *wi = (double *)calloc((size_t)n, sizeof(double));
+        for (int I = 1; I <= n; ++I) {
+            double sw = 0.0;
+            for (int q = 0; q < n; ++q) sw += ijw[(size_t)(I - 1) * n + q];
+            wi[I - 1] = sw;
+        }
+        P.i_ok = (uint8_t)alias_row(wi, n, P.i_bias, P.i_alias, P.i_prob);
+        free(wi);
     }
     free(w);
     free(ijw);
@@ -736,6 +833,266 @@ static double seq_next(draw_seq_t *q)
     return q->c[j & 1];
 }
 
+/* ------------------------------------ weighted PCHB particles and holes */
+
+static inline int near_zero_d(double x) { return fabs(x) <= EPS; }
+
+/* redraws of a constrained alias draw before it is given up as a null
+   excitation (the reference redraws without bound; each redraw succeeds
+   with probability >= 0.1, so the bound is never reached in practice) */
+#define O_MAX_REDRAW 1024
+
+/* constrained_sample_fast (aliasSampling.F90:505-535) of one sampler row
+   (bias, alias, prob over n entries; prob == NULL: an empty sampler) restricted
+   to the ascending set contain[0..nc) whose bitmask is mask, with the given
+   renormalisation: below redrawing_cutoff = 0.1 (:152) the CDF sampler over
+   prob(contain)/renorm (CDF_sampling.fpp:56-110: first cumulated value >= r),
+   else alias draws until one lies in the set.  Returns the drawn orbital (0:
+   none), its position in contain (1-based) and its probability.  A draw whose
+   probability is zero (a cumulated value at r == 0, or r beyond a rounded
+   last cumulated value, where the reference's search returns -1) is given up
+   as a null excitation. */
+static int constrained_sample(const double *bias, const int32_t *alias, const double *prob, int n, const int *contain,
+                              int nc, const uint64_t *mask, double renorm, draw_seq_t *q, int *pos, double *p)
+{
+    if (!prob || near_zero_d(renorm)) return 0;
+    if (renorm < 0.1) {
+        double r = seq_next(q), cum = 0.0;
+        for (int k = 0; k < nc; ++k) {
+            double pk = prob[contain[k] - 1] / renorm;
+            cum += pk;
+            if (!(cum < r)) {
+                if (pk == 0.0) return 0;
+                *pos = k + 1;
+                *p = pk;
+                return contain[k];
+            }
+        }
+        return 0;
+    }
+    for (int t = 0; t < O_MAX_REDRAW; ++t) {
+        int tgt = alias_sample(bias, alias, n, seq_next(q)) + 1;
+        if (!test_orb(mask, tgt)) continue;
+        int k = 0;
+        while (contain[k] != tgt) ++k;
+        *pos = k + 1;
+        *p = prob[tgt - 1] / renorm;
+        return *p == 0.0 ? 0 : tgt;
+    }
+    return 0;
+}
+
+/* sum of a row's probabilities over a set, in the set's order (sum(get_prob(..., contain))) */
+static double row_sum(const double *prob, const int *set, int ns)
+{
+    double r = 0.0;
+    if (!prob) return 0.0;
+    for (int k = 0; k < ns; ++k) r += prob[set[k] - 1];
+    return r;
+}
+
+static const double *j_row(int I) { return P.j_ok[I - 1] ? P.j_prob + (size_t)(I - 1) * P.nbi : NULL; }
+
+/* draw_PC_WeightedParticles_t (UNIF-FULL, gasci_pchb_doubles_select_particles.fpp:427-470)
+   and draw_PC_FullyWeightedParticles_t (FULL-FULL, :333-378): the ordered pair
+   (srcs[0], srcs[1]) and p({I, J}); 0 for a null draw */
+static int draw_weighted_particles(const osys_t *s, const uint64_t *ilut, const int32_t *nI, draw_seq_t *q,
+                                   int *elecs, int *srcs, double *p)
+{
+    const int nel = s->nel, n = P.nbi, full = s->particles == O_PART_FULL_FULL;
+    int set[O_MAX_NEL];
+    for (int k = 0; k < nel; ++k) set[k] = nI[k];
+    double pf[2] = {0.0, 0.0}, ps[2] = {0.0, 0.0}, rf = 0.0;
+    int pos = 0;
+    if (full) {
+        const double *ip = P.i_ok ? P.i_prob : NULL;
+        rf = row_sum(ip, set, nel);
+        srcs[0] = constrained_sample(P.i_bias, P.i_alias, ip, n, set, nel, ilut, rf, q, &pos, &pf[0]);
+        if (!srcs[0]) return 0;
+        elecs[0] = pos;
+    } else {
+        elecs[0] = (int)(seq_next(q) * (double)nel) + 1;
+        srcs[0] = nI[elecs[0] - 1];
+    }
+    const double *j1 = j_row(srcs[0]);
+    double rs1 = row_sum(j1, set, nel);
+    size_t o1 = (size_t)(srcs[0] - 1) * n;
+    srcs[1] = constrained_sample(P.j_bias + o1, P.j_alias + o1, j1, n, set, nel, ilut, rs1, q, &pos, &ps[0]);
+    if (!srcs[1]) return 0;
+    elecs[1] = pos;
+    const double *j2 = j_row(srcs[1]);
+    double rs2 = row_sum(j2, set, nel);
+    ps[1] = (!j2 || near_zero_d(rs2)) ? 0.0 : j2[srcs[0] - 1] / rs2;
+    if (full) {
+        pf[1] = near_zero_d(rf) ? 0.0 : P.i_prob[srcs[1] - 1] / rf;
+        *p = pf[0] * ps[0] + pf[1] * ps[1];
+    } else {
+        *p = (ps[0] + ps[1]) / (double)nel;
+    }
+    if (srcs[0] > srcs[1]) {
+        int t = srcs[0]; srcs[0] = srcs[1]; srcs[1] = t;
+        t = elecs[0]; elecs[0] = elecs[1]; elecs[1] = t;
+    }
+    return 1;
+}
+
+/* get_pgen_PC_WeightedParticles_t (:472-500) / get_pgen_PC_FullyWeightedParticles_t (:381-425) */
+static double pgen_weighted_particles(const osys_t *s, const int32_t *nI, int I, int J)
+{
+    const int nel = s->nel, full = s->particles == O_PART_FULL_FULL;
+    int set[O_MAX_NEL];
+    for (int k = 0; k < nel; ++k) set[k] = nI[k];
+    const double *jI = j_row(I), *jJ = j_row(J);
+    double rI = row_sum(jI, set, nel), rJ = row_sum(jJ, set, nel);
+    double psI = (!jI || near_zero_d(rI)) ? 0.0 : jI[J - 1] / rI;
+    double psJ = (!jJ || near_zero_d(rJ)) ? 0.0 : jJ[I - 1] / rJ;
+    if (!full) return (psI + psJ) / (double)nel;
+    const double *ip = P.i_ok ? P.i_prob : NULL;
+    double rf = row_sum(ip, set, nel);
+    double pfI = (!ip || near_zero_d(rf)) ? 0.0 : ip[I - 1] / rf;
+    double pfJ = (!ip || near_zero_d(rf)) ? 0.0 : ip[J - 1] / rf;
+    return pfI * psI + pfJ * psJ;
+}
+
+/* unoccupied spin orbitals (ascending) and their bitmask (get_unoccupied,
+   gasci_pchb_doubles_spinorb_fullyweighted.fpp:403-412) */
+static int unoccupied(const osys_t *s, const uint64_t *ilut, int *un, uint64_t *umask)
+{
+    const int n = s->nbasis, W = nwords(n);
+    int nu = 0;
+    for (int w = 0; w < W; ++w) umask[w] = 0;
+    for (int o = 1; o <= n; ++o)
+        if (!test_orb(ilut, o)) {
+            un[nu++] = o;
+            set_orb(umask, o);
+        }
+    return nu;
+}
+
+/* renormalisation of a hole row over the unoccupied orbitals: one minus the
+   occupied ones' share, or the direct sum when that is 1 or below 1e-5
+   (do_direct_calculation, aliasSampling.F90:967-1000) */
+static double hole_renorm(const double *row, const int32_t *nI, int nel, const int *un, int nu)
+{
+    int set[O_MAX_NEL];
+    for (int k = 0; k < nel; ++k) set[k] = nI[k];
+    double r = 1.0 - row_sum(row, set, nel);
+    if (r >= 1.0 || r < 1e-5) r = row_sum(row, un, nu);
+    return r;
+}
+
+static const double *a_row(int IJ) { return P.a_ok[IJ - 1] ? P.a_prob + (size_t)(IJ - 1) * P.nbi : NULL; }
+static const double *b_row(int IJ, int A)
+{
+    size_t r = (size_t)(IJ - 1) * P.nbi + (A - 1);
+    return P.b_ok[r] ? P.b_prob + r * P.nbi : NULL;
+}
+
+/* the fully weighted holes of GAS_doubles_PCHB_gen_exc
+   (gasci_pchb_doubles_spinorb_fullyweighted.fpp:172-223): (tgt[0], tgt[1]) in
+   drawing order and sum(p_first * p_second); 0 for a null draw */
+static int draw_full_holes(const osys_t *s, const uint64_t *ilut, const int32_t *nI, int I, int J, draw_seq_t *q,
+                           int *tgt, double *ph)
+{
+    const int n = P.nbi, nel = s->nel;
+    int un[128], pos = 0;
+    uint64_t umask[O_MAX_W];
+    int nu = unoccupied(s, ilut, un, umask);
+    int IJ = fuse(I, J);
+    const double *ar = a_row(IJ);
+    double rf = hole_renorm(ar, nI, nel, un, nu);
+    double pf[2], ps[2];
+    size_t oa = (size_t)(IJ - 1) * n;
+    tgt[0] = constrained_sample(P.a_bias + oa, P.a_alias + oa, ar, n, un, nu, umask, rf, q, &pos, &pf[0]);
+    if (!tgt[0]) return 0;
+    const double *b1 = b_row(IJ, tgt[0]);
+    double rs1 = hole_renorm(b1, nI, nel, un, nu);
+    if (near_zero_d(rs1)) return 0;
+    size_t ob = ((size_t)(IJ - 1) * n + (tgt[0] - 1)) * n;
+    tgt[1] = constrained_sample(P.b_bias + ob, P.b_alias + ob, b1, n, un, nu, umask, rs1, q, &pos, &ps[0]);
+    if (!tgt[1]) return 0;
+    pf[1] = (!ar || near_zero_d(rf)) ? 0.0 : ar[tgt[1] - 1] / rf;
+    const double *b2 = b_row(IJ, tgt[1]);
+    double rs2 = hole_renorm(b2, nI, nel, un, nu);
+    ps[1] = (!b2 || near_zero_d(rs2)) ? 0.0 : b2[tgt[0] - 1] / rs2;
+    *ph = pf[0] * ps[0] + pf[1] * ps[1];
+    return 1;
+}
+
+/* GAS_doubles_PCHB_get_pgen's hole factor (:246-296) */
+static double pgen_full_holes(const osys_t *s, const uint64_t *ilut, const int32_t *nI, int I, int J, int A, int B)
+{
+    const int nel = s->nel;
+    int un[128];
+    uint64_t umask[O_MAX_W];
+    int nu = unoccupied(s, ilut, un, umask);
+    int IJ = fuse(I, J);
+    const double *ar = a_row(IJ);
+    double rf = hole_renorm(ar, nI, nel, un, nu);
+    double pfA = (!ar || near_zero_d(rf)) ? 0.0 : ar[A - 1] / rf;
+    double pfB = (!ar || near_zero_d(rf)) ? 0.0 : ar[B - 1] / rf;
+    const double *bA = b_row(IJ, A), *bB = b_row(IJ, B);
+    double rA = hole_renorm(bA, nI, nel, un, nu), rB = hole_renorm(bB, nI, nel, un, nu);
+    double psA = (!bA || near_zero_d(rA)) ? 0.0 : bA[B - 1] / rA;
+    double psB = (!bB || near_zero_d(rB)) ? 0.0 : bB[A - 1] / rB;
+    return pfA * psA + pfB * psB;
+}
+
+/* PCHB doubles with weighted particles (UNIF-FULL, FULL-FULL) or fully
+   weighted holes: every uniform from the attempt's draw sequence, particles
+   first, then holes */
+static int gen_double_pchb_weighted(const osys_t *s, const uint64_t *ilut, const int32_t *nI, draw_seq_t *q,
+                                    int32_t *ex, int32_t *nJ, double *pgen)
+{
+    const int nel = s->nel;
+    int elecs[2], srcs[2];
+    double p;
+    if (s->particles == O_PART_UNIF_FULL || s->particles == O_PART_FULL_FULL) {
+        if (!draw_weighted_particles(s, ilut, nI, q, elecs, srcs, &p)) return 0;
+    } else if (s->particles == O_PART_UNIF_FAST) {
+        elecs[0] = (int)(seq_next(q) * (double)nel) + 1;
+        srcs[0] = nI[elecs[0] - 1];
+        if (!P.j_ok[srcs[0] - 1]) return 0;
+        size_t off = (size_t)(srcs[0] - 1) * P.nbi;
+        int j = alias_sample(P.j_bias + off, P.j_alias + off, P.nbi, seq_next(q));
+        double p_J_I = P.j_prob[off + j];
+        srcs[1] = j + 1;
+        if (!test_orb(ilut, srcs[1])) return 0;
+        for (int k = 0; k < nel; ++k) if (nI[k] == srcs[1]) elecs[1] = k + 1;
+        double p_I_J = P.j_ok[srcs[1] - 1] ? P.j_prob[(size_t)(srcs[1] - 1) * P.nbi + (srcs[0] - 1)] : 0.0;
+        p = (p_J_I + p_I_J) / (double)nel;
+        if (srcs[0] > srcs[1]) {
+            int t = srcs[0]; srcs[0] = srcs[1]; srcs[1] = t;
+            t = elecs[0]; elecs[0] = elecs[1]; elecs[1] = t;
+        }
+    } else {
+        if (!pick_biased_elecs(s, nI, seq_next(q), elecs, &p)) return 0;
+        srcs[0] = nI[elecs[0] - 1];
+        srcs[1] = nI[elecs[1] - 1];
+    }
+    int t[2];
+    double ph;
+    if (s->holes == O_HOLES_FULL_FULL) {
+        if (!draw_full_holes(s, ilut, nI, srcs[0], srcs[1], q, t, &ph)) return 0;
+    } else {
+        int ij = fuse(srcs[0], srcs[1]);
+        if (!P.ab_ok[ij - 1]) return 0;
+        size_t off = (size_t)(ij - 1) * P.ijmax;
+        int ab = alias_sample(P.ab_bias + off, P.ab_alias + off, P.ijmax, seq_next(q)) + 1;
+        ph = P.ab_prob[off + ab - 1];
+        int A = (int)floor((1.0 + sqrt(8.0 * (double)ab - 7.0)) / 2.0);
+        while (A * (A - 1) / 2 >= ab) A--;
+        while ((A + 1) * A / 2 < ab) A++;
+        t[0] = ab - A * (A - 1) / 2;
+        t[1] = A;
+        if (t[0] == t[1] || test_orb(ilut, t[0]) || test_orb(ilut, t[1])) return 0;
+        if (fabs(ph) <= EPS) return 0;
+    }
+    int tp = o_make_double(nI, nel, elecs[0], elecs[1], t[0], t[1], nJ, ex);
+    *pgen = p * ph;
+    return 1 + (tp ? 2 : 0);
+}
+
 /* spin class of a spin orbital: 1 alpha (even), 2 beta (odd), as ClassCountInd(spin, 0, 0) */
 static inline int cls(int orb) { return is_beta(orb) ? 2 : 1; }
 
@@ -951,7 +1308,12 @@ int32_t o_gen_excit_ctx(const osys_t *s, const uint64_t *ilut, const double *d, 
         pg = pg * s->psingles;
     } else {
         *ic = 2;
-        res = gen_double_pchb(s, ilut, nI, d, ex, nJ, &pg);
+        if (s->particles == O_PART_UNIF_FULL || s->particles == O_PART_FULL_FULL || s->holes == O_HOLES_FULL_FULL) {
+            draw_seq_t q = {s, d, ilut, iter, k, rep, 0, -1, {0.0, 0.0}};
+            res = gen_double_pchb_weighted(s, ilut, nI, &q, ex, nJ, &pg);
+        } else {
+            res = gen_double_pchb(s, ilut, nI, d, ex, nJ, &pg);
+        }
         pg = pg * (1.0 - s->psingles);
     }
     if (!res) return 0;
@@ -1012,7 +1374,9 @@ double o_calc_pgen(const osys_t *s, const uint64_t *ilut, int32_t ic, const int3
     if (ic == 2) {
         int I = ex[0], J = ex[1], n = P.nbi;
         double p;
-        if (s->particles == O_PART_UNIF_FAST) {
+        if (s->particles == O_PART_UNIF_FULL || s->particles == O_PART_FULL_FULL) {
+            p = pgen_weighted_particles(s, nI, I, J);
+        } else if (s->particles == O_PART_UNIF_FAST) {
             double a = P.j_ok[I - 1] ? P.j_prob[(size_t)(I - 1) * n + (J - 1)] : 0.0;
             double b = P.j_ok[J - 1] ? P.j_prob[(size_t)(J - 1) * n + (I - 1)] : 0.0;
             p = (a + b) / (double)nel;
@@ -1023,8 +1387,13 @@ double o_calc_pgen(const osys_t *s, const uint64_t *ilut, int32_t ic, const int3
             if (is_beta(I) == is_beta(J)) p = s->pparallel / (double)par;
             else p = (1.0 - s->pparallel) / (double)(nalpha * nbeta);
         }
-        int ij = fuse(I, J), ab = fuse(ex[2], ex[3]);
-        double ph = P.ab_ok[ij - 1] ? P.ab_prob[(size_t)(ij - 1) * P.ijmax + ab - 1] : 0.0;
+        double ph;
+        if (s->holes == O_HOLES_FULL_FULL) {
+            ph = pgen_full_holes(s, ilut, nI, I, J, ex[2], ex[3]);
+        } else {
+            int ij = fuse(I, J), ab = fuse(ex[2], ex[3]);
+            ph = P.ab_ok[ij - 1] ? P.ab_prob[(size_t)(ij - 1) * P.ijmax + ab - 1] : 0.0;
+        }
         p = p * ph;
         return (1.0 - s->psingles) * p;
     }
@@ -1209,6 +1578,52 @@ int64_t o_get_walkers(ostate_t *st, uint64_t *dets, double *sgns, uint32_t *flag
         n++;
     }
     return n;
+}
+
+/* generate_space_most_populated (semi_stoch_gen.F90:1031-1226) for one rank:
+   the target determinants of largest core_space_weight = sum_r |c_r|
+   (semi_stoch_procs.F90:1623-1642, the global core space), those below 1e-8
+   left out (:1094-1100), all of them when fewer are occupied.  Equal weights
+   are ordered by the determinant (ilut_lt ascending), where the reference
+   keeps the one it meets first in its storage order
+   (proc_most_populated_states, semi_stoch_procs.F90:1310-1421), so that the
+   choice is a function of the walker set.  out receives the chosen
+   determinants in that order (weight descending); returns their number. */
+typedef struct {
+    double w;
+    uint64_t d[O_MAX_W];
+} wdet_t;
+static int g_mp_w;
+static int cmp_wdet(const void *a, const void *b)
+{
+    const wdet_t *x = (const wdet_t *)a, *y = (const wdet_t *)b;
+    if (x->w > y->w) return -1;
+    if (x->w < y->w) return 1;
+    return ilut_cmp(x->d, y->d, g_mp_w);
+}
+
+int64_t o_most_populated(ostate_t *st, int64_t target, uint64_t *out)
+{
+    const int W = st->W;
+    wdet_t *c = (wdet_t *)malloc(sizeof(wdet_t) * (size_t)(st->totwalkers > 0 ? st->totwalkers : 1));
+    int64_t n = 0;
+    for (int64_t j = 0; j < st->totwalkers; ++j) {
+        uint64_t *row = cur_row(st, j);
+        double *sg = row_sign(st, row);
+        double w = 0.0;
+        for (int r = 0; r < st->S; ++r) w += fabs(sg[r]);
+        if (w < 1e-8) continue;
+        c[n].w = w;
+        for (int q = 0; q < O_MAX_W; ++q) c[n].d[q] = q < W ? row[q] : 0;
+        n++;
+    }
+    g_mp_w = W;
+    qsort(c, (size_t)n, sizeof(wdet_t), cmp_wdet);
+    int64_t k = n < target ? n : target;
+    for (int64_t i = 0; i < k; ++i)
+        for (int w = 0; w < W; ++w) out[i * W + w] = c[i].d[w];
+    free(c);
+    return k;
 }
 
 /* ------------------------------------------------------- core space */

@@ -40,7 +40,8 @@ extern "C" {
 
 enum { O_MODEL_HUBBARD = 0, O_MODEL_MOLECULAR = 1 };
 enum { O_SINGLES_UNIFORM = 0, O_SINGLES_HEATBATH = 1 };
-enum { O_PART_UNIF_UNIF = 0, O_PART_UNIF_FAST = 1 };
+enum { O_PART_UNIF_UNIF = 0, O_PART_UNIF_FAST = 1, O_PART_UNIF_FULL = 2, O_PART_FULL_FULL = 3 };
+enum { O_HOLES_FAST_FAST = 0, O_HOLES_FULL_FULL = 1 };
 
 typedef struct {
     int32_t nel, nbasis, nrep;          /* electrons, spin orbitals, lenof_sign */
@@ -72,6 +73,7 @@ typedef struct {
     int64_t nbuckets;
     uint32_t seed, pad2;
     int64_t max_walkers, max_spawned;
+    int32_t holes, pad3;                /* PCHB hole selection (O_HOLES_*) */
 } osys_t;
 
 typedef struct {
@@ -135,6 +137,7 @@ int32_t o_iterate(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_t 
 /* semistochastic core space: n dets (W words each); CSR of H_ij with diagonal
    H_ii - Hii; fp32 = values rounded to FP32 */
 int32_t o_set_core(ostate_t *st, const uint64_t *dets, int64_t n, int32_t fp32, int64_t *nnz);
+int64_t o_most_populated(ostate_t *st, int64_t target, uint64_t *out);
 void o_core_project(const ostate_t *st, const double *x, const double *shift, double tau, double *y);
 int64_t o_core_csr(const ostate_t *st, int64_t *rowptr, int32_t *cols, double *vals);
 /* annihilation of an explicit spawned list only (CompressSpawnedList + death-free

@@ -28,6 +28,7 @@ class OSys(ctypes.Structure):
         ("pparallel", ctypes.c_double), ("pchb_ready", ctypes.c_int32), ("uniform", ctypes.c_int32),
         ("random_hash2", ctypes.c_void_p), ("nbuckets", ctypes.c_int64), ("seed", ctypes.c_uint32),
         ("pad2", ctypes.c_uint32), ("max_walkers", ctypes.c_int64), ("max_spawned", ctypes.c_int64),
+        ("holes", ctypes.c_int32), ("pad3", ctypes.c_int32),
     ]
 
 
@@ -94,6 +95,7 @@ def lib():
             "o_set_core": (ctypes.c_int32, [_vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp]),
             "o_core_project": (None, [_vp, _vp, _vp, ctypes.c_double, _vp]),
             "o_core_csr": (ctypes.c_int64, [_vp, _vp, _vp, _vp]),
+            "o_most_populated": (ctypes.c_int64, [_vp, ctypes.c_int64, _vp]),
             "o_gen_excit_ctx": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32,
                                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
             "o_mt_create": (_vp, [_vp, ctypes.c_int32]),
@@ -144,6 +146,7 @@ class Oracle:
             self.keep += [tm, um]
             o.tmat, o.umat, o.umat_len, o.ecore = tm.ctypes.data, um.ctypes.data, um.size, s.ecore
             o.singles, o.particles = 1, spec.particles
+            o.holes = int(getattr(spec, "holes", 0))
             o.psingles, o.pparallel = spec.psingles, spec.pparallel
             o.uniform = int(getattr(spec, "excitgen", "pchb") == "uniform")
         o.tau = float(tau)
@@ -250,6 +253,13 @@ class Oracle:
         if rc:
             raise RuntimeError(f"o_set_core {rc}")
         return nnz.value
+
+    def most_populated(self, target):
+        """generate_space_most_populated: the target most populated
+        determinants (weight descending, ties by determinant)."""
+        out = np.zeros((max(int(target), 1), self.W), dtype=np.uint64)
+        n = self.L.o_most_populated(self.st, int(target), p(out))
+        return out[:n].view(np.int64)
 
     def core_csr(self):
         nnz = self.L.o_core_csr(self.st, None, None, None)
