@@ -1,18 +1,25 @@
 #!/usr/bin/env python3
-"""FCIQMC walker-iterations/s on one MI355X.
+"""FCIQMC walker-iterations/s on one MI355X, batched against plain FP64.
 
-Workload: synthetic C1 molecular
-Hamiltonian, 10 electrons in 20 spatial orbitals (seed 7, Cholesky rank 60),
-initiator FCIQMC (threshold 3), PCHB doubles (UNIF-UNIF particles, FAST-FAST
-spin-orbital-resolved holes) + on-the-fly heat-bath singles with pSingles 0.1,
-1e6 target walkers, tau = 0.005, SHIFTDAMP 0.5, MEMORYFACHASH 2, FP64 signs.
+Default workload (the metric's configuration): 32 independent problems of
+uneven size -- 16 replicas (distinct seeds) of the synthetic C1 molecular
+Hamiltonian with 14 electrons in 30 spatial orbitals (seed 7, Cholesky rank
+60; initiator threshold 3, PCHB doubles with UNIF-UNIF particles and
+FAST-FAST holes, heat-bath singles with pSingles 0.1, tau 0.005, 5e6 walkers
+each, a semistochastic core of the 50 000 most populated determinants) and
+16 half-filled Hubbard chains of 12-18 sites (U/t in [2, 8], walker targets
+log-uniform in [1e4, 2e6], tau 0.01).
 
-A step is one FCIQMC iteration (spawn, merge, death, annihilation, hash
-statistics) over the resident walker list; every 10 steps the host runs the
-shift update (NECI's update cycle, with TARGET-SHIFTDAMP so that the
-population settles at TOTALWALKERS).  The walker population is grown to the
-target untimed, then W warm-up steps, then K timed steps.
-value = sum over timed steps of TotParts / wall time of those steps.
+A step is one FCIQMC iteration of every problem (spawn, merge, death,
+annihilation, hash statistics; the core projection for the molecular ones).
+Every 10 steps the host runs each problem's update cycle (TARGET-SHIFTDAMP
+for the molecular replicas).  Populations are set up untimed, then W warm-up
+steps, then K timed steps, once in the plain FP64 form and once in the
+batched FP32/FP64 form.  value = the batched form's sum over timed steps and
+problems of TotParts / wall time of those steps.
+
+--workload mol10e20o runs one 10e/20o molecular engine (1e6 walkers) in plain
+FP64 instead.
 """
 import argparse
 import json
@@ -24,9 +31,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "walker-iterations/s (spawn+annihilate+death) on 1 GPU; batched vs plain FP64"
-STAGES = ["prepass", "scan", "spawn", "spawn_singles", "sort", "merge", "annihilate", "insert", "stats"]
+STAGES = ["prepass", "scan", "spawn", "spawn_singles", "sort", "merge", "annihilate", "insert", "stats", "core"]
 KERNELS = {"prepass": "k_prepass", "spawn": "k_spawn", "spawn_singles": "k_spawn_singles",
-           "annihilate": "k_merge_annihilate", "insert": "k_insert", "stats": "k_stats"}
+           "annihilate": "k_merge_annihilate", "insert": "k_insert", "stats": "k_stats", "core": "k_core_spmv"}
 SINGLE_KERNEL_STAGES = set(KERNELS)
 HBM_PEAK_GBS = 8000.0
 
@@ -44,18 +51,22 @@ def sort_key(nbasis, nel):
     return 8 * W, ((nbasis + 7) // 8 if W == 1 else (nbasis - 64 + 7) // 8 + 8)
 
 
-def algorithmic_bytes(stage, c, W=1, S=1, nbasis=40, psingles=0.1, nel=None):
+def algorithmic_bytes(stage, c, W=1, S=1, nbasis=40, psingles=0.1, nel=None, val_bytes=8):
     """Compulsory HBM bytes of one iteration's launch(es) of a stage: every input
     read once and every output written once, for the per-iteration counts c
     (Nd occupied dets, Na attempts, Ns spawns, Nu unique spawned dets, Nn new
-    dets).  Records: key 8W, signs 8S, flags 4.  The integral and heat-bath
+    dets; Nc stored core Hamiltonian elements and Ncore core dets, when the
+    run has a core space).  Records: key 8W, signs 8S, flags 4; a spawned
+    amplitude and a core Hamiltonian element take val_bytes (8 in FP64, 4
+    when stored in FP32).  A hash probe reads the matching entry's key and
+    slot index, K + 8 bytes, not the whole bucket line.  The integral and heat-bath
     tables (under 1 MB here) stay in L2 and count once, i.e. zero; the PCHB
     alias rows (21 MB at 40 spin orbitals) do not fit one XCD's 4 MB L2, so
     every double attempt counts its alias entry as the minimum an alias
     draw reads: bias 8 + alias 4 + probability 8 bytes (T_gen of a PCHB
     double).  Queued single attempts and their share of the spawns are their
     expected values psingles*Na and psingles*Ns."""
-    K, Sg = 8 * W, 8 * S
+    K, Sg, V = 8 * W, 8 * S, val_bytes
     Nd, Na, Ns, Nu, Nn = c["Nd"], c["Na"], c["Ns"], c["Nu"], c["Nn"]
     Nq, Nsq = psingles * Na, psingles * Ns
     if stage == "prepass":     # read key, signs, flags, diagH, offdiagH; write signs, flags, counts, pinfo
@@ -64,31 +75,34 @@ def algorithmic_bytes(stage, c, W=1, S=1, nbasis=40, psingles=0.1, nel=None):
         return Nd * S * (4 + 8)
     if stage == "spawn":       # parents once (key, pinfo, offset); one alias entry per double attempt;
         #                        queue (16 B) and double spawns written
-        return Nd * (K + S + 8 * S) + (Na - Nq) * 20 + Nq * 16 + (Ns - Nsq) * (K + 12)
+        return Nd * (K + S + 8 * S) + (Na - Nq) * 20 + Nq * 16 + (Ns - Nsq) * (K + V + 4)
     if stage == "spawn_singles":  # queue entry, parent key/pinfo/offset, single spawns written
-        return Nq * (16 + K + 1 + 8) + Nsq * (K + 12)
+        return Nq * (16 + K + 1 + 8) + Nsq * (K + V + 4)
     if stage == "sort":        # per pass: histogram read of keys, scatter read+write of (key, index)
         kb, passes = sort_key(nbasis, nel)
         return passes * Ns * (kb + 2 * (kb + 4))
     if stage == "merge":       # fused into the annihilate stage
         return 0.0
     if stage == "annihilate":  # merge and annihilation in one pass: sorted keys + index, gathered value + info
-        #                        per spawn; per unique det one hash bucket (64W B) and its state row
+        #                        per spawn; per unique det one hash probe (K + 8) and its state row
         #                        (S signs + flag word) read and written; new list written
-        return Ns * (K + 4 + 8 + 4) + Nu * 64 * W + Nu * 2 * (Sg + 8) + Nn * (K + Sg + 4)
-    if stage == "insert":      # new list, free-queue entry, bucket line, walker row (key, signs, flags, 3x8)
-        return Nn * (K + Sg + 4) + Nn * 8 + Nn * 64 * W + Nn * (K + Sg + 4 + 24)
+        return Ns * (K + 4 + V + 4) + Nu * (K + 8) + Nu * 2 * (Sg + 8) + Nn * (K + Sg + 4)
+    if stage == "insert":      # new list, free-queue entry, hash entry, walker row (key, signs, flags, 3x8)
+        return Nn * (K + Sg + 4) + Nn * 8 + Nn * (K + 8) + Nn * (K + Sg + 4 + 24)
     if stage == "stats":       # flags, signs, key per slot
         return Nd * (4 + Sg + K)
+    if stage == "core":        # core H.v: each stored element (value + int32 column) once, the core
+        #                        amplitudes read and the projected ones written
+        return c.get("Nc", 0.0) * (V + 4) + 2 * Sg * c.get("Ncore", 0.0)
     return 0.0
 
 
-def iteration_bytes(c, W=1, S=1, nbasis=40, psingles=0.1, nel=None):
+def iteration_bytes(c, W=1, S=1, nbasis=40, psingles=0.1, nel=None, val_bytes=8):
     """Compulsory HBM bytes of one whole iteration: the stages' figures of
     algorithmic_bytes summed, except that the sort counts one read and one
     write of (key, index) per spawn, the algorithmic minimum, not one per
     radix pass."""
-    tot = sum(algorithmic_bytes(st, c, W, S, nbasis, psingles, nel) for st in STAGES if st != "sort")
+    tot = sum(algorithmic_bytes(st, c, W, S, nbasis, psingles, nel, val_bytes) for st in STAGES if st != "sort")
     return tot + 2 * c["Ns"] * (sort_key(nbasis, nel)[0] + 4)
 
 
@@ -194,21 +208,31 @@ def build_workload(args):
     return spec
 
 
-def cpu_baseline(spec, eng, drv, seconds):
-    """The oracle (single-threaded CPU restatement) on the walker set the GPU
-    reached, for as many iterations as fit in the time budget."""
+def cpu_threads():
+    """Host threads of the CPU baseline: every core the process may use, at
+    most 16 (a one-GPU box's share of its host)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(16, n))
+
+
+def cpu_baseline(spec, tau, dets, signs, flags, shift, it0, seconds, core=None, threads=None):
+    """The oracle (the CPU restatement of PerformFCIMCycPar) on a walker set
+    the GPU reached, run as P hash-sharded states on P threads (one per host
+    core, as the reference runs one MPI rank per core), for as many
+    iterations as fit in the time budget.  Returns the cpu_baseline object."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
-    from oracle_ctypes import Oracle
+    from oracle_ctypes import ShardedOracle
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from common import neci_to_internal_flags
-    dets, signs, flags, _, _ = eng.download()
-    orc = Oracle(spec, drv.tau)
-    t_load = time.perf_counter()
+    P = threads or cpu_threads()
+    orc = ShardedOracle(spec, tau, P)
+    t_set = time.perf_counter()
     orc.add_walkers_bulk(dets, signs, neci_to_internal_flags(flags, spec.S))
-    t_load = time.perf_counter() - t_load
-    shift = drv.shift.copy()
-    it0 = drv.iter + 1
+    if core is not None:
+        orc.set_core(core)
+    t_set = time.perf_counter() - t_set
+    shift = np.array(shift, dtype=np.float64).copy()
     walker_iters = 0.0
     n = 0
     t0 = time.perf_counter()
@@ -216,15 +240,17 @@ def cpu_baseline(spec, eng, drv, seconds):
         st = orc.iterate(it0 + n, shift)
         if st["rc"]:
             raise RuntimeError(f"oracle iteration failed: {st['rc']}")
-        walker_iters += float(np.sum(st["tot_parts"]))
+        walker_iters += float(np.sum(st["tot_parts"][:spec.S]))
         n += 1
         if time.perf_counter() - t0 > seconds:
             break
     dt = time.perf_counter() - t0
-    return dict(value=walker_iters / dt, unit="walker-iterations/s", cores=1, kind="port",
-                sample=f"{n} iterations of the same workload from the GPU's walker set "
-                       f"({len(dets)} determinants, {walker_iters / n:.3g} walkers/iteration), "
-                       f"single-threaded oracle, {dt:.1f} s")
+    orc.close()
+    return dict(value=walker_iters / dt, unit="walker-iterations/s", cores=P, kind="port",
+                sample=f"{n} iterations of the oracle, hash-sharded over {P} threads, from the GPU's walker set "
+                       f"({len(dets)} determinants, {walker_iters / n:.3g} walkers per iteration"
+                       + (f", {len(core)}-determinant core" if core is not None else "")
+                       + f"), {dt:.1f} s (set-up {t_set:.1f} s not counted)")
 
 
 def batched_workload(n_mol_replicas=16, n_hub=16, seed=2024, mol_walkers=5e6, ncore=50000):
@@ -245,97 +271,216 @@ def batched_workload(n_mol_replicas=16, n_hub=16, seed=2024, mol_walkers=5e6, nc
     mol = synthetic_molecule(30, 14, 60, 7)
     hub_specs = [(Spec(HubbardChain(h["L"], h["L"], h["u"], bhub=-1.0), init_walkers=h["target"], seed=h["seed"],
                        memory_fac_part=4.0, memory_fac_spawn=4.0, memory_fac_hash=2.0), 0.01, h) for h in hubs]
-    mol_kw = dict(particles=0, psingles=0.1, memory_fac_spawn=4.0, memory_fac_part=3.0, memory_fac_hash=2.0)
+    mol_kw = dict(trunc_initiator=True, init_walk_no=3.0, particles=0, psingles=0.1, pparallel=0.5,
+                  memory_fac_spawn=4.0, memory_fac_part=3.0, memory_fac_hash=2.0)
     return mol, mol_kw, hub_specs, mol_walkers, ncore
 
 
+TAU_MOL = 0.005   # the metric's molecular tau (every timed iteration)
+TAU_GROW = 0.001  # set-up growth of the molecular population only
+
+
 def run_batched(args):
-    """The same engines iterated two ways, from the same grown populations:
-    plain (each problem its own engine, iterated alone with one launch per
-    kernel, one problem after the other) and batched (all engines as the
-    branches of one captured graph per iteration).  The molecular replicas
-    are engines of one replica each with distinct seeds."""
+    """Config 3: 32 problems iterated two ways from the same populations.
+    Plain FP64: each problem its own engine, iterated alone with one launch
+    per kernel, one problem after the other, core Hamiltonian and spawned
+    amplitudes in FP64.  Batched FP32/FP64: all engines as the branches of
+    one captured graph per iteration, core Hamiltonian and spawned
+    amplitudes stored in FP32, every sum, the energies and the shift in
+    FP64.  The molecular replicas start from one population grown to the
+    target (a common starting point, as replicas restarted from one
+    POPSFILE) and then follow their own random streams (distinct seeds)."""
     import numpy as np
     from neci_stable_amd import Spec
     from neci_stable_amd.engine import Batch, Engine
     from neci_stable_amd.fcimc import BatchedFciMCPar, FciMCPar
 
     mol, mol_kw, hub_specs, mol_walkers, ncore = batched_workload(args.mol_replicas, args.hub_systems,
-                                                                  mol_walkers=args.mol_walkers)
-    tau_mol = 0.001
-
-    def grown(spec, tau, target, n0):
-        e = Engine(spec)
-        e.start_single_part(n0)
-        d = FciMCPar(e, tau=tau, target_walkers=target, steps_shift=10, shift_damp=0.5)
-        while d.single_part.any() and d.iter < args.max_growth_iters:
-            d.run(10)
-        d.run(args.warmup)
-        return e, d
-
-    def with_core(e, d):
-        dets, sg, _, _, _ = e.download()
+                                                                  mol_walkers=args.mol_walkers, ncore=args.ncore)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    t_setup = time.perf_counter()
+    # one molecular population grown to the target under TARGET-SHIFTDAMP:
+    # at tau 0.005 the first update cycle (10 iterations at DiagSft 0)
+    # multiplies the population by ~100, past the walker table, so it grows
+    # at TAU_GROW, then settles at the metric's tau
+    g_spec = Spec(mol, init_walkers=mol_walkers, seed=6999, **mol_kw)
+    g = Engine(g_spec)
+    g.start_single_part(1000.0)
+    gd = FciMCPar(g, tau=TAU_GROW, target_walkers=mol_walkers, steps_shift=10, shift_damp=0.5,
+                  target_shiftdamp=True)
+    for tau in (TAU_GROW, TAU_MOL):
+        gd.tau = tau
+        settled = 0
+        while settled < 3 and gd.iter < args.max_growth_iters:
+            gd.run(10)
+            av = float(np.sum(gd.rows[-1]["tot_parts"]))
+            settled = settled + 1 if abs(av / mol_walkers - 1.0) < 0.1 else 0
+        log(f"# molecular growth at tau {tau}: {gd.iter} iterations, {av:.4g} walkers")
+    dets, sg, fl, _, _ = g.download()
+    core = None
+    if ncore:
         o = np.argsort(-np.abs(sg).sum(axis=1), kind="stable")[:ncore]
         core = np.ascontiguousarray(dets[np.sort(o)])
-        e.set_core(core, fp32=False)
-        d.run(10)
-        return core
+    log(f"# molecular population: {gd.iter} iterations, {np.abs(sg).sum():.4g} walkers on {len(dets)} "
+        f"determinants, shift {gd.shift[0]:.4f}")
+    rows = g.rows(dets, sg, fl)
+    g.close()
+    engines = []
+    for r in range(args.mol_replicas):
+        e = Engine(Spec(mol, init_walkers=mol_walkers, seed=7000 + r, **mol_kw))
+        e.upload(rows)
+        if core is not None:
+            e.set_core(core, fp32=False)
+        d = FciMCPar(e, tau=TAU_MOL, target_walkers=mol_walkers, steps_shift=10, shift_damp=0.5,
+                     target_shiftdamp=True, diag_shift=float(gd.shift[0]))
+        d.iter = gd.iter
+        d.old_av_walkers = gd.old_av_walkers
+        engines.append((e, d))
+    del rows
+    for spec, tau, h in hub_specs:
+        e = Engine(spec)
+        e.start_single_part(30.0)
+        d = FciMCPar(e, tau=tau, target_walkers=h["target"], steps_shift=10, shift_damp=0.5)
+        while d.single_part.any() and d.iter < args.max_growth_iters:
+            d.run(10)
+        engines.append((e, d))
+    drivers = [d for _, d in engines]
+    for d in drivers:
+        d.run(args.warmup)
+    t_setup = time.perf_counter() - t_setup
+    log(f"# {len(engines)} engines set up in {t_setup:.1f} s")
 
-    def timed(fn, drivers, k):
-        w0 = sum(d.walker_iters for d in drivers)
-        t0 = time.perf_counter()
-        fn(k)
-        dt = time.perf_counter() - t0
-        return (sum(d.walker_iters for d in drivers) - w0) / dt, dt
+    def totals():
+        return sum(d.totals["walker_iters"] for d in drivers)
 
     K = args.steps
-    engines = []
-    cores = []
-    for r in range(args.mol_replicas):
-        spec = Spec(mol, init_walkers=mol_walkers, seed=7000 + r, **mol_kw)
-        e, d = grown(spec, tau_mol, mol_walkers, 1e3)
-        cores.append(with_core(e, d))
-        engines.append((e, d))
-    for spec, tau, h in hub_specs:
-        engines.append(grown(spec, tau, h["target"], 30.0))
-    print(f"# grown: {len(engines)} engines", file=sys.stderr, flush=True)
-    drivers = [d for _, d in engines]
-    # plain FP64: FP64 core Hamiltonian and spawned amplitudes
-    p_rate, p_dt = timed(lambda k: [d.run(k) for d in drivers], drivers, K)
-    # batched FP32/FP64: core Hamiltonian and spawned amplitudes stored in FP32,
-    # sums, energies and the shift in FP64
+    # plain FP64: each engine alone, one after the other
+    per_plain = []
+    w0 = totals()
+    tp = 0.0
+    for e, d in engines:
+        a = d.totals["walker_iters"]
+        t0 = time.perf_counter()
+        d.run(K)
+        dt = time.perf_counter() - t0
+        tp += dt
+        per_plain.append((dt * 1e3 / K, (d.totals["walker_iters"] - a) / K))
+    p_rate = (totals() - w0) / tp
+    log(f"# plain FP64: {tp * 1e3 / K:.2f} ms per iteration of all problems")
+    # batched FP32/FP64
+    nnz0 = 0
     for i, (e, _) in enumerate(engines):
-        if i < len(cores):
-            e.set_core(cores[i], fp32=True)
+        if i < args.mol_replicas and core is not None:
+            nnz = e.set_core(core, fp32=True)
+            nnz0 = nnz0 or nnz
         e.set_spawn_precision(True)
     batch = Batch([e for e, _ in engines])
     runner = BatchedFciMCPar(batch, drivers)
-    runner.run(10)  # capture and first replays
-    b_rate, b_dt = timed(runner.run, drivers, K)
+    runner.run(max(args.warmup, 10))  # capture and first replays
+    gids, ngroups = batch.groups()
+    gm = int(gids[0])                  # the molecular replicas' launch group
+    mol = [i for i in range(len(engines)) if gids[i] == gm]
+    # one captured graph per iteration (the batch's default form)
+    w0 = totals()
+    t0 = time.perf_counter()
+    runner.run(K)
+    tg = time.perf_counter() - t0
+    g_rate = (totals() - w0) / tg
+    log(f"# batched FP32/FP64, graph replay: {tg * 1e3 / K:.2f} ms per iteration of all problems, "
+        f"{ngroups} launch groups")
+    # the roofline kernel: every stage of the molecular group timed over a
+    # short pass of direct launches, then the timed region carries events on
+    # the slowest single-kernel stage's launches only
+    batch.set_graph(False)
+    batch.set_profiling(True)
+    runner.run(10)
+    ms_all, nprof_all = batch.group_stage_times(gm)
+    per_stage = {STAGES[i]: ms_all[i] / max(nprof_all, 1) for i in range(len(STAGES))}
+    dom = max((k for k in per_stage if k in SINGLE_KERNEL_STAGES), key=per_stage.get)
+    batch.set_profiling(1 << STAGES.index(dom))
+    c0 = [dict(drivers[i].totals) for i in mol]
+    w0 = totals()
+    t0 = time.perf_counter()
+    runner.run(K)
+    tb = time.perf_counter() - t0
+    b_rate = (totals() - w0) / tb
+    ms, nprof = batch.group_stage_times(gm)
+    dom_ms = ms[STAGES.index(dom)] / max(nprof, 1)
+    batch.set_profiling(False)
+    log(f"# batched FP32/FP64, direct launches with events on {KERNELS[dom]}: {tb * 1e3 / K:.2f} ms per iteration")
+    # algorithmic bytes of one launch of the dominant kernel: the sum over
+    # the molecular group's engines of each engine's per-iteration figure
+    b = 0.0
+    e0 = engines[0][0]
+    for j, i in enumerate(mol):
+        cnt = window_counts(c0[j], drivers[i].totals)
+        cnt["Nc"], cnt["Ncore"] = float(nnz0), float(len(core) if core is not None else 0)
+        b += algorithmic_bytes(dom, cnt, W=e0.W, S=e0.S, nbasis=e0.nbasis, psingles=e0.psingles, nel=e0.nel,
+                               val_bytes=4)
+    achieved = b / (dom_ms * 1e-3) / 1e9
+    d0 = engines[0][1]
     batch.close()
     walkers = [float(np.sum(d.last["tot_parts"])) for d in drivers]
+    cpu = None
+    if not args.no_cpu:
+        dets0, sg0, fl0, _, _ = e0.download()
+        import dataclasses
+        cpu = cpu_baseline(dataclasses.replace(e0.spec, spawn_fp32=False), TAU_MOL, dets0, sg0, fl0, d0.shift,
+                           d0.iter + 1, args.cpu_seconds, core=core)
     for e, _ in engines:
         e.close()
-    line = {"metric": "walker-iterations/s on 1 GPU, batched vs plain FP64", "unit": "walker-iterations/s",
-            "n_gpus": 1, "steps": K, "batched": b_rate, "plain": p_rate, "ratio": b_rate / p_rate,
-            "ms_per_step": {"batched": b_dt * 1e3 / K, "plain": p_dt * 1e3 / K},
-            "dtype": {"plain": "f64 walkers, core Hamiltonian and spawned amplitudes",
-                      "batched": "f64 walkers and sums; f32-stored core Hamiltonian and spawned amplitudes"},
-            "data": "synthetic",
-            "config": {"workload": f"{args.mol_replicas} replicas of molecular 14e/30o (seed 7, rank 60, "
-                                   f"{ncore}-det core, one engine each) + {len(hub_specs)} Hubbard chains "
-                                   f"of 12-18 sites",
-                       "hubbard": [dict(sites=h["L"], u=round(h["u"], 3), target=round(h["target"]))
-                                   for _, _, h in hub_specs],
-                       "walkers_per_problem": walkers}}
+    line = {
+        "metric": METRIC, "value": b_rate, "unit": "walker-iterations/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": tb * 1e3 / K, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "precision": "walkers, sums, energies and shift in FP64; core Hamiltonian and spawned amplitudes "
+                     "stored in FP32 (the batched form)",
+        "data": "synthetic",
+        "config": {"workload": f"batched: {args.mol_replicas} replicas of molecular 14e/30o (seed 7, rank 60, "
+                               f"initiator 3, PCHB unif-unif + heat-bath singles, psingles 0.1, tau {TAU_MOL}, "
+                               f"{ncore}-determinant core, distinct seeds) + {len(hub_specs)} Hubbard chains of "
+                               f"12-18 sites (U/t in [2, 8], targets 1e4-2e6, tau 0.01)",
+                   "mol_target_walkers": mol_walkers, "walkers_per_problem": walkers,
+                   "total_walkers": float(sum(walkers))},
+        "plain": {"value": p_rate, "ms_per_step": tp * 1e3 / K,
+                  "form": "each problem its own engine, iterated alone with one launch per kernel, one problem "
+                          "after the other, FP64 core Hamiltonian and spawned amplitudes",
+                  "ms_per_problem": [round(x[0], 4) for x in per_plain]},
+        "batched_over_plain": b_rate / p_rate,
+        "graph_replay": {"value": g_rate, "ms_per_step": tg * 1e3 / K,
+                         "form": "the same batched chain replayed as one captured graph per iteration, no events"},
+        "launch_groups": int(ngroups),
+        "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": b, "launch_ms": dom_ms,
+                     "where": f"one launch covers the {len(mol)} molecular replicas; its launches timed with "
+                              "events over the timed region"},
+        "stage_ms_per_step": per_stage,
+        "cpu_baseline": cpu,
+    }
     print(json.dumps(line), flush=True)
+
+
+def window_counts(t0, t1):
+    """Per-iteration counts over the iterations between two snapshots of a
+    driver's running totals (every iteration counted, whatever the update
+    cycle boundaries)."""
+    n = t1["iters"] - t0["iters"]
+    if n <= 0:
+        raise ValueError("no iterations between the snapshots")
+    return dict(Nd=(t1["det_iters"] - t0["det_iters"]) / n, Na=(t1["attempts"] - t0["attempts"]) / n,
+                Ns=(t1["spawned"] - t0["spawned"]) / n, Nu=(t1["unique"] - t0["unique"]) / n,
+                Nn=(t1["new"] - t0["new"]) / n, walkers=(t1["walker_iters"] - t0["walker_iters"]) / n)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", choices=["batched", "mol10e20o"], default="batched",
+                    help="batched: the 32-problem batched-vs-plain workload (the metric's configuration); "
+                         "mol10e20o: one 10e/20o molecular engine, plain FP64")
     ap.add_argument("--profile-steps", type=int, default=50, help="steps of the all-stage timing pass")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--walkers", type=float, default=1e6)
@@ -349,9 +494,8 @@ def main():
     ap.add_argument("--summarize-trace", default=None, help="rocprofv3 kernel-trace CSV to summarise")
     ap.add_argument("--summarize-pmc", default=None, help="rocprofv3 counter-collection CSV to summarise")
     ap.add_argument("--last", type=int, default=60, help="iterations (or dispatches) summarised")
-    ap.add_argument("--batched", action="store_true",
-                    help="batched (one graph per iteration) vs plain form on the 32-problem workload")
     ap.add_argument("--mol-replicas", type=int, default=16)
+    ap.add_argument("--ncore", type=int, default=50000, help="core determinants of the molecular problems")
     ap.add_argument("--hub-systems", type=int, default=16)
     ap.add_argument("--mol-walkers", type=float, default=5e6)
     ap.add_argument("--traffic", nargs=4, default=None,
@@ -375,7 +519,7 @@ def main():
         print(json.dumps({"error": "this engine runs one GPU per process (one rank's walkers); "
                                    "multi-GPU runs are not part of this build", "n_gpus": args.gpus}))
         sys.exit(2)
-    if args.batched:
+    if args.workload == "batched":
         run_batched(args)
         return
     import numpy as np
@@ -417,7 +561,7 @@ def main():
     eng.set_profiling(False)
     per_stage = {STAGES[i]: ms_all[i] / max(nprof_all, 1) for i in range(len(STAGES))}
     dom = max((k for k in per_stage if k in SINGLE_KERNEL_STAGES), key=per_stage.get)
-    nrow0 = len(drv.rows)
+    tot0 = dict(drv.totals)
     eng.set_profiling(1 << STAGES.index(dom))
     w0 = drv.walker_iters
     t0 = time.perf_counter()
@@ -427,13 +571,8 @@ def main():
     ms, nprof = eng.stage_times()
     eng.set_profiling(False)
     dom_ms = ms[STAGES.index(dom)] / max(nprof, 1)
-    rows = drv.rows[nrow0:]
     K = args.steps
-    counts = dict(Nd=float(np.mean([r["tot_walkers"] for r in rows])),
-                  Na=float(sum(r["attempts"] for r in rows)) / K,
-                  Ns=float(sum(r["spawned"] for r in rows)) / K,
-                  Nu=float(sum(r["unique"] for r in rows)) / K,
-                  Nn=float(sum(r["new"] for r in rows)) / K)
+    counts = window_counts(tot0, drv.totals)  # every timed iteration, whatever the cycle boundaries
     # the roofline is that of one kernel: the slowest stage made of a single
     # kernel, its launches timed in the timed region itself
     b = algorithmic_bytes(dom, counts, W=eng.W, S=eng.S, nbasis=spec.nbasis, psingles=spec.psingles, nel=spec.nel)
@@ -471,7 +610,8 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(spec, eng, drv, args.cpu_seconds)
+        dets, sg, fl, _, _ = eng.download()
+        line["cpu_baseline"] = cpu_baseline(spec, tau, dets, sg, fl, drv.shift, drv.iter + 1, args.cpu_seconds)
     eng.close()
     print(json.dumps(line), flush=True)
 

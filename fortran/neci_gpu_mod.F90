@@ -12,6 +12,9 @@ module neci_gpu_mod
     integer(c_int), parameter, public :: NGPU_MODEL_HUBBARD = 0, NGPU_MODEL_MOLECULAR = 1
     integer(c_int), parameter, public :: NGPU_SINGLES_HEATBATH = 1
     integer(c_int), parameter, public :: NGPU_PART_UNIF_UNIF = 0, NGPU_PART_UNIF_FAST = 1
+    integer(c_int), parameter, public :: NGPU_PART_UNIF_FULL = 2, NGPU_PART_FULL_FULL = 3
+    integer(c_int), parameter, public :: NGPU_HOLES_FAST_FAST = 0, NGPU_HOLES_FULL_FULL = 1
+    integer(c_int), parameter, public :: NGPU_CORE_BUILD_AUTO = 0, NGPU_CORE_BUILD_JOIN = 1, NGPU_CORE_BUILD_SCAN = 2
 
     type, bind(c), public :: ngpu_sys_t
         integer(c_int32_t) :: nel, nbasis, n_replicas, model
@@ -47,7 +50,9 @@ module neci_gpu_mod
     public :: ngpu_diag_helement_batch, ngpu_set_excitgen_uniform, ngpu_set_core
     public :: ngpu_set_graph, ngpu_batch_create, ngpu_batch_iterate, ngpu_batch_destroy
     public :: ngpu_fcidump_header, ngpu_read_fcidump, ngpu_set_spawn_precision, ngpu_get_hash_stats
-    public :: ngpu_get_error
+    public :: ngpu_get_error, ngpu_set_excitgen_pchb, ngpu_download_walkers_dev, ngpu_most_populated
+    public :: ngpu_select_core, ngpu_set_core_build, ngpu_get_core_hamil, ngpu_batch_get_groups
+    public :: ngpu_batch_set_graph, ngpu_batch_set_profiling
 
     interface
         integer(c_int) function ngpu_create(h, sys) bind(c, name="ngpu_create")
@@ -210,6 +215,69 @@ module neci_gpu_mod
             integer(c_int64_t), intent(in) :: dets(*)
             integer(c_int64_t), value :: n
             real(c_double), intent(out) :: diag(*), offdiag(*)
+        end function
+        integer(c_int) function ngpu_set_excitgen_pchb(h, singles, particles, holes, psingles, pparallel) &
+            bind(c, name="ngpu_set_excitgen_pchb")
+            import :: c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: singles, particles, holes
+            real(c_double), value :: psingles, pparallel
+        end function
+        ! rows, diag_h, offdiag_h and n are device addresses (diag_h, offdiag_h may be c_null_ptr)
+        integer(c_int) function ngpu_download_walkers_dev(h, rows, diag_h, offdiag_h, cap, n) &
+            bind(c, name="ngpu_download_walkers_dev")
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: h, rows, diag_h, offdiag_h
+            integer(c_int64_t), value :: cap
+            type(c_ptr), value :: n
+        end function
+        integer(c_int) function ngpu_most_populated(h, target, dets, n) bind(c, name="ngpu_most_populated")
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: h
+            integer(c_int64_t), value :: target
+            integer(c_int64_t), intent(out) :: dets(*)
+            integer(c_int64_t), intent(out) :: n
+        end function
+        integer(c_int) function ngpu_select_core(h, target, fp32_values, dets, ncore, nnz) &
+            bind(c, name="ngpu_select_core")
+            import :: c_int, c_ptr, c_int64_t, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int64_t), value :: target
+            integer(c_int32_t), value :: fp32_values
+            integer(c_int64_t), intent(out) :: dets(*)
+            integer(c_int64_t), intent(out) :: ncore, nnz
+        end function
+        integer(c_int) function ngpu_set_core_build(h, method) bind(c, name="ngpu_set_core_build")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: method
+        end function
+        integer(c_int) function ngpu_get_core_hamil(h, rowptr, cols, vals, cap, nnz) &
+            bind(c, name="ngpu_get_core_hamil")
+            import :: c_int, c_ptr, c_int64_t, c_int32_t, c_double
+            type(c_ptr), value :: h
+            integer(c_int64_t), intent(out) :: rowptr(*)
+            integer(c_int32_t), intent(out) :: cols(*)
+            real(c_double), intent(out) :: vals(*)
+            integer(c_int64_t), value :: cap
+            integer(c_int64_t), intent(out) :: nnz
+        end function
+        integer(c_int) function ngpu_batch_get_groups(b, group_of_engine, n_groups) &
+            bind(c, name="ngpu_batch_get_groups")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: b
+            integer(c_int32_t), intent(out) :: group_of_engine(*)
+            integer(c_int32_t), intent(out) :: n_groups
+        end function
+        integer(c_int) function ngpu_batch_set_graph(b, on) bind(c, name="ngpu_batch_set_graph")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: b
+            integer(c_int32_t), value :: on
+        end function
+        integer(c_int) function ngpu_batch_set_profiling(b, on) bind(c, name="ngpu_batch_set_profiling")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: b
+            integer(c_int32_t), value :: on
         end function
     end interface
 end module neci_gpu_mod

@@ -52,9 +52,12 @@ typedef enum {
 enum { NGPU_MODEL_HUBBARD = 0, NGPU_MODEL_MOLECULAR = 1 };
 /* singles: "on-the-fly-heat-bath" (excit_gens_int_weighted.F90:442) */
 enum { NGPU_SINGLES_HEATBATH = 1 };
-/* PCHB doubles particle selection (gasci_pchb_doubles_select_particles.fpp:38-44);
-   holes are always FAST-FAST spin-orbital resolved */
-enum { NGPU_PART_UNIF_UNIF = 0, NGPU_PART_UNIF_FAST = 1 };
+/* PCHB doubles particle selection (gasci_pchb_doubles_select_particles.fpp:38-52) */
+enum { NGPU_PART_UNIF_UNIF = 0, NGPU_PART_UNIF_FAST = 1, NGPU_PART_UNIF_FULL = 2, NGPU_PART_FULL_FULL = 3 };
+/* PCHB doubles hole selection, spin-orbital resolved (pchb_excitgen.F90:216-262):
+   FAST-FAST (gasci_pchb_doubles_spinorb_fastweighted.fpp), FULL-FULL
+   (gasci_pchb_doubles_spinorb_fullyweighted.fpp) */
+enum { NGPU_HOLES_FAST_FAST = 0, NGPU_HOLES_FULL_FULL = 1 };
 /* flag bit layout of walker rows at the boundary */
 enum { NGPU_FLAGS_MNECI = 0, NGPU_FLAGS_NECI = 1 };
 
@@ -138,13 +141,23 @@ int ngpu_set_integrals(ngpu_t *h, const double *tmat2d, const double *umat, int6
 int ngpu_set_lattice(ngpu_t *h, const int32_t *nbr, int32_t nsites, int32_t maxnbr, double bhub,
                      double uhub);
 
-/* Molecular excitation generator: FCI-PCHB doubles (FAST-FAST, spin-orbital
+/* Molecular excitation generator: FCI-PCHB doubles (FAST-FAST holes, spin-orbital
    resolved, gasci_pchb_doubles_spinorb_fastweighted.fpp) with the given particle
    selection, singles with probability psingles (gen_exc_sd,
    excitation_generators.F90:112).  Builds the alias tables (aliasSampling.F90:209)
    on the host and uploads them.  Requires ngpu_set_integrals first. */
 int ngpu_set_excitgen(ngpu_t *h, int32_t singles, int32_t particles, double psingles,
                       double pparallel);
+/* ngpu_set_excitgen with the hole selection given: NGPU_HOLES_FAST_FAST, or
+   NGPU_HOLES_FULL_FULL, the fully weighted holes of
+   gasci_pchb_doubles_spinorb_fullyweighted.fpp:138-296 (first hole from
+   sum_B |<IJ||AB>|, second from |<IJ||AB>|, each restricted to the unoccupied
+   orbitals), which the reference binds when nBasis <= 4 nEl
+   (decide_on_PCHB_options, pchb_excitgen.F90:216-223).  Weighted particles
+   (UNIF-FULL, FULL-FULL) and weighted holes draw from the attempt's keyed
+   stream, redrawing where the reference's constrained samplers redraw. */
+int ngpu_set_excitgen_pchb(ngpu_t *h, int32_t singles, int32_t particles, int32_t holes, double psingles,
+                           double pparallel);
 
 /* Molecular excitation generator: NECI's uniform random excitation generator
    gen_rand_excit (GenRandSymExcitNUMod.F90:72-153, the default binding of
@@ -207,8 +220,46 @@ int ngpu_download_walkers_dev(ngpu_t *h, int64_t *rows, double *diag_h, double *
    core to core dets (FciMCPar.F90:1659-1668).  One core space per engine: a
    second call with the same determinants (same order) only stores the core
    Hamiltonian again in the precision fp32_values asks for; any other second
-   call returns NGPU_ERR_STATE. */
+   call returns NGPU_ERR_STATE.  Repeated determinants: NGPU_ERR_ARG. */
 int ngpu_set_core(ngpu_t *h, const int64_t *dets, int64_t ncore, int32_t fp32_values, int64_t *nnz);
+
+/* How ngpu_set_core builds the core Hamiltonian (call before it).  Every
+   method gives the same CSR (rows and columns ascending, the same values):
+   NGPU_CORE_BUILD_JOIN finds the connected pairs by joining the core
+   determinants on their (nel - 2)-electron subsets (a radix sort of
+   ncore * nel(nel-1)/2 entries), the role of calc_determ_hamil_opt
+   (fast_determ_hamil.F90:892-1548, semi_stoch_gen.F90:262-268);
+   NGPU_CORE_BUILD_SCAN tests every pair (ncore^2 excitation levels), the role
+   of calc_determ_hamil_sparse (sparse_arrays.F90:426-572, the reference's
+   fallback at semi_stoch_gen.F90:250-260, which is also taken here for fewer
+   than 4 electrons); NGPU_CORE_BUILD_AUTO (the default) takes the scan below
+   450 * nel(nel-1)/2 core determinants and the join from there on.
+   NGPU_ERR_STATE once the core is set. */
+enum { NGPU_CORE_BUILD_AUTO = 0, NGPU_CORE_BUILD_JOIN = 1, NGPU_CORE_BUILD_SCAN = 2 };
+int ngpu_set_core_build(ngpu_t *h, int32_t method);
+
+/* The core Hamiltonian as stored (print_hamiltonian, semi_stoch_procs.F90:
+   2051-2063, PRINT-CORE-HAMIL): rowptr (ncore + 1 entries), cols (0-based core
+   indices, ascending in each row) and vals (H_ij, diagonal H_ii - Hii; FP32
+   values widened).  With rowptr, cols and vals all NULL only *nnz is set;
+   NGPU_ERR_ARG when cap < nnz; NGPU_ERR_STATE without a core.  Waits. */
+int ngpu_get_core_hamil(ngpu_t *h, int64_t *rowptr, int32_t *cols, double *vals, int64_t cap, int64_t *nnz);
+
+/* generate_space_most_populated (semi_stoch_gen.F90:1031-1226) over the
+   resident walkers: the target determinants of largest weight sum_r |c_r|
+   (core_space_weight, semi_stoch_procs.F90:1623-1642, global core space),
+   weights below 1e-8 left out, all of them when fewer are occupied; chosen on
+   the device (stable radix sorts of the occupied slots).  Equal weights are
+   ordered by determinant (ilut_lt), where the reference keeps the first it
+   meets in its storage order.  dets (target * W words) receives them, weight
+   descending; *n their number.  Runs between iterations (waits). */
+int ngpu_most_populated(ngpu_t *h, int64_t target, int64_t *dets, int64_t *n);
+/* The semistochastic core space of the target most populated determinants
+   (ngpu_most_populated, then ngpu_set_core with them): the POPS-CORE core
+   space of init_semi_stochastic (semi_stoch_gen.F90:425-445).  dets (may be
+   NULL) receives the chosen determinants, *ncore their number. */
+int ngpu_select_core(ngpu_t *h, int64_t target, int32_t fp32_values, int64_t *dets, int64_t *ncore,
+                     int64_t *nnz);
 
 /* One FCIQMC iteration (PerformFCIMCycPar, FciMCPar.F90:1177-1920).  Runs
    asynchronously on the engine's stream when out == NULL; otherwise waits and
@@ -240,8 +291,9 @@ int ngpu_get_launch_shape(ngpu_t *h, int32_t *chunk_blocks, int32_t *grid_blocks
                           int32_t *single_lanes, int32_t *merge_blocks, int32_t *n_cu);
 
 /* Batches: several independent engines (different systems, sizes, replica
-   counts; one W for all) iterated together as ONE graph per iteration, one
-   branch per engine, launched on one stream.  No counterpart in the
+   counts) iterated together: one launch per kernel of the iteration chain
+   for each group of engines that share kernel instances
+   (ngpu_batch_get_groups), captured as ONE graph per iteration.  No counterpart in the
    reference, which runs one system per process; each engine's results are
    those of iterating it alone.  While batched, an engine is iterated only
    through ngpu_batch_iterate (ngpu_iterate returns NGPU_ERR_STATE); its
@@ -253,6 +305,21 @@ int ngpu_batch_create(ngpu_batch_t **b, ngpu_t *const *hs, int32_t n);
    in the order given to ngpu_batch_create.  Asynchronous. */
 int ngpu_batch_iterate(ngpu_batch_t *b, const ngpu_iter_in_t *ins);
 int ngpu_batch_destroy(ngpu_batch_t *b);
+/* The batch's launch groups: engines whose iteration chains use the same
+   kernel instances (determinant words, replica layout, excitation generator,
+   sort path, spawn-record and core precision, launch knobs).  Every kernel of
+   the chain is ONE launch for all the engines of a group, each engine taking
+   a share of its blocks in proportion to its walker capacity; the groups run
+   concurrently, each on its own stream.  group_of_engine[i] receives engine
+   i's group (either pointer may be NULL).  No counterpart in the reference. */
+int ngpu_batch_get_groups(ngpu_batch_t *b, int32_t *group_of_engine, int32_t *n_groups);
+/* on != 0 (the default): each batch iteration replays one captured graph;
+   0: the group launches are issued directly (needed for stage timing). */
+int ngpu_batch_set_graph(ngpu_batch_t *b, int32_t on);
+/* Stage timing of the batch's launches (mask as ngpu_set_profiling), while
+   the batch launches directly: each group's stage times are read with
+   ngpu_get_stage_times on the group's first engine. */
+int ngpu_batch_set_profiling(ngpu_batch_t *b, int32_t on);
 /* Wait for the last iteration and return its statistics. */
 int ngpu_get_stats(ngpu_t *h, ngpu_iter_out_t *out);
 /* Update cycle (calculate_new_shift_wrapper, fcimc_iter_utilities.F90:1403):
@@ -284,8 +351,9 @@ int ngpu_get_hash_stats(ngpu_t *h, int64_t *live, int64_t *tombstones, int64_t *
    kernel to the end of its last.  Stages: 0 prepass (initiator, proj-E, death),
    1 attempt scan (2 kernels), 2 spawn (all attempts; molecular: doubles, singles
    queued), 3 queued singles, 4 sort (all radix passes), 5 merge (empty: fused
-   into 6), 6 merge and annihilation, 7 insert, 8 hash statistics.  Stages 0, 2,
-   3, 6, 7 and 8 are one kernel each.  on is a bit mask of the stages to time
+   into 6), 6 merge and annihilation, 7 insert, 8 hash statistics, 9 the core
+   Hamiltonian's H.v (semistochastic projection).  Stages 0, 2, 3, 6, 7, 8 and
+   9 are one kernel each.  on is a bit mask of the stages to time
    (bit s for stage s; 0 switches timing off); each timed stage adds two events
    to its launches, so time only the stages needed. */
 int ngpu_set_profiling(ngpu_t *h, int32_t on);

@@ -18,6 +18,13 @@ MODEL_MOLECULAR = 1
 SINGLES_HEATBATH = 1
 PART_UNIF_UNIF = 0
 PART_UNIF_FAST = 1
+PART_UNIF_FULL = 2
+PART_FULL_FULL = 3
+HOLES_FAST_FAST = 0
+HOLES_FULL_FULL = 1
+CORE_BUILD_AUTO = 0
+CORE_BUILD_JOIN = 1
+CORE_BUILD_SCAN = 2
 FLAGS_MNECI = 0
 FLAGS_NECI = 1
 
@@ -101,9 +108,16 @@ _SIGS = [
     ("ngpu_set_spawn_precision", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("ngpu_set_excitgen", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                          ctypes.c_double]),
+    ("ngpu_set_excitgen_pchb", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_double, ctypes.c_double]),
     ("ngpu_set_hash", ctypes.c_int, [_vp, _vp]),
     ("ngpu_set_reference", ctypes.c_int, [_vp, _vp, ctypes.c_double]),
     ("ngpu_set_core", ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int32, P(ctypes.c_int64)]),
+    ("ngpu_set_core_build", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("ngpu_get_core_hamil", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int64, P(ctypes.c_int64)]),
+    ("ngpu_most_populated", ctypes.c_int, [_vp, ctypes.c_int64, _vp, P(ctypes.c_int64)]),
+    ("ngpu_select_core", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, _vp, P(ctypes.c_int64),
+                                        P(ctypes.c_int64)]),
     ("ngpu_upload_walkers", ctypes.c_int, [_vp, _vp, ctypes.c_int64]),
     ("ngpu_download_walkers", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int64, P(ctypes.c_int64)]),
     ("ngpu_download_walkers_dev", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int64, _vp]),
@@ -122,6 +136,9 @@ _SIGS = [
     ("ngpu_batch_create", ctypes.c_int, [P(_vp), _vp, ctypes.c_int32]),
     ("ngpu_batch_iterate", ctypes.c_int, [_vp, _vp]),
     ("ngpu_batch_destroy", ctypes.c_int, [_vp]),
+    ("ngpu_batch_get_groups", ctypes.c_int, [_vp, _vp, P(ctypes.c_int32)]),
+    ("ngpu_batch_set_graph", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("ngpu_batch_set_profiling", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("ngpu_get_stage_times", ctypes.c_int, [_vp, _vp, ctypes.c_int32, P(ctypes.c_int64)]),
     ("ngpu_find_walker_hash_batch", ctypes.c_int, [_vp, _vp, ctypes.c_int64, _vp]),
     ("ngpu_get_spawn_helement_batch", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, _vp]),

@@ -13,6 +13,10 @@ from .systems import (HubbardChain, nwords, psingles_from_reference, random_hash
 
 PART_UNIF_UNIF = 0
 PART_UNIF_FAST = 1
+PART_UNIF_FULL = 2
+PART_FULL_FULL = 3
+HOLES_FAST_FAST = 0
+HOLES_FULL_FULL = 1
 
 
 @dataclass
@@ -36,6 +40,7 @@ class Spec:
     n_hash_buckets: Optional[int] = None
     singles: int = 1
     particles: int = PART_UNIF_FAST
+    holes: int = HOLES_FAST_FAST  # PCHB hole selection: FAST-FAST or FULL-FULL (fully weighted)
     excitgen: str = "pchb"   # molecular: "pchb" (PCHB doubles + heat-bath singles) or "uniform" (gen_rand_excit)
     psingles: Optional[float] = None
     pparallel: float = 0.5

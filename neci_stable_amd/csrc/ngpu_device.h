@@ -266,6 +266,18 @@ struct DevSys {
   const double* j_prob;
   const uint8_t* j_ok;
   const int32_t* ab_tgt;       // tgtOrbs: ab_tgt[2*(ab-1)] = B <= A = ab_tgt[2*(ab-1)+1]
+  // weighted PCHB samplers: holes 0 FAST-FAST (ab_*), 1 FULL-FULL (a_*, b_*);
+  // particles 2 UNIF-FULL, 3 FULL-FULL (i_* and j_*)
+  int holes;
+  int i_ok;
+  const AliasEnt* i_tab;       // first particle over the spin orbitals
+  const double* i_prob;
+  const AliasEnt* a_tab;       // first hole per IJ: row IJ-1 of nbasis entries
+  const double* a_prob;
+  const uint8_t* a_ok;
+  const AliasEnt* b_tab;       // second hole per (IJ, A): row (IJ-1)*nbasis + A-1
+  const double* b_prob;
+  const uint8_t* b_ok;
   // heat-bath singles: (U(a,j,b,j), U(a,j,j,b)) at jk[2*(((a-1)*nsp + j-1)*nsp + b-1)]
   // for spatial a, j, b; copies of UMAT entries, rows contiguous in b
   const double* jk;
@@ -650,7 +662,7 @@ __device__ __host__ inline Excit gen_double_pchb(const DevSys& s, const uint64_t
 // replica | (block + 1) << 8, two uniforms per block.
 template <int W>
 struct DrawSeq {
-  const double* dr;
+  double d1, d2, d3, d4;  // the attempt's d1..d4, held by value
   uint32_t seed;
   const uint64_t* key;
   uint32_t iter;
@@ -660,10 +672,11 @@ struct DrawSeq {
   int blk;
   double c0, c1;
   __device__ __host__ DrawSeq(const double* d, uint32_t sd, const uint64_t* ky, uint32_t it, int r, uint32_t kk)
-      : dr(d), seed(sd), key(ky), iter(it), rep(r), k(kk), n(0), blk(-1), c0(0.0), c1(0.0) {}
+      : d1(d[1]), d2(d[2]), d3(d[3]), d4(d[4]), seed(sd), key(ky), iter(it), rep(r), k(kk), n(0), blk(-1),
+        c0(0.0), c1(0.0) {}
   __device__ __host__ double next() {
     ++n;
-    if (n <= 4) return dr[n];
+    if (n <= 4) return n == 1 ? d1 : n == 2 ? d2 : n == 3 ? d3 : d4;
     int j = n - 5, b = j >> 1;
     if (b != blk) {
       uint64_t dh = det_hash64<W>(key);
@@ -676,6 +689,229 @@ struct DrawSeq {
     return (j & 1) ? c1 : c0;
   }
 };
+
+// ------------------------------------------ weighted PCHB particles and holes
+// Restatement of gasci_pchb_doubles_select_particles.fpp (UNIF-FULL :427-500,
+// FULL-FULL :333-425) and gasci_pchb_doubles_spinorb_fullyweighted.fpp
+// (GAS_doubles_PCHB_gen_exc / get_pgen :138-296), every uniform from the
+// attempt's draw sequence: particles first, then holes.
+
+constexpr int kMaxRedraw = 1024;  // alias redraws before a constrained draw gives up
+
+// spin orbitals not in d (within nbasis)
+template <int W>
+__device__ __host__ inline void unocc_mask(const DevSys& s, const uint64_t* d, uint64_t* u) {
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    const int nb = s.nbasis - 64 * w;
+    const uint64_t valid = nb >= 64 ? ~0ull : (nb <= 0 ? 0ull : ((1ull << nb) - 1ull));
+    u[w] = ~d[w] & valid;
+  }
+}
+
+// sum(get_prob(contain)) of one sampler row over the set bits of x, in
+// ascending order (row == nullptr: an empty sampler, every probability 0)
+template <int W>
+__device__ __host__ inline double set_sum(const double* row, const uint64_t* x) {
+  if (!row) return 0.0;
+  double r = 0.0;
+#pragma unroll
+  for (int w = 0; w < W; ++w) r = accum_bits4(x[w], 64 * w, r, false, [&](int o) { return &row[o - 1]; });
+  return r;
+}
+
+// constrained_sample_fast (aliasSampling.F90:505-535): below the redrawing
+// cutoff 0.1 the CDF sampler over prob(contain)/renorm (CDF_sampling.fpp:
+// first cumulated value >= r), else alias draws until one lies in the set.
+// Returns the orbital (0: none) and its probability; a pick of probability
+// zero (a cumulated value at r == 0, or r beyond a rounded last cumulated
+// value) is a null draw.
+template <int W>
+__device__ __host__ inline int constrained_draw(const AliasEnt* tab, const double* prob, int n, const uint64_t* set,
+                                                double renorm, DrawSeq<W>& u, double& p) {
+  if (!prob || fabs(renorm) <= kEps) return 0;
+  if (renorm < 0.1) {
+    const double r = u.next();
+    double cum = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+      for (uint64_t x = set[w]; x; x &= x - 1) {
+        const int o = 64 * w + ctz64(x) + 1;
+        const double pk = prob[o - 1] / renorm;
+        cum += pk;
+        if (!(cum < r)) {
+          if (pk == 0.0) return 0;
+          p = pk;
+          return o;
+        }
+      }
+    return 0;
+  }
+  for (int t = 0; t < kMaxRedraw; ++t) {
+    double pa;
+    const int tgt = alias_sample(tab, n, u.next(), pa) + 1;
+    if (!test_orb<W>(set, tgt)) continue;
+    p = prob[tgt - 1] / renorm;
+    return p == 0.0 ? 0 : tgt;
+  }
+  return 0;
+}
+
+__device__ __host__ inline const double* j_row(const DevSys& s, int I) {
+  return s.j_ok[I - 1] ? s.j_prob + (size_t)(I - 1) * s.nbasis : nullptr;
+}
+
+// the ordered particle pair and p({I, J}) of UNIF-FULL / FULL-FULL; false: null
+template <int W>
+__device__ __host__ inline bool draw_weighted_particles(const DevSys& s, const uint64_t* d, DrawSeq<W>& u, int& s1,
+                                                        int& s2, double& p) {
+  const int n = s.nbasis, nel = s.nel;
+  const bool full = s.particles == 3;
+  double pf0 = 0.0, ps0 = 0.0, rf = 0.0;
+  if (full) {
+    const double* ip = s.i_ok ? s.i_prob : nullptr;
+    rf = set_sum<W>(ip, d);
+    s1 = constrained_draw<W>(s.i_tab, ip, n, d, rf, u, pf0);
+    if (!s1) return false;
+  } else {
+    const int e1 = (int)(u.next() * (double)nel) + 1;
+    s1 = select_orb<W>(d, ~0ull, e1);
+  }
+  const double* j1 = j_row(s, s1);
+  const double rs1 = set_sum<W>(j1, d);
+  s2 = constrained_draw<W>(s.j_tab + (size_t)(s1 - 1) * n, j1, n, d, rs1, u, ps0);
+  if (!s2) return false;
+  const double* j2 = j_row(s, s2);
+  const double rs2 = set_sum<W>(j2, d);
+  const double ps1 = (!j2 || fabs(rs2) <= kEps) ? 0.0 : j2[s1 - 1] / rs2;
+  if (full) {
+    const double pf1 = fabs(rf) <= kEps ? 0.0 : s.i_prob[s2 - 1] / rf;
+    p = pf0 * ps0 + pf1 * ps1;
+  } else {
+    p = (ps0 + ps1) / (double)nel;
+  }
+  if (s1 > s2) { int t = s1; s1 = s2; s2 = t; }
+  return true;
+}
+
+template <int W>
+__device__ __host__ inline double pgen_weighted_particles(const DevSys& s, const uint64_t* d, int I, int J) {
+  const double *jI = j_row(s, I), *jJ = j_row(s, J);
+  const double rI = set_sum<W>(jI, d), rJ = set_sum<W>(jJ, d);
+  const double psI = (!jI || fabs(rI) <= kEps) ? 0.0 : jI[J - 1] / rI;
+  const double psJ = (!jJ || fabs(rJ) <= kEps) ? 0.0 : jJ[I - 1] / rJ;
+  if (s.particles != 3) return (psI + psJ) / (double)s.nel;
+  const double* ip = s.i_ok ? s.i_prob : nullptr;
+  const double rf = set_sum<W>(ip, d);
+  const double pfI = (!ip || fabs(rf) <= kEps) ? 0.0 : ip[I - 1] / rf;
+  const double pfJ = (!ip || fabs(rf) <= kEps) ? 0.0 : ip[J - 1] / rf;
+  return pfI * psI + pfJ * psJ;
+}
+
+// renormalisation of a hole row over the unoccupied set: one minus the
+// occupied share, or the direct sum when that is 1 or below 1e-5
+// (do_direct_calculation, aliasSampling.F90:967-1000)
+template <int W>
+__device__ __host__ inline double hole_renorm(const double* row, const uint64_t* d, const uint64_t* un) {
+  double r = 1.0 - set_sum<W>(row, d);
+  if (r >= 1.0 || r < 1e-5) r = set_sum<W>(row, un);
+  return r;
+}
+
+__device__ __host__ inline const double* a_row(const DevSys& s, int IJ) {
+  return s.a_ok[IJ - 1] ? s.a_prob + (size_t)(IJ - 1) * s.nbasis : nullptr;
+}
+__device__ __host__ inline const double* b_row(const DevSys& s, int IJ, int A) {
+  const size_t r = (size_t)(IJ - 1) * s.nbasis + (size_t)(A - 1);
+  return s.b_ok[r] ? s.b_prob + r * s.nbasis : nullptr;
+}
+
+// the fully weighted holes: (t0, t1) in drawing order and sum(p_first * p_second)
+template <int W>
+__device__ __host__ inline bool draw_full_holes(const DevSys& s, const uint64_t* d, int I, int J, DrawSeq<W>& u,
+                                                int& t0, int& t1, double& ph) {
+  const int n = s.nbasis;
+  uint64_t un[W];
+  unocc_mask<W>(s, d, un);
+  const int IJ = fuse_index(I, J);
+  const double* ar = a_row(s, IJ);
+  const double rf = hole_renorm<W>(ar, d, un);
+  double pf0, ps0;
+  t0 = constrained_draw<W>(s.a_tab + (size_t)(IJ - 1) * n, ar, n, un, rf, u, pf0);
+  if (!t0) return false;
+  const double* b1 = b_row(s, IJ, t0);
+  const double rs1 = hole_renorm<W>(b1, d, un);
+  if (fabs(rs1) <= kEps) return false;
+  t1 = constrained_draw<W>(s.b_tab + ((size_t)(IJ - 1) * n + (size_t)(t0 - 1)) * n, b1, n, un, rs1, u, ps0);
+  if (!t1) return false;
+  const double pf1 = (!ar || fabs(rf) <= kEps) ? 0.0 : ar[t1 - 1] / rf;
+  const double* b2 = b_row(s, IJ, t1);
+  const double rs2 = hole_renorm<W>(b2, d, un);
+  const double ps1 = (!b2 || fabs(rs2) <= kEps) ? 0.0 : b2[t0 - 1] / rs2;
+  ph = pf0 * ps0 + pf1 * ps1;
+  return true;
+}
+
+template <int W>
+__device__ __host__ inline double pgen_full_holes(const DevSys& s, const uint64_t* d, int I, int J, int A, int B) {
+  uint64_t un[W];
+  unocc_mask<W>(s, d, un);
+  const int IJ = fuse_index(I, J);
+  const double* ar = a_row(s, IJ);
+  const double rf = hole_renorm<W>(ar, d, un);
+  const double pfA = (!ar || fabs(rf) <= kEps) ? 0.0 : ar[A - 1] / rf;
+  const double pfB = (!ar || fabs(rf) <= kEps) ? 0.0 : ar[B - 1] / rf;
+  const double *bA = b_row(s, IJ, A), *bB = b_row(s, IJ, B);
+  const double rA = hole_renorm<W>(bA, d, un), rB = hole_renorm<W>(bB, d, un);
+  const double psA = (!bA || fabs(rA) <= kEps) ? 0.0 : bA[B - 1] / rA;
+  const double psB = (!bB || fabs(rB) <= kEps) ? 0.0 : bB[A - 1] / rB;
+  return pfA * psA + pfB * psB;
+}
+
+// PCHB doubles with weighted particles or fully weighted holes
+template <int W>
+__device__ __host__ inline Excit gen_double_pchb_weighted(const DevSys& s, const uint64_t* d, DrawSeq<W>& u) {
+  Excit ex{};
+  const int nel = s.nel, n = s.nbasis;
+  int s1 = 0, s2 = 0;
+  double p;
+  if (s.particles >= 2) {
+    if (!draw_weighted_particles<W>(s, d, u, s1, s2, p)) return ex;
+  } else if (s.particles == 1) {
+    const int e1 = (int)(u.next() * (double)nel) + 1;
+    s1 = select_orb<W>(d, ~0ull, e1);
+    if (!s.j_ok[s1 - 1]) return ex;
+    double p_J_I;
+    const int j = alias_sample(s.j_tab + (size_t)(s1 - 1) * n, n, u.next(), p_J_I);
+    s2 = j + 1;
+    if (!test_orb<W>(d, s2)) return ex;
+    const double p_I_J = s.j_ok[s2 - 1] ? s.j_prob[(size_t)(s2 - 1) * n + (s1 - 1)] : 0.0;
+    p = (p_J_I + p_I_J) / (double)nel;
+    if (s1 > s2) { int t = s1; s1 = s2; s2 = t; }
+  } else {
+    pick_biased_elecs<W>(s, d, u.next(), s1, s2, p);
+    if (s1 == 0 || s2 == 0) return ex;
+  }
+  int t1, t2;
+  double ph;
+  if (s.holes == 1) {
+    if (!draw_full_holes<W>(s, d, s1, s2, u, t1, t2, ph)) return ex;
+    if (t1 > t2) { int t = t1; t1 = t2; t2 = t; }
+  } else {
+    const int ij = fuse_index(s1, s2);
+    if (!s.ab_ok[ij - 1]) return ex;
+    const int ab = alias_sample(s.ab_tab + (size_t)(ij - 1) * s.ijmax, s.ijmax, u.next(), ph) + 1;
+    t1 = s.ab_tgt[2 * (ab - 1)];
+    t2 = s.ab_tgt[2 * (ab - 1) + 1];
+    if (t1 == t2 || test_orb<W>(d, t1) || test_orb<W>(d, t2)) return ex;
+    if (fabs(ph) <= kEps) return ex;
+  }
+  ex.ic = 2;
+  ex.e0 = s1; ex.e1 = s2; ex.e2 = t1; ex.e3 = t2;
+  ex.parity = double_parity<W>(d, s1, s2, t1, t2);
+  ex.pgen = p * ph;
+  return ex;
+}
 
 // gen_rand_excit (GenRandSymExcitNUMod.F90:72-153) for C1 systems (one
 // irrep, no Lz): CreateDoubExcit (:155-185: PickElecPair :1021, the forbidden
@@ -864,6 +1100,10 @@ __device__ __host__ inline Excit gen_excit(const DevSys& s, const uint64_t* d, c
   if (dr[0] < s.psingles) {  // gen_exc_sd (excitation_generators.F90:112-138)
     ex = gen_single_heatbath<W>(s, d, dr[1], dr[2]);
     ex.pgen = ex.pgen * s.psingles;
+  } else if (s.particles >= 2 || s.holes) {
+    DrawSeq<W> u(dr, s.seed, d, iter, rep, k);
+    ex = gen_double_pchb_weighted<W>(s, d, u);
+    ex.pgen = ex.pgen * s.pdoubles;
   } else {
     ex = gen_double_pchb<W>(s, d, dr);
     ex.pgen = ex.pgen * s.pdoubles;

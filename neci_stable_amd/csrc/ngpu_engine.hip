@@ -10,12 +10,13 @@
 #include <vector>
 
 #include "ngpu_kernels.h"
+#include "ngpu_batch.h"
 
 using namespace ngpu;
 
 namespace {
 
-constexpr int kStages = 9;
+constexpr int kStages = 10;  // stage 9: the core Hamiltonian's H.v
 constexpr int kEvRing = 32;  // iterations whose stage events may be in flight
 
 #define HIP_OK(x)                                                        \
@@ -104,6 +105,7 @@ struct ngpu_engine {
   // semistochastic core space
   int64_t ncore = 0, core_nnz = 0;
   bool core_fp32 = false;
+  int core_build = NGPU_CORE_BUILD_AUTO;  // how the core Hamiltonian is built
   uint64_t* core_keys = nullptr;
   int64_t* core_slot = nullptr;
   int64_t* core_rowptr = nullptr;
@@ -146,6 +148,14 @@ struct ngpu_engine {
   double* d_j_prob = nullptr;
   uint8_t* d_j_ok = nullptr;
   int32_t* d_ab_tgt = nullptr;
+  AliasEnt* d_i_tab = nullptr;
+  double* d_i_prob = nullptr;
+  AliasEnt* d_a_tab = nullptr;
+  double* d_a_prob = nullptr;
+  uint8_t* d_a_ok = nullptr;
+  AliasEnt* d_b_tab = nullptr;
+  double* d_b_prob = nullptr;
+  uint8_t* d_b_ok = nullptr;
   bool have_hash = false, have_ref = false, have_model = false, have_excitgen = false;
   bool pending = false;
   // profiling
@@ -176,10 +186,31 @@ struct ngpu_engine {
   hipStream_t own_stream = nullptr;
 };
 
+// Engines of a batch whose chains use the same kernel instances: each kernel
+// of the chain is one launch for all of them (ngpu_batch.h).
+struct BatchGroup {
+  std::vector<ngpu_t*> hs;
+  int W = 1, S = 1;
+  int gen = 0;              // kGenHubbard, kGenPchb or kGenUniform
+  bool singles = false;     // queued heat-bath singles (PCHB generator)
+  int lanes = 4, pl = 4;    // heat-bath singles: lanes per attempt, orbitals per lane
+  int items = 4;            // radix tile: elements per thread
+  bool rank_sort = false;
+  std::vector<std::pair<int, int>> passes;  // (word, shift) of the word-key sort
+  bool core = false, core_fp32 = false;
+  int rec_words = 0;
+  EngDev* d_ed = nullptr;
+  uint16_t* d_map[kLkKinds] = {nullptr};
+  int tot[kLkKinds] = {0};
+};
+
 struct ngpu_batch {
   std::vector<ngpu_t*> hs;
+  std::vector<BatchGroup> groups;
+  bool groups_ok = false;   // groups and their device descriptors match the engines
+  bool use_graph = true;    // replay one captured graph per iteration (else launch directly)
   hipStream_t stream = nullptr;
-  std::vector<hipStream_t> side;
+  std::vector<hipStream_t> side;  // one per group
   std::vector<hipEvent_t> join;
   hipEvent_t fork = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -223,6 +254,7 @@ __global__ void k_begin_iter(ngpu_iter_out_t* out, double* part, Counters* cnt, 
     cnt->n_occ = 0;
     cnt->n_singles = 0;
     cnt->n_fail = 0;
+    cnt->ins_refused = 0;
     cnt->err = 0;
     out->tot_walkers = 0;
     out->n_attempts = out->n_spawned = out->n_unique = out->n_new = 0;
@@ -233,8 +265,8 @@ __global__ void k_begin_iter(ngpu_iter_out_t* out, double* part, Counters* cnt, 
 // sum of the block-statistics rows, final counters of the iteration, then
 // accumulation into the cycle sums (ENumCyc, HFCyc, NoBorn, ... are cycle
 // accumulators in the reference too)
-__global__ void k_end_iter(ngpu_iter_out_t* out, double* part, ngpu_iter_out_t* cyc, Counters* cnt,
-                           int64_t scap, int64_t cap, int64_t hash_entries, const IterParams* prm) {
+__device__ __forceinline__ void end_iter_body(ngpu_iter_out_t* out, double* part, ngpu_iter_out_t* cyc, Counters* cnt,
+                                              int64_t scap, int64_t cap, int64_t hash_entries, const IterParams* prm) {
   // halted: the output block keeps the failing iteration's statistics and
   // error, and nothing is added to the cycle
   if (cnt->halt) return;
@@ -305,8 +337,18 @@ __global__ void k_end_iter(ngpu_iter_out_t* out, double* part, ngpu_iter_out_t* 
     cnt->n_singles = 0;
     cnt->n_new = 0;
     cnt->n_fail = 0;
+    cnt->ins_refused = 0;
     cnt->err = 0;
   }
+}
+__global__ void k_end_iter(ngpu_iter_out_t* out, double* part, ngpu_iter_out_t* cyc, Counters* cnt,
+                           int64_t scap, int64_t cap, int64_t hash_entries, const IterParams* prm) {
+  end_iter_body(out, part, cyc, cnt, scap, cap, hash_entries, prm);
+}
+// the end of the iteration of every engine of a batch group: one block each
+__global__ void km_end_iter(const EngDev* __restrict__ ed) {
+  const EngDev& a = ed[blockIdx.x];
+  end_iter_body(a.dout, a.part, a.dcyc, a.cnt, a.scap, a.wk.cap, a.wk.L * (int64_t)kBucketEntries, a.prm);
 }
 
 __global__ void k_reset_cycle(ngpu_iter_out_t* cyc) {
@@ -455,11 +497,11 @@ void core_projection(ngpu_t* h) {
          h->S, h->ncore, h->core_x);
   int nbw = (int)std::min<int64_t>((h->ncore + kBlock / kWave - 1) / (kBlock / kWave), 8 * h->nb_grid);
   if (h->core_fp32)
-    launch(h, -1, -1, k_core_spmv<R, float>, dim3(nbw), dim3(kBlock), (const int64_t*)h->core_rowptr,
+    launch(h, 9, 9, k_core_spmv<R, float>, dim3(nbw), dim3(kBlock), (const int64_t*)h->core_rowptr,
            (const int32_t*)h->core_cols, (const float*)h->core_vals, (const double*)h->core_x, h->core_y, h->ncore,
            h->S, (const IterParams*)h->dparams);
   else
-    launch(h, -1, -1, k_core_spmv<R, double>, dim3(nbw), dim3(kBlock), (const int64_t*)h->core_rowptr,
+    launch(h, 9, 9, k_core_spmv<R, double>, dim3(nbw), dim3(kBlock), (const int64_t*)h->core_rowptr,
            (const int32_t*)h->core_cols, (const double*)h->core_vals, (const double*)h->core_x, h->core_y, h->ncore,
            h->S, (const IterParams*)h->dparams);
 }
@@ -483,6 +525,24 @@ void annihilation_chain(ngpu_t* h, bool with_core, bool list_out = false) {
          h->scap, h->wk.cap, h->wk.L * (int64_t)kBucketEntries, (const IterParams*)h->dparams);
 }
 
+// (word, shift) passes of the word-key sort of an engine (ilut_lt order:
+// least significant word first)
+std::vector<std::pair<int, int>> word_passes(const ngpu_t* h) {
+  std::vector<std::pair<int, int>> p;
+  const int hi = h->W == 1 ? h->sys.nbasis : 64, lo = h->W == 2 ? h->sys.nbasis - 64 : 0;
+  if (h->W == 2)
+    for (int sh = 0; sh < lo; sh += 8) p.push_back({1, sh});
+  for (int sh = 0; sh < hi; sh += 8) p.push_back({0, sh});
+  return p;
+}
+
+// the generator instance of an engine's spawn kernel
+int gen_of(const ngpu_t* h) {
+  if (h->sys.model == NGPU_MODEL_HUBBARD) return kGenHubbard;
+  if (h->ds.uniform) return kGenUniform;
+  return (h->ds.particles >= NGPU_PART_UNIF_FULL || h->ds.holes) ? kGenPchbW : kGenPchb;
+}
+
 template <int W, int R>
 void iterate_chain(ngpu_t* h) {
   launch(h, 0, 0, k_prepass<W, R>, dim3(h->nb_chunk), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart, h->att_cnt,
@@ -491,8 +551,10 @@ void iterate_chain(ngpu_t* h) {
   launch(h, 1, 1, k_scan_apply<W>, dim3(h->nb_chunk), dim3(kBlock), h->ds, h->wk, (const int32_t*)h->att_cnt,
          h->att_off, (const int64_t*)h->blk_part, (const int64_t*)h->blk_rm, (const uint8_t*)h->pinfo, h->cnt,
          h->S, h->tile_p, h->tcap);
-  auto spawn = h->sys.model == NGPU_MODEL_HUBBARD ? k_spawn<W, R, kGenHubbard>
-             : h->ds.uniform ? k_spawn<W, R, kGenUniform> : k_spawn<W, R, kGenPchb>;
+  const int gen = gen_of(h);
+  auto spawn = gen == kGenHubbard ? k_spawn<W, R, kGenHubbard>
+             : gen == kGenUniform ? k_spawn<W, R, kGenUniform>
+             : gen == kGenPchbW ? k_spawn<W, R, kGenPchbW> : k_spawn<W, R, kGenPchb>;
   launch(h, 2, 2, spawn, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart,
          (const int64_t*)h->att_off, (const int64_t*)h->tile_p, h->tcap, (const uint8_t*)h->pinfo, h->sp_rec,
          h->rank_sort ? h->rk : nullptr, h->scap, h->sq, (const IterParams*)h->dparams);
@@ -647,7 +709,9 @@ __device__ double calc_pgen(const DevSys& s, const uint64_t* d, int ic, const in
   if (ic == 2) {
     int I = ex[0], J = ex[1], n = s.nbasis;
     double p;
-    if (s.particles == 1) {
+    if (s.particles >= 2) {
+      p = pgen_weighted_particles<W>(s, d, I, J);
+    } else if (s.particles == 1) {
       double a = s.j_ok[I - 1] ? s.j_prob[(size_t)(I - 1) * n + (J - 1)] : 0.0;
       double b = s.j_ok[J - 1] ? s.j_prob[(size_t)(J - 1) * n + (I - 1)] : 0.0;
       p = (a + b) / (double)s.nel;
@@ -659,8 +723,13 @@ __device__ double calc_pgen(const DevSys& s, const uint64_t* d, int ic, const in
       if (is_beta(I) == is_beta(J)) p = s.pparallel / (double)par;
       else p = (1.0 - s.pparallel) / (double)(na * nb);
     }
-    int ij = fuse_index(I, J), ab = fuse_index(ex[2], ex[3]);
-    double ph = s.ab_ok[ij - 1] ? s.ab_prob[(size_t)(ij - 1) * s.ijmax + ab - 1] : 0.0;
+    double ph;
+    if (s.holes == 1) {
+      ph = pgen_full_holes<W>(s, d, I, J, ex[2], ex[3]);
+    } else {
+      int ij = fuse_index(I, J), ab = fuse_index(ex[2], ex[3]);
+      ph = s.ab_ok[ij - 1] ? s.ab_prob[(size_t)(ij - 1) * s.ijmax + ab - 1] : 0.0;
+    }
     p = p * ph;
     return s.pdoubles * p;
   }
@@ -891,7 +960,8 @@ int ngpu_destroy(ngpu_t* h) {
                   h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->digit_tot, h->sq, h->n_key,
                   h->n_sgn, h->n_flag, h->u_key, h->u_sgn, h->u_flag, h->u_count, h->cnt, h->dout, h->dpart, h->dcyc, h->own_params, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_binom, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
                   h->core_vals, h->core_x, h->core_y, h->d_ab_tab,
-                  h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt};
+                  h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt, h->d_i_tab, h->d_i_prob,
+                  h->d_a_tab, h->d_a_prob, h->d_a_ok, h->d_b_tab, h->d_b_prob, h->d_b_ok};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->hout) (void)hipHostFree(h->hout);
@@ -980,11 +1050,36 @@ int ngpu_set_lattice(ngpu_t* h, const int32_t* nbr, int32_t nsites, int32_t maxn
 // GAS_doubles_PCHB_spinorb_compute_samplers (gasci_pchb_doubles_spinorb_fastweighted.fpp:280-347)
 // for the FCI case and the J sampler of init_PC_WeightedParticles_t
 // (gasci_pchb_doubles_select_particles.fpp:254-301)
-int ngpu_set_excitgen(ngpu_t* h, int32_t singles, int32_t particles, double psingles, double pparallel) {
+extern "C++" {
+namespace {
+// one sampler row: alias table and probabilities; false (row left empty) when
+// the weights sum to zero (init_AliasSampler_t, aliasSampling.F90:376-398)
+bool alias_row(const std::vector<double>& w, AliasEnt* tab, double* prob) {
+  double sw = 0.0;
+  for (double x : w) sw += x;
+  if (fabs(sw) <= kEps) return false;
+  build_alias(w, tab, prob);
+  return true;
+}
+
+template <typename T>
+int upload_table(T** dst, const std::vector<T>& src) {
+  if (*dst) (void)hipFree(*dst);
+  *dst = nullptr;
+  HIP_OK(dalloc(dst, src.size()));
+  HIP_OK(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return NGPU_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+int ngpu_set_excitgen_pchb(ngpu_t* h, int32_t singles, int32_t particles, int32_t holes, double psingles,
+                           double pparallel) {
   if (h) h->graph_dirty = true;  // kernel arguments of the captured chain change
   if (!h || h->sys.model != NGPU_MODEL_MOLECULAR || !h->have_model) return NGPU_ERR_ARG;
   if (singles != NGPU_SINGLES_HEATBATH) return NGPU_ERR_ARG;
-  if (particles != NGPU_PART_UNIF_UNIF && particles != NGPU_PART_UNIF_FAST) return NGPU_ERR_ARG;
+  if (particles < NGPU_PART_UNIF_UNIF || particles > NGPU_PART_FULL_FULL) return NGPU_ERR_ARG;
+  if (holes != NGPU_HOLES_FAST_FAST && holes != NGPU_HOLES_FULL_FULL) return NGPU_ERR_ARG;
   if (!(psingles >= 0.0 && psingles <= 1.0) || !(pparallel >= 0.0 && pparallel <= 1.0)) return NGPU_ERR_ARG;
   h->ds.uniform = 0;
   const int n = h->sys.nbasis;
@@ -997,77 +1092,122 @@ int ngpu_set_excitgen(ngpu_t* h, int32_t singles, int32_t particles, double psin
   DevSys hs = h->ds;
   hs.tmat = tmat.data();
   hs.umat = umat.data();
-  std::vector<AliasEnt> ab_tab((size_t)ijmax * ijmax);
-  std::vector<double> ab_prob((size_t)ijmax * ijmax, 0.0);
-  std::vector<uint8_t> ab_ok(ijmax, 0);
+  // |<IJ||AB>| of a spin-allowed double, 0 otherwise (get_PCHB_weight,
+  // gasci_pchb_doubles_select_particles.fpp:596-609)
+  auto weight = [&](int I, int J, int A, int B) {
+    int t1 = std::min(A, B), t2 = std::max(A, B);
+    int ms_s = (is_beta(I) ? -1 : 1) + (is_beta(J) ? -1 : 1);
+    int ms_t = (is_beta(t1) ? -1 : 1) + (is_beta(t2) ? -1 : 1);
+    return ms_s == ms_t ? fabs(sltcnd_2(hs, I, J, t1, t2)) : 0.0;
+  };
   std::vector<double> ijw((size_t)n * n, 0.0);
-  std::vector<double> w(ijmax);
-  for (int I = 1; I <= n; ++I)
-    for (int J = I + 1; J <= n; ++J) {
-      std::fill(w.begin(), w.end(), 0.0);
-      for (int A = 1; A <= n; ++A) {
-        if (A == I || A == J) continue;
-        for (int B = 1; B <= n; ++B) {
-          if (B == I || B == J || B == A) continue;
-          int t1 = std::min(A, B), t2 = std::max(A, B);
-          int ms_s = (is_beta(I) ? -1 : 1) + (is_beta(J) ? -1 : 1);
-          int ms_t = (is_beta(t1) ? -1 : 1) + (is_beta(t2) ? -1 : 1);
-          if (ms_s == ms_t) w[fuse_index(A, B) - 1] = fabs(sltcnd_2(hs, I, J, t1, t2));
+  if (holes == NGPU_HOLES_FAST_FAST) {
+    // GAS_doubles_PCHB_spinorb_compute_samplers (gasci_pchb_doubles_spinorb_fastweighted.fpp:280-347)
+    std::vector<AliasEnt> ab_tab((size_t)ijmax * ijmax);
+    std::vector<double> ab_prob((size_t)ijmax * ijmax, 0.0);
+    std::vector<uint8_t> ab_ok(ijmax, 0);
+    std::vector<double> w(ijmax);
+    for (int I = 1; I <= n; ++I)
+      for (int J = I + 1; J <= n; ++J) {
+        std::fill(w.begin(), w.end(), 0.0);
+        for (int A = 1; A <= n; ++A) {
+          if (A == I || A == J) continue;
+          for (int B = 1; B <= n; ++B) {
+            if (B == I || B == J || B == A) continue;
+            w[fuse_index(A, B) - 1] = weight(I, J, A, B);
+          }
         }
+        double sw = 0.0;
+        for (double x : w) sw += x;
+        int IJ = fuse_index(I, J);
+        ab_ok[IJ - 1] = alias_row(w, &ab_tab[(size_t)(IJ - 1) * ijmax], &ab_prob[(size_t)(IJ - 1) * ijmax]) ? 1 : 0;
+        ijw[(size_t)(I - 1) + (size_t)(J - 1) * n] = sw;
+        ijw[(size_t)(J - 1) + (size_t)(I - 1) * n] = sw;
       }
-      double sw = 0.0;
-      for (double x : w) sw += x;
-      int IJ = fuse_index(I, J);
-      if (!(fabs(sw) <= kEps)) {
-        build_alias(w, &ab_tab[(size_t)(IJ - 1) * ijmax], &ab_prob[(size_t)(IJ - 1) * ijmax]);
-        ab_ok[IJ - 1] = 1;
+    std::vector<int32_t> tgt(2 * (size_t)ijmax);
+    for (int A = 1; A <= n; ++A)
+      for (int B = 1; B <= A; ++B) {
+        int AB = fuse_index(A, B);
+        tgt[2 * (AB - 1)] = B;
+        tgt[2 * (AB - 1) + 1] = A;
       }
-      ijw[(size_t)(I - 1) + (size_t)(J - 1) * n] = sw;
-      ijw[(size_t)(J - 1) + (size_t)(I - 1) * n] = sw;
-    }
-  std::vector<AliasEnt> j_tab((size_t)n * n);
-  std::vector<double> j_prob((size_t)n * n, 0.0);
+    if (int rc = upload_table(&h->d_ab_tab, ab_tab)) return rc;
+    if (int rc = upload_table(&h->d_ab_prob, ab_prob)) return rc;
+    if (int rc = upload_table(&h->d_ab_ok, ab_ok)) return rc;
+    if (int rc = upload_table(&h->d_ab_tgt, tgt)) return rc;
+  } else {
+    // GAS_doubles_PCHB_compute_samplers of the fully weighted holes
+    // (gasci_pchb_doubles_spinorb_fullyweighted.fpp:317-391): per (I < J), per
+    // A, the B row |<IJ||AB>|; the A row its sums; IJ_weights their sum
+    std::vector<AliasEnt> a_tab((size_t)ijmax * n), b_tab((size_t)ijmax * n * n);
+    std::vector<double> a_prob((size_t)ijmax * n, 0.0), b_prob((size_t)ijmax * n * n, 0.0);
+    std::vector<uint8_t> a_ok(ijmax, 0), b_ok((size_t)ijmax * n, 0);
+    std::vector<double> wa(n), wb(n);
+    for (int I = 1; I <= n; ++I)
+      for (int J = I + 1; J <= n; ++J) {
+        const int IJ = fuse_index(I, J);
+        std::fill(wa.begin(), wa.end(), 0.0);
+        for (int A = 1; A <= n; ++A) {
+          if (A == I || A == J) continue;
+          std::fill(wb.begin(), wb.end(), 0.0);
+          for (int B = 1; B <= n; ++B) {
+            if (B == I || B == J || B == A) continue;
+            wb[B - 1] = weight(I, J, A, B);
+          }
+          const size_t row = (size_t)(IJ - 1) * n + (size_t)(A - 1);
+          b_ok[row] = alias_row(wb, &b_tab[row * n], &b_prob[row * n]) ? 1 : 0;
+          double sw = 0.0;
+          for (double x : wb) sw += x;
+          wa[A - 1] = sw;
+        }
+        a_ok[IJ - 1] = alias_row(wa, &a_tab[(size_t)(IJ - 1) * n], &a_prob[(size_t)(IJ - 1) * n]) ? 1 : 0;
+        double sw = 0.0;
+        for (double x : wa) sw += x;
+        ijw[(size_t)(I - 1) + (size_t)(J - 1) * n] = sw;
+        ijw[(size_t)(J - 1) + (size_t)(I - 1) * n] = sw;
+      }
+    if (int rc = upload_table(&h->d_a_tab, a_tab)) return rc;
+    if (int rc = upload_table(&h->d_a_prob, a_prob)) return rc;
+    if (int rc = upload_table(&h->d_a_ok, a_ok)) return rc;
+    if (int rc = upload_table(&h->d_b_tab, b_tab)) return rc;
+    if (int rc = upload_table(&h->d_b_prob, b_prob)) return rc;
+    if (int rc = upload_table(&h->d_b_ok, b_ok)) return rc;
+  }
+  // particle samplers (init_PC_WeightedParticles_t, gasci_pchb_doubles_select_particles.fpp:254-301):
+  // J rows weights(:, I), the I row sum(weights, dim=1)
+  std::vector<AliasEnt> j_tab((size_t)n * n), i_tab(n);
+  std::vector<double> j_prob((size_t)n * n, 0.0), i_prob(n, 0.0), wi(n);
   std::vector<uint8_t> j_ok(n, 0);
   for (int I = 1; I <= n; ++I) {
     std::vector<double> row(ijw.begin() + (size_t)(I - 1) * n, ijw.begin() + (size_t)I * n);
+    j_ok[I - 1] = alias_row(row, &j_tab[(size_t)(I - 1) * n], &j_prob[(size_t)(I - 1) * n]) ? 1 : 0;
     double sw = 0.0;
     for (double x : row) sw += x;
-    if (!(fabs(sw) <= kEps)) {
-      build_alias(row, &j_tab[(size_t)(I - 1) * n], &j_prob[(size_t)(I - 1) * n]);
-      j_ok[I - 1] = 1;
-    }
+    wi[I - 1] = sw;
   }
-  std::vector<int32_t> tgt(2 * (size_t)ijmax);
-  for (int A = 1; A <= n; ++A)
-    for (int B = 1; B <= A; ++B) {
-      int AB = fuse_index(A, B);
-      tgt[2 * (AB - 1)] = B;
-      tgt[2 * (AB - 1) + 1] = A;
-    }
-  void* olds[] = {h->d_ab_tab, h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt};
-  for (void* p : olds)
-    if (p) (void)hipFree(p);
-  HIP_OK(dalloc(&h->d_ab_tab, ab_tab.size()));
-  HIP_OK(dalloc(&h->d_ab_prob, ab_prob.size()));
-  HIP_OK(dalloc(&h->d_ab_ok, ab_ok.size()));
-  HIP_OK(dalloc(&h->d_j_tab, j_tab.size()));
-  HIP_OK(dalloc(&h->d_j_prob, j_prob.size()));
-  HIP_OK(dalloc(&h->d_j_ok, j_ok.size()));
-  HIP_OK(dalloc(&h->d_ab_tgt, tgt.size()));
-  HIP_OK(hipMemcpy(h->d_ab_tab, ab_tab.data(), ab_tab.size() * sizeof(AliasEnt), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(h->d_ab_prob, ab_prob.data(), ab_prob.size() * 8, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(h->d_ab_ok, ab_ok.data(), ab_ok.size(), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(h->d_j_tab, j_tab.data(), j_tab.size() * sizeof(AliasEnt), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(h->d_j_prob, j_prob.data(), j_prob.size() * 8, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(h->d_j_ok, j_ok.data(), j_ok.size(), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(h->d_ab_tgt, tgt.data(), tgt.size() * 4, hipMemcpyHostToDevice));
-  h->ds.ab_tab = h->d_ab_tab;
-  h->ds.ab_prob = h->d_ab_prob;
-  h->ds.ab_ok = h->d_ab_ok;
+  const bool i_ok = alias_row(wi, i_tab.data(), i_prob.data());
+  if (int rc = upload_table(&h->d_j_tab, j_tab)) return rc;
+  if (int rc = upload_table(&h->d_j_prob, j_prob)) return rc;
+  if (int rc = upload_table(&h->d_j_ok, j_ok)) return rc;
+  if (int rc = upload_table(&h->d_i_tab, i_tab)) return rc;
+  if (int rc = upload_table(&h->d_i_prob, i_prob)) return rc;
+  h->ds.ab_tab = holes == NGPU_HOLES_FAST_FAST ? h->d_ab_tab : nullptr;
+  h->ds.ab_prob = holes == NGPU_HOLES_FAST_FAST ? h->d_ab_prob : nullptr;
+  h->ds.ab_ok = holes == NGPU_HOLES_FAST_FAST ? h->d_ab_ok : nullptr;
+  h->ds.ab_tgt = holes == NGPU_HOLES_FAST_FAST ? h->d_ab_tgt : nullptr;
+  h->ds.a_tab = holes == NGPU_HOLES_FULL_FULL ? h->d_a_tab : nullptr;
+  h->ds.a_prob = holes == NGPU_HOLES_FULL_FULL ? h->d_a_prob : nullptr;
+  h->ds.a_ok = holes == NGPU_HOLES_FULL_FULL ? h->d_a_ok : nullptr;
+  h->ds.b_tab = holes == NGPU_HOLES_FULL_FULL ? h->d_b_tab : nullptr;
+  h->ds.b_prob = holes == NGPU_HOLES_FULL_FULL ? h->d_b_prob : nullptr;
+  h->ds.b_ok = holes == NGPU_HOLES_FULL_FULL ? h->d_b_ok : nullptr;
   h->ds.j_tab = h->d_j_tab;
   h->ds.j_prob = h->d_j_prob;
   h->ds.j_ok = h->d_j_ok;
-  h->ds.ab_tgt = h->d_ab_tgt;
+  h->ds.i_tab = h->d_i_tab;
+  h->ds.i_prob = h->d_i_prob;
+  h->ds.i_ok = i_ok ? 1 : 0;
+  h->ds.holes = holes;
   h->ds.ijmax = ijmax;
   h->ds.singles = singles;
   h->ds.particles = particles;
@@ -1076,6 +1216,10 @@ int ngpu_set_excitgen(ngpu_t* h, int32_t singles, int32_t particles, double psin
   h->ds.pparallel = pparallel;
   h->have_excitgen = true;
   return NGPU_OK;
+}
+
+int ngpu_set_excitgen(ngpu_t* h, int32_t singles, int32_t particles, double psingles, double pparallel) {
+  return ngpu_set_excitgen_pchb(h, singles, particles, NGPU_HOLES_FAST_FAST, psingles, pparallel);
 }
 
 int ngpu_set_excitgen_uniform(ngpu_t* h, double psingles) {
@@ -1141,6 +1285,7 @@ int ngpu_upload_walkers(ngpu_t* h, const int64_t* rows, int64_t n) {
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_new, (unsigned long long)n);
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, h->stream, &h->cnt->err, 0u);
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_fail, 0ull);
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, h->stream, &h->cnt->ins_refused, 0u);
   {
     auto kern = h->W == 1 ? (h->S == 1 ? k_insert<1, 1> : k_insert<1, kMaxRep>)
                           : (h->S == 1 ? k_insert<2, 1> : k_insert<2, kMaxRep>);
@@ -1162,11 +1307,215 @@ int ngpu_upload_walkers(ngpu_t* h, const int64_t* rows, int64_t n) {
 
 extern "C++" {
 namespace {
-template <int W, int R>
-int set_core_impl(ngpu_t* h, int64_t ncore, bool fp32, int64_t* nnz_out) {
-  const int S = h->S;
+// ascending (key, index) radix sort of n elements with the spawn sort's
+// kernels: (word, shift) passes, the first reading kin (stride kstride) and
+// iin (nullptr: the identity), ping-ponging through kb[2], ib[2]; returns the
+// buffer pair holding the result
+template <typename K, int NW, bool FLIP, int IT>
+int radix_sort_pairs_in(ngpu_t* h, const K* kin, int kstride, const uint32_t* iin,
+                        const std::vector<std::pair<int, int>>& passes, const unsigned long long* n_ptr, int64_t n,
+                        K* const* kb, uint32_t* const* ib) {
+  int cur = -1;
+  for (size_t p = 0; p < passes.size(); ++p) {
+    const int out = (int)(p & 1);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_radix_hist<K, NW, FLIP, IT, 8>), dim3(h->nb_chunk), dim3(kBlock), 0,
+                       h->stream, kin, kstride, n_ptr, n, passes[p].first, passes[p].second, h->hist);
+    hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(kBlock), 0, h->stream, h->hist, h->nb_chunk, h->digit_tot);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_radix_scatter<K, NW, FLIP, IT, 8>), dim3(h->nb_chunk), dim3(kBlock), 0,
+                       h->stream, kin, kstride, iin, kb[out], ib[out], n_ptr, n, passes[p].first, passes[p].second,
+                       (const uint32_t*)h->hist, (const uint32_t*)h->digit_tot);
+    kin = kb[out];
+    kstride = NW;
+    iin = ib[out];
+    cur = out;
+  }
+  return cur;
+}
+// the same in the engine's spawn-sort buffers (n <= the spawn capacity)
+template <typename K, int NW, bool FLIP, int IT>
+int radix_sort_pairs(ngpu_t* h, const K* kin, int kstride, const uint32_t* iin,
+                     const std::vector<std::pair<int, int>>& passes, const unsigned long long* n_ptr, int64_t n) {
+  K* kb[2] = {(K*)h->sk[0], (K*)h->sk[1]};
+  return radix_sort_pairs_in<K, NW, FLIP, IT>(h, kin, kstride, iin, passes, n_ptr, n, kb, h->si);
+}
+
+// ---- the core Hamiltonian from its connections ----
+// Two determinants with excitation level <= 2 share nel - 2 electrons.  Each
+// core determinant is listed once under every (nel - 2)-electron subset of
+// its electrons; sorting the list by subset groups the connected pairs, and
+// a pair (i, j) of a group is kept in the one group whose subset is the
+// canonical one for the pair: their common electrons less the lowest ones, so
+// that each connection is found exactly once (level 2: the common electrons
+// themselves; level 1: less the lowest).  The pairs and the diagonal are
+// sorted by (row, column) and H_ij is evaluated per pair, as the all-pairs
+// scan does (off-diagonal zeros dropped), so both builds give the same CSR.
+// The role of calc_determ_hamil_opt (fast_determ_hamil.F90:892-1548), which
+// joins the core determinants through their alpha and beta strings.
+
+template <int W>
+__global__ void k_cs_subsets(const uint64_t* keys, int64_t ncore, int npair, uint64_t* sub, uint32_t* idx) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ncore; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t d[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) d[w] = keys[i * W + w];
+    int64_t o = i * npair;
+    // electron pairs (a < b) in ascending order of a, then b
+#pragma unroll
+    for (int wa = 0; wa < W; ++wa)
+      for (uint64_t xa = d[wa]; xa; xa &= xa - 1) {
+        const int ba = ctz64(xa);
+        uint64_t rest = xa & (xa - 1);
+#pragma unroll
+        for (int wb = 0; wb < W; ++wb) {
+          if (wb < wa) continue;
+          for (uint64_t xb = (wb == wa ? rest : d[wb]); xb; xb &= xb - 1) {
+            const int bb = ctz64(xb);
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+              uint64_t v = d[w];
+              if (w == wa) v &= ~(1ull << ba);
+              if (w == wb) v &= ~(1ull << bb);
+              sub[o * W + w] = v;
+            }
+            idx[o] = (uint32_t)i;
+            ++o;
+          }
+        }
+      }
+  }
+}
+
+// group starts of the sorted subset list: flag 1 where the subset changes
+template <int W>
+__global__ void k_cs_starts(const uint64_t* sub, int64_t m, uint32_t* start) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < m; t += (int64_t)gridDim.x * blockDim.x) {
+    bool s = t == 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) s = s || sub[t * W + w] != sub[(t - 1 > 0 ? t - 1 : 0) * W + w];
+    start[t] = s ? 1u : 0u;
+  }
+}
+
+// one wave per group: the canonical connected pairs (i, j), i != j, as
+// (i << 32 | j); pass 0 counts them, pass 1 writes them (wave-aggregated
+// appends)
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_cs_pairs(const uint64_t* sub, const uint32_t* idx, int64_t m,
+                                                     const int64_t* gstart, int64_t ngroups, const uint64_t* keys,
+                                                     int nel, int pass, unsigned long long* total, uint64_t* out) {
+  const int lane = lane_id();
+  const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; g < ngroups; g += nwaves) {
+    const int64_t g0 = gstart[g], g1 = g + 1 < ngroups ? gstart[g + 1] : m;
+    const int64_t n = g1 - g0;
+    const uint64_t* S = sub + g0 * W;
+    for (int64_t base = 0; base < n * n; base += kWave) {
+      const int64_t q = base + lane;
+      bool emit = false;
+      uint64_t rec = 0;
+      if (q < n * n) {
+        const int64_t a = q / n, b = q - a * n;
+        if (a != b) {
+          const uint32_t i = idx[g0 + a], j = idx[g0 + b];
+          uint64_t I[W];
+          int c = 0;
+#pragma unroll
+          for (int w = 0; w < W; ++w) {
+            I[w] = keys[(int64_t)i * W + w] & keys[(int64_t)j * W + w];
+            c += popc64(I[w]);
+          }
+          // the canonical subset: the common electrons less their lowest c - (nel - 2)
+          for (int drop = c - (nel - 2); drop > 0; --drop) {
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+              if (I[w]) {
+                I[w] &= I[w] - 1;
+                break;
+              }
+          }
+          bool same = c <= nel - 1;  // c == nel would be a repeated determinant
+#pragma unroll
+          for (int w = 0; w < W; ++w) same = same && I[w] == S[w];
+          emit = same;
+          rec = ((uint64_t)i << 32) | (uint64_t)j;
+        }
+      }
+      const uint64_t mask = __ballot(emit);
+      if (!mask) continue;
+      const int leader = __ffsll((unsigned long long)mask) - 1;
+      unsigned long long pos = 0;
+      if (lane == leader) pos = atomicAdd(total, (unsigned long long)popc64(mask));
+      pos = __shfl(pos, leader);
+      if (pass && emit) out[pos + popc64(mask & lanemask_lt())] = rec;
+    }
+  }
+}
+
+// the diagonal (i, i) of every row
+__global__ void k_cs_diag(int64_t ncore, uint64_t* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ncore; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = ((uint64_t)i << 32) | (uint64_t)i;
+}
+
+// H_ij of each sorted (row, column) pair, kept when on the diagonal or
+// nonzero (the all-pairs scan's rule)
+template <int W>
+__global__ void k_cs_eval(DevSys s, const uint64_t* keys, const uint64_t* pairs, int64_t np, double* hv,
+                          uint32_t* kp) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < np; t += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t pr = pairs[t];
+    const int64_t i = (int64_t)(pr >> 32), j = (int64_t)(pr & 0xFFFFFFFFull);
+    const uint64_t* di = keys + i * W;
+    const double h = (i == j) ? diag_helement<W>(s, di) - s.hii : helement_pair<W>(s, di, keys + j * W);
+    hv[t] = h;
+    kp[t] = ((i == j) || h != 0.0) ? 1u : 0u;
+  }
+}
+
+constexpr int64_t kCsTile = 4096;  // pairs per block of the CSR compaction
+
+__global__ void __launch_bounds__(kBlock) k_cs_tilecount(const uint32_t* kp, int64_t np, int64_t* counts) {
+  __shared__ int64_t lds[kBlock / kWave];
+  const int64_t b0 = (int64_t)blockIdx.x * kCsTile, b1 = std::min<int64_t>(b0 + kCsTile, np);
+  int64_t c = 0;
+  for (int64_t t = b0 + threadIdx.x; t < b1; t += kBlock) c += kp[t];
+  int64_t tot = 0;
+  (void)block_exclusive_scan(c, lds, tot);
+  if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+}
+
+// kept pairs to their CSR position (the tile's offset plus the kept pairs
+// before them in the tile); the first pair of each row sets its rowptr
+template <typename VT>
+__global__ void __launch_bounds__(kBlock) k_cs_write(const uint64_t* pairs, const double* hv, const uint32_t* kp,
+                                                     int64_t np, const int64_t* offsets, int64_t* rowptr,
+                                                     int32_t* cols, VT* vals) {
+  __shared__ int64_t lds[kBlock / kWave];
+  const int64_t b0 = (int64_t)blockIdx.x * kCsTile;
+  int64_t pos0 = offsets[blockIdx.x];
+  for (int64_t t0 = 0; t0 < kCsTile && b0 + t0 < np; t0 += kBlock) {
+    const int64_t t = b0 + t0 + threadIdx.x;
+    const bool keep = t < np && kp[t];
+    int64_t tot = 0;
+    const int64_t pos = pos0 + block_exclusive_scan(keep ? 1 : 0, lds, tot);
+    if (t < np) {
+      const uint64_t pr = pairs[t];
+      const int64_t i = (int64_t)(pr >> 32);
+      if (t == 0 || (int64_t)(pairs[t - 1] >> 32) != i) rowptr[i] = pos;  // a row starts with its first pair
+      if (keep) {
+        cols[pos] = (int32_t)(pr & 0xFFFFFFFFull);
+        vals[pos] = (VT)hv[t];
+      }
+    }
+    pos0 += tot;
+  }
+}
+
+// the core Hamiltonian by the all-pairs scan (every (i, j) tested): count,
+// scan on the host, fill
+template <int W>
+int core_rows_scan(ngpu_t* h, int64_t ncore, bool fp32, int64_t& nnz) {
   auto grid_rows = [&](int64_t n) { return dim3((unsigned)std::min<int64_t>((n + 3) / 4, 8 * (int64_t)h->nb_grid)); };
-  // core Hamiltonian: count, scan on the host, fill
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_core_rows<W, double>), grid_rows(ncore), dim3(kBlock), 0, h->stream, h->ds,
                      (const uint64_t*)h->core_keys, ncore, 0, h->core_rowptr + 1, (const int64_t*)nullptr,
                      (int32_t*)nullptr, (double*)nullptr);
@@ -1174,7 +1523,7 @@ int set_core_impl(ngpu_t* h, int64_t ncore, bool fp32, int64_t* nnz_out) {
   HIP_OK(hipMemcpyAsync(rp.data() + 1, h->core_rowptr + 1, (size_t)ncore * 8, hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
   for (int64_t i = 0; i < ncore; ++i) rp[i + 1] += rp[i];
-  const int64_t nnz = rp[ncore];
+  nnz = rp[ncore];
   HIP_OK(hipMemcpy(h->core_rowptr, rp.data(), rp.size() * 8, hipMemcpyHostToDevice));
   HIP_OK(dalloc(&h->core_cols, (size_t)nnz));
   if (fp32) {
@@ -1192,10 +1541,146 @@ int set_core_impl(ngpu_t* h, int64_t ncore, bool fp32, int64_t* nnz_out) {
                        (const uint64_t*)h->core_keys, ncore, 1, (int64_t*)nullptr, (const int64_t*)h->core_rowptr,
                        h->core_cols, v);
   }
+  return NGPU_OK;
+}
+
+// the core Hamiltonian from its connections (the subset join above)
+template <int W>
+int core_rows_join(ngpu_t* h, int64_t ncore, bool fp32, int64_t& nnz) {
+  const int nel = h->sys.nel;
+  const int64_t npair = (int64_t)nel * (nel - 1) / 2;
+  const int64_t m = ncore * npair;
+  if (nel < 2 || m > (int64_t)UINT32_MAX || ncore > (int64_t)INT32_MAX) return NGPU_ERR_ARG;
+  DevBuf<uint64_t> sk0, sk1;
+  DevBuf<uint32_t> si0, si1, start;
+  DevBuf<unsigned long long> cnt;
+  HIP_OK(sk0.alloc((size_t)m * W));
+  HIP_OK(sk1.alloc((size_t)m * W));
+  HIP_OK(si0.alloc((size_t)m));
+  HIP_OK(si1.alloc((size_t)m));
+  HIP_OK(start.alloc((size_t)m));
+  HIP_OK(cnt.alloc(2));
+  // every core determinant under each of its (nel - 2)-electron subsets, sorted by subset
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cs_subsets<W>), dim3(grid_for(ncore)), dim3(kBlock), 0, h->stream,
+                     (const uint64_t*)h->core_keys, ncore, (int)npair, sk0.p, si0.p);
+  const unsigned long long mm = (unsigned long long)m;
+  HIP_OK(hipMemcpyAsync(cnt.p, &mm, 8, hipMemcpyHostToDevice, h->stream));
+  uint64_t* kb[2] = {sk1.p, sk0.p};
+  uint32_t* ib[2] = {si1.p, si0.p};
+  // the scatter of pass p writes kb[p & 1]; the first pass reads sk0/si0, so
+  // the ping-pong starts in the other pair
+  const int cur = radix_sort_pairs_in<uint64_t, W, W == 2, 4>(h, sk0.p, W, si0.p, word_passes(h), cnt.p, m, kb, ib);
+  const uint64_t* ssub = kb[cur];
+  const uint32_t* sidx = ib[cur];
+  // group starts
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cs_starts<W>), dim3(grid_for(m)), dim3(kBlock), 0, h->stream, ssub, m,
+                     start.p);
+  std::vector<uint32_t> hs((size_t)m);
+  HIP_OK(hipMemcpyAsync(hs.data(), start.p, (size_t)m * 4, hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  std::vector<int64_t> gs;
+  for (int64_t t = 0; t < m; ++t)
+    if (hs[t]) gs.push_back(t);
+  const int64_t ng = (int64_t)gs.size();
+  DevBuf<int64_t> gstart;
+  HIP_OK(gstart.alloc((size_t)std::max<int64_t>(ng, 1)));
+  HIP_OK(hipMemcpy(gstart.p, gs.data(), (size_t)ng * 8, hipMemcpyHostToDevice));
+  // the canonical connected pairs: count, then write after the ncore diagonal entries
+  const unsigned long long zero = 0;
+  HIP_OK(hipMemcpyAsync(cnt.p, &zero, 8, hipMemcpyHostToDevice, h->stream));
+  const dim3 gw((unsigned)std::max<int64_t>(1, std::min<int64_t>((ng + 3) / 4, 16 * (int64_t)h->nb_grid)));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cs_pairs<W>), gw, dim3(kBlock), 0, h->stream, ssub, sidx, m,
+                     (const int64_t*)gstart.p, ng, (const uint64_t*)h->core_keys, nel, 0, cnt.p, (uint64_t*)nullptr);
+  unsigned long long npairs = 0;
+  HIP_OK(hipMemcpyAsync(&npairs, cnt.p, 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  const int64_t np = (int64_t)npairs + ncore;
+  DevBuf<uint64_t> pr0, pr1;
+  DevBuf<uint32_t> pi0, pi1;
+  HIP_OK(pr0.alloc((size_t)np));
+  HIP_OK(pr1.alloc((size_t)np));
+  HIP_OK(pi0.alloc((size_t)np));
+  HIP_OK(pi1.alloc((size_t)np));
+  hipLaunchKernelGGL(k_cs_diag, dim3(grid_for(ncore)), dim3(kBlock), 0, h->stream, ncore, pr0.p);
+  HIP_OK(hipMemcpyAsync(cnt.p, &zero, 8, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cs_pairs<W>), gw, dim3(kBlock), 0, h->stream, ssub, sidx, m,
+                     (const int64_t*)gstart.p, ng, (const uint64_t*)h->core_keys, nel, 1, cnt.p, pr0.p + ncore);
+  // sort by (row, column): the column bits, then the row bits
+  int cb = 1;
+  while (cb < 32 && ((int64_t)1 << cb) < ncore) ++cb;
+  std::vector<std::pair<int, int>> passes;
+  for (int sh = 0; sh < cb; sh += 8) passes.push_back({0, sh});
+  for (int sh = 32; sh < 32 + cb; sh += 8) passes.push_back({0, sh});
+  const unsigned long long npp = (unsigned long long)np;
+  HIP_OK(hipMemcpyAsync(cnt.p + 1, &npp, 8, hipMemcpyHostToDevice, h->stream));
+  uint64_t* pk[2] = {pr1.p, pr0.p};
+  uint32_t* pik[2] = {pi1.p, pi0.p};
+  const int pc = radix_sort_pairs_in<uint64_t, 1, false, 4>(h, pr0.p, 1, nullptr, passes, cnt.p + 1, np, pk, pik);
+  const uint64_t* pairs = pk[pc];
+  // values, the kept entries' positions, CSR
+  DevBuf<double> hv;
+  DevBuf<uint32_t> kp;
+  HIP_OK(hv.alloc((size_t)np));
+  HIP_OK(kp.alloc((size_t)np));
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cs_eval<W>), dim3(grid_for(np)), dim3(kBlock), 0, h->stream, h->ds,
+                     (const uint64_t*)h->core_keys, pairs, np, hv.p, kp.p);
+  const int64_t ntiles = (np + kCsTile - 1) / kCsTile;
+  DevBuf<int64_t> toff;
+  HIP_OK(toff.alloc((size_t)ntiles + 1));
+  hipLaunchKernelGGL(k_cs_tilecount, dim3((unsigned)ntiles), dim3(kBlock), 0, h->stream, (const uint32_t*)kp.p, np,
+                     toff.p);
+  std::vector<int64_t> ht((size_t)ntiles);
+  HIP_OK(hipMemcpyAsync(ht.data(), toff.p, (size_t)ntiles * 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  int64_t run = 0;
+  for (int64_t t = 0; t < ntiles; ++t) {
+    const int64_t c = ht[t];
+    ht[t] = run;
+    run += c;
+  }
+  nnz = run;
+  HIP_OK(hipMemcpy(toff.p, ht.data(), (size_t)ntiles * 8, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(h->core_rowptr + ncore, &nnz, 8, hipMemcpyHostToDevice));
+  HIP_OK(dalloc(&h->core_cols, (size_t)nnz));
+  if (fp32) {
+    float* v = nullptr;
+    HIP_OK(dalloc(&v, (size_t)nnz));
+    h->core_vals = v;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cs_write<float>), dim3((unsigned)ntiles), dim3(kBlock), 0, h->stream, pairs,
+                       (const double*)hv.p, (const uint32_t*)kp.p, np, (const int64_t*)toff.p, h->core_rowptr,
+                       h->core_cols, v);
+  } else {
+    double* v = nullptr;
+    HIP_OK(dalloc(&v, (size_t)nnz));
+    h->core_vals = v;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cs_write<double>), dim3((unsigned)ntiles), dim3(kBlock), 0, h->stream,
+                       pairs, (const double*)hv.p, (const uint32_t*)kp.p, np, (const int64_t*)toff.p, h->core_rowptr,
+                       h->core_cols, v);
+  }
+  HIP_OK(hipStreamSynchronize(h->stream));  // the temporary buffers are freed on return
+  return NGPU_OK;
+}
+
+template <int W, int R>
+int set_core_impl(ngpu_t* h, int64_t ncore, bool fp32, int64_t* nnz_out) {
+  const int S = h->S;
+  int64_t nnz = 0;
+  // fewer than 4 electrons: subsets of nel - 2 <= 1 electrons put most of the
+  // core in one group, so the scan is used.  Automatic choice: the scan costs
+  // about ncore^2 pair tests, the join about ncore * nel(nel-1)/2 sorted
+  // entries; measured at 26 electrons, 40 000 determinants, the scan took
+  // 33 ms and the join 123 ms, so the join is taken from ncore >= 450 pairs
+  // of electrons per determinant on.
+  const int64_t npair = (int64_t)h->sys.nel * (h->sys.nel - 1) / 2;
+  const bool scan = h->core_build == NGPU_CORE_BUILD_SCAN || h->sys.nel < 4 ||
+                    (h->core_build == NGPU_CORE_BUILD_AUTO && ncore < 450 * npair);
+  int rc = scan ? core_rows_scan<W>(h, ncore, fp32, nnz) : core_rows_join<W>(h, ncore, fp32, nnz);
+  if (rc) return rc;
   // core dets resident, flagged deterministic (and initiator: t_core_inits)
   uint32_t cflags = kFlagCore | (h->sys.t_trunc_initiator ? ((1u << S) - 1u) : 0u);
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_new, 0ull);
   hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, &h->cnt->n_fail, 0ull);
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, h->stream, &h->cnt->ins_refused, 0u);
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, h->stream, &h->cnt->err, 0u);
   dim3 gp((unsigned)std::min<int64_t>((ncore + kBlock - 1) / kBlock, h->nb_grid));
   hipLaunchKernelGGL(HIP_KERNEL_NAME(k_core_place<W>), gp, dim3(kBlock), 0, h->stream, h->ds, h->wk, h->cnt,
@@ -1222,6 +1707,22 @@ int set_core_impl(ngpu_t* h, int64_t ncore, bool fp32, int64_t* nnz_out) {
   return NGPU_OK;
 }
 
+// values of the stored CSR entries, recomputed from the integrals: one wave
+// per row
+template <int W, typename VT>
+__global__ void __launch_bounds__(kBlock) k_core_revalue(DevSys s, const uint64_t* keys, int64_t ncore,
+                                                         const int64_t* rowptr, const int32_t* cols, VT* vals) {
+  const int lane = lane_id();
+  const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < ncore; i += nwaves) {
+    const uint64_t* di = keys + i * W;
+    for (int64_t e = rowptr[i] + lane; e < rowptr[i + 1]; e += kWave) {
+      const int64_t j = cols[e];
+      vals[e] = (VT)((i == j) ? diag_helement<W>(s, di) - s.hii : helement_pair<W>(s, di, keys + j * W));
+    }
+  }
+}
+
 // the resident core's Hamiltonian values stored again in the other precision
 // (same rows and columns, each value recomputed from the integrals)
 template <int W>
@@ -1235,16 +1736,16 @@ int restore_core_values(ngpu_t* h, bool fp32) {
     float* v = nullptr;
     HIP_OK(dalloc(&v, (size_t)h->core_nnz));
     h->core_vals = v;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_core_rows<W, float>), grid, dim3(kBlock), 0, h->stream, h->ds,
-                       (const uint64_t*)h->core_keys, ncore, 1, (int64_t*)nullptr, (const int64_t*)h->core_rowptr,
-                       h->core_cols, v);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_core_revalue<W, float>), grid, dim3(kBlock), 0, h->stream, h->ds,
+                       (const uint64_t*)h->core_keys, ncore, (const int64_t*)h->core_rowptr,
+                       (const int32_t*)h->core_cols, v);
   } else {
     double* v = nullptr;
     HIP_OK(dalloc(&v, (size_t)h->core_nnz));
     h->core_vals = v;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_core_rows<W, double>), grid, dim3(kBlock), 0, h->stream, h->ds,
-                       (const uint64_t*)h->core_keys, ncore, 1, (int64_t*)nullptr, (const int64_t*)h->core_rowptr,
-                       h->core_cols, v);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_core_revalue<W, double>), grid, dim3(kBlock), 0, h->stream, h->ds,
+                       (const uint64_t*)h->core_keys, ncore, (const int64_t*)h->core_rowptr,
+                       (const int32_t*)h->core_cols, v);
   }
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(h->stream));
@@ -1253,6 +1754,155 @@ int restore_core_values(ngpu_t* h, bool fp32) {
 }
 }  // namespace
 }  // extern "C++"
+
+extern "C++" {
+namespace {
+
+// generate_space_most_populated's weight of a slot (core_space_weight,
+// semi_stoch_procs.F90:1623-1642, the global core space: sum_r |c_r|), or -1
+// for a slot out of use or below 1e-8 (semi_stoch_gen.F90:1094-1100)
+__device__ inline double mp_weight(const Walk& wk, int64_t slot, int S) {
+  if (wk.flags(slot) & kFlagRemoved) return -1.0;
+  double w = 0.0;
+  for (int r = 0; r < S; ++r) w += fabs(wk.sgn(slot, r));
+  return w < 1e-8 ? -1.0 : w;
+}
+
+constexpr int64_t kMpTile = 4096;  // slots per block of the candidate compaction
+
+__global__ void __launch_bounds__(kBlock) k_mp_count(Walk wk, const Counters* cnt, int S, int64_t* counts) {
+  __shared__ int64_t lds[kBlock / kWave];
+  const int64_t ns = (int64_t)cnt->nslots, b0 = (int64_t)blockIdx.x * kMpTile;
+  const int64_t b1 = std::min<int64_t>(b0 + kMpTile, ns);
+  int64_t c = 0;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += kBlock) c += mp_weight(wk, i, S) >= 0.0 ? 1 : 0;
+  int64_t tot = 0;
+  (void)block_exclusive_scan(c, lds, tot);
+  if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+}
+
+// candidates in slot order: keys (W words), index 0..n-1 and the weight
+// complemented (~bits), whose ascending order is the weights' descending order
+__global__ void __launch_bounds__(kBlock) k_mp_write(Walk wk, const Counters* cnt, int W, int S,
+                                                     const int64_t* offsets, uint64_t* keys, uint64_t* wkey) {
+  __shared__ int64_t lds[kBlock / kWave];
+  const int64_t ns = (int64_t)cnt->nslots, b0 = (int64_t)blockIdx.x * kMpTile;
+  int64_t pos0 = offsets[blockIdx.x];
+  for (int64_t t = 0; t < kMpTile && b0 + t < ns; t += kBlock) {
+    const int64_t i = b0 + t + threadIdx.x;
+    const double w = i < ns ? mp_weight(wk, i, S) : -1.0;
+    int64_t tot = 0;
+    const int64_t pos = pos0 + block_exclusive_scan(w >= 0.0 ? 1 : 0, lds, tot);
+    if (w >= 0.0) {
+      for (int q = 0; q < W; ++q) keys[pos * W + q] = wk.keys[i * W + q];
+      wkey[pos] = ~(uint64_t)__double_as_longlong(w);
+    }
+    pos0 += tot;
+  }
+}
+
+// the weights' keys in the order of a permutation
+__global__ void k_mp_gather(const uint64_t* wkey, const uint32_t* perm, int64_t n, uint64_t* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = wkey[perm[i]];
+}
+
+
+// generate_space_most_populated on the resident walkers: the candidates in
+// slot order, stably sorted by determinant (ilut_lt), then stably by weight
+// descending; the first target are the core space (weight descending, equal
+// weights in determinant order)
+template <int W>
+int most_populated_impl(ngpu_t* h, int64_t target, std::vector<uint64_t>& out) {
+  if (h->pending) {
+    int rc = finish(h, nullptr);
+    if (rc) return rc;
+  }
+  HIP_OK(hipMemcpy(h->hcnt, h->cnt, sizeof(Counters), hipMemcpyDeviceToHost));
+  const int64_t ns = (int64_t)h->hcnt->nslots;
+  const int64_t ntiles = std::max<int64_t>(1, (ns + kMpTile - 1) / kMpTile);
+  DevBuf<int64_t> counts;
+  DevBuf<uint64_t> ckeys, wkey, wperm;
+  DevBuf<unsigned long long> n_dev;
+  HIP_OK(counts.alloc((size_t)ntiles + 1));
+  HIP_OK(ckeys.alloc((size_t)std::max<int64_t>(ns, 1) * W));
+  HIP_OK(wkey.alloc((size_t)std::max<int64_t>(ns, 1)));
+  HIP_OK(wperm.alloc((size_t)std::max<int64_t>(ns, 1)));
+  HIP_OK(n_dev.alloc(1));
+  hipLaunchKernelGGL(k_mp_count, dim3((unsigned)ntiles), dim3(kBlock), 0, h->stream, h->wk, (const Counters*)h->cnt,
+                     h->S, counts.p);
+  std::vector<int64_t> hc((size_t)ntiles + 1, 0);
+  HIP_OK(hipMemcpyAsync(hc.data(), counts.p, (size_t)ntiles * 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  int64_t n = 0;
+  for (int64_t t = 0; t < ntiles; ++t) {
+    const int64_t c = hc[t];
+    hc[t] = n;
+    n += c;
+  }
+  out.clear();
+  if (n == 0) return NGPU_OK;
+  if (n > h->scap) return NGPU_ERR_SPAWN_FULL;  // the sort runs in the spawn-sort buffers
+  HIP_OK(hipMemcpy(counts.p, hc.data(), (size_t)ntiles * 8, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_mp_write, dim3((unsigned)ntiles), dim3(kBlock), 0, h->stream, h->wk, (const Counters*)h->cnt,
+                     W, h->S, (const int64_t*)counts.p, ckeys.p, wkey.p);
+  const unsigned long long nn = (unsigned long long)n;
+  HIP_OK(hipMemcpyAsync(n_dev.p, &nn, 8, hipMemcpyHostToDevice, h->stream));
+  // determinant order first (the tie order), then weight (stable)
+  int cur = radix_sort_pairs<uint64_t, W, W == 2, 4>(h, ckeys.p, W, nullptr, word_passes(h), n_dev.p, n);
+  hipLaunchKernelGGL(k_mp_gather, dim3(grid_for(n)), dim3(kBlock), 0, h->stream, (const uint64_t*)wkey.p,
+                     (const uint32_t*)h->si[cur], n, wperm.p);
+  std::vector<std::pair<int, int>> wp;
+  for (int sh = 0; sh < 64; sh += 8) wp.push_back({0, sh});
+  const uint32_t* perm = h->si[cur];
+  // the weight passes must not overwrite the permutation they read first
+  if (cur == 0) {
+    HIP_OK(hipMemcpyAsync(h->si[1], h->si[0], (size_t)n * 4, hipMemcpyDeviceToDevice, h->stream));
+    perm = h->si[1];
+  }
+  cur = radix_sort_pairs<uint64_t, 1, false, 4>(h, wperm.p, 1, perm, wp, n_dev.p, n);
+  const int64_t k = std::min(target, n);
+  std::vector<uint32_t> idx((size_t)k);
+  HIP_OK(hipMemcpyAsync(idx.data(), h->si[cur], (size_t)k * 4, hipMemcpyDeviceToHost, h->stream));
+  std::vector<uint64_t> all((size_t)n * W);
+  HIP_OK(hipMemcpyAsync(all.data(), ckeys.p, all.size() * 8, hipMemcpyDeviceToHost, h->stream));
+  HIP_OK(hipStreamSynchronize(h->stream));
+  out.resize((size_t)k * W);
+  for (int64_t i = 0; i < k; ++i)
+    for (int q = 0; q < W; ++q) out[(size_t)i * W + q] = all[(size_t)idx[i] * W + q];
+  return NGPU_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int ngpu_most_populated(ngpu_t* h, int64_t target, int64_t* dets, int64_t* n) {
+  int rc = ready(h);
+  if (rc) return rc;
+  if (target < 0 || !n || (target > 0 && !dets)) return NGPU_ERR_ARG;
+  if (h->batch) return NGPU_ERR_STATE;
+  std::vector<uint64_t> out;
+  rc = h->W == 1 ? most_populated_impl<1>(h, target, out) : most_populated_impl<2>(h, target, out);
+  if (rc) return rc;
+  *n = (int64_t)(out.size() / h->W);
+  if (!out.empty()) memcpy(dets, out.data(), out.size() * 8);
+  return NGPU_OK;
+}
+
+int ngpu_select_core(ngpu_t* h, int64_t target, int32_t fp32_values, int64_t* dets, int64_t* ncore, int64_t* nnz) {
+  int rc = ready(h);
+  if (rc) return rc;
+  if (target < 1 || target > (int64_t)INT32_MAX) return NGPU_ERR_ARG;
+  if (h->batch) return NGPU_ERR_STATE;
+  std::vector<uint64_t> out;
+  rc = h->W == 1 ? most_populated_impl<1>(h, target, out) : most_populated_impl<2>(h, target, out);
+  if (rc) return rc;
+  const int64_t k = (int64_t)(out.size() / h->W);
+  if (k == 0) return NGPU_ERR_STATE;  // no occupied determinant to build a core space from
+  if (ncore) *ncore = k;
+  if (dets) memcpy(dets, out.data(), out.size() * 8);
+  return ngpu_set_core(h, (const int64_t*)out.data(), k, fp32_values, nnz);
+}
 
 int ngpu_set_core(ngpu_t* h, const int64_t* dets, int64_t ncore, int32_t fp32_values, int64_t* nnz) {
   if (h) h->graph_dirty = true;  // kernel arguments of the captured chain change
@@ -1282,6 +1932,19 @@ int ngpu_set_core(ngpu_t* h, const int64_t* dets, int64_t ncore, int32_t fp32_va
     for (int w = 0; w < h->W; ++w) nel += __builtin_popcountll((uint64_t)dets[i * h->W + w]);
     if (nel != h->sys.nel) return NGPU_ERR_ARG;
   }
+  {  // the core space holds each determinant once
+    std::vector<int64_t> ord((size_t)ncore);
+    for (int64_t i = 0; i < ncore; ++i) ord[i] = i;
+    const int W = h->W;
+    auto lt = [&](int64_t a, int64_t b) {
+      for (int w = W - 1; w >= 0; --w)
+        if (dets[a * W + w] != dets[b * W + w]) return (uint64_t)dets[a * W + w] < (uint64_t)dets[b * W + w];
+      return false;
+    };
+    std::sort(ord.begin(), ord.end(), lt);
+    for (int64_t t = 1; t < ncore; ++t)
+      if (!lt(ord[t - 1], ord[t])) return NGPU_ERR_ARG;
+  }
   if (h->pending) {
     rc = finish(h, nullptr);
     if (rc) return rc;
@@ -1299,6 +1962,40 @@ int ngpu_set_core(ngpu_t* h, const int64_t* dets, int64_t ncore, int32_t fp32_va
   if (rc) return rc;
   h->ncore = ncore;
   h->ds.ncore = 1;
+  return NGPU_OK;
+}
+
+int ngpu_get_core_hamil(ngpu_t* h, int64_t* rowptr, int32_t* cols, double* vals, int64_t cap, int64_t* nnz) {
+  int rc = ready(h);
+  if (rc) return rc;
+  if (!nnz) return NGPU_ERR_ARG;
+  if (!h->ncore) return NGPU_ERR_STATE;
+  *nnz = h->core_nnz;
+  if (!rowptr && !cols && !vals) return NGPU_OK;
+  if (!rowptr || !cols || !vals) return NGPU_ERR_ARG;
+  if (cap < h->core_nnz) return NGPU_ERR_ARG;
+  if (h->pending) {
+    rc = finish(h, nullptr);
+    if (rc) return rc;
+  }
+  HIP_OK(hipStreamSynchronize(h->stream));
+  const size_t nz = (size_t)h->core_nnz;
+  HIP_OK(hipMemcpy(rowptr, h->core_rowptr, ((size_t)h->ncore + 1) * 8, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(cols, h->core_cols, nz * 4, hipMemcpyDeviceToHost));
+  if (h->core_fp32) {
+    std::vector<float> f(nz);
+    HIP_OK(hipMemcpy(f.data(), h->core_vals, nz * 4, hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < nz; ++e) vals[e] = (double)f[e];
+  } else {
+    HIP_OK(hipMemcpy(vals, h->core_vals, nz * 8, hipMemcpyDeviceToHost));
+  }
+  return NGPU_OK;
+}
+
+int ngpu_set_core_build(ngpu_t* h, int32_t method) {
+  if (!h || method < NGPU_CORE_BUILD_AUTO || method > NGPU_CORE_BUILD_SCAN) return NGPU_ERR_ARG;
+  if (h->ncore) return NGPU_ERR_STATE;
+  h->core_build = method;
   return NGPU_OK;
 }
 
@@ -1607,28 +2304,305 @@ int ngpu_set_graph(ngpu_t* h, int32_t on) {
   return NGPU_OK;
 }
 
-// ---- batches: several engines, one captured graph per iteration ----
+// ---- batches: several engines, one launch per kernel for each group ----
+extern "C++" {
 namespace {
+
+// the generator instance of an engine's spawn kernel
+
+// orbitals per lane of the heat-bath singles for lanes G and nsp spatial orbitals
+int singles_pl(int G, int nsp) {
+  if (G == 16) return nsp <= 16 ? 1 : nsp <= 32 ? 2 : 4;
+  if (G == 8) return nsp <= 16 ? 2 : nsp <= 24 ? 3 : nsp <= 32 ? 4 : 8;
+  return nsp <= 16 ? 4 : nsp <= 20 ? 5 : nsp <= 32 ? 8 : 16;
+}
+
+
+// an engine's group: same kernel instances and launch knobs
+BatchGroup group_of(const ngpu_t* h) {
+  BatchGroup g;
+  g.W = h->W;
+  g.S = h->S == 1 ? 1 : kMaxRep;
+  g.gen = gen_of(h);
+  g.singles = g.gen == kGenPchb || g.gen == kGenPchbW;
+  g.lanes = h->single_lanes;
+  g.pl = singles_pl(h->single_lanes, h->sys.nbasis / 2);
+  g.items = h->radix_items;
+  g.rank_sort = h->rank_sort;
+  if (!g.rank_sort) g.passes = word_passes(h);
+  g.core = h->ncore > 0;
+  g.core_fp32 = h->core_fp32;
+  g.rec_words = h->ds.rec_words;
+  return g;
+}
+
+bool same_group(const BatchGroup& a, const BatchGroup& b) {
+  return a.W == b.W && a.S == b.S && a.gen == b.gen && a.lanes == b.lanes && a.pl == b.pl && a.items == b.items &&
+         a.rank_sort == b.rank_sort && a.passes == b.passes && a.core == b.core && a.core_fp32 == b.core_fp32 &&
+         a.rec_words == b.rec_words;
+}
+
+void free_group(BatchGroup& g) {
+  if (g.d_ed) (void)hipFree(g.d_ed);
+  g.d_ed = nullptr;
+  for (int k = 0; k < kLkKinds; ++k) {
+    if (g.d_map[k]) (void)hipFree(g.d_map[k]);
+    g.d_map[k] = nullptr;
+  }
+}
+
+EngDev eng_dev(const ngpu_t* h) {
+  EngDev e{};
+  e.s = h->ds;
+  e.wk = h->wk;
+  e.cnt = h->cnt;
+  e.part = h->dpart;
+  e.att_cnt = h->att_cnt;
+  e.att_off = h->att_off;
+  e.pinfo = h->pinfo;
+  e.blk_part = h->blk_part;
+  e.blk_rm = h->blk_rm;
+  e.tile_p = h->tile_p;
+  e.tcap = h->tcap;
+  e.sp_rec = h->sp_rec;
+  e.rk = h->rk;
+  e.scap = h->scap;
+  e.sq = h->sq;
+  for (int q = 0; q < 2; ++q) {
+    e.sk[q] = h->sk[q];
+    e.si[q] = h->si[q];
+  }
+  e.hist = h->hist;
+  e.digit_tot = h->digit_tot;
+  e.n_key = h->n_key;
+  e.n_sgn = h->n_sgn;
+  e.n_flag = h->n_flag;
+  e.prm = h->dparams;
+  e.dout = h->dout;
+  e.dcyc = h->dcyc;
+  e.ncore = h->ncore;
+  e.core_slot = h->core_slot;
+  e.core_rowptr = h->core_rowptr;
+  e.core_cols = h->core_cols;
+  e.core_vals = h->core_vals;
+  e.core_x = h->core_x;
+  e.core_y = h->core_y;
+  return e;
+}
+
+// Groups of the batch's engines (in the order given, each group in engine
+// order) and their device descriptors.  Each engine's share of a launch is
+// its own launch shape scaled by its walker capacity against the group's
+// largest, so small problems do not take as many blocks as large ones.
+int build_groups(ngpu_batch* b) {
+  for (BatchGroup& g : b->groups) free_group(g);
+  b->groups.clear();
+  for (ngpu_t* h : b->hs) {
+    BatchGroup k = group_of(h);
+    BatchGroup* g = nullptr;
+    for (BatchGroup& x : b->groups)
+      if (same_group(x, k)) g = &x;
+    if (!g) {
+      b->groups.push_back(k);
+      g = &b->groups.back();
+    }
+    g->hs.push_back(h);
+  }
+  for (BatchGroup& g : b->groups) {
+    const size_t E = g.hs.size();
+    if (E > 65535) return NGPU_ERR_ARG;
+    int64_t wmax = 1;
+    for (ngpu_t* h : g.hs) wmax = std::max<int64_t>(wmax, h->wk.cap);
+    std::vector<EngDev> ed(E);
+    std::vector<uint16_t> map[kLkKinds];
+    for (int k = 0; k < kLkKinds; ++k) g.tot[k] = 0;
+    for (size_t i = 0; i < E; ++i) {
+      const ngpu_t* h = g.hs[i];
+      ed[i] = eng_dev(h);
+      auto scaled = [&](int nb) {
+        int64_t v = ((int64_t)nb * h->wk.cap + wmax - 1) / wmax;
+        return (int)std::max<int64_t>(std::min<int64_t>(v, nb), std::min(nb, 16));
+      };
+      int nb[kLkKinds];
+      nb[kLkChunk] = std::min(scaled(h->nb_chunk), h->max_chunk);
+      nb[kLkGrid] = scaled(h->nb_grid);
+      nb[kLkMerge] = scaled(h->nb_merge ? h->nb_merge : h->nb_grid);
+      nb[kLkCore] = (int)std::max<int64_t>(1, std::min<int64_t>((h->ncore + kBlock - 1) / kBlock, h->nb_grid));
+      nb[kLkCoreRow] = (int)std::max<int64_t>(
+          1, std::min<int64_t>((h->ncore + kBlock / kWave - 1) / (kBlock / kWave), 8 * (int64_t)h->nb_grid));
+      for (int k = 0; k < kLkKinds; ++k) {
+        if ((k == kLkCore || k == kLkCoreRow) && !g.core) nb[k] = 0;
+        ed[i].b0[k] = g.tot[k];
+        ed[i].nb[k] = nb[k];
+        g.tot[k] += nb[k];
+        map[k].insert(map[k].end(), (size_t)nb[k], (uint16_t)i);
+      }
+    }
+    HIP_OK(dalloc(&g.d_ed, E));
+    HIP_OK(hipMemcpy(g.d_ed, ed.data(), sizeof(EngDev) * E, hipMemcpyHostToDevice));
+    for (int k = 0; k < kLkKinds; ++k) {
+      if (map[k].empty()) continue;
+      HIP_OK(dalloc(&g.d_map[k], map[k].size()));
+      HIP_OK(hipMemcpy(g.d_map[k], map[k].data(), map[k].size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    }
+  }
+  while (b->side.size() < b->groups.size()) {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    b->side.push_back(st);
+    HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    b->join.push_back(ev);
+  }
+  b->groups_ok = true;
+  for (ngpu_t* h : b->hs) h->graph_dirty = false;
+  return NGPU_OK;
+}
+
+// Launches of one batch group on stream st.  The group's first engine holds
+// the stage events (launch() times the stages it is asked to), so its stage
+// times are the group's.
+template <typename F, typename... Args>
+void glaunch(ngpu_t* h0, int open, int close, F kern, int nblocks, int threads, size_t lds, Args... args) {
+  if (nblocks <= 0) return;
+  launch_lds(h0, open, close, kern, dim3((unsigned)nblocks), dim3((unsigned)threads), lds, args...);
+}
+
+template <int W, int R, int IT>
+void group_sort_merge(BatchGroup& g) {
+  ngpu_t* h0 = g.hs[0];
+  const int E = (int)g.hs.size();
+  const EngDev* ed = g.d_ed;
+  int cur = 0;
+  if (g.rank_sort) {
+    glaunch(h0, 4, -1, km_radix_hist_ranks<IT>, g.tot[kLkChunk], kBlock, 0, ed, (const uint16_t*)g.d_map[kLkChunk]);
+    glaunch(h0, -1, -1, km_radix_scan, E * kDigitBlocks, kBlock, 0, ed);
+    glaunch(h0, -1, -1, km_radix_scatter_ranks<IT>, g.tot[kLkChunk], kBlock, 0, ed,
+            (const uint16_t*)g.d_map[kLkChunk]);
+    glaunch(h0, -1, 4, km_seg_sort, E * kDigitBlocks, kSegThreads, 0, ed);
+    glaunch(h0, 6, 6, km_merge_annihilate<W, R, uint32_t, 1>, g.tot[kLkMerge], kBlock, 0, ed,
+            (const uint16_t*)g.d_map[kLkMerge], 0);
+    return;
+  }
+  const int np = (int)g.passes.size();
+  for (int p = 0; p < np; ++p) {
+    const int in = (p & 1) ^ 1;  // pass p writes sk[p & 1]
+    glaunch(h0, p == 0 ? 4 : -1, -1, km_radix_hist_words<uint64_t, W, W == 2, IT, 8>, g.tot[kLkChunk], kBlock, 0, ed,
+            (const uint16_t*)g.d_map[kLkChunk], p == 0 ? 1 : 0, in, g.passes[p].first, g.passes[p].second);
+    glaunch(h0, -1, -1, km_radix_scan, E * kDigitBlocks, kBlock, 0, ed);
+    glaunch(h0, -1, p + 1 == np ? 4 : -1, km_radix_scatter_words<uint64_t, W, W == 2, IT, 8>, g.tot[kLkChunk], kBlock,
+            0, ed, (const uint16_t*)g.d_map[kLkChunk], p == 0 ? 1 : 0, in, g.passes[p].first, g.passes[p].second);
+    cur = p & 1;
+  }
+  glaunch(h0, 6, 6, km_merge_annihilate<W, R, uint64_t, W>, g.tot[kLkMerge], kBlock, 0, ed,
+          (const uint16_t*)g.d_map[kLkMerge], cur);
+}
+
+template <int W, int R, int G>
+void group_spawn(BatchGroup& g) {
+  glaunch(g.hs[0], 2, 2, km_spawn<W, R, G>, g.tot[kLkGrid], kBlock, 0, (const EngDev*)g.d_ed,
+          (const uint16_t*)g.d_map[kLkGrid], g.rank_sort ? 1 : 0);
+}
+
+template <int W, int R, int PL, int GL>
+void group_singles(BatchGroup& g, int nsp) {
+  const size_t lds = sizeof(double) * (size_t)(kBlock / GL) * nsp;
+  glaunch(g.hs[0], 3, 3, km_spawn_singles<W, R, PL, GL>, g.tot[kLkGrid], kBlock, lds, (const EngDev*)g.d_ed,
+          (const uint16_t*)g.d_map[kLkGrid], g.rank_sort ? 1 : 0);
+}
+
+template <int W, int R>
+void group_chain_wr(BatchGroup& g) {
+  ngpu_t* h0 = g.hs[0];
+  const EngDev* ed = g.d_ed;
+  const uint16_t* const* m = g.d_map;
+  glaunch(h0, 0, 0, km_prepass<W, R>, g.tot[kLkChunk], kBlock, 0, ed, (const uint16_t*)m[kLkChunk]);
+  if (g.core) {
+    glaunch(h0, -1, -1, km_core_gather<R>, g.tot[kLkCore], kBlock, 0, ed, (const uint16_t*)m[kLkCore]);
+    if (g.core_fp32)
+      glaunch(h0, 9, 9, km_core_spmv<R, float>, g.tot[kLkCoreRow], kBlock, 0, ed, (const uint16_t*)m[kLkCoreRow]);
+    else
+      glaunch(h0, 9, 9, km_core_spmv<R, double>, g.tot[kLkCoreRow], kBlock, 0, ed, (const uint16_t*)m[kLkCoreRow]);
+  }
+  glaunch(h0, 1, 1, km_scan_apply<W>, g.tot[kLkChunk], kBlock, 0, ed, (const uint16_t*)m[kLkChunk]);
+  if (g.gen == kGenHubbard) group_spawn<W, R, kGenHubbard>(g);
+  else if (g.gen == kGenUniform) group_spawn<W, R, kGenUniform>(g);
+  else if (g.gen == kGenPchbW) group_spawn<W, R, kGenPchbW>(g);
+  else group_spawn<W, R, kGenPchb>(g);
+  if (g.singles) {
+    const int nsp = h0->sys.nbasis / 2;
+    switch (g.lanes * 100 + g.pl) {
+      case 1601: group_singles<W, R, 1, 16>(g, nsp); break;
+      case 1602: group_singles<W, R, 2, 16>(g, nsp); break;
+      case 1604: group_singles<W, R, 4, 16>(g, nsp); break;
+      case 802: group_singles<W, R, 2, 8>(g, nsp); break;
+      case 803: group_singles<W, R, 3, 8>(g, nsp); break;
+      case 804: group_singles<W, R, 4, 8>(g, nsp); break;
+      case 808: group_singles<W, R, 8, 8>(g, nsp); break;
+      case 404: group_singles<W, R, 4, 4>(g, nsp); break;
+      case 405: group_singles<W, R, 5, 4>(g, nsp); break;
+      case 408: group_singles<W, R, 8, 4>(g, nsp); break;
+      default: group_singles<W, R, 16, 4>(g, nsp); break;
+    }
+  }
+  if (g.core) glaunch(h0, -1, -1, km_core_apply<R>, g.tot[kLkCore], kBlock, 0, ed, (const uint16_t*)m[kLkCore]);
+  if (g.items == 2) group_sort_merge<W, R, 2>(g);
+  else if (g.items == 8) group_sort_merge<W, R, 8>(g);
+  else group_sort_merge<W, R, 4>(g);
+  glaunch(h0, 7, 7, km_insert<W, R>, g.tot[kLkGrid], kBlock, 0, ed, (const uint16_t*)m[kLkGrid]);
+  glaunch(h0, 8, 8, km_stats<W, R>, g.tot[kLkGrid], kBlock, 0, ed, (const uint16_t*)m[kLkGrid]);
+  glaunch(h0, -1, -1, km_end_iter, (int)g.hs.size(), kBlock, 0, ed);
+}
+
+void group_chain(BatchGroup& g) {
+  if (g.W == 1) g.S == 1 ? group_chain_wr<1, 1>(g) : group_chain_wr<1, kMaxRep>(g);
+  else g.S == 1 ? group_chain_wr<2, 1>(g) : group_chain_wr<2, kMaxRep>(g);
+}
+
+// one iteration of every group, each on its own stream forked from and
+// joined to the batch stream (captured into the batch graph, or launched
+// directly)
+int batch_chains(ngpu_batch* b) {
+  HIP_OK(hipEventRecord(b->fork, b->stream));
+  for (size_t i = 0; i < b->groups.size(); ++i) {
+    BatchGroup& g = b->groups[i];
+    ngpu_t* h0 = g.hs[0];
+    HIP_OK(hipStreamWaitEvent(b->side[i], b->fork, 0));
+    h0->stream = b->side[i];
+    const bool timed = h0->prof && h0->ev_ok && !b->use_graph;
+    const bool prof = h0->prof;
+    if (timed) {
+      h0->ev_cur = (int)(h0->ev_next % kEvRing);
+      harvest(h0, h0->ev_cur);
+      h0->ev_mask[h0->ev_cur] = 0;
+    } else {
+      h0->prof = false;  // no events inside a captured graph
+    }
+    group_chain(g);
+    h0->prof = prof;
+    if (timed) {
+      h0->ev_pending[h0->ev_cur] = true;
+      h0->ev_next++;
+    }
+    h0->stream = b->stream;
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(b->join[i], b->side[i]));
+    HIP_OK(hipStreamWaitEvent(b->stream, b->join[i], 0));
+  }
+  return NGPU_OK;
+}
 
 int capture_batch(ngpu_batch* b) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   b->gexec = nullptr;
   hipGraph_t g = nullptr;
   HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  HIP_OK(hipEventRecord(b->fork, b->stream));
-  for (size_t i = 0; i < b->hs.size(); ++i) {
-    ngpu_t* h = b->hs[i];
-    HIP_OK(hipStreamWaitEvent(b->side[i], b->fork, 0));
-    h->stream = b->side[i];
-    bool prof = h->prof;
-    h->prof = false;
-    run_chain(h);
-    h->prof = prof;
-    h->stream = b->stream;
-    HIP_OK(hipEventRecord(b->join[i], b->side[i]));
-    HIP_OK(hipStreamWaitEvent(b->stream, b->join[i], 0));
-  }
+  int rc = batch_chains(b);
   hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
   if (e != hipSuccess) return NGPU_ERR_HIP;
   e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
@@ -1636,7 +2610,20 @@ int capture_batch(ngpu_batch* b) {
     b->gexec = nullptr;
     return NGPU_ERR_HIP;
   }
-  for (ngpu_t* h : b->hs) h->graph_dirty = false;
+  return NGPU_OK;
+}
+
+// a set-up call on an engine (core space, precision, launch shape, ...)
+// changes its descriptor and possibly its group: rebuild them, and the graph
+int refresh_groups(ngpu_batch* b) {
+  bool stale = !b->groups_ok;
+  for (ngpu_t* h : b->hs) stale = stale || h->graph_dirty;
+  if (!stale) return NGPU_OK;
+  HIP_OK(hipStreamSynchronize(b->stream));
+  int rc = build_groups(b);
+  if (rc) return rc;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
   return NGPU_OK;
 }
 
@@ -1648,6 +2635,7 @@ void batch_release(ngpu_batch* b) {
     h->batch = nullptr;
     h->graph_dirty = true;
   }
+  for (BatchGroup& g : b->groups) free_group(g);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   for (hipStream_t st : b->side) (void)hipStreamDestroy(st);
   for (hipEvent_t ev : b->join) (void)hipEventDestroy(ev);
@@ -1662,6 +2650,7 @@ void batch_release(ngpu_batch* b) {
 }
 
 }  // namespace
+}  // extern "C++"
 
 int ngpu_batch_create(ngpu_batch** out, ngpu_t* const* hs, int32_t n) {
   if (!out || !hs || n < 1) return NGPU_ERR_ARG;
@@ -1680,14 +2669,6 @@ int ngpu_batch_create(ngpu_batch** out, ngpu_t* const* hs, int32_t n) {
   for (int q = 0; ok && q < ngpu_batch::kRing; ++q)
     ok = hipHostMalloc((void**)&b->hparams[q], sizeof(IterParams) * (size_t)n, hipHostMallocDefault) == hipSuccess &&
          hipEventCreateWithFlags(&b->slot_done[q], hipEventDisableTiming) == hipSuccess;
-  for (int32_t i = 0; ok && i < n; ++i) {
-    hipStream_t st = nullptr;
-    hipEvent_t ev = nullptr;
-    ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
-    if (ok) b->side.push_back(st);
-    ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
-    if (ok) b->join.push_back(ev);
-  }
   if (!ok) {
     batch_release(b);
     return NGPU_ERR_HIP;
@@ -1721,14 +2702,49 @@ int ngpu_batch_iterate(ngpu_batch* b, const ngpu_iter_in_t* ins) {
   HIP_OK(hipEventRecord(b->slot_done[q], b->stream));
   b->slot_used[q] = true;
   b->next_slot = (q + 1) % ngpu_batch::kRing;
-  bool stale = !b->gexec;
-  for (ngpu_t* h : b->hs) stale = stale || h->graph_dirty;
-  if (stale) {
-    int rc = capture_batch(b);
+  int rc = refresh_groups(b);
+  if (rc) return rc;
+  if (b->use_graph) {
+    if (!b->gexec) {
+      rc = capture_batch(b);
+      if (rc) return rc;
+    }
+    HIP_OK(hipGraphLaunch(b->gexec, b->stream));
+  } else {
+    rc = batch_chains(b);
     if (rc) return rc;
   }
-  HIP_OK(hipGraphLaunch(b->gexec, b->stream));
   for (ngpu_t* h : b->hs) h->pending = true;
+  return NGPU_OK;
+}
+
+int ngpu_batch_set_graph(ngpu_batch* b, int32_t on) {
+  if (!b) return NGPU_ERR_ARG;
+  b->use_graph = on != 0;
+  return NGPU_OK;
+}
+
+int ngpu_batch_get_groups(ngpu_batch* b, int32_t* group_of_engine, int32_t* n_groups) {
+  if (!b) return NGPU_ERR_ARG;
+  int rc = refresh_groups(b);
+  if (rc) return rc;
+  if (n_groups) *n_groups = (int32_t)b->groups.size();
+  if (group_of_engine)
+    for (size_t i = 0; i < b->hs.size(); ++i)
+      for (size_t g = 0; g < b->groups.size(); ++g)
+        for (ngpu_t* h : b->groups[g].hs)
+          if (h == b->hs[i]) group_of_engine[i] = (int32_t)g;
+  return NGPU_OK;
+}
+
+int ngpu_batch_set_profiling(ngpu_batch* b, int32_t on) {
+  if (!b) return NGPU_ERR_ARG;
+  for (ngpu_t* h : b->hs) {
+    h->prof = on != 0;
+    h->prof_mask = (uint32_t)on & ((1u << kStages) - 1u);
+    for (int s = 0; s < kStages; ++s) h->stage_ms[s] = 0.0;
+    h->prof_iters = 0;
+  }
   return NGPU_OK;
 }
 

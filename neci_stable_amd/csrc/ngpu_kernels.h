@@ -64,6 +64,7 @@ struct alignas(256) Counters {
   unsigned long long n_live, n_tomb, n_occ;
   unsigned long long n_attempts;
   unsigned long long n_fail;      // inserts refused (capacity errors)
+  unsigned int ins_refused;       // k_insert placed no new determinant (MaxWalkers would be passed)
   unsigned long long free_tail_ins;  // free_tail as k_insert saw it
   unsigned long long rm_pre;         // slots the iteration's prepass freed
   unsigned int err;
@@ -125,6 +126,15 @@ struct IterParams {
   uint32_t pad;
 };
 
+// The blocks a kernel body runs on: this block's index and the number of
+// blocks of its launch share.  A plain launch gives the hardware grid; a
+// batch launch (one launch covering several engines) gives each engine's
+// share of the blocks, so a body sees the same index space either way.
+struct Grid {
+  int64_t b, nb;
+};
+__device__ inline Grid hw_grid() { return Grid{(int64_t)blockIdx.x, (int64_t)gridDim.x}; }
+
 // ----------------------------------------------------------- wave utils
 __device__ inline int lane_id() { return __lane_id(); }
 
@@ -181,9 +191,9 @@ struct BlockStats {
   }
   // into partial row blockIdx % kStatCopies (InstNoAtHF has a single writer,
   // so summing the rows reproduces it)
-  __device__ void flush(double* part) {
+  __device__ void flush(double* part, int64_t blk) {
     __syncthreads();
-    double* g = part + (size_t)(blockIdx.x % kStatCopies) * kStatStride;
+    double* g = part + (size_t)(blk % kStatCopies) * kStatStride;
     for (int i = threadIdx.x; i < kStatStride; i += blockDim.x) {
       double v = acc[i];
       if (v == 0.0) continue;
@@ -338,17 +348,17 @@ __device__ inline int64_t hash_insert(const Walk& wk, const uint64_t* key, int64
 // rebuild of the walker hash in two grid-wide phases (a kernel boundary
 // between them): every entry emptied, then every live slot reinserted
 template <int W>
-__device__ inline void hash_clear_part(const Walk& wk) {
-  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < wk.L; b += (int64_t)gridDim.x * blockDim.x) {
+__device__ inline void hash_clear_part(const Grid& g, const Walk& wk) {
+  for (int64_t b = g.b * blockDim.x + threadIdx.x; b < wk.L; b += g.nb * blockDim.x) {
     int32_t* sl = (int32_t*)(wk.htab + (size_t)b * 8 * W + 4 * W);
 #pragma unroll
     for (int e = 0; e < kBucketEntries; ++e) sl[e] = kSlotEmpty;
   }
 }
 template <int W>
-__device__ inline void hash_refill_part(const DevSys& s, const Walk& wk, Counters* cnt, int64_t nslots) {
-  for (int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; slot < nslots;
-       slot += (int64_t)gridDim.x * blockDim.x) {
+__device__ inline void hash_refill_part(const Grid& g, const DevSys& s, const Walk& wk, Counters* cnt,
+                                        int64_t nslots) {
+  for (int64_t slot = g.b * blockDim.x + threadIdx.x; slot < nslots; slot += g.nb * blockDim.x) {
     if (wk.flags(slot) & kFlagRemoved) continue;
     uint64_t key[W];
 #pragma unroll
@@ -361,9 +371,9 @@ __device__ inline void hash_refill_part(const DevSys& s, const Walk& wk, Counter
   }
 }
 
-__device__ inline void chunk_of(int64_t n, int64_t& begin, int64_t& end) {
-  int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  begin = (int64_t)blockIdx.x * chunk;
+__device__ inline void chunk_of(const Grid& g, int64_t n, int64_t& begin, int64_t& end) {
+  int64_t chunk = (n + g.nb - 1) / g.nb;
+  begin = g.b * chunk;
   end = begin + chunk;
   if (end > n) end = n;
   if (begin > n) begin = n;
@@ -416,12 +426,12 @@ __device__ inline int64_t block_exclusive_scan(int64_t v, int64_t* lds, int64_t&
 
 // trips of a grid-stride loop that every lane of the grid runs (keeps wave-wide
 // calls convergent); the element of this lane in trip t is t*stride + gid
-__device__ inline int64_t grid_trips(int64_t n) {
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+__device__ inline int64_t grid_trips(const Grid& g, int64_t n) {
+  int64_t stride = g.nb * blockDim.x;
   return (n + stride - 1) / stride;
 }
-__device__ inline int64_t grid_elem(int64_t t) {
-  return t * (int64_t)gridDim.x * blockDim.x + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline int64_t grid_elem(const Grid& g, int64_t t) {
+  return t * g.nb * blockDim.x + g.b * blockDim.x + threadIdx.x;
 }
 
 // ------------------------------------------------------------- prepass
@@ -430,17 +440,14 @@ __device__ inline int64_t grid_elem(int64_t t) {
 // 1036-1321, 2160-2407).  Blocks own contiguous slot ranges so that the block
 // sums of attempt counts feed the scan directly.
 template <int W, int R>
-__global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters* cnt,
-                                                    double* part, int32_t* att_cnt,
-                                                    uint8_t* pinfo, int64_t* blk_part, int64_t* blk_rm,
-                                                    IterParams* prm, IterParams v, int use_v) {
+__device__ __forceinline__ void k_prepass_body(const Grid& g_, const DevSys& s, const Walk& wk, Counters* cnt, double* part, int32_t* att_cnt, uint8_t* pinfo, int64_t* blk_part, int64_t* blk_rm, IterParams* prm, const IterParams& v, int use_v) {
   if (cnt->halt) return;
   // the iteration's inputs: from the launch when launches are issued one by
   // one (block 0 publishes them for the later kernels), from device memory
   // in a replayed graph
   const double tau = use_v ? v.tau : prm->tau;
   const uint32_t iter = use_v ? v.iter : prm->iter;
-  if (use_v && blockIdx.x == 0 && threadIdx.x == 0) *prm = v;
+  if (use_v && g_.b == 0 && threadIdx.x == 0) *prm = v;
   __shared__ double acc_lds[kStatStride];
   __shared__ int64_t red[kBlock / kWave];
   BlockStats bs{acc_lds};
@@ -448,9 +455,9 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
   const int S = R == 1 ? 1 : s.S;
   int64_t nslots = (int64_t)cnt->nslots;
   const bool rebuild = cnt->rebuild != 0;
-  if (rebuild) hash_clear_part<W>(wk);  // the attempt scan refills the table
+  if (rebuild) hash_clear_part<W>(g_, wk);  // the attempt scan refills the table
   int64_t begin, end;
-  chunk_of(nslots, begin, end);
+  chunk_of(g_, nslots, begin, end);
   int64_t my_att = 0, my_rm = 0;
   bool death_err = false;
   LaneSum<R> a_idets, a_iwalk, a_ndets, a_nwalk, a_hf, a_doubs, a_enum, a_died, a_born, a_abort;
@@ -570,10 +577,14 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
   a_abort.done(bs, soff(F_NABORTED, 0));
   int64_t tot;
   (void)block_exclusive_scan(my_att, red, tot);
-  if (threadIdx.x == 0) blk_part[blockIdx.x] = tot;
+  if (threadIdx.x == 0) blk_part[g_.b] = tot;
   (void)block_exclusive_scan(my_rm, red, tot);
-  if (threadIdx.x == 0) blk_rm[blockIdx.x] = tot;
-  bs.flush(part);
+  if (threadIdx.x == 0) blk_rm[g_.b] = tot;
+  bs.flush(part, g_.b);
+}
+template <int W, int R>
+__global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters* cnt, double* part, int32_t* att_cnt, uint8_t* pinfo, int64_t* blk_part, int64_t* blk_rm, IterParams* prm, IterParams v, int use_v) {
+  k_prepass_body<W, R>(hw_grid(), s, wk, cnt, part, att_cnt, pinfo, blk_part, blk_rm, prm, v, use_v);
 }
 
 
@@ -583,25 +594,22 @@ __global__ void __launch_bounds__(kBlock) k_prepass(DevSys s, Walk wk, Counters*
 // a rebuild is due, also the refill of the walker hash that the prepass
 // cleared, before any lookup of the iteration.
 template <int W>
-__global__ void __launch_bounds__(kBlock) k_scan_apply(DevSys s, Walk wk, const int32_t* att_cnt, int64_t* att_off,
-                                                       const int64_t* part, const int64_t* part_rm,
-                                                       const uint8_t* pinfo, Counters* cnt, int S,
-                                                       int64_t* tile_p, int64_t tcap) {
+__device__ __forceinline__ void k_scan_apply_body(const Grid& g_, const DevSys& s, const Walk& wk, const int32_t* att_cnt, int64_t* att_off, const int64_t* part, const int64_t* part_rm, const uint8_t* pinfo, Counters* cnt, int S, int64_t* tile_p, int64_t tcap) {
   if (cnt->halt) return;
   __shared__ int64_t red[kBlock / kWave];
   int64_t nslots = (int64_t)cnt->nslots;
   if (cnt->rebuild) {
-    hash_refill_part<W>(s, wk, cnt, nslots);
+    hash_refill_part<W>(g_, s, wk, cnt, nslots);
     // the prepass's removals left no tombstones: the table now holds the live
     // slots only (the end-of-iteration accounting counts the later removals)
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (g_.b == 0 && threadIdx.x == 0) {
       cnt->n_tomb = 0;
       cnt->rebuilt = 1;
       cnt->n_rebuilds += 1;
     }
   }
   int64_t begin, end;
-  chunk_of(nslots, begin, end);
+  chunk_of(g_, nslots, begin, end);
   // this block's first attempt offset and first free-queue position: the
   // counts of the blocks before it.  The slots the prepass freed go, in slot
   // order, after the free queue's tail (the prepass appends nothing itself,
@@ -611,7 +619,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(DevSys s, Walk wk, const 
   const unsigned long long tail0 = cnt->free_tail0;
   constexpr int kPackShift = 40;
   int64_t loc = 0, loc_rm = 0;
-  for (int q = threadIdx.x; q < (int)blockIdx.x; q += blockDim.x) {
+  for (int q = threadIdx.x; q < (int)g_.b; q += blockDim.x) {
     loc += part[q];
     loc_rm += part_rm[q];
   }
@@ -660,12 +668,16 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(DevSys s, Walk wk, const 
       run += tot;
     }
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+  if (g_.b == g_.nb - 1 && threadIdx.x == 0) {
     att_off[nslots * S] = run;
     cnt->n_attempts = (unsigned long long)run;
     cnt->free_tail = tail0 + (unsigned long long)run_rm;
     cnt->rm_pre = (unsigned long long)run_rm;
   }
+}
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_scan_apply(DevSys s, Walk wk, const int32_t* att_cnt, int64_t* att_off, const int64_t* part, const int64_t* part_rm, const uint8_t* pinfo, Counters* cnt, int S, int64_t* tile_p, int64_t tcap) {
+  k_scan_apply_body<W>(hw_grid(), s, wk, att_cnt, att_off, part, part_rm, pinfo, cnt, S, tile_p, tcap);
 }
 
 // ------------------------------------------------------------- spawning
@@ -832,7 +844,11 @@ struct SpawnAcc {
 // generator bound at compile time, one k_spawn instantiation each, so that the
 // code of one generator (the uniform one's private arrays in particular) costs
 // the others nothing
-constexpr int kGenHubbard = 0, kGenPchb = 1, kGenUniform = 2;
+// generator instances of the spawn kernel: the lattice generator, PCHB with
+// FAST-FAST holes and UNIF-UNIF / UNIF-FAST particles (a fixed number of
+// uniforms), the uniform molecular generator, and PCHB with weighted
+// particles or holes (constrained samplers that may redraw)
+constexpr int kGenHubbard = 0, kGenPchb = 1, kGenUniform = 2, kGenPchbW = 3;
 
 // one spawning attempt (attempt_create_normal + create_particle); mode 0: any
 // excitation, 1: doubles only (singles are queued), 2: singles only.  Wave-wide.
@@ -855,7 +871,8 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* 
     for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
     double dr[6];
     draw2<W>(s.seed, key, iter, r, k, 1, dr[0], dr[1]);
-    bool single = G == kGenPchb && dr[0] < s.psingles;
+    constexpr bool kPchb = G == kGenPchb || G == kGenPchbW;
+    bool single = kPchb && dr[0] < s.psingles;
     if (mode == 1 && single) {
       queue = true;
     } else {
@@ -863,7 +880,7 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* 
       // 3 (d2, d4) only for the generators that read them
       draw2<W>(s.seed, key, iter, r, k, 2, dr[3], dr[5]);
       dr[2] = dr[4] = 0.0;
-      if (G == kGenUniform || (G == kGenPchb && (s.particles == 1 || single)))
+      if (G == kGenUniform || G == kGenPchbW || (G == kGenPchb && (s.particles == 1 || single)))
         draw2<W>(s.seed, key, iter, r, k, 3, dr[2], dr[4]);
       Excit ex;
       if constexpr (G == kGenHubbard) {
@@ -875,6 +892,10 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* 
       } else if (single) {
         ex = gen_single_heatbath<W>(s, key, dr[1], dr[2]);
         ex.pgen = ex.pgen * s.psingles;
+      } else if constexpr (G == kGenPchbW) {
+        DrawSeq<W> u(dr, s.seed, key, iter, r, k);
+        ex = gen_double_pchb_weighted<W>(s, key, u);
+        ex.pgen = ex.pgen * s.pdoubles;
       } else {
         ex = gen_double_pchb<W>(s, key, dr);
         ex.pgen = ex.pgen * s.pdoubles;
@@ -935,10 +956,7 @@ __device__ inline void spawn_attempt(const DevSys& s, const Walk& wk, Counters* 
 // singles (the heat-bath list is ~20x longer per attempt) go to a queue so that
 // waves do not idle on a few long lanes
 template <int W, int R, int G>
-__global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* cnt, double* part,
-                                                  const int64_t* att_off, const int64_t* tile_p, int64_t tcap,
-                                                  const uint8_t* pinfo, uint64_t* sp_rec, uint32_t* rk,
-                                                  int64_t scap, int64_t* sq, const IterParams* prm) {
+__device__ __forceinline__ void k_spawn_body(const Grid& g_, const DevSys& s, const Walk& wk, Counters* cnt, double* part, const int64_t* att_off, const int64_t* tile_p, int64_t tcap, const uint8_t* pinfo, uint64_t* sp_rec, uint32_t* rk, int64_t scap, int64_t* sq, const IterParams* prm) {
   if (cnt->halt) return;
   const double tau = prm->tau;
   const uint32_t iter = prm->iter;
@@ -954,16 +972,16 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
   const int64_t total = (int64_t)cnt->n_attempts;
   const int64_t npairs = (int64_t)cnt->nslots * s.S;
   bool overflow = false;
-  constexpr int mode = G == kGenPchb ? 1 : 0;
+  constexpr int mode = (G == kGenPchb || G == kGenPchbW) ? 1 : 0;
   SpawnAcc<R> acc;
   __shared__ int64_t pb[2];
   __shared__ int64_t woff[kParentWindow];
-  int64_t trips = grid_trips(total);
+  int64_t trips = grid_trips(g_, total);
   for (int64_t t = 0; t < trips; ++t) {
-    int64_t a = grid_elem(t);
+    int64_t a = grid_elem(g_, t);
     // the block's attempts are the consecutive tile a0 .. a0+kBlock-1: its
     // parents lie between the tile's first parent and the next tile's
-    int64_t a0 = grid_elem(t) - threadIdx.x;
+    int64_t a0 = grid_elem(g_, t) - threadIdx.x;
     int64_t tile = a0 / kBlock;
     if (threadIdx.x == 0) {
       int64_t lo = 0, hi = 0;
@@ -1011,7 +1029,11 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
   if (mode == 1) queue_flush(&qs, cnt, sq, scap, true, overflow);
   if (overflow) atomicOr(&cnt->err, 1u << NGPU_ERR_SPAWN_FULL);
   acc.done(bs, mode);
-  bs.flush(part);
+  bs.flush(part, g_.b);
+}
+template <int W, int R, int G>
+__global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* cnt, double* part, const int64_t* att_off, const int64_t* tile_p, int64_t tcap, const uint8_t* pinfo, uint64_t* sp_rec, uint32_t* rk, int64_t scap, int64_t* sq, const IterParams* prm) {
+  k_spawn_body<W, R, G>(hw_grid(), s, wk, cnt, part, att_off, tile_p, tcap, pinfo, sp_rec, rk, scap, sq, prm);
 }
 
 // Queued single excitations (molecular, heat-bath singles), G lanes per
@@ -1063,9 +1085,7 @@ __device__ inline void hb_rows(uint64_t x, int base, const double* row, int nsp,
 // PL: spatial orbitals per lane (nsp <= G*PL); dynamic LDS holds the
 // (kBlock/G)*nsp values of the block's attempts
 template <int W, int R, int PL, int G>
-__global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Counters* cnt, double* part,
-                                                          const uint8_t* pinfo, uint64_t* sp_rec, uint32_t* rk,
-                                                          int64_t scap, const int64_t* sq, const IterParams* prm) {
+__device__ __forceinline__ void k_spawn_singles_body(const Grid& g_, const DevSys& s, const Walk& wk, Counters* cnt, double* part, const uint8_t* pinfo, uint64_t* sp_rec, uint32_t* rk, int64_t scap, const int64_t* sq, const IterParams* prm) {
   if (cnt->halt) return;
   constexpr int kGroups = kBlock / G;
   const double tau = prm->tau;
@@ -1090,13 +1110,13 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
   const int gbase = lane_id() - lane_id() % G;
   const bool leader = g == 0;
   double* myv = hv_dyn + (size_t)gl * nsp;
-  const int64_t groups = (int64_t)gridDim.x * kGroups;
-  const int64_t grp = (int64_t)blockIdx.x * kGroups + gl;
+  const int64_t groups = (int64_t)g_.nb * kGroups;
+  const int64_t grp = (int64_t)g_.b * kGroups + gl;
   const int64_t trips = (nq + groups - 1) / groups;
   for (int64_t t = 0; t < trips; ++t) {
     // blocks whose groups are all past the queue skip the attempt (block-
     // uniform, so the body's barrier is reached by all of the block or none)
-    if (t * groups + (int64_t)blockIdx.x * kGroups < nq) {
+    if (t * groups + (int64_t)g_.b * kGroups < nq) {
       int64_t q = t * groups + grp;
       bool act = q < nq;
       int64_t p = act ? sq[2 * q] : 0;
@@ -1222,7 +1242,11 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
   spawn_flush<W>(&ss, cnt, sp_rec, scap, true, overflow, s, bl, rk);
   if (overflow) atomicOr(&cnt->err, 1u << NGPU_ERR_SPAWN_FULL);
   acc.done(bs, 2);
-  bs.flush(part);
+  bs.flush(part, g_.b);
+}
+template <int W, int R, int PL, int G>
+__global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Counters* cnt, double* part, const uint8_t* pinfo, uint64_t* sp_rec, uint32_t* rk, int64_t scap, const int64_t* sq, const IterParams* prm) {
+  k_spawn_singles_body<W, R, PL, G>(hw_grid(), s, wk, cnt, part, pinfo, sp_rec, rk, scap, sq, prm);
 }
 
 // ---------------------------------------------------------- radix sort
@@ -1243,13 +1267,13 @@ constexpr int kRadixBins = 1024;  // bins of the widest digit (10 bits)
 // get an empty range), so that a small spawned list is not spread over more
 // blocks than it fills
 template <int IT>
-__device__ inline void radix_chunk(int64_t n, int64_t& begin, int64_t& end) {
+__device__ inline void radix_chunk(const Grid& g, int64_t n, int64_t& begin, int64_t& end) {
   constexpr int kRadixTile = IT * kBlock;
   int64_t tiles = (n + kRadixTile - 1) / kRadixTile;
-  int64_t nb = tiles < (int64_t)gridDim.x ? tiles : (int64_t)gridDim.x;
+  int64_t nb = tiles < g.nb ? tiles : g.nb;
   if (nb < 1) nb = 1;
   int64_t per = (tiles + nb - 1) / nb * kRadixTile;
-  begin = (int64_t)blockIdx.x * per;
+  begin = g.b * per;
   end = begin + per;
   if (end > n) end = n;
   if (begin > n) begin = n;
@@ -1265,8 +1289,7 @@ __device__ inline uint32_t radix_digit(const K* k, int word, int shift) {
 }
 
 template <typename K, int NW, bool FLIP, int IT, int DB>
-__global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, int kstride, const unsigned long long* n_ptr,
-                                                       int64_t ncap, int word, int shift, uint32_t* hist) {
+__device__ __forceinline__ void k_radix_hist_body(const Grid& g_, const K* keys, int kstride, const unsigned long long* n_ptr, int64_t ncap, int word, int shift, uint32_t* hist) {
   constexpr int RB = 1 << DB;
   __shared__ uint32_t h[RB];
   for (int i = threadIdx.x; i < RB; i += blockDim.x) h[i] = 0;
@@ -1274,20 +1297,22 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, int kstrid
   int64_t n = (int64_t)*n_ptr;
   if (n > ncap) n = ncap;
   int64_t begin, end;
-  radix_chunk<IT>(n, begin, end);
+  radix_chunk<IT>(g_, n, begin, end);
   for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x)
     atomicAdd(&h[radix_digit<K, NW, FLIP, DB>(keys + i * kstride, word, shift)], 1u);
   __syncthreads();
-  for (int d = threadIdx.x; d < RB; d += blockDim.x) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
+  for (int d = threadIdx.x; d < RB; d += blockDim.x) hist[(size_t)d * g_.nb + g_.b] = h[d];
+}
+template <typename K, int NW, bool FLIP, int IT, int DB>
+__global__ void __launch_bounds__(kBlock) k_radix_hist(const K* keys, int kstride, const unsigned long long* n_ptr, int64_t ncap, int word, int shift, uint32_t* hist) {
+  k_radix_hist_body<K, NW, FLIP, IT, DB>(hw_grid(), keys, kstride, n_ptr, ncap, word, shift, hist);
 }
 
 // sort key of each spawned record's determinant: its colex rank
 // (binom[p*(nel+1)+i] = C(p, i)) times kRankMul, fused with the histogram of
 // the digit at shift
 template <int W, int IT, int DB>
-__global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, const unsigned long long* n_ptr,
-                                                      int64_t ncap, int rstride, const uint32_t* binom,
-                                                      int nel1, uint32_t* rank, uint32_t* hist, int shift) {
+__device__ __forceinline__ void k_rank_hist_body(const Grid& g_, const uint64_t* keys, const unsigned long long* n_ptr, int64_t ncap, int rstride, const uint32_t* binom, int nel1, uint32_t* rank, uint32_t* hist, int shift) {
   constexpr int RB = 1 << DB;
   __shared__ uint32_t h[RB];
   for (int i = threadIdx.x; i < RB; i += blockDim.x) h[i] = 0;
@@ -1295,20 +1320,24 @@ __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, cons
   int64_t n = (int64_t)*n_ptr;
   if (n > ncap) n = ncap;
   int64_t begin, end;
-  radix_chunk<IT>(n, begin, end);
+  radix_chunk<IT>(g_, n, begin, end);
   for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
     uint32_t r = colex_rank<W>(binom, nel1, keys + i * rstride) * kRankMul;
     rank[i] = r;
     atomicAdd(&h[(r >> shift) & (RB - 1u)], 1u);
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < RB; d += blockDim.x) hist[(size_t)d * gridDim.x + blockIdx.x] = h[d];
+  for (int d = threadIdx.x; d < RB; d += blockDim.x) hist[(size_t)d * g_.nb + g_.b] = h[d];
+}
+template <int W, int IT, int DB>
+__global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, const unsigned long long* n_ptr, int64_t ncap, int rstride, const uint32_t* binom, int nel1, uint32_t* rank, uint32_t* hist, int shift) {
+  k_rank_hist_body<W, IT, DB>(hw_grid(), keys, n_ptr, ncap, rstride, binom, nel1, rank, hist, shift);
 }
 
 // one block per digit: exclusive scan of that digit's per-block counts
-__global__ void __launch_bounds__(kBlock) k_radix_scan(uint32_t* hist, int nb, uint32_t* digit_tot) {
+__device__ __forceinline__ void k_radix_scan_body(const Grid& g_, uint32_t* hist, int nb, uint32_t* digit_tot) {
   __shared__ int64_t red[kBlock / kWave];
-  uint32_t* row = hist + (size_t)blockIdx.x * nb;
+  uint32_t* row = hist + (size_t)g_.b * nb;
   int per = (nb + kBlock - 1) / kBlock;
   int b0 = threadIdx.x * per;
   int64_t loc = 0;
@@ -1322,7 +1351,10 @@ __global__ void __launch_bounds__(kBlock) k_radix_scan(uint32_t* hist, int nb, u
       row[b0 + i] = (uint32_t)pre;
       pre += v;
     }
-  if (threadIdx.x == 0) digit_tot[blockIdx.x] = (uint32_t)tot;
+  if (threadIdx.x == 0) digit_tot[g_.b] = (uint32_t)tot;
+}
+__global__ void __launch_bounds__(kBlock) k_radix_scan(uint32_t* hist, int nb, uint32_t* digit_tot) {
+  k_radix_scan_body(hw_grid(), hist, nb, digit_tot);
 }
 
 // Stable scatter of one digit, in tiles of IT*kBlock elements: the
@@ -1330,10 +1362,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_scan(uint32_t* hist, int nb, u
 // and rounds, in element order), reordered by digit in LDS, then written out
 // so that consecutive threads write consecutive positions of each digit's run.
 template <typename K, int NW, bool FLIP, int IT, int DB>
-__global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstride, const uint32_t* iin, K* kout,
-                                                          uint32_t* iout, const unsigned long long* n_ptr,
-                                                          int64_t ncap, int word, int shift, const uint32_t* hist,
-                                                          const uint32_t* digit_tot) {
+__device__ __forceinline__ void k_radix_scatter_body(const Grid& g_, const K* kin, int kstride, const uint32_t* iin, K* kout, uint32_t* iout, const unsigned long long* n_ptr, int64_t ncap, int word, int shift, const uint32_t* hist, const uint32_t* digit_tot) {
   constexpr int RB = 1 << DB;          // digit values
   constexpr int DPT = RB / kBlock;     // digits owned by each thread (consecutive)
   constexpr int TILE = IT * kBlock;
@@ -1356,7 +1385,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
 #pragma unroll
     for (int q = 0; q < DPT; ++q) {
       const int d = d0 + q;
-      run[d] = (uint32_t)base + hist[(size_t)d * gridDim.x + blockIdx.x];
+      run[d] = (uint32_t)base + hist[(size_t)d * g_.nb + g_.b];
       base += digit_tot[d];
       cnt[d] = 0;
       for (int w = 0; w < kBlock / kWave; ++w) wcnt[w][d] = 0;
@@ -1366,7 +1395,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
   int64_t n = (int64_t)*n_ptr;
   if (n > ncap) n = ncap;
   int64_t begin, end;
-  radix_chunk<IT>(n, begin, end);
+  radix_chunk<IT>(g_, n, begin, end);
   const uint64_t ltmask = lanemask_lt();
   for (int64_t tb = begin; tb < end; tb += TILE) {
     K k[IT][NW];
@@ -1453,6 +1482,10 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
     }
     __syncthreads();
   }
+}
+template <typename K, int NW, bool FLIP, int IT, int DB>
+__global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstride, const uint32_t* iin, K* kout, uint32_t* iout, const unsigned long long* n_ptr, int64_t ncap, int word, int shift, const uint32_t* hist, const uint32_t* digit_tot) {
+  k_radix_scatter_body<K, NW, FLIP, IT, DB>(hw_grid(), kin, kstride, iin, kout, iout, n_ptr, ncap, word, shift, hist, digit_tot);
 }
 
 // ------------------------------------------------------- partition sort
@@ -1542,8 +1575,7 @@ __device__ inline void seg_pass(const uint32_t* sk, const uint32_t* si, uint32_t
 
 // block b sorts segment b (256 blocks of kSegThreads); digit_tot holds the
 // segment sizes, the counts of the top-byte pass
-__global__ void __launch_bounds__(kSegThreads) k_seg_sort(uint32_t* kin, uint32_t* iin, uint32_t* kout,
-                                                          uint32_t* iout, const uint32_t* digit_tot) {
+__device__ __forceinline__ void k_seg_sort_body(const Grid& g_, uint32_t* kin, uint32_t* iin, uint32_t* kout, uint32_t* iout, const uint32_t* digit_tot) {
   __shared__ uint32_t ka[kSegCap], ia[kSegCap], kb[kSegCap], ib[kSegCap];
   __shared__ SegScratch sc;
   __shared__ int64_t seg[2];
@@ -1551,9 +1583,9 @@ __global__ void __launch_bounds__(kSegThreads) k_seg_sort(uint32_t* kin, uint32_
   __syncthreads();
   if (threadIdx.x == 0) {  // this segment's start: the sizes of the segments before it
     int64_t st = 0;
-    for (int b = 0; b < (int)blockIdx.x; ++b) st += sc.cnt[b];
+    for (int b = 0; b < (int)g_.b; ++b) st += sc.cnt[b];
     seg[0] = st;
-    seg[1] = sc.cnt[blockIdx.x];
+    seg[1] = sc.cnt[g_.b];
   }
   __syncthreads();
   const int64_t start = seg[0], n = seg[1];
@@ -1577,6 +1609,9 @@ __global__ void __launch_bounds__(kSegThreads) k_seg_sort(uint32_t* kin, uint32_
     seg_pass(ok, oi, gk, gi, n, 8, &sc);
     seg_pass(gk, gi, ok, oi, n, 16, &sc);
   }
+}
+__global__ void __launch_bounds__(kSegThreads) k_seg_sort(uint32_t* kin, uint32_t* iin, uint32_t* kout, uint32_t* iout, const uint32_t* digit_tot) {
+  k_seg_sort_body(hw_grid(), kin, iin, kout, iout, digit_tot);
 }
 
 // ------------------------------------------------------------- merging
@@ -1635,19 +1670,16 @@ __device__ inline bool key_eq(const K* a, const K* b) {
 // point: run heads sum their runs as k_merge_annihilate does, and the list
 // goes to memory; no statistics (k_merge_annihilate counts them).
 template <int W, int R, typename K, int NW>
-__global__ void __launch_bounds__(kBlock) k_merge_list(DevSys s, Counters* cnt, const K* skey, const uint32_t* sidx,
-                                                       const uint64_t* sp_rec, int64_t scap, uint64_t* u_key,
-                                                       double* u_sgn, uint32_t* u_flag,
-                                                       unsigned long long* u_count) {
+__device__ __forceinline__ void k_merge_list_body(const Grid& g_, const DevSys& s, Counters* cnt, const K* skey, const uint32_t* sidx, const uint64_t* sp_rec, int64_t scap, uint64_t* u_key, double* u_sgn, uint32_t* u_flag, unsigned long long* u_count) {
   if (cnt->halt) return;
   __shared__ ListStage<W, R> ls;
   stage_init(&ls.st);
   const int S = R == 1 ? 1 : s.S;
   int64_t n = (int64_t)cnt->n_spawned;
   if (n > scap) n = scap;
-  int64_t trips = grid_trips(n);
+  int64_t trips = grid_trips(g_, n);
   for (int64_t t = 0; t < trips; ++t) {
-    int64_t i = grid_elem(t);
+    int64_t i = grid_elem(g_, t);
     bool keep = false;
     double sum[R];
     uint32_t flag = 0;
@@ -1690,6 +1722,10 @@ __global__ void __launch_bounds__(kBlock) k_merge_list(DevSys s, Counters* cnt, 
   }
   list_flush<W, R>(&ls, u_count, u_key, u_sgn, u_flag, S, true);
 }
+template <int W, int R, typename K, int NW>
+__global__ void __launch_bounds__(kBlock) k_merge_list(DevSys s, Counters* cnt, const K* skey, const uint32_t* sidx, const uint64_t* sp_rec, int64_t scap, uint64_t* u_key, double* u_sgn, uint32_t* u_flag, unsigned long long* u_count) {
+  k_merge_list_body<W, R, K, NW>(hw_grid(), s, cnt, skey, sidx, sp_rec, scap, u_key, u_sgn, u_flag, u_count);
+}
 
 // CompressSpawnedList (Annihilation.F90:249-515) + FindResidualParticle
 // (:551-634), then AnnihilateSpawnedParts (:965-1352), in one pass over the
@@ -1699,10 +1735,7 @@ __global__ void __launch_bounds__(kBlock) k_merge_list(DevSys s, Counters* cnt, 
 // one lane), absent ones go to the new list.  The lookups of one wave overlap
 // the record gathers of the others, and the merged list never goes to memory.
 template <int W, int R, typename K, int NW>
-__global__ void __launch_bounds__(kBlock) k_merge_annihilate(DevSys s, Walk wk, Counters* cnt, double* part,
-                                                             const K* skey, const uint32_t* sidx,
-                                                             const uint64_t* sp_rec, int64_t scap, uint64_t* n_key,
-                                                             double* n_sgn, uint32_t* n_flag, const IterParams* prm) {
+__device__ __forceinline__ void k_merge_annihilate_body(const Grid& g_, const DevSys& s, const Walk& wk, Counters* cnt, double* part, const K* skey, const uint32_t* sidx, const uint64_t* sp_rec, int64_t scap, uint64_t* n_key, double* n_sgn, uint32_t* n_flag, const IterParams* prm) {
   if (cnt->halt) return;
   const uint32_t iter = prm->iter;
   __shared__ double acc_lds[kStatStride];
@@ -1717,11 +1750,11 @@ __global__ void __launch_bounds__(kBlock) k_merge_annihilate(DevSys s, Walk wk, 
   const int S = R == 1 ? 1 : s.S;
   int64_t n = (int64_t)cnt->n_spawned;
   if (n > scap) n = scap;
-  int64_t trips = grid_trips(n);
+  int64_t trips = grid_trips(g_, n);
   double ann[R], st_ab[R], st_an[R], st_rm[R], st_bn[R];
   for (int r = 0; r < S; ++r) ann[r] = st_ab[r] = st_an[r] = st_rm[r] = st_bn[r] = 0.0;
   for (int64_t t = 0; t < trips; ++t) {
-    int64_t i = grid_elem(t);
+    int64_t i = grid_elem(g_, t);
     bool keep = false;
     double ss[R];
     uint32_t sflag = 0;
@@ -1844,18 +1877,25 @@ __global__ void __launch_bounds__(kBlock) k_merge_annihilate(DevSys s, Walk wk, 
   }
   __syncthreads();
   if (threadIdx.x == 0 && blk_unique)
-    atomicAdd(&cnt->uniq_part[blockIdx.x % Counters::kSpread].v, (unsigned long long)blk_unique);
-  bs.flush(part);
+    atomicAdd(&cnt->uniq_part[g_.b % Counters::kSpread].v, (unsigned long long)blk_unique);
+  bs.flush(part, g_.b);
+}
+template <int W, int R, typename K, int NW>
+__global__ void __launch_bounds__(kBlock) k_merge_annihilate(DevSys s, Walk wk, Counters* cnt, double* part, const K* skey, const uint32_t* sidx, const uint64_t* sp_rec, int64_t scap, uint64_t* n_key, double* n_sgn, uint32_t* n_flag, const IterParams* prm) {
+  k_merge_annihilate_body<W, R, K, NW>(hw_grid(), s, wk, cnt, part, skey, sidx, sp_rec, scap, n_key, n_sgn, n_flag, prm);
 }
 
 // ------------------------------------------------------ semistochastic core
 // Core amplitudes before the projection (no death acts on core dets, so the
 // signs after the prepass are the iteration's starting amplitudes).
 template <int R>
-__global__ void __launch_bounds__(kBlock) k_core_gather(const int64_t* core_slot, Walk wk, int S,
-                                                        int64_t ncore, double* x) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncore; i += (int64_t)gridDim.x * blockDim.x)
+__device__ __forceinline__ void k_core_gather_body(const Grid& g_, const int64_t* core_slot, const Walk& wk, int S, int64_t ncore, double* x) {
+  for (int64_t i = (int64_t)g_.b * blockDim.x + threadIdx.x; i < ncore; i += (int64_t)g_.nb * blockDim.x)
     for (int r = 0; r < S; ++r) x[i * S + r] = wk.sgn(core_slot[i], r);
+}
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_core_gather(const int64_t* core_slot, Walk wk, int S, int64_t ncore, double* x) {
+  k_core_gather_body<R>(hw_grid(), core_slot, wk, S, ncore, x);
 }
 
 // determ_projection (semi_stoch_procs.F90:105-241), one wave per core row:
@@ -1863,14 +1903,12 @@ __global__ void __launch_bounds__(kBlock) k_core_gather(const int64_t* core_slot
 // butterfly acc = acc + shfl_xor(acc, o) for o = 32..1 combines the lanes, and
 // y = (acc + S*x_i) * tau.  The CPU restatement sums in exactly this order.
 template <int R, typename VT>
-__global__ void __launch_bounds__(kBlock) k_core_spmv(const int64_t* rowptr, const int32_t* cols, const VT* vals,
-                                                      const double* x, double* y, int64_t ncore, int S,
-                                                      const IterParams* prm) {
+__device__ __forceinline__ void k_core_spmv_body(const Grid& g_, const int64_t* rowptr, const int32_t* cols, const VT* vals, const double* x, double* y, int64_t ncore, int S, const IterParams* prm) {
   const double tau = prm->tau;
   const double* sftv = prm->sft.v;
   const int lane = lane_id();
-  const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
-  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < ncore; i += nwaves) {
+  const int64_t nwaves = (int64_t)g_.nb * blockDim.x / kWave;
+  for (int64_t i = ((int64_t)g_.b * blockDim.x + threadIdx.x) / kWave; i < ncore; i += nwaves) {
     double acc[R];
     for (int r = 0; r < S; ++r) acc[r] = 0.0;
     const int64_t e = rowptr[i + 1];
@@ -1887,21 +1925,24 @@ __global__ void __launch_bounds__(kBlock) k_core_spmv(const int64_t* rowptr, con
       for (int r = 0; r < S; ++r) y[i * S + r] = (acc[r] + sftv[r] * x[i * S + r]) * tau;
   }
 }
+template <int R, typename VT>
+__global__ void __launch_bounds__(kBlock) k_core_spmv(const int64_t* rowptr, const int32_t* cols, const VT* vals, const double* x, double* y, int64_t ncore, int S, const IterParams* prm) {
+  k_core_spmv_body<R, VT>(hw_grid(), rowptr, cols, vals, x, y, ncore, S, prm);
+}
 
 // deterministic_annihilation (Annihilation.F90:930-963): the projected
 // amplitudes join the core dets' signs (births and annihilations counted)
 template <int R>
-__global__ void __launch_bounds__(kBlock) k_core_apply(DevSys s, Walk wk, const Counters* cnt, const int64_t* core_slot,
-                                                       const double* y, int64_t ncore, double* part) {
+__device__ __forceinline__ void k_core_apply_body(const Grid& g_, const DevSys& s, const Walk& wk, const Counters* cnt, const int64_t* core_slot, const double* y, int64_t ncore, double* part) {
   if (cnt->halt) return;
   __shared__ double acc_lds[kStatStride];
   BlockStats bs{acc_lds};
   bs.zero();
   const int S = R == 1 ? 1 : s.S;
   LaneSum<R> born, ann;
-  int64_t trips = grid_trips(ncore);
+  int64_t trips = grid_trips(g_, ncore);
   for (int64_t t = 0; t < trips; ++t) {
-    int64_t i = grid_elem(t);
+    int64_t i = grid_elem(g_, t);
     bool act = i < ncore;
     int64_t slot = act ? core_slot[i] : 0;
     for (int r = 0; r < S; ++r) {
@@ -1921,7 +1962,11 @@ __global__ void __launch_bounds__(kBlock) k_core_apply(DevSys s, Walk wk, const 
   }
   born.done(bs, soff(F_NBORN, 0));
   ann.done(bs, soff(F_NANNIHIL, 0));
-  bs.flush(part);
+  bs.flush(part, g_.b);
+}
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_core_apply(DevSys s, Walk wk, const Counters* cnt, const int64_t* core_slot, const double* y, int64_t ncore, double* part) {
+  k_core_apply_body<R>(hw_grid(), s, wk, cnt, core_slot, y, ncore, part);
 }
 
 // core Hamiltonian rows (calc_determ_hamil, fast_determ_hamil.F90:892-1548):
@@ -1929,12 +1974,10 @@ __global__ void __launch_bounds__(kBlock) k_core_apply(DevSys s, Walk wk, const 
 // connected pairs (diagonal H_ii - Hii), off-diagonal zeros dropped.  Pass 0
 // counts, pass 1 writes columns and values at rowptr[i].
 template <int W, typename VT>
-__global__ void __launch_bounds__(kBlock) k_core_rows(DevSys s, const uint64_t* keys, int64_t ncore, int pass,
-                                                      int64_t* count, const int64_t* rowptr, int32_t* cols,
-                                                      VT* vals) {
+__device__ __forceinline__ void k_core_rows_body(const Grid& g_, const DevSys& s, const uint64_t* keys, int64_t ncore, int pass, int64_t* count, const int64_t* rowptr, int32_t* cols, VT* vals) {
   const int lane = lane_id();
-  const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
-  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < ncore; i += nwaves) {
+  const int64_t nwaves = (int64_t)g_.nb * blockDim.x / kWave;
+  for (int64_t i = ((int64_t)g_.b * blockDim.x + threadIdx.x) / kWave; i < ncore; i += nwaves) {
     const uint64_t* di = keys + i * W;
     int64_t base = pass ? rowptr[i] : 0;
     for (int64_t j0 = 0; j0 < ncore; j0 += kWave) {
@@ -1959,15 +2002,17 @@ __global__ void __launch_bounds__(kBlock) k_core_rows(DevSys s, const uint64_t* 
     if (!pass && lane == 0) count[i] = base;
   }
 }
+template <int W, typename VT>
+__global__ void __launch_bounds__(kBlock) k_core_rows(DevSys s, const uint64_t* keys, int64_t ncore, int pass, int64_t* count, const int64_t* rowptr, int32_t* cols, VT* vals) {
+  k_core_rows_body<W, VT>(hw_grid(), s, keys, ncore, pass, count, rowptr, cols, vals);
+}
 
 // place the core dets: resident ones get the core (and initiator) flags and
 // their slot; absent ones are appended to the new list with zero signs
 template <int W>
-__global__ void __launch_bounds__(kBlock) k_core_place(DevSys s, Walk wk, Counters* cnt, const uint64_t* keys,
-                                                       int64_t ncore, uint32_t cflags, int64_t* core_slot,
-                                                       uint64_t* n_key, double* n_sgn, uint32_t* n_flag) {
+__device__ __forceinline__ void k_core_place_body(const Grid& g_, const DevSys& s, const Walk& wk, Counters* cnt, const uint64_t* keys, int64_t ncore, uint32_t cflags, int64_t* core_slot, uint64_t* n_key, double* n_sgn, uint32_t* n_flag) {
   const int S = s.S;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncore; i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = (int64_t)g_.b * blockDim.x + threadIdx.x; i < ncore; i += (int64_t)g_.nb * blockDim.x) {
     const uint64_t* d = keys + i * W;
     int64_t home = find_walker_hash<W>(d, s.rh, s.L) - 1;
     int64_t pos = hash_lookup<W>(wk, d, home, cnt);
@@ -1984,12 +2029,14 @@ __global__ void __launch_bounds__(kBlock) k_core_place(DevSys s, Walk wk, Counte
     }
   }
 }
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_core_place(DevSys s, Walk wk, Counters* cnt, const uint64_t* keys, int64_t ncore, uint32_t cflags, int64_t* core_slot, uint64_t* n_key, double* n_sgn, uint32_t* n_flag) {
+  k_core_place_body<W>(hw_grid(), s, wk, cnt, keys, ncore, cflags, core_slot, n_key, n_sgn, n_flag);
+}
 
 // AddNewHashDet (load_balancer.fpp:514-629) for the new list
 template <int W, int R>
-__global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* cnt, const uint64_t* n_key,
-                                                   const double* n_sgn, const uint32_t* n_flag,
-                                                   const unsigned long long* n_ptr) {
+__device__ __forceinline__ void k_insert_body(const Grid& g_, const DevSys& s, const Walk& wk, Counters* cnt, const uint64_t* n_key, const double* n_sgn, const uint32_t* n_flag, const unsigned long long* n_ptr) {
   if (cnt->halt) return;
   const int S = R == 1 ? 1 : s.S;
   int64_t nn = (int64_t)*n_ptr;
@@ -1999,10 +2046,21 @@ __global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* 
   const unsigned long long tail = cnt->free_tail;
   const int64_t avail = (int64_t)(tail - head);
   const int64_t ns0 = (int64_t)cnt->nslots;
-  if (blockIdx.x == 0 && threadIdx.x == 0) cnt->free_tail_ins = tail;  // for the slot bookkeeping
+  if (g_.b == 0 && threadIdx.x == 0) cnt->free_tail_ins = tail;  // for the slot bookkeeping
+  // MaxWalkers would be passed (AddNewHashDet's stop, load_balancer.fpp:540-544):
+  // no new determinant is placed, so the walker list the halted engine keeps
+  // does not depend on the order of the new list
+  if (ns0 + (nn > avail ? nn - avail : 0) > wk.cap) {
+    if (g_.b == 0 && threadIdx.x == 0) {
+      atomicOr(&cnt->err, 1u << NGPU_ERR_WALKERS_FULL);
+      cnt->n_fail = (unsigned long long)nn;
+      cnt->ins_refused = 1u;
+    }
+    return;
+  }
   unsigned long long my_reused = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = (int64_t)g_.b * blockDim.x + threadIdx.x; i < nn;
+       i += (int64_t)g_.nb * blockDim.x) {
     int64_t slot = i < avail ? wk.freeq[(head + (unsigned long long)i) % (unsigned long long)wk.cap]
                              : ns0 + (i - avail);
     if (slot >= wk.cap) {
@@ -2040,7 +2098,11 @@ __global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* 
   if (c && lane_id() == 0) atomicAdd(&blk_reused, c);
   __syncthreads();
   if (threadIdx.x == 0 && blk_reused)
-    atomicAdd(&cnt->reuse_part[blockIdx.x % Counters::kSpread].v, (unsigned long long)blk_reused);
+    atomicAdd(&cnt->reuse_part[g_.b % Counters::kSpread].v, (unsigned long long)blk_reused);
+}
+template <int W, int R>
+__global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* cnt, const uint64_t* n_key, const double* n_sgn, const uint32_t* n_flag, const unsigned long long* n_ptr) {
+  k_insert_body<W, R>(hw_grid(), s, wk, cnt, n_key, n_sgn, n_flag, n_ptr);
 }
 
 // bookkeeping of k_insert: queued slots used, fresh slots appended (beyond
@@ -2052,6 +2114,11 @@ __global__ void __launch_bounds__(kBlock) k_insert(DevSys s, Walk wk, Counters* 
 // bookkeeping (k_end_iter, or k_post_insert on the set-up paths) applies it.
 __device__ inline void slots_after_insert(const Counters* cnt, unsigned long long nn, int64_t cap,
                                           unsigned long long& used, unsigned long long& ns) {
+  if (cnt->ins_refused) {  // nothing was placed
+    used = 0;
+    ns = cnt->nslots;
+    return;
+  }
   unsigned long long avail = cnt->free_tail_ins - cnt->free_head;
   used = nn < avail ? nn : avail;
   ns = cnt->nslots + (nn - used);
@@ -2078,8 +2145,7 @@ __global__ void k_post_insert(Counters* cnt, const unsigned long long* n_ptr, in
 
 // CalcHashTableStats (load_balancer.fpp:646-805)
 template <int W, int R>
-__global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* cnt, double* part,
-                                                  const IterParams* prm) {
+__device__ __forceinline__ void k_stats_body(const Grid& g_, const DevSys& s, const Walk& wk, Counters* cnt, double* part, const IterParams* prm) {
   if (cnt->halt) return;
   const uint32_t iter = prm->iter;
   __shared__ double acc_lds[kStatStride];
@@ -2094,9 +2160,9 @@ __global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* c
   unsigned long long my_occ = 0;
   double tp[R], nq[R], rmv[R], bn[R];
   for (int r = 0; r < S; ++r) tp[r] = nq[r] = rmv[r] = bn[r] = 0.0;
-  int64_t trips = grid_trips(nslots);
+  int64_t trips = grid_trips(g_, nslots);
   for (int64_t t = 0; t < trips; ++t) {
-    int64_t slot = grid_elem(t);
+    int64_t slot = grid_elem(g_, t);
     bool rm = false;
     if (slot < nslots) {
       uint32_t f = wk.flags(slot);
@@ -2156,7 +2222,11 @@ __global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* c
     bs.add(soff(F_NBORN, r), bn[r]);
   }
   bs.add(X_NOCC, (double)my_occ);
-  bs.flush(part);
+  bs.flush(part, g_.b);
+}
+template <int W, int R>
+__global__ void __launch_bounds__(kBlock) k_stats(DevSys s, Walk wk, Counters* cnt, double* part, const IterParams* prm) {
+  k_stats_body<W, R>(hw_grid(), s, wk, cnt, part, prm);
 }
 
 // ---------------------------------------------------------- hash upkeep
@@ -2212,13 +2282,21 @@ __device__ inline void hash_account(Counters* cnt, int64_t entries) {
 // in-place rebuild for the set-up paths (upload, core placement, explicit
 // lists); iterations rebuild inside k_prepass and k_scan_apply instead
 template <int W>
+__device__ __forceinline__ void k_hash_clear_body(const Grid& g_, const Walk& wk, const Counters* cnt) {
+  if (hash_rebuild_due(cnt, wk.L * (int64_t)kBucketEntries)) hash_clear_part<W>(g_, wk);
+}
+template <int W>
 __global__ void __launch_bounds__(kBlock) k_hash_clear(Walk wk, const Counters* cnt) {
-  if (hash_rebuild_due(cnt, wk.L * (int64_t)kBucketEntries)) hash_clear_part<W>(wk);
+  k_hash_clear_body<W>(hw_grid(), wk, cnt);
 }
 
 template <int W>
+__device__ __forceinline__ void k_hash_refill_body(const Grid& g_, const DevSys& s, const Walk& wk, Counters* cnt) {
+  if (hash_rebuild_due(cnt, wk.L * (int64_t)kBucketEntries)) hash_refill_part<W>(g_, s, wk, cnt, (int64_t)cnt->nslots);
+}
+template <int W>
 __global__ void __launch_bounds__(kBlock) k_hash_refill(DevSys s, Walk wk, Counters* cnt) {
-  if (hash_rebuild_due(cnt, wk.L * (int64_t)kBucketEntries)) hash_refill_part<W>(s, wk, cnt, (int64_t)cnt->nslots);
+  k_hash_refill_body<W>(hw_grid(), s, wk, cnt);
 }
 
 __global__ void k_hash_done(Counters* cnt, int64_t entries) { hash_done(cnt, entries); }

@@ -50,8 +50,8 @@ class Engine:
             if spec.excitgen == "uniform":
                 check(self.lib.ngpu_set_excitgen_uniform(h, spec.psingles), "ngpu_set_excitgen_uniform")
             else:
-                check(self.lib.ngpu_set_excitgen(h, spec.singles, spec.particles, spec.psingles, spec.pparallel),
-                      "ngpu_set_excitgen")
+                check(self.lib.ngpu_set_excitgen_pchb(h, spec.singles, spec.particles, spec.holes, spec.psingles,
+                                                      spec.pparallel), "ngpu_set_excitgen_pchb")
         if spec.spawn_fp32:
             check(self.lib.ngpu_set_spawn_precision(h, 1), "ngpu_set_spawn_precision")
         self.random_hash = spec.random_hash
@@ -168,6 +168,42 @@ class Engine:
         self.ncore = dets.shape[0]
         return nnz.value
 
+    def set_core_build(self, method):
+        """How set_core builds the core Hamiltonian: CORE_BUILD_AUTO (the
+        default), CORE_BUILD_JOIN (connections by subset join) or
+        CORE_BUILD_SCAN (every pair tested)."""
+        check(self.lib.ngpu_set_core_build(self.h, int(method)), "ngpu_set_core_build")
+
+    def core_hamil(self):
+        """The stored core Hamiltonian as CSR: (rowptr, cols, vals)."""
+        nnz = ctypes.c_int64()
+        check(self.lib.ngpu_get_core_hamil(self.h, None, None, None, 0, ctypes.byref(nnz)), "ngpu_get_core_hamil")
+        rowptr = np.zeros(self.ncore + 1, dtype=np.int64)
+        cols = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        vals = np.zeros(max(nnz.value, 1), dtype=np.float64)
+        check(self.lib.ngpu_get_core_hamil(self.h, ptr(rowptr), ptr(cols), ptr(vals), nnz.value, ctypes.byref(nnz)),
+              "ngpu_get_core_hamil")
+        return rowptr, cols[:nnz.value].copy(), vals[:nnz.value].copy()
+
+    def most_populated(self, target):
+        """The target most populated determinants (ngpu_most_populated), weight
+        descending, equal weights in determinant order."""
+        out = np.zeros((max(int(target), 1), self.W), dtype=np.int64)
+        n = ctypes.c_int64()
+        check(self.lib.ngpu_most_populated(self.h, int(target), ptr(out), ctypes.byref(n)), "ngpu_most_populated")
+        return out[:n.value].copy()
+
+    def select_core(self, target, fp32=False):
+        """Core space of the target most populated determinants, chosen on the
+        device (ngpu_select_core); returns (core determinants, stored H_ij)."""
+        out = np.zeros((max(int(target), 1), self.W), dtype=np.int64)
+        n = ctypes.c_int64()
+        nnz = ctypes.c_int64()
+        check(self.lib.ngpu_select_core(self.h, int(target), int(bool(fp32)), ptr(out), ctypes.byref(n),
+                                        ctypes.byref(nnz)), "ngpu_select_core")
+        self.ncore = n.value
+        return out[:n.value].copy(), nnz.value
+
     def error(self):
         """Sticky error of the iteration chain: (status, iteration that failed),
         (0, 0) while the engine runs (ngpu_get_error)."""
@@ -201,7 +237,8 @@ class Engine:
         """Replay one captured graph per iteration (ngpu_set_graph)."""
         check(self.lib.ngpu_set_graph(self.h, int(on)), "ngpu_set_graph")
 
-    ALL_STAGES = (1 << 9) - 1
+    N_STAGES = 10
+    ALL_STAGES = (1 << N_STAGES) - 1
 
     def set_profiling(self, on=True):
         """Stage timing: True for every stage, False for none, or a bit mask of stages."""
@@ -209,9 +246,10 @@ class Engine:
         check(self.lib.ngpu_set_profiling(self.h, mask), "ngpu_set_profiling")
 
     def stage_times(self):
-        ms = np.zeros(9)
+        ms = np.zeros(self.N_STAGES)
         n = ctypes.c_int64()
-        check(self.lib.ngpu_get_stage_times(self.h, ptr(ms), 9, ctypes.byref(n)), "ngpu_get_stage_times")
+        check(self.lib.ngpu_get_stage_times(self.h, ptr(ms), self.N_STAGES, ctypes.byref(n)),
+              "ngpu_get_stage_times")
         return ms, n.value
 
     # -- batched plugin-surface entry points
@@ -317,6 +355,31 @@ class Batch:
                 inp.diag_sft[r] = float(sh[r])
             inp.iter = int(its[i]) & 0xFFFFFFFF
         check(self.lib.ngpu_batch_iterate(self.b, self._ins), "ngpu_batch_iterate")
+
+    def groups(self):
+        """Launch group of each engine (ngpu_batch_get_groups): engines of one
+        group share every launch of the iteration chain."""
+        g = np.zeros(len(self.engines), dtype=np.int32)
+        n = ctypes.c_int32()
+        check(self.lib.ngpu_batch_get_groups(self.b, ptr(g), ctypes.byref(n)), "ngpu_batch_get_groups")
+        return g, n.value
+
+    def set_graph(self, on=True):
+        """Replay one captured graph per iteration (True) or launch directly."""
+        check(self.lib.ngpu_batch_set_graph(self.b, int(bool(on))), "ngpu_batch_set_graph")
+
+    def set_profiling(self, on=True):
+        """Stage timing of the batch's launches (while launching directly):
+        True for every stage, False for none, or a bit mask of stages."""
+        mask = Engine.ALL_STAGES if on is True else (0 if on is False else int(on))
+        check(self.lib.ngpu_batch_set_profiling(self.b, mask), "ngpu_batch_set_profiling")
+
+    def group_stage_times(self, group):
+        """(ms per stage summed over the timed iterations, iterations) of one
+        launch group, held by the group's first engine."""
+        g, _ = self.groups()
+        first = int(np.flatnonzero(g == group)[0])
+        return self.engines[first].stage_times()
 
     def close(self):
         if getattr(self, "b", None):

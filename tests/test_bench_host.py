@@ -71,3 +71,35 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert msg["n_gpus"] == 2 and "one GPU" in msg["error"]
+
+
+def test_window_counts_cover_every_timed_step():
+    """The bench's per-step counts come from the driver's running totals, so
+    a timed window of 20 steps after 5 warm-up steps (not aligned to the
+    10-step update cycle) counts all 20 iterations."""
+    from test_driver_host import ScriptedBackend
+    from neci_stable_amd.fcimc import FciMCPar
+    d = FciMCPar(ScriptedBackend(), tau=0.01, target_walkers=10.0, steps_shift=10, shift_damp=0.1)
+    d.run(5)
+    t0 = dict(d.totals)
+    d.run(20)
+    c = bench.window_counts(t0, d.totals)
+    its = list(range(6, 26))
+    assert c["Na"] == pytest.approx(sum(1000 + i for i in its) / 20)
+    assert c["walkers"] == pytest.approx(c["Na"])   # one attempt per walker in the script
+    assert c["Ns"] == pytest.approx(sum(its) / 20)
+
+
+def test_fp32_values_and_core_stage_bytes():
+    """FP32 storage takes 4 bytes off every spawned amplitude written or
+    gathered and off every core Hamiltonian element; the core stage reads
+    each stored element (value + int32 column) once and the core vector."""
+    c = dict(COUNTS, Nc=1.18e7, Ncore=5e4)
+    nsd = 0.9 * COUNTS["Ns"]
+    assert (bench.algorithmic_bytes("spawn", c) - bench.algorithmic_bytes("spawn", c, val_bytes=4)
+            == pytest.approx(4 * nsd))
+    assert (bench.algorithmic_bytes("annihilate", c) - bench.algorithmic_bytes("annihilate", c, val_bytes=4)
+            == pytest.approx(4 * COUNTS["Ns"]))
+    assert bench.algorithmic_bytes("core", c) == pytest.approx(1.18e7 * 12 + 2 * 8 * 5e4)
+    assert bench.algorithmic_bytes("core", c, val_bytes=4) == pytest.approx(1.18e7 * 8 + 2 * 8 * 5e4)
+    assert bench.algorithmic_bytes("core", COUNTS) == 0.0   # no core space

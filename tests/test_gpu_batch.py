@@ -120,3 +120,93 @@ def test_batched_engine_refuses_own_iterate():
     a.iterate(1, tau, 0.3)
     a.close()
     b.close()
+
+
+def test_batch_groups_share_launches():
+    """Engines with the same kernel instances form one launch group (the two
+    one-replica Hubbard chains; each molecular replica count its own group),
+    and direct group launches with stage timing give the graph replay's
+    results."""
+    from neci_stable_amd.engine import Batch
+    members = [("hubbard10", 1, 3), ("mol8o6e", 2, 4), ("hubbard14", 1, 5), ("mol8o6e", 1, 6)]
+    alone, batched = [], []
+    for name, nrep, seed in members:
+        _, tau, a, b = engine_pair(name, nrep, seed)
+        alone.append((a, tau, nrep))
+        batched.append(b)
+    batch = Batch(batched)
+    g, n = batch.groups()
+    assert n == 3 and g[0] == g[2] and len({int(g[0]), int(g[1]), int(g[3])}) == 3
+    shifts = [np.linspace(0.2, 0.4, nrep) for _, _, nrep in alone]
+    for it in range(1, 31):
+        if it == 11:
+            batch.set_graph(False)
+            batch.set_profiling(True)
+        if it == 21:  # the times are read before timing is switched off (which clears them)
+            ms, nit = batch.group_stage_times(int(g[0]))
+            assert nit == 10 and ms[2] > 0.0 and ms[6] > 0.0  # spawn and merge-annihilation stages timed
+            batch.set_profiling(False)
+            batch.set_graph(True)
+        batch.iterate([it] * len(batched), [t for _, t, _ in alone], shifts)
+        for i, (a, tau, nrep) in enumerate(alone):
+            same_stats(a.iterate(it, tau, shifts[i]), batched[i].stats(), nrep, f"engine {i} iteration {it}")
+    for i, (a, _, _) in enumerate(alone):
+        assert same_walkers(a, batched[i]) > 20
+        a.close()
+    batch.close()
+    for b in batched:
+        b.close()
+
+
+def test_batched_mix_of_molecules_and_chains():
+    """A reduced form of the batched workload: two replicas (distinct seeds)
+    of the 14e/30o molecular Hamiltonian (seed 7, rank 60; PCHB doubles with
+    UNIF-UNIF particles, heat-bath singles, initiator 3, tau 0.002: about
+    650 occupied determinants after the 15 growth iterations and 6500 at the
+    end, in the oracle) with a 300-determinant
+    core space, Hamiltonian and spawned amplitudes stored in FP32, plus
+    half-filled Hubbard chains of 12, 13, 17 and 18 sites (colex-rank and
+    word-key sorts).  Every batched problem follows the same problem iterated
+    alone, iteration by iteration."""
+    from neci_stable_amd import HubbardChain, Spec
+    from neci_stable_amd.engine import Batch, Engine
+    from common import molecule
+    mol = molecule(30, 14, 60, 7)
+    specs = [(Spec(mol, init_walkers=2e5, seed=700 + r, particles=0, psingles=0.1, memory_fac_spawn=6.0), 0.002,
+              200.0) for r in range(2)]
+    for L, u in ((12, 3.0), (13, 5.5), (17, 2.5), (18, 7.0)):
+        specs.append((Spec(HubbardChain(L, L, u, bhub=-1.0), init_walkers=2e4, seed=800 + L), 0.01, 30.0))
+    pairs = []
+    for spec, tau, n0 in specs:
+        a, b = Engine(spec), Engine(spec)
+        a.start_single_part(n0)
+        b.start_single_part(n0)
+        pairs.append([a, b, tau])
+    for it in range(1, 16):  # grow both twins alone
+        for a, b, tau in pairs:
+            same_stats(a.iterate(it, tau, 0.0), b.iterate(it, tau, 0.0), 1, f"growth {it}")
+    for k in range(2):       # core space and FP32 storage for the molecules
+        a, b, _ = pairs[k]
+        d, sg, _, _, _ = a.download()
+        core = np.ascontiguousarray(d[np.sort(np.argsort(-np.abs(sg[:, 0]), kind="stable")[:300])])
+        assert a.set_core(core, fp32=True) == b.set_core(core, fp32=True)
+        a.set_spawn_precision(True)
+        b.set_spawn_precision(True)
+    batch = Batch([b for _, b, _ in pairs])
+    g, n = batch.groups()
+    assert g[0] == g[1] and n >= 3
+    print(f"launch groups {list(g)}", flush=True)
+    for it in range(16, 46):
+        batch.iterate([it] * len(pairs), [t for _, _, t in pairs], [0.0] * len(pairs))
+        for i, (a, b, tau) in enumerate(pairs):
+            same_stats(a.iterate(it, tau, 0.0), b.stats(), 1, f"problem {i} iteration {it}", exact=i >= 2)
+    for i, (a, b, _) in enumerate(pairs):
+        w1 = sort_walkers(*a.download()[:2])
+        w2 = sort_walkers(*b.download()[:2])
+        np.testing.assert_array_equal(w1[0], w2[0])
+        np.testing.assert_allclose(w1[1], w2[1], rtol=1e-9, atol=1e-9)
+        assert len(w1[0]) > 100
+        a.close()
+    batch.close()
+    for _, b, _ in pairs:
+        b.close()

@@ -110,6 +110,12 @@ SPECS = {
     # plain FCIQMC: no initiator approximation (TRUNCINITIATOR off)
     "hubbard10_plain": lambda **k: hubbard_spec(10, 10, 4.0, trunc_initiator=False, **k),
     "mol8o6e_plain": lambda **k: mol_spec(8, 6, particles=0, trunc_initiator=False, **k),
+    # weighted PCHB samplers: UNIF-FULL / FULL-FULL particles, fully weighted
+    # holes (the reference's choice when nBasis <= 4 nEl)
+    "mol8o6e_unif_full": lambda **k: mol_spec(8, 6, particles=2, holes=1, **k),
+    "mol8o6e_full_full": lambda **k: mol_spec(8, 6, particles=3, holes=1, **k),
+    "mol8o6e_full_fast": lambda **k: mol_spec(8, 6, particles=3, holes=0, **k),
+    "mol33o4e_w2_unif_full": lambda **k: mol_spec(33, 4, rank=4, mseed=3, particles=2, holes=1, **k),
 }
 
 
@@ -284,7 +290,9 @@ def test_annihilate_explicit_list_fp32(name):
                                             ("mol8o6e_unif_fast", 1, True), ("mol8o6e_uniform", 1, False),
                                             ("mol8o6e_uniform", 2, False), ("mol33o4e_w2_uniform", 1, False),
                                             ("hubbard10_plain", 1, False), ("hubbard10_plain", 2, False),
-                                            ("mol8o6e_plain", 1, False)])
+                                            ("mol8o6e_plain", 1, False), ("mol8o6e_unif_full", 1, False),
+                                            ("mol8o6e_full_full", 1, False), ("mol8o6e_full_fast", 2, False),
+                                            ("mol33o4e_w2_unif_full", 1, False), ("mol8o6e_unif_full", 1, True)])
 def test_iterations_match_oracle(name, nrep, real):
     """Full iterations (spawn, merge, death, annihilation, stats) from the same
     walker set: identical walker sets after every iteration."""
