@@ -283,6 +283,10 @@ struct DevSys {
   const double* jk;
   // diagonal elements: (U(i,j,i,j), U(i,j,j,i)) at jk0[2*((i-1)*nsp + j-1)]
   const double* jk0;
+  // heat-bath single weights relative to the reference determinant:
+  // fref[(p-1)*nsp + b-1] = sltcnd_1(ref, p, b) for every spin orbital p and
+  // the spatial orbital b taken with p's spin
+  const double* fref;
   int nsp;
   // C(p, i) at binom[p*nel1 + i] when spawns are sorted by colex rank, else null
   const uint32_t* binom;
@@ -535,9 +539,51 @@ __device__ __host__ inline Excit gen_hubbard(const DevSys& s, const uint64_t* d,
   return ex;
 }
 
-// gen_single_4ind_ex + select_orb_sing (excit_gens_int_weighted.F90:442-620).
-// The cumulative list is recomputed in a second pass in the same order
-// instead of being stored, which reproduces the same FP values.
+// The heat-bath weight |H(src -> orb)| of gen_single_4ind_ex
+// (excit_gens_int_weighted.F90:442-620) evaluated relative to the reference
+// determinant: H = sltcnd_1(ref, src, orb) (the fref table) plus, for every
+// orbital j that d occupies and the reference does not (ascending, src
+// skipped), U(a,j,b,j) - U(a,j,j,b) [same spin as src only for the second],
+// then minus the same for every orbital the reference occupies and d does
+// not.  The same value as sltcnd_1(d, src, orb) up to FP64 rounding (exact
+// zeros stay exact: every term of a zero element is zero); its cost is
+// 2 * (excitation level of d) rows instead of nel.  Only the weights take this
+// form: the matrix element of the spawn is sltcnd_1(d, src, orb) itself.
+template <int W>
+__device__ __host__ inline double hb_weight(const DevSys& s, const uint64_t* d, int src, int orb) {
+  const int a = spat(src), b = spat(orb), nsp = s.nsp;
+  double v = s.fref[(size_t)(src - 1) * nsp + (b - 1)];
+  const double* base = s.jk + 2 * ((size_t)(a - 1) * nsp * nsp + (size_t)(b - 1));
+  const size_t step = 2 * (size_t)nsp;
+  const bool beta = is_beta(src);
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    uint64_t x = d[w] & ~s.ref[w];
+    if ((src - 1) >> 6 == w) x &= ~(1ull << ((src - 1) & 63));
+    for (; x; x &= x - 1) {
+      const int o = 64 * w + ctz64(x) + 1;
+      const double* e = base + step * (size_t)(spat(o) - 1);
+      v += e[0];
+      if (is_beta(o) == beta) v -= e[1];
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    uint64_t x = s.ref[w] & ~d[w];
+    if ((src - 1) >> 6 == w) x &= ~(1ull << ((src - 1) & 63));
+    for (; x; x &= x - 1) {
+      const int o = 64 * w + ctz64(x) + 1;
+      const double* e = base + step * (size_t)(spat(o) - 1);
+      v -= e[0];
+      if (is_beta(o) == beta) v += e[1];
+    }
+  }
+  return v;
+}
+
+// gen_single_4ind_ex + select_orb_sing (excit_gens_int_weighted.F90:442-620),
+// weights by hb_weight.  The cumulative list is recomputed in a second pass in
+// the same order instead of being stored, which reproduces the same FP values.
 template <int W>
 __device__ __host__ inline Excit gen_single_heatbath(const DevSys& s, const uint64_t* d, double dsel,
                                                      double dorb) {
@@ -551,17 +597,16 @@ __device__ __host__ inline Excit gen_single_heatbath(const DevSys& s, const uint
   for (int i = 1; i <= nsp; ++i) {
     int orb = 2 * i - off;
     double cpt = 0.0;
-    if (!test_orb<W>(d, orb)) cpt = fabs(sltcnd_1<W>(s, d, src, orb));
+    if (!test_orb<W>(d, orb)) cpt = fabs(hb_weight<W>(s, d, src, orb));
     cum_sum += cpt;
   }
   if (fabs(cum_sum) <= kEps) return ex;
   double r = dorb * cum_sum;
-  double cum = 0.0, cpt = 0.0, v = 0.0;
+  double cum = 0.0, cpt = 0.0;
   int orb = 0;
   for (int i = 1; i <= nsp; ++i) {
     orb = 2 * i - off;
-    v = test_orb<W>(d, orb) ? 0.0 : sltcnd_1<W>(s, d, src, orb);
-    cpt = fabs(v);
+    cpt = test_orb<W>(d, orb) ? 0.0 : fabs(hb_weight<W>(s, d, src, orb));
     cum += cpt;
     if (!(cum < r) || i == nsp) break;
   }
@@ -573,7 +618,7 @@ __device__ __host__ inline Excit gen_single_heatbath(const DevSys& s, const uint
   ex.e1 = orb;
   ex.parity = single_parity<W>(d, src, orb);
   ex.pgen = pg;
-  ex.hel = v;  // = sltcnd_1(src, orb), the element the spawn needs
+  ex.hel = sltcnd_1<W>(s, d, src, orb);  // the element the spawn needs
   ex.hel_ok = true;
   return ex;
 }

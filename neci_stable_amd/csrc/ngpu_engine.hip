@@ -141,6 +141,7 @@ struct ngpu_engine {
   double* d_umat = nullptr;
   double* d_jk = nullptr;
   double* d_jk0 = nullptr;
+  double* d_fref = nullptr;  // reference-relative heat-bath single weights
   AliasEnt* d_ab_tab = nullptr;
   double* d_ab_prob = nullptr;
   uint8_t* d_ab_ok = nullptr;
@@ -698,7 +699,7 @@ __device__ double calc_pgen(const DevSys& s, const uint64_t* d, int ic, const in
     for (int i = 1; i <= nsp; ++i) {
       int orb = 2 * i - off;
       if (!test_orb<W>(d, orb)) {
-        double cpt = fabs(sltcnd_1<W>(s, d, src, orb));
+        double cpt = fabs(hb_weight<W>(s, d, src, orb));
         cum_sum += cpt;
         if (orb == tgt) cpt_tgt = cpt;
       }
@@ -958,7 +959,7 @@ int ngpu_destroy(ngpu_t* h) {
   void* ptrs[] = {h->wk.keys, h->wk.st, h->wk.diagH, h->wk.offdiagH, h->wk.freeq,
                   h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->blk_rm, h->sp_rec,
                   h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->digit_tot, h->sq, h->n_key,
-                  h->n_sgn, h->n_flag, h->u_key, h->u_sgn, h->u_flag, h->u_count, h->cnt, h->dout, h->dpart, h->dcyc, h->own_params, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_binom, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
+                  h->n_sgn, h->n_flag, h->u_key, h->u_sgn, h->u_flag, h->u_count, h->cnt, h->dout, h->dpart, h->dcyc, h->own_params, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_fref, h->d_binom, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
                   h->core_vals, h->core_x, h->core_y, h->d_ab_tab,
                   h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt, h->d_i_tab, h->d_i_prob,
                   h->d_a_tab, h->d_a_prob, h->d_a_ok, h->d_b_tab, h->d_b_prob, h->d_b_ok};
@@ -975,6 +976,41 @@ int ngpu_destroy(ngpu_t* h) {
   delete h;
   return NGPU_OK;
 }
+
+extern "C++" {
+namespace {
+// fref[(p-1)*nsp + b-1] = sltcnd_1(ref, p, b with p's spin) for every spin
+// orbital p (the reference-relative heat-bath weights start from it)
+template <int W>
+__global__ void k_fock_ref(DevSys s, double* out) {
+  const int n = s.nbasis * s.nsp;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const int p = 1 + t / s.nsp, b = 1 + t % s.nsp;
+    out[t] = sltcnd_1<W>(s, s.ref, p, 2 * b - (is_beta(p) ? 1 : 0));
+  }
+}
+
+// (re)builds fref once both the integrals and the reference are set
+int refresh_fref(ngpu_t* h) {
+  if (h->sys.model != NGPU_MODEL_MOLECULAR || !h->have_model || !h->have_ref) return NGPU_OK;
+  const size_t n = (size_t)h->sys.nbasis * h->ds.nsp;
+  if (h->stream) HIP_OK(hipStreamSynchronize(h->stream));
+  if (h->d_fref) (void)hipFree(h->d_fref);
+  h->d_fref = nullptr;
+  h->ds.fref = nullptr;
+  HIP_OK(dalloc(&h->d_fref, n));
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+  if (h->W == 1)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fock_ref<1>), grid, dim3(kBlock), 0, h->stream, h->ds, h->d_fref);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fock_ref<2>), grid, dim3(kBlock), 0, h->stream, h->ds, h->d_fref);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(h->stream));
+  h->ds.fref = h->d_fref;
+  return NGPU_OK;
+}
+}  // namespace
+}  // extern "C++"
 
 int ngpu_set_integrals(ngpu_t* h, const double* tmat2d, const double* umat, int64_t umat_len, double ecore) {
   if (h) h->graph_dirty = true;  // kernel arguments of the captured chain change
@@ -1025,7 +1061,7 @@ int ngpu_set_integrals(ngpu_t* h, const double* tmat2d, const double* umat, int6
   h->ds.nsp = (int)nsp;
   h->ds.ecore = ecore;
   h->have_model = true;
-  return NGPU_OK;
+  return refresh_fref(h);
 }
 
 int ngpu_set_lattice(ngpu_t* h, const int32_t* nbr, int32_t nsites, int32_t maxnbr, double bhub, double uhub) {
@@ -1265,7 +1301,7 @@ int ngpu_set_reference(ngpu_t* h, const int64_t* ilut_ref, double hii) {
   h->ds.ref[1] = h->W > 1 ? (uint64_t)ilut_ref[1] : 0;
   h->ds.hii = hii;
   h->have_ref = true;
-  return NGPU_OK;
+  return refresh_fref(h);
 }
 
 int ngpu_upload_walkers(ngpu_t* h, const int64_t* rows, int64_t n) {

@@ -1038,45 +1038,61 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
 
 // Queued single excitations (molecular, heat-bath singles), G lanes per
 // attempt (G = 4, 8 or 16, a launch-shape choice).  gen_single_heatbath's two
-// sequential sweeps over the same-spin orbitals b need |H(src -> b)| for every
-// b; lane g of a group evaluates the orbitals b = g+1, g+1+G, ... from the
-// (Coulomb, exchange) rows jk[a][j][*] (contiguous in b, so a group's loads for
-// one occupied orbital j are one contiguous run).  Each value is summed in
-// sltcnd_1's order (Coulomb over the occupied orbitals, then exchange over the
-// same-spin ones, then the one-body term) and stored in LDS; both sweeps then
-// run in ascending b over LDS on every lane of the group, so the selected
-// orbital, pgen and H_ij are bit-identical to the one-lane generator.  The
-// attempt's three Philox blocks are spread over the group's lanes and shared.
-// More lanes per attempt shorten each attempt's chain of loads (measured at
-// 20 spatial orbitals: 1, 2 and 4 lanes took 116, 64 and 46 us for 1e5
-// attempts); fewer lanes would repeat the sequential sweeps on fewer lanes.
+// sequential sweeps need the weight |hb_weight(src -> b)| of every same-spin
+// orbital b; lane g of a group evaluates the orbitals b = g+1, g+1+G, ...:
+// the reference-relative element fref[src][b] (one contiguous row), then the
+// (Coulomb, exchange) pairs jk[a][j][b] of the orbitals j in which the parent
+// and the reference differ (contiguous in b, so a group's loads for one j are
+// one contiguous run), added in hb_weight's order, and stores it in LDS; both
+// sweeps then run in ascending b over LDS on every lane of the group, so the
+// selected orbital and pgen are bit-identical to the one-lane generator.  The
+// spawn's matrix element is sltcnd_1 of the selected orbital (group leader).
+// The attempt's three Philox blocks are spread over the group's lanes and
+// shared.  More lanes per attempt shorten each attempt's chain of loads
+// (measured at 20 spatial orbitals: 1, 2 and 4 lanes took 116, 64 and 46 us
+// for 1e5 attempts); fewer lanes would repeat the sequential sweeps on fewer
+// lanes.
 
-// Accumulates sign * rows[b] into v for the occupied orbitals of x (ascending,
-// src removed), two orbitals per step so that both rows' loads are in flight
-// before the ordered additions.
+// Adds sign * (U(a,j,b,j) - [same spin] U(a,j,j,b)) into v for the orbitals j
+// of x (ascending, src removed), two orbitals per step so that both pairs'
+// loads are in flight before the ordered additions.
 template <int PL, int G>
-__device__ inline void hb_rows(uint64_t x, int base, const double* row, int nsp, int g, int mcount, double sign,
-                               double* v) {
+__device__ inline void hb_corr(uint64_t x, int base, const double* row, int nsp, int g, int mcount, bool plus,
+                               int src, double* v) {
   while (x) {
     int o1 = base + ctz64(x) + 1;
     x &= x - 1;
     bool two = x != 0;
     int o2 = two ? base + ctz64(x) + 1 : o1;
     if (two) x &= x - 1;
-    const double* r1 = row + (size_t)2 * (spat(o1) - 1) * nsp + 2 * g;
-    const double* r2 = row + (size_t)2 * (spat(o2) - 1) * nsp + 2 * g;
-    double a1[PL], a2[PL];
+    const bool s1 = ((o1 ^ src) & 1) == 0, s2 = ((o2 ^ src) & 1) == 0;
+    const double2* r1 = (const double2*)(row + (size_t)2 * (spat(o1) - 1) * nsp) + g;
+    const double2* r2 = (const double2*)(row + (size_t)2 * (spat(o2) - 1) * nsp) + g;
+    double2 a1[PL], a2[PL];
 #pragma unroll
     for (int m = 0; m < PL; ++m) {
       bool in = m < mcount && g + m * G < nsp;
-      a1[m] = in ? r1[2 * m * G] : 0.0;
-      a2[m] = (in && two) ? r2[2 * m * G] : 0.0;
+      a1[m] = in ? r1[m * G] : double2{0.0, 0.0};
+      a2[m] = (in && two) ? r2[m * G] : double2{0.0, 0.0};
     }
 #pragma unroll
     for (int m = 0; m < PL; ++m) {
       if (m < mcount && g + m * G < nsp) {
-        v[m] = sign > 0.0 ? v[m] + a1[m] : v[m] - a1[m];
-        if (two) v[m] = sign > 0.0 ? v[m] + a2[m] : v[m] - a2[m];
+        if (plus) {
+          v[m] += a1[m].x;
+          if (s1) v[m] -= a1[m].y;
+          if (two) {
+            v[m] += a2[m].x;
+            if (s2) v[m] -= a2[m].y;
+          }
+        } else {
+          v[m] -= a1[m].x;
+          if (s1) v[m] += a1[m].y;
+          if (two) {
+            v[m] -= a2[m].x;
+            if (s2) v[m] += a2[m].y;
+          }
+        }
       }
     }
   }
@@ -1139,29 +1155,30 @@ __device__ __forceinline__ void k_spawn_singles_body(const Grid& g_, const DevSy
       int src = select_orb<W>(key, ~0ull, elec);
       int a = spat(src), off = is_beta(src) ? 1 : 0;
       const double* row = s.jk + (size_t)2 * (a - 1) * nsp * nsp;
+      const double* frow = s.fref + (size_t)(src - 1) * nsp;
       double v[PL];
 #pragma unroll
-      for (int m = 0; m < PL; ++m) v[m] = 0.0;
-      uint64_t same = off ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
-      // Coulomb terms over every occupied orbital but src, then exchange over the same-spin ones
+      for (int m = 0; m < PL; ++m) v[m] = (m < mcount && g + m * G < nsp) ? frow[g + m * G] : 0.0;
+      // the orbitals the parent occupies beyond the reference, then the
+      // reference's orbitals the parent lacks (hb_weight's order)
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        uint64_t x = key[w];
+        uint64_t x = key[w] & ~s.ref[w];
         if ((src - 1) >> 6 == w) x &= ~(1ull << ((src - 1) & 63));
-        hb_rows<PL, G>(x, 64 * w, row, nsp, g, mcount, 1.0, v);
+        hb_corr<PL, G>(x, 64 * w, row, nsp, g, mcount, true, src, v);
       }
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        uint64_t x = key[w] & same;
+        uint64_t x = s.ref[w] & ~key[w];
         if ((src - 1) >> 6 == w) x &= ~(1ull << ((src - 1) & 63));
-        hb_rows<PL, G>(x, 64 * w, row + 1, nsp, g, mcount, -1.0, v);
+        hb_corr<PL, G>(x, 64 * w, row, nsp, g, mcount, false, src, v);
       }
 #pragma unroll
       for (int m = 0; m < PL; ++m) {
         int b = g + m * G + 1;
         if (m < mcount && b <= nsp) {
           int orb = 2 * b - off;
-          myv[b - 1] = test_orb<W>(key, orb) ? 0.0 : v[m] + T(s, src, orb);
+          myv[b - 1] = test_orb<W>(key, orb) ? 0.0 : v[m];
         }
       }
       __syncthreads();
@@ -1169,11 +1186,10 @@ __device__ __forceinline__ void k_spawn_singles_body(const Grid& g_, const DevSy
       double cum_sum = 0.0;
       for (int b = 0; b < nsp; ++b) cum_sum += fabs(myv[b]);
       double rr = d2 * cum_sum;
-      double cum = 0.0, cpt = 0.0, vsel = 0.0;
+      double cum = 0.0, cpt = 0.0;
       int bsel = 0;
       for (int b = 1; b <= nsp; ++b) {
         double vb = myv[b - 1];
-        vsel = vb;
         cpt = fabs(vb);
         bsel = b;
         cum += cpt;
@@ -1203,7 +1219,8 @@ __device__ __forceinline__ void k_spawn_singles_body(const Grid& g_, const DevSy
           double pg = cpt / cum_sum;
           pg = pg / (double)s.nel;
           pg = pg * s.psingles;
-          double hel = single_parity<W>(key, src, tgt) ? -vsel : vsel;
+          const double hv = sltcnd_1<W>(s, key, src, tgt);
+          double hel = single_parity<W>(key, src, tgt) ? -hv : hv;
           uint8_t pi = pinfo[p];
           double wsign = (pi & 1) ? -1.0 : 1.0;
           nspawn = attempt_create(s, tau, hel, pg, wsign, d5, raw);

@@ -673,7 +673,37 @@ static int gen_hubbard(const osys_t *s, const uint64_t *ilut, const int32_t *nI,
     return 1 + (o_make_single(nI, nel, elec, orbs[ind], nJ, ex) ? 2 : 0);
 }
 
-/* gen_single_4ind_ex + select_orb_sing (excit_gens_int_weighted.F90:442-620) */
+/* The weight |H(src -> orb)| of gen_single_4ind_ex (excit_gens_int_weighted.F90:
+   442-620, there sltcnd_1 of the determinant itself) evaluated relative to the
+   reference: sltcnd_1(ref, src, orb), plus U(a,j,b,j) - [same spin]
+   U(a,j,j,b) for every orbital j the determinant occupies and the reference
+   does not (ascending, src skipped), minus the same for every orbital the
+   reference occupies and the determinant does not.  Equal to sltcnd_1(nI, src,
+   orb) up to FP64 rounding; the device's hb_weight sums in this order. */
+static double hb_weight(const osys_t *s, const uint64_t *ilut, int src, int orb)
+{
+    int W = nwords(s->nbasis);
+    int32_t nref[O_MAX_NEL];
+    o_decode(s->ref, W, nref);
+    double v = sltcnd_1(s, nref, src, orb);
+    int a = spat(src), b = spat(orb);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int o = 1; o <= s->nbasis; ++o) {
+            if (o == src) continue;
+            int in_d = test_orb(ilut, o), in_ref = test_orb(s->ref, o);
+            if (pass == 0 && in_d && !in_ref) {
+                v += umat(s, a, spat(o), b, spat(o));
+                if (is_beta(o) == is_beta(src)) v -= umat(s, a, spat(o), spat(o), b);
+            } else if (pass == 1 && !in_d && in_ref) {
+                v -= umat(s, a, spat(o), b, spat(o));
+                if (is_beta(o) == is_beta(src)) v += umat(s, a, spat(o), spat(o), b);
+            }
+        }
+    return v;
+}
+
+/* gen_single_4ind_ex + select_orb_sing (excit_gens_int_weighted.F90:442-620),
+   weights by hb_weight */
 static int gen_single_heatbath(const osys_t *s, const uint64_t *ilut, const int32_t *nI,
                                double dsel, double dorb, int32_t *ex, int32_t *nJ, double *pgen)
 {
@@ -685,7 +715,7 @@ static int gen_single_heatbath(const osys_t *s, const uint64_t *ilut, const int3
     for (int i = 0; i < nsp; ++i) {
         int orb = 2 * (i + 1) - (is_beta(src) ? 1 : 0);
         double cpt = 0.0;
-        if (!test_orb(ilut, orb)) cpt = fabs(sltcnd_1(s, nI, src, orb));
+        if (!test_orb(ilut, orb)) cpt = fabs(hb_weight(s, ilut, src, orb));
         cpt_arr[i] = cpt;
         cum_sum += cpt;
         cum[i] = cum_sum;
@@ -1363,7 +1393,7 @@ double o_calc_pgen(const osys_t *s, const uint64_t *ilut, int32_t ic, const int3
         for (int i = 0; i < nsp; ++i) {
             int orb = 2 * (i + 1) - (is_beta(tgt) ? 1 : 0);
             if (!test_orb(ilut, orb)) {
-                double cpt = fabs(sltcnd_1(s, nI, src, orb));
+                double cpt = fabs(hb_weight(s, ilut, src, orb));
                 cum_sum += cpt;
                 if (orb == tgt) cpt_tgt = cpt;
             }
