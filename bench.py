@@ -5,8 +5,9 @@ Default workload (the metric's configuration): 32 independent problems of
 uneven size -- 16 replicas (distinct seeds) of the synthetic C1 molecular
 Hamiltonian with 14 electrons in 30 spatial orbitals (seed 7, Cholesky rank
 60; initiator threshold 3, PCHB doubles with UNIF-UNIF particles and
-FAST-FAST holes, heat-bath singles with pSingles 0.1, tau 0.005, 5e6 walkers
-each, a semistochastic core of the 50 000 most populated determinants) and
+FAST-FAST holes, heat-bath singles with pSingles 0.1, tau 0.001 (0.005 is
+past this Hamiltonian's stable time step), 5e6 walkers each, a
+semistochastic core of the 50 000 most populated determinants) and
 16 half-filled Hubbard chains of 12-18 sites (U/t in [2, 8], walker targets
 log-uniform in [1e4, 2e6], tau 0.01).
 
@@ -276,8 +277,13 @@ def batched_workload(n_mol_replicas=16, n_hub=16, seed=2024, mol_walkers=5e6, nc
     return mol, mol_kw, hub_specs, mol_walkers, ncore
 
 
-TAU_MOL = 0.005   # the metric's molecular tau (every timed iteration)
-TAU_GROW = 0.001  # set-up growth of the molecular population only
+# tau of the 14e/30o molecular problems.  The 10e/20o workload's 0.005 is past
+# this Hamiltonian's stable time step: from a population settled at tau 0.001
+# (1e5 walkers, its shift held), one iteration grows the population by 0.1 %
+# at tau 0.001, 3.6 % at 0.002, 13 % at 0.003, 45 % at 0.004 and 80 % at
+# 0.005 (the oracle; the engine at 4.6e6 walkers: x1.15, x1.43, x1.68, x1.84,
+# x1.93 in the first five iterations at 0.005, then the spawn list overflows).
+TAU_MOL = 0.001
 
 
 def run_batched(args):
@@ -299,23 +305,17 @@ def run_batched(args):
                                                                   mol_walkers=args.mol_walkers, ncore=args.ncore)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     t_setup = time.perf_counter()
-    # one molecular population grown to the target under TARGET-SHIFTDAMP:
-    # at tau 0.005 the first update cycle (10 iterations at DiagSft 0)
-    # multiplies the population by ~100, past the walker table, so it grows
-    # at TAU_GROW, then settles at the metric's tau
+    # one molecular population grown to the target under TARGET-SHIFTDAMP
     g_spec = Spec(mol, init_walkers=mol_walkers, seed=6999, **mol_kw)
     g = Engine(g_spec)
     g.start_single_part(1000.0)
-    gd = FciMCPar(g, tau=TAU_GROW, target_walkers=mol_walkers, steps_shift=10, shift_damp=0.5,
+    gd = FciMCPar(g, tau=TAU_MOL, target_walkers=mol_walkers, steps_shift=10, shift_damp=0.5,
                   target_shiftdamp=True)
-    for tau in (TAU_GROW, TAU_MOL):
-        gd.tau = tau
-        settled = 0
-        while settled < 3 and gd.iter < args.max_growth_iters:
-            gd.run(10)
-            av = float(np.sum(gd.rows[-1]["tot_parts"]))
-            settled = settled + 1 if abs(av / mol_walkers - 1.0) < 0.1 else 0
-        log(f"# molecular growth at tau {tau}: {gd.iter} iterations, {av:.4g} walkers")
+    settled = 0
+    while settled < 3 and gd.iter < args.max_growth_iters:
+        gd.run(10)
+        av = float(np.sum(gd.rows[-1]["tot_parts"]))
+        settled = settled + 1 if abs(av / mol_walkers - 1.0) < 0.1 else 0
     dets, sg, fl, _, _ = g.download()
     core = None
     if ncore:
