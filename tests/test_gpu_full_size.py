@@ -68,6 +68,22 @@ def grow(eng, tau, target, n0, shift_damp=0.5, max_iter=4000, t_max=240.0):
     return drv
 
 
+def check_top_weights(core, d, s, n):
+    """The device's choice holds the n largest weights sum_r |c_r| (equal
+    weights at the boundary may be other determinants than a storage-order
+    choice would take)."""
+    w = np.abs(s).sum(axis=1)
+    dt = np.dtype([(f"w{i}", np.int64) for i in range(d.shape[1])])
+    dv = np.ascontiguousarray(d).view(dt).ravel()
+    cv = np.ascontiguousarray(core).view(dt).ravel()
+    o = np.argsort(dv)
+    pos = np.searchsorted(dv[o], cv)
+    assert len(core) == n and np.all(pos < len(dv)) and np.array_equal(dv[o][pos], cv)
+    chosen = np.sort(w[o][pos])
+    top = np.sort(np.partition(w, len(w) - n)[len(w) - n:])
+    np.testing.assert_array_equal(chosen, top)
+
+
 def most_populated(d, s, n):
     o = np.argsort(-np.abs(s).sum(axis=1), kind="stable")[:n]
     return np.ascontiguousarray(d[np.sort(o)])
@@ -82,9 +98,12 @@ def test_two_word_26e40o_5e7_walkers_fp32_core():
     eng = Engine(spec)
     drv = grow(eng, 0.0002, 5e7, 1e5)
     d, s = check_state(eng, spec, drv.last)
-    core = most_populated(d, s, 20000)
-    nnz = eng.set_core(core, fp32=True)
-    print(f"core: 20000 dets, {nnz} stored elements (FP32)", flush=True)
+    t0 = time.time()
+    core, nnz = eng.select_core(20000, fp32=True)  # chosen and built on the device
+    print(f"core: {len(core)} dets chosen and built in {time.time() - t0:.2f} s, {nnz} stored elements (FP32)",
+          flush=True)
+    check_top_weights(core, d, s, 20000)
+    eng.set_spawn_precision(True)  # FP32-stored spawned amplitudes, FP64 sums
     t0 = time.time()
     w0 = drv.walker_iters
     drv.run(20)
@@ -107,7 +126,8 @@ def test_14e30o_core50k_fp32_matches_fp64():
     drv.run(200)
     d, s = check_state(eng, spec, drv.last)
     f = eng.download()[2]
-    core = most_populated(d, s, 50000)
+    core, _ = eng.select_core(50000)  # chosen on the device
+    check_top_weights(core, d, s, 50000)
     start_rows = eng.rows(d, s, f)
     shift0 = drv.shift.copy()
     it0 = drv.iter
