@@ -15,6 +15,7 @@ module neci_gpu_mod
     integer(c_int), parameter, public :: NGPU_PART_UNIF_FULL = 2, NGPU_PART_FULL_FULL = 3
     integer(c_int), parameter, public :: NGPU_HOLES_FAST_FAST = 0, NGPU_HOLES_FULL_FULL = 1
     integer(c_int), parameter, public :: NGPU_CORE_BUILD_AUTO = 0, NGPU_CORE_BUILD_JOIN = 1, NGPU_CORE_BUILD_SCAN = 2
+    integer(c_int), parameter, public :: NGPU_GROUP_CORE = 1, NGPU_GROUP_SHARED_CORE = 2
 
     type, bind(c), public :: ngpu_sys_t
         integer(c_int32_t) :: nel, nbasis, n_replicas, model
@@ -52,7 +53,7 @@ module neci_gpu_mod
     public :: ngpu_fcidump_header, ngpu_read_fcidump, ngpu_set_spawn_precision, ngpu_get_hash_stats
     public :: ngpu_get_error, ngpu_set_excitgen_pchb, ngpu_download_walkers_dev, ngpu_most_populated
     public :: ngpu_select_core, ngpu_set_core_build, ngpu_get_core_hamil, ngpu_batch_get_groups
-    public :: ngpu_batch_set_graph, ngpu_batch_set_profiling
+    public :: ngpu_batch_set_graph, ngpu_batch_set_profiling, ngpu_batch_get_group_flags
 
     interface
         integer(c_int) function ngpu_create(h, sys) bind(c, name="ngpu_create")
@@ -268,6 +269,12 @@ module neci_gpu_mod
             type(c_ptr), value :: b
             integer(c_int32_t), intent(out) :: group_of_engine(*)
             integer(c_int32_t), intent(out) :: n_groups
+        end function
+        integer(c_int) function ngpu_batch_get_group_flags(b, group, flags) bind(c, name="ngpu_batch_get_group_flags")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: b
+            integer(c_int32_t), value :: group
+            integer(c_int32_t), intent(out) :: flags
         end function
         integer(c_int) function ngpu_batch_set_graph(b, on) bind(c, name="ngpu_batch_set_graph")
             import :: c_int, c_ptr, c_int32_t

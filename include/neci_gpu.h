@@ -313,6 +313,14 @@ int ngpu_batch_destroy(ngpu_batch_t *b);
    concurrently, each on its own stream.  group_of_engine[i] receives engine
    i's group (either pointer may be NULL).  No counterpart in the reference. */
 int ngpu_batch_get_groups(ngpu_batch_t *b, int32_t *group_of_engine, int32_t *n_groups);
+/* Properties of launch group `group`: NGPU_GROUP_CORE when its engines have a
+   core space; NGPU_GROUP_SHARED_CORE when every engine of the group has the
+   same core determinants (replicas of one system, as under the reference's
+   global core space, semi_stoch_procs.F90:1623-1642), so one projection reads
+   the core Hamiltonian once for all of them (its sums run in another order
+   than an engine's own projection: the same amplitudes to rounding). */
+enum { NGPU_GROUP_CORE = 1, NGPU_GROUP_SHARED_CORE = 2 };
+int ngpu_batch_get_group_flags(ngpu_batch_t *b, int32_t group, int32_t *flags);
 /* on != 0 (the default): each batch iteration replays one captured graph;
    0: the group launches are issued directly (needed for stage timing). */
 int ngpu_batch_set_graph(ngpu_batch_t *b, int32_t on);

@@ -22,6 +22,8 @@ PART_UNIF_FULL = 2
 PART_FULL_FULL = 3
 HOLES_FAST_FAST = 0
 HOLES_FULL_FULL = 1
+GROUP_CORE = 1
+GROUP_SHARED_CORE = 2
 CORE_BUILD_AUTO = 0
 CORE_BUILD_JOIN = 1
 CORE_BUILD_SCAN = 2
@@ -137,6 +139,7 @@ _SIGS = [
     ("ngpu_batch_iterate", ctypes.c_int, [_vp, _vp]),
     ("ngpu_batch_destroy", ctypes.c_int, [_vp]),
     ("ngpu_batch_get_groups", ctypes.c_int, [_vp, _vp, P(ctypes.c_int32)]),
+    ("ngpu_batch_get_group_flags", ctypes.c_int, [_vp, ctypes.c_int32, P(ctypes.c_int32)]),
     ("ngpu_batch_set_graph", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("ngpu_batch_set_profiling", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("ngpu_get_stage_times", ctypes.c_int, [_vp, _vp, ctypes.c_int32, P(ctypes.c_int64)]),

@@ -62,6 +62,7 @@ struct EngDev {
   const void* core_vals;
   double* core_x;
   double* core_y;
+  int32_t core_col0;     // shared core space: this engine's first column of the group's vectors
   int32_t b0[kLkKinds];  // first block of this engine's share, per launch kind
   int32_t nb[kLkKinds];  // blocks of the share
 };
@@ -178,7 +179,69 @@ __global__ void __launch_bounds__(kBlock) km_core_gather(const EngDev* __restric
                                                          const uint16_t* __restrict__ map) {
   Grid g;
   const EngDev& a = eng_of(ed, map, kLkCore, g);
-  k_core_gather_body<R>(g, a.core_slot, a.wk, a.s.S, a.ncore, a.core_x);
+  k_core_gather_body<R>(g, a.core_slot, a.wk, a.s.S, a.ncore, a.core_x, a.s.S);
+}
+
+// Core space shared by every engine of the group (the same determinants, so
+// the same Hamiltonian, as replicas of one system have under a global core
+// space): the engines' amplitudes are the C columns of one interleaved vector
+// X[i * C + column], the Hamiltonian is read once for all of them, and
+// Y[i * C + column] = (H.X + shift * X) * tau per column.
+template <int R>
+__global__ void __launch_bounds__(kBlock) km_core_gather_shared(const EngDev* __restrict__ ed,
+                                                                const uint16_t* __restrict__ map, double* X, int C) {
+  Grid g;
+  const EngDev& a = eng_of(ed, map, kLkCore, g);
+  k_core_gather_body<R>(g, a.core_slot, a.wk, a.s.S, a.ncore, X + a.core_col0, C);
+}
+
+constexpr int kCoreCols = 16;  // columns per pass of the shared projection
+
+// One wave per row; lane l works on column c0 + (l & 15) and the row's
+// nonzeros q = q0 + (l >> 4), +4, ..., so each load instruction of the wave
+// reads four whole 128-byte rows of X (16 columns) instead of 64 scattered
+// elements; the four partial sums of a column meet in two butterfly steps.
+// The order of the additions differs from an engine's own projection
+// (k_core_spmv_body), so the results agree with it to rounding.
+// colmap[c] = engine << 8 | replica of column c.
+template <typename VT>
+__global__ void __launch_bounds__(kBlock) km_core_spmv_shared(const EngDev* __restrict__ ed, const int32_t* colmap,
+                                                              const double* X, double* Y, int C) {
+  const EngDev& a = ed[0];
+  const int64_t* rowptr = a.core_rowptr;
+  const int32_t* cols = a.core_cols;
+  const VT* vals = (const VT*)a.core_vals;
+  const int64_t ncore = a.ncore;
+  const int lane = lane_id(), k = lane & (kCoreCols - 1), sub = lane / kCoreCols;
+  constexpr int kSubs = kWave / kCoreCols;
+  const int64_t nwaves = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; i < ncore; i += nwaves) {
+    const int64_t q0 = rowptr[i], q1 = rowptr[i + 1];
+    for (int c0 = 0; c0 < C; c0 += kCoreCols) {
+      const bool on = c0 + k < C;
+      double acc = 0.0;
+      for (int64_t q = q0 + sub; q < q1; q += kSubs) {
+        const double v = (double)vals[q];
+        if (on) acc = acc - v * X[(size_t)cols[q] * C + c0 + k];
+      }
+#pragma unroll
+      for (int o = kCoreCols; o < kWave; o <<= 1) acc = acc + __shfl_xor(acc, o);
+      if (sub == 0 && on) {
+        const int c = c0 + k, m = colmap[c];
+        const IterParams* prm = ed[m >> 8].prm;
+        Y[i * C + c] = (acc + prm->sft.v[m & 255] * X[i * C + c]) * prm->tau;
+      }
+    }
+  }
+}
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) km_core_apply_shared(const EngDev* __restrict__ ed,
+                                                               const uint16_t* __restrict__ map, const double* Y,
+                                                               int C) {
+  Grid g;
+  const EngDev& a = eng_of(ed, map, kLkCore, g);
+  k_core_apply_body<R>(g, a.s, a.wk, a.cnt, a.core_slot, Y + a.core_col0, a.ncore, a.part, C);
 }
 template <int R, typename VT>
 __global__ void __launch_bounds__(kBlock) km_core_spmv(const EngDev* __restrict__ ed, const uint16_t* __restrict__ map) {
@@ -192,7 +255,7 @@ __global__ void __launch_bounds__(kBlock) km_core_apply(const EngDev* __restrict
                                                         const uint16_t* __restrict__ map) {
   Grid g;
   const EngDev& a = eng_of(ed, map, kLkCore, g);
-  k_core_apply_body<R>(g, a.s, a.wk, a.cnt, a.core_slot, a.core_y, a.ncore, a.part);
+  k_core_apply_body<R>(g, a.s, a.wk, a.cnt, a.core_slot, a.core_y, a.ncore, a.part, a.s.S);
 }
 
 template <int W, int R>

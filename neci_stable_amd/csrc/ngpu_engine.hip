@@ -199,6 +199,13 @@ struct BatchGroup {
   bool rank_sort = false;
   std::vector<std::pair<int, int>> passes;  // (word, shift) of the word-key sort
   bool core = false, core_fp32 = false;
+  // every engine of the group has the same core determinants: one projection
+  // over the interleaved vectors X, Y of ccols columns (kernels in ngpu_batch.h)
+  bool shared_core = false;
+  int ccols = 0;
+  double* d_X = nullptr;
+  double* d_Y = nullptr;
+  int32_t* d_colmap = nullptr;
   int rec_words = 0;
   EngDev* d_ed = nullptr;
   uint16_t* d_map[kLkKinds] = {nullptr};
@@ -2381,6 +2388,11 @@ bool same_group(const BatchGroup& a, const BatchGroup& b) {
 void free_group(BatchGroup& g) {
   if (g.d_ed) (void)hipFree(g.d_ed);
   g.d_ed = nullptr;
+  for (void* p : {(void*)g.d_X, (void*)g.d_Y, (void*)g.d_colmap})
+    if (p) (void)hipFree(p);
+  g.d_X = g.d_Y = nullptr;
+  g.d_colmap = nullptr;
+  g.shared_core = false;
   for (int k = 0; k < kLkKinds; ++k) {
     if (g.d_map[k]) (void)hipFree(g.d_map[k]);
     g.d_map[k] = nullptr;
@@ -2474,6 +2486,35 @@ int build_groups(ngpu_batch* b) {
         map[k].insert(map[k].end(), (size_t)nb[k], (uint16_t)i);
       }
     }
+    if (g.core && E > 1) {  // the same core space in every engine (replicas of one system)
+      const ngpu_t* h0 = g.hs[0];
+      bool same = true;
+      std::vector<uint64_t> k0((size_t)h0->ncore * h0->W), kx(k0.size());
+      HIP_OK(hipMemcpy(k0.data(), h0->core_keys, k0.size() * 8, hipMemcpyDeviceToHost));
+      for (size_t i = 1; i < E && same; ++i) {
+        const ngpu_t* h = g.hs[i];
+        same = h->ncore == h0->ncore && h->core_nnz == h0->core_nnz && h->core_fp32 == h0->core_fp32;
+        if (same) {
+          HIP_OK(hipMemcpy(kx.data(), h->core_keys, kx.size() * 8, hipMemcpyDeviceToHost));
+          same = memcmp(k0.data(), kx.data(), k0.size() * 8) == 0;
+        }
+      }
+      if (same) {
+        std::vector<int32_t> colmap;
+        int C = 0;
+        for (size_t i = 0; i < E; ++i) {
+          ed[i].core_col0 = C;
+          for (int r = 0; r < g.hs[i]->S; ++r) colmap.push_back((int32_t)(i << 8) | r);
+          C += g.hs[i]->S;
+        }
+        HIP_OK(dalloc(&g.d_X, (size_t)h0->ncore * C));
+        HIP_OK(dalloc(&g.d_Y, (size_t)h0->ncore * C));
+        HIP_OK(dalloc(&g.d_colmap, colmap.size()));
+        HIP_OK(hipMemcpy(g.d_colmap, colmap.data(), colmap.size() * 4, hipMemcpyHostToDevice));
+        g.shared_core = true;
+        g.ccols = C;
+      }
+    }
     HIP_OK(dalloc(&g.d_ed, E));
     HIP_OK(hipMemcpy(g.d_ed, ed.data(), sizeof(EngDev) * E, hipMemcpyHostToDevice));
     for (int k = 0; k < kLkKinds; ++k) {
@@ -2553,7 +2594,18 @@ void group_chain_wr(BatchGroup& g) {
   const EngDev* ed = g.d_ed;
   const uint16_t* const* m = g.d_map;
   glaunch(h0, 0, 0, km_prepass<W, R>, g.tot[kLkChunk], kBlock, 0, ed, (const uint16_t*)m[kLkChunk]);
-  if (g.core) {
+  if (g.core && g.shared_core) {
+    glaunch(h0, -1, -1, km_core_gather_shared<R>, g.tot[kLkCore], kBlock, 0, ed, (const uint16_t*)m[kLkCore], g.d_X,
+            g.ccols);
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((h0->ncore + kBlock / kWave - 1) / (kBlock / kWave),
+                                                                8 * (int64_t)h0->nb_grid));
+    if (g.core_fp32)
+      glaunch(h0, 9, 9, km_core_spmv_shared<float>, nb, kBlock, 0, ed, (const int32_t*)g.d_colmap,
+              (const double*)g.d_X, g.d_Y, g.ccols);
+    else
+      glaunch(h0, 9, 9, km_core_spmv_shared<double>, nb, kBlock, 0, ed, (const int32_t*)g.d_colmap,
+              (const double*)g.d_X, g.d_Y, g.ccols);
+  } else if (g.core) {
     glaunch(h0, -1, -1, km_core_gather<R>, g.tot[kLkCore], kBlock, 0, ed, (const uint16_t*)m[kLkCore]);
     if (g.core_fp32)
       glaunch(h0, 9, 9, km_core_spmv<R, float>, g.tot[kLkCoreRow], kBlock, 0, ed, (const uint16_t*)m[kLkCoreRow]);
@@ -2581,7 +2633,11 @@ void group_chain_wr(BatchGroup& g) {
       default: group_singles<W, R, 16, 4>(g, nsp); break;
     }
   }
-  if (g.core) glaunch(h0, -1, -1, km_core_apply<R>, g.tot[kLkCore], kBlock, 0, ed, (const uint16_t*)m[kLkCore]);
+  if (g.core && g.shared_core)
+    glaunch(h0, -1, -1, km_core_apply_shared<R>, g.tot[kLkCore], kBlock, 0, ed, (const uint16_t*)m[kLkCore],
+            (const double*)g.d_Y, g.ccols);
+  else if (g.core)
+    glaunch(h0, -1, -1, km_core_apply<R>, g.tot[kLkCore], kBlock, 0, ed, (const uint16_t*)m[kLkCore]);
   if (g.items == 2) group_sort_merge<W, R, 2>(g);
   else if (g.items == 8) group_sort_merge<W, R, 8>(g);
   else group_sort_merge<W, R, 4>(g);
@@ -2770,6 +2826,16 @@ int ngpu_batch_get_groups(ngpu_batch* b, int32_t* group_of_engine, int32_t* n_gr
       for (size_t g = 0; g < b->groups.size(); ++g)
         for (ngpu_t* h : b->groups[g].hs)
           if (h == b->hs[i]) group_of_engine[i] = (int32_t)g;
+  return NGPU_OK;
+}
+
+int ngpu_batch_get_group_flags(ngpu_batch* b, int32_t group, int32_t* flags) {
+  if (!b || !flags) return NGPU_ERR_ARG;
+  int rc = refresh_groups(b);
+  if (rc) return rc;
+  if (group < 0 || group >= (int32_t)b->groups.size()) return NGPU_ERR_ARG;
+  const BatchGroup& g = b->groups[(size_t)group];
+  *flags = (g.core ? NGPU_GROUP_CORE : 0) | (g.shared_core ? NGPU_GROUP_SHARED_CORE : 0);
   return NGPU_OK;
 }
 

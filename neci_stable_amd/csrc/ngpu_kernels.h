@@ -1904,15 +1904,17 @@ __global__ void __launch_bounds__(kBlock) k_merge_annihilate(DevSys s, Walk wk, 
 
 // ------------------------------------------------------ semistochastic core
 // Core amplitudes before the projection (no death acts on core dets, so the
-// signs after the prepass are the iteration's starting amplitudes).
+// signs after the prepass are the iteration's starting amplitudes):
+// x[i * xs + r] = the amplitude of core det i, replica r (xs = S for an
+// engine's own vector; a wider stride for a vector shared by a batch group)
 template <int R>
-__device__ __forceinline__ void k_core_gather_body(const Grid& g_, const int64_t* core_slot, const Walk& wk, int S, int64_t ncore, double* x) {
+__device__ __forceinline__ void k_core_gather_body(const Grid& g_, const int64_t* core_slot, const Walk& wk, int S, int64_t ncore, double* x, int xs) {
   for (int64_t i = (int64_t)g_.b * blockDim.x + threadIdx.x; i < ncore; i += (int64_t)g_.nb * blockDim.x)
-    for (int r = 0; r < S; ++r) x[i * S + r] = wk.sgn(core_slot[i], r);
+    for (int r = 0; r < S; ++r) x[i * xs + r] = wk.sgn(core_slot[i], r);
 }
 template <int R>
 __global__ void __launch_bounds__(kBlock) k_core_gather(const int64_t* core_slot, Walk wk, int S, int64_t ncore, double* x) {
-  k_core_gather_body<R>(hw_grid(), core_slot, wk, S, ncore, x);
+  k_core_gather_body<R>(hw_grid(), core_slot, wk, S, ncore, x, S);
 }
 
 // determ_projection (semi_stoch_procs.F90:105-241), one wave per core row:
@@ -1950,7 +1952,7 @@ __global__ void __launch_bounds__(kBlock) k_core_spmv(const int64_t* rowptr, con
 // deterministic_annihilation (Annihilation.F90:930-963): the projected
 // amplitudes join the core dets' signs (births and annihilations counted)
 template <int R>
-__device__ __forceinline__ void k_core_apply_body(const Grid& g_, const DevSys& s, const Walk& wk, const Counters* cnt, const int64_t* core_slot, const double* y, int64_t ncore, double* part) {
+__device__ __forceinline__ void k_core_apply_body(const Grid& g_, const DevSys& s, const Walk& wk, const Counters* cnt, const int64_t* core_slot, const double* y, int64_t ncore, double* part, int ys) {
   if (cnt->halt) return;
   __shared__ double acc_lds[kStatStride];
   BlockStats bs{acc_lds};
@@ -1965,7 +1967,7 @@ __device__ __forceinline__ void k_core_apply_body(const Grid& g_, const DevSys& 
     for (int r = 0; r < S; ++r) {
       double b = 0.0, a = 0.0;
       if (act) {
-        double yy = y[i * S + r], c = wk.sgn(slot, r);
+        double yy = y[i * ys + r], c = wk.sgn(slot, r);
         wk.sgn(slot, r) = yy + c;
         b = fabs(yy);
         if (c * yy < 0.0) {
@@ -1983,7 +1985,7 @@ __device__ __forceinline__ void k_core_apply_body(const Grid& g_, const DevSys& 
 }
 template <int R>
 __global__ void __launch_bounds__(kBlock) k_core_apply(DevSys s, Walk wk, const Counters* cnt, const int64_t* core_slot, const double* y, int64_t ncore, double* part) {
-  k_core_apply_body<R>(hw_grid(), s, wk, cnt, core_slot, y, ncore, part);
+  k_core_apply_body<R>(hw_grid(), s, wk, cnt, core_slot, y, ncore, part, R == 1 ? 1 : s.S);
 }
 
 // core Hamiltonian rows (calc_determ_hamil, fast_determ_hamil.F90:892-1548):

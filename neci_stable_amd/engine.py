@@ -364,6 +364,12 @@ class Batch:
         check(self.lib.ngpu_batch_get_groups(self.b, ptr(g), ctypes.byref(n)), "ngpu_batch_get_groups")
         return g, n.value
 
+    def group_flags(self, group):
+        """ngpu_batch_get_group_flags: GROUP_CORE | GROUP_SHARED_CORE bits."""
+        f = ctypes.c_int32()
+        check(self.lib.ngpu_batch_get_group_flags(self.b, int(group), ctypes.byref(f)), "ngpu_batch_get_group_flags")
+        return f.value
+
     def set_graph(self, on=True):
         """Replay one captured graph per iteration (True) or launch directly."""
         check(self.lib.ngpu_batch_set_graph(self.b, int(bool(on))), "ngpu_batch_set_graph")

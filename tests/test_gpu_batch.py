@@ -210,3 +210,58 @@ def test_batched_mix_of_molecules_and_chains():
     batch.close()
     for _, b, _ in pairs:
         b.close()
+
+
+def test_batch_shared_core_space():
+    """Replicas of one system with the same core determinants (two engines of
+    two replicas and one of three, one launch group) share one core
+    projection that reads the core Hamiltonian once (GROUP_SHARED_CORE);
+    every engine follows the same engine iterated alone (to 1e-10: the shared
+    projection sums in another order).  When one engine's core differs, the
+    group projects each engine on its own, with the same results."""
+    from neci_stable_amd._lib import GROUP_CORE, GROUP_SHARED_CORE
+    from neci_stable_amd.engine import Batch, Engine
+    mk, tau, n0 = CASES["mol8o6e"]
+    g0 = Engine(mk(init_walkers=4000, n_replicas=1, seed=90))
+    g0.start_single_part(n0)
+    for it in range(1, 16):
+        g0.iterate(it, tau, 0.0)
+    core = g0.most_populated(60)
+    g0.close()
+    assert len(core) == 60
+    for differ in (False, True):
+        alone, batched, shifts = [], [], []
+        for i, nrep in enumerate((2, 2, 3)):
+            spec = mk(init_walkers=4000, n_replicas=nrep, seed=91 + i)
+            pair = []
+            for _ in range(2):
+                e = Engine(spec)
+                e.start_single_part(n0)
+                e.set_core(core[:-1] if (differ and i == 1) else core, fp32=(i == 0))
+                pair.append(e)
+            alone.append(pair[0])
+            batched.append(pair[1])
+            shifts.append(np.linspace(0.1, 0.3, nrep))
+        # the first engine stores its core Hamiltonian in FP32: another group
+        batch = Batch(batched)
+        g, n = batch.groups()
+        assert n == 2 and g[1] == g[2] and g[0] != g[1]
+        flags = batch.group_flags(int(g[1]))
+        assert flags & GROUP_CORE
+        assert bool(flags & GROUP_SHARED_CORE) == (not differ)
+        for it in range(1, 26):
+            batch.iterate([it] * 3, [tau] * 3, shifts)
+            for i, a in enumerate(alone):
+                same_stats(a.iterate(it, tau, shifts[i]), batched[i].stats(), a.S, f"engine {i} iteration {it}",
+                           exact=False)
+        for a, b in zip(alone, batched):
+            w1 = sort_walkers(*a.download()[:3])
+            w2 = sort_walkers(*b.download()[:3])
+            np.testing.assert_array_equal(w1[0], w2[0])
+            np.testing.assert_array_equal(w1[2], w2[2])
+            np.testing.assert_allclose(w1[1], w2[1], rtol=1e-9, atol=1e-9)
+            assert len(w1[0]) > 20
+            a.close()
+        batch.close()
+        for b in batched:
+            b.close()
