@@ -113,14 +113,17 @@ def _kernel_short(name):
     return n.split("::")[-1]
 
 
-def summarize_trace(path, last, per_iter_kernel="k_end_iter"):
+def summarize_trace(path, last, per_iter_kernel="k_end_iter", ends_per_iter=1):
     """Steady-state table from a rocprofv3 kernel-trace CSV: per kernel, the
-    dispatches of the last `last` iterations (an iteration ends with one
-    dispatch of per_iter_kernel), average and per-iteration duration."""
+    dispatches of the last `last` iterations (an iteration ends with
+    ends_per_iter dispatches of per_iter_kernel: one for a single engine, one
+    per launch group for a batch), average and per-iteration duration."""
     import csv
     rows = list(csv.DictReader(open(path)))
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
-    ends = [i for i, r in enumerate(rows) if per_iter_kernel in r["Kernel_Name"]]
+    ends = [i for i, r in enumerate(rows) if _kernel_short(r["Kernel_Name"]) == per_iter_kernel]
+    ends = ends[ends_per_iter - 1::ends_per_iter] if len(ends) % ends_per_iter == 0 else \
+        ends[len(ends) % ends_per_iter + ends_per_iter - 1::ends_per_iter]
     if len(ends) <= last:
         raise SystemExit(f"only {len(ends)} iterations in {path}")
     window = rows[ends[-last - 1] + 1: ends[-1] + 1]
@@ -184,8 +187,11 @@ def traffic_table(fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, copy_bytes
     table = {"fetch_scale": sf, "write_scale": sw, "copy_bytes": copy_bytes, "kernels": {}}
     for k in sorted(set(f) & set(w)):
         name = k.split("<")[0]
-        table["kernels"][name] = {"instance": k, "fetch_kib": f[k], "write_kib": w[k],
-                                  "hbm_bytes_per_launch": sf * f[k] * 1024.0 + sw * w[k] * 1024.0}
+        row = {"instance": k, "fetch_kib": f[k], "write_kib": w[k],
+               "hbm_bytes_per_launch": sf * f[k] * 1024.0 + sw * w[k] * 1024.0}
+        # instances of one kernel (other generators, word counts): the busiest
+        if name not in table["kernels"] or row["hbm_bytes_per_launch"] > table["kernels"][name]["hbm_bytes_per_launch"]:
+            table["kernels"][name] = row
     return table
 
 
@@ -418,6 +424,13 @@ def run_batched(args):
         b += algorithmic_bytes(dom, cnt, W=e0.W, S=e0.S, nbasis=e0.nbasis, psingles=e0.psingles, nel=e0.nel,
                                val_bytes=4)
     achieved = b / (dom_ms * 1e-3) / 1e9
+    # HBM bytes per launch of that kernel's group instance from the FETCH_SIZE /
+    # WRITE_SIZE counter runs of this workload (profiles/), when collected
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_batched_14e30o_hubbard.json")
+    if os.path.exists(tpath) and args.mol_replicas == 16 and args.hub_systems == 16:
+        with open(tpath) as f:
+            traffic = json.load(f)["kernels"].get("km" + KERNELS[dom][1:], {}).get("hbm_bytes_per_launch")
     d0 = engines[0][1]
     batch.close()
     walkers = [float(np.sum(d.last["tot_parts"])) for d in drivers]
@@ -451,7 +464,7 @@ def run_batched(args):
                          "form": "the same batched chain replayed as one captured graph per iteration, no events"},
         "launch_groups": int(ngroups),
         "roofline": {"bound": "hbm", "kernel": KERNELS[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": b, "launch_ms": dom_ms,
                      "where": f"one launch covers the {len(mol)} molecular replicas; its launches timed with "
                               "events over the timed region"},
@@ -494,6 +507,9 @@ def main():
     ap.add_argument("--summarize-trace", default=None, help="rocprofv3 kernel-trace CSV to summarise")
     ap.add_argument("--summarize-pmc", default=None, help="rocprofv3 counter-collection CSV to summarise")
     ap.add_argument("--last", type=int, default=60, help="iterations (or dispatches) summarised")
+    ap.add_argument("--per-iter-kernel", default="k_end_iter", help="kernel that ends an iteration (trace summary)")
+    ap.add_argument("--ends-per-iter", type=int, default=1,
+                    help="dispatches of --per-iter-kernel per iteration (launch groups of a batch)")
     ap.add_argument("--mol-replicas", type=int, default=16)
     ap.add_argument("--ncore", type=int, default=50000, help="core determinants of the molecular problems")
     ap.add_argument("--hub-systems", type=int, default=16)
@@ -506,7 +522,7 @@ def main():
         copy_calibration(args.copy_calibration, 5)
         return
     if args.summarize_trace:
-        summarize_trace(args.summarize_trace, args.last)
+        summarize_trace(args.summarize_trace, args.last, args.per_iter_kernel, args.ends_per_iter)
         return
     if args.traffic:
         print(json.dumps(traffic_table(*args.traffic, copy_bytes=args.copy_calibration or 268435456,
