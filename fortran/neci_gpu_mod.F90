@@ -16,6 +16,7 @@ module neci_gpu_mod
     integer(c_int), parameter, public :: NGPU_HOLES_FAST_FAST = 0, NGPU_HOLES_FULL_FULL = 1
     integer(c_int), parameter, public :: NGPU_CORE_BUILD_AUTO = 0, NGPU_CORE_BUILD_JOIN = 1, NGPU_CORE_BUILD_SCAN = 2
     integer(c_int), parameter, public :: NGPU_GROUP_CORE = 1, NGPU_GROUP_SHARED_CORE = 2
+    integer(c_int), parameter, public :: NGPU_CHECK_TABLE = 1, NGPU_CHECK_PGEN = 2, NGPU_CHECK_SELFTEST = 4
 
     type, bind(c), public :: ngpu_sys_t
         integer(c_int32_t) :: nel, nbasis, n_replicas, model
@@ -54,6 +55,7 @@ module neci_gpu_mod
     public :: ngpu_get_error, ngpu_set_excitgen_pchb, ngpu_download_walkers_dev, ngpu_most_populated
     public :: ngpu_select_core, ngpu_set_core_build, ngpu_get_core_hamil, ngpu_batch_get_groups
     public :: ngpu_batch_set_graph, ngpu_batch_set_profiling, ngpu_batch_get_group_flags
+    public :: ngpu_set_checks, ngpu_get_check_counts
 
     interface
         integer(c_int) function ngpu_create(h, sys) bind(c, name="ngpu_create")
@@ -275,6 +277,17 @@ module neci_gpu_mod
             type(c_ptr), value :: b
             integer(c_int32_t), value :: group
             integer(c_int32_t), intent(out) :: flags
+        end function
+        integer(c_int) function ngpu_set_checks(h, mask) bind(c, name="ngpu_set_checks")
+            import :: c_int, c_ptr, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int32_t), value :: mask
+        end function
+        integer(c_int) function ngpu_get_check_counts(h, counts, n) bind(c, name="ngpu_get_check_counts")
+            import :: c_int, c_ptr, c_int64_t, c_int32_t
+            type(c_ptr), value :: h
+            integer(c_int64_t), intent(out) :: counts(*)
+            integer(c_int32_t), value :: n
         end function
         integer(c_int) function ngpu_batch_set_graph(b, on) bind(c, name="ngpu_batch_set_graph")
             import :: c_int, c_ptr, c_int32_t

@@ -45,8 +45,10 @@ typedef enum {
     NGPU_ERR_HIP = 5,
     NGPU_ERR_HASH_FULL = 6,    /* every bucket of the walker hash is taken (MEMORYFACHASH) */
     NGPU_ERR_STATE = 7,        /* call out of order (e.g. iterate before set_reference) */
-    NGPU_ERR_EXCITGEN = 8      /* uniform generator found no allowed orbital within the reference's
+    NGPU_ERR_EXCITGEN = 8,     /* uniform generator found no allowed orbital within the reference's
                                   attempt bounds (Stop_All sites, GenRandSymExcitNUMod.F90:724-870) */
+    NGPU_ERR_CHECK = 9         /* an invariant of ngpu_set_checks failed (the reference's ASSERTs and
+                                  CalcHashTableStats stop_all, load_balancer.fpp:796-804) */
 } ngpu_status;
 
 enum { NGPU_MODEL_HUBBARD = 0, NGPU_MODEL_MOLECULAR = 1 };
@@ -353,6 +355,28 @@ int ngpu_get_error(ngpu_t *h, int32_t *status, uint32_t *iter);
    in the reference, whose chained buckets (hash.F90) have no tombstones; any
    argument may be NULL. */
 int ngpu_get_hash_stats(ngpu_t *h, int64_t *live, int64_t *tombstones, int64_t *entries, int64_t *rebuilds);
+
+/* Invariant checks re-evaluated on the device after every iteration (the
+   reference's DEBUG_ ASSERTs and hard checks; off by default, results are the
+   same either way):
+   NGPU_CHECK_TABLE: every walker slot below the high-water mark is a hole of
+     the free queue or a live determinant with nel electrons that a hash
+     lookup finds at its own slot; holes == free-queue length (AnnihilatedDet
+     == HolesInList, load_balancer.fpp:796-804); live slots == the live count;
+   NGPU_CHECK_PGEN: one excitation drawn from every occupied determinant and
+     replica has pgen equal to calc_pgen to 1e-12 relative (the ASSERT after
+     every draw, gasci_pchb_doubles_spinorb_fastweighted.fpp:191,231);
+   NGPU_CHECK_SELFTEST: with NGPU_CHECK_PGEN, the recomputed pgen is scaled by
+     1 + 1e-6, so the check fails (for testing the report).
+   A failed check halts the engine with NGPU_ERR_CHECK, as any other error of
+   the iteration.  Single-engine iterations only (not batches). */
+enum { NGPU_CHECK_TABLE = 1, NGPU_CHECK_PGEN = 2, NGPU_CHECK_SELFTEST = 4 };
+int ngpu_set_checks(ngpu_t *h, int32_t mask);
+/* Cumulative check counts (n <= 7): iterations checked, slots with a wrong
+   electron count, live slots the hash does not find at their slot, iterations
+   whose holes differ from the free-queue length, iterations whose live slots
+   differ from the live count, pgen tests, pgen mismatches. */
+int ngpu_get_check_counts(ngpu_t *h, int64_t *counts, int32_t n);
 
 /* Stage timing with events on the engine stream (set_timer/halt_timer analogue,
    lib/timing.F90:122-188).  Each stage is timed from the start of its first

@@ -22,6 +22,9 @@ PART_UNIF_FULL = 2
 PART_FULL_FULL = 3
 HOLES_FAST_FAST = 0
 HOLES_FULL_FULL = 1
+CHECK_TABLE = 1
+CHECK_PGEN = 2
+CHECK_SELFTEST = 4
 GROUP_CORE = 1
 GROUP_SHARED_CORE = 2
 CORE_BUILD_AUTO = 0
@@ -40,6 +43,7 @@ STATUS = {
     6: "NGPU_ERR_HASH_FULL (increase MEMORYFACHASH)",
     7: "NGPU_ERR_STATE",
     8: "NGPU_ERR_EXCITGEN (uniform generator found no allowed orbital)",
+    9: "NGPU_ERR_CHECK (an invariant check failed)",
 }
 
 
@@ -129,6 +133,8 @@ _SIGS = [
     ("ngpu_get_error", ctypes.c_int, [_vp, P(ctypes.c_int32), P(ctypes.c_uint32)]),
     ("ngpu_get_hash_stats", ctypes.c_int, [_vp, P(ctypes.c_int64), P(ctypes.c_int64), P(ctypes.c_int64),
                                            P(ctypes.c_int64)]),
+    ("ngpu_set_checks", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("ngpu_get_check_counts", ctypes.c_int, [_vp, _vp, ctypes.c_int32]),
     ("ngpu_set_profiling", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("ngpu_set_graph", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("ngpu_set_launch_shape", ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,

@@ -182,6 +182,8 @@ struct ngpu_engine {
   // replay of the captured iteration chain
   bool use_graph = false;
   bool graph_dirty = true;
+  uint32_t checks = 0;                  // NGPU_CHECK_* bits (ngpu_set_checks)
+  unsigned long long* d_chk = nullptr;  // per-iteration and cumulative check counts
   hipGraphExec_t gexec = nullptr;
   ngpu_batch* batch = nullptr;
   hipStream_t own_stream = nullptr;
@@ -515,6 +517,9 @@ void core_projection(ngpu_t* h) {
 }
 
 // merge -> annihilate -> insert -> stats, shared by iterate and the explicit-list entry
+template <int W>
+void check_chain(ngpu_t* h, bool gen);
+
 template <int W, int R>
 void annihilation_chain(ngpu_t* h, bool with_core, bool list_out = false) {
   // the projected core amplitudes join the walkers before the spawns do, as
@@ -531,6 +536,7 @@ void annihilation_chain(ngpu_t* h, bool with_core, bool list_out = false) {
          (const IterParams*)h->dparams);
   launch(h, -1, -1, k_end_iter, dim3(1), dim3(kBlock), h->dout, h->dpart, h->dcyc, h->cnt,
          h->scap, h->wk.cap, h->wk.L * (int64_t)kBucketEntries, (const IterParams*)h->dparams);
+  if (h->checks) check_chain<W>(h, with_core);
 }
 
 // (word, shift) passes of the word-key sort of an engine (ilut_lt order:
@@ -742,6 +748,136 @@ __device__ double calc_pgen(const DevSys& s, const uint64_t* d, int ic, const in
     return s.pdoubles * p;
   }
   return 0.0;
+}
+
+// ---- invariant checks (ngpu_set_checks) ----
+// Counts of one iteration in d_chk[0..7], cumulative counts in d_chk[8..15]:
+// holes, live slots, wrong electron counts, hash misses, pgen tests, pgen
+// mismatches; then the cumulative ones (ngpu_get_check_counts order).
+enum { kChkHoles = 0, kChkLive, kChkNel, kChkHash, kChkPgenN, kChkPgenBad, kChkCum = 8 };
+
+__device__ inline void chk_add(unsigned long long* chk, int i, unsigned long long v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (lane_id() == 0 && v) atomicAdd(&chk[i], v);
+}
+
+// the walker table after the iteration's bookkeeping: every slot below the
+// high-water mark is either removed (a hole in the free queue) or live, and a
+// live one carries nel electrons and is found by a hash lookup at its own slot
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_check_table(DevSys s, Walk wk, Counters* cnt, unsigned long long* chk) {
+  const int64_t nslots = (int64_t)cnt->nslots;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t trips = (nslots + stride - 1) / stride;
+  for (int64_t t = 0; t < trips; ++t) {  // every lane runs every trip (wave-wide sums)
+    const int64_t slot = t * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long holes = 0, live = 0, bad_nel = 0, bad_hash = 0;
+    if (slot < nslots) {
+      if (wk.flags(slot) & kFlagRemoved) {
+        holes = 1;
+      } else {
+        live = 1;
+        uint64_t key[W];
+        int nel = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          key[w] = wk.keys[slot * W + w];
+          nel += popc64(key[w]);
+        }
+        bad_nel = nel != s.nel;
+        const int64_t home = find_walker_hash<W>(key, s.rh, s.L) - 1;
+        const int64_t pos = hash_lookup<W>(wk, key, home, cnt);
+        const int64_t hp = wk.hpos(slot);
+        const int32_t* sl = (const int32_t*)(wk.htab + (size_t)(hp >> 2) * 8 * W + 4 * W);
+        bad_hash = pos != slot || sl[hp & 3] != (int32_t)slot;
+      }
+    }
+    chk_add(chk, kChkHoles, holes);
+    chk_add(chk, kChkLive, live);
+    chk_add(chk, kChkNel, bad_nel);
+    chk_add(chk, kChkHash, bad_hash);
+  }
+}
+
+// the generators' pgen against calc_pgen (the reference asserts
+// pgen .isclose. get_pgen after every draw, gasci_pchb_doubles_spinorb_
+// fastweighted.fpp:191,231): one attempt of every occupied determinant and
+// replica, drawn as the iteration's first attempt; off = 1 shifts the
+// recomputed value by 1e-6 (a deliberate mismatch, for testing the report)
+template <int W>
+__global__ void __launch_bounds__(kBlock) k_check_pgen(DevSys s, Walk wk, const Counters* cnt,
+                                                       unsigned long long* chk, const IterParams* prm, int off) {
+  const int64_t nslots = (int64_t)cnt->nslots;
+  const int S = s.S;
+  const int64_t n = nslots * S;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t trips = (n + stride - 1) / stride;
+  for (int64_t t = 0; t < trips; ++t) {
+    const int64_t q = t * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long tests = 0, bad = 0;
+    if (q < n) {
+      const int64_t slot = q / S;
+      const int r = (int)(q - slot * S);
+      if (!(wk.flags(slot) & kFlagRemoved) && wk.sgn(slot, r) != 0.0) {
+        uint64_t key[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
+        double dr[6];
+        draw2<W>(s.seed, key, prm->iter, r, 0, 1, dr[0], dr[1]);
+        draw2<W>(s.seed, key, prm->iter, r, 0, 2, dr[3], dr[5]);
+        draw2<W>(s.seed, key, prm->iter, r, 0, 3, dr[2], dr[4]);
+        const Excit ex = gen_excit<W>(s, key, dr, prm->iter, r, 0, nullptr);
+        if (ex.ic) {
+          const int32_t e[4] = {ex.e0, ex.e1, ex.e2, ex.e3};
+          double p2 = calc_pgen<W>(s, key, ex.ic, e);
+          if (off) p2 *= 1.0 + 1e-6;
+          tests = 1;
+          const double a = fabs(ex.pgen), b = fabs(p2);
+          bad = !(fabs(ex.pgen - p2) <= 1e-12 * (a > b ? a : b));
+        }
+      }
+    }
+    chk_add(chk, kChkPgenN, tests);
+    chk_add(chk, kChkPgenBad, bad);
+  }
+}
+
+// the verdict (one thread): CalcHashTableStats's AnnihilatedDet == HolesInList
+// (load_balancer.fpp:796-804) as holes == free-queue length, live slots ==
+// the live count, and no bad slot or pgen.  A failure halts the chain like any
+// other error of the iteration (NGPU_ERR_CHECK).
+__global__ void k_check_final(Counters* cnt, unsigned long long* chk, ngpu_iter_out_t* out, ngpu_iter_out_t* cyc,
+                              const IterParams* prm) {
+  const unsigned long long queue = cnt->free_tail - cnt->free_head;
+  const bool holes_bad = chk[kChkHoles] != queue, live_bad = chk[kChkLive] != cnt->n_live;
+  chk[kChkCum + 0] += 1;  // iterations checked
+  chk[kChkCum + 1] += chk[kChkNel];
+  chk[kChkCum + 2] += chk[kChkHash];
+  chk[kChkCum + 3] += holes_bad ? 1 : 0;
+  chk[kChkCum + 4] += live_bad ? 1 : 0;
+  chk[kChkCum + 5] += chk[kChkPgenN];
+  chk[kChkCum + 6] += chk[kChkPgenBad];
+  const bool bad = chk[kChkNel] || chk[kChkHash] || chk[kChkPgenBad] || holes_bad || live_bad;
+  for (int i = 0; i < kChkCum; ++i) chk[i] = 0;
+  if (bad && !cnt->halt) {
+    const unsigned int m = 1u << NGPU_ERR_CHECK;
+    cnt->halt = m;
+    cnt->err_iter = prm->iter;
+    out->err |= (int32_t)m;
+    cyc->err |= (int32_t)m;
+  }
+}
+
+template <int W>
+void check_chain(ngpu_t* h, bool gen) {
+  if (h->checks & NGPU_CHECK_TABLE)
+    launch(h, -1, -1, k_check_table<W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, h->cnt, h->d_chk);
+  if (gen && (h->checks & NGPU_CHECK_PGEN))
+    launch(h, -1, -1, k_check_pgen<W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->wk, (const Counters*)h->cnt,
+           h->d_chk, (const IterParams*)h->dparams, (h->checks & NGPU_CHECK_SELFTEST) ? 1 : 0);
+  launch(h, -1, -1, k_check_final, dim3(1), dim3(1), h->cnt, h->d_chk, h->dout, h->dcyc,
+         (const IterParams*)h->dparams);
 }
 
 template <int W>
@@ -966,7 +1102,7 @@ int ngpu_destroy(ngpu_t* h) {
   void* ptrs[] = {h->wk.keys, h->wk.st, h->wk.diagH, h->wk.offdiagH, h->wk.freeq,
                   h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->blk_rm, h->sp_rec,
                   h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->digit_tot, h->sq, h->n_key,
-                  h->n_sgn, h->n_flag, h->u_key, h->u_sgn, h->u_flag, h->u_count, h->cnt, h->dout, h->dpart, h->dcyc, h->own_params, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_fref, h->d_binom, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
+                  h->n_sgn, h->n_flag, h->u_key, h->u_sgn, h->u_flag, h->u_count, h->cnt, h->dout, h->dpart, h->dcyc, h->own_params, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_fref, h->d_chk, h->d_binom, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
                   h->core_vals, h->core_x, h->core_y, h->d_ab_tab,
                   h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt, h->d_i_tab, h->d_i_prob,
                   h->d_a_tab, h->d_a_prob, h->d_a_ok, h->d_b_tab, h->d_b_prob, h->d_b_ok};
@@ -2032,6 +2168,37 @@ int ngpu_get_core_hamil(ngpu_t* h, int64_t* rowptr, int32_t* cols, double* vals,
   } else {
     HIP_OK(hipMemcpy(vals, h->core_vals, nz * 8, hipMemcpyDeviceToHost));
   }
+  return NGPU_OK;
+}
+
+int ngpu_set_checks(ngpu_t* h, int32_t mask) {
+  if (!h || mask < 0 || (mask & ~(NGPU_CHECK_TABLE | NGPU_CHECK_PGEN | NGPU_CHECK_SELFTEST))) return NGPU_ERR_ARG;
+  if (h->batch) return NGPU_ERR_STATE;
+  if (h->pending) {
+    int rc = finish(h, nullptr);
+    if (rc) return rc;
+  }
+  if (!h->d_chk) {
+    HIP_OK(dalloc(&h->d_chk, 16));
+    HIP_OK(hipMemset(h->d_chk, 0, 16 * sizeof(unsigned long long)));
+  }
+  h->checks = (uint32_t)mask;
+  h->graph_dirty = true;  // the captured chain changes
+  return NGPU_OK;
+}
+
+int ngpu_get_check_counts(ngpu_t* h, int64_t* counts, int32_t n) {
+  if (!h || !counts || n < 1 || n > 7) return NGPU_ERR_ARG;
+  for (int i = 0; i < n; ++i) counts[i] = 0;
+  if (!h->d_chk) return NGPU_OK;
+  if (h->pending) {
+    int rc = finish(h, nullptr);
+    if (rc && rc != NGPU_ERR_CHECK) return rc;
+  }
+  unsigned long long c[16];
+  HIP_OK(hipStreamSynchronize(h->stream));
+  HIP_OK(hipMemcpy(c, h->d_chk, sizeof(c), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; ++i) counts[i] = (int64_t)c[8 + i];
   return NGPU_OK;
 }
 

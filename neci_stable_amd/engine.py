@@ -223,6 +223,19 @@ class Engine:
         check(self.lib.ngpu_set_spawn_precision(self.h, int(bool(fp32))), "ngpu_set_spawn_precision")
         self.spec.spawn_fp32 = bool(fp32)
 
+    def set_checks(self, mask):
+        """Invariant checks after every iteration (ngpu_set_checks): CHECK_TABLE,
+        CHECK_PGEN, CHECK_SELFTEST bits; 0 switches them off."""
+        check(self.lib.ngpu_set_checks(self.h, int(mask)), "ngpu_set_checks")
+
+    def check_counts(self):
+        """Cumulative counts: iterations, bad electron counts, hash misses,
+        hole mismatches, live-count mismatches, pgen tests, pgen mismatches."""
+        c = np.zeros(7, dtype=np.int64)
+        check(self.lib.ngpu_get_check_counts(self.h, ptr(c), 7), "ngpu_get_check_counts")
+        return dict(zip(("iterations", "bad_nel", "hash_misses", "hole_mismatches", "live_mismatches",
+                         "pgen_tests", "pgen_mismatches"), (int(x) for x in c)))
+
     def set_launch_shape(self, chunk_blocks=0, grid_blocks=0, radix_items=0, single_lanes=0, merge_blocks=0):
         check(self.lib.ngpu_set_launch_shape(self.h, int(chunk_blocks), int(grid_blocks), int(radix_items),
                                              int(single_lanes), int(merge_blocks)), "ngpu_set_launch_shape")
