@@ -547,8 +547,9 @@ __device__ __host__ inline Excit gen_hubbard(const DevSys& s, const uint64_t* d,
 // then minus the same for every orbital the reference occupies and d does
 // not.  The same value as sltcnd_1(d, src, orb) up to FP64 rounding (exact
 // zeros stay exact: every term of a zero element is zero); its cost is
-// 2 * (excitation level of d) rows instead of nel.  Only the weights take this
-// form: the matrix element of the spawn is sltcnd_1(d, src, orb) itself.
+// 2 * (excitation level of d) rows instead of nel.  The signed value of the
+// selected orbital is also the element the single spawns with (the
+// generator's HElGen), so it is not summed a second time.
 template <int W>
 __device__ __host__ inline double hb_weight(const DevSys& s, const uint64_t* d, int src, int orb) {
   const int a = spat(src), b = spat(orb), nsp = s.nsp;
@@ -602,11 +603,12 @@ __device__ __host__ inline Excit gen_single_heatbath(const DevSys& s, const uint
   }
   if (fabs(cum_sum) <= kEps) return ex;
   double r = dorb * cum_sum;
-  double cum = 0.0, cpt = 0.0;
+  double cum = 0.0, cpt = 0.0, v = 0.0;
   int orb = 0;
   for (int i = 1; i <= nsp; ++i) {
     orb = 2 * i - off;
-    cpt = test_orb<W>(d, orb) ? 0.0 : fabs(hb_weight<W>(s, d, src, orb));
+    v = test_orb<W>(d, orb) ? 0.0 : hb_weight<W>(s, d, src, orb);
+    cpt = fabs(v);
     cum += cpt;
     if (!(cum < r) || i == nsp) break;
   }
@@ -618,7 +620,7 @@ __device__ __host__ inline Excit gen_single_heatbath(const DevSys& s, const uint
   ex.e1 = orb;
   ex.parity = single_parity<W>(d, src, orb);
   ex.pgen = pg;
-  ex.hel = sltcnd_1<W>(s, d, src, orb);  // the element the spawn needs
+  ex.hel = v;  // the element the spawn uses (hb_weight's sum, no parity)
   ex.hel_ok = true;
   return ex;
 }

@@ -1045,8 +1045,8 @@ __global__ void __launch_bounds__(kBlock) k_spawn(DevSys s, Walk wk, Counters* c
 // and the reference differ (contiguous in b, so a group's loads for one j are
 // one contiguous run), added in hb_weight's order, and stores it in LDS; both
 // sweeps then run in ascending b over LDS on every lane of the group, so the
-// selected orbital and pgen are bit-identical to the one-lane generator.  The
-// spawn's matrix element is sltcnd_1 of the selected orbital (group leader).
+// selected orbital, pgen and the spawn's element (the selected weight, signed)
+// are bit-identical to the one-lane generator.
 // The attempt's three Philox blocks are spread over the group's lanes and
 // shared.  More lanes per attempt shorten each attempt's chain of loads
 // (measured at 20 spatial orbitals: 1, 2 and 4 lanes took 116, 64 and 46 us
@@ -1186,10 +1186,11 @@ __device__ __forceinline__ void k_spawn_singles_body(const Grid& g_, const DevSy
       double cum_sum = 0.0;
       for (int b = 0; b < nsp; ++b) cum_sum += fabs(myv[b]);
       double rr = d2 * cum_sum;
-      double cum = 0.0, cpt = 0.0;
+      double cum = 0.0, cpt = 0.0, vsel = 0.0;
       int bsel = 0;
       for (int b = 1; b <= nsp; ++b) {
         double vb = myv[b - 1];
+        vsel = vb;
         cpt = fabs(vb);
         bsel = b;
         cum += cpt;
@@ -1219,8 +1220,7 @@ __device__ __forceinline__ void k_spawn_singles_body(const Grid& g_, const DevSy
           double pg = cpt / cum_sum;
           pg = pg / (double)s.nel;
           pg = pg * s.psingles;
-          const double hv = sltcnd_1<W>(s, key, src, tgt);
-          double hel = single_parity<W>(key, src, tgt) ? -hv : hv;
+          double hel = single_parity<W>(key, src, tgt) ? -vsel : vsel;  // the weight's own sum
           uint8_t pi = pinfo[p];
           double wsign = (pi & 1) ? -1.0 : 1.0;
           nspawn = attempt_create(s, tau, hel, pg, wsign, d5, raw);

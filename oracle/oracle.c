@@ -679,7 +679,8 @@ static int gen_hubbard(const osys_t *s, const uint64_t *ilut, const int32_t *nI,
    U(a,j,j,b) for every orbital j the determinant occupies and the reference
    does not (ascending, src skipped), minus the same for every orbital the
    reference occupies and the determinant does not.  Equal to sltcnd_1(nI, src,
-   orb) up to FP64 rounding; the device's hb_weight sums in this order. */
+   orb) up to FP64 rounding; the device's hb_weight sums in this order.  The
+   signed value is also the element a heat-bath single spawns with. */
 static double hb_weight(const osys_t *s, const uint64_t *ilut, int src, int orb)
 {
     int W = nwords(s->nbasis);
@@ -700,6 +701,11 @@ static double hb_weight(const osys_t *s, const uint64_t *ilut, int src, int orb)
             }
         }
     return v;
+}
+
+double o_hb_weight(const osys_t *s, const uint64_t *ilut, int32_t src, int32_t orb)
+{
+    return hb_weight(s, ilut, src, orb);
 }
 
 /* gen_single_4ind_ex + select_orb_sing (excit_gens_int_weighted.F90:442-620),
@@ -2160,8 +2166,16 @@ static int det_loop(ostate_t *st, uint32_t iter, const double *diag_sft, ostats_
                 if (core && (ci ? core_index_has(ci, ilutJ) : (st->ncore && is_core_det(st, ilutJ)))) continue;
                 int32_t nI[O_MAX_NEL];
                 o_decode(row, W, nI);
-                /* attempt_create_normal (fcimc_pointed_fns.F90:178-491) */
-                double hel = o_get_helement_ex(s, nI, ic, ex, tpar);
+                /* attempt_create_normal (fcimc_pointed_fns.F90:178-491); a heat-bath
+                   single spawns with the element its weight was drawn from (hb_weight,
+                   the generator's HElGen), a double with sltcnd_2 */
+                double hel;
+                if (ic == 1 && s->model == O_MODEL_MOLECULAR && !s->uniform) {
+                    hel = hb_weight(s, row, ex[0], ex[1]);
+                    if (tpar) hel = -hel;
+                } else {
+                    hel = o_get_helement_ex(s, nI, ic, ex, tpar);
+                }
                 double prob = pgen * s->avmc;
                 double nspawn = -s->tau * hel * wsign / prob;
                 if (fabs(nspawn) > out->max_cyc_spawn) out->max_cyc_spawn = fabs(nspawn);

@@ -121,6 +121,8 @@ int32_t o_gen_excit(const osys_t *s, const uint64_t *ilut, const double *d, int3
 double o_calc_pgen(const osys_t *s, const uint64_t *ilut, int32_t ic, const int32_t *ex);
 /* as o_gen_excit with the draw context of spawn attempt k of (det, iter, replica):
    generators with rejection loops draw beyond d[1..4] from that context */
+/* the heat-bath single weight H(src -> orb) relative to the reference (signed) */
+double o_hb_weight(const osys_t *s, const uint64_t *ilut, int32_t src, int32_t orb);
 int32_t o_gen_excit_ctx(const osys_t *s, const uint64_t *ilut, const double *d, uint32_t iter, int32_t rep,
                         uint32_t k, int32_t *nJ, uint64_t *ilutJ, int32_t *ic, int32_t *ex, int32_t *tparity,
                         double *pgen, int32_t *fail);

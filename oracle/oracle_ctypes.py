@@ -84,6 +84,7 @@ def lib():
             "o_get_pchb_row": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp]),
             "o_gen_excit": (ctypes.c_int32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
             "o_calc_pgen": (ctypes.c_double, [_vp, _vp, ctypes.c_int32, _vp]),
+            "o_hb_weight": (ctypes.c_double, [_vp, _vp, ctypes.c_int32, ctypes.c_int32]),
             "o_create": (_vp, [_vp]),
             "o_destroy": (None, [_vp]),
             "o_add_walker": (ctypes.c_int32, [_vp, _vp, _vp, ctypes.c_uint32]),
@@ -308,6 +309,12 @@ class Oracle:
         if not ok:
             return None, fail.value
         return (dj.view(np.int64), ic.value, ex.copy(), tp.value, pg.value), fail.value
+
+    def hb_weight(self, det, src, orb):
+        """The heat-bath single weight H(src -> orb) relative to the reference
+        (signed, no parity): the value a heat-bath single spawns with."""
+        det = np.ascontiguousarray(det, dtype=np.int64).view(np.uint64)
+        return self.L.o_hb_weight(ctypes.byref(self.sys), p(det), int(src), int(orb))
 
     def calc_pgen(self, det, ic, ex):
         det = np.ascontiguousarray(det, dtype=np.int64).view(np.uint64)

@@ -136,3 +136,34 @@ def test_core_build_rejects_repeated_determinants_and_late_method_change():
     with pytest.raises(NgpuError):
         eng.set_core_build(CORE_BUILD_SCAN)
     eng.close()
+
+
+def test_default_build_equals_oracle_50k_core_14e_30o():
+    """The 50 000 most populated determinants of a grown 14e/30o population
+    (seed 7, rank 60): the default build (the join at this size) gives the
+    oracle's core Hamiltonian, rows and columns exactly and values to 1e-12."""
+    from neci_stable_amd import Spec, synthetic_molecule
+    spec = Spec(synthetic_molecule(30, 14, 60, 7), init_walkers=4e5, seed=71, particles=0, psingles=0.1,
+                memory_fac_spawn=6.0)
+    eng = engine(spec)
+    eng.start_single_part(1000.0)
+    it = 0
+    while True:
+        for _ in range(20):
+            it += 1
+            eng.iterate(it, 0.001, 0.0, wait=False)
+        assert eng.sync() == 0
+        if float(np.sum(eng.stats()["tot_parts"])) > 2e5 or it >= 600:
+            break
+    core = eng.most_populated(50000)
+    eng.close()
+    assert len(core) == 50000, (it, len(core))
+    (rp, cols, vals), dt = _built(spec, core, CORE_BUILD_AUTO, False)
+    print(f"50 000 core determinants after {it} iterations: {len(cols)} elements, built in {dt:.3f} s", flush=True)
+    orc = oracle(spec, 0.001)
+    assert orc.set_core(core) == len(cols)
+    orp, ocols, ovals = orc.core_csr()
+    orc.close()
+    np.testing.assert_array_equal(rp, orp)
+    np.testing.assert_array_equal(cols, ocols)
+    np.testing.assert_allclose(vals, ovals, rtol=1e-12, atol=1e-13)

@@ -262,3 +262,30 @@ def test_hubbard_generator_pgen_closed_form():
                 assert abs(pg - expect) <= RTOL * expect
                 total += pg
         assert total == pytest.approx(1.0 - p_null, rel=0, abs=1e-12)
+
+
+@pytest.mark.parametrize("nspat,nel,nalpha", [(5, 4, 2), (6, 5, 3), (7, 6, 3)])
+def test_heatbath_single_element_is_the_exact_element(nspat, nel, nalpha):
+    """The reference-relative weight a heat-bath single is drawn with and
+    spawns with (the oracle's hb_weight, which the device restates term for
+    term) equals |<D_i^a|H|D>| from second quantisation to 1e-12, for every
+    single excitation of determinants up to several excitations away from the
+    reference."""
+    mol = molecule(nspat, nel, rank=6, seed=13)
+    nb = 2 * nspat
+    spec = Spec(mol, init_walkers=100, particles=0, psingles=0.2)
+    o = oc.Oracle(spec, 0.01)
+    n = 0
+    for orbs in dets_with(nb, nel, nalpha, 8, seed=nspat):
+        det = encode_det(orbs, nb)
+        occ = set(orbs)
+        for i in orbs:
+            for a in range(1, nb + 1):
+                if a in occ or a % 2 != i % 2:
+                    continue
+                w = o.hb_weight(det, i, a)
+                h = abs(sq_element(mol.h1, mol.eri, sorted((occ - {i}) | {a}), orbs))
+                assert abs(abs(w) - h) <= 1e-12 * max(h, 1.0), (orbs, i, a, w, h)
+                n += 1
+    assert n > 50
+    o.close()
