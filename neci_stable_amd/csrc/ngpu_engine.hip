@@ -35,6 +35,91 @@ hipError_t dalloc(T** p, size_t n) {
   return hipMalloc((void**)p, n * sizeof(T));
 }
 
+// Read-only tables (integrals, heat-bath rows, alias tables, lattice) shared
+// by every engine of the process on one device whose table has the same
+// bytes: replicas of one system then read one copy, which the L2 and the
+// MALL can hold, instead of one copy each.  Reference counted; the last
+// engine to let go frees it.
+struct SharedTable {
+  int dev;
+  size_t bytes;
+  uint64_t hash;
+  std::vector<unsigned char> host;
+  void* d;
+  int refs;
+};
+std::mutex g_tab_mu;
+std::vector<SharedTable*> g_tabs;
+
+uint64_t table_hash(const unsigned char* p, size_t n) {
+  uint64_t h = 0x9E3779B97F4A7C15ull ^ (uint64_t)n;
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    memcpy(&w, p + i, 8);
+    h = (h ^ (w * 0xBF58476D1CE4E5B9ull)) * 0x94D049BB133111EBull;
+    h ^= h >> 29;
+  }
+  for (; i < n; ++i) h = (h ^ p[i]) * 0x100000001B3ull;
+  return h;
+}
+
+// *dst: a device copy of the bytes (an existing identical table, or a new one)
+int table_acquire(const void* host, size_t bytes, void** dst) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return NGPU_ERR_HIP;
+  if (bytes == 0) bytes = 1;
+  const unsigned char* hp = (const unsigned char*)host;
+  const uint64_t hh = table_hash(hp, bytes);
+  std::lock_guard<std::mutex> lock(g_tab_mu);
+  for (SharedTable* t : g_tabs)
+    if (t->dev == dev && t->bytes == bytes && t->hash == hh && memcmp(t->host.data(), hp, bytes) == 0) {
+      t->refs++;
+      *dst = t->d;
+      return NGPU_OK;
+    }
+  void* d = nullptr;
+  if (hipMalloc(&d, bytes) != hipSuccess) return NGPU_ERR_HIP;
+  if (hipMemcpy(d, hp, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return NGPU_ERR_HIP;
+  }
+  SharedTable* t = new SharedTable{dev, bytes, hh, std::vector<unsigned char>(hp, hp + bytes), d, 1};
+  g_tabs.push_back(t);
+  *dst = d;
+  return NGPU_OK;
+}
+
+// releases device memory: a shared table when its last user lets go, any
+// other allocation at once
+void table_release(void* d) {
+  if (!d) return;
+  {
+    std::lock_guard<std::mutex> lock(g_tab_mu);
+    for (size_t i = 0; i < g_tabs.size(); ++i)
+      if (g_tabs[i]->d == d) {
+        if (--g_tabs[i]->refs == 0) {
+          (void)hipFree(d);
+          delete g_tabs[i];
+          g_tabs.erase(g_tabs.begin() + (long)i);
+        }
+        return;
+      }
+  }
+  (void)hipFree(d);
+}
+
+template <typename T>
+int table_upload(T** dst, const T* host, size_t n) {
+  table_release(*dst);
+  *dst = nullptr;
+  void* d = nullptr;
+  int rc = table_acquire(host, n * sizeof(T), &d);
+  if (rc) return rc;
+  *dst = (T*)d;
+  return NGPU_OK;
+}
+
 // NECI flag bit numbers (bit_rep_data.F90:87-118)
 const int kMneciInit[16] = {13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 57, 58};
 const int kMneciDeterm[16] = {41, 42, 43, 44, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 61, 62};
@@ -1106,8 +1191,7 @@ int ngpu_destroy(ngpu_t* h) {
                   h->core_vals, h->core_x, h->core_y, h->d_ab_tab,
                   h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt, h->d_i_tab, h->d_i_prob,
                   h->d_a_tab, h->d_a_prob, h->d_a_ok, h->d_b_tab, h->d_b_prob, h->d_b_ok};
-  for (void* p : ptrs)
-    if (p) (void)hipFree(p);
+  for (void* p : ptrs) table_release(p);
   if (h->hout) (void)hipHostFree(h->hout);
   if (h->hcnt) (void)hipHostFree(h->hcnt);
   if (h->ev_ok)
@@ -1138,17 +1222,21 @@ int refresh_fref(ngpu_t* h) {
   if (h->sys.model != NGPU_MODEL_MOLECULAR || !h->have_model || !h->have_ref) return NGPU_OK;
   const size_t n = (size_t)h->sys.nbasis * h->ds.nsp;
   if (h->stream) HIP_OK(hipStreamSynchronize(h->stream));
-  if (h->d_fref) (void)hipFree(h->d_fref);
+  table_release(h->d_fref);
   h->d_fref = nullptr;
   h->ds.fref = nullptr;
-  HIP_OK(dalloc(&h->d_fref, n));
+  DevBuf<double> tmp;
+  HIP_OK(tmp.alloc(n));
   const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
   if (h->W == 1)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fock_ref<1>), grid, dim3(kBlock), 0, h->stream, h->ds, h->d_fref);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fock_ref<1>), grid, dim3(kBlock), 0, h->stream, h->ds, tmp.p);
   else
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fock_ref<2>), grid, dim3(kBlock), 0, h->stream, h->ds, h->d_fref);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fock_ref<2>), grid, dim3(kBlock), 0, h->stream, h->ds, tmp.p);
   HIP_OK(hipGetLastError());
+  std::vector<double> host(n);
+  HIP_OK(hipMemcpyAsync(host.data(), tmp.p, n * 8, hipMemcpyDeviceToHost, h->stream));
   HIP_OK(hipStreamSynchronize(h->stream));
+  if (int rc = table_upload(&h->d_fref, host.data(), n)) return rc;
   h->ds.fref = h->d_fref;
   return NGPU_OK;
 }
@@ -1162,14 +1250,8 @@ int ngpu_set_integrals(ngpu_t* h, const double* tmat2d, const double* umat, int6
   int64_t npair = nsp * (nsp + 1) / 2;
   if (umat_len < npair * (npair + 1) / 2) return NGPU_ERR_ARG;
   size_t nt = (size_t)h->sys.nbasis * h->sys.nbasis;
-  if (h->d_tmat) (void)hipFree(h->d_tmat);
-  if (h->d_umat) (void)hipFree(h->d_umat);
-  h->d_tmat = nullptr;
-  h->d_umat = nullptr;
-  HIP_OK(dalloc(&h->d_tmat, nt));
-  HIP_OK(dalloc(&h->d_umat, (size_t)umat_len));
-  HIP_OK(hipMemcpy(h->d_tmat, tmat2d, nt * 8, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(h->d_umat, umat, (size_t)umat_len * 8, hipMemcpyHostToDevice));
+  if (int rc = table_upload(&h->d_tmat, tmat2d, nt)) return rc;
+  if (int rc = table_upload(&h->d_umat, umat, (size_t)umat_len)) return rc;
   {
     // (Coulomb, exchange) rows of the heat-bath single-excitation weights
     std::vector<double> jk((size_t)nsp * nsp * nsp * 2);
@@ -1180,10 +1262,7 @@ int ngpu_set_integrals(ngpu_t* h, const double* tmat2d, const double* umat, int6
           jk[e] = umat[umat_ind(a, j, b, j) - 1];
           jk[e + 1] = umat[umat_ind(a, j, j, b) - 1];
         }
-    if (h->d_jk) (void)hipFree(h->d_jk);
-    h->d_jk = nullptr;
-    HIP_OK(dalloc(&h->d_jk, jk.size()));
-    HIP_OK(hipMemcpy(h->d_jk, jk.data(), jk.size() * 8, hipMemcpyHostToDevice));
+    if (int rc = table_upload(&h->d_jk, jk.data(), jk.size())) return rc;
     // (J, K) pairs of the diagonal elements
     std::vector<double> jk0((size_t)nsp * nsp * 2);
     for (int i = 1; i <= nsp; ++i)
@@ -1192,10 +1271,7 @@ int ngpu_set_integrals(ngpu_t* h, const double* tmat2d, const double* umat, int6
         jk0[e] = umat[umat_ind(i, j, i, j) - 1];
         jk0[e + 1] = umat[umat_ind(i, j, j, i) - 1];
       }
-    if (h->d_jk0) (void)hipFree(h->d_jk0);
-    h->d_jk0 = nullptr;
-    HIP_OK(dalloc(&h->d_jk0, jk0.size()));
-    HIP_OK(hipMemcpy(h->d_jk0, jk0.data(), jk0.size() * 8, hipMemcpyHostToDevice));
+    if (int rc = table_upload(&h->d_jk0, jk0.data(), jk0.size())) return rc;
   }
   h->ds.tmat = h->d_tmat;
   h->ds.umat = h->d_umat;
@@ -1213,10 +1289,7 @@ int ngpu_set_lattice(ngpu_t* h, const int32_t* nbr, int32_t nsites, int32_t maxn
   if (nsites * 2 != h->sys.nbasis || maxnbr < 1 || maxnbr > 16) return NGPU_ERR_ARG;
   for (int64_t i = 0; i < (int64_t)nsites * maxnbr; ++i)
     if (nbr[i] < 0 || nbr[i] > nsites) return NGPU_ERR_ARG;
-  if (h->d_nbr) (void)hipFree(h->d_nbr);
-  h->d_nbr = nullptr;
-  HIP_OK(dalloc(&h->d_nbr, (size_t)nsites * maxnbr));
-  HIP_OK(hipMemcpy(h->d_nbr, nbr, (size_t)nsites * maxnbr * 4, hipMemcpyHostToDevice));
+  if (int rc = table_upload(&h->d_nbr, nbr, (size_t)nsites * maxnbr)) return rc;
   h->ds.nbr = h->d_nbr;
   h->ds.nsites = nsites;
   h->ds.maxnbr = maxnbr;
@@ -1243,11 +1316,7 @@ bool alias_row(const std::vector<double>& w, AliasEnt* tab, double* prob) {
 
 template <typename T>
 int upload_table(T** dst, const std::vector<T>& src) {
-  if (*dst) (void)hipFree(*dst);
-  *dst = nullptr;
-  HIP_OK(dalloc(dst, src.size()));
-  HIP_OK(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
-  return NGPU_OK;
+  return table_upload(dst, src.data(), src.size());
 }
 }  // namespace
 }  // extern "C++"
