@@ -171,6 +171,63 @@ def synthetic_molecule(nspat, nel, rank, seed):
     return MolecularSystem(nspat, nel, h, eri, ecore=0.0)
 
 
+def freeze_core(mol, nfrozen=1, orbitals=None):
+    """The FREEZE keyword's core part (FREEZE n 0 freezes n = 2*nfrozen spin
+    orbitals; IntFREEZEBASIS, Integrals_neci.F90:1083-1140 for ECORE and
+    :1228-1340 for TMAT): the frozen spatial orbitals are doubly
+    occupied in every determinant and leave the basis.  Their mean field moves
+    into the one-electron integrals and the core energy:
+      h'_pq = h_pq + sum_c [2 (pq|cc) - (pc|cq)],
+      E'    = E_core + sum_c 2 h_cc + sum_cd [2 (cc|dd) - (cd|dc)],
+    so every Hamiltonian element between determinants with the frozen
+    orbitals doubly occupied is unchanged.  The frozen orbitals are the
+    nfrozen lowest by h_pp (the reference orders by orbital energy; the two
+    agree for core orbitals), or the 0-based spatial indices in orbitals.
+    Returns a MolecularSystem of nspat - nfrozen orbitals and
+    nel - 2 nfrozen electrons (orbsym kept for the remaining orbitals)."""
+    n = mol.nspat
+    if orbitals is None:
+        core = [int(c) for c in np.argsort(np.diag(mol.h1), kind="stable")[:nfrozen]]
+    else:
+        core = sorted(int(c) for c in orbitals)
+    if len(set(core)) != len(core) or any(c < 0 or c >= n for c in core) or 2 * len(core) > mol.nel:
+        raise ValueError(f"cannot freeze orbitals {core} of {n} with {mol.nel} electrons")
+    act = [p for p in range(n) if p not in core]
+
+    def eri(p, q, r, s):  # chemist (pq|rs), 0-based
+        if mol.eri is not None:
+            return mol.eri[p, q, r, s]
+        return mol.umat[umat_ind(p + 1, r + 1, q + 1, s + 1) - 1]
+
+    e = mol.ecore
+    for c in core:
+        e += 2.0 * mol.h1[c, c]
+        for d in core:
+            e += 2.0 * eri(c, c, d, d) - eri(c, d, d, c)
+    na = len(act)
+    h = np.zeros((na, na))
+    for a, p in enumerate(act):
+        for b, q in enumerate(act):
+            v = mol.h1[p, q]
+            for c in core:
+                v += 2.0 * eri(p, q, c, c) - eri(p, c, c, q)
+            h[a, b] = v
+    u = np.zeros(umat_len(na))
+    for i in range(1, na + 1):
+        for j in range(1, i + 1):
+            for k in range(1, na + 1):
+                for l in range(1, k + 1):
+                    if i * (i - 1) // 2 + j < k * (k - 1) // 2 + l:
+                        continue
+                    u[umat_ind(i, k, j, l) - 1] = eri(act[i - 1], act[j - 1], act[k - 1], act[l - 1])
+    out = MolecularSystem(na, mol.nel - 2 * len(core), h, None, ecore=e, ms2=mol.ms2)
+    out.umat = u
+    if getattr(mol, "orbsym", None) is not None:
+        out.orbsym = np.asarray(mol.orbsym)[act]
+        out.isym = getattr(mol, "isym", 1)
+    return out
+
+
 # --------------------------------------------------------------- FCIDUMP
 
 
