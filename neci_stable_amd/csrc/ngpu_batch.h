@@ -111,15 +111,16 @@ __global__ void __launch_bounds__(kBlock) km_spawn_singles(const EngDev* __restr
 }
 
 // radix pass p of the word sort: the first pass reads the keys inside the
-// spawn records, later passes the previous pass's output (ping-pong cur)
+// spawn records (first = 1) or the 64-bit colex ranks in sk[cur] (first = 2),
+// later passes the previous pass's output (ping-pong cur)
 template <typename K, int NW, bool FLIP, int IT, int DB>
 __global__ void __launch_bounds__(kBlock) km_radix_hist_words(const EngDev* __restrict__ ed,
                                                               const uint16_t* __restrict__ map, int first, int cur,
                                                               int word, int shift) {
   Grid g;
   const EngDev& a = eng_of(ed, map, kLkChunk, g);
-  const K* kin = first ? (const K*)a.sp_rec : (const K*)a.sk[cur];
-  const int kstride = first ? a.s.rec_words : NW;
+  const K* kin = first == 1 ? (const K*)a.sp_rec : (const K*)a.sk[cur];
+  const int kstride = first == 1 ? a.s.rec_words : NW;
   k_radix_hist_body<K, NW, FLIP, IT, DB>(g, kin, kstride, &a.cnt->n_spawned, a.scap, word, shift, a.hist);
 }
 
@@ -129,11 +130,22 @@ __global__ void __launch_bounds__(kBlock) km_radix_scatter_words(const EngDev* _
                                                                  int cur, int word, int shift) {
   Grid g;
   const EngDev& a = eng_of(ed, map, kLkChunk, g);
-  const K* kin = first ? (const K*)a.sp_rec : (const K*)a.sk[cur];
-  const int kstride = first ? a.s.rec_words : NW;
+  const K* kin = first == 1 ? (const K*)a.sp_rec : (const K*)a.sk[cur];
+  const int kstride = first == 1 ? a.s.rec_words : NW;
   const uint32_t* iin = first ? nullptr : a.si[cur];
   k_radix_scatter_body<K, NW, FLIP, IT, DB>(g, kin, kstride, iin, (K*)a.sk[cur ^ 1], a.si[cur ^ 1], &a.cnt->n_spawned,
                                             a.scap, word, shift, a.hist, a.digit_tot);
+}
+
+// the 64-bit colex ranks of the spawned list into sk[1] (the first word-sort
+// pass's input) with the histogram of their low digit
+template <int IT>
+__global__ void __launch_bounds__(kBlock) km_rank64_hist(const EngDev* __restrict__ ed,
+                                                         const uint16_t* __restrict__ map) {
+  Grid g;
+  const EngDev& a = eng_of(ed, map, kLkChunk, g);
+  k_rank64_hist_body<IT, 8>(g, a.sp_rec, a.s.rec_words, &a.cnt->n_spawned, a.scap, a.s.binom64, a.s.nbasis,
+                            a.s.nel1, (uint64_t*)a.sk[1], a.hist);
 }
 
 // the colex-rank sort: top-byte pass over the ranks the spawn kernels wrote

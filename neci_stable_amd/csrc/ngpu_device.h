@@ -291,6 +291,8 @@ struct DevSys {
   // C(p, i) at binom[p*nel1 + i] when spawns are sorted by colex rank, else null
   const uint32_t* binom;
   int nel1;
+  // C(p, i) in 64 bits when spawns are sorted by 64-bit colex rank, else null
+  const uint64_t* binom64;
   // spawned-list records: FP64 amplitudes in 4 words (key, key or 0, value,
   // info), or FP32 amplitudes in W + 1 words (key words, then info << 32 |
   // FP32 bits); sums over spawns stay FP64 either way

@@ -210,6 +210,11 @@ struct ngpu_engine {
   int rank_bits = 0;
   uint32_t* d_binom = nullptr;  // C(p, i) at [p*(nel+1)+i]
   uint32_t* rk = nullptr;       // ranks of the spawned list
+  // one-word determinants with C(nbasis, nel) < 2^64: the word sort runs on
+  // 64-bit colex ranks when they need fewer 8-bit digits than the words
+  bool rank64 = false;
+  int rank64_bits = 0;
+  uint64_t* d_binom64 = nullptr;  // C(p, i) at [p*(nel+1)+i], 64 bits
   int64_t* sq = nullptr;  // queued single-excitation attempts
   // merged spawned list, kept only for the explicit-list entry point
   uint64_t* u_key = nullptr;
@@ -284,6 +289,7 @@ struct BatchGroup {
   int lanes = 4, pl = 4;    // heat-bath singles: lanes per attempt, orbitals per lane
   int items = 4;            // radix tile: elements per thread
   bool rank_sort = false;
+  bool rank64 = false;                      // the word-key sort runs on 64-bit colex ranks
   std::vector<std::pair<int, int>> passes;  // (word, shift) of the word-key sort
   bool core = false, core_fp32 = false;
   // every engine of the group has the same core determinants: one projection
@@ -502,6 +508,26 @@ void hash_upkeep(ngpu_t* h) {
   launch(h, -1, -1, k_hash_done, dim3(1), dim3(1), h->cnt, h->wk.L * (int64_t)kBucketEntries);
 }
 
+// (word, shift) passes of the word-key sort of an engine (ilut_lt order:
+// least significant word first)
+std::vector<std::pair<int, int>> word_passes(const ngpu_t* h) {
+  std::vector<std::pair<int, int>> p;
+  const int hi = h->W == 1 ? h->sys.nbasis : 64, lo = h->W == 2 ? h->sys.nbasis - 64 : 0;
+  if (h->W == 2)
+    for (int sh = 0; sh < lo; sh += 8) p.push_back({1, sh});
+  for (int sh = 0; sh < hi; sh += 8) p.push_back({0, sh});
+  return p;
+}
+
+// (word, shift) passes of the spawn sort: the determinant words, or the
+// 64-bit colex ranks' bytes
+std::vector<std::pair<int, int>> sort_passes(const ngpu_t* h) {
+  if (!h->rank64) return word_passes(h);
+  std::vector<std::pair<int, int>> p;
+  for (int sh = 0; sh < h->rank64_bits; sh += 8) p.push_back({0, sh});
+  return p;
+}
+
 // sort of the spawned list (stage 4) then the merge of equal determinants
 // fused with their annihilation into the walker list (stage 6; stage 5, the
 // merge alone, is empty since the fusion); ranked: the spawn kernels already
@@ -541,26 +567,30 @@ void sort_and_merge_it(ngpu_t* h, bool ranked, bool list_out) {
            h->n_key, h->n_sgn, h->n_flag, (const IterParams*)h->dparams);
     return;
   }
-  // determinant keys in ilut_lt order: least significant word first
-  int nbits_hi = W == 1 ? h->sys.nbasis : 64;      // bits used in word 0
-  int nbits_lo = W == 2 ? h->sys.nbasis - 64 : 0;  // bits used in word 1
-  struct Pass { int word, shift; };
-  std::vector<Pass> passes;
-  if (W == 2)
-    for (int sh = 0; sh < nbits_lo; sh += 8) passes.push_back({1, sh});
-  for (int sh = 0; sh < nbits_hi; sh += 8) passes.push_back({0, sh});
+  // determinant keys in ilut_lt order: least significant word first (or
+  // their 64-bit colex ranks, the same order)
+  const std::vector<std::pair<int, int>> passes = sort_passes(h);
   int cur = 0;
   const uint64_t* kin = h->sp_rec;
   int kstride = h->ds.rec_words;  // the first pass reads the keys inside the records
   const uint32_t* iin = nullptr;
+  if (h->rank64) {  // the first pass reads the ranks, written to sk[1]
+    kin = h->sk[1];
+    kstride = 1;
+  }
   for (size_t p = 0; p < passes.size(); ++p) {
     int out = (int)(p & 1);
     bool first = p == 0, last = p + 1 == passes.size();
-    launch(h, first ? 4 : -1, -1, k_radix_hist<uint64_t, W, W == 2, IT, 8>, dim3(h->nb_chunk), dim3(kBlock), kin, kstride,
-           n_ptr, h->scap, passes[p].word, passes[p].shift, h->hist);
+    if (first && h->rank64)
+      launch(h, 4, -1, k_rank64_hist<IT, 8>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec,
+             h->ds.rec_words, n_ptr, h->scap, (const uint64_t*)h->d_binom64, h->sys.nbasis, h->sys.nel + 1, h->sk[1],
+             h->hist);
+    else
+      launch(h, first ? 4 : -1, -1, k_radix_hist<uint64_t, W, W == 2, IT, 8>, dim3(h->nb_chunk), dim3(kBlock), kin,
+             kstride, n_ptr, h->scap, passes[p].first, passes[p].second, h->hist);
     launch(h, -1, -1, k_radix_scan, dim3(256), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
     launch(h, -1, last ? 4 : -1, k_radix_scatter<uint64_t, W, W == 2, IT, 8>, dim3(h->nb_chunk), dim3(kBlock), kin, kstride, iin,
-           h->sk[out], h->si[out], n_ptr, h->scap, passes[p].word, passes[p].shift, (const uint32_t*)h->hist,
+           h->sk[out], h->si[out], n_ptr, h->scap, passes[p].first, passes[p].second, (const uint32_t*)h->hist,
            (const uint32_t*)h->digit_tot);
     kin = h->sk[out];
     kstride = W;
@@ -622,17 +652,6 @@ void annihilation_chain(ngpu_t* h, bool with_core, bool list_out = false) {
   launch(h, -1, -1, k_end_iter, dim3(1), dim3(kBlock), h->dout, h->dpart, h->dcyc, h->cnt,
          h->scap, h->wk.cap, h->wk.L * (int64_t)kBucketEntries, (const IterParams*)h->dparams);
   if (h->checks) check_chain<W>(h, with_core);
-}
-
-// (word, shift) passes of the word-key sort of an engine (ilut_lt order:
-// least significant word first)
-std::vector<std::pair<int, int>> word_passes(const ngpu_t* h) {
-  std::vector<std::pair<int, int>> p;
-  const int hi = h->W == 1 ? h->sys.nbasis : 64, lo = h->W == 2 ? h->sys.nbasis - 64 : 0;
-  if (h->W == 2)
-    for (int sh = 0; sh < lo; sh += 8) p.push_back({1, sh});
-  for (int sh = 0; sh < hi; sh += 8) p.push_back({0, sh});
-  return p;
 }
 
 // the generator instance of an engine's spawn kernel
@@ -1123,6 +1142,24 @@ int ngpu_create(ngpu_t** out, const ngpu_sys_t* sys) {
       h->ds.binom = h->d_binom;
       h->ds.nel1 = sys->nel + 1;
       fail(dalloc(&h->rk, (size_t)scap));
+    } else if (h->W == 1 && total < ((unsigned __int128)1 << 64) && nb * (ne + 1) <= kBinom64Lds) {
+      int bits = 1;
+      while (bits < 64 && ((unsigned __int128)1 << bits) < total) ++bits;
+      if ((bits + 7) / 8 < (nb + 7) / 8) {
+        h->rank64 = true;
+        h->rank64_bits = bits;
+        // entries above the total belong to no determinant's rank
+        std::vector<uint64_t> b((size_t)nb * (ne + 1));
+        for (int p = 0; p < nb; ++p)
+          for (int i = 0; i <= ne; ++i) {
+            unsigned __int128 v = c[(size_t)p * (ne + 1) + i];
+            b[(size_t)p * (ne + 1) + i] = v < total ? (uint64_t)v : 0;
+          }
+        fail(dalloc(&h->d_binom64, b.size()));
+        if (rc == NGPU_OK) fail(hipMemcpy(h->d_binom64, b.data(), b.size() * 8, hipMemcpyHostToDevice));
+        h->ds.binom64 = h->d_binom64;
+        h->ds.nel1 = sys->nel + 1;
+      }
     }
   }
   fail(dalloc(&h->digit_tot, (size_t)kRadixBins));
@@ -1187,7 +1224,7 @@ int ngpu_destroy(ngpu_t* h) {
   void* ptrs[] = {h->wk.keys, h->wk.st, h->wk.diagH, h->wk.offdiagH, h->wk.freeq,
                   h->wk.htab, h->att_cnt, h->att_off, h->pinfo, h->blk_part, h->blk_rm, h->sp_rec,
                   h->sk[0], h->sk[1], h->si[0], h->si[1], h->hist, h->digit_tot, h->sq, h->n_key,
-                  h->n_sgn, h->n_flag, h->u_key, h->u_sgn, h->u_flag, h->u_count, h->cnt, h->dout, h->dpart, h->dcyc, h->own_params, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_fref, h->d_chk, h->d_binom, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
+                  h->n_sgn, h->n_flag, h->u_key, h->u_sgn, h->u_flag, h->u_count, h->cnt, h->dout, h->dpart, h->dcyc, h->own_params, h->d_rh, h->d_nbr, h->d_tmat, h->d_umat, h->d_jk, h->d_jk0, h->d_fref, h->d_chk, h->d_binom, h->d_binom64, h->rk, h->tile_p, h->core_keys, h->core_slot, h->core_rowptr, h->core_cols,
                   h->core_vals, h->core_x, h->core_y, h->d_ab_tab,
                   h->d_ab_prob, h->d_ab_ok, h->d_j_tab, h->d_j_prob, h->d_j_ok, h->d_ab_tgt, h->d_i_tab, h->d_i_prob,
                   h->d_a_tab, h->d_a_prob, h->d_a_ok, h->d_b_tab, h->d_b_prob, h->d_b_ok};
@@ -2587,8 +2624,6 @@ int ngpu_set_graph(ngpu_t* h, int32_t on) {
 extern "C++" {
 namespace {
 
-// the generator instance of an engine's spawn kernel
-
 // orbitals per lane of the heat-bath singles for lanes G and nsp spatial orbitals
 int singles_pl(int G, int nsp) {
   if (G == 16) return nsp <= 16 ? 1 : nsp <= 32 ? 2 : 4;
@@ -2608,7 +2643,8 @@ BatchGroup group_of(const ngpu_t* h) {
   g.pl = singles_pl(h->single_lanes, h->sys.nbasis / 2);
   g.items = h->radix_items;
   g.rank_sort = h->rank_sort;
-  if (!g.rank_sort) g.passes = word_passes(h);
+  g.rank64 = h->rank64;
+  if (!g.rank_sort) g.passes = sort_passes(h);
   g.core = h->ncore > 0;
   g.core_fp32 = h->core_fp32;
   g.rec_words = h->ds.rec_words;
@@ -2617,7 +2653,7 @@ BatchGroup group_of(const ngpu_t* h) {
 
 bool same_group(const BatchGroup& a, const BatchGroup& b) {
   return a.W == b.W && a.S == b.S && a.gen == b.gen && a.lanes == b.lanes && a.pl == b.pl && a.items == b.items &&
-         a.rank_sort == b.rank_sort && a.passes == b.passes && a.core == b.core && a.core_fp32 == b.core_fp32 &&
+         a.rank_sort == b.rank_sort && a.rank64 == b.rank64 && a.passes == b.passes && a.core == b.core && a.core_fp32 == b.core_fp32 &&
          a.rec_words == b.rec_words;
 }
 
@@ -2800,11 +2836,16 @@ void group_sort_merge(BatchGroup& g) {
   const int np = (int)g.passes.size();
   for (int p = 0; p < np; ++p) {
     const int in = (p & 1) ^ 1;  // pass p writes sk[p & 1]
-    glaunch(h0, p == 0 ? 4 : -1, -1, km_radix_hist_words<uint64_t, W, W == 2, IT, 8>, g.tot[kLkChunk], kBlock, 0, ed,
-            (const uint16_t*)g.d_map[kLkChunk], p == 0 ? 1 : 0, in, g.passes[p].first, g.passes[p].second);
+    // the first pass reads the records' words (1) or the ranks in sk[1] (2)
+    const int first = p > 0 ? 0 : g.rank64 ? 2 : 1;
+    if (first == 2)
+      glaunch(h0, 4, -1, km_rank64_hist<IT>, g.tot[kLkChunk], kBlock, 0, ed, (const uint16_t*)g.d_map[kLkChunk]);
+    else
+      glaunch(h0, p == 0 ? 4 : -1, -1, km_radix_hist_words<uint64_t, W, W == 2, IT, 8>, g.tot[kLkChunk], kBlock, 0,
+              ed, (const uint16_t*)g.d_map[kLkChunk], first, in, g.passes[p].first, g.passes[p].second);
     glaunch(h0, -1, -1, km_radix_scan, E * kDigitBlocks, kBlock, 0, ed);
     glaunch(h0, -1, p + 1 == np ? 4 : -1, km_radix_scatter_words<uint64_t, W, W == 2, IT, 8>, g.tot[kLkChunk], kBlock,
-            0, ed, (const uint16_t*)g.d_map[kLkChunk], p == 0 ? 1 : 0, in, g.passes[p].first, g.passes[p].second);
+            0, ed, (const uint16_t*)g.d_map[kLkChunk], first, in, g.passes[p].first, g.passes[p].second);
     cur = p & 1;
   }
   glaunch(h0, 6, 6, km_merge_annihilate<W, R, uint64_t, W>, g.tot[kLkMerge], kBlock, 0, ed,

@@ -1351,6 +1351,41 @@ __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, cons
   k_rank_hist_body<W, IT, DB>(hw_grid(), keys, n_ptr, ncap, rstride, binom, nel1, rank, hist, shift);
 }
 
+// the 64-bit colex-rank sort of one-word determinants: each record's rank
+// (binom[p*nel1 + i] = C(p, i) in 64 bits, staged in LDS) written to rank[i],
+// fused with the histogram of its low digit.  For a fixed electron number the
+// colex order is the order of the determinant words as integers, so the
+// sorted list is the word sort's, in fewer passes (C(nbasis, nel) < 2^64)
+constexpr int kBinom64Lds = 64 * 33;
+template <int IT, int DB>
+__device__ __forceinline__ void k_rank64_hist_body(const Grid& g_, const uint64_t* rec, int rstride, const unsigned long long* n_ptr, int64_t ncap, const uint64_t* binom, int nbasis, int nel1, uint64_t* rank, uint32_t* hist) {
+  constexpr int RB = 1 << DB;
+  __shared__ uint32_t h[RB];
+  __shared__ uint64_t bt[kBinom64Lds];
+  for (int i = threadIdx.x; i < RB; i += blockDim.x) h[i] = 0;
+  for (int i = threadIdx.x; i < nbasis * nel1; i += blockDim.x) bt[i] = binom[i];
+  __syncthreads();
+  int64_t n = (int64_t)*n_ptr;
+  if (n > ncap) n = ncap;
+  int64_t begin, end;
+  radix_chunk<IT>(g_, n, begin, end);
+  for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
+    uint64_t x = rec[i * rstride], r = 0;
+    for (int c = 1; x; ++c) {
+      r += bt[ctz64(x) * nel1 + c];
+      x &= x - 1;
+    }
+    rank[i] = r;
+    atomicAdd(&h[(uint32_t)r & (RB - 1u)], 1u);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < RB; d += blockDim.x) hist[(size_t)d * g_.nb + g_.b] = h[d];
+}
+template <int IT, int DB>
+__global__ void __launch_bounds__(kBlock) k_rank64_hist(const uint64_t* rec, int rstride, const unsigned long long* n_ptr, int64_t ncap, const uint64_t* binom, int nbasis, int nel1, uint64_t* rank, uint32_t* hist) {
+  k_rank64_hist_body<IT, DB>(hw_grid(), rec, rstride, n_ptr, ncap, binom, nbasis, nel1, rank, hist);
+}
+
 // one block per digit: exclusive scan of that digit's per-block counts
 __device__ __forceinline__ void k_radix_scan_body(const Grid& g_, uint32_t* hist, int nb, uint32_t* digit_tot) {
   __shared__ int64_t red[kBlock / kWave];

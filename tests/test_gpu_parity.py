@@ -104,6 +104,8 @@ SPECS = {
     # themselves instead of 32-bit colex ranks
     "hubbard18": lambda **k: hubbard_spec(18, 18, 4.0, **k),
     "mol33o8e_w2": lambda **k: mol_spec(33, 8, rank=4, mseed=3, **k),
+    # C(60, 14) ~ 5e13: the spawn sort runs on 46-bit colex ranks (6 byte passes, not 8)
+    "mol30o14e_rank64": lambda **k: mol_spec(30, 14, rank=8, mseed=5, **k),
     # uniform random excitations (gen_rand_excit) with rejection loops
     "mol8o6e_uniform": lambda **k: mol_spec(8, 6, excitgen="uniform", psingles=0.2, **k),
     "mol33o4e_w2_uniform": lambda **k: mol_spec(33, 4, rank=4, mseed=3, excitgen="uniform", **k),
@@ -217,7 +219,8 @@ def test_attempt_die_and_create():
         assert child[i] == exp
 
 
-@pytest.mark.parametrize("name", ["hubbard10", "mol8o6e_unif_fast", "mol33o4e_w2", "hubbard18", "mol33o8e_w2"])
+@pytest.mark.parametrize("name", ["hubbard10", "mol8o6e_unif_fast", "mol33o4e_w2", "hubbard18", "mol33o8e_w2",
+                                  "mol30o14e_rank64"])
 def test_annihilate_explicit_list(name):
     spec = SPECS[name](init_walkers=3000)
     eng = engine(spec)
@@ -248,7 +251,7 @@ def test_annihilate_explicit_list(name):
     assert_same_walkers(eng, orc)
 
 
-@pytest.mark.parametrize("name", ["hubbard10", "mol33o4e_w2", "mol33o8e_w2"])
+@pytest.mark.parametrize("name", ["hubbard10", "mol33o4e_w2", "mol33o8e_w2", "mol30o14e_rank64"])
 def test_annihilate_explicit_list_fp32(name):
     """An explicit list with real amplitudes under FP32 spawn storage: the
     engine rounds each input amplitude to FP32, so the oracle, given the same
@@ -292,7 +295,8 @@ def test_annihilate_explicit_list_fp32(name):
                                             ("hubbard10_plain", 1, False), ("hubbard10_plain", 2, False),
                                             ("mol8o6e_plain", 1, False), ("mol8o6e_unif_full", 1, False),
                                             ("mol8o6e_full_full", 1, False), ("mol8o6e_full_fast", 2, False),
-                                            ("mol33o4e_w2_unif_full", 1, False), ("mol8o6e_unif_full", 1, True)])
+                                            ("mol33o4e_w2_unif_full", 1, False), ("mol8o6e_unif_full", 1, True),
+                                            ("mol30o14e_rank64", 1, False), ("mol30o14e_rank64", 2, True)])
 def test_iterations_match_oracle(name, nrep, real):
     """Full iterations (spawn, merge, death, annihilation, stats) from the same
     walker set: identical walker sets after every iteration."""
@@ -300,7 +304,8 @@ def test_iterations_match_oracle(name, nrep, real):
 
 
 @pytest.mark.parametrize("name,nrep", [("hubbard10", 1), ("hubbard10", 2), ("mol8o6e_unif_fast", 1),
-                                       ("mol33o4e_w2", 1), ("mol8o6e_uniform", 1), ("hubbard18", 1)])
+                                       ("mol33o4e_w2", 1), ("mol8o6e_uniform", 1), ("hubbard18", 1),
+                                       ("mol30o14e_rank64", 1)])
 def test_iterations_fp32_spawns_match_oracle(name, nrep):
     """Spawned amplitudes stored in FP32 with real coefficients: the engine and
     the oracle round each spawn to FP32 at the same point, so walker signs and
@@ -311,7 +316,8 @@ def test_iterations_fp32_spawns_match_oracle(name, nrep):
 
 def run_iterations(name, nrep, real, spawn_fp32=False):
     # the 8-electron 2-word molecule grows ~1.4x per iteration at tau 0.02
-    tau = 0.01 if name.startswith("hub") else (0.005 if name == "mol33o8e_w2" else 0.02)
+    tau = 0.01 if name.startswith("hub") else (0.005 if name == "mol33o8e_w2" else
+                                               0.002 if name.startswith("mol30o14e") else 0.02)
     spec = SPECS[name](init_walkers=4000, n_replicas=nrep, all_real_coeff=real, seed=77, spawn_fp32=spawn_fp32)
     eng = engine(spec)
     orc = oracle(spec, tau)
