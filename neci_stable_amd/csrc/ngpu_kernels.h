@@ -464,22 +464,33 @@ __device__ __forceinline__ void k_prepass_body(const Grid& g_, const DevSys& s, 
   for (int64_t base = begin; base < end; base += blockDim.x) {
     int64_t slot = base + threadIdx.x;
     bool valid = slot < end;
-    uint32_t f = valid ? wk.flags(slot) : kFlagRemoved;
+    // every field of the slot is loaded at once (lanes past the chunk load
+    // the step's first slot) and selected afterwards, so that the loads are
+    // one round trip to memory rather than three dependent ones
+    const int64_t ls = valid ? slot : base;
+    const uint32_t f_ld = wk.flags(ls);
+    uint64_t k_ld[W];
+    double s_ld[R];
+#pragma unroll
+    for (int w = 0; w < W; ++w) k_ld[w] = wk.keys[ls * W + w];
+    for (int r = 0; r < S; ++r) s_ld[r] = wk.sgn(ls, r);
+    const double hoff_ld = wk.offdiagH[ls], kii_ld = wk.diagH[ls];
+    uint32_t f = valid ? f_ld : kFlagRemoved;
     bool live = !(f & kFlagRemoved);
     uint64_t key[W];
     double sg[R];
     bool occ = false;
 #pragma unroll
-    for (int w = 0; w < W; ++w) key[w] = live ? wk.keys[slot * W + w] : 0ull;
+    for (int w = 0; w < W; ++w) key[w] = live ? k_ld[w] : 0ull;
     for (int r = 0; r < S; ++r) {
-      sg[r] = live ? wk.sgn(slot, r) : 0.0;
+      sg[r] = live ? s_ld[r] : 0.0;
       occ = occ || !(fabs(sg[r]) < 1.e-12);
     }
     bool core = (f & kFlagCore) != 0;
     uint8_t pi0 = 0;
     int exlevel = occ ? excit_level<W>(s.ref, key) : -1;
-    double hoff = occ ? wk.offdiagH[slot] : 0.0;
-    double kii = occ ? wk.diagH[slot] : 0.0;
+    double hoff = occ ? hoff_ld : 0.0;
+    double kii = occ ? kii_ld : 0.0;
     bool isref = occ && det_eq<W>(key, s.ref);
     bool any_left = false;
     double cp[R];
@@ -2218,17 +2229,26 @@ __device__ __forceinline__ void k_stats_body(const Grid& g_, const DevSys& s, co
   for (int64_t t = 0; t < trips; ++t) {
     int64_t slot = grid_elem(g_, t);
     bool rm = false;
+    // the slot's flags, signs and determinant in one round trip to memory
+    // (lanes past the table load slot 0, which exists when any lane runs)
+    const int64_t ls = slot < nslots ? slot : 0;
+    const uint32_t f_ld = wk.flags(ls);
+    double s_ld[R];
+    uint64_t k_ld[W];
+    for (int r = 0; r < S; ++r) s_ld[r] = wk.sgn(ls, r);
+#pragma unroll
+    for (int w = 0; w < W; ++w) k_ld[w] = wk.keys[ls * W + w];
     if (slot < nslots) {
-      uint32_t f = wk.flags(slot);
+      uint32_t f = f_ld;
       if (!(f & kFlagRemoved)) {
         bool determ = (f & kFlagCore) != 0;
         double sg[R];
         bool occ = false;
-        for (int r = 0; r < S; ++r) { sg[r] = wk.sgn(slot, r); occ = occ || !(fabs(sg[r]) < 1.e-12); }
+        for (int r = 0; r < S; ++r) { sg[r] = s_ld[r]; occ = occ || !(fabs(sg[r]) < 1.e-12); }
         if (occ || determ) {
           uint64_t key[W];
 #pragma unroll
-          for (int w = 0; w < W; ++w) key[w] = wk.keys[slot * W + w];
+          for (int w = 0; w < W; ++w) key[w] = k_ld[w];
           if (s.t_real) {
             double thr = s.occ_thresh;
             bool changed = false;
