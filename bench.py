@@ -353,6 +353,17 @@ def run_batched(args):
     drivers = [d for _, d in engines]
     for d in drivers:
         d.run(args.warmup)
+    mol_engines = [e for e, _ in engines[:args.mol_replicas]]
+    tuned = None
+    if not args.no_autotune and mol_engines:
+        # the plain form's launch shape: tuned on one molecular replica and
+        # given to all of them (the same system at the same size)
+        from neci_stable_amd.autotune import autotune
+        tp_ = autotune(drivers[0], steps=10, warm=3, repeats=1)
+        for e in mol_engines[1:]:
+            e.set_launch_shape(**tp_["shape"])
+        tuned = {"plain": tp_["shape"]}
+        log(f"# autotuned launch shape, plain molecular engines: {tp_['shape']}")
     t_setup = time.perf_counter() - t_setup
     log(f"# {len(engines)} engines set up in {t_setup:.1f} s")
 
@@ -383,6 +394,15 @@ def run_batched(args):
     batch = Batch([e for e, _ in engines])
     runner = BatchedFciMCPar(batch, drivers)
     runner.run(max(args.warmup, 10))  # capture and first replays
+    if tuned is not None:
+        # the batched form's launch shape: the molecular group's engines tuned
+        # together on the batch's own iterations (regrouped and recaptured
+        # per candidate)
+        from neci_stable_amd.autotune import tune_engines
+        tb_ = tune_engines(mol_engines, runner.run, steps=10, warm=3, repeats=1)
+        tuned["batched"] = tb_["shape"]
+        log(f"# autotuned launch shape, batched molecular group: {tb_['shape']}")
+        runner.run(max(args.warmup, 10))
     gids, ngroups = batch.groups()
     gm = int(gids[0])                  # the molecular replicas' launch group
     mol = [i for i in range(len(engines)) if gids[i] == gm]
@@ -454,7 +474,8 @@ def run_batched(args):
                                f"{ncore}-determinant core, distinct seeds) + {len(hub_specs)} Hubbard chains of "
                                f"12-18 sites (U/t in [2, 8], targets 1e4-2e6, tau 0.01)",
                    "mol_target_walkers": mol_walkers, "walkers_per_problem": walkers,
-                   "total_walkers": float(sum(walkers))},
+                   "total_walkers": float(sum(walkers)),
+                   "autotuned_launch_shape": tuned},
         "plain": {"value": p_rate, "ms_per_step": tp * 1e3 / K,
                   "form": "each problem its own engine, iterated alone with one launch per kernel, one problem "
                           "after the other, FP64 core Hamiltonian and spawned amplitudes",
