@@ -359,7 +359,7 @@ def run_batched(args):
         # the plain form's launch shape: tuned on one molecular replica and
         # given to all of them (the same system at the same size)
         from neci_stable_amd.autotune import autotune
-        tp_ = autotune(drivers[0], steps=10, warm=3, repeats=1)
+        tp_ = autotune(drivers[0], steps=10, warm=10, repeats=1)  # whole update cycles: the drivers stay aligned
         for e in mol_engines[1:]:
             e.set_launch_shape(**tp_["shape"])
         tuned = {"plain": tp_["shape"]}

@@ -204,7 +204,8 @@ class FciMCPar:
 class BatchedFciMCPar:
     """FciMCPar for several engines at once: one ngpu_batch_iterate per
     iteration runs every engine's iteration; each driver keeps its own shift
-    and update cycle (all drivers share StepsSft)."""
+    and update cycle (all drivers share StepsSft; their cycles need not be
+    aligned)."""
 
     def __init__(self, batch, drivers):
         self.batch = batch
@@ -218,8 +219,9 @@ class BatchedFciMCPar:
         t0 = time.perf_counter()
         done = 0
         while done < n_iter:
-            d0 = self.drivers[0]
-            k = min(self.steps_shift - d0.iter % self.steps_shift, n_iter - done)
+            # no driver passes the end of its update cycle without a read
+            # (drivers may be at different points of their cycles)
+            k = min(min(self.steps_shift - d.iter % self.steps_shift for d in self.drivers), n_iter - done)
             for _ in range(k):
                 for d in self.drivers:
                     d.iter += 1

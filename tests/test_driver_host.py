@@ -8,6 +8,8 @@ the engine's cycle API.
   over a window of iterations (the bench's timed steps after its warm-up) are
   exact whatever the window's alignment to update cycles: with 5 warm-up and
   20 timed steps, attempts per step equal the walkers per step.
+* The batched driver reads every driver at the end of each of its update
+  cycles, also when the drivers' cycles are not aligned.
 """
 import numpy as np
 import pytest
@@ -82,3 +84,32 @@ def test_partial_cycles_and_totals(warmup, steps):
     assert dt["walker_iters"] == float(np.sum(1000 + its))
     assert dt["attempts"] / steps == pytest.approx(dt["walker_iters"] / steps)
     assert d.walker_iters == float(np.sum(1000 + np.arange(1, n + 1)))
+
+
+class ScriptedBatch:
+    """ngpu_batch_iterate over scripted backends."""
+
+    def __init__(self, backends):
+        self.backends = backends
+
+    def iterate(self, iters, taus, shifts):
+        for b, it, t, sh in zip(self.backends, iters, taus, shifts):
+            b.iterate(it, t, sh)
+
+
+def test_batched_driver_with_unaligned_cycles():
+    from neci_stable_amd.fcimc import BatchedFciMCPar
+    backends = [ScriptedBackend() for _ in range(3)]
+    seen = [[] for _ in backends]
+    drivers = [FciMCPar(b, tau=0.01, target_walkers=10.0, steps_shift=10, shift_damp=0.1,
+                        log=lambda row, k=k: seen[k].append(row["iter"])) for k, b in enumerate(backends)]
+    drivers[0].run(13)  # the first driver three iterations into a cycle
+    drivers[1].run(5)   # the second five
+    runner = BatchedFciMCPar(ScriptedBatch(backends), drivers)
+    runner.run(37)
+    assert [d.iter for d in drivers] == [50, 42, 37]
+    assert seen[0] == [10, 20, 30, 40, 50]
+    assert seen[1] == [10, 20, 30, 40]
+    assert seen[2] == [10, 20, 30]
+    for k, d in enumerate(drivers):
+        assert d.totals["iters"] == d.iter

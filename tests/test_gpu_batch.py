@@ -265,3 +265,60 @@ def test_batch_shared_core_space():
         batch.close()
         for b in batched:
             b.close()
+
+
+def test_batch_launch_shape_changes_keep_results():
+    """The launch shape of batched engines changed between iterations (the
+    batch regroups and recaptures; each knob of the autotuner in turn): every
+    batched problem still follows the same problem iterated alone with the
+    default shape.  Two replicas of the 14e/30o molecule (64-bit rank sort,
+    FP32 spawns, a 300-determinant core) and two Hubbard chains."""
+    from neci_stable_amd import HubbardChain, Spec
+    from neci_stable_amd.engine import Batch, Engine
+    from common import molecule
+    mol = molecule(30, 14, 60, 7)
+    specs = [(Spec(mol, init_walkers=2e5, seed=710 + r, particles=0, psingles=0.1, memory_fac_spawn=6.0), 0.002,
+              200.0) for r in range(2)]
+    for L, u in ((12, 3.0), (17, 2.5)):
+        specs.append((Spec(HubbardChain(L, L, u, bhub=-1.0), init_walkers=2e4, seed=820 + L), 0.01, 30.0))
+    pairs = []
+    for spec, tau, n0 in specs:
+        a, b = Engine(spec), Engine(spec)
+        a.start_single_part(n0)
+        b.start_single_part(n0)
+        pairs.append([a, b, tau])
+    for it in range(1, 16):
+        for a, b, tau in pairs:
+            same_stats(a.iterate(it, tau, 0.0), b.iterate(it, tau, 0.0), 1, f"growth {it}")
+    d, sg, _, _, _ = pairs[0][0].download()
+    core = np.ascontiguousarray(d[np.sort(np.argsort(-np.abs(sg[:, 0]), kind="stable")[:300])])
+    for k in range(2):
+        a, b, _ = pairs[k]
+        assert a.set_core(core, fp32=True) == b.set_core(core, fp32=True)
+        a.set_spawn_precision(True)
+        b.set_spawn_precision(True)
+    batch = Batch([b for _, b, _ in pairs])
+    cu = pairs[0][1].launch_shape()["n_cu"]
+    shapes = [dict(radix_items=2), dict(radix_items=8), dict(chunk_blocks=2 * cu), dict(chunk_blocks=8 * cu),
+              dict(grid_blocks=3 * cu), dict(grid_blocks=8 * cu), dict(single_lanes=8), dict(single_lanes=16),
+              dict(merge_blocks=2 * cu), dict(merge_blocks=4 * cu)]
+    it = 15
+    for shape in shapes:
+        print(f"shape {shape}", flush=True)
+        for _, b, _ in pairs:
+            b.set_launch_shape(**shape)
+        for _ in range(3):
+            it += 1
+            batch.iterate([it] * len(pairs), [t for _, _, t in pairs], [0.0] * len(pairs))
+            for i, (a, b, tau) in enumerate(pairs):
+                same_stats(a.iterate(it, tau, 0.0), b.stats(), 1, f"problem {i} iteration {it} {shape}",
+                           exact=i >= 2)
+    for i, (a, b, _) in enumerate(pairs):
+        w1 = sort_walkers(*a.download()[:2])
+        w2 = sort_walkers(*b.download()[:2])
+        np.testing.assert_array_equal(w1[0], w2[0])
+        np.testing.assert_allclose(w1[1], w2[1], rtol=1e-9, atol=1e-9)
+        a.close()
+    batch.close()
+    for _, b, _ in pairs:
+        b.close()
