@@ -252,7 +252,12 @@ struct DevSys {
   double hii, ecore, bhub, uhub;
   double psingles, pdoubles, pparallel;
   uint32_t seed;
+  // spin counts of the reference determinant (every determinant of the run
+  // has them) and pick_biased_elecs's two pair probabilities for those
+  // counts, pparallel / #parallel pairs and (1 - pparallel) / #opposite
+  // pairs, divided once on the host (the same IEEE quotients)
   int nalpha_ref, nbeta_ref;
+  double pg_par_ref, pg_anti_ref;
   uint64_t ref[2];
   int64_t L;                   // walker hash buckets
   const int32_t* rh;           // RandomHash2
@@ -643,19 +648,24 @@ __device__ __host__ inline void pick_biased_elecs(const DevSys& s, const uint64_
   o1 = o2 = 0;
   pgen = 0.0;
   if (r < pb ? par == 0 : ab == 0) return;  // no pair of the drawn spin type
+  const bool ref_counts = nalpha == s.nalpha_ref && nbeta == s.nbeta_ref;
   if (r < pb) {
     r = (r / pb) * (double)par;
     int idx = (int)floor(r);
     uint64_t mask = kA;
     if (idx >= aa) { idx -= aa; mask = kB; }
-    int n1 = 2;
-    while (n1 * (n1 - 1) / 2 <= idx) ++n1;
+    // the smallest n1 >= 2 with n1 (n1 - 1) / 2 > idx: the root of the
+    // quadratic, then one integer correction each way
+    int n1 = (int)floorf(0.5f * (1.0f + sqrtf(8.0f * (float)idx + 1.0f))) + 1;
+    if (n1 < 2) n1 = 2;
+    if (n1 * (n1 - 1) / 2 <= idx) ++n1;
+    if (n1 > 2 && (n1 - 1) * (n1 - 2) / 2 > idx) --n1;
     int n2 = idx + 1 - ((n1 - 1) * (n1 - 2)) / 2;
     o1 = select_orb<W>(d, mask, n2);
     o2 = select_orb<W>(d, mask, n1);
-    pgen = pb / (double)par;
+    pgen = ref_counts ? s.pg_par_ref : pb / (double)par;
   } else {
-    pgen = (1.0 - pb) / (double)ab;
+    pgen = ref_counts ? s.pg_anti_ref : (1.0 - pb) / (double)ab;
     r = ((r - pb) / (1.0 - pb)) * (double)ab;
     int idx = (int)floor(r);
     int na = 1 + idx % nalpha;

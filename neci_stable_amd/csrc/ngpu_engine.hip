@@ -1358,6 +1358,23 @@ int upload_table(T** dst, const std::vector<T>& src) {
 }  // namespace
 }  // extern "C++"
 
+// the reference's spin counts and pick_biased_elecs's pair probabilities for
+// them (every determinant of the run has the reference's spin counts)
+static void update_pair_probs(ngpu_t* h) {
+  const uint64_t kA = 0xAAAAAAAAAAAAAAAAull, kB = 0x5555555555555555ull;
+  int na = 0, nb = 0;
+  for (int w = 0; w < h->W; ++w) {
+    na += __builtin_popcountll(h->ds.ref[w] & kA);
+    nb += __builtin_popcountll(h->ds.ref[w] & kB);
+  }
+  h->ds.nalpha_ref = na;
+  h->ds.nbeta_ref = nb;
+  const int par = na * (na - 1) / 2 + nb * (nb - 1) / 2, ab = na * nb;
+  const double pb = h->ds.pparallel;
+  h->ds.pg_par_ref = par > 0 ? pb / (double)par : 0.0;
+  h->ds.pg_anti_ref = ab > 0 ? (1.0 - pb) / (double)ab : 0.0;
+}
+
 int ngpu_set_excitgen_pchb(ngpu_t* h, int32_t singles, int32_t particles, int32_t holes, double psingles,
                            double pparallel) {
   if (h) h->graph_dirty = true;  // kernel arguments of the captured chain change
@@ -1499,6 +1516,7 @@ int ngpu_set_excitgen_pchb(ngpu_t* h, int32_t singles, int32_t particles, int32_
   h->ds.psingles = psingles;
   h->ds.pdoubles = 1.0 - psingles;
   h->ds.pparallel = pparallel;
+  update_pair_probs(h);
   h->have_excitgen = true;
   return NGPU_OK;
 }
@@ -1550,6 +1568,7 @@ int ngpu_set_reference(ngpu_t* h, const int64_t* ilut_ref, double hii) {
   h->ds.ref[1] = h->W > 1 ? (uint64_t)ilut_ref[1] : 0;
   h->ds.hii = hii;
   h->have_ref = true;
+  update_pair_probs(h);
   return refresh_fref(h);
 }
 
