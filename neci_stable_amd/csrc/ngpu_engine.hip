@@ -684,9 +684,11 @@ void iterate_chain(ngpu_t* h) {
     auto kern = G == 16 ? (nsp <= 16 ? k_spawn_singles<W, R, 1, 16> : nsp <= 32 ? k_spawn_singles<W, R, 2, 16>
                            : k_spawn_singles<W, R, 4, 16>)
               : G == 8 ? (nsp <= 16 ? k_spawn_singles<W, R, 2, 8> : nsp <= 24 ? k_spawn_singles<W, R, 3, 8>
-                          : nsp <= 32 ? k_spawn_singles<W, R, 4, 8> : k_spawn_singles<W, R, 8, 8>)
+                          : nsp <= 32 ? k_spawn_singles<W, R, 4, 8> : nsp <= 40 ? k_spawn_singles<W, R, 5, 8>
+                          : k_spawn_singles<W, R, 8, 8>)
               : nsp <= 16 ? k_spawn_singles<W, R, 4, 4> : nsp <= 20 ? k_spawn_singles<W, R, 5, 4>
-              : nsp <= 32 ? k_spawn_singles<W, R, 8, 4> : k_spawn_singles<W, R, 16, 4>;
+              : nsp <= 32 ? k_spawn_singles<W, R, 8, 4> : nsp <= 40 ? k_spawn_singles<W, R, 10, 4>
+              : k_spawn_singles<W, R, 16, 4>;
     launch_lds(h, 3, 3, kern, dim3(h->nb_grid), dim3(kBlock), lds, h->ds, h->wk, h->cnt, h->dpart,
                (const uint8_t*)h->pinfo, h->sp_rec, h->rank_sort ? h->rk : nullptr, h->scap, (const int64_t*)h->sq,
                (const IterParams*)h->dparams);
@@ -2646,8 +2648,8 @@ namespace {
 // orbitals per lane of the heat-bath singles for lanes G and nsp spatial orbitals
 int singles_pl(int G, int nsp) {
   if (G == 16) return nsp <= 16 ? 1 : nsp <= 32 ? 2 : 4;
-  if (G == 8) return nsp <= 16 ? 2 : nsp <= 24 ? 3 : nsp <= 32 ? 4 : 8;
-  return nsp <= 16 ? 4 : nsp <= 20 ? 5 : nsp <= 32 ? 8 : 16;
+  if (G == 8) return nsp <= 16 ? 2 : nsp <= 24 ? 3 : nsp <= 32 ? 4 : nsp <= 40 ? 5 : 8;
+  return nsp <= 16 ? 4 : nsp <= 20 ? 5 : nsp <= 32 ? 8 : nsp <= 40 ? 10 : 16;
 }
 
 
@@ -2922,10 +2924,12 @@ void group_chain_wr(BatchGroup& g) {
       case 802: group_singles<W, R, 2, 8>(g, nsp); break;
       case 803: group_singles<W, R, 3, 8>(g, nsp); break;
       case 804: group_singles<W, R, 4, 8>(g, nsp); break;
+      case 805: group_singles<W, R, 5, 8>(g, nsp); break;
       case 808: group_singles<W, R, 8, 8>(g, nsp); break;
       case 404: group_singles<W, R, 4, 4>(g, nsp); break;
       case 405: group_singles<W, R, 5, 4>(g, nsp); break;
       case 408: group_singles<W, R, 8, 4>(g, nsp); break;
+      case 410: group_singles<W, R, 10, 4>(g, nsp); break;
       default: group_singles<W, R, 16, 4>(g, nsp); break;
     }
   }
