@@ -111,8 +111,9 @@ __global__ void __launch_bounds__(kBlock) km_spawn_singles(const EngDev* __restr
 }
 
 // radix pass p of the word sort: the first pass reads the keys inside the
-// spawn records (first = 1) or the 64-bit colex ranks in sk[cur] (first = 2),
-// later passes the previous pass's output (ping-pong cur)
+// spawn records (first = 1) or the 64-bit rank keys in sk[cur] (first = 2,
+// the rank path's partition pass), later passes the previous pass's output
+// (ping-pong cur)
 template <typename K, int NW, bool FLIP, int IT, int DB>
 __global__ void __launch_bounds__(kBlock) km_radix_hist_words(const EngDev* __restrict__ ed,
                                                               const uint16_t* __restrict__ map, int first, int cur,
@@ -137,15 +138,15 @@ __global__ void __launch_bounds__(kBlock) km_radix_scatter_words(const EngDev* _
                                             a.scap, word, shift, a.hist, a.digit_tot);
 }
 
-// the 64-bit colex ranks of the spawned list into sk[1] (the first word-sort
-// pass's input) with the histogram of their low digit
+// the 64-bit rank path's keys of the spawned list into sk[0] (the partition
+// pass's input) with the histogram of their top byte
 template <int IT>
 __global__ void __launch_bounds__(kBlock) km_rank64_hist(const EngDev* __restrict__ ed,
-                                                         const uint16_t* __restrict__ map) {
+                                                         const uint16_t* __restrict__ map, int bits) {
   Grid g;
   const EngDev& a = eng_of(ed, map, kLkChunk, g);
   k_rank64_hist_body<IT, 8>(g, a.sp_rec, a.s.rec_words, &a.cnt->n_spawned, a.scap, a.s.binom64, a.s.nbasis,
-                            a.s.nel1, (uint64_t*)a.sk[1], a.hist);
+                            a.s.nel1, bits, a.sk[0], a.hist);
 }
 
 // the colex-rank sort: top-byte pass over the ranks the spawn kernels wrote
@@ -174,7 +175,13 @@ __global__ void __launch_bounds__(kBlock) km_radix_scan(const EngDev* __restrict
 __global__ void __launch_bounds__(kSegThreads) km_seg_sort(const EngDev* __restrict__ ed) {
   const EngDev& a = ed[blockIdx.x / kDigitBlocks];
   const Grid g{(int64_t)(blockIdx.x % kDigitBlocks), kDigitBlocks};
-  k_seg_sort_body(g, (uint32_t*)a.sk[1], a.si[1], (uint32_t*)a.sk[0], a.si[0], a.digit_tot);
+  k_seg_sort_body<uint32_t>(g, (uint32_t*)a.sk[1], a.si[1], (uint32_t*)a.sk[0], a.si[0], a.digit_tot, 3);
+}
+// the 64-bit rank path's segments: sk[1] -> sk[0], npass bytes below the top one
+__global__ void __launch_bounds__(kSegThreads) km_seg_sort64(const EngDev* __restrict__ ed, int npass) {
+  const EngDev& a = ed[blockIdx.x / kDigitBlocks];
+  const Grid g{(int64_t)(blockIdx.x % kDigitBlocks), kDigitBlocks};
+  k_seg_sort_body<uint64_t>(g, a.sk[1], a.si[1], a.sk[0], a.si[0], a.digit_tot, npass);
 }
 
 template <int W, int R, typename K, int NW>

@@ -210,8 +210,9 @@ struct ngpu_engine {
   int rank_bits = 0;
   uint32_t* d_binom = nullptr;  // C(p, i) at [p*(nel+1)+i]
   uint32_t* rk = nullptr;       // ranks of the spawned list
-  // one-word determinants with C(nbasis, nel) < 2^64: the word sort runs on
-  // 64-bit colex ranks when they need fewer 8-bit digits than the words
+  // one-word determinants with C(nbasis, nel) < 2^64 whose colex rank needs
+  // fewer bytes than the word: the spawn sort partitions on the top byte of
+  // the (spread) 64-bit rank, then sorts each segment (k_seg_sort64)
   bool rank64 = false;
   int rank64_bits = 0;
   uint64_t* d_binom64 = nullptr;  // C(p, i) at [p*(nel+1)+i], 64 bits
@@ -289,7 +290,8 @@ struct BatchGroup {
   int lanes = 4, pl = 4;    // heat-bath singles: lanes per attempt, orbitals per lane
   int items = 4;            // radix tile: elements per thread
   bool rank_sort = false;
-  bool rank64 = false;                      // the word-key sort runs on 64-bit colex ranks
+  bool rank64 = false;                      // the 64-bit rank partition sort
+  int rank64_bits = 0;
   std::vector<std::pair<int, int>> passes;  // (word, shift) of the word-key sort
   bool core = false, core_fp32 = false;
   // every engine of the group has the same core determinants: one projection
@@ -519,15 +521,6 @@ std::vector<std::pair<int, int>> word_passes(const ngpu_t* h) {
   return p;
 }
 
-// (word, shift) passes of the spawn sort: the determinant words, or the
-// 64-bit colex ranks' bytes
-std::vector<std::pair<int, int>> sort_passes(const ngpu_t* h) {
-  if (!h->rank64) return word_passes(h);
-  std::vector<std::pair<int, int>> p;
-  for (int sh = 0; sh < h->rank64_bits; sh += 8) p.push_back({0, sh});
-  return p;
-}
-
 // sort of the spawned list (stage 4) then the merge of equal determinants
 // fused with their annihilation into the walker list (stage 6; stage 5, the
 // merge alone, is empty since the fusion); ranked: the spawn kernels already
@@ -567,27 +560,40 @@ void sort_and_merge_it(ngpu_t* h, bool ranked, bool list_out) {
            h->n_key, h->n_sgn, h->n_flag, (const IterParams*)h->dparams);
     return;
   }
-  // determinant keys in ilut_lt order: least significant word first (or
-  // their 64-bit colex ranks, the same order)
-  const std::vector<std::pair<int, int>> passes = sort_passes(h);
+  if (h->rank64) {
+    // the 64-bit rank keys (spread over the top byte) into sk[0], one stable
+    // pass on the top byte into sk[1], then each top-byte segment sorted by
+    // the bytes below it (k_seg_sort64) back into sk[0]
+    const int bits = h->rank64_bits;
+    launch(h, 4, -1, k_rank64_hist<IT, 8>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec,
+           h->ds.rec_words, n_ptr, h->scap, (const uint64_t*)h->d_binom64, h->sys.nbasis, h->sys.nel + 1, bits,
+           h->sk[0], h->hist);
+    launch(h, -1, -1, k_radix_scan, dim3(256), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
+    launch(h, -1, -1, k_radix_scatter<uint64_t, 1, false, IT, 8>, dim3(h->nb_chunk), dim3(kBlock),
+           (const uint64_t*)h->sk[0], 1, (const uint32_t*)nullptr, h->sk[1], h->si[1], n_ptr, h->scap, 0, bits - 8,
+           (const uint32_t*)h->hist, (const uint32_t*)h->digit_tot);
+    launch(h, -1, 4, k_seg_sort64, dim3(256), dim3(kSegThreads), h->sk[1], h->si[1], h->sk[0], h->si[0],
+           (const uint32_t*)h->digit_tot, (bits - 8 + 7) / 8);
+    if (list_out)
+      launch(h, -1, -1, k_merge_list<W, R, uint64_t, W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->cnt,
+             (const uint64_t*)h->sk[0], (const uint32_t*)h->si[0], (const uint64_t*)h->sp_rec, h->scap, h->u_key,
+             h->u_sgn, h->u_flag, h->u_count);
+    launch(h, 6, 6, k_merge_annihilate<W, R, uint64_t, W>, dim3(h->nb_merge ? h->nb_merge : h->nb_grid),
+           dim3(kBlock), h->ds, h->wk, h->cnt, h->dpart, (const uint64_t*)h->sk[0], (const uint32_t*)h->si[0],
+           (const uint64_t*)h->sp_rec, h->scap, h->n_key, h->n_sgn, h->n_flag, (const IterParams*)h->dparams);
+    return;
+  }
+  // determinant keys in ilut_lt order: least significant word first
+  const std::vector<std::pair<int, int>> passes = word_passes(h);
   int cur = 0;
   const uint64_t* kin = h->sp_rec;
   int kstride = h->ds.rec_words;  // the first pass reads the keys inside the records
   const uint32_t* iin = nullptr;
-  if (h->rank64) {  // the first pass reads the ranks, written to sk[1]
-    kin = h->sk[1];
-    kstride = 1;
-  }
   for (size_t p = 0; p < passes.size(); ++p) {
     int out = (int)(p & 1);
     bool first = p == 0, last = p + 1 == passes.size();
-    if (first && h->rank64)
-      launch(h, 4, -1, k_rank64_hist<IT, 8>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec,
-             h->ds.rec_words, n_ptr, h->scap, (const uint64_t*)h->d_binom64, h->sys.nbasis, h->sys.nel + 1, h->sk[1],
-             h->hist);
-    else
-      launch(h, first ? 4 : -1, -1, k_radix_hist<uint64_t, W, W == 2, IT, 8>, dim3(h->nb_chunk), dim3(kBlock), kin,
-             kstride, n_ptr, h->scap, passes[p].first, passes[p].second, h->hist);
+    launch(h, first ? 4 : -1, -1, k_radix_hist<uint64_t, W, W == 2, IT, 8>, dim3(h->nb_chunk), dim3(kBlock), kin,
+           kstride, n_ptr, h->scap, passes[p].first, passes[p].second, h->hist);
     launch(h, -1, -1, k_radix_scan, dim3(256), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
     launch(h, -1, last ? 4 : -1, k_radix_scatter<uint64_t, W, W == 2, IT, 8>, dim3(h->nb_chunk), dim3(kBlock), kin, kstride, iin,
            h->sk[out], h->si[out], n_ptr, h->scap, passes[p].first, passes[p].second, (const uint32_t*)h->hist,
@@ -2665,7 +2671,8 @@ BatchGroup group_of(const ngpu_t* h) {
   g.items = h->radix_items;
   g.rank_sort = h->rank_sort;
   g.rank64 = h->rank64;
-  if (!g.rank_sort) g.passes = sort_passes(h);
+  g.rank64_bits = h->rank64_bits;
+  if (!g.rank_sort) g.passes = word_passes(h);
   g.core = h->ncore > 0;
   g.core_fp32 = h->core_fp32;
   g.rec_words = h->ds.rec_words;
@@ -2674,7 +2681,8 @@ BatchGroup group_of(const ngpu_t* h) {
 
 bool same_group(const BatchGroup& a, const BatchGroup& b) {
   return a.W == b.W && a.S == b.S && a.gen == b.gen && a.lanes == b.lanes && a.pl == b.pl && a.items == b.items &&
-         a.rank_sort == b.rank_sort && a.rank64 == b.rank64 && a.passes == b.passes && a.core == b.core && a.core_fp32 == b.core_fp32 &&
+         a.rank_sort == b.rank_sort && a.rank64 == b.rank64 && a.rank64_bits == b.rank64_bits &&
+         a.passes == b.passes && a.core == b.core && a.core_fp32 == b.core_fp32 &&
          a.rec_words == b.rec_words;
 }
 
@@ -2854,16 +2862,25 @@ void group_sort_merge(BatchGroup& g) {
             (const uint16_t*)g.d_map[kLkMerge], 0);
     return;
   }
+  if (g.rank64) {
+    // 64-bit rank keys into sk[0], the top-byte partition into sk[1], the
+    // segments sorted back into sk[0]
+    const int bits = g.rank64_bits;
+    glaunch(h0, 4, -1, km_rank64_hist<IT>, g.tot[kLkChunk], kBlock, 0, ed, (const uint16_t*)g.d_map[kLkChunk], bits);
+    glaunch(h0, -1, -1, km_radix_scan, E * kDigitBlocks, kBlock, 0, ed);
+    glaunch(h0, -1, -1, km_radix_scatter_words<uint64_t, 1, false, IT, 8>, g.tot[kLkChunk], kBlock, 0, ed,
+            (const uint16_t*)g.d_map[kLkChunk], 2, 0, 0, bits - 8);
+    glaunch(h0, -1, 4, km_seg_sort64, E * kDigitBlocks, kSegThreads, 0, ed, (bits - 8 + 7) / 8);
+    glaunch(h0, 6, 6, km_merge_annihilate<W, R, uint64_t, W>, g.tot[kLkMerge], kBlock, 0, ed,
+            (const uint16_t*)g.d_map[kLkMerge], 0);
+    return;
+  }
   const int np = (int)g.passes.size();
   for (int p = 0; p < np; ++p) {
     const int in = (p & 1) ^ 1;  // pass p writes sk[p & 1]
-    // the first pass reads the records' words (1) or the ranks in sk[1] (2)
-    const int first = p > 0 ? 0 : g.rank64 ? 2 : 1;
-    if (first == 2)
-      glaunch(h0, 4, -1, km_rank64_hist<IT>, g.tot[kLkChunk], kBlock, 0, ed, (const uint16_t*)g.d_map[kLkChunk]);
-    else
-      glaunch(h0, p == 0 ? 4 : -1, -1, km_radix_hist_words<uint64_t, W, W == 2, IT, 8>, g.tot[kLkChunk], kBlock, 0,
-              ed, (const uint16_t*)g.d_map[kLkChunk], first, in, g.passes[p].first, g.passes[p].second);
+    const int first = p == 0 ? 1 : 0;  // the first pass reads the records' words
+    glaunch(h0, p == 0 ? 4 : -1, -1, km_radix_hist_words<uint64_t, W, W == 2, IT, 8>, g.tot[kLkChunk], kBlock, 0,
+            ed, (const uint16_t*)g.d_map[kLkChunk], first, in, g.passes[p].first, g.passes[p].second);
     glaunch(h0, -1, -1, km_radix_scan, E * kDigitBlocks, kBlock, 0, ed);
     glaunch(h0, -1, p + 1 == np ? 4 : -1, km_radix_scatter_words<uint64_t, W, W == 2, IT, 8>, g.tot[kLkChunk], kBlock,
             0, ed, (const uint16_t*)g.d_map[kLkChunk], first, in, g.passes[p].first, g.passes[p].second);

@@ -1362,20 +1362,25 @@ __global__ void __launch_bounds__(kBlock) k_rank_hist(const uint64_t* keys, cons
   k_rank_hist_body<W, IT, DB>(hw_grid(), keys, n_ptr, ncap, rstride, binom, nel1, rank, hist, shift);
 }
 
-// the 64-bit colex-rank sort of one-word determinants: each record's rank
-// (binom[p*nel1 + i] = C(p, i) in 64 bits, staged in LDS) written to rank[i],
-// fused with the histogram of its low digit.  For a fixed electron number the
-// colex order is the order of the determinant words as integers, so the
-// sorted list is the word sort's, in fewer passes (C(nbasis, nel) < 2^64)
+// the 64-bit colex-rank sort of one-word determinants (C(nbasis, nel) <
+// 2^64, the rank shorter than the word): each record's key, its colex rank
+// (binom[p*nel1 + i] = C(p, i) in 64 bits, staged in LDS) times an odd
+// constant modulo 2^bits (a bijection of bits-bit values that spreads the
+// ranks, which cluster, evenly over the top byte), written to rank[i] and
+// fused with the histogram of the top byte (the digit at shift = bits - 8).
+// The partition pass on that byte then leaves 256 segments for k_seg_sort64.
 constexpr int kBinom64Lds = 64 * 33;
+constexpr uint64_t kRank64Mul = 0x9E3779B97F4A7C15ull;
 template <int IT, int DB>
-__device__ __forceinline__ void k_rank64_hist_body(const Grid& g_, const uint64_t* rec, int rstride, const unsigned long long* n_ptr, int64_t ncap, const uint64_t* binom, int nbasis, int nel1, uint64_t* rank, uint32_t* hist) {
+__device__ __forceinline__ void k_rank64_hist_body(const Grid& g_, const uint64_t* rec, int rstride, const unsigned long long* n_ptr, int64_t ncap, const uint64_t* binom, int nbasis, int nel1, int bits, uint64_t* rank, uint32_t* hist) {
   constexpr int RB = 1 << DB;
   __shared__ uint32_t h[RB];
   __shared__ uint64_t bt[kBinom64Lds];
   for (int i = threadIdx.x; i < RB; i += blockDim.x) h[i] = 0;
   for (int i = threadIdx.x; i < nbasis * nel1; i += blockDim.x) bt[i] = binom[i];
   __syncthreads();
+  const uint64_t mask = bits >= 64 ? ~0ull : ((1ull << bits) - 1ull);
+  const int shift = bits - DB;
   int64_t n = (int64_t)*n_ptr;
   if (n > ncap) n = ncap;
   int64_t begin, end;
@@ -1386,15 +1391,16 @@ __device__ __forceinline__ void k_rank64_hist_body(const Grid& g_, const uint64_
       r += bt[ctz64(x) * nel1 + c];
       x &= x - 1;
     }
+    r = (r * kRank64Mul) & mask;
     rank[i] = r;
-    atomicAdd(&h[(uint32_t)r & (RB - 1u)], 1u);
+    atomicAdd(&h[(uint32_t)(r >> shift) & (RB - 1u)], 1u);
   }
   __syncthreads();
   for (int d = threadIdx.x; d < RB; d += blockDim.x) hist[(size_t)d * g_.nb + g_.b] = h[d];
 }
 template <int IT, int DB>
-__global__ void __launch_bounds__(kBlock) k_rank64_hist(const uint64_t* rec, int rstride, const unsigned long long* n_ptr, int64_t ncap, const uint64_t* binom, int nbasis, int nel1, uint64_t* rank, uint32_t* hist) {
-  k_rank64_hist_body<IT, DB>(hw_grid(), rec, rstride, n_ptr, ncap, binom, nbasis, nel1, rank, hist);
+__global__ void __launch_bounds__(kBlock) k_rank64_hist(const uint64_t* rec, int rstride, const unsigned long long* n_ptr, int64_t ncap, const uint64_t* binom, int nbasis, int nel1, int bits, uint64_t* rank, uint32_t* hist) {
+  k_rank64_hist_body<IT, DB>(hw_grid(), rec, rstride, n_ptr, ncap, binom, nbasis, nel1, bits, rank, hist);
 }
 
 // one block per digit: exclusive scan of that digit's per-block counts
@@ -1592,22 +1598,24 @@ __device__ inline void seg_scan_counts(SegScratch* sc) {
 // one stable 8-bit pass, src -> dst (LDS or global), in tiles of one key per
 // thread: ballots rank the keys of a digit within each wave, the waves'
 // counts in LDS order the waves.  Every thread of the block calls it.
-__device__ inline void seg_pass(const uint32_t* sk, const uint32_t* si, uint32_t* dk, uint32_t* di, int64_t n,
-                                int shift, SegScratch* sc) {
+template <typename K>
+__device__ inline void seg_pass(const K* sk, const uint32_t* si, K* dk, uint32_t* di, int64_t n, int shift,
+                                SegScratch* sc) {
   const int wv = threadIdx.x / kWave;
   const uint64_t ltmask = lanemask_lt();
   for (int t = threadIdx.x; t < 256; t += blockDim.x) sc->cnt[t] = 0;
   for (int t = threadIdx.x; t < kSegWaves * 256; t += blockDim.x) (&sc->wcnt[0][0])[t] = 0;
   __syncthreads();
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&sc->cnt[(sk[i] >> shift) & 255u], 1u);
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&sc->cnt[(uint32_t)(sk[i] >> shift) & 255u], 1u);
   __syncthreads();
   seg_scan_counts(sc);
   __syncthreads();
   for (int64_t t0 = 0; t0 < n; t0 += blockDim.x) {
     const int64_t i = t0 + threadIdx.x;
     const bool valid = i < n;
-    const uint32_t k = valid ? sk[i] : 0u, v = valid ? si[i] : 0u;
-    const uint32_t d = (k >> shift) & 255u;
+    const K k = valid ? sk[i] : (K)0;
+    const uint32_t v = valid ? si[i] : 0u;
+    const uint32_t d = (uint32_t)(k >> shift) & 255u;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int bit = 0; bit < 8; ++bit) {
@@ -1636,10 +1644,14 @@ __device__ inline void seg_pass(const uint32_t* sk, const uint32_t* si, uint32_t
   }
 }
 
-// block b sorts segment b (256 blocks of kSegThreads); digit_tot holds the
-// segment sizes, the counts of the top-byte pass
-__device__ __forceinline__ void k_seg_sort_body(const Grid& g_, uint32_t* kin, uint32_t* iin, uint32_t* kout, uint32_t* iout, const uint32_t* digit_tot) {
-  __shared__ uint32_t ka[kSegCap], ia[kSegCap], kb[kSegCap], ib[kSegCap];
+// block b sorts segment b (256 blocks of kSegThreads) by the npass bytes
+// below the top byte; digit_tot holds the segment sizes, the counts of the
+// top-byte pass.  Keys of 32 bits (colex rank times an odd constant, three
+// passes) or 64 bits (the 64-bit rank path).
+template <typename K>
+__device__ __forceinline__ void k_seg_sort_body(const Grid& g_, K* kin, uint32_t* iin, K* kout, uint32_t* iout, const uint32_t* digit_tot, int npass) {
+  __shared__ K ka[kSegCap], kb[kSegCap];
+  __shared__ uint32_t ia[kSegCap], ib[kSegCap];
   __shared__ SegScratch sc;
   __shared__ int64_t seg[2];
   if (threadIdx.x < 256) sc.cnt[threadIdx.x] = digit_tot[threadIdx.x];
@@ -1653,28 +1665,46 @@ __device__ __forceinline__ void k_seg_sort_body(const Grid& g_, uint32_t* kin, u
   __syncthreads();
   const int64_t start = seg[0], n = seg[1];
   if (n == 0) return;  // block-uniform
-  uint32_t *gk = kin + start, *gi = iin + start, *ok = kout + start, *oi = iout + start;
+  K *gk = kin + start, *ok = kout + start;
+  uint32_t *gi = iin + start, *oi = iout + start;
   if (n <= kSegCap) {
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
       ka[i] = gk[i];
       ia[i] = gi[i];
     }
     __syncthreads();
-    seg_pass(ka, ia, kb, ib, n, 0, &sc);
-    seg_pass(kb, ib, ka, ia, n, 8, &sc);
-    seg_pass(ka, ia, kb, ib, n, 16, &sc);
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-      ok[i] = kb[i];
-      oi[i] = ib[i];
+    K *sk = ka, *dk = kb;
+    uint32_t *si = ia, *di = ib;
+    for (int p = 0; p < npass; ++p) {
+      seg_pass<K>(sk, si, dk, di, n, 8 * p, &sc);
+      K* tk = sk; sk = dk; dk = tk;
+      uint32_t* ti = si; si = di; di = ti;
     }
-  } else {
-    seg_pass(gk, gi, ok, oi, n, 0, &sc);
-    seg_pass(ok, oi, gk, gi, n, 8, &sc);
-    seg_pass(gk, gi, ok, oi, n, 16, &sc);
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      ok[i] = sk[i];
+      oi[i] = si[i];
+    }
+  } else {  // through global memory, the input serving as scratch
+    K *sk = gk, *dk = ok;
+    uint32_t *si = gi, *di = oi;
+    for (int p = 0; p < npass; ++p) {
+      seg_pass<K>(sk, si, dk, di, n, 8 * p, &sc);
+      K* tk = sk; sk = dk; dk = tk;
+      uint32_t* ti = si; si = di; di = ti;
+    }
+    if (sk != ok) {  // an even number of passes ended in the input
+      for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        ok[i] = sk[i];
+        oi[i] = si[i];
+      }
+    }
   }
 }
 __global__ void __launch_bounds__(kSegThreads) k_seg_sort(uint32_t* kin, uint32_t* iin, uint32_t* kout, uint32_t* iout, const uint32_t* digit_tot) {
-  k_seg_sort_body(hw_grid(), kin, iin, kout, iout, digit_tot);
+  k_seg_sort_body<uint32_t>(hw_grid(), kin, iin, kout, iout, digit_tot, 3);
+}
+__global__ void __launch_bounds__(kSegThreads) k_seg_sort64(uint64_t* kin, uint32_t* iin, uint64_t* kout, uint32_t* iout, const uint32_t* digit_tot, int npass) {
+  k_seg_sort_body<uint64_t>(hw_grid(), kin, iin, kout, iout, digit_tot, npass);
 }
 
 // ------------------------------------------------------------- merging

@@ -1,14 +1,17 @@
 """The spawn sort at sizes where its per-segment sort leaves LDS.
 
-The colex-rank sort makes one stable pass on the top byte of rank*odd
-constant, then sorts each of the 256 top-byte segments by the low 24 bits:
-in LDS up to 4096 keys, through global memory beyond.  Through the
+The colex-rank sorts make one stable pass on the top byte of rank*odd
+constant (32-bit keys, or 64-bit keys modulo 2^bits), then sort each of the
+256 top-byte segments by the bytes below it: in LDS up to 4096 keys, through
+global memory beyond.  Through the
 explicit-list entry point (ngpu_annihilate, the same sort and merge as an
 iteration) on an empty walker list:
 * 1.5e6 spawns over 3e5 determinants of 10 electrons in 40 spin orbitals
   (segments of ~5900 keys, all past the LDS size);
 * a skewed list in which one determinant takes 1e5 of 3e5 spawns (its
-  segment alone passes the LDS size, the others stay inside it).
+  segment alone passes the LDS size, the others stay inside it);
+* the same two lists, and one that stays inside LDS, over determinants of
+  14 electrons in 60 spin orbitals (the 64-bit rank sort).
 The merged list must equal a numpy group-by of the spawns (integer values,
 so every sum is exact) and the oracle's merge of the same list; initiator
 flags and aborts follow from the group-by as in the oracle pinning test.
@@ -53,23 +56,30 @@ def _check(eng, orc, sdet, sval, sinfo):
     return len(m1[0])
 
 
-@pytest.mark.parametrize("case", ["large", "skewed"])
-def test_partition_sort_segments_past_lds(case):
+@pytest.mark.parametrize("case,nspat,nel", [("large", 20, 10), ("skewed", 20, 10), ("large", 30, 14),
+                                            ("skewed", 30, 14), ("lds", 30, 14)])
+def test_partition_sort_segments_past_lds(case, nspat, nel):
+    """10e/20o: the 32-bit colex-rank sort; 14e/30o: the 64-bit one (46-bit
+    keys, five byte passes per segment; "lds": 4e5 spawns, every segment
+    inside LDS)."""
     from neci_stable_amd.engine import Engine
     from oracle_ctypes import Oracle
     rng = np.random.default_rng(17)
     if case == "large":
-        pool = np.unique(_random_dets(rng, 300_000), axis=0)
+        pool = np.unique(_random_dets(rng, 300_000, nspat, nel), axis=0)
         sdet = pool[rng.integers(0, len(pool), 1_500_000)]
+    elif case == "lds":
+        pool = np.unique(_random_dets(rng, 200_000, nspat, nel), axis=0)
+        sdet = pool[rng.integers(0, len(pool), 400_000)]
     else:
-        pool = np.unique(_random_dets(rng, 100_000), axis=0)
+        pool = np.unique(_random_dets(rng, 100_000, nspat, nel), axis=0)
         sdet = pool[rng.integers(0, len(pool), 300_000)]
         sdet[rng.permutation(len(sdet))[:100_000]] = pool[0]
     n = len(sdet)
     sval = rng.integers(-3, 4, n).astype(float)
     sval[sval == 0] = 1.0
     sinfo = ((rng.random(n) < 0.3).astype(np.uint32) << 8)
-    spec = mol_spec(20, 10, rank=8, init_walkers=n, memory_fac_spawn=1.2, memory_fac_part=1.2,
+    spec = mol_spec(nspat, nel, rank=8, init_walkers=n, memory_fac_spawn=1.2, memory_fac_part=1.2,
                     memory_fac_hash=1.5)
     eng = Engine(spec)
     orc = Oracle(spec, 0.01)
