@@ -145,8 +145,8 @@ __global__ void __launch_bounds__(kBlock) km_rank64_hist(const EngDev* __restric
                                                          const uint16_t* __restrict__ map, int bits) {
   Grid g;
   const EngDev& a = eng_of(ed, map, kLkChunk, g);
-  k_rank64_hist_body<IT, 8>(g, a.sp_rec, a.s.rec_words, &a.cnt->n_spawned, a.scap, a.s.binom64, a.s.nbasis,
-                            a.s.nel1, bits, a.sk[0], a.hist);
+  k_rank64_hist_body<IT, kSegBits64>(g, a.sp_rec, a.s.rec_words, &a.cnt->n_spawned, a.scap, a.s.binom64,
+                                     a.s.nbasis, a.s.nel1, bits, a.sk[0], a.hist);
 }
 
 // the colex-rank sort: top-byte pass over the ranks the spawn kernels wrote
@@ -166,10 +166,10 @@ __global__ void __launch_bounds__(kBlock) km_radix_scatter_ranks(const EngDev* _
                                                   &a.cnt->n_spawned, a.scap, 0, 24, a.hist, a.digit_tot);
 }
 
-// kDigitBlocks blocks per engine: engine blockIdx / kDigitBlocks
-__global__ void __launch_bounds__(kBlock) km_radix_scan(const EngDev* __restrict__ ed) {
-  const EngDev& a = ed[blockIdx.x / kDigitBlocks];
-  const Grid g{(int64_t)(blockIdx.x % kDigitBlocks), kDigitBlocks};
+// one block per digit of every engine: engine blockIdx / digits
+__global__ void __launch_bounds__(kBlock) km_radix_scan(const EngDev* __restrict__ ed, int digits) {
+  const EngDev& a = ed[blockIdx.x / digits];
+  const Grid g{(int64_t)(blockIdx.x % digits), digits};
   k_radix_scan_body(g, a.hist, a.nb[kLkChunk], a.digit_tot);
 }
 __global__ void __launch_bounds__(kSegThreads) km_seg_sort(const EngDev* __restrict__ ed) {
@@ -177,10 +177,12 @@ __global__ void __launch_bounds__(kSegThreads) km_seg_sort(const EngDev* __restr
   const Grid g{(int64_t)(blockIdx.x % kDigitBlocks), kDigitBlocks};
   k_seg_sort_body<uint32_t>(g, (uint32_t*)a.sk[1], a.si[1], (uint32_t*)a.sk[0], a.si[0], a.digit_tot, 3);
 }
-// the 64-bit rank path's segments: sk[1] -> sk[0], npass bytes below the top one
+// the 64-bit rank path's 2^kSegBits64 segments per engine: sk[1] -> sk[0],
+// npass bytes below the top digit
 __global__ void __launch_bounds__(kSegThreads) km_seg_sort64(const EngDev* __restrict__ ed, int npass) {
-  const EngDev& a = ed[blockIdx.x / kDigitBlocks];
-  const Grid g{(int64_t)(blockIdx.x % kDigitBlocks), kDigitBlocks};
+  constexpr int kSeg = 1 << kSegBits64;
+  const EngDev& a = ed[blockIdx.x / kSeg];
+  const Grid g{(int64_t)(blockIdx.x % kSeg), kSeg};
   k_seg_sort_body<uint64_t>(g, a.sk[1], a.si[1], a.sk[0], a.si[0], a.digit_tot, npass);
 }
 

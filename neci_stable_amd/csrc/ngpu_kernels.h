@@ -1567,6 +1567,10 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
 // larger ones.  The keys are colex ranks times an odd constant (a bijection
 // of 32-bit values), so the top byte spreads the keys evenly over segments.
 constexpr int kSegCap = 4096;
+// 64-bit keys: a smaller segment in LDS, so that two 1024-thread blocks fit a
+// compute unit (the 64-bit path partitions on 9 bits, ~half the segment size)
+constexpr int kSegCap64 = 2048;
+constexpr int kSegBits64 = 9;  // top digit of the 64-bit rank partition
 
 constexpr int kSegThreads = 1024;  // threads of a k_seg_sort block
 constexpr int kSegWaves = kSegThreads / kWave;
@@ -1644,30 +1648,39 @@ __device__ inline void seg_pass(const K* sk, const uint32_t* si, K* dk, uint32_t
   }
 }
 
-// block b sorts segment b (256 blocks of kSegThreads) by the npass bytes
-// below the top byte; digit_tot holds the segment sizes, the counts of the
-// top-byte pass.  Keys of 32 bits (colex rank times an odd constant, three
-// passes) or 64 bits (the 64-bit rank path).
+// block b sorts segment b (one block of kSegThreads per segment) by the npass
+// bytes below the top digit; digit_tot holds the segment sizes, the counts of
+// the top-digit pass.  Keys of 32 bits (colex rank times an odd constant, a
+// top byte and three passes) or 64 bits (the 64-bit rank path: a 9-bit top
+// digit, 512 segments).
 template <typename K>
 __device__ __forceinline__ void k_seg_sort_body(const Grid& g_, K* kin, uint32_t* iin, K* kout, uint32_t* iout, const uint32_t* digit_tot, int npass) {
-  __shared__ K ka[kSegCap], kb[kSegCap];
-  __shared__ uint32_t ia[kSegCap], ib[kSegCap];
+  constexpr int CAP = sizeof(K) == 8 ? kSegCap64 : kSegCap;
+  __shared__ K ka[CAP], kb[CAP];
+  __shared__ uint32_t ia[CAP], ib[CAP];
   __shared__ SegScratch sc;
   __shared__ int64_t seg[2];
-  if (threadIdx.x < 256) sc.cnt[threadIdx.x] = digit_tot[threadIdx.x];
-  __syncthreads();
-  if (threadIdx.x == 0) {  // this segment's start: the sizes of the segments before it
-    int64_t st = 0;
-    for (int b = 0; b < (int)g_.b; ++b) st += sc.cnt[b];
-    seg[0] = st;
-    seg[1] = sc.cnt[g_.b];
+  __shared__ int64_t wsum[kSegWaves];
+  {  // this segment's start: the sizes of the segments before it
+    int64_t v = 0;
+    for (int b = threadIdx.x; b < (int)g_.b; b += blockDim.x) v += digit_tot[b];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane_id() == 0) wsum[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t st = 0;
+      for (int q = 0; q < kSegWaves; ++q) st += wsum[q];
+      seg[0] = st;
+      seg[1] = digit_tot[g_.b];
+    }
+    __syncthreads();
   }
-  __syncthreads();
   const int64_t start = seg[0], n = seg[1];
   if (n == 0) return;  // block-uniform
   K *gk = kin + start, *ok = kout + start;
   uint32_t *gi = iin + start, *oi = iout + start;
-  if (n <= kSegCap) {
+  if (n <= CAP) {
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
       ka[i] = gk[i];
       ia[i] = gi[i];
