@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(kBlock) km_rank64_hist(const EngDev* __restric
                                                          const uint16_t* __restrict__ map, int bits) {
   Grid g;
   const EngDev& a = eng_of(ed, map, kLkChunk, g);
-  k_rank64_hist_body<IT, kSegBits64>(g, a.sp_rec, a.s.rec_words, &a.cnt->n_spawned, a.scap, a.s.binom64,
+  k_rank64_hist_body<IT, kSegBits>(g, a.sp_rec, a.s.rec_words, &a.cnt->n_spawned, a.scap, a.s.binom64,
                                      a.s.nbasis, a.s.nel1, bits, a.sk[0], a.hist);
 }
 
@@ -155,15 +155,17 @@ __global__ void __launch_bounds__(kBlock) km_radix_hist_ranks(const EngDev* __re
                                                               const uint16_t* __restrict__ map) {
   Grid g;
   const EngDev& a = eng_of(ed, map, kLkChunk, g);
-  k_radix_hist_body<uint32_t, 1, false, IT, 8>(g, a.rk, 1, &a.cnt->n_spawned, a.scap, 0, 24, a.hist);
+  k_radix_hist_body<uint32_t, 1, false, IT, kSegBits>(g, a.rk, 1, &a.cnt->n_spawned, a.scap, 0, 32 - kSegBits,
+                                                   a.hist);
 }
 template <int IT>
 __global__ void __launch_bounds__(kBlock) km_radix_scatter_ranks(const EngDev* __restrict__ ed,
                                                                  const uint16_t* __restrict__ map) {
   Grid g;
   const EngDev& a = eng_of(ed, map, kLkChunk, g);
-  k_radix_scatter_body<uint32_t, 1, false, IT, 8>(g, a.rk, 1, nullptr, (uint32_t*)a.sk[1], a.si[1],
-                                                  &a.cnt->n_spawned, a.scap, 0, 24, a.hist, a.digit_tot);
+  k_radix_scatter_body<uint32_t, 1, false, IT, kSegBits>(g, a.rk, 1, nullptr, (uint32_t*)a.sk[1], a.si[1],
+                                                         &a.cnt->n_spawned, a.scap, 0, 32 - kSegBits, a.hist,
+                                                         a.digit_tot);
 }
 
 // one block per digit of every engine: engine blockIdx / digits
@@ -173,16 +175,16 @@ __global__ void __launch_bounds__(kBlock) km_radix_scan(const EngDev* __restrict
   k_radix_scan_body(g, a.hist, a.nb[kLkChunk], a.digit_tot);
 }
 __global__ void __launch_bounds__(kSegThreads) km_seg_sort(const EngDev* __restrict__ ed) {
-  const EngDev& a = ed[blockIdx.x / kDigitBlocks];
-  const Grid g{(int64_t)(blockIdx.x % kDigitBlocks), kDigitBlocks};
-  k_seg_sort_body<uint32_t>(g, (uint32_t*)a.sk[1], a.si[1], (uint32_t*)a.sk[0], a.si[0], a.digit_tot, 3);
+  const EngDev& a = ed[blockIdx.x / kSegs];
+  const Grid g{(int64_t)(blockIdx.x % kSegs), kSegs};
+  k_seg_sort_body<uint32_t>(g, (uint32_t*)a.sk[1], a.si[1], (uint32_t*)a.sk[0], a.si[0], a.digit_tot,
+                            (32 - kSegBits + 7) / 8);
 }
-// the 64-bit rank path's 2^kSegBits64 segments per engine: sk[1] -> sk[0],
+// the 64-bit rank path's kSegs segments per engine: sk[1] -> sk[0],
 // npass bytes below the top digit
 __global__ void __launch_bounds__(kSegThreads) km_seg_sort64(const EngDev* __restrict__ ed, int npass) {
-  constexpr int kSeg = 1 << kSegBits64;
-  const EngDev& a = ed[blockIdx.x / kSeg];
-  const Grid g{(int64_t)(blockIdx.x % kSeg), kSeg};
+  const EngDev& a = ed[blockIdx.x / kSegs];
+  const Grid g{(int64_t)(blockIdx.x % kSegs), kSegs};
   k_seg_sort_body<uint64_t>(g, a.sk[1], a.si[1], a.sk[0], a.si[0], a.digit_tot, npass);
 }
 

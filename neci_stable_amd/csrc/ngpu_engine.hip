@@ -531,17 +531,18 @@ std::vector<std::pair<int, int>> word_passes(const ngpu_t* h) {
 template <int W, int R, int IT>
 int sort_ranks(ngpu_t* h, bool ranked) {
   const unsigned long long* n_ptr = &h->cnt->n_spawned;
+  constexpr int kTop = 32 - kSegBits;  // the partition digit's shift
   if (ranked)
-    launch(h, 4, -1, k_radix_hist<uint32_t, 1, false, IT, 8>, dim3(h->nb_chunk), dim3(kBlock),
-           (const uint32_t*)h->rk, 1, n_ptr, h->scap, 0, 24, h->hist);
+    launch(h, 4, -1, k_radix_hist<uint32_t, 1, false, IT, kSegBits>, dim3(h->nb_chunk), dim3(kBlock),
+           (const uint32_t*)h->rk, 1, n_ptr, h->scap, 0, kTop, h->hist);
   else
-    launch(h, 4, -1, k_rank_hist<W, IT, 8>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec, n_ptr,
-           h->scap, h->ds.rec_words, (const uint32_t*)h->d_binom, h->sys.nel + 1, h->rk, h->hist, 24);
-  launch(h, -1, -1, k_radix_scan, dim3(256), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
-  launch(h, -1, -1, k_radix_scatter<uint32_t, 1, false, IT, 8>, dim3(h->nb_chunk), dim3(kBlock),
-         (const uint32_t*)h->rk, 1, (const uint32_t*)nullptr, (uint32_t*)h->sk[1], h->si[1], n_ptr, h->scap, 0, 24,
+    launch(h, 4, -1, k_rank_hist<W, IT, kSegBits>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec,
+           n_ptr, h->scap, h->ds.rec_words, (const uint32_t*)h->d_binom, h->sys.nel + 1, h->rk, h->hist, kTop);
+  launch(h, -1, -1, k_radix_scan, dim3(kSegs), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
+  launch(h, -1, -1, k_radix_scatter<uint32_t, 1, false, IT, kSegBits>, dim3(h->nb_chunk), dim3(kBlock),
+         (const uint32_t*)h->rk, 1, (const uint32_t*)nullptr, (uint32_t*)h->sk[1], h->si[1], n_ptr, h->scap, 0, kTop,
          (const uint32_t*)h->hist, (const uint32_t*)h->digit_tot);
-  launch(h, -1, 4, k_seg_sort, dim3(256), dim3(kSegThreads), (uint32_t*)h->sk[1], h->si[1], (uint32_t*)h->sk[0],
+  launch(h, -1, 4, k_seg_sort, dim3(kSegs), dim3(kSegThreads), (uint32_t*)h->sk[1], h->si[1], (uint32_t*)h->sk[0],
          h->si[0], (const uint32_t*)h->digit_tot);
   return 0;
 }
@@ -565,16 +566,15 @@ void sort_and_merge_it(ngpu_t* h, bool ranked, bool list_out) {
     // pass on the top byte into sk[1], then each top-byte segment sorted by
     // the bytes below it (k_seg_sort64) back into sk[0]
     const int bits = h->rank64_bits;
-    constexpr int kSeg = 1 << kSegBits64;
-    launch(h, 4, -1, k_rank64_hist<IT, kSegBits64>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec,
+    launch(h, 4, -1, k_rank64_hist<IT, kSegBits>, dim3(h->nb_chunk), dim3(kBlock), (const uint64_t*)h->sp_rec,
            h->ds.rec_words, n_ptr, h->scap, (const uint64_t*)h->d_binom64, h->sys.nbasis, h->sys.nel + 1, bits,
            h->sk[0], h->hist);
-    launch(h, -1, -1, k_radix_scan, dim3(kSeg), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
-    launch(h, -1, -1, k_radix_scatter<uint64_t, 1, false, IT, kSegBits64>, dim3(h->nb_chunk), dim3(kBlock),
+    launch(h, -1, -1, k_radix_scan, dim3(kSegs), dim3(kBlock), h->hist, h->nb_chunk, h->digit_tot);
+    launch(h, -1, -1, k_radix_scatter<uint64_t, 1, false, IT, kSegBits>, dim3(h->nb_chunk), dim3(kBlock),
            (const uint64_t*)h->sk[0], 1, (const uint32_t*)nullptr, h->sk[1], h->si[1], n_ptr, h->scap, 0,
-           bits - kSegBits64, (const uint32_t*)h->hist, (const uint32_t*)h->digit_tot);
-    launch(h, -1, 4, k_seg_sort64, dim3(kSeg), dim3(kSegThreads), h->sk[1], h->si[1], h->sk[0], h->si[0],
-           (const uint32_t*)h->digit_tot, (bits - kSegBits64 + 7) / 8);
+           bits - kSegBits, (const uint32_t*)h->hist, (const uint32_t*)h->digit_tot);
+    launch(h, -1, 4, k_seg_sort64, dim3(kSegs), dim3(kSegThreads), h->sk[1], h->si[1], h->sk[0], h->si[0],
+           (const uint32_t*)h->digit_tot, (bits - kSegBits + 7) / 8);
     if (list_out)
       launch(h, -1, -1, k_merge_list<W, R, uint64_t, W>, dim3(h->nb_grid), dim3(kBlock), h->ds, h->cnt,
              (const uint64_t*)h->sk[0], (const uint32_t*)h->si[0], (const uint64_t*)h->sp_rec, h->scap, h->u_key,
@@ -2855,10 +2855,10 @@ void group_sort_merge(BatchGroup& g) {
   int cur = 0;
   if (g.rank_sort) {
     glaunch(h0, 4, -1, km_radix_hist_ranks<IT>, g.tot[kLkChunk], kBlock, 0, ed, (const uint16_t*)g.d_map[kLkChunk]);
-    glaunch(h0, -1, -1, km_radix_scan, E * kDigitBlocks, kBlock, 0, ed, kDigitBlocks);
+    glaunch(h0, -1, -1, km_radix_scan, E * kSegs, kBlock, 0, ed, kSegs);
     glaunch(h0, -1, -1, km_radix_scatter_ranks<IT>, g.tot[kLkChunk], kBlock, 0, ed,
             (const uint16_t*)g.d_map[kLkChunk]);
-    glaunch(h0, -1, 4, km_seg_sort, E * kDigitBlocks, kSegThreads, 0, ed);
+    glaunch(h0, -1, 4, km_seg_sort, E * kSegs, kSegThreads, 0, ed);
     glaunch(h0, 6, 6, km_merge_annihilate<W, R, uint32_t, 1>, g.tot[kLkMerge], kBlock, 0, ed,
             (const uint16_t*)g.d_map[kLkMerge], 0);
     return;
@@ -2867,12 +2867,11 @@ void group_sort_merge(BatchGroup& g) {
     // 64-bit rank keys into sk[0], the top-byte partition into sk[1], the
     // segments sorted back into sk[0]
     const int bits = g.rank64_bits;
-    constexpr int kSeg = 1 << kSegBits64;
     glaunch(h0, 4, -1, km_rank64_hist<IT>, g.tot[kLkChunk], kBlock, 0, ed, (const uint16_t*)g.d_map[kLkChunk], bits);
-    glaunch(h0, -1, -1, km_radix_scan, E * kSeg, kBlock, 0, ed, kSeg);
-    glaunch(h0, -1, -1, km_radix_scatter_words<uint64_t, 1, false, IT, kSegBits64>, g.tot[kLkChunk], kBlock, 0, ed,
-            (const uint16_t*)g.d_map[kLkChunk], 2, 0, 0, bits - kSegBits64);
-    glaunch(h0, -1, 4, km_seg_sort64, E * kSeg, kSegThreads, 0, ed, (bits - kSegBits64 + 7) / 8);
+    glaunch(h0, -1, -1, km_radix_scan, E * kSegs, kBlock, 0, ed, kSegs);
+    glaunch(h0, -1, -1, km_radix_scatter_words<uint64_t, 1, false, IT, kSegBits>, g.tot[kLkChunk], kBlock, 0, ed,
+            (const uint16_t*)g.d_map[kLkChunk], 2, 0, 0, bits - kSegBits);
+    glaunch(h0, -1, 4, km_seg_sort64, E * kSegs, kSegThreads, 0, ed, (bits - kSegBits + 7) / 8);
     glaunch(h0, 6, 6, km_merge_annihilate<W, R, uint64_t, W>, g.tot[kLkMerge], kBlock, 0, ed,
             (const uint16_t*)g.d_map[kLkMerge], 0);
     return;

@@ -1566,11 +1566,12 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const K* kin, int kstr
 // keys and through global memory (the input buffers serve as scratch) for
 // larger ones.  The keys are colex ranks times an odd constant (a bijection
 // of 32-bit values), so the top byte spreads the keys evenly over segments.
-constexpr int kSegCap = 4096;
-// 64-bit keys: a smaller segment in LDS, so that two 1024-thread blocks fit a
-// compute unit (the 64-bit path partitions on 9 bits, ~half the segment size)
-constexpr int kSegCap64 = 2048;
-constexpr int kSegBits64 = 9;  // top digit of the 64-bit rank partition
+// Both rank paths partition on a 9-bit top digit (512 segments) and sort up
+// to 2048 keys of a segment in LDS, so that two (64-bit keys) or three
+// (32-bit keys) 1024-thread blocks fit a compute unit.
+constexpr int kSegCap = 2048;
+constexpr int kSegBits = 9;  // top digit of the rank partitions
+constexpr int kSegs = 1 << kSegBits;
 
 constexpr int kSegThreads = 1024;  // threads of a k_seg_sort block
 constexpr int kSegWaves = kSegThreads / kWave;
@@ -1650,12 +1651,11 @@ __device__ inline void seg_pass(const K* sk, const uint32_t* si, K* dk, uint32_t
 
 // block b sorts segment b (one block of kSegThreads per segment) by the npass
 // bytes below the top digit; digit_tot holds the segment sizes, the counts of
-// the top-digit pass.  Keys of 32 bits (colex rank times an odd constant, a
-// top byte and three passes) or 64 bits (the 64-bit rank path: a 9-bit top
-// digit, 512 segments).
+// the top-digit pass.  Keys of 32 bits (colex rank times an odd constant,
+// three passes) or 64 bits (the 64-bit rank path).
 template <typename K>
 __device__ __forceinline__ void k_seg_sort_body(const Grid& g_, K* kin, uint32_t* iin, K* kout, uint32_t* iout, const uint32_t* digit_tot, int npass) {
-  constexpr int CAP = sizeof(K) == 8 ? kSegCap64 : kSegCap;
+  constexpr int CAP = kSegCap;
   __shared__ K ka[CAP], kb[CAP];
   __shared__ uint32_t ia[CAP], ib[CAP];
   __shared__ SegScratch sc;
@@ -1714,7 +1714,7 @@ __device__ __forceinline__ void k_seg_sort_body(const Grid& g_, K* kin, uint32_t
   }
 }
 __global__ void __launch_bounds__(kSegThreads) k_seg_sort(uint32_t* kin, uint32_t* iin, uint32_t* kout, uint32_t* iout, const uint32_t* digit_tot) {
-  k_seg_sort_body<uint32_t>(hw_grid(), kin, iin, kout, iout, digit_tot, 3);
+  k_seg_sort_body<uint32_t>(hw_grid(), kin, iin, kout, iout, digit_tot, (32 - kSegBits + 7) / 8);
 }
 __global__ void __launch_bounds__(kSegThreads) k_seg_sort64(uint64_t* kin, uint32_t* iin, uint64_t* kout, uint32_t* iout, const uint32_t* digit_tot, int npass) {
   k_seg_sort_body<uint64_t>(hw_grid(), kin, iin, kout, iout, digit_tot, npass);

@@ -1,13 +1,13 @@
 """The spawn sort at sizes where its per-segment sort leaves LDS.
 
-The colex-rank sorts make one stable pass on the top byte of rank*odd
+The colex-rank sorts make one stable pass on the top 9 bits of rank*odd
 constant (32-bit keys, or 64-bit keys modulo 2^bits), then sort each of the
-256 top-byte segments by the bytes below it: in LDS up to 4096 keys, through
-global memory beyond.  Through the
+512 segments by the bytes below: in LDS up to 2048 keys, through global
+memory beyond.  Through the
 explicit-list entry point (ngpu_annihilate, the same sort and merge as an
 iteration) on an empty walker list:
 * 1.5e6 spawns over 3e5 determinants of 10 electrons in 40 spin orbitals
-  (segments of ~5900 keys, all past the LDS size);
+  (segments of ~2900 keys, all past the LDS size);
 * a skewed list in which one determinant takes 1e5 of 3e5 spawns (its
   segment alone passes the LDS size, the others stay inside it);
 * the same two lists, and one that stays inside LDS, over determinants of
@@ -60,8 +60,8 @@ def _check(eng, orc, sdet, sval, sinfo):
                                             ("skewed", 30, 14), ("lds", 30, 14)])
 def test_partition_sort_segments_past_lds(case, nspat, nel):
     """10e/20o: the 32-bit colex-rank sort; 14e/30o: the 64-bit one (46-bit
-    keys, five byte passes per segment; "lds": 4e5 spawns, every segment
-    inside LDS)."""
+    keys, five byte passes per segment; "lds": 4e5 spawns, ~780 per segment,
+    every segment inside LDS)."""
     from neci_stable_amd.engine import Engine
     from oracle_ctypes import Oracle
     rng = np.random.default_rng(17)
