@@ -1284,8 +1284,12 @@ __global__ void __launch_bounds__(kBlock) k_spawn_singles(DevSys s, Walk wk, Cou
 // spawn order, which a stable sort keeps), so the sort key may be any
 // injective function of the determinant.  When C(nbasis, nel) < 2^32 the key
 // is the determinant's 32-bit colex rank (sum over its i-th occupied bit at
-// position p of C(p, i)): fewer passes over narrower keys.  Otherwise the
-// key is the determinant itself, ordered as ilut_lt orders it.
+// position p of C(p, i)) times an odd constant; for one-word determinants
+// with C(nbasis, nel) < 2^64 whose rank is shorter than the word, the 64-bit
+// rank times an odd constant modulo 2^bits.  Both are sorted by one pass on
+// their top digit and a per-segment sort (k_seg_sort, k_seg_sort64).
+// Otherwise the key is the determinant itself, ordered as ilut_lt orders it,
+// in LSD passes.
 constexpr int kRadixBins = 1024;  // bins of the widest digit (10 bits)
 // radix tiles: IT elements per thread, IT*kBlock per tile (IT = 2, 4 or 8,
 // chosen per engine at run time)
