@@ -687,7 +687,7 @@ void iterate_chain(ngpu_t* h) {
     const int nsp = h->sys.nbasis / 2;
     // G lanes per attempt, PL >= nsp / G orbitals per lane
     const int G = h->single_lanes;
-    const size_t lds = sizeof(double) * (size_t)(kBlock / G) * nsp;
+    const size_t lds = 2 * sizeof(double) * (size_t)(kBlock / G) * nsp;  // weights and running sums
     auto kern = G == 16 ? (nsp <= 16 ? k_spawn_singles<W, R, 1, 16> : nsp <= 32 ? k_spawn_singles<W, R, 2, 16>
                            : k_spawn_singles<W, R, 4, 16>)
               : G == 8 ? (nsp <= 16 ? k_spawn_singles<W, R, 2, 8> : nsp <= 24 ? k_spawn_singles<W, R, 3, 8>
@@ -2899,7 +2899,7 @@ void group_spawn(BatchGroup& g) {
 
 template <int W, int R, int PL, int GL>
 void group_singles(BatchGroup& g, int nsp) {
-  const size_t lds = sizeof(double) * (size_t)(kBlock / GL) * nsp;
+  const size_t lds = 2 * sizeof(double) * (size_t)(kBlock / GL) * nsp;  // weights and running sums
   glaunch(g.hs[0], 3, 3, km_spawn_singles<W, R, PL, GL>, g.tot[kLkGrid], kBlock, lds, (const EngDev*)g.d_ed,
           (const uint16_t*)g.d_map[kLkGrid], g.rank_sort ? 1 : 0);
 }

@@ -1121,7 +1121,8 @@ __device__ __forceinline__ void k_spawn_singles_body(const Grid& g_, const DevSy
   __shared__ SpawnStage<W> ss;
   __shared__ BinomLds bl;
   bl.load(s, rk);
-  extern __shared__ double hv_dyn[];  // signed H(src -> b) of each group's attempt
+  // per group: the signed H(src -> b) of its attempt, then their running sums
+  extern __shared__ double hv_dyn[];
   BlockStats bs{acc_lds};
   bs.zero();
   stage_init(&ss.st);
@@ -1136,7 +1137,8 @@ __device__ __forceinline__ void k_spawn_singles_body(const Grid& g_, const DevSy
   const int gl = threadIdx.x / G;        // group within the block
   const int gbase = lane_id() - lane_id() % G;
   const bool leader = g == 0;
-  double* myv = hv_dyn + (size_t)gl * nsp;
+  double* myv = hv_dyn + (size_t)gl * 2 * nsp;
+  double* mycum = myv + nsp;
   const int64_t groups = (int64_t)g_.nb * kGroups;
   const int64_t grp = (int64_t)g_.b * kGroups + gl;
   const int64_t trips = (nq + groups - 1) / groups;
@@ -1193,20 +1195,26 @@ __device__ __forceinline__ void k_spawn_singles_body(const Grid& g_, const DevSy
         }
       }
       __syncthreads();
-      // sweep 1: normalisation; sweep 2: the first b whose running sum reaches r
+      // the normalisation, keeping the running sums (every lane of the group
+      // writes the same values, and reads its own writes below); then the
+      // first b whose running sum reaches r, by bisection over those sums:
+      // the running sums of a second sweep in the same order are the same
+      // values, so this is its selection (the last b when none reaches r)
       double cum_sum = 0.0;
-      for (int b = 0; b < nsp; ++b) cum_sum += fabs(myv[b]);
-      double rr = d2 * cum_sum;
-      double cum = 0.0, cpt = 0.0, vsel = 0.0;
-      int bsel = 0;
-      for (int b = 1; b <= nsp; ++b) {
-        double vb = myv[b - 1];
-        vsel = vb;
-        cpt = fabs(vb);
-        bsel = b;
-        cum += cpt;
-        if (!(cum < rr) || b == nsp) break;
+      for (int b = 0; b < nsp; ++b) {
+        cum_sum += fabs(myv[b]);
+        mycum[b] = cum_sum;
       }
+      double rr = d2 * cum_sum;
+      int lo = 0, hi = nsp - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (!(mycum[mid] < rr)) hi = mid;
+        else lo = mid + 1;
+      }
+      const int bsel = lo + 1;
+      const double vsel = myv[lo];
+      const double cpt = fabs(vsel);
       bool mine_att = act && leader;
       bool emit = false;
       uint64_t child[W];
